@@ -1,0 +1,239 @@
+/*
+ * sgipc.h -- C ABI of the MI355X (gfx950) Newton linear-solve library.
+ *
+ * Drop-in boundary for StiffGIPC's linear-solve hot path.  Every entry point
+ * takes plain pointers and sizes; array arguments are DEVICE pointers unless
+ * the comment says "host".  Conventions shared with the reference:
+ *   - a 3x3 block is 9 doubles (or floats), column-major like Eigen::Matrix3d:
+ *     v[c*3 + r] = M(r, c);
+ *   - vectors are contiguous doubles of length 3*rows (double3 aliasing);
+ *   - the assembled matrix is upper-triangular symmetric BCOO (row <= col),
+ *     sorted by (row, col), unique (the output of Converter::srbk_convert);
+ *   - contact pairs are int4 in the reference encoding, point-triangle =
+ *     (-v-1, t0, t1, t2).
+ *
+ * Error behaviour: the reference aborts on any CUDA error (CUDA_SAFE_CALL,
+ * StiffGIPC/cuda_tools/cuda_tools.h:36-46).  Here every function returns an
+ * int status and never aborts; sgipc_last_error() gives the message.
+ * Threading: one host thread per context; all work is ordered on the
+ * context's HIP stream and every function returns after its results are
+ * complete (synchronous semantics, like the reference's callers expect).
+ */
+#ifndef SGIPC_H
+#define SGIPC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SGIPC_ABI_VERSION 1
+
+enum sgipc_status {
+    SGIPC_OK = 0,
+    SGIPC_EMPTY = 1,      /* nothing to do (n = 0 or nnz = 0); outputs untouched  */
+    SGIPC_NOT_SPD = 2,    /* p.Ap <= 0 met during PCG                             */
+    SGIPC_MAXITER = 3,    /* iteration cap reached before the tolerance           */
+    SGIPC_CAPACITY = 4,   /* a caller-provided buffer is too small                 */
+    SGIPC_BAD_INDEX = 5,  /* an index lies outside its declared range             */
+    SGIPC_BAD_ARG = 6,    /* null pointer, unsupported size or option             */
+    SGIPC_HIP_ERROR = 7   /* a HIP runtime call failed                            */
+};
+
+enum sgipc_precision {
+    SGIPC_FP64 = 0,       /* FP64 SpMV and FP64 MAS apply (the plain path)                */
+    SGIPC_FP32_MAS = 1,   /* FP64 SpMV/vectors, FP32 MAS apply (the reference's setting)  */
+    SGIPC_FP32_IR = 2     /* FP32 SpMV + FP32 MAS inside FP64 iterative refinement        */
+};
+
+enum sgipc_stop_mode {
+    SGIPC_STOP_REF_RZ = 0,        /* |r.z| <= tol * r0.z0 (pcg_solver.cu:279-280)       */
+    SGIPC_STOP_TRUE_RESIDUAL = 1  /* ||b - A x|| <= tol * ||b||                         */
+};
+
+typedef struct sgipc_ctx sgipc_ctx;
+typedef struct sgipc_mas sgipc_mas;
+typedef struct sgipc_system sgipc_system;
+
+/* ---- context ------------------------------------------------------------- */
+int sgipc_abi_version(void);
+int sgipc_create(sgipc_ctx** out, int device);
+int sgipc_destroy(sgipc_ctx* ctx);
+/* Use an existing HIP stream (hipStream_t, host handle) instead of the context's own. */
+int sgipc_set_stream(sgipc_ctx* ctx, void* hip_stream);
+int sgipc_synchronize(sgipc_ctx* ctx);
+const char* sgipc_last_error(const sgipc_ctx* ctx);
+
+/* ---- device memory helpers (for bindings without a device allocator) ----- */
+int sgipc_malloc(sgipc_ctx* ctx, void** ptr, size_t bytes);
+int sgipc_free(sgipc_ctx* ctx, void* ptr);
+int sgipc_memcpy_htod(sgipc_ctx* ctx, void* dst, const void* host_src, size_t bytes);
+int sgipc_memcpy_dtoh(sgipc_ctx* ctx, void* host_dst, const void* src, size_t bytes);
+int sgipc_memcpy_dtod(sgipc_ctx* ctx, void* dst, const void* src, size_t bytes);
+int sgipc_memset(sgipc_ctx* ctx, void* dst, int value, size_t bytes);
+
+/* ---- Hessian assembly (triplet writers) ------------------------------------
+ * Replaces _calculate_fem_gradient_hessian (StiffGIPC/femEnergy.cu:1719-1857):
+ * stable Neo-Hookean tets, gradient += dt^2 vol P:dF/dx (grad may be NULL),
+ * PSD-projected Hessian as 10 upper blocks per tet written at
+ * tvals/trows/tcols[10*t ...].  tets: uint4 per tet; dm_inv: row-major 3x3;
+ * mu/lam: per-tet lengthRate/volumeRate; hess_offset is added to vertex ids. */
+int sgipc_fem_tet_gradient_hessian(sgipc_ctx* ctx, const double* x, const uint32_t* tets,
+                                   const double* dm_inv, const double* vol, const double* mu,
+                                   const double* lam, int n_tets, double dt, int hess_offset,
+                                   double* grad, double* tvals, int* trows, int* tcols);
+
+/* Replaces the point-triangle branch of _calBarrierGradientAndHessian
+ * (StiffGIPC/GIPC.cu:4733-4984, RANK 2, NEWF) together with the contact
+ * lower-block swap (StiffGIPC/GIPC.cu:10329-10363): 16 blocks per pair at
+ * tvals[16*p ...], blocks below the diagonal moved to (col,row) and zeroed.
+ * dhat_sq is the reference's dHat (squared distance).  pair_x (optional, 12
+ * doubles per pair) supplies the pair's local geometry; NULL reads x at the
+ * pair's vertex ids like the reference. */
+int sgipc_barrier_pt_gradient_hessian(sgipc_ctx* ctx, const double* x, const int* pairs,
+                                      const double* pair_x, int n_pairs, double dhat_sq,
+                                      double kappa, int hess_offset, double* grad, double* tvals,
+                                      int* trows, int* tcols);
+
+/* Replaces the mass-diagonal writer of computeGradientAndHessian
+ * (StiffGIPC/GIPC.cu:10466-10485): mass[i]*I at (i+off, i+off). */
+int sgipc_mass_diagonal(sgipc_ctx* ctx, const double* mass, int n, int hess_offset,
+                        double* tvals, int* trows, int* tcols);
+
+/* Replaces Converter::srbk_convert (StiffGIPC/linear_system/utils/converter.cu:
+ * 32-51): radix sort by (row,col), sum duplicates, unique upper BCOO written to
+ * ovals/orows/ocols (capacity blocks); *out_nnz receives the count.  The
+ * input arrays are left unchanged. */
+int sgipc_convert_srbk(sgipc_ctx* ctx, const double* tvals, const int* trows, const int* tcols,
+                       int n_triplets, double* ovals, int* orows, int* ocols, int capacity,
+                       int* out_nnz);
+
+/* ---- SpMV ------------------------------------------------------------------
+ * Replaces Spmv::srbk_spmv (StiffGIPC/linear_system/utils/spmv.cu:188-313):
+ * y = a*A*x + b*y for the symmetric matrix stored as upper BCOO. */
+int sgipc_spmv_srbk(sgipc_ctx* ctx, double a, const double* vals, const int* rows,
+                    const int* cols, int nnz, const double* x, double b, double* y, int n_rows);
+/* FP32 variant (matrix, x and y in float). */
+int sgipc_spmv_srbk_f32(sgipc_ctx* ctx, float a, const float* vals, const int* rows,
+                        const int* cols, int nnz, const float* x, float b, float* y, int n_rows);
+/* Round FP64 blocks/vectors to FP32 (n values). */
+int sgipc_convert_f64_to_f32(sgipc_ctx* ctx, const double* in, float* out, size_t n);
+
+/* ---- MAS preconditioner (MASPreconditioner, StiffGIPC/MASPreconditioner.cuh:59-107) */
+/* bank_size is the domain size B (16 = the reference's BANKSIZE, or 32). */
+int sgipc_mas_create(sgipc_ctx* ctx, sgipc_mas** out, int bank_size);
+int sgipc_mas_destroy(sgipc_mas* mas);
+/* initPreconditioner_Neighbor (MASPreconditioner.cu:2291-2328): vertex count,
+ * collision node offset and the vertex-neighbour CSR (device arrays, copied). */
+int sgipc_mas_init_neighbor(sgipc_mas* mas, int vert_num, int collision_node_offset,
+                            int total_neighbor_num, const uint32_t* nbr_start,
+                            const uint32_t* nbr_num, const uint32_t* nbr_list);
+/* initPreconditioner_Matrix (:2330-2358): sizes the buffers from the
+ * contact-free hierarchy. */
+int sgipc_mas_init_matrix(sgipc_mas* mas);
+/* setPreconditioner_bcoo (:2207-2245): rebuilds the hierarchy with the current
+ * contact pairs, assembles the Galerkin domain matrices from the unique upper
+ * BCOO (blocks whose row and column both lie in [offset, offset+vert_num)) and
+ * inverts them in FP64.  precision: SGIPC_FP64 stores the inverses in FP64,
+ * otherwise FP32. */
+int sgipc_mas_set_preconditioner_bcoo(sgipc_mas* mas, const double* vals, const int* rows,
+                                      const int* cols, int nnz, int offset, const int* pairs,
+                                      int n_pairs, int precision);
+/* preconditioning (:2248-2289): z = M^-1 r on the vert_num FEM vertices. */
+int sgipc_mas_preconditioning(sgipc_mas* mas, const double* r, double* z);
+/* Level introspection (host outputs): *level_num, offsets[level_num+1],
+ * counts[level_num]. */
+int sgipc_mas_levels(sgipc_mas* mas, int* level_num, int* offsets, int* counts);
+/* Hierarchy tables (device outputs): coarse_table[vert_num*(level_num-1)]
+ * (global slot ids, vertex-major) and going_next[total slots]. */
+int sgipc_mas_tables(sgipc_mas* mas, int* coarse_table, int* going_next);
+/* Inverse domain matrices as FP64 upper blocks (device output,
+ * total_slots/B * B(B+1)/2 * 9 doubles). */
+int sgipc_mas_inverse_blocks(sgipc_mas* mas, double* out);
+
+/* ---- PCG (PCGSolver::solve, StiffGIPC/linear_system/solver/pcg_solver.cu:158-332) */
+typedef struct sgipc_pcg_opts {
+    double tol;        /* relative tolerance (1e-5 for the true-residual rule)        */
+    int stop_mode;     /* enum sgipc_stop_mode                                         */
+    int max_iter;      /* <= 0: 0.3 * dof like the reference (pcg_solver.cu:174)     */
+    int precision;     /* enum sgipc_precision                                         */
+    int check_every;   /* iterations between host looks at the stop flag (<= 0: 8)   */
+    int ir_max_outer;  /* FP32_IR: refinement steps (<= 0: 20)                         */
+    double ir_inner_tol; /* FP32_IR: inner relative tolerance (<= 0: 1e-3)            */
+} sgipc_pcg_opts;
+
+typedef struct sgipc_pcg_stats {
+    int iterations;          /* PCG iterations (summed over refinement steps)        */
+    int outer_iterations;    /* refinement steps (1 unless FP32_IR)                   */
+    double rel_residual;     /* ||b - A x|| / ||b|| in FP64 at exit                   */
+    double rz;               /* last r.z                                               */
+    double rz0;
+    double ms_total;         /* wall time of the call                                  */
+    int status;
+} sgipc_pcg_stats;
+
+void sgipc_pcg_default_opts(sgipc_pcg_opts* opts);
+/* x = A^-1 b with A the unique upper BCOO (FP64), preconditioned by mas (NULL =
+ * identity), which must have been set on the same matrix.  For SGIPC_FP32_IR
+ * the FP32 copy of the matrix is made internally. */
+int sgipc_pcg_solve(sgipc_ctx* ctx, const double* vals, const int* rows, const int* cols, int nnz,
+                    int n_rows, sgipc_mas* mas, const double* b, double* x,
+                    const sgipc_pcg_opts* opts, sgipc_pcg_stats* stats);
+
+/* ---- whole linear system (GlobalLinearSystem::solve_linear_system,
+ *      StiffGIPC/linear_system/linear_system/global_linear_system.cu:136-148) --
+ * One object owns a tet scene's device inputs, the triplet arena, the BCOO, the
+ * MAS and the PCG buffers; solve() runs assembly (barrier, tets, mass),
+ * conversion, the MAS build and PCG. */
+typedef struct sgipc_tet_scene_desc {
+    int n_verts;
+    int n_tets;
+    int n_pairs;
+    int total_neighbor_num;
+    double dt;
+    double dhat_sq;
+    double kappa;
+    /* device arrays */
+    const double* x;          /* 3n                                  */
+    const uint32_t* tets;     /* 4 per tet                           */
+    const double* dm_inv;     /* 9 per tet, row-major                */
+    const double* vol;        /* per tet                             */
+    const double* mu;         /* per tet, lengthRate                 */
+    const double* lam;        /* per tet, volumeRate                 */
+    const double* mass;       /* per vertex                          */
+    const int* pairs;         /* 4 per pair                          */
+    const double* pair_x;     /* 12 per pair or NULL                 */
+    const uint32_t* nbr_start;
+    const uint32_t* nbr_num;
+    const uint32_t* nbr_list;
+} sgipc_tet_scene_desc;
+
+typedef struct sgipc_solve_timing {
+    double ms_assemble;      /* element Hessians + gradient                 */
+    double ms_convert;       /* triplets -> unique BCOO                     */
+    double ms_mas_build;     /* hierarchy + Galerkin + inverse              */
+    double ms_pcg;           /* PCG (incl. refinement)                      */
+    double ms_total;
+    int nnz;                 /* unique upper blocks                          */
+    int n_triplets;
+} sgipc_solve_timing;
+
+int sgipc_system_create(sgipc_ctx* ctx, sgipc_system** out, const sgipc_tet_scene_desc* desc,
+                        int bank_size);
+int sgipc_system_destroy(sgipc_system* sys);
+/* b, x: device vectors of 3*n_verts doubles.  grad (optional, device) receives
+ * the assembled energy gradient. */
+int sgipc_system_solve(sgipc_system* sys, const double* b, double* x, double* grad,
+                       const sgipc_pcg_opts* opts, sgipc_pcg_stats* stats,
+                       sgipc_solve_timing* timing);
+/* Views of the system's current BCOO (device pointers owned by the system). */
+int sgipc_system_bcoo(sgipc_system* sys, const double** vals, const int** rows, const int** cols,
+                      int* nnz);
+sgipc_mas* sgipc_system_mas(sgipc_system* sys);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SGIPC_H */

@@ -1,0 +1,266 @@
+"""CPU ORACLE -- TEST INFRASTRUCTURE ONLY.
+
+ctypes wrapper over ``liboracle.so`` (built from ``sgipc_oracle.c`` by the
+Makefile in this folder).  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import this package, and only as the
+checker; the product library never uses it.
+
+Every function here restates the reference algorithm cited in the C source;
+see ``sgipc_oracle.h`` for what pins each one.
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import os
+import time
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+c_int_p = ctypes.POINTER(ctypes.c_int)
+c_uint_p = ctypes.POINTER(ctypes.c_uint)
+c_dbl_p = ctypes.POINTER(ctypes.c_double)
+
+
+def _ptr(a, t):
+    if a is None:
+        return None
+    return a.ctypes.data_as(t)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(path)
+        L.orc_tet_snh.argtypes = [ctypes.c_int, c_dbl_p, c_int_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p,
+                                  ctypes.c_double, ctypes.c_int, c_dbl_p, c_dbl_p, c_int_p, c_int_p]
+        L.orc_tet_snh_energy.argtypes = [c_dbl_p, c_dbl_p, ctypes.c_double, ctypes.c_double]
+        L.orc_tet_snh_energy.restype = ctypes.c_double
+        L.orc_snh_dpdf.argtypes = [c_dbl_p, ctypes.c_double, ctypes.c_double, c_dbl_p, c_dbl_p]
+        L.orc_barrier_pt.argtypes = [ctypes.c_int, c_dbl_p, c_int_p, c_dbl_p, ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_int, c_dbl_p, c_dbl_p, c_int_p, c_int_p]
+        L.orc_barrier_energy.argtypes = [ctypes.c_double] * 3
+        L.orc_barrier_energy.restype = ctypes.c_double
+        L.orc_barrier_lambda0.argtypes = [ctypes.c_double] * 3
+        L.orc_barrier_lambda0.restype = ctypes.c_double
+        L.orc_mass_diag.argtypes = [ctypes.c_int, c_dbl_p, ctypes.c_int, c_dbl_p, c_int_p, c_int_p]
+        L.orc_convert.argtypes = [ctypes.c_long, c_dbl_p, c_int_p, c_int_p, c_dbl_p, c_int_p, c_int_p]
+        L.orc_convert.restype = ctypes.c_long
+        L.orc_spmv.argtypes = [ctypes.c_double, ctypes.c_long, c_dbl_p, c_int_p, c_int_p, c_dbl_p,
+                               ctypes.c_double, c_dbl_p, ctypes.c_int]
+        L.orc_mas_create.argtypes = [ctypes.c_int, ctypes.c_int, c_uint_p, c_uint_p, c_uint_p, ctypes.c_int]
+        L.orc_mas_create.restype = ctypes.c_void_p
+        L.orc_mas_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_mas_build.argtypes = [ctypes.c_void_p, ctypes.c_long, c_dbl_p, c_int_p, c_int_p,
+                                    ctypes.c_int, c_int_p, ctypes.c_int]
+        L.orc_mas_apply.argtypes = [ctypes.c_void_p, c_dbl_p, c_dbl_p, ctypes.c_int]
+        L.orc_mas_levels.argtypes = [ctypes.c_void_p, c_int_p, c_int_p, c_int_p]
+        L.orc_mas_levels.restype = ctypes.c_int
+        L.orc_mas_tables.argtypes = [ctypes.c_void_p, c_int_p, c_int_p]
+        L.orc_mas_matrices.argtypes = [ctypes.c_void_p, c_dbl_p, ctypes.c_int]
+        L.orc_mas_num_levels_for.argtypes = [ctypes.c_int, ctypes.c_int]
+        L.orc_mas_num_levels_for.restype = ctypes.c_int
+        L.orc_pcg.argtypes = [ctypes.c_long, c_dbl_p, c_int_p, c_int_p, ctypes.c_int, ctypes.c_void_p,
+                              c_dbl_p, c_dbl_p, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                              ctypes.c_int, c_dbl_p]
+        L.orc_pcg.restype = ctypes.c_int
+        L.orc_set_fault.argtypes = [ctypes.c_int]
+        _LIB = L
+    return _LIB
+
+
+def set_fault(k: int):
+    lib().orc_set_fault(int(k))
+
+
+# ---------------------------------------------------------------- element terms
+def tet_snh(scene, hess_offset=0, with_grad=True):
+    L = lib()
+    m = scene.n_tets
+    vals = np.zeros((10 * m, 9))
+    rows = np.zeros(10 * m, dtype=np.int32)
+    cols = np.zeros(10 * m, dtype=np.int32)
+    grad = np.zeros(3 * scene.n) if with_grad else None
+    x = np.ascontiguousarray(scene.x, dtype=np.float64)
+    tets = np.ascontiguousarray(scene.tets, dtype=np.int32)
+    L.orc_tet_snh(m, _ptr(x, c_dbl_p), _ptr(tets, c_int_p), _ptr(scene.dm_inv, c_dbl_p),
+                  _ptr(scene.vol, c_dbl_p), _ptr(scene.mu, c_dbl_p), _ptr(scene.lam, c_dbl_p),
+                  scene.dt, hess_offset, _ptr(grad, c_dbl_p), _ptr(vals, c_dbl_p),
+                  _ptr(rows, c_int_p), _ptr(cols, c_int_p))
+    return vals, rows, cols, grad
+
+
+def barrier_pt(scene, hess_offset=0, with_grad=True):
+    L = lib()
+    p = scene.n_pairs
+    vals = np.zeros((16 * p, 9))
+    rows = np.zeros(16 * p, dtype=np.int32)
+    cols = np.zeros(16 * p, dtype=np.int32)
+    grad = np.zeros(3 * scene.n) if with_grad else None
+    if p:
+        pairs = np.ascontiguousarray(scene.pairs, dtype=np.int32)
+        px = None if scene.pair_x is None else np.ascontiguousarray(scene.pair_x, dtype=np.float64)
+        L.orc_barrier_pt(p, _ptr(np.ascontiguousarray(scene.x), c_dbl_p), _ptr(pairs, c_int_p),
+                         _ptr(px, c_dbl_p), scene.dhat ** 2, scene.kappa, hess_offset,
+                         _ptr(grad, c_dbl_p), _ptr(vals, c_dbl_p), _ptr(rows, c_int_p), _ptr(cols, c_int_p))
+    return vals, rows, cols, grad
+
+
+def mass_diag(scene, hess_offset=0):
+    n = scene.n
+    vals = np.zeros((n, 9))
+    rows = np.zeros(n, dtype=np.int32)
+    cols = np.zeros(n, dtype=np.int32)
+    lib().orc_mass_diag(n, _ptr(np.ascontiguousarray(scene.mass), c_dbl_p), hess_offset,
+                        _ptr(vals, c_dbl_p), _ptr(rows, c_int_p), _ptr(cols, c_int_p))
+    return vals, rows, cols
+
+
+def triplets(scene, with_grad=False):
+    """All triplets in the reference's arena order: contacts, tets, mass."""
+    bv, br, bc, bg = barrier_pt(scene, with_grad=with_grad)
+    tv, tr, tc, tg = tet_snh(scene, with_grad=with_grad)
+    mv, mr, mc = mass_diag(scene)
+    vals = np.concatenate([bv, tv, mv])
+    rows = np.concatenate([br, tr, mr])
+    cols = np.concatenate([bc, tc, mc])
+    grad = (bg + tg) if with_grad else None
+    return vals, rows, cols, grad
+
+
+def convert(vals, rows, cols):
+    n = rows.size
+    ov = np.zeros((max(n, 1), 9))
+    orr = np.zeros(max(n, 1), dtype=np.int32)
+    oc = np.zeros(max(n, 1), dtype=np.int32)
+    vals = np.ascontiguousarray(vals, dtype=np.float64)
+    rows = np.ascontiguousarray(rows, dtype=np.int32)
+    cols = np.ascontiguousarray(cols, dtype=np.int32)
+    nnz = lib().orc_convert(n, _ptr(vals, c_dbl_p), _ptr(rows, c_int_p), _ptr(cols, c_int_p),
+                            _ptr(ov, c_dbl_p), _ptr(orr, c_int_p), _ptr(oc, c_int_p))
+    return ov[:nnz].copy(), orr[:nnz].copy(), oc[:nnz].copy()
+
+
+def assemble(scene):
+    """Unique upper BCOO (vals (nnz,9) column-major blocks, rows, cols)."""
+    v, r, c, _ = triplets(scene)
+    return convert(v, r, c)
+
+
+def spmv(a, vals, rows, cols, x, b, y, n_rows):
+    y = np.ascontiguousarray(y, dtype=np.float64).copy()
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    lib().orc_spmv(a, rows.size, _ptr(np.ascontiguousarray(vals), c_dbl_p), _ptr(rows, c_int_p),
+                   _ptr(cols, c_int_p), _ptr(x, c_dbl_p), b, _ptr(y, c_dbl_p), n_rows)
+    return y
+
+
+# ---------------------------------------------------------------------- MAS
+class Mas:
+    """Oracle MAS preconditioner (identity slot map)."""
+
+    def __init__(self, scene_or_n, bank_size=32, nbr=None, collision_node_offset=0):
+        if nbr is None:
+            s = scene_or_n
+            n, nbr = s.n, (s.nbr_start, s.nbr_num, s.nbr_list)
+        else:
+            n = int(scene_or_n)
+        self.n, self.B = n, bank_size
+        self._nbr = [np.ascontiguousarray(a, dtype=np.uint32) for a in nbr]
+        if self._nbr[2].size == 0:
+            self._nbr[2] = np.zeros(1, dtype=np.uint32)
+        self.h = lib().orc_mas_create(n, bank_size, *[_ptr(a, c_uint_p) for a in self._nbr],
+                                      collision_node_offset)
+        if not self.h:
+            raise ValueError("orc_mas_create failed")
+
+    def build(self, vals, rows, cols, offset=0, pairs=None):
+        pairs = np.zeros((0, 4), np.int32) if pairs is None else np.ascontiguousarray(pairs, np.int32)
+        self._keep = (np.ascontiguousarray(vals), rows, cols, pairs)
+        lib().orc_mas_build(self.h, rows.size, _ptr(self._keep[0], c_dbl_p), _ptr(rows, c_int_p),
+                            _ptr(cols, c_int_p), offset, _ptr(pairs if pairs.size else None, c_int_p),
+                            pairs.shape[0])
+
+    def apply(self, r, fp32=False):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.zeros_like(r)
+        lib().orc_mas_apply(self.h, _ptr(r, c_dbl_p), _ptr(z, c_dbl_p), int(bool(fp32)))
+        return z
+
+    def levels(self):
+        L = ctypes.c_int()
+        off = np.zeros(8, np.int32)
+        cnt = np.zeros(8, np.int32)
+        total = lib().orc_mas_levels(self.h, ctypes.byref(L), _ptr(off, c_int_p), _ptr(cnt, c_int_p))
+        return L.value, off[:L.value + 1].copy(), cnt[:L.value].copy(), total
+
+    def tables(self):
+        L, off, cnt, total = self.levels()
+        ct = np.zeros(max(self.n * (L - 1), 1), np.int32)
+        gn = np.zeros(max(total, 1), np.int32)
+        lib().orc_mas_tables(self.h, _ptr(ct, c_int_p), _ptr(gn, c_int_p))
+        return ct[:self.n * (L - 1)].reshape(self.n, L - 1), gn[:total]
+
+    def matrices(self, inverse=False):
+        L, off, cnt, total = self.levels()
+        nd = total // self.B
+        nb = self.B * (self.B + 1) // 2
+        out = np.zeros(nd * nb * 9)
+        lib().orc_mas_matrices(self.h, _ptr(out, c_dbl_p), int(bool(inverse)))
+        return out.reshape(nd, nb, 9)
+
+    def close(self):
+        if self.h:
+            lib().orc_mas_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def num_levels(n, bank_size):
+    return lib().orc_mas_num_levels_for(n, bank_size)
+
+
+# ---------------------------------------------------------------------- PCG
+@dataclasses.dataclass
+class OracleSolve:
+    x: np.ndarray
+    iterations: int
+    true_rel_residual: float
+    seconds: float
+    build_seconds: float
+
+
+def pcg(vals, rows, cols, n_rows, b, mas=None, tol=1e-5, stop_mode=1, max_iter=None, mas_fp32=False):
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros_like(b)
+    if max_iter is None:
+        max_iter = max(2, int(0.3 * b.size))
+    stats = np.zeros(4)
+    lib().orc_pcg(rows.size, _ptr(np.ascontiguousarray(vals), c_dbl_p), _ptr(rows, c_int_p),
+                  _ptr(cols, c_int_p), n_rows, mas.h if mas is not None else None, _ptr(b, c_dbl_p),
+                  _ptr(x, c_dbl_p), tol, stop_mode, max_iter, int(bool(mas_fp32)), _ptr(stats, c_dbl_p))
+    return x, int(stats[0]), float(stats[1])
+
+
+def solve_scene(scene, bank_size=32, tol=1e-5, stop_mode=1, mas_fp32=False, max_iter=None):
+    """Assemble, build MAS and run PCG on the CPU (the plain FP64 path)."""
+    t0 = time.perf_counter()
+    vals, rows, cols = assemble(scene)
+    mas = Mas(scene, bank_size)
+    mas.build(vals, rows, cols, 0, scene.pairs)
+    t1 = time.perf_counter()
+    x, it, res = pcg(vals, rows, cols, scene.n, scene.rhs, mas, tol, stop_mode, max_iter, mas_fp32)
+    t2 = time.perf_counter()
+    mas.close()
+    return OracleSolve(x=x, iterations=it, true_rel_residual=res, seconds=t2 - t0, build_seconds=t1 - t0)

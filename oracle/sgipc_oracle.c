@@ -1,0 +1,928 @@
+/*
+ * CPU ORACLE -- TEST INFRASTRUCTURE ONLY (see sgipc_oracle.h).
+ *
+ * Plain C restatement of the reference's Newton linear-solve path.  Each
+ * function cites the reference file:line it follows.  The oracle is written for
+ * clarity first; OpenMP parallel loops keep the full-size scenes tractable when
+ * it is timed as the CPU baseline.
+ */
+#include "sgipc_oracle.h"
+
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+static int g_fault = 0;
+void orc_set_fault(int fault) { g_fault = fault; }
+
+/* ------------------------------------------------------------------------- */
+/* small dense helpers                                                        */
+/* ------------------------------------------------------------------------- */
+static double det3(const double F[3][3])
+{
+    return F[0][0] * (F[1][1] * F[2][2] - F[1][2] * F[2][1]) -
+           F[0][1] * (F[1][0] * F[2][2] - F[1][2] * F[2][0]) +
+           F[0][2] * (F[1][0] * F[2][1] - F[1][1] * F[2][0]);
+}
+
+static int levi(int i, int j, int k)
+{
+    if (i == j || j == k || i == k) return 0;
+    /* even permutations of (0,1,2) */
+    if ((i == 0 && j == 1 && k == 2) || (i == 1 && j == 2 && k == 0) || (i == 2 && j == 0 && k == 1))
+        return 1;
+    return -1;
+}
+
+/* Cyclic Jacobi eigen-decomposition of a symmetric n x n matrix (row-major).
+ * On return A holds the eigenvalues on its diagonal, V (row-major) the
+ * eigenvectors as columns. */
+static void jacobi_eig(int n, double* A, double* V)
+{
+    for (int i = 0; i < n * n; ++i) V[i] = 0.0;
+    for (int i = 0; i < n; ++i) V[i * n + i] = 1.0;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0, tot = 0.0;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) {
+                double a = A[i * n + j] * A[i * n + j];
+                tot += a;
+                if (i != j) off += a;
+            }
+        if (off <= 1e-32 * tot || off == 0.0) break;
+        for (int p = 0; p < n - 1; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                double apq = A[p * n + q];
+                if (apq == 0.0) continue;
+                double app = A[p * n + p], aqq = A[q * n + q];
+                double theta = (aqq - app) / (2.0 * apq);
+                double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < n; ++k) {
+                    double akp = A[k * n + p], akq = A[k * n + q];
+                    A[k * n + p] = c * akp - s * akq;
+                    A[k * n + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; ++k) {
+                    double apk = A[p * n + k], aqk = A[q * n + k];
+                    A[p * n + k] = c * apk - s * aqk;
+                    A[q * n + k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < n; ++k) {
+                    double vkp = V[k * n + p], vkq = V[k * n + q];
+                    V[k * n + p] = c * vkp - s * vkq;
+                    V[k * n + q] = s * vkp + c * vkq;
+                }
+            }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Stable Neo-Hookean tet                                                     */
+/* StiffGIPC/femEnergy.cu:432-459 (PK1), 1641-1717 (projected Hessian),       */
+/* 1719-1857 (kernel), 1389-1420 (dF/dx), 324-350 (Ds).                       */
+/* Psi(F) = mu/2 (|F|^2 - 3) - mu (J - 1) + lam/2 (J - 1)^2 with mu = lengthRate,  */
+/* lam = volumeRate; P = mu F + (lam (J-1) - mu) dJ/dF.                        */
+/* vec(F) is column-major: q = 3*j + i  <->  F(i,j).                          */
+/* ------------------------------------------------------------------------- */
+static void tet_F(const double* xs /* 12 */, const double* dminv /* 9 rm */, double F[3][3])
+{
+    double Ds[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) Ds[i][k] = xs[3 * (k + 1) + i] - xs[i];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double s = 0;
+            for (int k = 0; k < 3; ++k) s += Ds[i][k] * dminv[3 * k + j];
+            F[i][j] = s;
+        }
+}
+
+double orc_tet_snh_energy(const double* x12, const double* dm_inv, double mu, double lam)
+{
+    double F[3][3];
+    tet_F(x12, dm_inv, F);
+    double I2 = 0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) I2 += F[i][j] * F[i][j];
+    double J = det3(F);
+    return 0.5 * mu * (I2 - 3.0) - mu * (J - 1.0) + 0.5 * lam * (J - 1.0) * (J - 1.0);
+}
+
+static void snh_hessian9(const double F[3][3], double mu, double lam, double H[81])
+{
+    double J = det3(F);
+    double g[9];
+    /* cofactor = dJ/dF */
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double s = 0;
+            for (int k = 0; k < 3; ++k)
+                for (int l = 0; l < 3; ++l)
+                    for (int m = 0; m < 3; ++m)
+                        for (int n = 0; n < 3; ++n)
+                            s += 0.5 * levi(i, k, m) * levi(j, l, n) * F[k][l] * F[m][n];
+            g[3 * j + i] = s;
+        }
+    double c = lam * (J - 1.0) - mu;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            for (int k = 0; k < 3; ++k)
+                for (int l = 0; l < 3; ++l) {
+                    int q = 3 * j + i, r = 3 * l + k;
+                    double hj = 0;
+                    for (int m = 0; m < 3; ++m)
+                        for (int n = 0; n < 3; ++n) hj += levi(i, k, m) * levi(j, l, n) * F[m][n];
+                    H[q * 9 + r] = (q == r ? mu : 0.0) + lam * g[q] * g[r] + c * hj;
+                }
+}
+
+static void project_psd9(const double H[81], double P[81])
+{
+    double A[81], V[81];
+    memcpy(A, H, sizeof(A));
+    /* symmetrise against round-off before the eigen-solve */
+    for (int i = 0; i < 9; ++i)
+        for (int j = i + 1; j < 9; ++j) {
+            double s = 0.5 * (A[i * 9 + j] + A[j * 9 + i]);
+            A[i * 9 + j] = A[j * 9 + i] = s;
+        }
+    jacobi_eig(9, A, V);
+    double lam[9];
+    for (int i = 0; i < 9; ++i) lam[i] = A[i * 9 + i] > 0.0 ? A[i * 9 + i] : 0.0;
+    for (int i = 0; i < 9; ++i)
+        for (int j = 0; j < 9; ++j) {
+            double s = 0;
+            for (int k = 0; k < 9; ++k) s += V[i * 9 + k] * lam[k] * V[j * 9 + k];
+            P[i * 9 + j] = s;
+        }
+}
+
+void orc_snh_dpdf(const double* F9, double mu, double lam, double* H81, double* Hproj81)
+{
+    double F[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) F[i][j] = F9[3 * j + i];
+    double H[81];
+    snh_hessian9(F, mu, lam, H);
+    if (H81) memcpy(H81, H, sizeof(H));
+    if (Hproj81) project_psd9(H, Hproj81);
+}
+
+/* upper-block writer of femEnergy.cu:1832-1856: block (ii,jj), ii<=jj local;
+ * if the global row > col the transposed block is stored at (col,row). */
+static void write_upper_blocks(int nv, const int* gid, const double* H /* (3nv)^2 rm */,
+                               double* tvals, int* trows, int* tcols)
+{
+    int n3 = 3 * nv, kk = 0;
+    for (int ii = 0; ii < nv; ++ii)
+        for (int jj = ii; jj < nv; ++jj, ++kk) {
+            int row = gid[ii], col = gid[jj];
+            double* v = tvals + 9 * (size_t)kk;
+            if (row <= col) {
+                trows[kk] = row; tcols[kk] = col;
+                for (int r = 0; r < 3; ++r)
+                    for (int c = 0; c < 3; ++c) v[c * 3 + r] = H[(3 * ii + r) * n3 + 3 * jj + c];
+            } else {
+                trows[kk] = col; tcols[kk] = row;
+                for (int r = 0; r < 3; ++r)
+                    for (int c = 0; c < 3; ++c) v[c * 3 + r] = H[(3 * jj + r) * n3 + 3 * ii + c];
+            }
+        }
+}
+
+int orc_tet_snh(int n_tets, const double* x, const int* tets, const double* dm_inv,
+                const double* vol, const double* mu, const double* lam, double dt,
+                int hess_offset, double* grad, double* tvals, int* trows, int* tcols)
+{
+#pragma omp parallel for schedule(static)
+    for (int t = 0; t < n_tets; ++t) {
+        const int* tv = tets + 4 * (size_t)t;
+        double xs[12];
+        for (int a = 0; a < 4; ++a)
+            for (int d = 0; d < 3; ++d) xs[3 * a + d] = x[3 * (size_t)tv[a] + d];
+        const double* Dmi = dm_inv + 9 * (size_t)t;
+        double F[3][3];
+        tet_F(xs, Dmi, F);
+        double Hq[81], Hp[81];
+        snh_hessian9(F, mu[t], lam[t], Hq);
+        project_psd9(Hq, Hp);
+        /* D(a,j) = dF(:,j)/dx_a coefficient */
+        double D[4][3];
+        for (int j = 0; j < 3; ++j) {
+            D[0][j] = -(Dmi[0 * 3 + j] + Dmi[1 * 3 + j] + Dmi[2 * 3 + j]);
+            for (int a = 1; a < 4; ++a) D[a][j] = Dmi[(a - 1) * 3 + j];
+        }
+        double scale = dt * dt * vol[t];
+        if (grad) {
+            double J = det3(F);
+            double P[3][3], cof[3][3];
+            cof[0][0] = F[1][1] * F[2][2] - F[1][2] * F[2][1];
+            cof[0][1] = F[1][2] * F[2][0] - F[1][0] * F[2][2];
+            cof[0][2] = F[1][0] * F[2][1] - F[1][1] * F[2][0];
+            cof[1][0] = F[2][1] * F[0][2] - F[2][2] * F[0][1];
+            cof[1][1] = F[2][2] * F[0][0] - F[2][0] * F[0][2];
+            cof[1][2] = F[2][0] * F[0][1] - F[2][1] * F[0][0];
+            cof[2][0] = F[0][1] * F[1][2] - F[1][1] * F[0][2];
+            cof[2][1] = F[0][2] * F[1][0] - F[0][0] * F[1][2];
+            cof[2][2] = F[0][0] * F[1][1] - F[0][1] * F[1][0];
+            double c = lam[t] * (J - 1.0) - mu[t];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) P[i][j] = mu[t] * F[i][j] + c * cof[i][j];
+            for (int a = 0; a < 4; ++a)
+                for (int d = 0; d < 3; ++d) {
+                    double s = 0;
+                    for (int j = 0; j < 3; ++j) s += D[a][j] * P[d][j];
+                    s *= scale;
+#pragma omp atomic
+                    grad[3 * (size_t)tv[a] + d] += s;
+                }
+        }
+        double H12[144];
+        for (int a = 0; a < 4; ++a)
+            for (int d = 0; d < 3; ++d)
+                for (int b = 0; b < 4; ++b)
+                    for (int e = 0; e < 3; ++e) {
+                        double s = 0;
+                        for (int j = 0; j < 3; ++j)
+                            for (int jp = 0; jp < 3; ++jp)
+                                s += D[a][j] * D[b][jp] * Hp[(3 * j + d) * 9 + 3 * jp + e];
+                        H12[(3 * a + d) * 12 + 3 * b + e] = scale * s;
+                    }
+        int gid[4] = {tv[0] + hess_offset, tv[1] + hess_offset, tv[2] + hess_offset,
+                      tv[3] + hess_offset};
+        write_upper_blocks(4, gid, H12, tvals + 90 * (size_t)t, trows + 10 * (size_t)t,
+                           tcols + 10 * (size_t)t);
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Point-triangle barrier, GIPC rank-2 variant                               */
+/* StiffGIPC/GIPC.cu:4733-4984 (NEWF branch), mlbvh.cu:234-242 (_d_PT),       */
+/* GIPC_PDerivative.cuh:2095-2291 (only column 8 of dF/dx is non-zero: the    */
+/* gradient of s = dist/sqrt(dHat)).                                          */
+/* E(I5) = kappa dHat^2 (I5-1)^2 ln(I5)^2 with I5 = s^2; lambda0 = d2E/ds2.   */
+/* ------------------------------------------------------------------------- */
+double orc_barrier_energy(double I5, double dhat_sq, double kappa)
+{
+    double L = log(I5);
+    return kappa * dhat_sq * dhat_sq * (I5 - 1.0) * (I5 - 1.0) * L * L;
+}
+
+double orc_barrier_lambda0(double I5, double dhat_sq, double kappa)
+{
+    double L = log(I5);
+    double lsign = (g_fault == 3) ? -1.0 : 1.0;
+    return -(4.0 * kappa * dhat_sq * dhat_sq *
+             (4.0 * I5 + lsign * L - 3.0 * I5 * I5 * L * L + 6.0 * I5 * L - 2.0 * I5 * I5 +
+              I5 * L * L - 7.0 * I5 * I5 * L - 2.0)) /
+           I5;
+}
+
+int orc_barrier_pt(int n_pairs, const double* x, const int* pairs, const double* pair_x,
+                   double dhat_sq, double kappa, int hess_offset, double* grad,
+                   double* tvals, int* trows, int* tcols)
+{
+    const double gass = 1e-6;
+#pragma omp parallel for schedule(static)
+    for (int p = 0; p < n_pairs; ++p) {
+        const int* pr = pairs + 4 * (size_t)p;
+        int vid[4] = {-pr[0] - 1, pr[1], pr[2], pr[3]};
+        double P[4][3];
+        for (int a = 0; a < 4; ++a)
+            for (int d = 0; d < 3; ++d)
+                P[a][d] = pair_x ? pair_x[12 * (size_t)p + 3 * a + d] : x[3 * (size_t)vid[a] + d];
+        double e1[3], e2[3], w[3], b[3];
+        for (int d = 0; d < 3; ++d) {
+            e1[d] = P[2][d] - P[1][d];
+            e2[d] = P[3][d] - P[1][d];
+            w[d] = P[0][d] - P[1][d];
+        }
+        b[0] = e1[1] * e2[2] - e1[2] * e2[1];
+        b[1] = e1[2] * e2[0] - e1[0] * e2[2];
+        b[2] = e1[0] * e2[1] - e1[1] * e2[0];
+        double bb = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
+        double aTb = w[0] * b[0] + w[1] * b[1] + w[2] * b[2];
+        double d2 = aTb * aTb / bb;
+        double dis = sqrt(d2);
+        double sq = sqrt(dhat_sq);
+        double I5 = d2 / dhat_sq;
+        double nb = sqrt(bb), sgn = aTb >= 0 ? 1.0 : -1.0;
+        double nh[3] = {b[0] / nb, b[1] / nb, b[2] / nb};
+        /* barycentric coordinates of the foot point */
+        double c1[3], c2[3];
+        c1[0] = w[1] * e2[2] - w[2] * e2[1];
+        c1[1] = w[2] * e2[0] - w[0] * e2[2];
+        c1[2] = w[0] * e2[1] - w[1] * e2[0];
+        c2[0] = e1[1] * w[2] - e1[2] * w[1];
+        c2[1] = e1[2] * w[0] - e1[0] * w[2];
+        c2[2] = e1[0] * w[1] - e1[1] * w[0];
+        double be1 = (c1[0] * b[0] + c1[1] * b[1] + c1[2] * b[2]) / bb;
+        double be2 = (c2[0] * b[0] + c2[1] * b[1] + c2[2] * b[2]) / bb;
+        double be0 = 1.0 - be1 - be2;
+        double coef[4] = {1.0, -be0, -be1, -be2};
+        double g[12];
+        for (int a = 0; a < 4; ++a)
+            for (int d = 0; d < 3; ++d) g[3 * a + d] = sgn * coef[a] * nh[d] / sq;
+        double Il = I5;
+        if (d2 < gass * dhat_sq) Il = gass;
+        double lam0 = orc_barrier_lambda0(Il, dhat_sq, kappa);
+        if (grad) {
+            double L = log(I5);
+            double s = dis / sq;
+            double pk = 2.0 * (2.0 * kappa * dhat_sq * dhat_sq * L * (I5 - 1.0) * (I5 + I5 * L - 1.0)) / I5;
+            for (int a = 0; a < 4; ++a)
+                for (int d = 0; d < 3; ++d) {
+                    double v = g[3 * a + d] * s * pk;
+#pragma omp atomic
+                    grad[3 * (size_t)vid[a] + d] += v;
+                }
+        }
+        double* tv = tvals + 144 * (size_t)p;
+        int* tr = trows + 16 * (size_t)p;
+        int* tc = tcols + 16 * (size_t)p;
+        for (int ii = 0; ii < 4; ++ii)
+            for (int jj = 0; jj < 4; ++jj) {
+                int kk = ii * 4 + jj;
+                int row = vid[ii] + hess_offset, col = vid[jj] + hess_offset;
+                double* v = tv + 9 * kk;
+                if (row <= col) {
+                    tr[kk] = row; tc[kk] = col;
+                    for (int r = 0; r < 3; ++r)
+                        for (int c = 0; c < 3; ++c) v[c * 3 + r] = lam0 * g[3 * ii + r] * g[3 * jj + c];
+                } else {
+                    tr[kk] = col; tc[kk] = row;
+                    for (int e = 0; e < 9; ++e) v[e] = 0.0;
+                }
+            }
+    }
+    return 0;
+}
+
+int orc_mass_diag(int n, const double* mass, int hess_offset, double* tvals, int* trows,
+                  int* tcols)
+{
+    for (int i = 0; i < n; ++i) {
+        double* v = tvals + 9 * (size_t)i;
+        for (int e = 0; e < 9; ++e) v[e] = 0.0;
+        v[0] = v[4] = v[8] = mass[i];
+        trows[i] = tcols[i] = i + hess_offset;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Converter::srbk_convert -- StiffGIPC/linear_system/utils/converter.cu:32-51, */
+/* 88-131, 171-238: radix sort on (row<<32)+col, reduce by key.               */
+/* ------------------------------------------------------------------------- */
+static const uint64_t* g_sort_keys;
+static int cmp_idx(const void* a, const void* b)
+{
+    long ia = *(const long*)a, ib = *(const long*)b;
+    uint64_t ka = g_sort_keys[ia], kb = g_sort_keys[ib];
+    if (ka < kb) return -1;
+    if (ka > kb) return 1;
+    return ia < ib ? -1 : (ia > ib);
+}
+
+long orc_convert(long n_trip, const double* tvals, const int* trows, const int* tcols,
+                 double* ovals, int* orows, int* ocols)
+{
+    if (n_trip <= 0) return 0;
+    uint64_t* keys = (uint64_t*)malloc(sizeof(uint64_t) * n_trip);
+    long* idx = (long*)malloc(sizeof(long) * n_trip);
+    for (long i = 0; i < n_trip; ++i) {
+        keys[i] = ((uint64_t)(uint32_t)trows[i] << 32) + (uint64_t)(uint32_t)tcols[i];
+        idx[i] = i;
+    }
+    g_sort_keys = keys;
+    qsort(idx, (size_t)n_trip, sizeof(long), cmp_idx);
+    long u = -1;
+    uint64_t prev = 0;
+    for (long s = 0; s < n_trip; ++s) {
+        long i = idx[s];
+        if (u < 0 || keys[i] != prev) {
+            ++u;
+            prev = keys[i];
+            orows[u] = trows[i];
+            ocols[u] = tcols[i];
+            for (int e = 0; e < 9; ++e) ovals[9 * u + e] = 0.0;
+        }
+        for (int e = 0; e < 9; ++e) ovals[9 * u + e] += tvals[9 * i + e];
+    }
+    free(keys);
+    free(idx);
+    return u + 1;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Spmv::srbk_spmv -- StiffGIPC/linear_system/utils/spmv.cu:188-313            */
+/* y = b*y; for each upper block (i,j,H): y_i += a H x_j; if i != j          */
+/* y_j += a H^T x_i.                                                          */
+/* ------------------------------------------------------------------------- */
+void orc_spmv(double a, long nnz, const double* vals, const int* rows, const int* cols,
+              const double* x, double b, double* y, int n_rows)
+{
+    long n3 = 3L * n_rows;
+    for (long i = 0; i < n3; ++i) y[i] = (b != 0.0) ? b * y[i] : 0.0;
+    int nt = 1;
+#ifdef _OPENMP
+    nt = omp_get_max_threads();
+#endif
+    double* priv = (double*)calloc((size_t)nt * n3, sizeof(double));
+#pragma omp parallel
+    {
+        int t = 0;
+#ifdef _OPENMP
+        t = omp_get_thread_num();
+#endif
+        double* yp = priv + (size_t)t * n3;
+#pragma omp for schedule(static)
+        for (long k = 0; k < nnz; ++k) {
+            int i = rows[k], j = cols[k];
+            const double* H = vals + 9 * k;
+            const double* xj = x + 3L * j;
+            const double* xi = x + 3L * i;
+            for (int r = 0; r < 3; ++r)
+                yp[3L * i + r] += a * (H[0 * 3 + r] * xj[0] + H[1 * 3 + r] * xj[1] + H[2 * 3 + r] * xj[2]);
+            if (i != j && g_fault != 1)
+                for (int c = 0; c < 3; ++c)
+                    yp[3L * j + c] += a * (H[c * 3 + 0] * xi[0] + H[c * 3 + 1] * xi[1] + H[c * 3 + 2] * xi[2]);
+        }
+#pragma omp for schedule(static)
+        for (long e = 0; e < n3; ++e) {
+            double s = 0;
+            for (int q = 0; q < nt; ++q) s += priv[(size_t)q * n3 + e];
+            y[e] += s;
+        }
+    }
+    free(priv);
+}
+
+/* ------------------------------------------------------------------------- */
+/* MAS preconditioner -- StiffGIPC/MASPreconditioner.cu                       */
+/* Hierarchy (ReorderRealtime :1785-1826 and the kernels :29-625,1132-1548):  */
+/* at level l the nodes are split into domains of B consecutive ids; the      */
+/* graph of (mesh edges + contact cliques) mapped to level l and restricted   */
+/* to edges inside one domain is split into connected components; component  */
+/* ids at level l+1 run over the domains in order and, inside a domain, over  */
+/* the components by their lowest member.  Slots of level l start at off_l,   */
+/* off_1 = ceil(n/B)*B, off_{l+1} = off_l + ceil(c_l/B)*B.                    */
+/* Galerkin blocks (PrepareHessian_bcoo :1829-2106) are P_l^T A P_l limited   */
+/* to pairs of nodes in one level-l domain; each domain is inverted densely   */
+/* in FP64 by Gauss-Jordan without pivoting, zero diagonal -> 1 (:630-726).   */
+/* Apply (:2248-2289): z = sum_l P_l blockinv(A_l) P_l^T r.                   */
+/* ------------------------------------------------------------------------- */
+#define ORC_MAX_LEVELS 6
+
+struct orc_mas {
+    int n, B, L, coff;
+    unsigned *nbr_start, *nbr_num, *nbr_list;
+    long nbr_total;
+    int counts[ORC_MAX_LEVELS + 1];
+    int offsets[ORC_MAX_LEVELS + 2];
+    int total;
+    int* map;        /* map[l*n + v] local id of v at level l, l = 0..L-1 */
+    int* going_next; /* [total] */
+    int nblk;
+    double* Mgal;    /* [domains * nblk * 9] */
+    double* Minv;
+    float* Minv32;
+};
+
+int orc_mas_num_levels_for(int n, int B)
+{
+    /* computeNumLevels, MASPreconditioner.cu:1731-1750 */
+    int nLevel = 1;
+    int levelSz = (n + B - 1) / B * B;
+    while (levelSz > B) {
+        levelSz /= B;
+        nLevel++;
+        levelSz = (levelSz + B - 1) / B * B;
+    }
+    nLevel = nLevel + 1;
+    return nLevel > ORC_MAX_LEVELS ? ORC_MAX_LEVELS : nLevel;
+}
+
+orc_mas* orc_mas_create(int n, int bank, const unsigned* nbr_start, const unsigned* nbr_num,
+                        const unsigned* nbr_list, int collision_node_offset)
+{
+    if (n < 1 || (bank != 16 && bank != 32)) return NULL;
+    orc_mas* m = (orc_mas*)calloc(1, sizeof(orc_mas));
+    m->n = n;
+    m->B = bank;
+    m->coff = collision_node_offset;
+    m->L = orc_mas_num_levels_for(n, bank);
+    long tot = 0;
+    for (int v = 0; v < n; ++v) {
+        long e = (long)nbr_start[v] + nbr_num[v];
+        if (e > tot) tot = e;
+    }
+    m->nbr_total = tot;
+    m->nbr_start = (unsigned*)malloc(sizeof(unsigned) * n);
+    m->nbr_num = (unsigned*)malloc(sizeof(unsigned) * n);
+    m->nbr_list = (unsigned*)malloc(sizeof(unsigned) * (tot > 0 ? tot : 1));
+    memcpy(m->nbr_start, nbr_start, sizeof(unsigned) * n);
+    memcpy(m->nbr_num, nbr_num, sizeof(unsigned) * n);
+    if (tot) memcpy(m->nbr_list, nbr_list, sizeof(unsigned) * tot);
+    m->map = (int*)malloc(sizeof(int) * (size_t)n * m->L);
+    m->nblk = bank * (bank + 1) / 2;
+    return m;
+}
+
+void orc_mas_destroy(orc_mas* m)
+{
+    if (!m) return;
+    free(m->nbr_start); free(m->nbr_num); free(m->nbr_list); free(m->map);
+    free(m->going_next); free(m->Mgal); free(m->Minv); free(m->Minv32);
+    free(m);
+}
+
+static int uf_find(int* par, int a)
+{
+    while (par[a] != a) {
+        par[a] = par[par[a]];
+        a = par[a];
+    }
+    return a;
+}
+static void uf_union(int* par, int a, int b)
+{
+    a = uf_find(par, a);
+    b = uf_find(par, b);
+    if (a != b) {
+        if (a < b) par[b] = a; else par[a] = b;
+    }
+}
+
+/* Decode a contact pair into up to 4 vertex ids (minus node offset, -1 if none).
+ * _buildCollisionConnection_new, MASPreconditioner.cu:1132-1548. */
+static int decode_pair(const int* pr, int coff, int out[4])
+{
+    int v[4] = {pr[0], pr[1], pr[2], pr[3]};
+    int cnt = 4;
+    if (v[0] >= 0) {
+        if (v[3] < 0) v[3] = -v[3] - 1;
+    } else {
+        v[0] = -v[0] - 1;
+        if (v[2] < 0) {
+            if (v[1] < 0) {
+                v[1] = -v[1] - 1; v[2] = -v[2] - 1; v[3] = -v[3] - 1;
+            } else {
+                cnt = 2;
+            }
+        } else if (v[3] < 0) {
+            if (v[1] < 0) {
+                v[1] = -v[1] - 1; v[3] = -v[3] - 1;
+            } else {
+                cnt = 3;
+            }
+        }
+    }
+    for (int i = 0; i < 4; ++i) out[i] = (i < cnt) ? v[i] - coff : -1;
+    return cnt;
+}
+
+static void mas_hierarchy(orc_mas* m, const int* pairs, int n_pairs)
+{
+    int n = m->n, B = m->B, L = m->L;
+    for (int v = 0; v < n; ++v) m->map[v] = v;
+    m->counts[0] = n;
+    m->offsets[0] = 0;
+    m->offsets[1] = (n + B - 1) / B * B;
+    int* par = (int*)malloc(sizeof(int) * n);
+    int* newid = (int*)malloc(sizeof(int) * n);
+    for (int l = 0; l < L; ++l) {
+        /* build level l+1 from level l (also for l = L-1, as the reference
+         * computes c_L for the final offset, although it is never stored) */
+        int cl = m->counts[l];
+        const int* ml = m->map + (size_t)l * n;
+        for (int a = 0; a < cl; ++a) par[a] = a;
+        for (int u = 0; u < n; ++u) {
+            int a = ml[u];
+            for (unsigned k = 0; k < m->nbr_num[u]; ++k) {
+                int v = (int)m->nbr_list[m->nbr_start[u] + k];
+                int b = ml[v];
+                if (a != b && a / B == b / B) uf_union(par, a, b);
+            }
+        }
+        for (int p = 0; p < n_pairs; ++p) {
+            int vid[4];
+            decode_pair(pairs + 4 * (size_t)p, m->coff, vid);
+            int cv[4];
+            for (int i = 0; i < 4; ++i) cv[i] = (vid[i] >= 0 && vid[i] < n) ? ml[vid[i]] : -1;
+            for (int i = 0; i < 4; ++i)
+                for (int j = i + 1; j < 4; ++j)
+                    if (cv[i] >= 0 && cv[j] >= 0 && cv[i] != cv[j] && cv[i] / B == cv[j] / B)
+                        uf_union(par, cv[i], cv[j]);
+        }
+        /* number components domain by domain, in order of their lowest member */
+        int next = 0;
+        for (int a = 0; a < cl; ++a) newid[a] = -1;
+        for (int a = 0; a < cl; ++a) {
+            int r = uf_find(par, a);
+            if (newid[r] < 0) newid[r] = next++;
+            newid[a] = newid[r];
+        }
+        m->counts[l + 1] = next;
+        m->offsets[l + 1] = m->offsets[l] + (cl + B - 1) / B * B;
+        if (l + 1 < L) {
+            int* mn = m->map + (size_t)(l + 1) * n;
+            for (int u = 0; u < n; ++u) mn[u] = newid[ml[u]];
+        }
+    }
+    m->total = m->offsets[L];
+    free(m->going_next);
+    m->going_next = (int*)malloc(sizeof(int) * (size_t)(m->total > 0 ? m->total : 1));
+    for (int s = 0; s < m->total; ++s) m->going_next[s] = -1;
+    for (int l = 0; l + 1 < L; ++l) {
+        const int* ml = m->map + (size_t)l * n;
+        const int* mn = m->map + (size_t)(l + 1) * n;
+        for (int u = 0; u < n; ++u) m->going_next[m->offsets[l] + ml[u]] = m->offsets[l + 1] + mn[u];
+    }
+    /* the last level points into level L (never read by the apply); newid[]
+     * still holds the level-L ids of the level-(L-1) nodes */
+    {
+        int l = L - 1;
+        const int* ml = m->map + (size_t)l * n;
+        for (int u = 0; u < n; ++u) m->going_next[m->offsets[l] + ml[u]] = m->offsets[L] + newid[ml[u]];
+    }
+    free(par);
+    free(newid);
+}
+
+static inline int blk_index(int B, int r, int c) { return B * r - r * (r + 1) / 2 + c; }
+
+static void add_block(double* dst, const double* H, int transpose, double sgn)
+{
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+            dst[c * 3 + r] += sgn * (transpose ? H[r * 3 + c] : H[c * 3 + r]);
+}
+
+/* Gauss-Jordan inverse without pivoting of an n x n row-major matrix. */
+static void gj_inverse(int n, double* A, double* I)
+{
+    for (int i = 0; i < n * n; ++i) I[i] = 0.0;
+    for (int i = 0; i < n; ++i) I[i * n + i] = 1.0;
+    for (int j = 0; j < n; ++j) {
+        double piv = A[j * n + j];
+        double inv = 1.0 / piv;
+        for (int c = 0; c < n; ++c) {
+            A[j * n + c] *= inv;
+            I[j * n + c] *= inv;
+        }
+        for (int k = 0; k < n; ++k) {
+            if (k == j) continue;
+            double f = A[k * n + j];
+            if (f == 0.0) continue;
+            for (int c = 0; c < n; ++c) {
+                A[k * n + c] -= f * A[j * n + c];
+                I[k * n + c] -= f * I[j * n + c];
+            }
+        }
+    }
+}
+
+int orc_mas_build(orc_mas* m, long nnz, const double* vals, const int* rows, const int* cols,
+                  int offset, const int* pairs, int n_pairs)
+{
+    int n = m->n, B = m->B, L = m->L, nb = m->nblk;
+    mas_hierarchy(m, pairs, n_pairs);
+    int ndom = m->total / B;
+    free(m->Mgal); free(m->Minv); free(m->Minv32);
+    m->Mgal = (double*)calloc((size_t)ndom * nb * 9, sizeof(double));
+    m->Minv = (double*)calloc((size_t)ndom * nb * 9, sizeof(double));
+    m->Minv32 = (float*)calloc((size_t)ndom * nb * 9, sizeof(float));
+    for (long k = 0; k < nnz; ++k) {
+        int i = rows[k] - offset, j = cols[k] - offset;
+        if (i < 0 || j < 0 || i >= n || j >= n) continue;
+        const double* H = vals + 9 * k;
+        for (int l = 0; l < L; ++l) {
+            int a = m->map[(size_t)l * n + i], b = m->map[(size_t)l * n + j];
+            if (a / B != b / B) continue;
+            int dom = (m->offsets[l] + a) / B;
+            double* D = m->Mgal + (size_t)dom * nb * 9;
+            int ra = a % B, rb = b % B;
+            if (a == b) {
+                double* blk = D + 9 * blk_index(B, ra, ra);
+                add_block(blk, H, 0, 1.0);
+                if (i != j && !(g_fault == 2 && l > 0)) add_block(blk, H, 1, 1.0);
+            } else if (a < b) {
+                add_block(D + 9 * blk_index(B, ra, rb), H, 0, 1.0);
+            } else {
+                add_block(D + 9 * blk_index(B, rb, ra), H, 1, 1.0);
+            }
+        }
+    }
+    int n3 = 3 * B;
+#pragma omp parallel
+    {
+        double* A = (double*)malloc(sizeof(double) * n3 * n3);
+        double* I = (double*)malloc(sizeof(double) * n3 * n3);
+#pragma omp for schedule(dynamic, 4)
+        for (int d = 0; d < ndom; ++d) {
+            const double* D = m->Mgal + (size_t)d * nb * 9;
+            for (int rb = 0; rb < B; ++rb)
+                for (int cb = 0; cb < B; ++cb)
+                    for (int i = 0; i < 3; ++i)
+                        for (int j = 0; j < 3; ++j) {
+                            double v;
+                            if (cb >= rb) v = D[9 * blk_index(B, rb, cb) + j * 3 + i];
+                            else v = D[9 * blk_index(B, cb, rb) + i * 3 + j];
+                            A[(3 * rb + i) * n3 + 3 * cb + j] = v;
+                        }
+            for (int i = 0; i < n3; ++i)
+                if (A[i * n3 + i] == 0.0) A[i * n3 + i] = 1.0;
+            gj_inverse(n3, A, I);
+            /* symmetrise the 3x3 diagonal blocks from their upper part */
+            for (int rb = 0; rb < B; ++rb)
+                for (int i = 0; i < 3; ++i)
+                    for (int j = i + 1; j < 3; ++j)
+                        I[(3 * rb + j) * n3 + 3 * rb + i] = I[(3 * rb + i) * n3 + 3 * rb + j];
+            double* O = m->Minv + (size_t)d * nb * 9;
+            float* O32 = m->Minv32 + (size_t)d * nb * 9;
+            for (int rb = 0; rb < B; ++rb)
+                for (int cb = rb; cb < B; ++cb)
+                    for (int i = 0; i < 3; ++i)
+                        for (int j = 0; j < 3; ++j) {
+                            double v = I[(3 * rb + i) * n3 + 3 * cb + j];
+                            O[9 * blk_index(B, rb, cb) + j * 3 + i] = v;
+                            O32[9 * blk_index(B, rb, cb) + j * 3 + i] = (float)v;
+                        }
+        }
+        free(A);
+        free(I);
+    }
+    return 0;
+}
+
+void orc_mas_apply(orc_mas* m, const double* r, double* z, int fp32)
+{
+    int n = m->n, B = m->B, L = m->L, nb = m->nblk, tot = m->total;
+    int ndom = tot / B;
+    if (!fp32) {
+        double* R = (double*)calloc((size_t)tot * 3, sizeof(double));
+        double* Z = (double*)calloc((size_t)tot * 3, sizeof(double));
+        for (int l = 0; l < L; ++l) {
+            const int* ml = m->map + (size_t)l * n;
+            for (int v = 0; v < n; ++v)
+                for (int d = 0; d < 3; ++d) R[3 * (size_t)(m->offsets[l] + ml[v]) + d] += r[3 * (size_t)v + d];
+        }
+#pragma omp parallel for schedule(static)
+        for (int dm = 0; dm < ndom; ++dm) {
+            const double* M = m->Minv + (size_t)dm * nb * 9;
+            const double* Rd = R + 3 * (size_t)dm * B;
+            double* Zd = Z + 3 * (size_t)dm * B;
+            for (int rb = 0; rb < B; ++rb)
+                for (int cb = rb; cb < B; ++cb) {
+                    const double* H = M + 9 * blk_index(B, rb, cb);
+                    for (int i = 0; i < 3; ++i)
+                        Zd[3 * rb + i] += H[0 * 3 + i] * Rd[3 * cb] + H[1 * 3 + i] * Rd[3 * cb + 1] +
+                                          H[2 * 3 + i] * Rd[3 * cb + 2];
+                    if (cb != rb)
+                        for (int j = 0; j < 3; ++j)
+                            Zd[3 * cb + j] += H[j * 3 + 0] * Rd[3 * rb] + H[j * 3 + 1] * Rd[3 * rb + 1] +
+                                              H[j * 3 + 2] * Rd[3 * rb + 2];
+                }
+        }
+#pragma omp parallel for schedule(static)
+        for (int v = 0; v < n; ++v)
+            for (int d = 0; d < 3; ++d) {
+                double s = 0;
+                for (int l = 0; l < L; ++l) s += Z[3 * (size_t)(m->offsets[l] + m->map[(size_t)l * n + v]) + d];
+                z[3 * (size_t)v + d] = s;
+            }
+        free(R);
+        free(Z);
+    } else {
+        float* R = (float*)calloc((size_t)tot * 3, sizeof(float));
+        float* Z = (float*)calloc((size_t)tot * 3, sizeof(float));
+        for (int l = 0; l < L; ++l) {
+            const int* ml = m->map + (size_t)l * n;
+            for (int v = 0; v < n; ++v)
+                for (int d = 0; d < 3; ++d) R[3 * (size_t)(m->offsets[l] + ml[v]) + d] += (float)r[3 * (size_t)v + d];
+        }
+#pragma omp parallel for schedule(static)
+        for (int dm = 0; dm < ndom; ++dm) {
+            const float* M = m->Minv32 + (size_t)dm * nb * 9;
+            const float* Rd = R + 3 * (size_t)dm * B;
+            float* Zd = Z + 3 * (size_t)dm * B;
+            for (int rb = 0; rb < B; ++rb)
+                for (int cb = rb; cb < B; ++cb) {
+                    const float* H = M + 9 * blk_index(B, rb, cb);
+                    for (int i = 0; i < 3; ++i)
+                        Zd[3 * rb + i] += H[0 * 3 + i] * Rd[3 * cb] + H[1 * 3 + i] * Rd[3 * cb + 1] +
+                                          H[2 * 3 + i] * Rd[3 * cb + 2];
+                    if (cb != rb)
+                        for (int j = 0; j < 3; ++j)
+                            Zd[3 * cb + j] += H[j * 3 + 0] * Rd[3 * rb] + H[j * 3 + 1] * Rd[3 * rb + 1] +
+                                              H[j * 3 + 2] * Rd[3 * rb + 2];
+                }
+        }
+#pragma omp parallel for schedule(static)
+        for (int v = 0; v < n; ++v)
+            for (int d = 0; d < 3; ++d) {
+                float s = 0;
+                for (int l = 0; l < L; ++l) s += Z[3 * (size_t)(m->offsets[l] + m->map[(size_t)l * n + v]) + d];
+                z[3 * (size_t)v + d] = (double)s;
+            }
+        free(R);
+        free(Z);
+    }
+}
+
+int orc_mas_levels(const orc_mas* m, int* level_num, int* offsets, int* counts)
+{
+    *level_num = m->L;
+    for (int l = 0; l <= m->L; ++l) offsets[l] = m->offsets[l];
+    for (int l = 0; l < m->L; ++l) counts[l] = m->counts[l];
+    return m->total;
+}
+
+void orc_mas_tables(const orc_mas* m, int* coarse_table, int* going_next)
+{
+    int n = m->n, L = m->L;
+    if (coarse_table)
+        for (int v = 0; v < n; ++v)
+            for (int l = 1; l < L; ++l)
+                coarse_table[(size_t)v * (L - 1) + (l - 1)] = m->offsets[l] + m->map[(size_t)l * n + v];
+    if (going_next) memcpy(going_next, m->going_next, sizeof(int) * (size_t)m->total);
+}
+
+void orc_mas_matrices(const orc_mas* m, double* out, int inverse)
+{
+    size_t cnt = (size_t)(m->total / m->B) * m->nblk * 9;
+    memcpy(out, inverse ? m->Minv : m->Mgal, sizeof(double) * cnt);
+}
+
+/* ------------------------------------------------------------------------- */
+/* PCGSolver::pcg -- StiffGIPC/linear_system/solver/pcg_solver.cu:191-332     */
+/* ------------------------------------------------------------------------- */
+static double dot(long n, const double* a, const double* b)
+{
+    double s = 0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+    for (long i = 0; i < n; ++i) s += a[i] * b[i];
+    return s;
+}
+
+int orc_pcg(long nnz, const double* vals, const int* rows, const int* cols, int n_rows,
+            orc_mas* mas, const double* b, double* x, double tol, int stop_mode, int max_iter,
+            int mas_fp32, double* stats)
+{
+    long n3 = 3L * n_rows;
+    double* r = (double*)malloc(sizeof(double) * n3);
+    double* z = (double*)malloc(sizeof(double) * n3);
+    double* p = (double*)malloc(sizeof(double) * n3);
+    double* Ap = (double*)malloc(sizeof(double) * n3);
+    for (long i = 0; i < n3; ++i) { x[i] = 0.0; r[i] = b[i]; }
+    double bnorm = sqrt(dot(n3, b, b));
+    int k = 0, iters = 0;
+    double rz = 0, rz0 = 0;
+    if (bnorm > 0) {
+        if (mas) orc_mas_apply(mas, r, z, mas_fp32);
+        else memcpy(z, r, sizeof(double) * n3);
+        rz = dot(n3, r, z);
+        memcpy(p, z, sizeof(double) * n3);
+        rz0 = rz;
+        for (k = 1; k < max_iter; ++k) {
+            orc_spmv(1.0, nnz, vals, rows, cols, p, 0.0, Ap, n_rows);
+            double alpha = rz / dot(n3, p, Ap);
+#pragma omp parallel for schedule(static)
+            for (long i = 0; i < n3; ++i) {
+                x[i] += alpha * p[i];
+                r[i] -= alpha * Ap[i];
+            }
+            iters = k;
+            if (stop_mode == 0) {
+                if (fabs(rz) <= tol * rz0) break;
+            } else {
+                if (sqrt(dot(n3, r, r)) <= tol * bnorm) break;
+            }
+            if (mas) orc_mas_apply(mas, r, z, mas_fp32);
+            else memcpy(z, r, sizeof(double) * n3);
+            double rz_new = dot(n3, r, z);
+            double beta = rz_new / rz;
+#pragma omp parallel for schedule(static)
+            for (long i = 0; i < n3; ++i) p[i] = z[i] + beta * p[i];
+            rz = rz_new;
+        }
+    }
+    if (stats) {
+        orc_spmv(1.0, nnz, vals, rows, cols, x, 0.0, Ap, n_rows);
+        double rr = 0;
+        for (long i = 0; i < n3; ++i) rr += (b[i] - Ap[i]) * (b[i] - Ap[i]);
+        stats[0] = iters;
+        stats[1] = bnorm > 0 ? sqrt(rr) / bnorm : 0.0;
+        stats[2] = rz;
+        stats[3] = rz0;
+    }
+    free(r); free(z); free(p); free(Ap);
+    return k;
+}

@@ -1,0 +1,109 @@
+/*
+ * CPU ORACLE for the Newton linear-solve path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * This is a plain-C restatement of the reference's algorithms, used by tests/,
+ * __graft_entry__.smoke() and the cpu_baseline leg of bench.py as the checker
+ * (and as the CPU baseline "port").  The product library never links, loads or
+ * calls it.
+ *
+ * Conventions shared with the product ABI:
+ *   - a 3x3 block is 9 doubles, column-major (Eigen::Matrix3d): v[c*3 + r] = M(r, c);
+ *   - vectors are contiguous doubles of length 3*rows (double3 aliasing);
+ *   - the assembled matrix is upper-triangular symmetric BCOO (row <= col),
+ *     sorted by (row, col), unique.
+ *
+ * Pinning (what each function is checked against, see tests/test_oracle_*.py):
+ *   orc_tet_snh       energy closed form -> finite-difference gradient; analytic
+ *                     unprojected Hessian -> FD of the gradient; projected
+ *                     Hessian == numpy eigh-clamp; rest-state eigenvalues.
+ *   orc_barrier_pt    distance closed form -> FD gradient; lambda0 == d2E/ds2 (FD);
+ *                     rank-1 and PSD.
+ *   orc_convert       numpy lexsort + add.reduceat.
+ *   orc_spmv          scipy.sparse product of A_upper + A_upper^T - diag; dense.
+ *   orc_mas_*         independent set-based restatement in numpy/scipy (per-domain
+ *                     connected components, Galerkin P^T A P, numpy.linalg.inv,
+ *                     dense apply); M^-1 symmetric positive definite on small meshes.
+ *   orc_pcg           scipy.sparse.linalg.cg with the same preconditioner (iterates),
+ *                     true residual.
+ */
+#ifndef SGIPC_ORACLE_H
+#define SGIPC_ORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- element Hessians ------------------------------------------------------ */
+/* Stable Neo-Hookean (variant 1) tet: gradient (atomically accumulated into
+ * grad when non-NULL) and PSD-projected Hessian written as 10 upper blocks per
+ * tet at tvals/trows/tcols[10*t ...].  StiffGIPC/femEnergy.cu:1719-1857. */
+int orc_tet_snh(int n_tets, const double* x, const int* tets, const double* dm_inv,
+                const double* vol, const double* mu, const double* lam, double dt,
+                int hess_offset, double* grad, double* tvals, int* trows, int* tcols);
+
+/* Stable Neo-Hookean energy of one tet (for finite-difference checks). */
+double orc_tet_snh_energy(const double* x12, const double* dm_inv, double mu, double lam);
+/* Unprojected 9x9 dPsi/dF^2 (column-major vec(F)) and the projected one. */
+void orc_snh_dpdf(const double* F9, double mu, double lam, double* H81, double* Hproj81);
+
+/* Point-triangle barrier (GIPC rank-2 variant), 16 blocks per pair at
+ * tvals[16*p ...]; blocks below the diagonal are moved to (col,row) and zeroed,
+ * as the reference's contact partition does.  StiffGIPC/GIPC.cu:4733-4984,
+ * 10329-10363.  pair_x: 12 doubles of local geometry per pair, or NULL to read
+ * x at the pair's vertex ids.  dhat_sq is the reference's dHat (a squared
+ * distance). */
+int orc_barrier_pt(int n_pairs, const double* x, const int* pairs, const double* pair_x,
+                   double dhat_sq, double kappa, int hess_offset, double* grad,
+                   double* tvals, int* trows, int* tcols);
+double orc_barrier_energy(double I5, double dhat_sq, double kappa);
+double orc_barrier_lambda0(double I5, double dhat_sq, double kappa);
+
+/* mass[i] * I at (i+off, i+off).  StiffGIPC/GIPC.cu:10466-10485. */
+int orc_mass_diag(int n, const double* mass, int hess_offset, double* tvals, int* trows,
+                  int* tcols);
+
+/* ---- triplets -> unique upper BCOO (Converter::srbk_convert) ---------------- */
+long orc_convert(long n_trip, const double* tvals, const int* trows, const int* tcols,
+                 double* ovals, int* orows, int* ocols);
+
+/* ---- symmetric upper-BCOO SpMV: y = a*A*x + b*y (Spmv::srbk_spmv) --------- */
+void orc_spmv(double a, long nnz, const double* vals, const int* rows, const int* cols,
+              const double* x, double b, double* y, int n_rows);
+
+/* ---- MAS preconditioner (StiffGIPC/MASPreconditioner.cu) --------------------- */
+typedef struct orc_mas orc_mas;
+orc_mas* orc_mas_create(int n, int bank, const unsigned* nbr_start, const unsigned* nbr_num,
+                        const unsigned* nbr_list, int collision_node_offset);
+void orc_mas_destroy(orc_mas* m);
+int orc_mas_build(orc_mas* m, long nnz, const double* vals, const int* rows, const int* cols,
+                  int offset, const int* pairs, int n_pairs);
+void orc_mas_apply(orc_mas* m, const double* r, double* z, int fp32);
+/* level_num; offsets[level_num+1]; counts[level_num]. */
+int orc_mas_levels(const orc_mas* m, int* level_num, int* offsets, int* counts);
+/* coarse_table[n*(L-1)] (global slot ids, vertex-major), going_next[total slots]. */
+void orc_mas_tables(const orc_mas* m, int* coarse_table, int* going_next);
+/* Domain matrices, B(B+1)/2 blocks of 9 doubles per domain (inverse=0: Galerkin
+ * blocks, inverse=1: inverse blocks). */
+void orc_mas_matrices(const orc_mas* m, double* out, int inverse);
+int orc_mas_num_levels_for(int n, int bank);
+
+/* ---- PCG (PCGSolver::pcg) ------------------------------------------------- */
+/* stop_mode 0: reference |r.z| <= tol * r0.z0 checked after the x/r update
+ *              with the previous rz (pcg_solver.cu:279-280);
+ * stop_mode 1: recursive ||r||/||b|| <= tol.
+ * mas == NULL -> identity preconditioner.  stats[0]=iterations performed
+ * (x/r updates; the reference returns max_iter when the cap is hit),
+ * stats[1]=true relative residual, stats[2]=final rz, stats[3]=rz0. */
+int orc_pcg(long nnz, const double* vals, const int* rows, const int* cols, int n_rows,
+            orc_mas* mas, const double* b, double* x, double tol, int stop_mode, int max_iter,
+            int mas_fp32, double* stats);
+
+/* Deliberate-mistake switch for the tests that show the checks are sensitive:
+ * 0 = none, 1 = SpMV drops the transposed (lower) contribution, 2 = Galerkin
+ * coarse levels drop H^T on merged diagonals, 3 = barrier Hessian uses lambda0
+ * with the wrong sign of the log term. */
+void orc_set_fault(int fault);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,96 @@
+"""Build script for the gfx950 library (and the CPU oracle used by the tests).
+
+``python stiff-gipc_amd/build_lib.py`` compiles every ``csrc/*.hip`` /
+``csrc/*.cpp`` with hipcc for gfx950 into ``stiff-gipc_amd/lib/libsgipc.so``
+and runs ``make`` in ``oracle/``.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
+LIB_DIR = os.path.join(HERE, "lib")
+LIB = os.path.join(LIB_DIR, "libsgipc.so")
+ARCH = os.environ.get("SGIPC_OFFLOAD_ARCH", "gfx950")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+COMMON = [
+    "-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
+    # packed FP32 instructions are kept out of the generated code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-munsafe-fp-atomics", "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+    "-I", os.path.join(ROOT, "include"),
+]
+
+
+def _sources():
+    out = []
+    for f in sorted(os.listdir(CSRC)):
+        if f.endswith(".hip") or f.endswith(".cpp"):
+            out.append(os.path.join(CSRC, f))
+    return out
+
+
+def _compile(src, verbose):
+    obj = os.path.join(OBJ, os.path.basename(src) + ".o")
+    if os.path.exists(obj):
+        deps = [src] + [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith((".hpp", ".h"))]
+        deps.append(os.path.join(ROOT, "include", "sgipc.h"))
+        if all(os.path.getmtime(obj) >= os.path.getmtime(d) for d in deps):
+            return obj
+    cmd = [HIPCC, "-x", "hip", *COMMON, "-c", src, "-o", obj]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}\n{r.stderr}")
+    if r.stderr.strip() and verbose:
+        # hipcc's host pass warns about the GPU-only target feature
+        lines = [l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l and l.strip()]
+        if lines:
+            print("\n".join(lines[:40]), flush=True)
+    return obj
+
+
+def build_hip(verbose=False, jobs=8):
+    os.makedirs(OBJ, exist_ok=True)
+    os.makedirs(LIB_DIR, exist_ok=True)
+    srcs = _sources()
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(lambda s: _compile(s, verbose), srcs))
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle(verbose=False):
+    odir = os.path.join(ROOT, "oracle")
+    r = subprocess.run(["make", "-C", odir], capture_output=True, text=True)
+    if verbose and r.stdout.strip():
+        print(r.stdout.strip(), flush=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"oracle build failed:\n{r.stdout}\n{r.stderr}")
+
+
+def build_all(verbose=False):
+    build_oracle(verbose)
+    lib = build_hip(verbose)
+    if verbose:
+        print(f"built {lib}", flush=True)
+
+
+if __name__ == "__main__":
+    build_all(verbose="-q" not in sys.argv)

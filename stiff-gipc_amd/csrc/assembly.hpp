@@ -1,0 +1,33 @@
+// Hessian assembly launchers and the triplet -> BCOO converter.
+#pragma once
+#include "sgipc_internal.hpp"
+
+namespace sgipc {
+
+void launch_tet_snh(hipStream_t s, int n_tets, const double* x, const uint32_t* tets, const double* dm_inv,
+                    const double* vol, const double* mu, const double* lam, double dt, int hess_offset,
+                    double* grad, double* tvals, int* trows, int* tcols);
+void launch_barrier_pt(hipStream_t s, int n_pairs, const double* x, const int* pairs, const double* pair_x,
+                       double dhat_sq, double kappa, int hess_offset, double* grad, double* tvals, int* trows,
+                       int* tcols);
+void launch_mass_diag(hipStream_t s, int n, const double* mass, int hess_offset, double* tvals, int* trows,
+                      int* tcols);
+
+// Sort + reduce-by-key of 3x3 triplets into unique upper BCOO.  Buffers are
+// kept between calls.
+class Converter {
+  public:
+    // Returns the unique block count; throws SGIPC_CAPACITY / SGIPC_BAD_INDEX.
+    // index_bound: every row/col is < index_bound.
+    int convert(hipStream_t s, const double* tvals, const int* trows, const int* tcols, long n, int index_bound,
+                double* ovals, int* orows, int* ocols, int capacity);
+    int max_index(hipStream_t s, const int* rows, const int* cols, long n);
+    static int index_bits(int bound);
+
+  private:
+    DevBuf<uint64_t> keys_, skeys_;
+    DevBuf<int> idx_, sidx_, flag_, pos_, head_, scratch_int_;
+    DevBuf<unsigned char> temp_;
+};
+
+}  // namespace sgipc

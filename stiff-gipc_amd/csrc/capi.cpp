@@ -1,0 +1,455 @@
+// C ABI over the gfx950 kernels (include/sgipc.h): context, device memory
+// helpers, SpMV, MAS and PCG (FP64, reference FP32-MAS, and FP32 PCG inside
+// FP64 iterative refinement).  Every entry point catches sgipc::Error /
+// std::exception and turns it into a status code; nothing aborts the process.
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "capi_internal.hpp"
+
+namespace sgipc {
+
+thread_local std::string g_last_error;
+
+// FP64 / FP32 SpMV operators over caller-owned BCOO arrays.
+struct BcooOpF64 : SpmvOp<double> {
+    const double* vals;
+    const int* rows;
+    const int* cols;
+    int apply_dot(hipStream_t s, const double* p, double* Ap, double* part, const PcgCtl* ctl) const override
+    {
+        int np = 0;
+        launch_spmv_pcg_f64(s, vals, rows, cols, nnz, p, Ap, part, ctl, &np);
+        return np;
+    }
+};
+struct BcooOpF32 : SpmvOp<float> {
+    const float* vals;
+    const int* rows;
+    const int* cols;
+    int apply_dot(hipStream_t s, const float* p, float* Ap, double* part, const PcgCtl* ctl) const override
+    {
+        int np = 0;
+        launch_spmv_pcg_f32(s, vals, rows, cols, nnz, p, Ap, part, ctl, &np);
+        return np;
+    }
+};
+
+}  // namespace sgipc
+
+namespace {
+std::unique_ptr<sgipc::SolveWorkspace>& workspace_of(sgipc_ctx* ctx);
+}
+
+using namespace sgipc;
+
+extern "C" {
+
+int sgipc_abi_version(void) { return SGIPC_ABI_VERSION; }
+
+int sgipc_create(sgipc_ctx** out, int device)
+{
+    if (!out) return SGIPC_BAD_ARG;
+    *out = nullptr;
+    auto* ctx = new (std::nothrow) sgipc_ctx();
+    if (!ctx) return SGIPC_HIP_ERROR;
+    int st = guarded(nullptr, [&] {
+        int count = 0;
+        SGIPC_HIP(hipGetDeviceCount(&count));
+        SGIPC_CHECK(device >= 0 && device < count, SGIPC_BAD_ARG, "no such HIP device");
+        ctx->c.device = device;
+        SGIPC_HIP(hipSetDevice(device));
+        SGIPC_HIP(hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking));
+        ctx->c.own_stream = true;
+        hipDeviceProp_t prop;
+        SGIPC_HIP(hipGetDeviceProperties(&prop, device));
+        ctx->c.num_cus = prop.multiProcessorCount;
+    });
+    if (st != SGIPC_OK) {
+        delete ctx;
+        return st;
+    }
+    *out = ctx;
+    return SGIPC_OK;
+}
+
+int sgipc_destroy(sgipc_ctx* ctx)
+{
+    if (!ctx) return SGIPC_BAD_ARG;
+    int st = guarded(ctx, [&] {
+        workspace_of(ctx).reset();
+        if (ctx->c.own_stream && ctx->c.stream) SGIPC_HIP(hipStreamDestroy(ctx->c.stream));
+    });
+    delete ctx;
+    return st;
+}
+
+int sgipc_set_stream(sgipc_ctx* ctx, void* stream)
+{
+    if (!ctx) return SGIPC_BAD_ARG;
+    return guarded(ctx, [&] {
+        if (ctx->c.own_stream && ctx->c.stream) SGIPC_HIP(hipStreamDestroy(ctx->c.stream));
+        ctx->c.stream = static_cast<hipStream_t>(stream);
+        ctx->c.own_stream = false;
+    });
+}
+
+int sgipc_synchronize(sgipc_ctx* ctx)
+{
+    if (!ctx) return SGIPC_BAD_ARG;
+    return guarded(ctx, [&] { SGIPC_HIP(hipStreamSynchronize(ctx->c.stream)); });
+}
+
+const char* sgipc_last_error(const sgipc_ctx* ctx)
+{
+    if (ctx) return ctx->c.last_error.c_str();
+    return g_last_error.c_str();
+}
+
+int sgipc_malloc(sgipc_ctx* ctx, void** ptr, size_t bytes)
+{
+    if (!ctx || !ptr) return SGIPC_BAD_ARG;
+    return guarded(ctx, [&] { SGIPC_HIP(hipMalloc(ptr, bytes ? bytes : 1)); });
+}
+
+int sgipc_free(sgipc_ctx* ctx, void* ptr)
+{
+    if (!ctx) return SGIPC_BAD_ARG;
+    return guarded(ctx, [&] {
+        if (ptr) SGIPC_HIP(hipFree(ptr));
+    });
+}
+
+int sgipc_memcpy_htod(sgipc_ctx* ctx, void* dst, const void* src, size_t bytes)
+{
+    if (!ctx || (bytes && (!dst || !src))) return SGIPC_BAD_ARG;
+    return guarded(ctx, [&] {
+        if (bytes) SGIPC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->c.stream));
+        SGIPC_HIP(hipStreamSynchronize(ctx->c.stream));
+    });
+}
+
+int sgipc_memcpy_dtoh(sgipc_ctx* ctx, void* dst, const void* src, size_t bytes)
+{
+    if (!ctx || (bytes && (!dst || !src))) return SGIPC_BAD_ARG;
+    return guarded(ctx, [&] {
+        if (bytes) SGIPC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->c.stream));
+        SGIPC_HIP(hipStreamSynchronize(ctx->c.stream));
+    });
+}
+
+int sgipc_memcpy_dtod(sgipc_ctx* ctx, void* dst, const void* src, size_t bytes)
+{
+    if (!ctx || (bytes && (!dst || !src))) return SGIPC_BAD_ARG;
+    return guarded(ctx, [&] {
+        if (bytes) SGIPC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->c.stream));
+        SGIPC_HIP(hipStreamSynchronize(ctx->c.stream));
+    });
+}
+
+int sgipc_memset(sgipc_ctx* ctx, void* dst, int value, size_t bytes)
+{
+    if (!ctx || (bytes && !dst)) return SGIPC_BAD_ARG;
+    return guarded(ctx, [&] {
+        if (bytes) SGIPC_HIP(hipMemsetAsync(dst, value, bytes, ctx->c.stream));
+        SGIPC_HIP(hipStreamSynchronize(ctx->c.stream));
+    });
+}
+
+// ---------------------------------------------------------------- SpMV
+int sgipc_spmv_srbk(sgipc_ctx* ctx, double a, const double* vals, const int* rows, const int* cols,
+                    int nnz, const double* x, double b, double* y, int n_rows)
+{
+    if (!ctx) return SGIPC_BAD_ARG;
+    if (n_rows < 0 || nnz < 0) return SGIPC_BAD_ARG;
+    if (n_rows == 0) return SGIPC_EMPTY;
+    if ((nnz > 0 && (!vals || !rows || !cols || !x)) || !y) return SGIPC_BAD_ARG;
+    return guarded(ctx, [&] {
+        launch_spmv_f64(ctx->c.stream, a, vals, rows, cols, nnz, x, b, y, n_rows, nullptr, nullptr);
+        SGIPC_HIP(hipStreamSynchronize(ctx->c.stream));
+    });
+}
+
+int sgipc_spmv_srbk_f32(sgipc_ctx* ctx, float a, const float* vals, const int* rows, const int* cols,
+                        int nnz, const float* x, float b, float* y, int n_rows)
+{
+    if (!ctx) return SGIPC_BAD_ARG;
+    if (n_rows < 0 || nnz < 0) return SGIPC_BAD_ARG;
+    if (n_rows == 0) return SGIPC_EMPTY;
+    if ((nnz > 0 && (!vals || !rows || !cols || !x)) || !y) return SGIPC_BAD_ARG;
+    return guarded(ctx, [&] {
+        launch_spmv_f32(ctx->c.stream, a, vals, rows, cols, nnz, x, b, y, n_rows);
+        SGIPC_HIP(hipStreamSynchronize(ctx->c.stream));
+    });
+}
+
+int sgipc_convert_f64_to_f32(sgipc_ctx* ctx, const double* in, float* out, size_t n)
+{
+    if (!ctx || (n && (!in || !out))) return SGIPC_BAD_ARG;
+    return guarded(ctx, [&] {
+        launch_f64_to_f32(ctx->c.stream, in, out, n);
+        SGIPC_HIP(hipStreamSynchronize(ctx->c.stream));
+    });
+}
+
+// ---------------------------------------------------------------- MAS
+int sgipc_mas_create(sgipc_ctx* ctx, sgipc_mas** out, int bank_size)
+{
+    if (!ctx || !out) return SGIPC_BAD_ARG;
+    *out = nullptr;
+    if (bank_size != 16 && bank_size != 32) return SGIPC_BAD_ARG;
+    auto* m = new (std::nothrow) sgipc_mas();
+    if (!m) return SGIPC_HIP_ERROR;
+    m->ctx = ctx;
+    int st = guarded(ctx, [&] { m->m = std::make_unique<Mas>(&ctx->c, bank_size); });
+    if (st != SGIPC_OK) {
+        delete m;
+        return st;
+    }
+    *out = m;
+    return SGIPC_OK;
+}
+
+int sgipc_mas_destroy(sgipc_mas* mas)
+{
+    if (!mas) return SGIPC_BAD_ARG;
+    int st = guarded(mas->ctx, [&] { mas->m.reset(); });
+    delete mas;
+    return st;
+}
+
+int sgipc_mas_init_neighbor(sgipc_mas* mas, int vert_num, int collision_node_offset,
+                            int total_neighbor_num, const uint32_t* nbr_start, const uint32_t* nbr_num,
+                            const uint32_t* nbr_list)
+{
+    if (!mas) return SGIPC_BAD_ARG;
+    if (vert_num < 1) return SGIPC_EMPTY;
+    if (total_neighbor_num < 0) return SGIPC_BAD_ARG;
+    return guarded(mas->ctx, [&] {
+        mas->m->init_neighbor(vert_num, collision_node_offset, total_neighbor_num, nbr_start, nbr_num, nbr_list);
+    });
+}
+
+int sgipc_mas_init_matrix(sgipc_mas* mas)
+{
+    if (!mas) return SGIPC_BAD_ARG;
+    return guarded(mas->ctx, [&] { mas->m->init_matrix(); });
+}
+
+int sgipc_mas_set_preconditioner_bcoo(sgipc_mas* mas, const double* vals, const int* rows, const int* cols,
+                                      int nnz, int offset, const int* pairs, int n_pairs, int precision)
+{
+    if (!mas) return SGIPC_BAD_ARG;
+    if (nnz < 0 || n_pairs < 0) return SGIPC_BAD_ARG;
+    if (nnz > 0 && (!vals || !rows || !cols)) return SGIPC_BAD_ARG;
+    return guarded(mas->ctx, [&] {
+        mas->m->set_bcoo(vals, rows, cols, nnz, offset, pairs, n_pairs, precision);
+    });
+}
+
+int sgipc_mas_preconditioning(sgipc_mas* mas, const double* r, double* z)
+{
+    if (!mas || !r || !z) return SGIPC_BAD_ARG;
+    return guarded(mas->ctx, [&] {
+        hipStream_t s = mas->ctx->c.stream;
+        PcgStopDesc none{};
+        mas->m->apply<double>(s, r, z, nullptr, none);
+        SGIPC_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int sgipc_mas_levels(sgipc_mas* mas, int* level_num, int* offsets, int* counts)
+{
+    if (!mas || !level_num || !offsets || !counts) return SGIPC_BAD_ARG;
+    return guarded(mas->ctx, [&] { *level_num = mas->m->levels(offsets, counts); });
+}
+
+int sgipc_mas_tables(sgipc_mas* mas, int* coarse_table, int* going_next)
+{
+    if (!mas) return SGIPC_BAD_ARG;
+    return guarded(mas->ctx, [&] { mas->m->tables(coarse_table, going_next); });
+}
+
+int sgipc_mas_inverse_blocks(sgipc_mas* mas, double* out)
+{
+    if (!mas || !out) return SGIPC_BAD_ARG;
+    return guarded(mas->ctx, [&] {
+        SGIPC_CHECK(mas->m->built, SGIPC_BAD_ARG, "MAS not set");
+        mas->m->inverse_blocks(out);
+    });
+}
+
+// ---------------------------------------------------------------- PCG
+void sgipc_pcg_default_opts(sgipc_pcg_opts* o)
+{
+    if (!o) return;
+    o->tol = 1e-5;
+    o->stop_mode = SGIPC_STOP_TRUE_RESIDUAL;
+    o->max_iter = 0;
+    o->precision = SGIPC_FP64;
+    o->check_every = 8;
+    o->ir_max_outer = 20;
+    o->ir_inner_tol = 1e-3;
+}
+
+}  // extern "C"
+
+namespace sgipc {
+
+// Shared by sgipc_pcg_solve and the linear-system object.
+int pcg_solve_impl(Context& c, SolveWorkspace& ws, const double* vals, const int* rows, const int* cols,
+                   int nnz, int n_rows, Mas* mas, const double* b, double* x, const sgipc_pcg_opts& o,
+                   sgipc_pcg_stats* stats)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    hipStream_t s = c.stream;
+    const long n3 = 3L * n_rows;
+    int max_iter = o.max_iter > 0 ? o.max_iter : (int)(0.3 * (double)n3);
+    if (max_iter < 2) max_iter = 2;
+    sgipc_pcg_stats st{};
+    int status = SGIPC_OK;
+    ws.part.reserve(4096);
+    if (o.precision != SGIPC_FP32_IR) {
+        BcooOpF64 A;
+        A.nnz = nnz;
+        A.vals = vals;
+        A.rows = rows;
+        A.cols = cols;
+        MasPrecond<double> M(mas);
+        PcgOptions po;
+        po.tol = o.tol;
+        po.stop_mode = o.stop_mode;
+        po.max_iter = max_iter;
+        po.check_every = o.check_every;
+        PcgResult r = ws.pcg64.solve(s, A, mas ? &M : nullptr, b, x, n3, po);
+        st.iterations = r.iterations;
+        st.outer_iterations = 1;
+        st.rz = r.rz;
+        st.rz0 = r.rz0;
+        if (r.not_spd) status = SGIPC_NOT_SPD;
+        else if (!r.converged) status = SGIPC_MAXITER;
+    } else {
+        // FP64 iterative refinement around an FP32 PCG (FP32 SpMV + FP32 MAS,
+        // FP64 dot products / alpha / beta inside).
+        ws.vals32.reserve((size_t)nnz * 9);
+        ws.r32.reserve(n3);
+        ws.d32.reserve(n3);
+        ws.r64.reserve(n3);
+        ws.d64.reserve(n3);
+        launch_convert<double, float>(s, vals, ws.vals32.get(), (long)nnz * 9);
+        BcooOpF32 A;
+        A.nnz = nnz;
+        A.vals = ws.vals32.get();
+        A.rows = rows;
+        A.cols = cols;
+        MasPrecond<float> M(mas);
+        launch_fill<double>(s, x, 0.0, n3);
+        SGIPC_HIP(hipMemcpyAsync(ws.r64.get(), b, sizeof(double) * n3, hipMemcpyDeviceToDevice, s));
+        double* hp = nullptr;
+        SGIPC_HIP(hipHostMalloc(reinterpret_cast<void**>(&hp), 2 * 4096 * sizeof(double)));
+        auto host_sum = [&](const double* dpart, int np) {
+            SGIPC_HIP(hipMemcpyAsync(hp, dpart, sizeof(double) * np, hipMemcpyDeviceToHost, s));
+            SGIPC_HIP(hipStreamSynchronize(s));
+            double acc = 0.0;
+            for (int i = 0; i < np; ++i) acc += hp[i];
+            return acc;
+        };
+        int np = launch_dot_partials<double, double>(s, b, b, n3, ws.part.get());
+        const double bb = host_sum(ws.part.get(), np);
+        double rr = bb;
+        int total_it = 0, outer = 0;
+        bool ok = bb == 0.0;
+        int max_outer = o.ir_max_outer > 0 ? o.ir_max_outer : 20;
+        double inner_tol = o.ir_inner_tol > 0 ? o.ir_inner_tol : 1e-3;
+        double best_rr = rr;
+        while (!ok && outer < max_outer && total_it < max_iter) {
+            ++outer;
+            launch_convert<double, float>(s, ws.r64.get(), ws.r32.get(), n3);
+            PcgOptions po;
+            po.tol = inner_tol;
+            po.stop_mode = SGIPC_STOP_TRUE_RESIDUAL;
+            po.max_iter = max_iter - total_it;
+            po.check_every = o.check_every;
+            PcgResult r = ws.pcg32.solve(s, A, mas ? &M : nullptr, ws.r32.get(), ws.d32.get(), n3, po);
+            total_it += r.iterations;
+            if (r.not_spd) {
+                status = SGIPC_NOT_SPD;
+                break;
+            }
+            launch_convert<float, double>(s, ws.d32.get(), ws.d64.get(), n3);
+            // x += d ; r = b - A x (FP64)
+            launch_axpy_f64(s, 1.0, ws.d64.get(), x, n3);
+            int nrr = 0;
+            launch_residual(s, vals, rows, cols, nnz, n_rows, x, b, ws.r64.get(), ws.part.get(), &nrr);
+            rr = host_sum(ws.part.get(), nrr);
+            if (rr <= o.tol * o.tol * bb) ok = true;
+            if (rr > best_rr * 0.999999 && outer > 1) break;  // stagnation guard
+            if (rr < best_rr) best_rr = rr;
+        }
+        (void)hipHostFree(hp);
+        st.iterations = total_it;
+        st.outer_iterations = outer;
+        if (status == SGIPC_OK && !ok) status = SGIPC_MAXITER;
+    }
+    // true FP64 residual of the returned x
+    {
+        int nrr = 0;
+        ws.r64.reserve(n3);
+        launch_residual(s, vals, rows, cols, nnz, n_rows, x, b, ws.r64.get(), ws.part.get(), &nrr);
+        std::vector<double> h(nrr);
+        SGIPC_HIP(hipMemcpyAsync(h.data(), ws.part.get(), sizeof(double) * nrr, hipMemcpyDeviceToHost, s));
+        int nbb = launch_dot_partials<double, double>(s, b, b, n3, ws.part.get() + 2048);
+        std::vector<double> hb(nbb);
+        SGIPC_HIP(hipMemcpyAsync(hb.data(), ws.part.get() + 2048, sizeof(double) * nbb, hipMemcpyDeviceToHost, s));
+        SGIPC_HIP(hipStreamSynchronize(s));
+        double rr = 0, bb = 0;
+        for (double v : h) rr += v;
+        for (double v : hb) bb += v;
+        st.rel_residual = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+        if (o.stop_mode == SGIPC_STOP_TRUE_RESIDUAL && status == SGIPC_MAXITER && st.rel_residual <= o.tol)
+            status = SGIPC_OK;
+    }
+    st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    st.status = status;
+    if (stats) *stats = st;
+    return status;
+}
+
+}  // namespace sgipc
+
+namespace {
+std::unique_ptr<sgipc::SolveWorkspace>& workspace_of(sgipc_ctx* ctx)
+{
+    // one workspace per context, created on first use
+    static thread_local std::vector<std::pair<sgipc_ctx*, std::unique_ptr<sgipc::SolveWorkspace>>> table;
+    for (auto& e : table)
+        if (e.first == ctx) return e.second;
+    table.emplace_back(ctx, nullptr);
+    return table.back().second;
+}
+}  // namespace
+
+extern "C" int sgipc_pcg_solve(sgipc_ctx* ctx, const double* vals, const int* rows, const int* cols, int nnz,
+                               int n_rows, sgipc_mas* mas, const double* b, double* x,
+                               const sgipc_pcg_opts* opts, sgipc_pcg_stats* stats)
+{
+    if (!ctx || !b || !x) return SGIPC_BAD_ARG;
+    if (n_rows <= 0 || nnz <= 0) return SGIPC_EMPTY;
+    if (!vals || !rows || !cols) return SGIPC_BAD_ARG;
+    sgipc_pcg_opts o;
+    sgipc_pcg_default_opts(&o);
+    if (opts) o = *opts;
+    if (o.precision < SGIPC_FP64 || o.precision > SGIPC_FP32_IR) return SGIPC_BAD_ARG;
+    if (o.stop_mode != 0 && o.stop_mode != 1) return SGIPC_BAD_ARG;
+    if (mas && mas->m->n != n_rows) return SGIPC_BAD_ARG;
+    int status = SGIPC_OK;
+    int st = guarded(ctx, [&] {
+        auto& ws = workspace_of(ctx);
+        if (!ws) ws = std::make_unique<SolveWorkspace>();
+        status = pcg_solve_impl(ctx->c, *ws, vals, rows, cols, nnz, n_rows, mas ? mas->m.get() : nullptr, b, x, o,
+                                stats);
+    });
+    return st != SGIPC_OK ? st : status;
+}

@@ -1,0 +1,1087 @@
+// Multilevel additive Schwarz (MAS) preconditioner for gfx950.
+//
+// Semantics follow StiffGIPC/MASPreconditioner.cu (GpuMas backend, identity
+// slot map):
+//  * hierarchy (ReorderRealtime :1785-1826, kernels :29-625, contact
+//    connectivity :1132-1548): at level l the nodes are cut into domains of B
+//    consecutive ids; mesh edges and contact cliques mapped to level l that
+//    join two nodes of one domain are OR-ed into per-node bitmasks; each
+//    domain's connected components become the level-(l+1) nodes, numbered
+//    domain by domain and, inside a domain, by their lowest member;
+//    off_1 = ceil(n/B)*B and off_{l+1} = off_l + ceil(c_l/B)*B;
+//  * Galerkin domain matrices (PrepareHessian_bcoo :1829-2106): P_l^T A P_l
+//    restricted to node pairs of one level-l domain, upper 3x3 blocks;
+//  * FP64 Gauss-Jordan inverse without pivoting, zero diagonal -> 1 (:630-726);
+//  * apply (preconditioning :2248-2289): restrict, per-domain dense mat-vec,
+//    prolong.
+//
+// MI355X mapping:
+//  * level-0 domain matrices are never materialised: the inverse kernel pulls
+//    each domain's blocks straight from the sorted BCOO into register tiles,
+//    reduces the domain's contribution to the coarse levels in-block (one
+//    3x3 per level in the common single-component case) and inverts in FP64;
+//  * the inverse keeps a 6x6 tile of the 3Bx3B matrix per thread (256
+//    threads for B=32) and broadcasts the pivot row/column through LDS, one
+//    barrier per pivot (double-buffered);
+//  * restriction is a deterministic wavefront group-by (no atomics): every
+//    coarse node lives inside one finer domain, so its sum is formed by one
+//    wave; the local solve stages a domain's FP32/FP64 inverse blocks in LDS
+//    with coalesced 16-byte loads and computes each row in a fixed order;
+//    prolongation fuses the r.z partial for PCG.
+#include "mas.hpp"
+#include "wave_ops.hpp"
+
+#include <chrono>
+#include <cstring>
+
+namespace sgipc {
+
+namespace {
+
+__device__ __host__ __forceinline__ int blk_index(int B, int r, int c) { return B * r - r * (r + 1) / 2 + c; }
+
+// ------------------------------------------------------------------ hierarchy
+template <int B>
+__global__ void k_mask_fine(int level, int n, const int* __restrict__ map_l,
+                            const uint32_t* __restrict__ ns, const uint32_t* __restrict__ nn,
+                            const uint32_t* __restrict__ nl, uint32_t* __restrict__ mask)
+{
+    int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    int a = map_l ? map_l[u] : u;
+    uint32_t m = 0;
+    const uint32_t st = ns[u], cnt = nn[u];
+    for (uint32_t k = 0; k < cnt; ++k) {
+        int v = (int)nl[st + k];
+        int b = map_l ? map_l[v] : v;
+        if (a != b && a / B == b / B) m |= 1u << (b % B);
+    }
+    if (level == 0)
+        mask[u] = m;
+    else if (m)
+        atomicOr(mask + a, m);
+}
+
+__device__ __forceinline__ void decode_pair(int4 pr, int coff, int n, int v[4])
+{
+    int x = pr.x, y = pr.y, z = pr.z, w = pr.w;
+    int cnt = 4;
+    if (x >= 0) {
+        if (w < 0) w = -w - 1;
+    } else {
+        x = -x - 1;
+        if (z < 0) {
+            if (y < 0) {
+                y = -y - 1; z = -z - 1; w = -w - 1;
+            } else {
+                cnt = 2;
+            }
+        } else if (w < 0) {
+            if (y < 0) {
+                y = -y - 1; w = -w - 1;
+            } else {
+                cnt = 3;
+            }
+        }
+    }
+    int t[4] = {x, y, z, w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        int q = t[i] - coff;
+        v[i] = (i < cnt && q >= 0 && q < n) ? q : -1;
+    }
+}
+
+template <int B>
+__global__ void k_mask_contact(int n, int coff, const int4* __restrict__ pairs, int np,
+                               const int* __restrict__ map_l, uint32_t* __restrict__ mask)
+{
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    int v[4];
+    decode_pair(pairs[p], coff, n, v);
+    int cv[4];
+    uint32_t m[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) cv[i] = v[i] >= 0 ? (map_l ? map_l[v[i]] : v[i]) : -1;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = i + 1; j < 4; ++j)
+            if (cv[i] >= 0 && cv[j] >= 0 && cv[i] != cv[j] && cv[i] / B == cv[j] / B) {
+                m[i] |= 1u << (cv[j] % B);
+                m[j] |= 1u << (cv[i] % B);
+            }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        if (cv[i] >= 0 && m[i]) atomicOr(mask + cv[i], m[i]);
+}
+
+// Connected components inside each domain (closure of the bitmasks) and the
+// rank of every node's component among the domain's components.
+template <int B>
+__global__ __launch_bounds__(256) void k_closure(const int* __restrict__ meta, int level,
+                                                 const uint32_t* __restrict__ mask,
+                                                 int* __restrict__ comp_rank, int* __restrict__ dom_count)
+{
+    __shared__ uint32_t sm[256];
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = meta[kMetaCnt + level];
+    const int lane = a % B;
+    const bool real = a < c;
+    uint32_t conn = real ? (mask[a] | (1u << lane)) : 0u;
+    sm[threadIdx.x] = conn;
+    __syncthreads();
+    const int base = (threadIdx.x / B) * B;
+    uint32_t visited = real ? (1u << lane) : 0u;
+    while (true) {
+        uint32_t todo = conn & ~visited;
+        if (!todo) break;
+        int nb = __ffs(todo) - 1;
+        visited |= 1u << nb;
+        conn |= sm[base + nb];
+    }
+    const int leader = real ? __ffs(conn) - 1 : -1;
+    const bool is_leader = real && leader == lane;
+    const uint64_t bal = __ballot(is_leader);
+    const int shift = ((threadIdx.x & 63) / B) * B;
+    const uint32_t dom_leaders =
+        (uint32_t)((bal >> shift) & (B == 32 ? 0xffffffffull : 0xffffull));
+    if (real) comp_rank[a] = __popc(dom_leaders & ((1u << leader) - 1u));
+    const int ndom = (c + B - 1) / B;
+    if (lane == 0 && a / B < ndom) dom_count[a / B] = __popc(dom_leaders);
+}
+
+// Exclusive scan of the per-domain component counts (single workgroup).
+__global__ __launch_bounds__(1024) void k_scan_domains(int* __restrict__ meta, int level, int B,
+                                                       const int* __restrict__ dom_count,
+                                                       int* __restrict__ dom_prefix)
+{
+    __shared__ int ws[16];
+    const int c = meta[kMetaCnt + level];
+    const int ndom = (c + B - 1) / B;
+    const int per = (ndom + blockDim.x - 1) / blockDim.x;
+    const int beg = threadIdx.x * per;
+    int end = beg + per;
+    if (end > ndom) end = ndom;
+    int local = 0;
+    for (int d = beg; d < end; ++d) local += dom_count[d];
+    // inclusive scan of `local` across the block
+    int v = local;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        int t = __shfl_up(v, off, 64);
+        if (lane >= off) v += t;
+    }
+    if (lane == 63) ws[w] = v;
+    __syncthreads();
+    if (w == 0) {
+        int s = lane < (int)(blockDim.x >> 6) ? ws[lane] : 0;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            int t = __shfl_up(s, off, 64);
+            if (lane >= off) s += t;
+        }
+        if (lane < 16) ws[lane] = s;
+    }
+    __syncthreads();
+    int excl = v - local + (w > 0 ? ws[w - 1] : 0);
+    for (int d = beg; d < end; ++d) {
+        dom_prefix[d] = excl;
+        excl += dom_count[d];
+    }
+    if (threadIdx.x == blockDim.x - 1) {
+        int totalc = (w > 0 ? ws[w - 1] : 0) + v;
+        meta[kMetaCnt + level + 1] = totalc;
+        meta[kMetaOff + level + 1] = meta[kMetaOff + level] + ndom * B;
+    }
+}
+
+__global__ void k_assign(const int* __restrict__ meta, int level, int B, const int* __restrict__ comp_rank,
+                         const int* __restrict__ dom_prefix, int* __restrict__ next_id,
+                         int* __restrict__ going_next)
+{
+    int a = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = meta[kMetaCnt + level];
+    if (a >= c) return;
+    int id = dom_prefix[a / B] + comp_rank[a];
+    next_id[a] = id;
+    going_next[meta[kMetaOff + level] + a] = meta[kMetaOff + level + 1] + id;
+}
+
+__global__ void k_map_next(int n, const int* __restrict__ map_l, const int* __restrict__ next_id,
+                           int* __restrict__ map_next)
+{
+    int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    int a = map_l ? map_l[u] : u;
+    map_next[u] = next_id[a];
+}
+
+// row_ptr[i] = first block with local row >= i (rows sorted), i in [0, n].
+__global__ void k_row_ptr(const int* __restrict__ rows, int nnz, int offset, int n, int* __restrict__ row_ptr)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    int key = i + offset, lo = 0, hi = nnz;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (rows[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    row_ptr[i] = lo;
+}
+
+// ------------------------------------------------------------------ Galerkin
+__device__ __forceinline__ void load_block(const double* __restrict__ v, double h[9])
+{
+#pragma unroll
+    for (int e = 0; e < 9; ++e) h[e] = v[e];
+}
+
+// Cross-domain blocks (row and column in different level-0 domains) climb the
+// hierarchy and are added at every level where both ends share a domain.
+template <int B>
+__global__ __launch_bounds__(256) void k_galerkin_cross(const int* __restrict__ meta, int L, int n,
+                                                        const int* __restrict__ map,
+                                                        const double* __restrict__ vals,
+                                                        const int* __restrict__ rows,
+                                                        const int* __restrict__ cols, int nnz, int offset,
+                                                        double* __restrict__ mgal)
+{
+    constexpr int nblk = B * (B + 1) / 2;
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    bool valid = k < nnz;
+    int i = 0, j = 0;
+    if (valid) {
+        i = rows[k] - offset;
+        j = cols[k] - offset;
+        valid = i >= 0 && j >= 0 && i < n && j < n && (i / B != j / B);
+    }
+    double h[9];
+    if (valid) load_block(vals + 9 * k, h);
+    else
+#pragma unroll
+        for (int e = 0; e < 9; ++e) h[e] = 0.0;
+    const int ndom0 = meta[kMetaOff + 1] / B;
+    for (int l = 1; l < L; ++l) {
+        int a = 0, b = 0;
+        if (valid) {
+            a = map[(long)l * n + i];
+            b = map[(long)l * n + j];
+        }
+        const bool same = valid && (a / B == b / B);
+        double v[9];
+        int key = -1;
+        if (same) {
+            const int dom = (meta[kMetaOff + l] + a) / B - ndom0;
+            const int ra = a % B, rb = b % B;
+            if (a == b) {
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) v[c * 3 + r] = h[c * 3 + r] + h[r * 3 + c];
+                key = dom * nblk + blk_index(B, ra, ra);
+            } else if (a < b) {
+#pragma unroll
+                for (int e = 0; e < 9; ++e) v[e] = h[e];
+                key = dom * nblk + blk_index(B, ra, rb);
+            } else {
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) v[c * 3 + r] = h[r * 3 + c];
+                key = dom * nblk + blk_index(B, rb, ra);
+            }
+        }
+        const uint64_t act = __ballot(same);
+        if (act == 0) continue;
+        const int leader = __ffsll((unsigned long long)act) - 1;
+        const int k0 = __shfl(key, leader, 64);
+        const bool uniform = __ballot(same && key != k0) == 0;
+        if (uniform) {
+            double* dst = mgal + (long)k0 * 9;
+#pragma unroll
+            for (int e = 0; e < 9; ++e) {
+                double t = wave_sum(same ? v[e] : 0.0);
+                if (lane == leader) atomic_add(dst + e, t);
+            }
+        } else if (same) {
+            double* dst = mgal + (long)key * 9;
+#pragma unroll
+            for (int e = 0; e < 9; ++e) atomic_add(dst + e, v[e]);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ inverse
+template <int B>
+struct InvShape {
+    static constexpr int N = 3 * B;
+    static constexpr int G = N / 6;  // tile grid edge
+    static constexpr int T = G * G;  // threads
+};
+
+// In-place Gauss-Jordan without pivoting on a 3Bx3B matrix distributed in 6x6
+// register tiles (thread (ty,tx) holds rows 6ty.., cols 6tx..).
+template <int B>
+__device__ __forceinline__ void gj_invert(double a[6][6], double* __restrict__ colbuf,
+                                          double* __restrict__ rowbuf, int ty, int tx)
+{
+    constexpr int N = InvShape<B>::N;
+    // zero diagonal -> 1 (padding slots), MASPreconditioner.cu:660-666
+    if (ty == tx) {
+#pragma unroll
+        for (int d = 0; d < 6; ++d)
+            if (a[d][d] == 0.0) a[d][d] = 1.0;
+    }
+    for (int j = 0; j < N; ++j) {
+        double* cb = colbuf + (j & 1) * N;
+        double* rb = rowbuf + (j & 1) * N;
+        const int jt = j / 6, jo = j % 6;
+        if (tx == jt) {
+#pragma unroll
+            for (int r = 0; r < 6; ++r)
+                if (r == jo) {
+#pragma unroll
+                    for (int rr = 0; rr < 6; ++rr) cb[6 * ty + rr] = a[rr][r];
+                }
+        }
+        if (ty == jt) {
+#pragma unroll
+            for (int r = 0; r < 6; ++r)
+                if (r == jo) {
+#pragma unroll
+                    for (int cc = 0; cc < 6; ++cc) rb[6 * tx + cc] = a[r][cc];
+                }
+        }
+        __syncthreads();
+        const double ip = 1.0 / cb[j];
+        double rc[6], cr[6];
+#pragma unroll
+        for (int cc = 0; cc < 6; ++cc) rc[cc] = rb[6 * tx + cc] * ip;
+#pragma unroll
+        for (int rr = 0; rr < 6; ++rr) cr[rr] = cb[6 * ty + rr];
+#pragma unroll
+        for (int rr = 0; rr < 6; ++rr)
+#pragma unroll
+            for (int cc = 0; cc < 6; ++cc) a[rr][cc] -= cr[rr] * rc[cc];
+        if (ty == jt) {  // pivot row: scaled
+#pragma unroll
+            for (int r = 0; r < 6; ++r)
+                if (r == jo) {
+#pragma unroll
+                    for (int cc = 0; cc < 6; ++cc) a[r][cc] = rc[cc];
+                }
+        }
+        if (tx == jt) {  // pivot column: -col/p, pivot 1/p
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+                if (c == jo) {
+#pragma unroll
+                    for (int rr = 0; rr < 6; ++rr) a[rr][c] = -cr[rr] * ip;
+                    if (ty == jt) {
+#pragma unroll
+                        for (int r = 0; r < 6; ++r)
+                            if (r == jo) a[r][c] = ip;
+                    }
+                }
+        }
+    }
+}
+
+// Store the upper 3x3 blocks of the inverse (diagonal blocks symmetrised from
+// their upper part, MASPreconditioner.cu:700-705).
+template <int B, typename TO>
+__device__ __forceinline__ void store_inverse(double a[6][6], int ty, int tx, TO* __restrict__ out)
+{
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) {
+            const int rb = 2 * ty + bi, cb = 2 * tx + bj;
+            if (rb > cb) continue;
+            TO* dst = out + 9 * blk_index(B, rb, cb);
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    double v = a[3 * bi + r][3 * bj + c];
+                    if (rb == cb && r > c) v = a[3 * bi + c][3 * bj + r];
+                    dst[c * 3 + r] = (TO)v;
+                }
+        }
+}
+
+// Find block (u, v) (local ids, u <= v) in the sorted BCOO; false if absent.
+__device__ __forceinline__ bool find_block(const int* __restrict__ row_ptr, const int* __restrict__ cols,
+                                           int u, int v, int offset, long* pos)
+{
+    int lo = row_ptr[u], hi = row_ptr[u + 1];
+    const int key = v + offset;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        int c = cols[mid];
+        if (c == key) {
+            *pos = mid;
+            return true;
+        }
+        if (c < key) lo = mid + 1; else hi = mid;
+    }
+    return false;
+}
+
+constexpr int kMaxLocalComp = 8;
+
+// Level-0 domains: gather blocks from the BCOO, push the domain's Galerkin
+// contribution to the coarse levels, invert, store.
+template <int B, typename TO>
+__global__ __launch_bounds__(InvShape<B>::T) void k_inv_level0(
+    const int* __restrict__ meta, int L, int n, const int* __restrict__ map,
+    const int* __restrict__ going_next, const double* __restrict__ vals,
+    const int* __restrict__ cols, const int* __restrict__ row_ptr, int offset,
+    double* __restrict__ mgal, TO* __restrict__ minv)
+{
+    constexpr int N = InvShape<B>::N, G = InvShape<B>::G;
+    constexpr int nblk = B * (B + 1) / 2;
+    __shared__ double colbuf[2 * N], rowbuf[2 * N];
+    __shared__ double S[kMaxLocalComp * kMaxLocalComp * 9];
+    __shared__ double red[16];
+    const int d = blockIdx.x;
+    const int ty = threadIdx.x / G, tx = threadIdx.x % G;
+    double a[6][6];
+    int unode[2], vnode[2];  // level-1 ids of the tile's block rows / cols
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi) {
+        int u = d * B + 2 * ty + bi, v = d * B + 2 * tx + bi;
+        unode[bi] = (u < n && L > 1) ? map[n + u] : -1;
+        vnode[bi] = (v < n && L > 1) ? map[n + v] : -1;
+    }
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) {
+            const int rb = 2 * ty + bi, cb = 2 * tx + bj;
+            const int u = d * B + rb, v = d * B + cb;
+            double h[9];
+#pragma unroll
+            for (int e = 0; e < 9; ++e) h[e] = 0.0;
+            if (u < n && v < n) {
+                long pos;
+                if (rb <= cb) {
+                    if (find_block(row_ptr, cols, u, v, offset, &pos)) load_block(vals + 9 * pos, h);
+                } else {
+                    if (find_block(row_ptr, cols, v, u, offset, &pos)) {
+                        double t[9];
+                        load_block(vals + 9 * pos, t);
+#pragma unroll
+                        for (int r = 0; r < 3; ++r)
+#pragma unroll
+                            for (int c = 0; c < 3; ++c) h[c * 3 + r] = t[r * 3 + c];
+                    }
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) a[3 * bi + r][3 * bj + c] = h[c * 3 + r];
+        }
+
+    // ---- Galerkin contribution of this domain to levels >= 1
+    if (L > 1) {
+        const int first = map[n + d * B];
+        const int next_first = ((d + 1) * B < n) ? map[n + (d + 1) * B] : meta[kMetaCnt + 1];
+        const int k = next_first - first;
+        const int ndom0 = meta[kMetaOff + 1] / B;
+        if (k == 1) {
+            double s[9];
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    double t = 0.0;
+#pragma unroll
+                    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                        for (int bj = 0; bj < 2; ++bj)
+                            if (unode[bi] >= 0 && vnode[bj] >= 0) t += a[3 * bi + r][3 * bj + c];
+                    s[c * 3 + r] = block_sum(t, red);
+                }
+            if (threadIdx.x == 0) {
+                int slot = meta[kMetaOff + 1] + first;
+                for (int l = 1; l < L; ++l) {
+                    const int ra = slot % B;
+                    double* dst = mgal + ((long)(slot / B - ndom0) * nblk + blk_index(B, ra, ra)) * 9;
+#pragma unroll
+                    for (int e = 0; e < 9; ++e) atomic_add(dst + e, s[e]);
+                    if (l + 1 < L) slot = going_next[slot];
+                }
+            }
+        } else if (k <= kMaxLocalComp) {
+            for (int e = threadIdx.x; e < k * k * 9; e += blockDim.x) S[e] = 0.0;
+            __syncthreads();
+#pragma unroll
+            for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj) {
+                    if (unode[bi] < 0 || vnode[bj] < 0) continue;
+                    double* dst = S + ((unode[bi] - first) * k + (vnode[bj] - first)) * 9;
+#pragma unroll
+                    for (int r = 0; r < 3; ++r)
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) atomic_add(dst + c * 3 + r, a[3 * bi + r][3 * bj + c]);
+                }
+            __syncthreads();
+            if ((int)threadIdx.x < k * k) {
+                const int p = threadIdx.x / k, q = threadIdx.x % k;
+                int sp = meta[kMetaOff + 1] + first + p, sq = meta[kMetaOff + 1] + first + q;
+                const double* src = S + (p * k + q) * 9;
+                for (int l = 1; l < L; ++l) {
+                    if (sp / B == sq / B && sp <= sq) {
+                        double* dst = mgal + ((long)(sp / B - ndom0) * nblk + blk_index(B, sp % B, sq % B)) * 9;
+#pragma unroll
+                        for (int e = 0; e < 9; ++e) atomic_add(dst + e, src[e]);
+                    }
+                    if (l + 1 < L) {
+                        sp = going_next[sp];
+                        sq = going_next[sq];
+                    }
+                }
+            }
+        } else {
+            // fragmented domain: every block climbs on its own
+#pragma unroll
+            for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj) {
+                    if (unode[bi] < 0 || vnode[bj] < 0) continue;
+                    int sp = meta[kMetaOff + 1] + unode[bi], sq = meta[kMetaOff + 1] + vnode[bj];
+                    for (int l = 1; l < L; ++l) {
+                        if (sp / B == sq / B && sp <= sq) {
+                            double* dst = mgal + ((long)(sp / B - ndom0) * nblk + blk_index(B, sp % B, sq % B)) * 9;
+#pragma unroll
+                            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                                for (int c = 0; c < 3; ++c) atomic_add(dst + c * 3 + r, a[3 * bi + r][3 * bj + c]);
+                        }
+                        if (l + 1 < L) {
+                            sp = going_next[sp];
+                            sq = going_next[sq];
+                        }
+                    }
+                }
+        }
+    }
+
+    gj_invert<B>(a, colbuf, rowbuf, ty, tx);
+    store_inverse<B, TO>(a, ty, tx, minv + (long)d * nblk * 9);
+}
+
+// Coarse domains: blocks from the Galerkin array.
+template <int B, typename TO>
+__global__ __launch_bounds__(InvShape<B>::T) void k_inv_coarse(const double* __restrict__ mgal, int ndom0,
+                                                              TO* __restrict__ minv)
+{
+    constexpr int N = InvShape<B>::N, G = InvShape<B>::G;
+    constexpr int nblk = B * (B + 1) / 2;
+    __shared__ double colbuf[2 * N], rowbuf[2 * N];
+    const int dc = blockIdx.x;
+    const int ty = threadIdx.x / G, tx = threadIdx.x % G;
+    const double* M = mgal + (long)dc * nblk * 9;
+    double a[6][6];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) {
+            const int rb = 2 * ty + bi, cb = 2 * tx + bj;
+            double h[9];
+            if (rb <= cb) {
+                load_block(M + 9 * blk_index(B, rb, cb), h);
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) a[3 * bi + r][3 * bj + c] = h[c * 3 + r];
+            } else {
+                load_block(M + 9 * blk_index(B, cb, rb), h);
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) a[3 * bi + r][3 * bj + c] = h[r * 3 + c];
+            }
+        }
+    gj_invert<B>(a, colbuf, rowbuf, ty, tx);
+    store_inverse<B, TO>(a, ty, tx, minv + (long)(ndom0 + dc) * nblk * 9);
+}
+
+// ------------------------------------------------------------------ apply
+template <typename TS>
+struct Vec3 {
+    TS x, y, z;
+};
+
+// Level 0 -> slots and level 1 sums; evaluates the PCG stop rule first.
+template <int B, typename TV, typename TS>
+__global__ __launch_bounds__(256) void k_restrict0(PcgStopDesc sd, const int* __restrict__ meta, int n,
+                                                   int L, const int* __restrict__ map1,
+                                                   const TV* __restrict__ r, TS* __restrict__ R)
+{
+    __shared__ double smem[16];
+    __shared__ int s_done;
+    if (sd.ctl != nullptr) {
+        // one read per workgroup: block 0 of this kernel may raise the flag
+        if (threadIdx.x == 0) s_done = sd.ctl->done;
+        __syncthreads();
+        if (s_done) return;
+        double rz = block_sum_array(sd.rz, sd.n_rz, smem);
+        double rr = block_sum_array(sd.rr, sd.n_rr, smem);
+        bool stop = (sd.ctl->stop_mode == 0) ? fabs(rz) <= sd.ctl->tol * sd.ctl->rz0
+                                             : rr <= sd.ctl->tol * sd.ctl->tol * sd.ctl->bb;
+        if (stop) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                sd.ctl->done = 1;
+                sd.ctl->final_rr = rr;
+                sd.ctl->final_rz = rz;
+            }
+            return;
+        }
+    }
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int npad = meta[kMetaOff + 1];
+    const bool real = s < n;
+    TS v0 = 0, v1 = 0, v2 = 0;
+    if (real) {
+        v0 = (TS)r[3 * (long)s];
+        v1 = (TS)r[3 * (long)s + 1];
+        v2 = (TS)r[3 * (long)s + 2];
+    }
+    if (s < npad) {
+        R[3 * (long)s] = v0;
+        R[3 * (long)s + 1] = v1;
+        R[3 * (long)s + 2] = v2;
+    }
+    if (L > 1) {
+        // group-by level-1 node inside the wave (a node lives in one domain)
+        int node = real ? map1[s] : -1;
+        bool pending = node >= 0;
+        const int lane = threadIdx.x & 63;
+        while (true) {
+            const uint64_t act = __ballot(pending);
+            if (act == 0) break;
+            const int leader = __ffsll((unsigned long long)act) - 1;
+            const int key = __shfl(node, leader, 64);
+            const bool mine = pending && node == key;
+            TS s0 = wave_sum(mine ? v0 : TS(0));
+            TS s1 = wave_sum(mine ? v1 : TS(0));
+            TS s2 = wave_sum(mine ? v2 : TS(0));
+            if (lane == leader) {
+                const long o = 3 * (long)(npad + key);
+                R[o] = s0;
+                R[o + 1] = s1;
+                R[o + 2] = s2;
+            }
+            if (mine) pending = false;
+        }
+        // zero the padding slots of level 1
+        const int c1 = meta[kMetaCnt + 1];
+        const int pad = meta[kMetaOff + 2] - npad - c1;
+        if (s < pad) {
+            const long o = 3 * (long)(npad + c1 + s);
+            R[o] = 0;
+            R[o + 1] = 0;
+            R[o + 2] = 0;
+        }
+    }
+}
+
+// Level l-1 -> level l sums (l >= 2).
+template <typename TS>
+__global__ __launch_bounds__(256) void k_restrict_up(const int* __restrict__ done, const int* __restrict__ meta,
+                                                     int level, const int* __restrict__ going_next,
+                                                     TS* __restrict__ R)
+{
+    if (done != nullptr && *done) return;
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int cprev = meta[kMetaCnt + level - 1];
+    const int offprev = meta[kMetaOff + level - 1];
+    const bool real = a < cprev;
+    TS v0 = 0, v1 = 0, v2 = 0;
+    int parent = -1;
+    if (real) {
+        const long o = 3 * (long)(offprev + a);
+        v0 = R[o];
+        v1 = R[o + 1];
+        v2 = R[o + 2];
+        parent = going_next[offprev + a];
+    }
+    bool pending = real;
+    const int lane = threadIdx.x & 63;
+    while (true) {
+        const uint64_t act = __ballot(pending);
+        if (act == 0) break;
+        const int leader = __ffsll((unsigned long long)act) - 1;
+        const int key = __shfl(parent, leader, 64);
+        const bool mine = pending && parent == key;
+        TS s0 = wave_sum(mine ? v0 : TS(0));
+        TS s1 = wave_sum(mine ? v1 : TS(0));
+        TS s2 = wave_sum(mine ? v2 : TS(0));
+        if (lane == leader) {
+            R[3 * (long)key] = s0;
+            R[3 * (long)key + 1] = s1;
+            R[3 * (long)key + 2] = s2;
+        }
+        if (mine) pending = false;
+    }
+    const int c = meta[kMetaCnt + level];
+    const int off = meta[kMetaOff + level];
+    const int pad = meta[kMetaOff + level + 1] - off - c;
+    if (a < pad) {
+        const long o = 3 * (long)(off + c + a);
+        R[o] = 0;
+        R[o + 1] = 0;
+        R[o + 2] = 0;
+    }
+}
+
+template <typename TS>
+struct Vec16;
+template <>
+struct Vec16<double> {
+    using V = double2;
+    static constexpr int k = 2;
+};
+template <>
+struct Vec16<float> {
+    using V = float4;
+    static constexpr int k = 4;
+};
+
+// Z[domain] = Minv[domain] * R[domain], one workgroup per domain.
+template <int B, typename TS>
+__global__ __launch_bounds__(128) void k_local_solve(const int* __restrict__ done, const TS* __restrict__ minv,
+                                                      const TS* __restrict__ R, TS* __restrict__ Z)
+{
+    if (done != nullptr && *done) return;
+    constexpr int nblk = B * (B + 1) / 2;
+    constexpr int nval = nblk * 9;
+    using V = typename Vec16<TS>::V;
+    constexpr int kv = Vec16<TS>::k;
+    __shared__ __align__(16) TS sM[nval + 4];
+    __shared__ TS sR[3 * B];
+    const int d = blockIdx.x;
+    const TS* src = minv + (long)d * nval;
+    // nval*sizeof(TS) is a multiple of 16 for B in {16, 32}
+    const V* vsrc = reinterpret_cast<const V*>(src);
+    for (int e = threadIdx.x; e < nval / kv; e += blockDim.x) reinterpret_cast<V*>(sM)[e] = vsrc[e];
+    for (int e = threadIdx.x; e < 3 * B; e += blockDim.x) sR[e] = R[3 * (long)d * B + e];
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t < 3 * B) {
+        const int rb = t / 3, i = t % 3;
+        TS acc = 0;
+        for (int cb = 0; cb < B; ++cb) {
+            const TS* m;
+            if (cb >= rb) {
+                m = sM + 9 * blk_index(B, rb, cb);
+                acc += m[i] * sR[3 * cb] + m[3 + i] * sR[3 * cb + 1] + m[6 + i] * sR[3 * cb + 2];
+            } else {
+                m = sM + 9 * blk_index(B, cb, rb);
+                acc += m[3 * i] * sR[3 * cb] + m[3 * i + 1] * sR[3 * cb + 1] + m[3 * i + 2] * sR[3 * cb + 2];
+            }
+        }
+        Z[3 * (long)d * B + t] = acc;
+    }
+}
+
+// z[v] = sum_l Z[slot_l(v)], with r.z partials.
+template <typename TV, typename TS>
+__global__ __launch_bounds__(256) void k_prolong(const int* __restrict__ done, const int* __restrict__ meta,
+                                                 int n, int L, const int* __restrict__ map,
+                                                 const TS* __restrict__ Z, const TV* __restrict__ r,
+                                                 TV* __restrict__ z, double* __restrict__ rz_part)
+{
+    __shared__ double smem[16];
+    if (done != nullptr && *done) return;
+    int offs[kMasMaxLevels];
+#pragma unroll
+    for (int l = 0; l < kMasMaxLevels; ++l) offs[l] = l < L ? meta[kMetaOff + l] : 0;
+    double acc = 0.0;
+    for (long u = (long)blockIdx.x * blockDim.x + threadIdx.x; u < n; u += (long)gridDim.x * blockDim.x) {
+        TS z0 = Z[3 * u], z1 = Z[3 * u + 1], z2 = Z[3 * u + 2];
+        for (int l = 1; l < L; ++l) {
+            const long o = 3 * (long)(offs[l] + map[(long)l * n + u]);
+            z0 += Z[o];
+            z1 += Z[o + 1];
+            z2 += Z[o + 2];
+        }
+        const TV w0 = (TV)z0, w1 = (TV)z1, w2 = (TV)z2;
+        z[3 * u] = w0;
+        z[3 * u + 1] = w1;
+        z[3 * u + 2] = w2;
+        acc += (double)r[3 * u] * (double)w0 + (double)r[3 * u + 1] * (double)w1 + (double)r[3 * u + 2] * (double)w2;
+    }
+    double s = block_sum(acc, smem);
+    if (threadIdx.x == 0 && rz_part) rz_part[blockIdx.x] = s;
+}
+
+int grid_for(long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+}  // namespace
+
+int mas_num_levels(int n, int B)
+{
+    int nLevel = 1;
+    int levelSz = (n + B - 1) / B * B;
+    while (levelSz > B) {
+        levelSz /= B;
+        nLevel++;
+        levelSz = (levelSz + B - 1) / B * B;
+    }
+    nLevel = nLevel + 1;
+    return nLevel > kMasMaxLevels ? kMasMaxLevels : nLevel;
+}
+
+Mas::Mas(Context* c, int bank) : ctx(c), B(bank)
+{
+    SGIPC_CHECK(bank == 16 || bank == 32, SGIPC_BAD_ARG, "bank_size must be 16 or 32");
+}
+
+void Mas::init_neighbor(int vn, int cof, int total_nbr, const uint32_t* start, const uint32_t* num,
+                        const uint32_t* list)
+{
+    SGIPC_CHECK(vn >= 1, SGIPC_EMPTY, "MAS needs at least one vertex");
+    SGIPC_CHECK(start && num && (list || total_nbr == 0), SGIPC_BAD_ARG, "null neighbour arrays");
+    n = vn;
+    coff = cof;
+    L = mas_num_levels(n, B);
+    hipStream_t s = ctx->stream;
+    nbr_start_.reserve(n);
+    nbr_num_.reserve(n);
+    nbr_list_.reserve(total_nbr > 0 ? total_nbr : 1);
+    SGIPC_HIP(hipMemcpyAsync(nbr_start_.get(), start, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s));
+    SGIPC_HIP(hipMemcpyAsync(nbr_num_.get(), num, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s));
+    if (total_nbr > 0)
+        SGIPC_HIP(hipMemcpyAsync(nbr_list_.get(), list, sizeof(uint32_t) * total_nbr, hipMemcpyDeviceToDevice, s));
+    map_.reserve((size_t)L * n);
+    meta_.reserve(kMetaSize);
+    // every level has at most n nodes: off_L <= L * (n + B)
+    going_next_.reserve((size_t)L * (n + B));
+    mask_.reserve(n);
+    comp_rank_.reserve(n + 64);
+    dom_count_.reserve(n / B + 2);
+    dom_prefix_.reserve(n / B + 2);
+    next_id_.reserve(n);
+    row_ptr_.reserve(n + 1);
+    built = false;
+}
+
+template <int BB>
+void Mas::build_hierarchy_t(const int* pairs, int n_pairs)
+{
+    hipStream_t s = ctx->stream;
+    std::vector<int> hmeta(kMetaSize, 0);
+    hmeta[kMetaCnt + 0] = n;
+    hmeta[kMetaOff + 0] = 0;
+    SGIPC_HIP(hipMemcpyAsync(meta_.get(), hmeta.data(), sizeof(int) * kMetaSize, hipMemcpyHostToDevice, s));
+    const int gn = grid_for(n, 256);
+    for (int l = 0; l < L; ++l) {
+        const int* map_l = l == 0 ? nullptr : map_.get() + (size_t)l * n;
+        if (l > 0) launch_fill<int>(s, reinterpret_cast<int*>(mask_.get()), 0, n);
+        k_mask_fine<BB><<<gn, 256, 0, s>>>(l, n, map_l, nbr_start_.get(), nbr_num_.get(), nbr_list_.get(), mask_.get());
+        SGIPC_LAUNCH_CHECK();
+        if (n_pairs > 0) {
+            k_mask_contact<BB><<<grid_for(n_pairs, 256), 256, 0, s>>>(n, coff, reinterpret_cast<const int4*>(pairs),
+                                                                     n_pairs, map_l, mask_.get());
+            SGIPC_LAUNCH_CHECK();
+        }
+        // upper bound on the node count of level l is n, padded to whole blocks
+        const int gpad = grid_for((long)((n + BB - 1) / BB) * BB, 256);
+        k_closure<BB><<<gpad, 256, 0, s>>>(meta_.get(), l, mask_.get(), comp_rank_.get(), dom_count_.get());
+        SGIPC_LAUNCH_CHECK();
+        k_scan_domains<<<1, 1024, 0, s>>>(meta_.get(), l, BB, dom_count_.get(), dom_prefix_.get());
+        SGIPC_LAUNCH_CHECK();
+        k_assign<<<gn, 256, 0, s>>>(meta_.get(), l, BB, comp_rank_.get(), dom_prefix_.get(), next_id_.get(),
+                                    going_next_.get());
+        SGIPC_LAUNCH_CHECK();
+        if (l + 1 < L) {
+            k_map_next<<<gn, 256, 0, s>>>(n, map_l, next_id_.get(), map_.get() + (size_t)(l + 1) * n);
+            SGIPC_LAUNCH_CHECK();
+        }
+    }
+    SGIPC_HIP(hipMemcpyAsync(hmeta.data(), meta_.get(), sizeof(int) * kMetaSize, hipMemcpyDeviceToHost, s));
+    SGIPC_HIP(hipStreamSynchronize(s));
+    for (int l = 0; l <= L; ++l) {
+        h_cnt[l] = hmeta[kMetaCnt + l];
+        h_off[l] = hmeta[kMetaOff + l];
+    }
+    total = h_off[L];
+}
+
+void Mas::build_hierarchy(const int* pairs, int n_pairs)
+{
+    if (B == 32) build_hierarchy_t<32>(pairs, n_pairs);
+    else build_hierarchy_t<16>(pairs, n_pairs);
+}
+
+template <int BB>
+void Mas::assemble_invert_t(const double* vals, const int* rows, const int* cols, int nnz, int offset)
+{
+    hipStream_t s = ctx->stream;
+    constexpr int nblk = BB * (BB + 1) / 2;
+    const int ndom0 = h_off[1] / BB;
+    const int ndom = total / BB;
+    const int ndomc = ndom - ndom0;
+    mgal_.reserve((size_t)(ndomc > 0 ? ndomc : 1) * nblk * 9);
+    if (ndomc > 0) launch_fill<double>(s, mgal_.get(), 0.0, (long)ndomc * nblk * 9);
+    k_row_ptr<<<grid_for(n + 1, 256), 256, 0, s>>>(rows, nnz, offset, n, row_ptr_.get());
+    SGIPC_LAUNCH_CHECK();
+    if (L > 1 && nnz > 0) {
+        k_galerkin_cross<BB><<<grid_for(nnz, 256), 256, 0, s>>>(meta_.get(), L, n, map_.get(), vals, rows, cols,
+                                                               nnz, offset, mgal_.get());
+        SGIPC_LAUNCH_CHECK();
+    }
+    constexpr int T = InvShape<BB>::T;
+    if (precision == SGIPC_FP64) {
+        minv64_.reserve((size_t)ndom * nblk * 9);
+        k_inv_level0<BB, double><<<ndom0, T, 0, s>>>(meta_.get(), L, n, map_.get(), going_next_.get(), vals, cols,
+                                                    row_ptr_.get(), offset, mgal_.get(), minv64_.get());
+        SGIPC_LAUNCH_CHECK();
+        if (ndomc > 0) {
+            k_inv_coarse<BB, double><<<ndomc, T, 0, s>>>(mgal_.get(), ndom0, minv64_.get());
+            SGIPC_LAUNCH_CHECK();
+        }
+    } else {
+        minv32_.reserve((size_t)ndom * nblk * 9);
+        k_inv_level0<BB, float><<<ndom0, T, 0, s>>>(meta_.get(), L, n, map_.get(), going_next_.get(), vals, cols,
+                                                   row_ptr_.get(), offset, mgal_.get(), minv32_.get());
+        SGIPC_LAUNCH_CHECK();
+        if (ndomc > 0) {
+            k_inv_coarse<BB, float><<<ndomc, T, 0, s>>>(mgal_.get(), ndom0, minv32_.get());
+            SGIPC_LAUNCH_CHECK();
+        }
+    }
+    R64_.reserve((size_t)total * 3);
+    Z64_.reserve((size_t)total * 3);
+    R32_.reserve((size_t)total * 3);
+    Z32_.reserve((size_t)total * 3);
+}
+
+void Mas::init_matrix()
+{
+    SGIPC_CHECK(n >= 1, SGIPC_EMPTY, "init_neighbor first");
+    build_hierarchy(nullptr, 0);
+}
+
+void Mas::set_bcoo(const double* vals, const int* rows, const int* cols, int nnz, int offset,
+                   const int* pairs, int n_pairs, int prec)
+{
+    SGIPC_CHECK(n >= 1, SGIPC_EMPTY, "init_neighbor first");
+    SGIPC_CHECK(nnz >= 0, SGIPC_BAD_ARG, "negative nnz");
+    SGIPC_CHECK(n_pairs == 0 || pairs != nullptr, SGIPC_BAD_ARG, "null pairs");
+    auto t0 = std::chrono::steady_clock::now();
+    precision = prec == SGIPC_FP64 ? SGIPC_FP64 : SGIPC_FP32_MAS;
+    build_hierarchy(pairs, n_pairs);
+    if (B == 32) assemble_invert_t<32>(vals, rows, cols, nnz, offset);
+    else assemble_invert_t<16>(vals, rows, cols, nnz, offset);
+    SGIPC_HIP(hipStreamSynchronize(ctx->stream));
+    ms_build = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    built = true;
+}
+
+template <int BB, typename TV, typename TS>
+int Mas::apply_t(hipStream_t s, const TV* r, TV* z, double* rz_part, const PcgStopDesc& sd)
+{
+    TS* R;
+    TS* Z;
+    const TS* minv;
+    if constexpr (sizeof(TS) == 8) {
+        R = R64_.get();
+        Z = Z64_.get();
+        minv = minv64_.get();
+    } else {
+        R = R32_.get();
+        Z = Z32_.get();
+        minv = minv32_.get();
+    }
+    const int* done = sd.ctl ? &sd.ctl->done : nullptr;
+    const int npad = h_off[1];
+    k_restrict0<BB, TV, TS><<<grid_for(npad, 256), 256, 0, s>>>(sd, meta_.get(), n, L, map_.get() + n, r, R);
+    SGIPC_LAUNCH_CHECK();
+    for (int l = 2; l < L; ++l) {
+        int c = h_cnt[l - 1];
+        int pad = h_off[l + 1] - h_off[l] - h_cnt[l];
+        int work = c > pad ? c : pad;
+        k_restrict_up<TS><<<grid_for(work > 0 ? work : 1, 256), 256, 0, s>>>(done, meta_.get(), l, going_next_.get(), R);
+        SGIPC_LAUNCH_CHECK();
+    }
+    k_local_solve<BB, TS><<<total / BB, 128, 0, s>>>(done, minv, R, Z);
+    SGIPC_LAUNCH_CHECK();
+    const int g = vec_grid(n);
+    k_prolong<TV, TS><<<g, 256, 0, s>>>(done, meta_.get(), n, L, map_.get(), Z, r, z, rz_part);
+    SGIPC_LAUNCH_CHECK();
+    return g;
+}
+
+template <typename TV>
+int Mas::apply(hipStream_t s, const TV* r, TV* z, double* rz_part, const PcgStopDesc& sd)
+{
+    SGIPC_CHECK(built, SGIPC_BAD_ARG, "MAS used before setPreconditioner");
+    if (precision == SGIPC_FP64) {
+        if (B == 32) return apply_t<32, TV, double>(s, r, z, rz_part, sd);
+        return apply_t<16, TV, double>(s, r, z, rz_part, sd);
+    }
+    if (B == 32) return apply_t<32, TV, float>(s, r, z, rz_part, sd);
+    return apply_t<16, TV, float>(s, r, z, rz_part, sd);
+}
+template int Mas::apply<double>(hipStream_t, const double*, double*, double*, const PcgStopDesc&);
+template int Mas::apply<float>(hipStream_t, const float*, float*, double*, const PcgStopDesc&);
+
+int Mas::levels(int* offsets, int* counts) const
+{
+    for (int l = 0; l <= L; ++l) offsets[l] = h_off[l];
+    for (int l = 0; l < L; ++l) counts[l] = h_cnt[l];
+    return L;
+}
+
+namespace {
+__global__ void k_tables(int n, int L, const int* __restrict__ meta, const int* __restrict__ map,
+                         int* __restrict__ coarse_table)
+{
+    int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    for (int l = 1; l < L; ++l)
+        coarse_table[(long)u * (L - 1) + (l - 1)] = meta[kMetaOff + l] + map[(long)l * n + u];
+}
+__global__ void k_widen(const float* __restrict__ in, double* __restrict__ out, long cnt)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cnt) out[i] = (double)in[i];
+}
+}  // namespace
+
+void Mas::tables(int* coarse_table, int* going_next) const
+{
+    hipStream_t s = ctx->stream;
+    if (coarse_table && L > 1) {
+        k_tables<<<grid_for(n, 256), 256, 0, s>>>(n, L, meta_.get(), map_.get(), coarse_table);
+        SGIPC_LAUNCH_CHECK();
+    }
+    if (going_next)
+        SGIPC_HIP(hipMemcpyAsync(going_next, going_next_.get(), sizeof(int) * total, hipMemcpyDeviceToDevice, s));
+    SGIPC_HIP(hipStreamSynchronize(s));
+}
+
+void Mas::inverse_blocks(double* out) const
+{
+    hipStream_t s = ctx->stream;
+    long cnt = (long)(total / B) * (B * (B + 1) / 2) * 9;
+    if (precision == SGIPC_FP64) {
+        SGIPC_HIP(hipMemcpyAsync(out, minv64_.get(), sizeof(double) * cnt, hipMemcpyDeviceToDevice, s));
+    } else {
+        k_widen<<<grid_for(cnt, 256), 256, 0, s>>>(minv32_.get(), out, cnt);
+        SGIPC_LAUNCH_CHECK();
+    }
+    SGIPC_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace sgipc

@@ -1,0 +1,75 @@
+// Multilevel additive Schwarz preconditioner (host object).
+#pragma once
+#include "pcg.hpp"
+#include "sgipc_internal.hpp"
+
+namespace sgipc {
+
+constexpr int kMasMaxLevels = 6;
+// layout of the device metadata array
+constexpr int kMetaCnt = 0;   // counts c_0..c_L
+constexpr int kMetaOff = 8;   // offsets off_0..off_L
+constexpr int kMetaSize = 32;
+
+class Mas {
+  public:
+    Mas(Context* ctx, int bank);
+    void init_neighbor(int n, int coff, int total_nbr, const uint32_t* start, const uint32_t* num,
+                       const uint32_t* list);
+    void init_matrix();
+    void set_bcoo(const double* vals, const int* rows, const int* cols, int nnz, int offset,
+                  const int* pairs, int n_pairs, int precision);
+    // z = M^-1 r, rz partials; see Preconditioner.
+    template <typename TV>
+    int apply(hipStream_t s, const TV* r, TV* z, double* rz_part, const PcgStopDesc& sd);
+    int levels(int* offsets, int* counts) const;
+    void tables(int* coarse_table, int* going_next) const;
+    void inverse_blocks(double* out) const;
+
+    Context* ctx;
+    int B;
+    int n = 0, L = 0, coff = 0, precision = 0;
+    int h_cnt[kMasMaxLevels + 2] = {0};
+    int h_off[kMasMaxLevels + 2] = {0};
+    int total = 0;
+    bool built = false;
+    double ms_build = 0.0;
+
+  private:
+    void build_hierarchy(const int* pairs, int n_pairs);
+    template <int BB>
+    void build_hierarchy_t(const int* pairs, int n_pairs);
+    template <int BB>
+    void assemble_invert_t(const double* vals, const int* rows, const int* cols, int nnz, int offset);
+    template <int BB, typename TV, typename TS>
+    int apply_t(hipStream_t s, const TV* r, TV* z, double* rz_part, const PcgStopDesc& sd);
+
+    DevBuf<uint32_t> nbr_start_, nbr_num_, nbr_list_;
+    DevBuf<int> map_;        // [L][n], row 0 unused (identity)
+    DevBuf<int> meta_;
+    DevBuf<int> going_next_;
+    DevBuf<uint32_t> mask_;
+    DevBuf<int> comp_rank_, dom_count_, dom_prefix_, next_id_;
+    DevBuf<int> row_ptr_;
+    DevBuf<double> mgal_;     // coarse-level Galerkin blocks
+    DevBuf<double> minv64_;
+    DevBuf<float> minv32_;
+    DevBuf<double> R64_, Z64_;
+    DevBuf<float> R32_, Z32_;
+    int apply_partials_ = 0;
+};
+
+// Adapter giving the PCG driver a preconditioner over a Mas.
+template <typename TV>
+struct MasPrecond : Preconditioner<TV> {
+    Mas* mas;
+    explicit MasPrecond(Mas* m) : mas(m) {}
+    int apply(hipStream_t s, const TV* r, TV* z, double* rz_part, const PcgStopDesc& sd) override
+    {
+        return mas->template apply<TV>(s, r, z, rz_part, sd);
+    }
+};
+
+int mas_num_levels(int n, int bank);
+
+}  // namespace sgipc

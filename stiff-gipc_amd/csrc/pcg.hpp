@@ -1,0 +1,101 @@
+// PCG driver interfaces (device-resident scalars, chunked host checks).
+#pragma once
+#include "sgipc_internal.hpp"
+
+namespace sgipc {
+
+// Device-resident solver state.  Written by kernels, read by the host once
+// per chunk of iterations.
+struct PcgCtl {
+    double rz0;
+    double bb;
+    double tol;
+    double final_rr;
+    double final_rz;
+    double last_alpha;
+    int stop_mode;
+    int done;
+    int iters;
+    int not_spd;
+};
+
+// What the first kernel after the x/r update needs to evaluate the stop rule.
+struct PcgStopDesc {
+    PcgCtl* ctl = nullptr;
+    const double* rz = nullptr;  // partials of the rz used for alpha
+    int n_rz = 0;
+    const double* rr = nullptr;  // partials of ||r||^2 after the update
+    int n_rr = 0;
+};
+
+struct PcgOptions {
+    double tol = 1e-5;
+    int stop_mode = 1;     // 0 = reference |rz| <= tol*rz0, 1 = ||r|| <= tol*||b||
+    int max_iter = 1000;
+    int check_every = 8;
+};
+
+struct PcgResult {
+    int iterations = 0;
+    bool converged = false;
+    bool not_spd = false;
+    double rel_residual = 0.0;  // recursive residual at exit
+    double rz = 0.0;
+    double rz0 = 0.0;
+};
+
+// Ap = A p (Ap must be zero on entry) + per-workgroup partials of p.Ap; returns
+// the partial count.  Kernels return at once when ctl->done is set.
+template <typename TV>
+struct SpmvOp {
+    int nnz = 0;
+    virtual ~SpmvOp() = default;
+    virtual int apply_dot(hipStream_t s, const TV* p, TV* Ap, double* pap_part, const PcgCtl* ctl) const = 0;
+};
+
+// z = M r + per-workgroup partials of r.z; evaluates `stop` first when its ctl
+// is set (and then writes nothing).  Returns the partial count, which must not
+// depend on the call.
+template <typename TV>
+struct Preconditioner {
+    virtual ~Preconditioner() = default;
+    virtual int apply(hipStream_t s, const TV* r, TV* z, double* rz_part, const PcgStopDesc& stop) = 0;
+};
+
+template <typename TV>
+class PcgSolver {
+  public:
+    ~PcgSolver();
+    PcgResult solve(hipStream_t s, const SpmvOp<TV>& A, Preconditioner<TV>* M, const TV* b, TV* x,
+                    long n3, const PcgOptions& opt);
+    void reserve(long n3, int nnz);
+
+  private:
+    DevBuf<TV> x_, r_, z_, p_, Ap_;
+    DevBuf<double> part_rz_[2], part_pap_, part_rr_, part_bb_;
+    DevBuf<PcgCtl> ctl_;
+    PcgCtl* host_ctl_ = nullptr;
+    hipEvent_t ev_[2] = {nullptr, nullptr};
+};
+
+int vec_grid(long n);
+void launch_axpy_f64(hipStream_t s, double a, const double* x, double* y, long n);
+template <typename TA, typename TB>
+void launch_convert(hipStream_t s, const TA* a, TB* b, long n);
+template <typename T>
+void launch_fill(hipStream_t s, T* a, T v, long n);
+template <typename TA, typename TB>
+int launch_dot_partials(hipStream_t s, const TA* a, const TB* b, long n, double* part);
+void launch_residual(hipStream_t s, const double* vals, const int* rows, const int* cols, int nnz,
+                     int n_rows, const double* x, const double* b, double* r, double* rr_part,
+                     int* n_rr);
+
+// SpMV kernels that honour the PCG done flag.
+void launch_spmv_pcg_f64(hipStream_t s, const double* vals, const int* rows, const int* cols,
+                         int nnz, const double* p, double* Ap, double* pap_part, const PcgCtl* ctl,
+                         int* n_part);
+void launch_spmv_pcg_f32(hipStream_t s, const float* vals, const int* rows, const int* cols,
+                         int nnz, const float* p, float* Ap, double* pap_part, const PcgCtl* ctl,
+                         int* n_part);
+
+}  // namespace sgipc

@@ -1,0 +1,247 @@
+// Assembly entry points and the whole-linear-system object of the C ABI
+// (GlobalLinearSystem::solve_linear_system, StiffGIPC/linear_system/
+// linear_system/global_linear_system.cu:8-148): element Hessians into the
+// triplet arena in the reference's order (contacts, tets, mass), conversion to
+// unique upper BCOO, MAS rebuild with the current contact pairs, PCG.
+#include <chrono>
+#include <cmath>
+#include <memory>
+#include <vector>
+
+#include "capi_internal.hpp"
+
+using namespace sgipc;
+
+namespace {
+
+template <typename F>
+int run(sgipc_ctx* ctx, F&& f)
+{
+    return guarded(ctx, std::forward<F>(f));
+}
+
+Converter& converter_of(sgipc_ctx* ctx)
+{
+    static thread_local std::vector<std::pair<sgipc_ctx*, std::unique_ptr<Converter>>> table;
+    for (auto& e : table)
+        if (e.first == ctx) return *e.second;
+    table.emplace_back(ctx, std::make_unique<Converter>());
+    return *table.back().second;
+}
+
+}  // namespace
+
+struct sgipc_system {
+    sgipc_ctx* ctx = nullptr;
+    sgipc_tet_scene_desc d{};
+    int B = 32;
+    sgipc_mas* mas = nullptr;
+    DevBuf<double> tvals;
+    DevBuf<int> trows, tcols;
+    DevBuf<double> bvals;
+    DevBuf<int> brows, bcols;
+    int nnz = 0;
+    long n_trip = 0;
+    int capacity = 0;
+    Converter conv;
+    SolveWorkspace* ws = nullptr;
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+};
+
+extern "C" {
+
+int sgipc_fem_tet_gradient_hessian(sgipc_ctx* ctx, const double* x, const uint32_t* tets, const double* dm_inv,
+                                   const double* vol, const double* mu, const double* lam, int n_tets, double dt,
+                                   int hess_offset, double* grad, double* tvals, int* trows, int* tcols)
+{
+    if (!ctx || n_tets < 0) return SGIPC_BAD_ARG;
+    if (n_tets == 0) return SGIPC_EMPTY;
+    if (!x || !tets || !dm_inv || !vol || !mu || !lam || !tvals || !trows || !tcols) return SGIPC_BAD_ARG;
+    return run(ctx, [&] {
+        launch_tet_snh(ctx->c.stream, n_tets, x, tets, dm_inv, vol, mu, lam, dt, hess_offset, grad, tvals, trows,
+                       tcols);
+        SGIPC_HIP(hipStreamSynchronize(ctx->c.stream));
+    });
+}
+
+int sgipc_barrier_pt_gradient_hessian(sgipc_ctx* ctx, const double* x, const int* pairs, const double* pair_x,
+                                      int n_pairs, double dhat_sq, double kappa, int hess_offset, double* grad,
+                                      double* tvals, int* trows, int* tcols)
+{
+    if (!ctx || n_pairs < 0) return SGIPC_BAD_ARG;
+    if (n_pairs == 0) return SGIPC_EMPTY;
+    if (!pairs || (!x && !pair_x) || !tvals || !trows || !tcols || !(dhat_sq > 0)) return SGIPC_BAD_ARG;
+    return run(ctx, [&] {
+        launch_barrier_pt(ctx->c.stream, n_pairs, x, pairs, pair_x, dhat_sq, kappa, hess_offset, grad, tvals, trows,
+                          tcols);
+        SGIPC_HIP(hipStreamSynchronize(ctx->c.stream));
+    });
+}
+
+int sgipc_mass_diagonal(sgipc_ctx* ctx, const double* mass, int n, int hess_offset, double* tvals, int* trows,
+                        int* tcols)
+{
+    if (!ctx || n < 0) return SGIPC_BAD_ARG;
+    if (n == 0) return SGIPC_EMPTY;
+    if (!mass || !tvals || !trows || !tcols) return SGIPC_BAD_ARG;
+    return run(ctx, [&] {
+        launch_mass_diag(ctx->c.stream, n, mass, hess_offset, tvals, trows, tcols);
+        SGIPC_HIP(hipStreamSynchronize(ctx->c.stream));
+    });
+}
+
+int sgipc_convert_srbk(sgipc_ctx* ctx, const double* tvals, const int* trows, const int* tcols, int n_triplets,
+                       double* ovals, int* orows, int* ocols, int capacity, int* out_nnz)
+{
+    if (!ctx || n_triplets < 0 || capacity < 0 || !out_nnz) return SGIPC_BAD_ARG;
+    *out_nnz = 0;
+    if (n_triplets == 0) return SGIPC_EMPTY;
+    if (!tvals || !trows || !tcols || !ovals || !orows || !ocols) return SGIPC_BAD_ARG;
+    return run(ctx, [&] {
+        Converter& cv = converter_of(ctx);
+        int mx = cv.max_index(ctx->c.stream, trows, tcols, n_triplets);
+        *out_nnz = cv.convert(ctx->c.stream, tvals, trows, tcols, n_triplets, mx + 1, ovals, orows, ocols, capacity);
+    });
+}
+
+int sgipc_mas_create(sgipc_ctx* ctx, sgipc_mas** out, int bank_size);
+int sgipc_mas_destroy(sgipc_mas* mas);
+
+int sgipc_system_create(sgipc_ctx* ctx, sgipc_system** out, const sgipc_tet_scene_desc* desc, int bank_size)
+{
+    if (!ctx || !out || !desc) return SGIPC_BAD_ARG;
+    *out = nullptr;
+    const auto& d = *desc;
+    if (d.n_verts <= 0) return SGIPC_EMPTY;
+    if (d.n_tets < 0 || d.n_pairs < 0 || d.total_neighbor_num < 0) return SGIPC_BAD_ARG;
+    if (!d.x || !d.mass || !d.nbr_start || !d.nbr_num || (d.total_neighbor_num > 0 && !d.nbr_list)) return SGIPC_BAD_ARG;
+    if (d.n_tets > 0 && (!d.tets || !d.dm_inv || !d.vol || !d.mu || !d.lam)) return SGIPC_BAD_ARG;
+    if (d.n_pairs > 0 && (!d.pairs || !(d.dhat_sq > 0))) return SGIPC_BAD_ARG;
+    if (bank_size != 16 && bank_size != 32) return SGIPC_BAD_ARG;
+    auto* sys = new (std::nothrow) sgipc_system();
+    if (!sys) return SGIPC_HIP_ERROR;
+    sys->ctx = ctx;
+    sys->d = d;
+    sys->B = bank_size;
+    int st = sgipc_mas_create(ctx, &sys->mas, bank_size);
+    if (st == SGIPC_OK)
+        st = run(ctx, [&] {
+            sys->mas->m->init_neighbor(d.n_verts, 0, d.total_neighbor_num, d.nbr_start, d.nbr_num, d.nbr_list);
+            sys->mas->m->init_matrix();
+            sys->n_trip = 16L * d.n_pairs + 10L * d.n_tets + d.n_verts;
+            SGIPC_CHECK(sys->n_trip < (1L << 31), SGIPC_CAPACITY, "triplet count exceeds 2^31");
+            sys->tvals.reserve((size_t)sys->n_trip * 9);
+            sys->trows.reserve(sys->n_trip);
+            sys->tcols.reserve(sys->n_trip);
+            sys->capacity = (int)sys->n_trip;
+            sys->bvals.reserve((size_t)sys->capacity * 9);
+            sys->brows.reserve(sys->capacity);
+            sys->bcols.reserve(sys->capacity);
+            sys->ws = new SolveWorkspace();
+            for (auto& e : sys->ev) SGIPC_HIP(hipEventCreate(&e));
+        });
+    if (st != SGIPC_OK) {
+        if (sys->mas) sgipc_mas_destroy(sys->mas);
+        delete sys->ws;
+        delete sys;
+        return st;
+    }
+    *out = sys;
+    return SGIPC_OK;
+}
+
+int sgipc_system_destroy(sgipc_system* sys)
+{
+    if (!sys) return SGIPC_BAD_ARG;
+    int st = run(sys->ctx, [&] {
+        for (auto& e : sys->ev)
+            if (e) SGIPC_HIP(hipEventDestroy(e));
+        delete sys->ws;
+        sys->ws = nullptr;
+        sys->tvals.release();
+        sys->trows.release();
+        sys->tcols.release();
+        sys->bvals.release();
+        sys->brows.release();
+        sys->bcols.release();
+    });
+    if (sys->mas) sgipc_mas_destroy(sys->mas);
+    delete sys;
+    return st;
+}
+
+int sgipc_system_solve(sgipc_system* sys, const double* b, double* x, double* grad, const sgipc_pcg_opts* opts,
+                       sgipc_pcg_stats* stats, sgipc_solve_timing* timing)
+{
+    if (!sys || !b || !x) return SGIPC_BAD_ARG;
+    sgipc_pcg_opts o;
+    sgipc_pcg_default_opts(&o);
+    if (opts) o = *opts;
+    if (o.precision < SGIPC_FP64 || o.precision > SGIPC_FP32_IR) return SGIPC_BAD_ARG;
+    int status = SGIPC_OK;
+    int st = run(sys->ctx, [&] {
+        const auto& d = sys->d;
+        hipStream_t s = sys->ctx->c.stream;
+        auto t0 = std::chrono::steady_clock::now();
+        SGIPC_HIP(hipEventRecord(sys->ev[0], s));
+        // ---- assembly (arena order: contacts, tets, mass)
+        if (grad) launch_fill<double>(s, grad, 0.0, 3L * d.n_verts);
+        long off = 0;
+        launch_barrier_pt(s, d.n_pairs, d.x, d.pairs, d.pair_x, d.dhat_sq, d.kappa, 0, grad, sys->tvals.get(),
+                          sys->trows.get(), sys->tcols.get());
+        off += 16L * d.n_pairs;
+        launch_tet_snh(s, d.n_tets, d.x, d.tets, d.dm_inv, d.vol, d.mu, d.lam, d.dt, 0, grad,
+                       sys->tvals.get() + 9 * off, sys->trows.get() + off, sys->tcols.get() + off);
+        off += 10L * d.n_tets;
+        launch_mass_diag(s, d.n_verts, d.mass, 0, sys->tvals.get() + 9 * off, sys->trows.get() + off,
+                         sys->tcols.get() + off);
+        off += d.n_verts;
+        SGIPC_HIP(hipEventRecord(sys->ev[1], s));
+        // ---- triplets -> unique upper BCOO
+        sys->nnz = sys->conv.convert(s, sys->tvals.get(), sys->trows.get(), sys->tcols.get(), off, d.n_verts,
+                                     sys->bvals.get(), sys->brows.get(), sys->bcols.get(), sys->capacity);
+        SGIPC_HIP(hipEventRecord(sys->ev[2], s));
+        // ---- MAS rebuild with the current contacts
+        const int mprec = (o.precision == SGIPC_FP64) ? SGIPC_FP64 : SGIPC_FP32_MAS;
+        sys->mas->m->set_bcoo(sys->bvals.get(), sys->brows.get(), sys->bcols.get(), sys->nnz, 0, d.pairs, d.n_pairs,
+                              mprec);
+        SGIPC_HIP(hipEventRecord(sys->ev[3], s));
+        // ---- PCG
+        sgipc_pcg_stats pst{};
+        status = pcg_solve_impl(sys->ctx->c, *sys->ws, sys->bvals.get(), sys->brows.get(), sys->bcols.get(), sys->nnz,
+                                d.n_verts, sys->mas->m.get(), b, x, o, &pst);
+        SGIPC_HIP(hipEventRecord(sys->ev[4], s));
+        SGIPC_HIP(hipEventSynchronize(sys->ev[4]));
+        auto t1 = std::chrono::steady_clock::now();
+        if (stats) {
+            *stats = pst;
+            stats->ms_total = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        }
+        if (timing) {
+            float ms[4];
+            for (int i = 0; i < 4; ++i) SGIPC_HIP(hipEventElapsedTime(&ms[i], sys->ev[i], sys->ev[i + 1]));
+            timing->ms_assemble = ms[0];
+            timing->ms_convert = ms[1];
+            timing->ms_mas_build = ms[2];
+            timing->ms_pcg = ms[3];
+            timing->ms_total = std::chrono::duration<double, std::milli>(t1 - t0).count();
+            timing->nnz = sys->nnz;
+            timing->n_triplets = (int)off;
+        }
+    });
+    return st != SGIPC_OK ? st : status;
+}
+
+int sgipc_system_bcoo(sgipc_system* sys, const double** vals, const int** rows, const int** cols, int* nnz)
+{
+    if (!sys || !vals || !rows || !cols || !nnz) return SGIPC_BAD_ARG;
+    *vals = sys->bvals.get();
+    *rows = sys->brows.get();
+    *cols = sys->bcols.get();
+    *nnz = sys->nnz;
+    return SGIPC_OK;
+}
+
+sgipc_mas* sgipc_system_mas(sgipc_system* sys) { return sys ? sys->mas : nullptr; }
+
+}  // extern "C"

@@ -1,0 +1,272 @@
+"""Python host mirror of the reference's linear-solve call surface.
+
+Class and method names follow StiffGIPC's C++ interface so that callers and
+tests read like the reference's own code:
+
+* ``Spmv.srbk_spmv``            -- StiffGIPC/linear_system/utils/spmv.h:19-26
+* ``Converter.srbk_convert``    -- StiffGIPC/linear_system/utils/converter.h:40-43
+* ``MASPreconditioner``         -- StiffGIPC/MASPreconditioner.cuh:59-107
+* ``PCGSolver.solve``           -- StiffGIPC/linear_system/solver/pcg_solver.h:18-45
+* ``GlobalLinearSystem.solve_linear_system``
+                                -- StiffGIPC/linear_system/linear_system/global_linear_system.h:66
+
+Every call goes through the C ABI into the gfx950 library; arrays are device
+arrays (:class:`DevArray`).  Errors raise :class:`SgipcError` carrying the
+ABI status (the reference aborts instead).
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+
+import numpy as np
+
+from . import _lib
+from ._lib import (FP32_IR, FP32_MAS, FP64, PRECISIONS, STOP_REF_RZ, STOP_TRUE_RESIDUAL, Context, DevArray,
+                   PcgOpts, PcgStats, SgipcError, SolveTiming, TetSceneDesc, check, lib)
+
+__all__ = ["Context", "DevArray", "Spmv", "Converter", "MASPreconditioner", "PCGSolver", "GlobalLinearSystem",
+           "LinearSystem", "SolveResult", "SgipcError", "FP64", "FP32_MAS", "FP32_IR", "STOP_REF_RZ",
+           "STOP_TRUE_RESIDUAL", "pcg_opts", "upload_scene"]
+
+
+def _p(a):
+    return None if a is None else ctypes.c_void_p(a.ptr)
+
+
+class Spmv:
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+
+    def srbk_spmv(self, a, vals, rows, cols, nnz, x, b, y, n_rows):
+        """y = a*A*x + b*y, A symmetric stored as unique upper BCOO."""
+        if vals.dtype == np.float32:
+            return check(lib().sgipc_spmv_srbk_f32(self.ctx.h, a, _p(vals), _p(rows), _p(cols), nnz, _p(x), b, _p(y),
+                                                   n_rows), self.ctx.h)
+        return check(lib().sgipc_spmv_srbk(self.ctx.h, a, _p(vals), _p(rows), _p(cols), nnz, _p(x), b, _p(y), n_rows),
+                     self.ctx.h)
+
+
+class Converter:
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+
+    def srbk_convert(self, tvals, trows, tcols, n_triplets, capacity=None):
+        """Triplets -> unique upper BCOO; returns (vals, rows, cols, nnz) device arrays."""
+        cap = n_triplets if capacity is None else capacity
+        ov = self.ctx.empty((max(cap, 1), 9), np.float64)
+        orr = self.ctx.empty(max(cap, 1), np.int32)
+        oc = self.ctx.empty(max(cap, 1), np.int32)
+        nnz = ctypes.c_int()
+        check(lib().sgipc_convert_srbk(self.ctx.h, _p(tvals), _p(trows), _p(tcols), n_triplets, _p(ov), _p(orr),
+                                       _p(oc), cap, ctypes.byref(nnz)), self.ctx.h)
+        return ov, orr, oc, nnz.value
+
+
+class MASPreconditioner:
+    """Multilevel additive Schwarz preconditioner on the device."""
+
+    def __init__(self, ctx: Context, bank_size=32):
+        self.ctx = ctx
+        self.bank_size = bank_size
+        h = ctypes.c_void_p()
+        check(lib().sgipc_mas_create(ctx.h, ctypes.byref(h), bank_size), ctx.h)
+        self.h = h
+        self.vert_num = 0
+
+    def initPreconditioner_Neighbor(self, vertNum, collision_node_offset, totalNeighborNum, nbr_start, nbr_num,
+                                    nbr_list):
+        self.vert_num = vertNum
+        return check(lib().sgipc_mas_init_neighbor(self.h, vertNum, collision_node_offset, totalNeighborNum,
+                                                   _p(nbr_start), _p(nbr_num), _p(nbr_list)), self.ctx.h)
+
+    def initPreconditioner_Matrix(self):
+        return check(lib().sgipc_mas_init_matrix(self.h), self.ctx.h)
+
+    def setPreconditioner_bcoo(self, vals, rows, cols, nnz, offset=0, pairs=None, n_pairs=0, precision=FP64):
+        return check(lib().sgipc_mas_set_preconditioner_bcoo(self.h, _p(vals), _p(rows), _p(cols), nnz, offset,
+                                                             _p(pairs), n_pairs, precision), self.ctx.h)
+
+    def preconditioning(self, R, Z):
+        return check(lib().sgipc_mas_preconditioning(self.h, _p(R), _p(Z)), self.ctx.h)
+
+    def levels(self):
+        L = ctypes.c_int()
+        off = (ctypes.c_int * 8)()
+        cnt = (ctypes.c_int * 8)()
+        check(lib().sgipc_mas_levels(self.h, ctypes.byref(L), off, cnt), self.ctx.h)
+        n = L.value
+        return n, np.array(off[:n + 1], np.int32), np.array(cnt[:n], np.int32), int(off[n])
+
+    def tables(self):
+        L, off, cnt, total = self.levels()
+        ct = self.ctx.empty(max(self.vert_num * (L - 1), 1), np.int32)
+        gn = self.ctx.empty(max(total, 1), np.int32)
+        check(lib().sgipc_mas_tables(self.h, _p(ct), _p(gn)), self.ctx.h)
+        ctab = ct.download()[: self.vert_num * (L - 1)].reshape(self.vert_num, L - 1)
+        gnext = gn.download()[:total]
+        ct.free()
+        gn.free()
+        return ctab, gnext
+
+    def inverse_blocks(self):
+        L, off, cnt, total = self.levels()
+        B = self.bank_size
+        nb = B * (B + 1) // 2
+        out = self.ctx.empty((total // B, nb, 9), np.float64)
+        check(lib().sgipc_mas_inverse_blocks(self.h, _p(out)), self.ctx.h)
+        a = out.download()
+        out.free()
+        return a
+
+    def close(self):
+        if self.h and self.ctx.h:
+            lib().sgipc_mas_destroy(self.h)
+        self.h = None
+
+
+def pcg_opts(tol=1e-5, stop_mode=STOP_TRUE_RESIDUAL, max_iter=0, precision=FP64, check_every=8, ir_max_outer=20,
+             ir_inner_tol=1e-3):
+    o = PcgOpts()
+    lib().sgipc_pcg_default_opts(ctypes.byref(o))
+    o.tol = tol
+    o.stop_mode = stop_mode
+    o.max_iter = max_iter
+    o.precision = PRECISIONS[precision] if isinstance(precision, str) else precision
+    o.check_every = check_every
+    o.ir_max_outer = ir_max_outer
+    o.ir_inner_tol = ir_inner_tol
+    return o
+
+
+class PCGSolver:
+    def __init__(self, ctx: Context, **opts):
+        self.ctx = ctx
+        self.opts = pcg_opts(**opts)
+
+    def solve(self, vals, rows, cols, nnz, n_rows, mas, b, x):
+        """x = A^-1 b (device arrays); returns the PcgStats of the solve."""
+        st = PcgStats()
+        status = lib().sgipc_pcg_solve(self.ctx.h, _p(vals), _p(rows), _p(cols), nnz, n_rows,
+                                       mas.h if mas is not None else None, _p(b), _p(x), ctypes.byref(self.opts),
+                                       ctypes.byref(st))
+        check(status, self.ctx.h, allow=(_lib.SGIPC_MAXITER,))
+        return st
+
+
+@dataclasses.dataclass
+class SolveResult:
+    x: np.ndarray
+    iterations: int
+    outer_iterations: int
+    true_rel_residual: float
+    status: int
+    timing: dict
+
+
+def upload_scene(ctx: Context, scene):
+    """Copy a TetScene's solver inputs to the device; returns (desc, keepalive)."""
+    keep = {}
+
+    def put(name, a, dtype):
+        if a is None:
+            keep[name] = None
+            return None
+        d = ctx.to_device(np.ascontiguousarray(a, dtype=dtype))
+        keep[name] = d
+        return d.ptr
+
+    desc = TetSceneDesc()
+    desc.n_verts = scene.n
+    desc.n_tets = scene.n_tets
+    desc.n_pairs = scene.n_pairs
+    desc.total_neighbor_num = int(scene.nbr_list.size)
+    desc.dt = scene.dt
+    desc.dhat_sq = scene.dhat ** 2
+    desc.kappa = scene.kappa
+    desc.x = put("x", scene.x.reshape(-1), np.float64)
+    desc.tets = put("tets", scene.tets.reshape(-1), np.uint32)
+    desc.dm_inv = put("dm_inv", scene.dm_inv.reshape(-1), np.float64)
+    desc.vol = put("vol", scene.vol, np.float64)
+    desc.mu = put("mu", scene.mu, np.float64)
+    desc.lam = put("lam", scene.lam, np.float64)
+    desc.mass = put("mass", scene.mass, np.float64)
+    desc.pairs = put("pairs", scene.pairs.reshape(-1) if scene.n_pairs else None, np.int32)
+    desc.pair_x = put("pair_x", scene.pair_x.reshape(-1) if scene.pair_x is not None and scene.n_pairs else None,
+                      np.float64)
+    desc.nbr_start = put("nbr_start", scene.nbr_start, np.uint32)
+    desc.nbr_num = put("nbr_num", scene.nbr_num, np.uint32)
+    desc.nbr_list = put("nbr_list", scene.nbr_list if scene.nbr_list.size else np.zeros(1, np.uint32), np.uint32)
+    return desc, keep
+
+
+class GlobalLinearSystem:
+    """Whole Newton linear solve on the device: assembly (contacts, tets,
+    mass), conversion to unique upper BCOO, MAS rebuild, PCG."""
+
+    def __init__(self, ctx: Context, scene, bank_size=32):
+        self.ctx = ctx
+        self.scene = scene
+        self.desc, self._keep = upload_scene(ctx, scene)
+        h = ctypes.c_void_p()
+        check(lib().sgipc_system_create(ctx.h, ctypes.byref(h), ctypes.byref(self.desc), bank_size), ctx.h)
+        self.h = h
+        self.b = ctx.to_device(scene.rhs, np.float64)
+        self.x = ctx.empty(3 * scene.n, np.float64)
+        self.grad = ctx.empty(3 * scene.n, np.float64)
+
+    def solve_linear_system(self, tol=1e-5, precision="fp64", stop_mode=STOP_TRUE_RESIDUAL, want_grad=False,
+                            **kw):
+        o = pcg_opts(tol=tol, precision=precision, stop_mode=stop_mode, **kw)
+        st = PcgStats()
+        tm = SolveTiming()
+        status = lib().sgipc_system_solve(self.h, self.b.vp, self.x.vp, self.grad.vp if want_grad else None,
+                                          ctypes.byref(o), ctypes.byref(st), ctypes.byref(tm))
+        check(status, self.ctx.h, allow=(_lib.SGIPC_MAXITER,))
+        return st, tm
+
+    def bcoo(self):
+        """Host copies of the current unique upper BCOO (vals (nnz,9), rows, cols)."""
+        v, r, c, n = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_int()
+        check(lib().sgipc_system_bcoo(self.h, ctypes.byref(v), ctypes.byref(r), ctypes.byref(c), ctypes.byref(n)),
+              self.ctx.h)
+        nnz = n.value
+        vals = np.empty((nnz, 9))
+        rows = np.empty(nnz, np.int32)
+        cols = np.empty(nnz, np.int32)
+        for host, dev in ((vals, v), (rows, r), (cols, c)):
+            if nnz:
+                check(lib().sgipc_memcpy_dtoh(self.ctx.h, host.ctypes.data, dev, host.nbytes), self.ctx.h)
+        return vals, rows, cols
+
+    def mas_handle(self):
+        return lib().sgipc_system_mas(self.h)
+
+    def close(self):
+        if self.h and self.ctx.h:
+            lib().sgipc_system_destroy(self.h)
+        self.h = None
+
+
+class LinearSystem:
+    """Convenience owner of a Context + GlobalLinearSystem for one scene."""
+
+    def __init__(self, scene, bank_size=32, device=0):
+        self.ctx = Context(device)
+        self.sys = GlobalLinearSystem(self.ctx, scene, bank_size)
+        self.scene = scene
+
+    def solve(self, tol=1e-5, precision="fp64", stop_mode=STOP_TRUE_RESIDUAL, want_grad=False, **kw):
+        st, tm = self.sys.solve_linear_system(tol=tol, precision=precision, stop_mode=stop_mode, want_grad=want_grad,
+                                              **kw)
+        x = self.sys.x.download()
+        timing = {k: getattr(tm, k) for k, _ in SolveTiming._fields_}
+        return SolveResult(x=x, iterations=st.iterations, outer_iterations=st.outer_iterations,
+                           true_rel_residual=st.rel_residual, status=st.status, timing=timing)
+
+    def gradient(self):
+        return self.sys.grad.download()
+
+    def close(self):
+        self.sys.close()
+        self.ctx.close()

@@ -1,0 +1,271 @@
+"""Seeded synthetic scenes for the Newton linear-solve path.
+
+Each generator returns a :class:`TetScene` holding exactly the per-element and
+per-vertex inputs the reference's assembly kernels read, computed the way the
+reference's headless bootstrap computes them:
+
+* masses, rest ``DmInverse`` (row-major 3x3) and rest volume per tet, and the
+  per-tet Lame pair ``mu' = 4 mu / 3``, ``lambda' = lambda + 5 mu / 6``
+  (``app/common/sim_bootstrap.cpp:22-88``, ``init_fem``);
+* ``Ds`` columns ``x1-x0, x2-x0, x3-x0`` (``StiffGIPC/femEnergy.cu:324-350``);
+* the symmetric vertex-neighbour CSR built from the unique tet edges
+  (``StiffGIPC/load_mesh.cpp:934-1010``, ``getVertNeighbors``), which is what the
+  MAS hierarchy reads;
+* point-triangle contact pairs in the reference's ``int4`` encoding
+  ``(-v-1, t0, t1, t2)`` (``StiffGIPC/mlbvh.cu:539-760``).
+
+Grids use the Kuhn (Freudenthal) 6-tet split of each cube, which is conforming,
+and vertices are relabelled in Morton (Z-curve) order so that MAS domains of
+consecutive vertices are compact sub-boxes.  Rest positions are the regular
+grid; the current positions are the rest positions jittered by a seeded uniform
+offset, so the deformation gradients differ from the identity and the PSD
+projection of the elastic Hessian is exercised.
+
+The system matrix follows the reference scaling ``H = M + dt^2 (K + B)``
+(``StiffGIPC/GIPC.cu:10466-10485``, ``StiffGIPC/femEnergy.cu:1820``): it differs
+from ``M/dt^2 + K`` by the constant factor ``dt^2`` only, so relative residuals
+and relative solution differences are unchanged.
+"""
+from __future__ import annotations
+
+import dataclasses
+import numpy as np
+
+__all__ = [
+    "TetScene",
+    "lame_from_young",
+    "morton3",
+    "kuhn_grid",
+    "tet_grid_scene",
+    "scene_c0",
+    "scene_c1",
+    "scene_tiny",
+    "seed_pt_pairs",
+]
+
+
+@dataclasses.dataclass
+class TetScene:
+    """Inputs of one Newton linear solve on a tet mesh (+ optional contacts)."""
+
+    name: str
+    x: np.ndarray            # (n, 3) float64 current positions
+    rest: np.ndarray         # (n, 3) float64 rest positions
+    tets: np.ndarray         # (m, 4) int32, positively oriented at rest
+    dm_inv: np.ndarray       # (m, 3, 3) float64 row-major rest Dm^{-1}
+    vol: np.ndarray          # (m,) float64 rest volume
+    mu: np.ndarray           # (m,) float64 lengthRate  = 4 mu / 3
+    lam: np.ndarray          # (m,) float64 volumeRate  = lambda + 5 mu / 6
+    mass: np.ndarray         # (n,) float64 lumped mass
+    dt: float
+    nbr_start: np.ndarray    # (n,) uint32 CSR start of the vertex-neighbour list
+    nbr_num: np.ndarray      # (n,) uint32 neighbour count
+    nbr_list: np.ndarray     # (nnz,) uint32 neighbour ids
+    pairs: np.ndarray        # (p, 4) int32 contact pairs, reference int4 encoding
+    pair_x: np.ndarray | None  # (p, 4, 3) float64 local contact geometry or None
+    dhat: float              # barrier activation distance (m); reference dHat = dhat^2
+    kappa: float
+    rhs: np.ndarray          # (3n,) float64 right-hand side
+    seeds: dict
+
+    @property
+    def n(self) -> int:
+        return int(self.x.shape[0])
+
+    @property
+    def n_tets(self) -> int:
+        return int(self.tets.shape[0])
+
+    @property
+    def n_pairs(self) -> int:
+        return int(self.pairs.shape[0])
+
+    @property
+    def n_triplets(self) -> int:
+        """Triplet count written by the assembly kernels (16/pair, 10/tet, 1/vertex)."""
+        return 16 * self.n_pairs + 10 * self.n_tets + self.n
+
+
+def lame_from_young(E, nu):
+    """Return the reference's (lengthRate, volumeRate) pair for Young's modulus E.
+
+    ``app/common/sim_bootstrap.cpp:27-31,57-62``: mu = E / (2(1+nu)),
+    lambda = E nu / ((1+nu)(1-2nu)); lengthRate = 4 mu / 3 and
+    volumeRate = lambda + 5 mu / 6 (the stable Neo-Hookean re-parameterisation).
+    """
+    E = np.asarray(E, dtype=np.float64)
+    mu = E / (2.0 * (1.0 + nu))
+    lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+    return 4.0 * mu / 3.0, lam + 5.0 * mu / 6.0
+
+
+def _part1by2(v: np.ndarray) -> np.ndarray:
+    v = v.astype(np.uint64) & np.uint64(0x1FFFFF)
+    v = (v | (v << np.uint64(32))) & np.uint64(0x1F00000000FFFF)
+    v = (v | (v << np.uint64(16))) & np.uint64(0x1F0000FF0000FF)
+    v = (v | (v << np.uint64(8))) & np.uint64(0x100F00F00F00F00F)
+    v = (v | (v << np.uint64(4))) & np.uint64(0x10C30C30C30C30C3)
+    v = (v | (v << np.uint64(2))) & np.uint64(0x1249249249249249)
+    return v
+
+
+def morton3(i, j, k) -> np.ndarray:
+    """64-bit Morton code with x in the lowest bit of each triple."""
+    return _part1by2(i) | (_part1by2(j) << np.uint64(1)) | (_part1by2(k) << np.uint64(2))
+
+
+# Kuhn split: for each axis permutation (a, b, c) the tet 000 -> e_a -> e_a+e_b -> 111.
+_KUHN_PERMS = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
+_PERM_SIGN = {(0, 1, 2): 1, (1, 2, 0): 1, (2, 0, 1): 1, (0, 2, 1): -1, (1, 0, 2): -1, (2, 1, 0): -1}
+
+
+def kuhn_grid(nx: int, ny: int, nz: int):
+    """Vertex lattice coordinates (n,3) int and positively oriented tets (m,4) int64.
+
+    Vertices are numbered lexicographically (x fastest); relabel afterwards.
+    """
+    ii, jj, kk = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    lat = np.stack([ii.ravel(order="F"), jj.ravel(order="F"), kk.ravel(order="F")], axis=1)
+
+    def vid(i, j, k):
+        return i + nx * (j + ny * k)
+
+    ci, cj, ck = np.meshgrid(np.arange(nx - 1), np.arange(ny - 1), np.arange(nz - 1), indexing="ij")
+    ci, cj, ck = ci.ravel(order="F"), cj.ravel(order="F"), ck.ravel(order="F")
+    tets = []
+    for perm in _KUHN_PERMS:
+        a, b, _ = perm
+        o = np.zeros(3, dtype=np.int64)
+        p1 = o.copy(); p1[a] = 1
+        p2 = p1.copy(); p2[b] = 1
+        p3 = np.ones(3, dtype=np.int64)
+        corners = [o, p1, p2, p3]
+        ids = [vid(ci + c[0], cj + c[1], ck + c[2]) for c in corners]
+        if _PERM_SIGN[perm] < 0:
+            ids[1], ids[2] = ids[2], ids[1]
+        tets.append(np.stack(ids, axis=1))
+    tets = np.stack(tets, axis=1).reshape(-1, 4)  # 6 consecutive tets per cube
+    return lat, tets
+
+
+def _edges_csr(n: int, tets: np.ndarray):
+    """Symmetric vertex-neighbour CSR from the unique tet edges."""
+    e = np.concatenate([tets[:, [a, b]] for a, b in ((0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3))])
+    e = np.sort(e, axis=1)
+    key = np.unique(e[:, 0].astype(np.int64) * n + e[:, 1])
+    u, v = key // n, key % n
+    src = np.concatenate([u, v])
+    dst = np.concatenate([v, u])
+    order = np.lexsort((dst, src))
+    src, dst = src[order], dst[order]
+    num = np.bincount(src, minlength=n).astype(np.uint32)
+    start = np.zeros(n, dtype=np.uint32)
+    start[1:] = np.cumsum(num)[:-1]
+    return start, num, dst.astype(np.uint32), int(key.size)
+
+
+def _surface_triangles(tets: np.ndarray) -> np.ndarray:
+    """Boundary faces of a tet mesh (faces referenced by exactly one tet)."""
+    faces = np.concatenate([tets[:, [1, 2, 3]], tets[:, [0, 3, 2]], tets[:, [0, 1, 3]], tets[:, [0, 2, 1]]])
+    s = np.sort(faces, axis=1)
+    _, idx, cnt = np.unique(s, axis=0, return_index=True, return_counts=True)
+    return faces[idx[cnt == 1]]
+
+
+def seed_pt_pairs(x: np.ndarray, tets: np.ndarray, n_pairs: int, dhat: float, seed: int):
+    """Seeded point-triangle pairs with prescribed squared distance d^2 ~ U(0, dhat^2).
+
+    Vertex p is a random surface vertex and (t0,t1,t2) a random surface triangle
+    not incident to p.  The barrier Hessian depends on the relative geometry
+    only, so each pair carries its own local geometry: the triangle at its
+    current positions and the point placed at distance d along the triangle
+    normal above a random interior barycentric point (so the point-triangle
+    distance is the point-plane distance, as for a reference PT pair).
+    Returns (pairs (p,4) int32 in the reference encoding, pair_x (p,4,3)).
+    """
+    rng = np.random.default_rng(seed)
+    tris = _surface_triangles(tets)
+    surf_v = np.unique(tris)
+    pv = surf_v[rng.integers(0, surf_v.size, size=n_pairs)]
+    ti = rng.integers(0, tris.shape[0], size=n_pairs)
+    t = tris[ti]
+    bad = (t == pv[:, None]).any(axis=1)
+    while bad.any():
+        ti[bad] = rng.integers(0, tris.shape[0], size=int(bad.sum()))
+        t = tris[ti]
+        bad = (t == pv[:, None]).any(axis=1)
+    d2 = rng.uniform(0.0, 1.0, size=n_pairs) * dhat * dhat
+    d2 = np.maximum(d2, 1e-12 * dhat * dhat)
+    bary = rng.uniform(0.1, 1.0, size=(n_pairs, 3))
+    bary /= bary.sum(axis=1, keepdims=True)
+    side = np.where(rng.uniform(size=n_pairs) < 0.5, -1.0, 1.0)
+    t0, t1, t2 = x[t[:, 0]], x[t[:, 1]], x[t[:, 2]]
+    nrm = np.cross(t1 - t0, t2 - t0)
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    foot = bary[:, 0:1] * t0 + bary[:, 1:2] * t1 + bary[:, 2:3] * t2
+    p = foot + (side * np.sqrt(d2))[:, None] * nrm
+    pairs = np.stack([-pv - 1, t[:, 0], t[:, 1], t[:, 2]], axis=1).astype(np.int32)
+    pair_x = np.stack([p, t0, t1, t2], axis=1)
+    return pairs, pair_x
+
+
+def tet_grid_scene(name: str, N: int, *, spacing=0.01, jitter=0.1, E=1e5, E_loguniform=None,
+                   nu=0.45, density=1e3, dt=0.01, n_pairs=0, dhat=1e-3, kappa=1e6,
+                   seed_jitter=0, seed_rhs=1, seed_E=2, seed_pairs=3) -> TetScene:
+    """Regular N^3 vertex grid, Kuhn split, Morton order, seeded jitter/E/pairs/rhs."""
+    lat, tets = kuhn_grid(N, N, N)
+    code = morton3(lat[:, 0], lat[:, 1], lat[:, 2])
+    order = np.argsort(code, kind="stable")          # new id -> old id
+    relabel = np.empty_like(order)
+    relabel[order] = np.arange(order.size)            # old id -> new id
+    lat = lat[order]
+    tets = relabel[tets]
+    n = lat.shape[0]
+    rest = lat.astype(np.float64) * spacing
+    rng = np.random.default_rng(seed_jitter)
+    x = rest + rng.uniform(-jitter * spacing, jitter * spacing, size=rest.shape)
+
+    Dm = np.stack([rest[tets[:, 1]] - rest[tets[:, 0]],
+                   rest[tets[:, 2]] - rest[tets[:, 0]],
+                   rest[tets[:, 3]] - rest[tets[:, 0]]], axis=2)   # columns
+    detm = np.linalg.det(Dm)
+    assert (detm > 0).all()
+    dm_inv = np.linalg.inv(Dm)
+    vol = np.abs(detm) / 6.0
+    mass = np.zeros(n)
+    np.add.at(mass, tets.ravel(), np.repeat(vol * density / 4.0, 4))
+
+    m = tets.shape[0]
+    if E_loguniform is not None:
+        lo, hi = E_loguniform
+        Es = 10.0 ** np.random.default_rng(seed_E).uniform(np.log10(lo), np.log10(hi), size=m)
+    else:
+        Es = np.full(m, float(E))
+    mu, lam = lame_from_young(Es, nu)
+
+    start, num, lst, _ = _edges_csr(n, tets)
+    if n_pairs:
+        pairs, pair_x = seed_pt_pairs(x, tets, n_pairs, dhat, seed_pairs)
+    else:
+        pairs, pair_x = np.zeros((0, 4), dtype=np.int32), None
+    rhs = np.random.default_rng(seed_rhs).standard_normal(3 * n)
+    return TetScene(name=name, x=np.ascontiguousarray(x), rest=rest, tets=tets.astype(np.int32),
+                    dm_inv=np.ascontiguousarray(dm_inv), vol=vol, mu=np.ascontiguousarray(mu),
+                    lam=np.ascontiguousarray(lam), mass=mass, dt=dt, nbr_start=start, nbr_num=num,
+                    nbr_list=lst, pairs=pairs, pair_x=pair_x, dhat=dhat, kappa=kappa, rhs=rhs,
+                    seeds=dict(jitter=seed_jitter, rhs=seed_rhs, E=seed_E, pairs=seed_pairs))
+
+
+def scene_c0() -> TetScene:
+    """16^3 grid, E = 1e5 Pa, no contact (the CPU-runnable case)."""
+    return tet_grid_scene("c0_grid16", 16)
+
+
+def scene_c1(n_pairs: int = 50_000) -> TetScene:
+    """64^3 grid, per-tet E log-uniform in [1e4, 1e9] Pa, 50k point-triangle pairs."""
+    return tet_grid_scene("c1_grid64_contact", 64, E_loguniform=(1e4, 1e9), n_pairs=n_pairs)
+
+
+def scene_tiny(N: int = 6, n_pairs: int = 12, E_loguniform=(1e4, 1e9)) -> TetScene:
+    """Small grid with contacts and a wide stiffness range, for dense checks."""
+    return tet_grid_scene(f"tiny_grid{N}", N, E_loguniform=E_loguniform, n_pairs=n_pairs)
