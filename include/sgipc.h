@@ -121,6 +121,15 @@ int sgipc_spmv_srbk_f32(sgipc_ctx* ctx, float a, const float* vals, const int* r
 /* Round FP64 blocks/vectors to FP32 (n values). */
 int sgipc_convert_f64_to_f32(sgipc_ctx* ctx, const double* in, float* out, size_t n);
 
+/* Measurement helper: expands the upper BCOO into both triangles (as the PCG
+ * solver does) and returns the average duration (ms) of the PCG SpMV kernel
+ * (row-owned, emits the p.Ap partials) over `reps` back-to-back launches, timed
+ * with HIP events on the context's stream.  precision: SGIPC_FP64 or
+ * SGIPC_FP32_IR (FP32 values).  *full_nnz (optional) receives the expanded
+ * block count. */
+int sgipc_spmv_time(sgipc_ctx* ctx, int precision, const double* vals, const int* rows,
+                    const int* cols, int nnz, int n_rows, int reps, double* avg_ms, int* full_nnz);
+
 /* ---- MAS preconditioner (MASPreconditioner, StiffGIPC/MASPreconditioner.cuh:59-107) */
 /* bank_size is the domain size B (16 = the reference's BANKSIZE, or 32). */
 int sgipc_mas_create(sgipc_ctx* ctx, sgipc_mas** out, int bank_size);
@@ -149,6 +158,9 @@ int sgipc_mas_levels(sgipc_mas* mas, int* level_num, int* offsets, int* counts);
 /* Hierarchy tables (device outputs): coarse_table[vert_num*(level_num-1)]
  * (global slot ids, vertex-major) and going_next[total slots]. */
 int sgipc_mas_tables(sgipc_mas* mas, int* coarse_table, int* going_next);
+/* Measurement helper: average duration (ms) of one preconditioning (all of its
+ * kernels) over `reps` applications, HIP events on the context's stream. */
+int sgipc_mas_time_apply(sgipc_mas* mas, const double* r, double* z, int reps, double* avg_ms);
 /* Inverse domain matrices as FP64 upper blocks (device output,
  * total_slots/B * B(B+1)/2 * 9 doubles). */
 int sgipc_mas_inverse_blocks(sgipc_mas* mas, double* out);

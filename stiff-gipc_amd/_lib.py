@@ -70,6 +70,8 @@ _SIGS = {
     "sgipc_spmv_srbk": (ci, [vp, cd, vp, vp, vp, ci, vp, cd, vp, ci]),
     "sgipc_spmv_srbk_f32": (ci, [vp, cf, vp, vp, vp, ci, vp, cf, vp, ci]),
     "sgipc_convert_f64_to_f32": (ci, [vp, vp, vp, csz]),
+    "sgipc_spmv_time": (ci, [vp, ci, vp, vp, vp, ci, ci, ci, ctypes.POINTER(cd), ctypes.POINTER(ci)]),
+    "sgipc_mas_time_apply": (ci, [vp, vp, vp, ci, ctypes.POINTER(cd)]),
     "sgipc_mas_create": (ci, [vp, ctypes.POINTER(vp), ci]),
     "sgipc_mas_destroy": (ci, [vp]),
     "sgipc_mas_init_neighbor": (ci, [vp, ci, ci, ci, vp, vp, vp]),

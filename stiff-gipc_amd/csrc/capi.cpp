@@ -13,29 +13,24 @@ namespace sgipc {
 
 thread_local std::string g_last_error;
 
-// FP64 / FP32 SpMV operators over caller-owned BCOO arrays.
-struct BcooOpF64 : SpmvOp<double> {
-    const double* vals;
-    const int* rows;
-    const int* cols;
-    int apply_dot(hipStream_t s, const double* p, double* Ap, double* part, const PcgCtl* ctl) const override
+// PCG operator over the expanded (both triangles) matrix: Ap = A p written
+// row by row (no zero-fill needed) with the p.Ap partials.
+template <typename TV>
+struct FullOp : SpmvOp<TV> {
+    const FullMatrix* m;
+    int apply_dot(hipStream_t s, const TV* p, TV* Ap, double* part, const PcgCtl* ctl) const override
     {
-        int np = 0;
-        launch_spmv_pcg_f64(s, vals, rows, cols, nnz, p, Ap, part, ctl, &np);
-        return np;
+        return m->apply<TV>(s, TV(1), p, TV(0), Ap, part, ctl ? &ctl->done : nullptr);
     }
 };
-struct BcooOpF32 : SpmvOp<float> {
-    const float* vals;
-    const int* rows;
-    const int* cols;
-    int apply_dot(hipStream_t s, const float* p, float* Ap, double* part, const PcgCtl* ctl) const override
-    {
-        int np = 0;
-        launch_spmv_pcg_f32(s, vals, rows, cols, nnz, p, Ap, part, ctl, &np);
-        return np;
-    }
-};
+
+// r = b - A x in FP64 with ||r||^2 partials; returns the partial count.
+int residual(hipStream_t s, const FullMatrix& A, const double* x, const double* b, double* r, double* part, long n3)
+{
+    SGIPC_HIP(hipMemcpyAsync(r, b, sizeof(double) * n3, hipMemcpyDeviceToDevice, s));
+    A.apply<double>(s, -1.0, x, 1.0, r, nullptr, nullptr);
+    return launch_dot_partials<double, double>(s, r, r, n3, part);
+}
 
 }  // namespace sgipc
 
@@ -185,6 +180,51 @@ int sgipc_spmv_srbk_f32(sgipc_ctx* ctx, float a, const float* vals, const int* r
     });
 }
 
+int sgipc_spmv_time(sgipc_ctx* ctx, int precision, const double* vals, const int* rows, const int* cols, int nnz,
+                    int n_rows, int reps, double* avg_ms, int* full_nnz)
+{
+    if (!ctx || !vals || !rows || !cols || !avg_ms || reps < 1 || nnz < 1 || n_rows < 1) return SGIPC_BAD_ARG;
+    if (precision != SGIPC_FP64 && precision != SGIPC_FP32_IR) return SGIPC_BAD_ARG;
+    return guarded(ctx, [&] {
+        hipStream_t s = ctx->c.stream;
+        FullMatrix A;
+        A.build(s, vals, rows, cols, nnz, n_rows, precision == SGIPC_FP32_IR);
+        if (full_nnz) *full_nnz = A.nnz;
+        const long n3 = 3L * n_rows;
+        DevBuf<double> x64, y64, part;
+        DevBuf<float> x32, y32;
+        part.reserve(4096);
+        if (precision == SGIPC_FP64) {
+            x64.reserve(n3);
+            y64.reserve(n3);
+            launch_fill<double>(s, x64.get(), 1.0, n3);
+        } else {
+            x32.reserve(n3);
+            y32.reserve(n3);
+            launch_fill<float>(s, x32.get(), 1.0f, n3);
+        }
+        auto launch = [&] {
+            if (precision == SGIPC_FP64)
+                A.apply<double>(s, 1.0, x64.get(), 0.0, y64.get(), part.get(), nullptr);
+            else
+                A.apply<float>(s, 1.0f, x32.get(), 0.0f, y32.get(), part.get(), nullptr);
+        };
+        launch();  // warm-up
+        hipEvent_t e0, e1;
+        SGIPC_HIP(hipEventCreate(&e0));
+        SGIPC_HIP(hipEventCreate(&e1));
+        SGIPC_HIP(hipEventRecord(e0, s));
+        for (int i = 0; i < reps; ++i) launch();
+        SGIPC_HIP(hipEventRecord(e1, s));
+        SGIPC_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        SGIPC_HIP(hipEventElapsedTime(&ms, e0, e1));
+        *avg_ms = ms / reps;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    });
+}
+
 int sgipc_convert_f64_to_f32(sgipc_ctx* ctx, const double* in, float* out, size_t n)
 {
     if (!ctx || (n && (!in || !out))) return SGIPC_BAD_ARG;
@@ -260,6 +300,28 @@ int sgipc_mas_preconditioning(sgipc_mas* mas, const double* r, double* z)
     });
 }
 
+int sgipc_mas_time_apply(sgipc_mas* mas, const double* r, double* z, int reps, double* avg_ms)
+{
+    if (!mas || !r || !z || !avg_ms || reps < 1) return SGIPC_BAD_ARG;
+    return guarded(mas->ctx, [&] {
+        hipStream_t s = mas->ctx->c.stream;
+        PcgStopDesc none{};
+        hipEvent_t e0, e1;
+        SGIPC_HIP(hipEventCreate(&e0));
+        SGIPC_HIP(hipEventCreate(&e1));
+        mas->m->apply<double>(s, r, z, nullptr, none);
+        SGIPC_HIP(hipEventRecord(e0, s));
+        for (int i = 0; i < reps; ++i) mas->m->apply<double>(s, r, z, nullptr, none);
+        SGIPC_HIP(hipEventRecord(e1, s));
+        SGIPC_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        SGIPC_HIP(hipEventElapsedTime(&ms, e0, e1));
+        *avg_ms = ms / reps;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    });
+}
+
 int sgipc_mas_levels(sgipc_mas* mas, int* level_num, int* offsets, int* counts)
 {
     if (!mas || !level_num || !offsets || !counts) return SGIPC_BAD_ARG;
@@ -311,12 +373,11 @@ int pcg_solve_impl(Context& c, SolveWorkspace& ws, const double* vals, const int
     sgipc_pcg_stats st{};
     int status = SGIPC_OK;
     ws.part.reserve(4096);
+    ws.full.build(s, vals, rows, cols, nnz, n_rows, o.precision == SGIPC_FP32_IR);
     if (o.precision != SGIPC_FP32_IR) {
-        BcooOpF64 A;
-        A.nnz = nnz;
-        A.vals = vals;
-        A.rows = rows;
-        A.cols = cols;
+        FullOp<double> A;
+        A.nnz = ws.full.nnz;
+        A.m = &ws.full;
         MasPrecond<double> M(mas);
         PcgOptions po;
         po.tol = o.tol;
@@ -333,17 +394,13 @@ int pcg_solve_impl(Context& c, SolveWorkspace& ws, const double* vals, const int
     } else {
         // FP64 iterative refinement around an FP32 PCG (FP32 SpMV + FP32 MAS,
         // FP64 dot products / alpha / beta inside).
-        ws.vals32.reserve((size_t)nnz * 9);
         ws.r32.reserve(n3);
         ws.d32.reserve(n3);
         ws.r64.reserve(n3);
         ws.d64.reserve(n3);
-        launch_convert<double, float>(s, vals, ws.vals32.get(), (long)nnz * 9);
-        BcooOpF32 A;
-        A.nnz = nnz;
-        A.vals = ws.vals32.get();
-        A.rows = rows;
-        A.cols = cols;
+        FullOp<float> A;
+        A.nnz = ws.full.nnz;
+        A.m = &ws.full;
         MasPrecond<float> M(mas);
         launch_fill<double>(s, x, 0.0, n3);
         SGIPC_HIP(hipMemcpyAsync(ws.r64.get(), b, sizeof(double) * n3, hipMemcpyDeviceToDevice, s));
@@ -381,8 +438,7 @@ int pcg_solve_impl(Context& c, SolveWorkspace& ws, const double* vals, const int
             launch_convert<float, double>(s, ws.d32.get(), ws.d64.get(), n3);
             // x += d ; r = b - A x (FP64)
             launch_axpy_f64(s, 1.0, ws.d64.get(), x, n3);
-            int nrr = 0;
-            launch_residual(s, vals, rows, cols, nnz, n_rows, x, b, ws.r64.get(), ws.part.get(), &nrr);
+            int nrr = residual(s, ws.full, x, b, ws.r64.get(), ws.part.get(), n3);
             rr = host_sum(ws.part.get(), nrr);
             if (rr <= o.tol * o.tol * bb) ok = true;
             if (rr > best_rr * 0.999999 && outer > 1) break;  // stagnation guard
@@ -395,9 +451,8 @@ int pcg_solve_impl(Context& c, SolveWorkspace& ws, const double* vals, const int
     }
     // true FP64 residual of the returned x
     {
-        int nrr = 0;
         ws.r64.reserve(n3);
-        launch_residual(s, vals, rows, cols, nnz, n_rows, x, b, ws.r64.get(), ws.part.get(), &nrr);
+        int nrr = residual(s, ws.full, x, b, ws.r64.get(), ws.part.get(), n3);
         std::vector<double> h(nrr);
         SGIPC_HIP(hipMemcpyAsync(h.data(), ws.part.get(), sizeof(double) * nrr, hipMemcpyDeviceToHost, s));
         int nbb = launch_dot_partials<double, double>(s, b, b, n3, ws.part.get() + 2048);

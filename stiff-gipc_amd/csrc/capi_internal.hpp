@@ -8,6 +8,7 @@
 #include "mas.hpp"
 #include "pcg.hpp"
 #include "sgipc_internal.hpp"
+#include "spmv_full.hpp"
 
 struct sgipc_ctx {
     sgipc::Context c;
@@ -47,9 +48,10 @@ int guarded(sgipc_ctx* ctx, F&& f)
 
 // Solver workspace (PCG buffers, FP32 matrix copy for refinement).
 struct SolveWorkspace {
+    FullMatrix full;  // both triangles, row-owned SpMV of the PCG loop
     PcgSolver<double> pcg64;
     PcgSolver<float> pcg32;
-    DevBuf<float> vals32, r32, d32;
+    DevBuf<float> r32, d32;
     DevBuf<double> r64, d64;
     DevBuf<double> part;
 };

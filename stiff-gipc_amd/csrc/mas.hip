@@ -29,6 +29,7 @@
 //    with coalesced 16-byte loads and computes each row in a fixed order;
 //    prolongation fuses the r.z partial for PCG.
 #include "mas.hpp"
+#include "assembly.hpp"
 #include "wave_ops.hpp"
 
 #include <chrono>
@@ -239,80 +240,73 @@ __device__ __forceinline__ void load_block(const double* __restrict__ v, double 
     for (int e = 0; e < 9; ++e) h[e] = v[e];
 }
 
-// Cross-domain blocks (row and column in different level-0 domains) climb the
-// hierarchy and are added at every level where both ends share a domain.
+// Galerkin coarsening of a full upper operator one level up:
+// (i, j, H) with i <= j at level l-1 maps to (I, J) = (next(i), next(j)) at
+// level l; H goes to (I, J), H^T to (J, I), and both halves land on the
+// diagonal when I == J (H + H^T, or H alone for a diagonal block).  The
+// mapped triplets are then merged by the deterministic converter, so
+// A_l = P_l^T A_{l-1} P_l is formed without atomics.
+__global__ __launch_bounds__(256) void k_coarsen_map(const double* __restrict__ vals, const int* __restrict__ rows,
+                                                     const int* __restrict__ cols, int nnz, int offset, int n_prev,
+                                                     const int* __restrict__ going_next, int off_prev, int off_next,
+                                                     double* __restrict__ ov, int* __restrict__ orow,
+                                                     int* __restrict__ ocol)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nnz) return;
+    const int i = rows[k] - offset, j = cols[k] - offset;
+    double* dst = ov + 9 * (long)k;
+    if (i < 0 || j < 0 || i >= n_prev || j >= n_prev) {
+        // outside the preconditioned range: contributes nothing
+#pragma unroll
+        for (int e = 0; e < 9; ++e) dst[e] = 0.0;
+        orow[k] = 0;
+        ocol[k] = 0;
+        return;
+    }
+    const int I = going_next[off_prev + i] - off_next;
+    const int J = going_next[off_prev + j] - off_next;
+    double h[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) h[e] = vals[9 * (long)k + e];
+    if (I == J) {
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) dst[c * 3 + r] = (i == j) ? h[c * 3 + r] : h[c * 3 + r] + h[r * 3 + c];
+        orow[k] = I;
+        ocol[k] = I;
+    } else if (I < J) {
+#pragma unroll
+        for (int e = 0; e < 9; ++e) dst[e] = h[e];
+        orow[k] = I;
+        ocol[k] = J;
+    } else {
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) dst[c * 3 + r] = h[r * 3 + c];
+        orow[k] = J;
+        ocol[k] = I;
+    }
+}
+
+// Blocks of a level-l operator whose ends share a level-l domain become that
+// domain's Galerkin blocks (assignment: the operator's blocks are unique).
 template <int B>
-__global__ __launch_bounds__(256) void k_galerkin_cross(const int* __restrict__ meta, int L, int n,
-                                                        const int* __restrict__ map,
-                                                        const double* __restrict__ vals,
-                                                        const int* __restrict__ rows,
-                                                        const int* __restrict__ cols, int nnz, int offset,
-                                                        double* __restrict__ mgal)
+__global__ __launch_bounds__(256) void k_extract_domains(const double* __restrict__ vals, const int* __restrict__ rows,
+                                                         const int* __restrict__ cols, int nnz, int off_l, int ndom0,
+                                                         double* __restrict__ mgal)
 {
     constexpr int nblk = B * (B + 1) / 2;
-    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int lane = threadIdx.x & 63;
-    bool valid = k < nnz;
-    int i = 0, j = 0;
-    if (valid) {
-        i = rows[k] - offset;
-        j = cols[k] - offset;
-        valid = i >= 0 && j >= 0 && i < n && j < n && (i / B != j / B);
-    }
-    double h[9];
-    if (valid) load_block(vals + 9 * k, h);
-    else
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nnz) return;
+    const int I = rows[k], J = cols[k];
+    if (I / B != J / B) return;
+    const int dom = (off_l + I) / B - ndom0;
+    double* dst = mgal + ((long)dom * nblk + blk_index(B, I % B, J % B)) * 9;
 #pragma unroll
-        for (int e = 0; e < 9; ++e) h[e] = 0.0;
-    const int ndom0 = meta[kMetaOff + 1] / B;
-    for (int l = 1; l < L; ++l) {
-        int a = 0, b = 0;
-        if (valid) {
-            a = map[(long)l * n + i];
-            b = map[(long)l * n + j];
-        }
-        const bool same = valid && (a / B == b / B);
-        double v[9];
-        int key = -1;
-        if (same) {
-            const int dom = (meta[kMetaOff + l] + a) / B - ndom0;
-            const int ra = a % B, rb = b % B;
-            if (a == b) {
-#pragma unroll
-                for (int r = 0; r < 3; ++r)
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) v[c * 3 + r] = h[c * 3 + r] + h[r * 3 + c];
-                key = dom * nblk + blk_index(B, ra, ra);
-            } else if (a < b) {
-#pragma unroll
-                for (int e = 0; e < 9; ++e) v[e] = h[e];
-                key = dom * nblk + blk_index(B, ra, rb);
-            } else {
-#pragma unroll
-                for (int r = 0; r < 3; ++r)
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) v[c * 3 + r] = h[r * 3 + c];
-                key = dom * nblk + blk_index(B, rb, ra);
-            }
-        }
-        const uint64_t act = __ballot(same);
-        if (act == 0) continue;
-        const int leader = __ffsll((unsigned long long)act) - 1;
-        const int k0 = __shfl(key, leader, 64);
-        const bool uniform = __ballot(same && key != k0) == 0;
-        if (uniform) {
-            double* dst = mgal + (long)k0 * 9;
-#pragma unroll
-            for (int e = 0; e < 9; ++e) {
-                double t = wave_sum(same ? v[e] : 0.0);
-                if (lane == leader) atomic_add(dst + e, t);
-            }
-        } else if (same) {
-            double* dst = mgal + (long)key * 9;
-#pragma unroll
-            for (int e = 0; e < 9; ++e) atomic_add(dst + e, v[e]);
-        }
-    }
+    for (int e = 0; e < 9; ++e) dst[e] = vals[9 * (long)k + e];
 }
 
 // ------------------------------------------------------------------ inverse
@@ -432,32 +426,20 @@ __device__ __forceinline__ bool find_block(const int* __restrict__ row_ptr, cons
     return false;
 }
 
-constexpr int kMaxLocalComp = 8;
-
-// Level-0 domains: gather blocks from the BCOO, push the domain's Galerkin
-// contribution to the coarse levels, invert, store.
+// Level-0 domains: gather the domain's blocks straight from the sorted BCOO
+// into register tiles, invert, store.
 template <int B, typename TO>
-__global__ __launch_bounds__(InvShape<B>::T) void k_inv_level0(
-    const int* __restrict__ meta, int L, int n, const int* __restrict__ map,
-    const int* __restrict__ going_next, const double* __restrict__ vals,
-    const int* __restrict__ cols, const int* __restrict__ row_ptr, int offset,
-    double* __restrict__ mgal, TO* __restrict__ minv)
+__global__ __launch_bounds__(InvShape<B>::T) void k_inv_level0(int n, const double* __restrict__ vals,
+                                                               const int* __restrict__ cols,
+                                                               const int* __restrict__ row_ptr, int offset,
+                                                               TO* __restrict__ minv)
 {
     constexpr int N = InvShape<B>::N, G = InvShape<B>::G;
     constexpr int nblk = B * (B + 1) / 2;
     __shared__ double colbuf[2 * N], rowbuf[2 * N];
-    __shared__ double S[kMaxLocalComp * kMaxLocalComp * 9];
-    __shared__ double red[16];
     const int d = blockIdx.x;
     const int ty = threadIdx.x / G, tx = threadIdx.x % G;
     double a[6][6];
-    int unode[2], vnode[2];  // level-1 ids of the tile's block rows / cols
-#pragma unroll
-    for (int bi = 0; bi < 2; ++bi) {
-        int u = d * B + 2 * ty + bi, v = d * B + 2 * tx + bi;
-        unode[bi] = (u < n && L > 1) ? map[n + u] : -1;
-        vnode[bi] = (v < n && L > 1) ? map[n + v] : -1;
-    }
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
@@ -487,93 +469,6 @@ __global__ __launch_bounds__(InvShape<B>::T) void k_inv_level0(
 #pragma unroll
                 for (int c = 0; c < 3; ++c) a[3 * bi + r][3 * bj + c] = h[c * 3 + r];
         }
-
-    // ---- Galerkin contribution of this domain to levels >= 1
-    if (L > 1) {
-        const int first = map[n + d * B];
-        const int next_first = ((d + 1) * B < n) ? map[n + (d + 1) * B] : meta[kMetaCnt + 1];
-        const int k = next_first - first;
-        const int ndom0 = meta[kMetaOff + 1] / B;
-        if (k == 1) {
-            double s[9];
-#pragma unroll
-            for (int r = 0; r < 3; ++r)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    double t = 0.0;
-#pragma unroll
-                    for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-                        for (int bj = 0; bj < 2; ++bj)
-                            if (unode[bi] >= 0 && vnode[bj] >= 0) t += a[3 * bi + r][3 * bj + c];
-                    s[c * 3 + r] = block_sum(t, red);
-                }
-            if (threadIdx.x == 0) {
-                int slot = meta[kMetaOff + 1] + first;
-                for (int l = 1; l < L; ++l) {
-                    const int ra = slot % B;
-                    double* dst = mgal + ((long)(slot / B - ndom0) * nblk + blk_index(B, ra, ra)) * 9;
-#pragma unroll
-                    for (int e = 0; e < 9; ++e) atomic_add(dst + e, s[e]);
-                    if (l + 1 < L) slot = going_next[slot];
-                }
-            }
-        } else if (k <= kMaxLocalComp) {
-            for (int e = threadIdx.x; e < k * k * 9; e += blockDim.x) S[e] = 0.0;
-            __syncthreads();
-#pragma unroll
-            for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-                for (int bj = 0; bj < 2; ++bj) {
-                    if (unode[bi] < 0 || vnode[bj] < 0) continue;
-                    double* dst = S + ((unode[bi] - first) * k + (vnode[bj] - first)) * 9;
-#pragma unroll
-                    for (int r = 0; r < 3; ++r)
-#pragma unroll
-                        for (int c = 0; c < 3; ++c) atomic_add(dst + c * 3 + r, a[3 * bi + r][3 * bj + c]);
-                }
-            __syncthreads();
-            if ((int)threadIdx.x < k * k) {
-                const int p = threadIdx.x / k, q = threadIdx.x % k;
-                int sp = meta[kMetaOff + 1] + first + p, sq = meta[kMetaOff + 1] + first + q;
-                const double* src = S + (p * k + q) * 9;
-                for (int l = 1; l < L; ++l) {
-                    if (sp / B == sq / B && sp <= sq) {
-                        double* dst = mgal + ((long)(sp / B - ndom0) * nblk + blk_index(B, sp % B, sq % B)) * 9;
-#pragma unroll
-                        for (int e = 0; e < 9; ++e) atomic_add(dst + e, src[e]);
-                    }
-                    if (l + 1 < L) {
-                        sp = going_next[sp];
-                        sq = going_next[sq];
-                    }
-                }
-            }
-        } else {
-            // fragmented domain: every block climbs on its own
-#pragma unroll
-            for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-                for (int bj = 0; bj < 2; ++bj) {
-                    if (unode[bi] < 0 || vnode[bj] < 0) continue;
-                    int sp = meta[kMetaOff + 1] + unode[bi], sq = meta[kMetaOff + 1] + vnode[bj];
-                    for (int l = 1; l < L; ++l) {
-                        if (sp / B == sq / B && sp <= sq) {
-                            double* dst = mgal + ((long)(sp / B - ndom0) * nblk + blk_index(B, sp % B, sq % B)) * 9;
-#pragma unroll
-                            for (int r = 0; r < 3; ++r)
-#pragma unroll
-                                for (int c = 0; c < 3; ++c) atomic_add(dst + c * 3 + r, a[3 * bi + r][3 * bj + c]);
-                        }
-                        if (l + 1 < L) {
-                            sp = going_next[sp];
-                            sq = going_next[sq];
-                        }
-                    }
-                }
-        }
-    }
-
     gj_invert<B>(a, colbuf, rowbuf, ty, tx);
     store_inverse<B, TO>(a, ty, tx, minv + (long)d * nblk * 9);
 }
@@ -756,9 +651,14 @@ struct Vec16<float> {
     static constexpr int k = 4;
 };
 
-// Z[domain] = Minv[domain] * R[domain], one workgroup per domain.
+// Z[domain] = Minv[domain] * R[domain], one 256-thread workgroup per domain.
+// All loads of the domain's inverse blocks are issued before any LDS store so
+// the whole 19/38 KB domain is in flight at once.  Thread (rb, g) forms the
+// partial product of block row rb with the column blocks cb = g, g+G, ...;
+// the G partials of a row sit in adjacent lanes and are summed by shuffles in
+// a fixed order.
 template <int B, typename TS>
-__global__ __launch_bounds__(128) void k_local_solve(const int* __restrict__ done, const TS* __restrict__ minv,
+__global__ __launch_bounds__(256) void k_local_solve(const int* __restrict__ done, const TS* __restrict__ minv,
                                                       const TS* __restrict__ R, TS* __restrict__ Z)
 {
     if (done != nullptr && *done) return;
@@ -766,30 +666,56 @@ __global__ __launch_bounds__(128) void k_local_solve(const int* __restrict__ don
     constexpr int nval = nblk * 9;
     using V = typename Vec16<TS>::V;
     constexpr int kv = Vec16<TS>::k;
-    __shared__ __align__(16) TS sM[nval + 4];
+    constexpr int nvec = nval / kv;
+    constexpr int per = (nvec + 255) / 256;
+    constexpr int G = 256 / B;  // threads per block row
+    __shared__ __align__(16) TS sM[nval];
     __shared__ TS sR[3 * B];
     const int d = blockIdx.x;
-    const TS* src = minv + (long)d * nval;
-    // nval*sizeof(TS) is a multiple of 16 for B in {16, 32}
-    const V* vsrc = reinterpret_cast<const V*>(src);
-    for (int e = threadIdx.x; e < nval / kv; e += blockDim.x) reinterpret_cast<V*>(sM)[e] = vsrc[e];
-    for (int e = threadIdx.x; e < 3 * B; e += blockDim.x) sR[e] = R[3 * (long)d * B + e];
+    const V* vsrc = reinterpret_cast<const V*>(minv + (long)d * nval);
+    V tmp[per];
+    // unconditional (clamped) loads so all of them are in flight together
+#pragma unroll
+    for (int q = 0; q < per; ++q) {
+        const int e = threadIdx.x + 256 * q;
+        tmp[q] = vsrc[e < nvec ? e : nvec - 1];
+    }
+    __builtin_amdgcn_sched_barrier(0);  // keep every load ahead of the LDS stores
+    if (threadIdx.x < 3 * B) sR[threadIdx.x] = R[3 * (long)d * B + threadIdx.x];
+#pragma unroll
+    for (int q = 0; q < per; ++q) {
+        const int e = threadIdx.x + 256 * q;
+        if (e < nvec) reinterpret_cast<V*>(sM)[e] = tmp[q];
+    }
     __syncthreads();
-    const int t = threadIdx.x;
-    if (t < 3 * B) {
-        const int rb = t / 3, i = t % 3;
-        TS acc = 0;
-        for (int cb = 0; cb < B; ++cb) {
-            const TS* m;
-            if (cb >= rb) {
-                m = sM + 9 * blk_index(B, rb, cb);
-                acc += m[i] * sR[3 * cb] + m[3 + i] * sR[3 * cb + 1] + m[6 + i] * sR[3 * cb + 2];
-            } else {
-                m = sM + 9 * blk_index(B, cb, rb);
-                acc += m[3 * i] * sR[3 * cb] + m[3 * i + 1] * sR[3 * cb + 1] + m[3 * i + 2] * sR[3 * cb + 2];
-            }
+    const int rb = threadIdx.x / G, g = threadIdx.x % G;
+    TS z0 = 0, z1 = 0, z2 = 0;
+#pragma unroll
+    for (int cb = g; cb < B; cb += G) {
+        const TS r0 = sR[3 * cb], r1 = sR[3 * cb + 1], r2 = sR[3 * cb + 2];
+        if (cb >= rb) {
+            const TS* m = sM + 9 * blk_index(B, rb, cb);
+            z0 += m[0] * r0 + m[3] * r1 + m[6] * r2;
+            z1 += m[1] * r0 + m[4] * r1 + m[7] * r2;
+            z2 += m[2] * r0 + m[5] * r1 + m[8] * r2;
+        } else {
+            const TS* m = sM + 9 * blk_index(B, cb, rb);
+            z0 += m[0] * r0 + m[1] * r1 + m[2] * r2;
+            z1 += m[3] * r0 + m[4] * r1 + m[5] * r2;
+            z2 += m[6] * r0 + m[7] * r1 + m[8] * r2;
         }
-        Z[3 * (long)d * B + t] = acc;
+    }
+#pragma unroll
+    for (int off = G / 2; off >= 1; off >>= 1) {
+        z0 += __shfl_xor(z0, off, 64);
+        z1 += __shfl_xor(z1, off, 64);
+        z2 += __shfl_xor(z2, off, 64);
+    }
+    if (g == 0) {
+        const long o = 3 * ((long)d * B + rb);
+        Z[o] = z0;
+        Z[o + 1] = z1;
+        Z[o + 2] = z2;
     }
 }
 
@@ -935,16 +861,44 @@ void Mas::assemble_invert_t(const double* vals, const int* rows, const int* cols
     if (ndomc > 0) launch_fill<double>(s, mgal_.get(), 0.0, (long)ndomc * nblk * 9);
     k_row_ptr<<<grid_for(n + 1, 256), 256, 0, s>>>(rows, nnz, offset, n, row_ptr_.get());
     SGIPC_LAUNCH_CHECK();
+    // coarse operators A_l = P_l^T A_{l-1} P_l, level by level
     if (L > 1 && nnz > 0) {
-        k_galerkin_cross<BB><<<grid_for(nnz, 256), 256, 0, s>>>(meta_.get(), L, n, map_.get(), vals, rows, cols,
-                                                               nnz, offset, mgal_.get());
-        SGIPC_LAUNCH_CHECK();
+        coarse_v_[0].reserve((size_t)nnz * 9);
+        coarse_v_[1].reserve((size_t)nnz * 9);
+        coarse_r_[0].reserve(nnz);
+        coarse_r_[1].reserve(nnz);
+        coarse_c_[0].reserve(nnz);
+        coarse_c_[1].reserve(nnz);
+        map_v_.reserve((size_t)nnz * 9);
+        map_r_.reserve(nnz);
+        map_c_.reserve(nnz);
+        const double* pv = vals;
+        const int* pr = rows;
+        const int* pc = cols;
+        int pn = nnz, poff = offset;
+        for (int l = 1; l < L; ++l) {
+            k_coarsen_map<<<grid_for(pn, 256), 256, 0, s>>>(pv, pr, pc, pn, poff, h_cnt[l - 1], going_next_.get(),
+                                                            h_off[l - 1], h_off[l], map_v_.get(), map_r_.get(),
+                                                            map_c_.get());
+            SGIPC_LAUNCH_CHECK();
+            const int q = l & 1;
+            const int cn = conv_.convert(s, map_v_.get(), map_r_.get(), map_c_.get(), pn, h_cnt[l], coarse_v_[q].get(),
+                                         coarse_r_[q].get(), coarse_c_[q].get(), nnz);
+            k_extract_domains<BB><<<grid_for(cn, 256), 256, 0, s>>>(coarse_v_[q].get(), coarse_r_[q].get(),
+                                                                     coarse_c_[q].get(), cn, h_off[l], ndom0,
+                                                                     mgal_.get());
+            SGIPC_LAUNCH_CHECK();
+            pv = coarse_v_[q].get();
+            pr = coarse_r_[q].get();
+            pc = coarse_c_[q].get();
+            pn = cn;
+            poff = 0;
+        }
     }
     constexpr int T = InvShape<BB>::T;
     if (precision == SGIPC_FP64) {
         minv64_.reserve((size_t)ndom * nblk * 9);
-        k_inv_level0<BB, double><<<ndom0, T, 0, s>>>(meta_.get(), L, n, map_.get(), going_next_.get(), vals, cols,
-                                                    row_ptr_.get(), offset, mgal_.get(), minv64_.get());
+        k_inv_level0<BB, double><<<ndom0, T, 0, s>>>(n, vals, cols, row_ptr_.get(), offset, minv64_.get());
         SGIPC_LAUNCH_CHECK();
         if (ndomc > 0) {
             k_inv_coarse<BB, double><<<ndomc, T, 0, s>>>(mgal_.get(), ndom0, minv64_.get());
@@ -952,8 +906,7 @@ void Mas::assemble_invert_t(const double* vals, const int* rows, const int* cols
         }
     } else {
         minv32_.reserve((size_t)ndom * nblk * 9);
-        k_inv_level0<BB, float><<<ndom0, T, 0, s>>>(meta_.get(), L, n, map_.get(), going_next_.get(), vals, cols,
-                                                   row_ptr_.get(), offset, mgal_.get(), minv32_.get());
+        k_inv_level0<BB, float><<<ndom0, T, 0, s>>>(n, vals, cols, row_ptr_.get(), offset, minv32_.get());
         SGIPC_LAUNCH_CHECK();
         if (ndomc > 0) {
             k_inv_coarse<BB, float><<<ndomc, T, 0, s>>>(mgal_.get(), ndom0, minv32_.get());
@@ -1014,7 +967,7 @@ int Mas::apply_t(hipStream_t s, const TV* r, TV* z, double* rz_part, const PcgSt
         k_restrict_up<TS><<<grid_for(work > 0 ? work : 1, 256), 256, 0, s>>>(done, meta_.get(), l, going_next_.get(), R);
         SGIPC_LAUNCH_CHECK();
     }
-    k_local_solve<BB, TS><<<total / BB, 128, 0, s>>>(done, minv, R, Z);
+    k_local_solve<BB, TS><<<total / BB, 256, 0, s>>>(done, minv, R, Z);
     SGIPC_LAUNCH_CHECK();
     const int g = vec_grid(n);
     k_prolong<TV, TS><<<g, 256, 0, s>>>(done, meta_.get(), n, L, map_.get(), Z, r, z, rz_part);

@@ -1,5 +1,6 @@
 // Multilevel additive Schwarz preconditioner (host object).
 #pragma once
+#include "assembly.hpp"
 #include "pcg.hpp"
 #include "sgipc_internal.hpp"
 
@@ -56,7 +57,10 @@ class Mas {
     DevBuf<float> minv32_;
     DevBuf<double> R64_, Z64_;
     DevBuf<float> R32_, Z32_;
-    int apply_partials_ = 0;
+    // coarse operators (ping-pong) and mapped triplets of the coarsening
+    DevBuf<double> coarse_v_[2], map_v_;
+    DevBuf<int> coarse_r_[2], coarse_c_[2], map_r_, map_c_;
+    Converter conv_;
 };
 
 // Adapter giving the PCG driver a preconditioner over a Mas.

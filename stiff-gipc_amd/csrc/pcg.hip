@@ -87,7 +87,7 @@ __global__ __launch_bounds__(kVecThreads) void pcg_update_xr(
     }
 }
 
-// p = z + beta p; Ap = 0.  beta = rz_new / rz_old.
+// p = z + beta p.  beta = rz_new / rz_old.
 template <typename TV>
 __global__ __launch_bounds__(kVecThreads) void pcg_update_p(
     const PcgCtl* __restrict__ ctl, const double* __restrict__ rz_new_part,
@@ -101,7 +101,6 @@ __global__ __launch_bounds__(kVecThreads) void pcg_update_p(
     double beta = rzn / rzo;
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (long)gridDim.x * blockDim.x) {
         p[i] = (TV)((double)z[i] + beta * (double)p[i]);
-        Ap[i] = TV(0);
     }
 }
 
