@@ -56,6 +56,7 @@ enum sgipc_stop_mode {
 typedef struct sgipc_ctx sgipc_ctx;
 typedef struct sgipc_mas sgipc_mas;
 typedef struct sgipc_system sgipc_system;
+typedef struct sgipc_spmat sgipc_spmat;
 
 /* ---- context ------------------------------------------------------------- */
 int sgipc_abi_version(void);
@@ -97,6 +98,34 @@ int sgipc_barrier_pt_gradient_hessian(sgipc_ctx* ctx, const double* x, const int
                                       double kappa, int hess_offset, double* grad, double* tvals,
                                       int* trows, int* tcols);
 
+/* Replaces _calculate_triangle_fem_gradient_hessian (StiffGIPC/femEnergy.cu:
+ * 2146-2215): Baraff-Witkin stretch + shear cloth triangles, gradient
+ * (atomically added into grad when non-NULL) and projected Hessian as 6 upper
+ * blocks per triangle at tvals/trows/tcols[6*t ...].  tris: uint3 per
+ * triangle; dm_inv2: 2x2 rest inverse, row-major; area: rest area times
+ * thickness. */
+int sgipc_fem_tri_gradient_hessian(sgipc_ctx* ctx, const double* x, const uint32_t* tris,
+                                   const double* dm_inv2, const double* area, int n_tris,
+                                   double k_stretch, double k_shear, double strain_rate, double dt,
+                                   int hess_offset, double* grad, double* tvals, int* trows,
+                                   int* tcols);
+
+/* Replaces PrepareQuadBendingQ (StiffGIPC/femEnergy.cu:2490-2618, a host loop
+ * there): Q (16 doubles, row-major 4x4) per hinge from rest positions.
+ * edges, adj: uint2 per hinge; adj.y == 0xFFFFFFFF marks a boundary edge. */
+int sgipc_quad_bending_prepare(sgipc_ctx* ctx, const double* rest_x, const uint32_t* edges,
+                               const uint32_t* adj, int n_edges, double* q16);
+
+/* Replaces _calculate_quad_bending_gradient_hessian (StiffGIPC/femEnergy.cu:
+ * 2620-2754): gradient coef*Q*x and 10 upper blocks coef*Q_ab*I per hinge
+ * (coef = |rest edge| * k_bend * dt^2); boundary hinges write ten zero blocks
+ * at (0, 0). */
+int sgipc_quad_bending_gradient_hessian(sgipc_ctx* ctx, const double* x, const double* rest_x,
+                                        const uint32_t* edges, const uint32_t* adj,
+                                        const double* q16, int n_edges, double k_bend, double dt,
+                                        int hess_offset, double* grad, double* tvals, int* trows,
+                                        int* tcols);
+
 /* Replaces the mass-diagonal writer of computeGradientAndHessian
  * (StiffGIPC/GIPC.cu:10466-10485): mass[i]*I at (i+off, i+off). */
 int sgipc_mass_diagonal(sgipc_ctx* ctx, const double* mass, int n, int hess_offset,
@@ -121,14 +150,26 @@ int sgipc_spmv_srbk_f32(sgipc_ctx* ctx, float a, const float* vals, const int* r
 /* Round FP64 blocks/vectors to FP32 (n values). */
 int sgipc_convert_f64_to_f32(sgipc_ctx* ctx, const double* in, float* out, size_t n);
 
-/* Measurement helper: expands the upper BCOO into both triangles (as the PCG
- * solver does) and returns the average duration (ms) of the PCG SpMV kernel
- * (row-owned, emits the p.Ap partials) over `reps` back-to-back launches, timed
- * with HIP events on the context's stream.  precision: SGIPC_FP64 or
- * SGIPC_FP32_IR (FP32 values).  *full_nnz (optional) receives the expanded
- * block count. */
+/* Prepared symmetric matrix for repeated SpMVs (the PCG loop's use of
+ * Spmv::srbk_spmv, StiffGIPC/linear_system/solver/pcg_solver.cu:191-332):
+ * create() lays the upper BCOO out in row tiles (in-tile blocks stored once,
+ * cross-tile blocks in both rows; an FP32 copy when want_f32), apply() runs
+ * y = a*A*x + b*y without atomics.  The layout copies the values: the input
+ * arrays may change or be freed after create(). */
+int sgipc_spmat_create(sgipc_ctx* ctx, sgipc_spmat** out, const double* vals, const int* rows,
+                       const int* cols, int nnz, int n_rows, int want_f32);
+int sgipc_spmat_destroy(sgipc_spmat* m);
+int sgipc_spmat_apply(sgipc_spmat* m, double a, const double* x, double b, double* y);
+int sgipc_spmat_apply_f32(sgipc_spmat* m, float a, const float* x, float b, float* y);
+
+/* Measurement helper: lays the upper BCOO out as the PCG solver does (row
+ * tiles, in-tile blocks once, cross-tile blocks in both rows) and returns the
+ * average duration (ms) of the PCG SpMV kernel (emits the p.Ap partials) over
+ * `reps` back-to-back launches, timed with HIP events on the context's stream.
+ * precision: SGIPC_FP64 or SGIPC_FP32_IR (FP32 values).  *stored_blocks
+ * (optional) receives the number of blocks the layout stores. */
 int sgipc_spmv_time(sgipc_ctx* ctx, int precision, const double* vals, const int* rows,
-                    const int* cols, int nnz, int n_rows, int reps, double* avg_ms, int* full_nnz);
+                    const int* cols, int nnz, int n_rows, int reps, double* avg_ms, int* stored_blocks);
 
 /* ---- MAS preconditioner (MASPreconditioner, StiffGIPC/MASPreconditioner.cuh:59-107) */
 /* bank_size is the domain size B (16 = the reference's BANKSIZE, or 32). */
@@ -196,10 +237,12 @@ int sgipc_pcg_solve(sgipc_ctx* ctx, const double* vals, const int* rows, const i
 
 /* ---- whole linear system (GlobalLinearSystem::solve_linear_system,
  *      StiffGIPC/linear_system/linear_system/global_linear_system.cu:136-148) --
- * One object owns a tet scene's device inputs, the triplet arena, the BCOO, the
- * MAS and the PCG buffers; solve() runs assembly (barrier, tets, mass),
- * conversion, the MAS build and PCG. */
-typedef struct sgipc_tet_scene_desc {
+ * One object owns a scene's device inputs, the triplet arena, the BCOO, the MAS
+ * and the PCG buffers; solve() runs assembly in the reference's arena order
+ * (contacts, tets, bending hinges, cloth triangles, mass;
+ * StiffGIPC/GIPC.cu:10291-10485), conversion, the MAS build and PCG.  Every
+ * element set is optional (count 0, pointers NULL). */
+typedef struct sgipc_scene_desc {
     int n_verts;
     int n_tets;
     int n_pairs;
@@ -220,7 +263,19 @@ typedef struct sgipc_tet_scene_desc {
     const uint32_t* nbr_start;
     const uint32_t* nbr_num;
     const uint32_t* nbr_list;
-} sgipc_tet_scene_desc;
+    /* cloth triangles (Baraff-Witkin) */
+    int n_tris;
+    double k_stretch, k_shear, strain_rate;
+    const uint32_t* tris;     /* 3 per triangle                      */
+    const double* dm_inv2;    /* 4 per triangle, row-major           */
+    const double* area;       /* per triangle, area * thickness      */
+    /* quadratic bending hinges */
+    int n_hinges;
+    double k_bend;
+    const double* rest_x;     /* 3n rest positions                    */
+    const uint32_t* hinge_edges; /* 2 per hinge                       */
+    const uint32_t* hinge_adj;   /* 2 per hinge, 0xFFFFFFFF = boundary */
+} sgipc_scene_desc;
 
 typedef struct sgipc_solve_timing {
     double ms_assemble;      /* element Hessians + gradient                 */
@@ -232,7 +287,7 @@ typedef struct sgipc_solve_timing {
     int n_triplets;
 } sgipc_solve_timing;
 
-int sgipc_system_create(sgipc_ctx* ctx, sgipc_system** out, const sgipc_tet_scene_desc* desc,
+int sgipc_system_create(sgipc_ctx* ctx, sgipc_system** out, const sgipc_scene_desc* desc,
                         int bank_size);
 int sgipc_system_destroy(sgipc_system* sys);
 /* b, x: device vectors of 3*n_verts doubles.  grad (optional, device) receives

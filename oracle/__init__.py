@@ -71,6 +71,15 @@ def lib():
                               ctypes.c_int, c_dbl_p]
         L.orc_pcg.restype = ctypes.c_int
         L.orc_set_fault.argtypes = [ctypes.c_int]
+        L.orc_tri_bw.argtypes = [ctypes.c_int, c_dbl_p, c_int_p, c_dbl_p, c_dbl_p] + [ctypes.c_double] * 4 + \
+            [ctypes.c_int, c_dbl_p, c_dbl_p, c_int_p, c_int_p]
+        L.orc_tri_bw_energy.argtypes = [c_dbl_p, c_dbl_p] + [ctypes.c_double] * 4
+        L.orc_tri_bw_energy.restype = ctypes.c_double
+        L.orc_bw_dpdf.argtypes = [c_dbl_p] + [ctypes.c_double] * 3 + [c_dbl_p, c_dbl_p]
+        L.orc_quad_bending_q.argtypes = [ctypes.c_int, c_dbl_p, c_int_p, c_int_p, c_dbl_p]
+        L.orc_quad_bending.argtypes = [ctypes.c_int, c_dbl_p, c_dbl_p, c_int_p, c_int_p, c_dbl_p,
+                                       ctypes.c_double, ctypes.c_double, ctypes.c_int, c_dbl_p, c_dbl_p,
+                                       c_int_p, c_int_p]
         _LIB = L
     return _LIB
 
@@ -112,6 +121,65 @@ def barrier_pt(scene, hess_offset=0, with_grad=True):
     return vals, rows, cols, grad
 
 
+def tri_bw(scene, hess_offset=0, with_grad=True):
+    """Baraff-Witkin cloth triangles: 6 blocks per triangle."""
+    L = lib()
+    m = scene.n_tris
+    vals = np.zeros((6 * m, 9))
+    rows = np.zeros(6 * m, dtype=np.int32)
+    cols = np.zeros(6 * m, dtype=np.int32)
+    grad = np.zeros(3 * scene.n) if with_grad else None
+    if m:
+        L.orc_tri_bw(m, _ptr(np.ascontiguousarray(scene.x), c_dbl_p),
+                     _ptr(np.ascontiguousarray(scene.tris, dtype=np.int32), c_int_p),
+                     _ptr(np.ascontiguousarray(scene.dm_inv2), c_dbl_p), _ptr(np.ascontiguousarray(scene.area), c_dbl_p),
+                     scene.k_stretch, scene.k_shear, scene.strain_rate, scene.dt, hess_offset,
+                     _ptr(grad, c_dbl_p), _ptr(vals, c_dbl_p), _ptr(rows, c_int_p), _ptr(cols, c_int_p))
+    return vals, rows, cols, grad
+
+
+def tri_bw_energy(x9, dm_inv2, area, k_stretch, k_shear, rate):
+    x9 = np.ascontiguousarray(x9, dtype=np.float64)
+    d = np.ascontiguousarray(dm_inv2, dtype=np.float64)
+    return lib().orc_tri_bw_energy(_ptr(x9, c_dbl_p), _ptr(d, c_dbl_p), area, k_stretch, k_shear, rate)
+
+
+def bw_dpdf(F6, k_stretch, k_shear, rate):
+    F6 = np.ascontiguousarray(F6, dtype=np.float64)
+    P, H = np.zeros(6), np.zeros(36)
+    lib().orc_bw_dpdf(_ptr(F6, c_dbl_p), k_stretch, k_shear, rate, _ptr(P, c_dbl_p), _ptr(H, c_dbl_p))
+    return P, H.reshape(6, 6)
+
+
+def quad_bending_q(rest_x, edges, adj):
+    e = np.ascontiguousarray(edges, dtype=np.int32)
+    a = np.ascontiguousarray(adj, dtype=np.int32)
+    Q = np.zeros((e.shape[0], 16))
+    lib().orc_quad_bending_q(e.shape[0], _ptr(np.ascontiguousarray(rest_x, dtype=np.float64), c_dbl_p),
+                             _ptr(e, c_int_p), _ptr(a, c_int_p), _ptr(Q, c_dbl_p))
+    return Q
+
+
+def quad_bending(scene, Q=None, hess_offset=0, with_grad=True):
+    """Quadratic bending hinges: 10 blocks per hinge."""
+    L = lib()
+    m = scene.n_hinges
+    vals = np.zeros((10 * m, 9))
+    rows = np.zeros(10 * m, dtype=np.int32)
+    cols = np.zeros(10 * m, dtype=np.int32)
+    grad = np.zeros(3 * scene.n) if with_grad else None
+    if m:
+        if Q is None:
+            Q = quad_bending_q(scene.rest, scene.edges, scene.adj)
+        L.orc_quad_bending(m, _ptr(np.ascontiguousarray(scene.x), c_dbl_p),
+                           _ptr(np.ascontiguousarray(scene.rest), c_dbl_p),
+                           _ptr(np.ascontiguousarray(scene.edges, dtype=np.int32), c_int_p),
+                           _ptr(np.ascontiguousarray(scene.adj, dtype=np.int32), c_int_p),
+                           _ptr(np.ascontiguousarray(Q), c_dbl_p), scene.k_bend, scene.dt, hess_offset,
+                           _ptr(grad, c_dbl_p), _ptr(vals, c_dbl_p), _ptr(rows, c_int_p), _ptr(cols, c_int_p))
+    return vals, rows, cols, grad
+
+
 def mass_diag(scene, hess_offset=0):
     n = scene.n
     vals = np.zeros((n, 9))
@@ -123,14 +191,20 @@ def mass_diag(scene, hess_offset=0):
 
 
 def triplets(scene, with_grad=False):
-    """All triplets in the reference's arena order: contacts, tets, mass."""
-    bv, br, bc, bg = barrier_pt(scene, with_grad=with_grad)
-    tv, tr, tc, tg = tet_snh(scene, with_grad=with_grad)
+    """All triplets in the reference's arena order (GIPC.cu:10291-10485):
+    contacts, tets, bending hinges, cloth triangles, mass."""
+    parts = [barrier_pt(scene, with_grad=with_grad)]
+    if getattr(scene, "n_tets", 0):
+        parts.append(tet_snh(scene, with_grad=with_grad))
+    if getattr(scene, "n_hinges", 0):
+        parts.append(quad_bending(scene, with_grad=with_grad))
+    if getattr(scene, "n_tris", 0):
+        parts.append(tri_bw(scene, with_grad=with_grad))
     mv, mr, mc = mass_diag(scene)
-    vals = np.concatenate([bv, tv, mv])
-    rows = np.concatenate([br, tr, mr])
-    cols = np.concatenate([bc, tc, mc])
-    grad = (bg + tg) if with_grad else None
+    vals = np.concatenate([p[0] for p in parts] + [mv])
+    rows = np.concatenate([p[1] for p in parts] + [mr])
+    cols = np.concatenate([p[2] for p in parts] + [mc])
+    grad = sum(p[3] for p in parts) if with_grad else None
     return vals, rows, cols, grad
 
 

@@ -262,6 +262,243 @@ int orc_tet_snh(int n_tets, const double* x, const int* tets, const double* dm_i
 }
 
 /* ------------------------------------------------------------------------- */
+/* Baraff-Witkin cloth triangle (stretch + shear)                             */
+/* StiffGIPC/femEnergy.cu:2146-2215 (kernel), 351-395 (dE/dF), 535-629 and    */
+/* 631-700 (projected stretch / shear Hessians), 967-1005 (energy),           */
+/* 1171-1210 (dF/dx), 304-322 (Ds).  F = Ds Dm^-1 is 3x2 with columns         */
+/* f_u, f_v; vec(F) is column-major (q = 3*c + r).                            */
+/*   E = area * ( k_st * sum_{w=u,v} psi(|f_w|) + k_sh * I6^2 ),              */
+/*   psi(s) = (s-1)^2 + [s>1] rate (s-1)^3,  I6 = f_u . f_v.                  */
+/* The Hessian is projected term by term: each stretch term onto             */
+/* psi'' f^ f^T + max(psi'/s, 0)(I - f^ f^T), the shear term onto the         */
+/* non-negative part of its eigen-decomposition.                             */
+/* ------------------------------------------------------------------------- */
+static void tri_F(const double* xs /* 9 */, const double* dminv /* 4 rm */, double F[3][2])
+{
+    for (int i = 0; i < 3; ++i) {
+        double d1 = xs[3 + i] - xs[i], d2 = xs[6 + i] - xs[i];
+        for (int j = 0; j < 2; ++j) F[i][j] = d1 * dminv[j] + d2 * dminv[2 + j];
+    }
+}
+
+static void stretch_terms(double sig, double rate, double* dpsi, double* d2psi)
+{
+    double e = sig - 1.0;
+    *dpsi = 2.0 * e + (sig > 1.0 ? 3.0 * rate * e * e : 0.0);
+    *d2psi = 2.0 + (sig > 1.0 ? 6.0 * rate * e : 0.0);
+}
+
+double orc_tri_bw_energy(const double* x9, const double* dm_inv2, double area, double k_st,
+                         double k_sh, double rate)
+{
+    double F[3][2];
+    tri_F(x9, dm_inv2, F);
+    double E = 0.0;
+    for (int c = 0; c < 2; ++c) {
+        double sig = sqrt(F[0][c] * F[0][c] + F[1][c] * F[1][c] + F[2][c] * F[2][c]);
+        double e = sig - 1.0;
+        E += k_st * (e * e + (sig * sig > 1.0 ? rate * e * e * e : 0.0));
+    }
+    double I6 = F[0][0] * F[0][1] + F[1][0] * F[1][1] + F[2][0] * F[2][1];
+    return area * (E + k_sh * I6 * I6);
+}
+
+/* dE/dF (per unit area) and the projected 6x6 d2E/dF2 (row-major over vec(F)). */
+static void bw_pepf_hq(const double F[3][2], double k_st, double k_sh, double rate, double P[3][2],
+                       double Hq[36])
+{
+    memset(Hq, 0, 36 * sizeof(double));
+    for (int c = 0; c < 2; ++c) {
+        double f[3] = {F[0][c], F[1][c], F[2][c]};
+        double sig = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+        double d1, d2;
+        stretch_terms(sig, rate, &d1, &d2);
+        double fh[3] = {f[0] / sig, f[1] / sig, f[2] / sig};
+        for (int r = 0; r < 3; ++r) P[r][c] = k_st * d1 * fh[r];
+        double iso = d1 / sig > 0.0 ? d1 / sig : 0.0;
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                Hq[(3 * c + a) * 6 + 3 * c + b] +=
+                    k_st * ((a == b ? iso : 0.0) + (d2 - iso) * fh[a] * fh[b]);
+    }
+    double I6 = F[0][0] * F[0][1] + F[1][0] * F[1][1] + F[2][0] * F[2][1];
+    for (int r = 0; r < 3; ++r) {
+        P[r][0] += k_sh * 2.0 * I6 * F[r][1];
+        P[r][1] += k_sh * 2.0 * I6 * F[r][0];
+    }
+    /* shear: d2(I6^2)/dF2 = 2 g g^T + 2 I6 S, g = (f_v, f_u), S swaps the columns */
+    double g[6] = {F[0][1], F[1][1], F[2][1], F[0][0], F[1][0], F[2][0]};
+    double A[36], V[36];
+    for (int a = 0; a < 6; ++a)
+        for (int b = 0; b < 6; ++b) A[a * 6 + b] = 2.0 * g[a] * g[b] + ((a % 3 == b % 3 && a / 3 != b / 3) ? 2.0 * I6 : 0.0);
+    jacobi_eig(6, A, V);
+    for (int a = 0; a < 6; ++a)
+        for (int b = 0; b < 6; ++b) {
+            double sum = 0.0;
+            for (int k = 0; k < 6; ++k) {
+                double l = A[k * 6 + k];
+                if (l > 0.0 || g_fault == 4) sum += V[a * 6 + k] * l * V[b * 6 + k];
+            }
+            Hq[a * 6 + b] += k_sh * sum;
+        }
+}
+
+void orc_bw_dpdf(const double* F6, double k_st, double k_sh, double rate, double* P6, double* Hproj36)
+{
+    double F[3][2], P[3][2], H[36];
+    for (int c = 0; c < 2; ++c)
+        for (int r = 0; r < 3; ++r) F[r][c] = F6[3 * c + r];
+    bw_pepf_hq(F, k_st, k_sh, rate, P, H);
+    if (P6)
+        for (int c = 0; c < 2; ++c)
+            for (int r = 0; r < 3; ++r) P6[3 * c + r] = P[r][c];
+    if (Hproj36) memcpy(Hproj36, H, sizeof(H));
+}
+
+int orc_tri_bw(int n_tris, const double* x, const int* tris, const double* dm_inv2, const double* area,
+               double k_st, double k_sh, double rate, double dt, int hess_offset, double* grad,
+               double* tvals, int* trows, int* tcols)
+{
+#pragma omp parallel for schedule(static)
+    for (int t = 0; t < n_tris; ++t) {
+        const int* tv = tris + 3 * (size_t)t;
+        double xs[9];
+        for (int a = 0; a < 3; ++a)
+            for (int d = 0; d < 3; ++d) xs[3 * a + d] = x[3 * (size_t)tv[a] + d];
+        const double* Dmi = dm_inv2 + 4 * (size_t)t;
+        double F[3][2], P[3][2], Hq[36];
+        tri_F(xs, Dmi, F);
+        bw_pepf_hq(F, k_st, k_sh, rate, P, Hq);
+        /* D[a][j]: dF(:,j)/dx_a */
+        double D[3][2];
+        for (int j = 0; j < 2; ++j) {
+            D[0][j] = -(Dmi[j] + Dmi[2 + j]);
+            D[1][j] = Dmi[j];
+            D[2][j] = Dmi[2 + j];
+        }
+        double scale = dt * dt * area[t];
+        if (grad)
+            for (int a = 0; a < 3; ++a)
+                for (int d = 0; d < 3; ++d) {
+                    double g = scale * (D[a][0] * P[d][0] + D[a][1] * P[d][1]);
+#pragma omp atomic
+                    grad[3 * (size_t)tv[a] + d] += g;
+                }
+        double H9[81];
+        for (int a = 0; a < 3; ++a)
+            for (int d = 0; d < 3; ++d)
+                for (int b = 0; b < 3; ++b)
+                    for (int e = 0; e < 3; ++e) {
+                        double sum = 0.0;
+                        for (int j = 0; j < 2; ++j)
+                            for (int jp = 0; jp < 2; ++jp)
+                                sum += D[a][j] * D[b][jp] * Hq[(3 * j + d) * 6 + 3 * jp + e];
+                        H9[(3 * a + d) * 9 + 3 * b + e] = scale * sum;
+                    }
+        int gid[3] = {tv[0] + hess_offset, tv[1] + hess_offset, tv[2] + hess_offset};
+        write_upper_blocks(3, gid, H9, tvals + 54 * (size_t)t, trows + 6 * (size_t)t,
+                           tcols + 6 * (size_t)t);
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Quadratic (constant-Q) bending, StiffGIPC/femEnergy.cu:2490-2618 (Q),      */
+/* 2620-2754 (kernel).  Vertex order [e0, e1, opposite of tri 1, opposite of  */
+/* tri 2]; Q = 3/(A0+A1) K K^T with cotangent weights K; E = coef/2 x^T Q x   */
+/* per coordinate, coef = |rest e0 - rest e1| k dt^2.  Boundary edges          */
+/* (adj[1] == -1) and degenerate hinges get Q = 0; the kernel then writes     */
+/* ten zero blocks at (0, 0).                                                 */
+/* ------------------------------------------------------------------------- */
+static double cot3(const double* a, const double* b)
+{
+    double c[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+    double cn = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+    if (cn < 1e-12) return 0.0;
+    return (a[0] * b[0] + a[1] * b[1] + a[2] * b[2]) / cn;
+}
+
+void orc_quad_bending_q(int n_edges, const double* rest_x, const int* edges, const int* adj, double* Q16)
+{
+    for (int i = 0; i < n_edges; ++i) {
+        double* Q = Q16 + 16 * (size_t)i;
+        memset(Q, 0, 16 * sizeof(double));
+        if (adj[2 * i + 1] == -1) continue;
+        int vid[4] = {edges[2 * i], edges[2 * i + 1], adj[2 * i], adj[2 * i + 1]};
+        double v[4][3];
+        for (int a = 0; a < 4; ++a)
+            for (int d = 0; d < 3; ++d) v[a][d] = rest_x[3 * (size_t)vid[a] + d];
+        double e0[3], e1[3], e2[3], e3[3], e4[3], m0[3];
+        for (int d = 0; d < 3; ++d) {
+            e0[d] = v[1][d] - v[0][d];
+            e1[d] = v[2][d] - v[0][d];
+            e2[d] = v[3][d] - v[0][d];
+            e3[d] = v[2][d] - v[1][d];
+            e4[d] = v[3][d] - v[1][d];
+            m0[d] = -e0[d];
+        }
+        double c01[3] = {e0[1] * e1[2] - e0[2] * e1[1], e0[2] * e1[0] - e0[0] * e1[2], e0[0] * e1[1] - e0[1] * e1[0]};
+        double c02[3] = {e0[1] * e2[2] - e0[2] * e2[1], e0[2] * e2[0] - e0[0] * e2[2], e0[0] * e2[1] - e0[1] * e2[0]};
+        double n01 = sqrt(c01[0] * c01[0] + c01[1] * c01[1] + c01[2] * c01[2]);
+        double n02 = sqrt(c02[0] * c02[0] + c02[1] * c02[1] + c02[2] * c02[2]);
+        if (n01 < 1e-12 || n02 < 1e-12) continue;
+        double A = 0.5 * n01 + 0.5 * n02;
+        if (A < 1e-12) continue;
+        double k01 = cot3(e0, e1), k02 = cot3(e0, e2), k03 = cot3(m0, e3), k04 = cot3(m0, e4);
+        double K[4] = {k03 + k04, k01 + k02, -k01 - k03, -k02 - k04};
+        int bad = 0;
+        for (int r = 0; r < 4; ++r)
+            for (int c = 0; c < 4; ++c) {
+                Q[r * 4 + c] = 3.0 / A * K[r] * K[c];
+                if (!isfinite(Q[r * 4 + c])) bad = 1;
+            }
+        if (bad) memset(Q, 0, 16 * sizeof(double));
+    }
+}
+
+int orc_quad_bending(int n_edges, const double* x, const double* rest_x, const int* edges, const int* adj,
+                     const double* Q16, double k_bend, double dt, int hess_offset, double* grad,
+                     double* tvals, int* trows, int* tcols)
+{
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n_edges; ++i) {
+        double* tv = tvals + 90 * (size_t)i;
+        int* tr = trows + 10 * (size_t)i;
+        int* tc = tcols + 10 * (size_t)i;
+        if (adj[2 * i + 1] == -1) {
+            memset(tv, 0, 90 * sizeof(double));
+            for (int k = 0; k < 10; ++k) tr[k] = tc[k] = 0;
+            continue;
+        }
+        int vid[4] = {edges[2 * i], edges[2 * i + 1], adj[2 * i], adj[2 * i + 1]};
+        const double* Q = Q16 + 16 * (size_t)i;
+        double L = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            double e = rest_x[3 * (size_t)vid[0] + d] - rest_x[3 * (size_t)vid[1] + d];
+            L += e * e;
+        }
+        double coef = sqrt(L) * k_bend * dt * dt;
+        if (grad)
+            for (int a = 0; a < 4; ++a)
+                for (int d = 0; d < 3; ++d) {
+                    double g = 0.0;
+                    for (int b = 0; b < 4; ++b) g += Q[a * 4 + b] * x[3 * (size_t)vid[b] + d];
+                    g *= coef;
+#pragma omp atomic
+                    grad[3 * (size_t)vid[a] + d] += g;
+                }
+        double H12[144];
+        for (int a = 0; a < 4; ++a)
+            for (int d = 0; d < 3; ++d)
+                for (int b = 0; b < 4; ++b)
+                    for (int e = 0; e < 3; ++e) H12[(3 * a + d) * 12 + 3 * b + e] = d == e ? coef * Q[a * 4 + b] : 0.0;
+        int gid[4] = {vid[0] + hess_offset, vid[1] + hess_offset, vid[2] + hess_offset, vid[3] + hess_offset};
+        write_upper_blocks(4, gid, H12, tv, tr, tc);
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
 /* Point-triangle barrier, GIPC rank-2 variant                               */
 /* StiffGIPC/GIPC.cu:4733-4984 (NEWF branch), mlbvh.cu:234-242 (_d_PT),       */
 /* GIPC_PDerivative.cuh:2095-2291 (only column 8 of dF/dx is non-zero: the    */

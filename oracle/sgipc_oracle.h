@@ -16,6 +16,11 @@
  *   orc_tet_snh       energy closed form -> finite-difference gradient; analytic
  *                     unprojected Hessian -> FD of the gradient; projected
  *                     Hessian == numpy eigh-clamp; rest-state eigenvalues.
+ *   orc_tri_bw        energy closed form -> FD gradient; each projected term ==
+ *                     numpy eigh-clamp of the FD Hessian of dE/dF (stretch-only
+ *                     and shear-only), additivity in (k_stretch, k_shear).
+ *   orc_quad_bending  Q 1 = 0, rank 1, PSD; equilateral hinge closed form; FD
+ *                     gradient of the quadratic energy; flat rest -> zero force.
  *   orc_barrier_pt    distance closed form -> FD gradient; lambda0 == d2E/ds2 (FD);
  *                     rank-1 and PSD.
  *   orc_convert       numpy lexsort + add.reduceat.
@@ -56,6 +61,27 @@ int orc_barrier_pt(int n_pairs, const double* x, const int* pairs, const double*
                    double* tvals, int* trows, int* tcols);
 double orc_barrier_energy(double I5, double dhat_sq, double kappa);
 double orc_barrier_lambda0(double I5, double dhat_sq, double kappa);
+
+/* Baraff-Witkin cloth triangle (stretch + shear), 6 upper blocks per triangle
+ * at tvals[6*t ...]; dm_inv2 is the 2x2 rest inverse (row-major), area the
+ * rest area.  StiffGIPC/femEnergy.cu:2146-2215. */
+int orc_tri_bw(int n_tris, const double* x, const int* tris, const double* dm_inv2, const double* area,
+               double k_stretch, double k_shear, double strain_rate, double dt, int hess_offset,
+               double* grad, double* tvals, int* trows, int* tcols);
+double orc_tri_bw_energy(const double* x9, const double* dm_inv2, double area, double k_stretch,
+                         double k_shear, double strain_rate);
+/* dE/dF (P6) and projected d2E/dF2 (6x6 row-major), vec(F) column-major, per unit area. */
+void orc_bw_dpdf(const double* F6, double k_stretch, double k_shear, double strain_rate, double* P6,
+                 double* Hproj36);
+
+/* Quadratic bending: Q (4x4 row-major) per hinge from rest positions
+ * (femEnergy.cu:2490-2618), and the gradient / 10 upper blocks per hinge
+ * (femEnergy.cu:2620-2754).  edges, adj: 2 ints per hinge; adj[1] == -1 marks
+ * a boundary edge. */
+void orc_quad_bending_q(int n_edges, const double* rest_x, const int* edges, const int* adj, double* Q16);
+int orc_quad_bending(int n_edges, const double* x, const double* rest_x, const int* edges, const int* adj,
+                     const double* Q16, double k_bend, double dt, int hess_offset, double* grad,
+                     double* tvals, int* trows, int* tcols);
 
 /* mass[i] * I at (i+off, i+off).  StiffGIPC/GIPC.cu:10466-10485. */
 int orc_mass_diag(int n, const double* mass, int hess_offset, double* tvals, int* trows,
@@ -100,7 +126,7 @@ int orc_pcg(long nnz, const double* vals, const int* rows, const int* cols, int 
 /* Deliberate-mistake switch for the tests that show the checks are sensitive:
  * 0 = none, 1 = SpMV drops the transposed (lower) contribution, 2 = Galerkin
  * coarse levels drop H^T on merged diagonals, 3 = barrier Hessian uses lambda0
- * with the wrong sign of the log term. */
+ * with the wrong sign of the log term, 4 = cloth shear Hessian left unprojected. */
 void orc_set_fault(int fault);
 
 #ifdef __cplusplus

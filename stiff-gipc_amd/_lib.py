@@ -38,11 +38,15 @@ class PcgStats(ctypes.Structure):
                 ("ms_total", cd), ("status", ci)]
 
 
-class TetSceneDesc(ctypes.Structure):
+class SceneDesc(ctypes.Structure):
+    """sgipc_scene_desc."""
     _fields_ = [("n_verts", ci), ("n_tets", ci), ("n_pairs", ci), ("total_neighbor_num", ci), ("dt", cd),
                 ("dhat_sq", cd), ("kappa", cd), ("x", vp), ("tets", vp), ("dm_inv", vp), ("vol", vp), ("mu", vp),
                 ("lam", vp), ("mass", vp), ("pairs", vp), ("pair_x", vp), ("nbr_start", vp), ("nbr_num", vp),
-                ("nbr_list", vp)]
+                ("nbr_list", vp),
+                ("n_tris", ci), ("k_stretch", cd), ("k_shear", cd), ("strain_rate", cd), ("tris", vp),
+                ("dm_inv2", vp), ("area", vp),
+                ("n_hinges", ci), ("k_bend", cd), ("rest_x", vp), ("hinge_edges", vp), ("hinge_adj", vp)]
 
 
 class SolveTiming(ctypes.Structure):
@@ -65,11 +69,18 @@ _SIGS = {
     "sgipc_memset": (ci, [vp, vp, ci, csz]),
     "sgipc_fem_tet_gradient_hessian": (ci, [vp, vp, vp, vp, vp, vp, vp, ci, cd, ci, vp, vp, vp, vp]),
     "sgipc_barrier_pt_gradient_hessian": (ci, [vp, vp, vp, vp, ci, cd, cd, ci, vp, vp, vp, vp]),
+    "sgipc_fem_tri_gradient_hessian": (ci, [vp, vp, vp, vp, vp, ci, cd, cd, cd, cd, ci, vp, vp, vp, vp]),
+    "sgipc_quad_bending_prepare": (ci, [vp, vp, vp, vp, ci, vp]),
+    "sgipc_quad_bending_gradient_hessian": (ci, [vp, vp, vp, vp, vp, vp, ci, cd, cd, ci, vp, vp, vp, vp]),
     "sgipc_mass_diagonal": (ci, [vp, vp, ci, ci, vp, vp, vp]),
     "sgipc_convert_srbk": (ci, [vp, vp, vp, vp, ci, vp, vp, vp, ci, ctypes.POINTER(ci)]),
     "sgipc_spmv_srbk": (ci, [vp, cd, vp, vp, vp, ci, vp, cd, vp, ci]),
     "sgipc_spmv_srbk_f32": (ci, [vp, cf, vp, vp, vp, ci, vp, cf, vp, ci]),
     "sgipc_convert_f64_to_f32": (ci, [vp, vp, vp, csz]),
+    "sgipc_spmat_create": (ci, [vp, ctypes.POINTER(vp), vp, vp, vp, ci, ci, ci]),
+    "sgipc_spmat_destroy": (ci, [vp]),
+    "sgipc_spmat_apply": (ci, [vp, cd, vp, cd, vp]),
+    "sgipc_spmat_apply_f32": (ci, [vp, cf, vp, cf, vp]),
     "sgipc_spmv_time": (ci, [vp, ci, vp, vp, vp, ci, ci, ci, ctypes.POINTER(cd), ctypes.POINTER(ci)]),
     "sgipc_mas_time_apply": (ci, [vp, vp, vp, ci, ctypes.POINTER(cd)]),
     "sgipc_mas_create": (ci, [vp, ctypes.POINTER(vp), ci]),
@@ -83,7 +94,7 @@ _SIGS = {
     "sgipc_mas_inverse_blocks": (ci, [vp, vp]),
     "sgipc_pcg_default_opts": (None, [ctypes.POINTER(PcgOpts)]),
     "sgipc_pcg_solve": (ci, [vp, vp, vp, vp, ci, ci, vp, vp, vp, ctypes.POINTER(PcgOpts), ctypes.POINTER(PcgStats)]),
-    "sgipc_system_create": (ci, [vp, ctypes.POINTER(vp), ctypes.POINTER(TetSceneDesc), ci]),
+    "sgipc_system_create": (ci, [vp, ctypes.POINTER(vp), ctypes.POINTER(SceneDesc), ci]),
     "sgipc_system_destroy": (ci, [vp]),
     "sgipc_system_solve": (ci, [vp, vp, vp, vp, ctypes.POINTER(PcgOpts), ctypes.POINTER(PcgStats),
                                 ctypes.POINTER(SolveTiming)]),

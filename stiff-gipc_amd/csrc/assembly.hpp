@@ -10,6 +10,14 @@ void launch_tet_snh(hipStream_t s, int n_tets, const double* x, const uint32_t* 
 void launch_barrier_pt(hipStream_t s, int n_pairs, const double* x, const int* pairs, const double* pair_x,
                        double dhat_sq, double kappa, int hess_offset, double* grad, double* tvals, int* trows,
                        int* tcols);
+void launch_tri_bw(hipStream_t s, int n_tris, const double* x, const uint32_t* tris, const double* dm_inv2,
+                   const double* area, double k_stretch, double k_shear, double strain_rate, double dt,
+                   int hess_offset, double* grad, double* tvals, int* trows, int* tcols);
+void launch_quad_bending_q(hipStream_t s, int n_edges, const double* rest, const uint32_t* edges,
+                           const uint32_t* adj, double* Q16);
+void launch_quad_bending(hipStream_t s, int n_edges, const double* x, const double* rest, const uint32_t* edges,
+                         const uint32_t* adj, const double* Q16, double k_bend, double dt, int hess_offset,
+                         double* grad, double* tvals, int* trows, int* tcols);
 void launch_mass_diag(hipStream_t s, int n, const double* mass, int hess_offset, double* tvals, int* trows,
                       int* tcols);
 

@@ -180,8 +180,57 @@ int sgipc_spmv_srbk_f32(sgipc_ctx* ctx, float a, const float* vals, const int* r
     });
 }
 
+int sgipc_spmat_create(sgipc_ctx* ctx, sgipc_spmat** out, const double* vals, const int* rows, const int* cols,
+                       int nnz, int n_rows, int want_f32)
+{
+    if (!ctx || !out) return SGIPC_BAD_ARG;
+    *out = nullptr;
+    if (nnz < 0 || n_rows < 0) return SGIPC_BAD_ARG;
+    if (nnz == 0 || n_rows == 0) return SGIPC_EMPTY;
+    if (!vals || !rows || !cols) return SGIPC_BAD_ARG;
+    auto* m = new (std::nothrow) sgipc_spmat();
+    if (!m) return SGIPC_HIP_ERROR;
+    m->ctx = ctx;
+    int st = guarded(ctx, [&] {
+        m->m.build(ctx->c.stream, vals, rows, cols, nnz, n_rows, want_f32 != 0);
+        SGIPC_HIP(hipStreamSynchronize(ctx->c.stream));
+    });
+    if (st != SGIPC_OK) {
+        delete m;
+        return st;
+    }
+    *out = m;
+    return SGIPC_OK;
+}
+
+int sgipc_spmat_destroy(sgipc_spmat* m)
+{
+    if (!m) return SGIPC_BAD_ARG;
+    int st = guarded(m->ctx, [&] { SGIPC_HIP(hipStreamSynchronize(m->ctx->c.stream)); });
+    delete m;
+    return st;
+}
+
+int sgipc_spmat_apply(sgipc_spmat* m, double a, const double* x, double b, double* y)
+{
+    if (!m || !x || !y) return SGIPC_BAD_ARG;
+    return guarded(m->ctx, [&] {
+        m->m.apply<double>(m->ctx->c.stream, a, x, b, y, nullptr, nullptr);
+        SGIPC_HIP(hipStreamSynchronize(m->ctx->c.stream));
+    });
+}
+
+int sgipc_spmat_apply_f32(sgipc_spmat* m, float a, const float* x, float b, float* y)
+{
+    if (!m || !x || !y) return SGIPC_BAD_ARG;
+    return guarded(m->ctx, [&] {
+        m->m.apply<float>(m->ctx->c.stream, a, x, b, y, nullptr, nullptr);
+        SGIPC_HIP(hipStreamSynchronize(m->ctx->c.stream));
+    });
+}
+
 int sgipc_spmv_time(sgipc_ctx* ctx, int precision, const double* vals, const int* rows, const int* cols, int nnz,
-                    int n_rows, int reps, double* avg_ms, int* full_nnz)
+                    int n_rows, int reps, double* avg_ms, int* stored_blocks)
 {
     if (!ctx || !vals || !rows || !cols || !avg_ms || reps < 1 || nnz < 1 || n_rows < 1) return SGIPC_BAD_ARG;
     if (precision != SGIPC_FP64 && precision != SGIPC_FP32_IR) return SGIPC_BAD_ARG;
@@ -189,11 +238,11 @@ int sgipc_spmv_time(sgipc_ctx* ctx, int precision, const double* vals, const int
         hipStream_t s = ctx->c.stream;
         FullMatrix A;
         A.build(s, vals, rows, cols, nnz, n_rows, precision == SGIPC_FP32_IR);
-        if (full_nnz) *full_nnz = A.nnz;
+        if (stored_blocks) *stored_blocks = (int)A.stored_blocks(s);
         const long n3 = 3L * n_rows;
         DevBuf<double> x64, y64, part;
         DevBuf<float> x32, y32;
-        part.reserve(4096);
+        part.reserve(A.partials() > 0 ? A.partials() : 1);
         if (precision == SGIPC_FP64) {
             x64.reserve(n3);
             y64.reserve(n3);
@@ -377,6 +426,7 @@ int pcg_solve_impl(Context& c, SolveWorkspace& ws, const double* vals, const int
     if (o.precision != SGIPC_FP32_IR) {
         FullOp<double> A;
         A.nnz = ws.full.nnz;
+        A.max_partials = ws.full.partials();
         A.m = &ws.full;
         MasPrecond<double> M(mas);
         PcgOptions po;
@@ -400,6 +450,7 @@ int pcg_solve_impl(Context& c, SolveWorkspace& ws, const double* vals, const int
         ws.d64.reserve(n3);
         FullOp<float> A;
         A.nnz = ws.full.nnz;
+        A.max_partials = ws.full.partials();
         A.m = &ws.full;
         MasPrecond<float> M(mas);
         launch_fill<double>(s, x, 0.0, n3);

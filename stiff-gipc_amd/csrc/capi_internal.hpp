@@ -19,6 +19,11 @@ struct sgipc_mas {
     std::unique_ptr<sgipc::Mas> m;
 };
 
+struct sgipc_spmat {
+    sgipc_ctx* ctx;
+    sgipc::FullMatrix m;
+};
+
 namespace sgipc {
 
 extern thread_local std::string g_last_error;
@@ -48,7 +53,7 @@ int guarded(sgipc_ctx* ctx, F&& f)
 
 // Solver workspace (PCG buffers, FP32 matrix copy for refinement).
 struct SolveWorkspace {
-    FullMatrix full;  // both triangles, row-owned SpMV of the PCG loop
+    FullMatrix full;  // tiled symmetric layout of the PCG loop's SpMV
     PcgSolver<double> pcg64;
     PcgSolver<float> pcg32;
     DevBuf<float> r32, d32;

@@ -255,7 +255,7 @@ void launch_residual(hipStream_t s, const double* vals, const int* rows, const i
 
 // ------------------------------------------------------------------ solver
 template <typename TV>
-void PcgSolver<TV>::reserve(long n3, int nnz)
+void PcgSolver<TV>::reserve(long n3, int nnz, int max_partials)
 {
     x_.reserve(n3);
     r_.reserve(n3);
@@ -263,7 +263,7 @@ void PcgSolver<TV>::reserve(long n3, int nnz)
     p_.reserve(n3);
     Ap_.reserve(n3);
     int g = vec_grid(n3);
-    int gs = spmv_partials_needed(nnz > 0 ? nnz : 1);
+    int gs = max_partials > 0 ? max_partials : spmv_partials_needed(nnz > 0 ? nnz : 1);
     int gmax = g > gs ? g : gs;
     if (gmax < 1024) gmax = 1024;
     part_rz_[0].reserve(gmax);
@@ -288,7 +288,7 @@ PcgResult PcgSolver<TV>::solve(hipStream_t s, const SpmvOp<TV>& A, Preconditione
                                const TV* b, TV* x_out, long n3, const PcgOptions& opt)
 {
     PcgResult res{};
-    reserve(n3, A.nnz);
+    reserve(n3, A.nnz, A.max_partials);
     if (!ev_[0]) {
         SGIPC_HIP(hipEventCreateWithFlags(&ev_[0], hipEventDisableTiming));
         SGIPC_HIP(hipEventCreateWithFlags(&ev_[1], hipEventDisableTiming));

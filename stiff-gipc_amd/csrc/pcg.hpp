@@ -49,6 +49,7 @@ struct PcgResult {
 template <typename TV>
 struct SpmvOp {
     int nnz = 0;
+    int max_partials = 0;  // partials apply_dot may write
     virtual ~SpmvOp() = default;
     virtual int apply_dot(hipStream_t s, const TV* p, TV* Ap, double* pap_part, const PcgCtl* ctl) const = 0;
 };
@@ -68,7 +69,7 @@ class PcgSolver {
     ~PcgSolver();
     PcgResult solve(hipStream_t s, const SpmvOp<TV>& A, Preconditioner<TV>* M, const TV* b, TV* x,
                     long n3, const PcgOptions& opt);
-    void reserve(long n3, int nnz);
+    void reserve(long n3, int nnz, int max_partials);
 
   private:
     DevBuf<TV> x_, r_, z_, p_, Ap_;

@@ -1,20 +1,29 @@
-// SELL-64-sigma symmetric SpMV for the PCG loop on gfx950.
+// Tile-symmetric SELL SpMV for the PCG loop on gfx950.
 //
 // The reference streams the upper BCOO and scatters the transposed half with
 // atomics (Spmv::srbk_spmv, StiffGIPC/linear_system/utils/spmv.cu:188-313).
-// For the hundreds of SpMVs of one solve the matrix is instead expanded once
-// into both triangles (the transposed half ordered by a stable (col,row)
-// sort, so the layout is deterministic) and packed as SELL-64-sigma:
-//   * rows are stably sorted by length (longest first), 64 rows per slice;
-//     the elastic rows all have 15 blocks, so slices are almost unpadded and
-//     the surface rows that carry contact blocks gather in a few wide slices;
-//   * a slice's blocks are stored slot by slot, component-major: value c of
-//     slot s of lane l sits at ((S_q + s) * 9 + c) * 64 + l, so each of the 9
-//     loads of a slot is one contiguous 512-byte wavefront access;
-//   * one lane owns one row and accumulates it in registers: no LDS, no
-//     shuffles, no atomics, no zero-fill of y; every row is written once.
-// x^T(Ax) partials for PCG are formed on the way.
+// Here the matrix is re-laid out once per Newton step so that each upper
+// block is read from HBM once per SpMV without atomics:
+//
+//   * rows are cut into tiles of kTile consecutive (Morton-ordered) rows; one
+//     workgroup owns a tile and writes every row of it exactly once;
+//   * an upper block (i, j) with both ends in the same tile is stored once, in
+//     row i.  Its H x_j goes to row i in registers, its H^T x_i to a slot of
+//     the tile's LDS array, numbered in (j, i) order, that row j adds up after
+//     a barrier (fixed order, so the result is deterministic);
+//   * a block whose ends lie in different tiles is stored twice, as H in row i
+//     and as H^T in row j (about a fifth of the blocks on Morton-ordered
+//     meshes with 512-row tiles);
+//   * inside a tile the rows are sorted by entry count and packed as SELL-64:
+//     slot s of lane l holds value c at ((S_q + s) * 9 + c) * 64 + l, so each
+//     of the 9 value loads of a slot is one contiguous 512-byte access; a
+//     lane past its row's end reads lane 0's address of the same slot (already
+//     fetched) and masks the product, so padding costs no HBM bytes;
+//   * two waves share a slice, each taking half of its slots, and combine the
+//     two partial rows through LDS.
+// Per tile: kTile rows, kSplit * kTile threads; x^T(Ax) partials per tile.
 #include <cstring>
+#include <vector>
 
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
@@ -26,83 +35,140 @@ namespace sgipc {
 
 namespace {
 
-constexpr int kWaves = 4;
-constexpr int kThreads = 64 * kWaves;
-constexpr int kMaxGrid = 2048;
+constexpr int kTile = 512;                 // rows per workgroup
+constexpr int kSplit = 2;                  // waves per slice
+constexpr int kThreads = kTile * kSplit;   // 1024
+constexpr int kTcap = 4096;                // in-tile transposed slots per tile (LDS)
+constexpr uint32_t kInTile = 0x80000000u;
 
 template <typename TM, typename TV, bool kDot>
-__global__ __launch_bounds__(kThreads) void k_spmv_sell(const TM* __restrict__ vals, const int* __restrict__ cols,
+__global__ __launch_bounds__(kThreads) void k_spmv_tsym(const TM* __restrict__ vals, const uint32_t* __restrict__ cw,
                                                         const int* __restrict__ slice_ptr,
                                                         const int* __restrict__ slice_w, const int* __restrict__ perm,
-                                                        int n_rows, int n_slices, TV a, const TV* __restrict__ x,
-                                                        TV b, TV* __restrict__ y, double* __restrict__ dot_part,
-                                                        const int* __restrict__ done)
+                                                        const int* __restrict__ plen,
+                                                        const uint32_t* __restrict__ tinfo, TV a,
+                                                        const TV* __restrict__ x, TV b, TV* __restrict__ y,
+                                                        double* __restrict__ dot_part, const int* __restrict__ done)
 {
-    __shared__ double red[16];
+    __shared__ TV tl[kTcap * 3];
+    __shared__ TV red[3 * kTile];
+    __shared__ double dred[16];
     if (done != nullptr && *done) return;
-    const int lane = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
-    double dot = 0.0;
-    for (int q = blockIdx.x * kWaves + w; q < n_slices; q += gridDim.x * kWaves) {
-        const int p = q * 64 + lane;
-        const int i = p < n_rows ? perm[p] : -1;
-        const long S = slice_ptr[q];
-        const int W = slice_w[q];
-        TV acc0 = 0, acc1 = 0, acc2 = 0;
-        const int* cp = cols + S * 64 + lane;
-        const TM* vp = vals + S * 9 * 64 + lane;
-        int s = 0;
-        for (; s + 2 <= W; s += 2) {
-            const int j0 = cp[(long)s * 64];
-            const int j1 = cp[(long)(s + 1) * 64];
-            TM h0[9], h1[9];
+    const int tile = blockIdx.x;
+    const int part = threadIdx.x / kTile;
+    const int r = threadIdx.x % kTile;
+    const int lane = r & 63;
+    const int q = tile * (kTile / 64) + (r >> 6);
+    const int p = q * 64 + lane;
+    const int i = perm[p];
+    const int len = plen[p];
+    const long base = (long)tile * kTile;
+    TV xi0 = 0, xi1 = 0, xi2 = 0;
+    if (i >= 0) {
+        xi0 = x[3 * (long)i];
+        xi1 = x[3 * (long)i + 1];
+        xi2 = x[3 * (long)i + 2];
+    }
+    const long S = slice_ptr[q];
+    const int W = slice_w[q];
+    const int s1 = (W * (part + 1)) / kSplit;
+    int s = (W * part) / kSplit;
+    TV a0 = 0, a1 = 0, a2 = 0;
+    const uint32_t* cp = cw + S * 64;
+    const TM* vp = vals + S * 9 * 64;
+    for (; s + 2 <= s1; s += 2) {
+        const bool act0 = s < len, act1 = s + 1 < len;
+        const int l0 = act0 ? lane : 0, l1 = act1 ? lane : 0;
+        const uint32_t w0 = cp[(long)s * 64 + l0], w1 = cp[(long)(s + 1) * 64 + l1];
+        TM h[9], g[9];
 #pragma unroll
-            for (int c = 0; c < 9; ++c) {
-                h0[c] = vp[((long)s * 9 + c) * 64];
-                h1[c] = vp[((long)(s + 1) * 9 + c) * 64];
-            }
-            const TV a0 = x[3 * (long)j0], a1 = x[3 * (long)j0 + 1], a2 = x[3 * (long)j0 + 2];
-            const TV b0 = x[3 * (long)j1], b1 = x[3 * (long)j1 + 1], b2 = x[3 * (long)j1 + 2];
-            acc0 += (TV)h0[0] * a0 + (TV)h0[3] * a1 + (TV)h0[6] * a2;
-            acc1 += (TV)h0[1] * a0 + (TV)h0[4] * a1 + (TV)h0[7] * a2;
-            acc2 += (TV)h0[2] * a0 + (TV)h0[5] * a1 + (TV)h0[8] * a2;
-            acc0 += (TV)h1[0] * b0 + (TV)h1[3] * b1 + (TV)h1[6] * b2;
-            acc1 += (TV)h1[1] * b0 + (TV)h1[4] * b1 + (TV)h1[7] * b2;
-            acc2 += (TV)h1[2] * b0 + (TV)h1[5] * b1 + (TV)h1[8] * b2;
+        for (int c = 0; c < 9; ++c) {
+            h[c] = vp[((long)s * 9 + c) * 64 + l0];
+            g[c] = vp[((long)(s + 1) * 9 + c) * 64 + l1];
         }
-        if (s < W) {
-            const int j0 = cp[(long)s * 64];
-            TM h0[9];
-#pragma unroll
-            for (int c = 0; c < 9; ++c) h0[c] = vp[((long)s * 9 + c) * 64];
-            const TV a0 = x[3 * (long)j0], a1 = x[3 * (long)j0 + 1], a2 = x[3 * (long)j0 + 2];
-            acc0 += (TV)h0[0] * a0 + (TV)h0[3] * a1 + (TV)h0[6] * a2;
-            acc1 += (TV)h0[1] * a0 + (TV)h0[4] * a1 + (TV)h0[7] * a2;
-            acc2 += (TV)h0[2] * a0 + (TV)h0[5] * a1 + (TV)h0[8] * a2;
+        const bool in0 = act0 && (w0 & kInTile), in1 = act1 && (w1 & kInTile);
+        const long j0 = (w0 & kInTile) ? base + ((w0 >> 16) & 0x7FFF) : (long)w0;
+        const long j1 = (w1 & kInTile) ? base + ((w1 >> 16) & 0x7FFF) : (long)w1;
+        const TV m0 = act0 ? TV(1) : TV(0), m1 = act1 ? TV(1) : TV(0);
+        const TV x0 = m0 * x[3 * j0], x1 = m0 * x[3 * j0 + 1], x2 = m0 * x[3 * j0 + 2];
+        const TV z0 = m1 * x[3 * j1], z1 = m1 * x[3 * j1 + 1], z2 = m1 * x[3 * j1 + 2];
+        a0 += (TV)h[0] * x0 + (TV)h[3] * x1 + (TV)h[6] * x2;
+        a1 += (TV)h[1] * x0 + (TV)h[4] * x1 + (TV)h[7] * x2;
+        a2 += (TV)h[2] * x0 + (TV)h[5] * x1 + (TV)h[8] * x2;
+        a0 += (TV)g[0] * z0 + (TV)g[3] * z1 + (TV)g[6] * z2;
+        a1 += (TV)g[1] * z0 + (TV)g[4] * z1 + (TV)g[7] * z2;
+        a2 += (TV)g[2] * z0 + (TV)g[5] * z1 + (TV)g[8] * z2;
+        if (in0) {
+            const int ts = w0 & 0xFFFF;
+            tl[3 * ts] = (TV)h[0] * xi0 + (TV)h[1] * xi1 + (TV)h[2] * xi2;
+            tl[3 * ts + 1] = (TV)h[3] * xi0 + (TV)h[4] * xi1 + (TV)h[5] * xi2;
+            tl[3 * ts + 2] = (TV)h[6] * xi0 + (TV)h[7] * xi1 + (TV)h[8] * xi2;
         }
-        if (i >= 0) {
-            TV* yi = y + 3 * (long)i;
-            if (b == TV(0)) {
-                yi[0] = a * acc0;
-                yi[1] = a * acc1;
-                yi[2] = a * acc2;
-            } else {
-                yi[0] = a * acc0 + b * yi[0];
-                yi[1] = a * acc1 + b * yi[1];
-                yi[2] = a * acc2 + b * yi[2];
-            }
-            if (kDot)
-                dot += (double)x[3 * (long)i] * (double)acc0 + (double)x[3 * (long)i + 1] * (double)acc1 +
-                       (double)x[3 * (long)i + 2] * (double)acc2;
+        if (in1) {
+            const int ts = w1 & 0xFFFF;
+            tl[3 * ts] = (TV)g[0] * xi0 + (TV)g[1] * xi1 + (TV)g[2] * xi2;
+            tl[3 * ts + 1] = (TV)g[3] * xi0 + (TV)g[4] * xi1 + (TV)g[5] * xi2;
+            tl[3 * ts + 2] = (TV)g[6] * xi0 + (TV)g[7] * xi1 + (TV)g[8] * xi2;
         }
     }
+    if (s < s1) {
+        const bool act0 = s < len;
+        const int l0 = act0 ? lane : 0;
+        const uint32_t w0 = cp[(long)s * 64 + l0];
+        TM h[9];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) h[c] = vp[((long)s * 9 + c) * 64 + l0];
+        const bool in0 = act0 && (w0 & kInTile);
+        const long j0 = (w0 & kInTile) ? base + ((w0 >> 16) & 0x7FFF) : (long)w0;
+        const TV m0 = act0 ? TV(1) : TV(0);
+        const TV x0 = m0 * x[3 * j0], x1 = m0 * x[3 * j0 + 1], x2 = m0 * x[3 * j0 + 2];
+        a0 += (TV)h[0] * x0 + (TV)h[3] * x1 + (TV)h[6] * x2;
+        a1 += (TV)h[1] * x0 + (TV)h[4] * x1 + (TV)h[7] * x2;
+        a2 += (TV)h[2] * x0 + (TV)h[5] * x1 + (TV)h[8] * x2;
+        if (in0) {
+            const int ts = w0 & 0xFFFF;
+            tl[3 * ts] = (TV)h[0] * xi0 + (TV)h[1] * xi1 + (TV)h[2] * xi2;
+            tl[3 * ts + 1] = (TV)h[3] * xi0 + (TV)h[4] * xi1 + (TV)h[5] * xi2;
+            tl[3 * ts + 2] = (TV)h[6] * xi0 + (TV)h[7] * xi1 + (TV)h[8] * xi2;
+        }
+    }
+    if (part != 0) {
+        red[r] = a0;
+        red[kTile + r] = a1;
+        red[2 * kTile + r] = a2;
+    }
+    __syncthreads();
+    double dot = 0.0;
+    if (part == 0 && i >= 0) {
+        a0 += red[r];
+        a1 += red[kTile + r];
+        a2 += red[2 * kTile + r];
+        const uint32_t ti = tinfo[i];
+        const int tb = (int)(ti >> 16), tc = (int)(ti & 0xFFFF);
+        for (int k = tb; k < tb + tc; ++k) {
+            a0 += tl[3 * k];
+            a1 += tl[3 * k + 1];
+            a2 += tl[3 * k + 2];
+        }
+        TV* yi = y + 3 * (long)i;
+        if (b == TV(0)) {
+            yi[0] = a * a0;
+            yi[1] = a * a1;
+            yi[2] = a * a2;
+        } else {
+            yi[0] = a * a0 + b * yi[0];
+            yi[1] = a * a1 + b * yi[1];
+            yi[2] = a * a2 + b * yi[2];
+        }
+        if (kDot) dot = (double)xi0 * (double)a0 + (double)xi1 * (double)a1 + (double)xi2 * (double)a2;
+    }
     if (kDot) {
-        double sblk = block_sum(dot, red);
-        if (threadIdx.x == 0) dot_part[blockIdx.x] = sblk;
+        const double sblk = block_sum(dot, dred);
+        if (threadIdx.x == 0) dot_part[tile] = sblk;
     }
 }
 
-// ---- expansion kernels (upper BCOO -> both triangles, rows sorted by column)
+// ---- layout build -------------------------------------------------------
 __global__ void k_colrow_keys(const int* __restrict__ rows, const int* __restrict__ cols, int nnz, int kbits,
                               uint64_t* __restrict__ keys, int* __restrict__ idx)
 {
@@ -112,7 +178,7 @@ __global__ void k_colrow_keys(const int* __restrict__ rows, const int* __restric
     idx[k] = k;
 }
 
-// up_ptr[i] = first upper block of row i; col_ptr[j] = first sorted entry of column j.
+// up_ptr[i] = first upper block of row i; col_ptr[j] = first col-sorted entry of column j.
 __global__ void k_ptrs(const int* __restrict__ rows, const uint64_t* __restrict__ skeys, int nnz, int n, int kbits,
                        int* __restrict__ up_ptr, int* __restrict__ col_ptr)
 {
@@ -120,7 +186,7 @@ __global__ void k_ptrs(const int* __restrict__ rows, const uint64_t* __restrict_
     if (i > n) return;
     int lo = 0, hi = nnz;
     while (lo < hi) {
-        int mid = (lo + hi) >> 1;
+        const int mid = (lo + hi) >> 1;
         if (rows[mid] < i) lo = mid + 1; else hi = mid;
     }
     up_ptr[i] = lo;
@@ -128,125 +194,139 @@ __global__ void k_ptrs(const int* __restrict__ rows, const uint64_t* __restrict_
     lo = 0;
     hi = nnz;
     while (lo < hi) {
-        int mid = (lo + hi) >> 1;
+        const int mid = (lo + hi) >> 1;
         if (skeys[mid] < key) lo = mid + 1; else hi = mid;
     }
     col_ptr[i] = lo;
 }
 
-// strict lower count of row i = entries of column i with row < i
-__global__ void k_row_len(const int* __restrict__ up_ptr, const int* __restrict__ col_ptr,
-                          const uint64_t* __restrict__ skeys, int n, int kbits, int* __restrict__ low_cnt,
-                          int* __restrict__ full_len)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int cb = col_ptr[i], ce = col_ptr[i + 1];
-    int low = ce - cb;
-    if (low > 0) {
-        const uint64_t last = skeys[ce - 1];
-        const int r = (int)(last & ((1ull << kbits) - 1ull));
-        if (r == i) --low;  // the diagonal block closes its column
-    }
-    low_cnt[i] = low;
-    full_len[i] = low + (up_ptr[i + 1] - up_ptr[i]);
-}
-
-__device__ __forceinline__ void copy9(const double* __restrict__ s, double* __restrict__ d, bool tr)
-{
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) d[c * 3 + r] = tr ? s[r * 3 + c] : s[c * 3 + r];
-}
-
-__global__ void k_scatter_upper(const double* __restrict__ vals, const int* __restrict__ rows,
-                                const int* __restrict__ cols, int nnz, const int* __restrict__ up_ptr,
-                                const int* __restrict__ low_cnt, const int* __restrict__ full_ptr,
-                                double* __restrict__ fv, int* __restrict__ fc)
-{
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nnz) return;
-    const int i = rows[k];
-    const int pos = full_ptr[i] + low_cnt[i] + (k - up_ptr[i]);
-    copy9(vals + 9 * (long)k, fv + 9 * (long)pos, false);
-    fc[pos] = cols[k];
-}
-
-__global__ void k_scatter_lower(const double* __restrict__ vals, const uint64_t* __restrict__ skeys,
-                                const int* __restrict__ sidx, int nnz, int kbits, const int* __restrict__ col_ptr,
-                                const int* __restrict__ full_ptr, double* __restrict__ fv, int* __restrict__ fc)
+// in-tile candidate flag per col-sorted entry (i != j, same tile)
+__global__ void k_cand(const uint64_t* __restrict__ skeys, int nnz, int kbits, int* __restrict__ cand)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nnz) return;
     const uint64_t key = skeys[s];
     const int j = (int)(key >> kbits);
-    const int k = (int)(key & ((1ull << kbits) - 1ull));
-    if (k == j) return;  // diagonal block: placed by the upper scatter
-    const int pos = full_ptr[j] + (s - col_ptr[j]);
-    copy9(vals + 9 * (long)sidx[s], fv + 9 * (long)pos, true);
-    fc[pos] = k;
+    const int i = (int)(key & ((1ull << kbits) - 1ull));
+    cand[s] = (i != j && i / kTile == j / kTile) ? 1 : 0;
 }
 
-// ---- SELL packing
-__global__ void k_len_keys(const int* __restrict__ row_ptr, int n, uint32_t* __restrict__ key, int* __restrict__ iota)
+// Per column j: the upper word of every block (i, j), the in-tile slot range of
+// column j and the count of lower (cross-tile) entries row j stores.
+// A candidate whose rank in its tile reaches kTcap is stored as cross-tile.
+__global__ void k_col_words(const uint64_t* __restrict__ skeys, const int* __restrict__ sidx,
+                            const int* __restrict__ cscan, const int* __restrict__ col_ptr, int n, int kbits,
+                            uint32_t* __restrict__ up_word, uint32_t* __restrict__ tinfo, int* __restrict__ low_cnt)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int tile0 = (j / kTile) * kTile;
+    const int rank_base = cscan[col_ptr[tile0]];
+    int tb = -1, tcount = 0, low = 0;
+    for (int s = col_ptr[j]; s < col_ptr[j + 1]; ++s) {
+        const int i = (int)(skeys[s] & ((1ull << kbits) - 1ull));
+        const int b = sidx[s];
+        if (i == j) {
+            up_word[b] = (uint32_t)j;
+            continue;
+        }
+        const bool same = i / kTile == j / kTile;
+        const int rank = cscan[s] - rank_base;
+        if (same && rank < kTcap) {
+            up_word[b] = kInTile | ((uint32_t)(j - tile0) << 16) | (uint32_t)rank;
+            if (tb < 0) tb = rank;
+            ++tcount;
+        } else {
+            up_word[b] = (uint32_t)j;
+            ++low;
+        }
+    }
+    tinfo[j] = ((uint32_t)(tb < 0 ? 0 : tb) << 16) | (uint32_t)tcount;
+    low_cnt[j] = low;
+}
+
+__global__ void k_row_keys(const int* __restrict__ up_ptr, const int* __restrict__ low_cnt, int n,
+                           int* __restrict__ len, uint64_t* __restrict__ key, int* __restrict__ iota)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int len = row_ptr[i + 1] - row_ptr[i];
-    key[i] = 0xFFFFu - (uint32_t)(len < 0xFFFF ? len : 0xFFFF);  // ascending key = descending length
+    const int L = (up_ptr[i + 1] - up_ptr[i]) + low_cnt[i];
+    len[i] = L;
+    key[i] = ((uint64_t)(i / kTile) << 16) | (uint64_t)(0xFFFFu - (uint32_t)(L < 0xFFFF ? L : 0xFFFF));
     iota[i] = i;
 }
 
-__global__ void k_slice_w(const int* __restrict__ perm, const int* __restrict__ row_ptr, int n, int n_slices,
-                          int* __restrict__ slice_w)
+// perm/plen over all tile positions (rows past n are padding: -1 / 0)
+__global__ void k_perm_fill(const int* __restrict__ sorted, const int* __restrict__ len, int n, int n_pos,
+                            int* __restrict__ perm, int* __restrict__ plen)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_pos) return;
+    const int i = p < n ? sorted[p] : -1;
+    perm[p] = i;
+    plen[p] = i >= 0 ? len[i] : 0;
+}
+
+__global__ void k_slice_w(const int* __restrict__ plen, int n_slices, int* __restrict__ slice_w)
 {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q > n_slices) return;
-    if (q == n_slices) {
-        slice_w[q] = 0;  // scan sentinel
-        return;
-    }
-    const int i = perm[q * 64];  // longest row of the slice
-    slice_w[q] = row_ptr[i + 1] - row_ptr[i];
+    slice_w[q] = q == n_slices ? 0 : plen[q * 64];  // longest row of the slice (sorted in its tile)
 }
 
-template <bool kF32>
-__global__ void k_sell_pack(const double* __restrict__ fv, const int* __restrict__ fc,
-                            const int* __restrict__ row_ptr, const int* __restrict__ perm, int n,
-                            const int* __restrict__ slice_ptr, const int* __restrict__ slice_w,
-                            double* __restrict__ sv, float* __restrict__ sv32, int* __restrict__ sc)
+__device__ __forceinline__ void put9(const double* __restrict__ s, bool tr, long slot, int lane,
+                                     double* __restrict__ sv, float* __restrict__ sv32)
+{
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double v = tr ? s[r * 3 + c] : s[c * 3 + r];
+            sv[(slot * 9 + c * 3 + r) * 64 + lane] = v;
+            if (sv32) sv32[(slot * 9 + c * 3 + r) * 64 + lane] = (float)v;
+        }
+}
+
+// One thread per tile position: the row's upper blocks (diagonal first, by
+// column), then its cross-tile lower entries (by row), then zero padding.
+__global__ void k_pack(const double* __restrict__ vals, const uint32_t* __restrict__ up_word,
+                       const uint64_t* __restrict__ skeys, const int* __restrict__ sidx,
+                       const int* __restrict__ up_ptr, const int* __restrict__ col_ptr, const int* __restrict__ perm,
+                       const int* __restrict__ slice_ptr, const int* __restrict__ slice_w, int n_pos, int kbits,
+                       double* __restrict__ sv, float* __restrict__ sv32, uint32_t* __restrict__ scw)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_pos) return;
     const int q = p >> 6, lane = p & 63;
-    if (q * 64 >= ((n + 63) / 64) * 64) return;
-    const int i = p < n ? perm[p] : -1;
-    const int beg = i >= 0 ? row_ptr[i] : 0;
-    const int len = i >= 0 ? row_ptr[i + 1] - beg : 0;
+    const int i = perm[p];
     const long S = slice_ptr[q];
     const int W = slice_w[q];
-    for (int s = 0; s < W; ++s) {
-        const long slot = S + s;
-        const bool real = s < len;
-        sc[slot * 64 + lane] = real ? fc[beg + s] : (i >= 0 ? i : 0);
+    int k = 0;
+    if (i >= 0) {
+        for (int b = up_ptr[i]; b < up_ptr[i + 1]; ++b, ++k) {
+            scw[(S + k) * 64 + lane] = up_word[b];
+            put9(vals + 9 * (long)b, false, S + k, lane, sv, sv32);
+        }
+        for (int s = col_ptr[i]; s < col_ptr[i + 1]; ++s) {
+            const int r = (int)(skeys[s] & ((1ull << kbits) - 1ull));
+            const int b = sidx[s];
+            if (r == i || (up_word[b] & kInTile)) continue;
+            scw[(S + k) * 64 + lane] = (uint32_t)r;
+            put9(vals + 9 * (long)b, true, S + k, lane, sv, sv32);
+            ++k;
+        }
+    }
+    for (; k < W; ++k) {
+        scw[(S + k) * 64 + lane] = (uint32_t)(i >= 0 ? i : 0);
 #pragma unroll
         for (int c = 0; c < 9; ++c) {
-            const double v = real ? fv[9 * (long)(beg + s) + c] : 0.0;
-            sv[(slot * 9 + c) * 64 + lane] = v;
-            if (kF32) sv32[(slot * 9 + c) * 64 + lane] = (float)v;
+            sv[((S + k) * 9 + c) * 64 + lane] = 0.0;
+            if (sv32) sv32[((S + k) * 9 + c) * 64 + lane] = 0.0f;
         }
     }
 }
 
 inline int g256(long n) { return (int)((n + 255) / 256); }
-
-int grid_sell(int n_slices)
-{
-    int g = (n_slices + kWaves - 1) / kWaves;
-    if (g > kMaxGrid) g = kMaxGrid;
-    if (g < 1) g = 1;
-    return g;
-}
 
 }  // namespace
 
@@ -255,37 +335,43 @@ void FullMatrix::build(hipStream_t s, const double* vals, const int* rows, const
 {
     SGIPC_CHECK(nnz_up > 0 && n > 0, SGIPC_EMPTY, "empty matrix");
     n_rows = n;
+    nnz = nnz_up;
     has_f32 = want_f32;
     int kbits = 1;
     while (kbits < 31 && (1L << kbits) < (long)n) ++kbits;
+    n_tiles = (n + kTile - 1) / kTile;
+    const int n_pos = n_tiles * kTile;
+    n_slices = n_pos / 64;
     keys_.reserve(nnz_up);
     skeys_.reserve(nnz_up);
     idx_.reserve(nnz_up);
     sidx_.reserve(nnz_up);
+    cand_.reserve(nnz_up + 1);
+    cscan_.reserve(nnz_up + 1);
+    up_word_.reserve(nnz_up);
     up_ptr_.reserve(n + 1);
     col_ptr_.reserve(n + 1);
     low_cnt_.reserve(n);
-    full_len_.reserve(n + 1);
-    row_ptr_.reserve(n + 1);
-    len_key_.reserve(n);
-    len_key_sorted_.reserve(n);
+    len_.reserve(n);
+    rkey_.reserve(n);
+    rkey_sorted_.reserve(n);
     iota_.reserve(n);
-    perm.reserve(n);
-    n_slices = (n + 63) / 64;
+    sorted_.reserve(n);
+    perm.reserve(n_pos);
+    plen.reserve(n_pos);
+    tinfo.reserve(n);
     slice_w.reserve(n_slices + 1);
     slice_ptr.reserve(n_slices + 1);
 
-    // ---- expansion
-    k_colrow_keys<<<g256(nnz_up), 256, 0, s>>>(rows, cols, nnz_up, kbits, keys_.get(), idx_.get());
-    SGIPC_LAUNCH_CHECK();
+    int tbits = 1;
+    while (tbits < 40 && (1L << tbits) < (long)n_tiles) ++tbits;
     size_t need = 0, need2 = 0, need3 = 0, need4 = 0;
     SGIPC_HIP(rocprim::radix_sort_pairs(nullptr, need, keys_.get(), skeys_.get(), idx_.get(), sidx_.get(),
                                         (size_t)nnz_up, 0, 2 * kbits, s));
-    SGIPC_HIP(rocprim::exclusive_scan(nullptr, need2, full_len_.get(), row_ptr_.get(), 0, (size_t)n + 1,
+    SGIPC_HIP(rocprim::exclusive_scan(nullptr, need2, cand_.get(), cscan_.get(), 0, (size_t)nnz_up + 1,
                                       rocprim::plus<int>(), s));
-    SGIPC_HIP(rocprim::radix_sort_pairs(nullptr, need3, reinterpret_cast<uint32_t*>(len_key_.get()),
-                                        reinterpret_cast<uint32_t*>(len_key_sorted_.get()), iota_.get(), perm.get(),
-                                        (size_t)n, 0, 16, s));
+    SGIPC_HIP(rocprim::radix_sort_pairs(nullptr, need3, rkey_.get(), rkey_sorted_.get(), iota_.get(), sorted_.get(),
+                                        (size_t)n, 0, 16 + tbits, s));
     SGIPC_HIP(rocprim::exclusive_scan(nullptr, need4, slice_w.get(), slice_ptr.get(), 0, (size_t)n_slices + 1,
                                       rocprim::plus<int>(), s));
     size_t nt = need;
@@ -293,86 +379,87 @@ void FullMatrix::build(hipStream_t s, const double* vals, const int* rows, const
     if (need3 > nt) nt = need3;
     if (need4 > nt) nt = need4;
     temp_.reserve(nt);
+
+    // column-major order of the upper blocks
+    k_colrow_keys<<<g256(nnz_up), 256, 0, s>>>(rows, cols, nnz_up, kbits, keys_.get(), idx_.get());
+    SGIPC_LAUNCH_CHECK();
     SGIPC_HIP(rocprim::radix_sort_pairs(temp_.get(), need, keys_.get(), skeys_.get(), idx_.get(), sidx_.get(),
                                         (size_t)nnz_up, 0, 2 * kbits, s));
     k_ptrs<<<g256(n + 1), 256, 0, s>>>(rows, skeys_.get(), nnz_up, n, kbits, up_ptr_.get(), col_ptr_.get());
     SGIPC_LAUNCH_CHECK();
-    k_row_len<<<g256(n), 256, 0, s>>>(up_ptr_.get(), col_ptr_.get(), skeys_.get(), n, kbits, low_cnt_.get(),
-                                      full_len_.get());
+    // in-tile ranks: exclusive scan of the candidate flags in column order
+    k_cand<<<g256(nnz_up), 256, 0, s>>>(skeys_.get(), nnz_up, kbits, cand_.get());
     SGIPC_LAUNCH_CHECK();
-    SGIPC_HIP(hipMemsetAsync(full_len_.get() + n, 0, sizeof(int), s));
-    SGIPC_HIP(rocprim::exclusive_scan(temp_.get(), need2, full_len_.get(), row_ptr_.get(), 0, (size_t)n + 1,
+    SGIPC_HIP(hipMemsetAsync(cand_.get() + nnz_up, 0, sizeof(int), s));
+    SGIPC_HIP(rocprim::exclusive_scan(temp_.get(), need2, cand_.get(), cscan_.get(), 0, (size_t)nnz_up + 1,
                                       rocprim::plus<int>(), s));
-    int h_nnz = 0;
-    SGIPC_HIP(hipMemcpyAsync(&h_nnz, row_ptr_.get() + n, sizeof(int), hipMemcpyDeviceToHost, s));
-    SGIPC_HIP(hipStreamSynchronize(s));
-    nnz = h_nnz;
-    fvals_.reserve((size_t)nnz * 9);
-    fcols_.reserve(nnz);
-    k_scatter_upper<<<g256(nnz_up), 256, 0, s>>>(vals, rows, cols, nnz_up, up_ptr_.get(), low_cnt_.get(),
-                                                 row_ptr_.get(), fvals_.get(), fcols_.get());
+    k_col_words<<<g256(n), 256, 0, s>>>(skeys_.get(), sidx_.get(), cscan_.get(), col_ptr_.get(), n, kbits,
+                                        up_word_.get(), tinfo.get(), low_cnt_.get());
     SGIPC_LAUNCH_CHECK();
-    k_scatter_lower<<<g256(nnz_up), 256, 0, s>>>(vals, skeys_.get(), sidx_.get(), nnz_up, kbits, col_ptr_.get(),
-                                                 row_ptr_.get(), fvals_.get(), fcols_.get());
+    // rows sorted by entry count inside each tile
+    k_row_keys<<<g256(n), 256, 0, s>>>(up_ptr_.get(), low_cnt_.get(), n, len_.get(), rkey_.get(), iota_.get());
     SGIPC_LAUNCH_CHECK();
-
-    // ---- SELL-64-sigma packing
-    k_len_keys<<<g256(n), 256, 0, s>>>(row_ptr_.get(), n, reinterpret_cast<uint32_t*>(len_key_.get()), iota_.get());
+    SGIPC_HIP(rocprim::radix_sort_pairs(temp_.get(), need3, rkey_.get(), rkey_sorted_.get(), iota_.get(),
+                                        sorted_.get(), (size_t)n, 0, 16 + tbits, s));
+    k_perm_fill<<<g256(n_pos), 256, 0, s>>>(sorted_.get(), len_.get(), n, n_pos, perm.get(), plen.get());
     SGIPC_LAUNCH_CHECK();
-    SGIPC_HIP(rocprim::radix_sort_pairs(temp_.get(), need3, reinterpret_cast<uint32_t*>(len_key_.get()),
-                                        reinterpret_cast<uint32_t*>(len_key_sorted_.get()), iota_.get(), perm.get(),
-                                        (size_t)n, 0, 16, s));
-    k_slice_w<<<g256(n_slices + 1), 256, 0, s>>>(perm.get(), row_ptr_.get(), n, n_slices, slice_w.get());
+    k_slice_w<<<g256(n_slices + 1), 256, 0, s>>>(plen.get(), n_slices, slice_w.get());
     SGIPC_LAUNCH_CHECK();
     SGIPC_HIP(rocprim::exclusive_scan(temp_.get(), need4, slice_w.get(), slice_ptr.get(), 0, (size_t)n_slices + 1,
                                       rocprim::plus<int>(), s));
     int h_slots = 0;
     SGIPC_HIP(hipMemcpyAsync(&h_slots, slice_ptr.get() + n_slices, sizeof(int), hipMemcpyDeviceToHost, s));
     SGIPC_HIP(hipStreamSynchronize(s));
+    SGIPC_CHECK(h_slots >= 0, SGIPC_CAPACITY, "SELL slot count overflows int");
     n_slots = h_slots;
     sell_vals.reserve((size_t)n_slots * 64 * 9);
     sell_cols.reserve((size_t)n_slots * 64);
     if (want_f32) sell_vals32.reserve((size_t)n_slots * 64 * 9);
-    const long threads = (long)n_slices * 64;
-    if (want_f32)
-        k_sell_pack<true><<<g256(threads), 256, 0, s>>>(fvals_.get(), fcols_.get(), row_ptr_.get(), perm.get(), n,
-                                                        slice_ptr.get(), slice_w.get(), sell_vals.get(),
-                                                        sell_vals32.get(), sell_cols.get());
-    else
-        k_sell_pack<false><<<g256(threads), 256, 0, s>>>(fvals_.get(), fcols_.get(), row_ptr_.get(), perm.get(), n,
-                                                         slice_ptr.get(), slice_w.get(), sell_vals.get(), nullptr,
-                                                         sell_cols.get());
+    k_pack<<<g256(n_pos), 256, 0, s>>>(vals, up_word_.get(), skeys_.get(), sidx_.get(), up_ptr_.get(), col_ptr_.get(),
+                                       perm.get(), slice_ptr.get(), slice_w.get(), n_pos, kbits, sell_vals.get(),
+                                       want_f32 ? sell_vals32.get() : nullptr, sell_cols.get());
     SGIPC_LAUNCH_CHECK();
 }
 
-int FullMatrix::partials() const { return grid_sell(n_slices); }
+int FullMatrix::partials() const { return n_tiles; }
+
+long FullMatrix::stored_blocks(hipStream_t s) const
+{
+    std::vector<int> h((size_t)n_tiles * kTile);
+    SGIPC_HIP(hipMemcpyAsync(h.data(), plen.get(), sizeof(int) * h.size(), hipMemcpyDeviceToHost, s));
+    SGIPC_HIP(hipStreamSynchronize(s));
+    long t = 0;
+    for (int v : h) t += v;
+    return t;
+}
 
 template <typename TV>
 int FullMatrix::apply(hipStream_t s, TV a, const TV* x, TV b, TV* y, double* dot_part, const int* done) const
 {
-    const int g = grid_sell(n_slices);
+    SGIPC_CHECK(n_tiles > 0, SGIPC_EMPTY, "matrix was not built");
+    const int g = n_tiles;
     if constexpr (sizeof(TV) == 8) {
         if (dot_part)
-            k_spmv_sell<double, double, true><<<g, kThreads, 0, s>>>(sell_vals.get(), sell_cols.get(), slice_ptr.get(),
-                                                                    slice_w.get(), perm.get(), n_rows, n_slices, a, x,
-                                                                    b, y, dot_part, done);
+            k_spmv_tsym<double, double, true><<<g, kThreads, 0, s>>>(
+                sell_vals.get(), sell_cols.get(), slice_ptr.get(), slice_w.get(), perm.get(), plen.get(), tinfo.get(),
+                a, x, b, y, dot_part, done);
         else
-            k_spmv_sell<double, double, false><<<g, kThreads, 0, s>>>(sell_vals.get(), sell_cols.get(),
-                                                                     slice_ptr.get(), slice_w.get(), perm.get(), n_rows,
-                                                                     n_slices, a, x, b, y, nullptr, done);
+            k_spmv_tsym<double, double, false><<<g, kThreads, 0, s>>>(
+                sell_vals.get(), sell_cols.get(), slice_ptr.get(), slice_w.get(), perm.get(), plen.get(), tinfo.get(),
+                a, x, b, y, nullptr, done);
     } else {
         SGIPC_CHECK(has_f32, SGIPC_BAD_ARG, "FP32 copy of the matrix was not built");
         if (dot_part)
-            k_spmv_sell<float, float, true><<<g, kThreads, 0, s>>>(sell_vals32.get(), sell_cols.get(), slice_ptr.get(),
-                                                                  slice_w.get(), perm.get(), n_rows, n_slices, a, x, b,
-                                                                  y, dot_part, done);
+            k_spmv_tsym<float, float, true><<<g, kThreads, 0, s>>>(
+                sell_vals32.get(), sell_cols.get(), slice_ptr.get(), slice_w.get(), perm.get(), plen.get(),
+                tinfo.get(), a, x, b, y, dot_part, done);
         else
-            k_spmv_sell<float, float, false><<<g, kThreads, 0, s>>>(sell_vals32.get(), sell_cols.get(),
-                                                                   slice_ptr.get(), slice_w.get(), perm.get(), n_rows,
-                                                                   n_slices, a, x, b, y, nullptr, done);
+            k_spmv_tsym<float, float, false><<<g, kThreads, 0, s>>>(
+                sell_vals32.get(), sell_cols.get(), slice_ptr.get(), slice_w.get(), perm.get(), plen.get(),
+                tinfo.get(), a, x, b, y, nullptr, done);
     }
     SGIPC_LAUNCH_CHECK();
-    return g;
+    return dot_part ? g : 0;
 }
 template int FullMatrix::apply<double>(hipStream_t, double, const double*, double, double*, double*, const int*) const;
 template int FullMatrix::apply<float>(hipStream_t, float, const float*, float, float*, double*, const int*) const;

@@ -1,5 +1,6 @@
-// Symmetric matrix expanded to both triangles and laid out as SELL-64-sigma for
-// the PCG SpMV.
+// Symmetric matrix laid out for the PCG SpMV: row tiles owned by one
+// workgroup, in-tile upper blocks stored once (transposed half summed in LDS),
+// cross-tile blocks stored in both rows, SELL-64 packing inside each tile.
 #pragma once
 #include "sgipc_internal.hpp"
 
@@ -7,32 +8,36 @@ namespace sgipc {
 
 class FullMatrix {
   public:
-    // Expand a unique upper BCOO (rows sorted, row <= col, indices < n) into
-    // both triangles and pack it as SELL: rows stably sorted by length, 64 rows
-    // per slice, each slice as wide as its longest row, block values stored
-    // component-major per slice.  want_f32 also keeps FP32 values.
+    // Lay out a unique upper BCOO (rows sorted, row <= col, indices < n).
+    // want_f32 also keeps FP32 values.
     void build(hipStream_t s, const double* vals, const int* rows, const int* cols, int nnz_up, int n,
                bool want_f32);
-    // y = a*A*x + b*y; dot_part (optional) receives per-workgroup x^T(A x)
-    // partials; returns the partial count.  Returns at once when *done != 0.
+    // y = a*A*x + b*y; dot_part (optional) receives one x^T(A x) partial per
+    // tile; returns the partial count (0 without dot_part).  Returns at once
+    // when *done != 0.
     template <typename TV>
     int apply(hipStream_t s, TV a, const TV* x, TV b, TV* y, double* dot_part, const int* done) const;
+    // partial count of apply() with dot_part
     int partials() const;
+    // blocks stored in the layout (upper blocks + cross-tile transposed
+    // copies, padding excluded); synchronises the stream
+    long stored_blocks(hipStream_t s) const;
 
     int n_rows = 0;
-    int nnz = 0;         // blocks of the expanded matrix
-    long n_slots = 0;    // SELL slots (64 blocks each, padding included)
+    int nnz = 0;         // unique upper blocks of the source matrix
+    long n_slots = 0;    // SELL slots (64 entries each, padding included)
     int n_slices = 0;
+    int n_tiles = 0;
     bool has_f32 = false;
     DevBuf<double> sell_vals;
     DevBuf<float> sell_vals32;
-    DevBuf<int> sell_cols, slice_ptr, slice_w, perm;
+    DevBuf<uint32_t> sell_cols, tinfo;
+    DevBuf<int> slice_ptr, slice_w, perm, plen;
 
   private:
-    DevBuf<double> fvals_;
-    DevBuf<int> frows_, fcols_, row_ptr_;
-    DevBuf<uint64_t> keys_, skeys_;
-    DevBuf<int> idx_, sidx_, up_ptr_, col_ptr_, low_cnt_, full_len_, len_key_, len_key_sorted_, iota_;
+    DevBuf<uint64_t> keys_, skeys_, rkey_, rkey_sorted_;
+    DevBuf<int> idx_, sidx_, cand_, cscan_, up_ptr_, col_ptr_, low_cnt_, len_, iota_, sorted_;
+    DevBuf<uint32_t> up_word_;
     DevBuf<unsigned char> temp_;
 };
 

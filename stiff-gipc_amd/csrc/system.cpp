@@ -1,8 +1,9 @@
 // Assembly entry points and the whole-linear-system object of the C ABI
 // (GlobalLinearSystem::solve_linear_system, StiffGIPC/linear_system/
 // linear_system/global_linear_system.cu:8-148): element Hessians into the
-// triplet arena in the reference's order (contacts, tets, mass), conversion to
-// unique upper BCOO, MAS rebuild with the current contact pairs, PCG.
+// triplet arena in the reference's order (contacts, tets, bending hinges, cloth
+// triangles, mass; StiffGIPC/GIPC.cu:10291-10485), conversion to unique upper
+// BCOO, MAS rebuild with the current contact pairs, PCG.
 #include <chrono>
 #include <cmath>
 #include <memory>
@@ -33,8 +34,9 @@ Converter& converter_of(sgipc_ctx* ctx)
 
 struct sgipc_system {
     sgipc_ctx* ctx = nullptr;
-    sgipc_tet_scene_desc d{};
+    sgipc_scene_desc d{};
     int B = 32;
+    DevBuf<double> hinge_q;  // Q per hinge, computed from the rest shape at create
     sgipc_mas* mas = nullptr;
     DevBuf<double> tvals;
     DevBuf<int> trows, tcols;
@@ -78,6 +80,47 @@ int sgipc_barrier_pt_gradient_hessian(sgipc_ctx* ctx, const double* x, const int
     });
 }
 
+int sgipc_fem_tri_gradient_hessian(sgipc_ctx* ctx, const double* x, const uint32_t* tris, const double* dm_inv2,
+                                   const double* area, int n_tris, double k_stretch, double k_shear,
+                                   double strain_rate, double dt, int hess_offset, double* grad, double* tvals,
+                                   int* trows, int* tcols)
+{
+    if (!ctx || n_tris < 0) return SGIPC_BAD_ARG;
+    if (n_tris == 0) return SGIPC_EMPTY;
+    if (!x || !tris || !dm_inv2 || !area || !tvals || !trows || !tcols) return SGIPC_BAD_ARG;
+    return run(ctx, [&] {
+        launch_tri_bw(ctx->c.stream, n_tris, x, tris, dm_inv2, area, k_stretch, k_shear, strain_rate, dt, hess_offset,
+                      grad, tvals, trows, tcols);
+        SGIPC_HIP(hipStreamSynchronize(ctx->c.stream));
+    });
+}
+
+int sgipc_quad_bending_prepare(sgipc_ctx* ctx, const double* rest_x, const uint32_t* edges, const uint32_t* adj,
+                               int n_edges, double* q16)
+{
+    if (!ctx || n_edges < 0) return SGIPC_BAD_ARG;
+    if (n_edges == 0) return SGIPC_EMPTY;
+    if (!rest_x || !edges || !adj || !q16) return SGIPC_BAD_ARG;
+    return run(ctx, [&] {
+        launch_quad_bending_q(ctx->c.stream, n_edges, rest_x, edges, adj, q16);
+        SGIPC_HIP(hipStreamSynchronize(ctx->c.stream));
+    });
+}
+
+int sgipc_quad_bending_gradient_hessian(sgipc_ctx* ctx, const double* x, const double* rest_x, const uint32_t* edges,
+                                        const uint32_t* adj, const double* q16, int n_edges, double k_bend, double dt,
+                                        int hess_offset, double* grad, double* tvals, int* trows, int* tcols)
+{
+    if (!ctx || n_edges < 0) return SGIPC_BAD_ARG;
+    if (n_edges == 0) return SGIPC_EMPTY;
+    if (!x || !rest_x || !edges || !adj || !q16 || !tvals || !trows || !tcols) return SGIPC_BAD_ARG;
+    return run(ctx, [&] {
+        launch_quad_bending(ctx->c.stream, n_edges, x, rest_x, edges, adj, q16, k_bend, dt, hess_offset, grad, tvals,
+                            trows, tcols);
+        SGIPC_HIP(hipStreamSynchronize(ctx->c.stream));
+    });
+}
+
 int sgipc_mass_diagonal(sgipc_ctx* ctx, const double* mass, int n, int hess_offset, double* tvals, int* trows,
                         int* tcols)
 {
@@ -107,7 +150,7 @@ int sgipc_convert_srbk(sgipc_ctx* ctx, const double* tvals, const int* trows, co
 int sgipc_mas_create(sgipc_ctx* ctx, sgipc_mas** out, int bank_size);
 int sgipc_mas_destroy(sgipc_mas* mas);
 
-int sgipc_system_create(sgipc_ctx* ctx, sgipc_system** out, const sgipc_tet_scene_desc* desc, int bank_size)
+int sgipc_system_create(sgipc_ctx* ctx, sgipc_system** out, const sgipc_scene_desc* desc, int bank_size)
 {
     if (!ctx || !out || !desc) return SGIPC_BAD_ARG;
     *out = nullptr;
@@ -117,6 +160,9 @@ int sgipc_system_create(sgipc_ctx* ctx, sgipc_system** out, const sgipc_tet_scen
     if (!d.x || !d.mass || !d.nbr_start || !d.nbr_num || (d.total_neighbor_num > 0 && !d.nbr_list)) return SGIPC_BAD_ARG;
     if (d.n_tets > 0 && (!d.tets || !d.dm_inv || !d.vol || !d.mu || !d.lam)) return SGIPC_BAD_ARG;
     if (d.n_pairs > 0 && (!d.pairs || !(d.dhat_sq > 0))) return SGIPC_BAD_ARG;
+    if (d.n_tris < 0 || d.n_hinges < 0) return SGIPC_BAD_ARG;
+    if (d.n_tris > 0 && (!d.tris || !d.dm_inv2 || !d.area)) return SGIPC_BAD_ARG;
+    if (d.n_hinges > 0 && (!d.rest_x || !d.hinge_edges || !d.hinge_adj)) return SGIPC_BAD_ARG;
     if (bank_size != 16 && bank_size != 32) return SGIPC_BAD_ARG;
     auto* sys = new (std::nothrow) sgipc_system();
     if (!sys) return SGIPC_HIP_ERROR;
@@ -128,7 +174,12 @@ int sgipc_system_create(sgipc_ctx* ctx, sgipc_system** out, const sgipc_tet_scen
         st = run(ctx, [&] {
             sys->mas->m->init_neighbor(d.n_verts, 0, d.total_neighbor_num, d.nbr_start, d.nbr_num, d.nbr_list);
             sys->mas->m->init_matrix();
-            sys->n_trip = 16L * d.n_pairs + 10L * d.n_tets + d.n_verts;
+            sys->n_trip = 16L * d.n_pairs + 10L * d.n_tets + 10L * d.n_hinges + 6L * d.n_tris + d.n_verts;
+            if (d.n_hinges > 0) {
+                sys->hinge_q.reserve(16L * d.n_hinges);
+                launch_quad_bending_q(ctx->c.stream, d.n_hinges, d.rest_x, d.hinge_edges, d.hinge_adj,
+                                      sys->hinge_q.get());
+            }
             SGIPC_CHECK(sys->n_trip < (1L << 31), SGIPC_CAPACITY, "triplet count exceeds 2^31");
             sys->tvals.reserve((size_t)sys->n_trip * 9);
             sys->trows.reserve(sys->n_trip);
@@ -164,6 +215,7 @@ int sgipc_system_destroy(sgipc_system* sys)
         sys->bvals.release();
         sys->brows.release();
         sys->bcols.release();
+        sys->hinge_q.release();
     });
     if (sys->mas) sgipc_mas_destroy(sys->mas);
     delete sys;
@@ -184,7 +236,7 @@ int sgipc_system_solve(sgipc_system* sys, const double* b, double* x, double* gr
         hipStream_t s = sys->ctx->c.stream;
         auto t0 = std::chrono::steady_clock::now();
         SGIPC_HIP(hipEventRecord(sys->ev[0], s));
-        // ---- assembly (arena order: contacts, tets, mass)
+        // ---- assembly (arena order: contacts, tets, hinges, triangles, mass)
         if (grad) launch_fill<double>(s, grad, 0.0, 3L * d.n_verts);
         long off = 0;
         launch_barrier_pt(s, d.n_pairs, d.x, d.pairs, d.pair_x, d.dhat_sq, d.kappa, 0, grad, sys->tvals.get(),
@@ -193,6 +245,12 @@ int sgipc_system_solve(sgipc_system* sys, const double* b, double* x, double* gr
         launch_tet_snh(s, d.n_tets, d.x, d.tets, d.dm_inv, d.vol, d.mu, d.lam, d.dt, 0, grad,
                        sys->tvals.get() + 9 * off, sys->trows.get() + off, sys->tcols.get() + off);
         off += 10L * d.n_tets;
+        launch_quad_bending(s, d.n_hinges, d.x, d.rest_x, d.hinge_edges, d.hinge_adj, sys->hinge_q.get(), d.k_bend,
+                            d.dt, 0, grad, sys->tvals.get() + 9 * off, sys->trows.get() + off, sys->tcols.get() + off);
+        off += 10L * d.n_hinges;
+        launch_tri_bw(s, d.n_tris, d.x, d.tris, d.dm_inv2, d.area, d.k_stretch, d.k_shear, d.strain_rate, d.dt, 0, grad,
+                      sys->tvals.get() + 9 * off, sys->trows.get() + off, sys->tcols.get() + off);
+        off += 6L * d.n_tris;
         launch_mass_diag(s, d.n_verts, d.mass, 0, sys->tvals.get() + 9 * off, sys->trows.get() + off,
                          sys->tcols.get() + off);
         off += d.n_verts;

@@ -23,9 +23,9 @@ import numpy as np
 
 from . import _lib
 from ._lib import (FP32_IR, FP32_MAS, FP64, PRECISIONS, STOP_REF_RZ, STOP_TRUE_RESIDUAL, Context, DevArray,
-                   PcgOpts, PcgStats, SgipcError, SolveTiming, TetSceneDesc, check, lib)
+                   PcgOpts, PcgStats, SgipcError, SolveTiming, SceneDesc, check, lib)
 
-__all__ = ["Context", "DevArray", "Spmv", "Converter", "MASPreconditioner", "PCGSolver", "GlobalLinearSystem",
+__all__ = ["Context", "DevArray", "Spmv", "SpMat", "Converter", "MASPreconditioner", "PCGSolver", "GlobalLinearSystem",
            "LinearSystem", "SolveResult", "SgipcError", "FP64", "FP32_MAS", "FP32_IR", "STOP_REF_RZ",
            "STOP_TRUE_RESIDUAL", "pcg_opts", "upload_scene"]
 
@@ -45,6 +45,34 @@ class Spmv:
                                                    n_rows), self.ctx.h)
         return check(lib().sgipc_spmv_srbk(self.ctx.h, a, _p(vals), _p(rows), _p(cols), nnz, _p(x), b, _p(y), n_rows),
                      self.ctx.h)
+
+
+class SpMat:
+    """Prepared symmetric matrix (row-tiled layout) for repeated SpMVs."""
+
+    def __init__(self, ctx: Context, vals, rows, cols, nnz, n_rows, want_f32=False):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        check(lib().sgipc_spmat_create(ctx.h, ctypes.byref(h), _p(vals), _p(rows), _p(cols), nnz, n_rows,
+                                       int(want_f32)), ctx.h)
+        self.h = h
+
+    def apply(self, a, x, b, y):
+        """y = a*A*x + b*y on device arrays (float64, or float32 with want_f32)."""
+        if x.dtype == np.float32:
+            return check(lib().sgipc_spmat_apply_f32(self.h, a, _p(x), b, _p(y)), self.ctx.h)
+        return check(lib().sgipc_spmat_apply(self.h, a, _p(x), b, _p(y)), self.ctx.h)
+
+    def close(self):
+        if self.h:
+            lib().sgipc_spmat_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Converter:
@@ -165,7 +193,8 @@ class SolveResult:
 
 
 def upload_scene(ctx: Context, scene):
-    """Copy a TetScene's solver inputs to the device; returns (desc, keepalive)."""
+    """Copy a scene's solver inputs (tets, cloth triangles, hinges, contacts) to
+    the device; returns (desc, keepalive)."""
     keep = {}
 
     def put(name, a, dtype):
@@ -176,7 +205,7 @@ def upload_scene(ctx: Context, scene):
         keep[name] = d
         return d.ptr
 
-    desc = TetSceneDesc()
+    desc = SceneDesc()
     desc.n_verts = scene.n
     desc.n_tets = scene.n_tets
     desc.n_pairs = scene.n_pairs
@@ -185,11 +214,24 @@ def upload_scene(ctx: Context, scene):
     desc.dhat_sq = scene.dhat ** 2
     desc.kappa = scene.kappa
     desc.x = put("x", scene.x.reshape(-1), np.float64)
-    desc.tets = put("tets", scene.tets.reshape(-1), np.uint32)
-    desc.dm_inv = put("dm_inv", scene.dm_inv.reshape(-1), np.float64)
-    desc.vol = put("vol", scene.vol, np.float64)
-    desc.mu = put("mu", scene.mu, np.float64)
-    desc.lam = put("lam", scene.lam, np.float64)
+    if scene.n_tets:
+        desc.tets = put("tets", scene.tets.reshape(-1), np.uint32)
+        desc.dm_inv = put("dm_inv", scene.dm_inv.reshape(-1), np.float64)
+        desc.vol = put("vol", scene.vol, np.float64)
+        desc.mu = put("mu", scene.mu, np.float64)
+        desc.lam = put("lam", scene.lam, np.float64)
+    desc.n_tris = scene.n_tris
+    if scene.n_tris:
+        desc.k_stretch, desc.k_shear, desc.strain_rate = scene.k_stretch, scene.k_shear, scene.strain_rate
+        desc.tris = put("tris", scene.tris.reshape(-1), np.uint32)
+        desc.dm_inv2 = put("dm_inv2", scene.dm_inv2.reshape(-1), np.float64)
+        desc.area = put("area", scene.area, np.float64)
+    desc.n_hinges = scene.n_hinges
+    if scene.n_hinges:
+        desc.k_bend = scene.k_bend
+        desc.rest_x = put("rest_x", scene.rest.reshape(-1), np.float64)
+        desc.hinge_edges = put("hinge_edges", scene.edges.reshape(-1).astype(np.int64) & 0xFFFFFFFF, np.uint32)
+        desc.hinge_adj = put("hinge_adj", scene.adj.reshape(-1).astype(np.int64) & 0xFFFFFFFF, np.uint32)
     desc.mass = put("mass", scene.mass, np.float64)
     desc.pairs = put("pairs", scene.pairs.reshape(-1) if scene.n_pairs else None, np.int32)
     desc.pair_x = put("pair_x", scene.pair_x.reshape(-1) if scene.pair_x is not None and scene.n_pairs else None,
@@ -202,7 +244,8 @@ def upload_scene(ctx: Context, scene):
 
 class GlobalLinearSystem:
     """Whole Newton linear solve on the device: assembly (contacts, tets,
-    mass), conversion to unique upper BCOO, MAS rebuild, PCG."""
+    bending hinges, cloth triangles, mass), conversion to unique upper BCOO,
+    MAS rebuild, PCG."""
 
     def __init__(self, ctx: Context, scene, bank_size=32):
         self.ctx = ctx

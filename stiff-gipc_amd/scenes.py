@@ -33,6 +33,11 @@ import numpy as np
 
 __all__ = [
     "TetScene",
+    "ClothScene",
+    "cloth_grid_scene",
+    "scene_cloth_tiny",
+    "scene_c4",
+    "morton2",
     "lame_from_young",
     "morton3",
     "kuhn_grid",
@@ -81,9 +86,82 @@ class TetScene:
         return int(self.pairs.shape[0])
 
     @property
+    def n_tris(self) -> int:
+        return 0
+
+    @property
+    def n_hinges(self) -> int:
+        return 0
+
+    @property
     def n_triplets(self) -> int:
         """Triplet count written by the assembly kernels (16/pair, 10/tet, 1/vertex)."""
         return 16 * self.n_pairs + 10 * self.n_tets + self.n
+
+
+@dataclasses.dataclass
+class ClothScene:
+    """Inputs of one Newton linear solve on a cloth sheet (+ optional contacts).
+
+    Per-triangle inputs follow the reference's cloth set-up
+    (``StiffGIPC/gl_main.cu:578-584,622-641``): ``dm_inv2`` is the inverse of
+    the 2x2 rest edge matrix after rotating the rest triangle into the xy
+    plane (``femEnergy.cu:256-300``), ``area`` is the rest area times the
+    cloth thickness, the lumped mass is density * area / 3 per corner,
+    ``k_stretch = E / (2 (1 + nu))`` and ``k_shear = 0.03 k_stretch rate``.
+    Hinges list each interior edge with the two opposite vertices
+    (``adj[:, 1] == -1`` on the boundary), the layout
+    ``_calculate_quad_bending_gradient_hessian`` reads (femEnergy.cu:2623-2754).
+    """
+
+    name: str
+    x: np.ndarray            # (n, 3) current positions
+    rest: np.ndarray         # (n, 3) rest positions
+    tris: np.ndarray         # (m, 3) int32
+    dm_inv2: np.ndarray      # (m, 2, 2) row-major
+    area: np.ndarray         # (m,) rest area * thickness
+    k_stretch: float
+    k_shear: float
+    strain_rate: float
+    edges: np.ndarray        # (e, 2) int32 hinge edge
+    adj: np.ndarray          # (e, 2) int32 opposite vertices, -1 on the boundary
+    k_bend: float
+    mass: np.ndarray
+    dt: float
+    nbr_start: np.ndarray
+    nbr_num: np.ndarray
+    nbr_list: np.ndarray
+    pairs: np.ndarray
+    pair_x: np.ndarray | None
+    dhat: float
+    kappa: float
+    rhs: np.ndarray
+    seeds: dict
+
+    @property
+    def n(self) -> int:
+        return int(self.x.shape[0])
+
+    @property
+    def n_tets(self) -> int:
+        return 0
+
+    @property
+    def n_tris(self) -> int:
+        return int(self.tris.shape[0])
+
+    @property
+    def n_hinges(self) -> int:
+        return int(self.edges.shape[0])
+
+    @property
+    def n_pairs(self) -> int:
+        return int(self.pairs.shape[0])
+
+    @property
+    def n_triplets(self) -> int:
+        """16/pair, 10/hinge, 6/triangle, 1/vertex."""
+        return 16 * self.n_pairs + 10 * self.n_hinges + 6 * self.n_tris + self.n
 
 
 def lame_from_young(E, nu):
@@ -107,6 +185,21 @@ def _part1by2(v: np.ndarray) -> np.ndarray:
     v = (v | (v << np.uint64(4))) & np.uint64(0x10C30C30C30C30C3)
     v = (v | (v << np.uint64(2))) & np.uint64(0x1249249249249249)
     return v
+
+
+def _part1by1(v: np.ndarray) -> np.ndarray:
+    v = v.astype(np.uint64) & np.uint64(0xFFFFFFFF)
+    v = (v | (v << np.uint64(16))) & np.uint64(0x0000FFFF0000FFFF)
+    v = (v | (v << np.uint64(8))) & np.uint64(0x00FF00FF00FF00FF)
+    v = (v | (v << np.uint64(4))) & np.uint64(0x0F0F0F0F0F0F0F0F)
+    v = (v | (v << np.uint64(2))) & np.uint64(0x3333333333333333)
+    v = (v | (v << np.uint64(1))) & np.uint64(0x5555555555555555)
+    return v
+
+
+def morton2(i, j) -> np.ndarray:
+    """64-bit 2D Morton code with i in the lowest bit of each pair."""
+    return _part1by1(i) | (_part1by1(j) << np.uint64(1))
 
 
 def morton3(i, j, k) -> np.ndarray:
@@ -173,9 +266,15 @@ def _surface_triangles(tets: np.ndarray) -> np.ndarray:
 
 
 def seed_pt_pairs(x: np.ndarray, tets: np.ndarray, n_pairs: int, dhat: float, seed: int):
+    """Seeded point-triangle pairs on the boundary of a tet mesh; see
+    :func:`seed_pt_pairs_on`."""
+    return seed_pt_pairs_on(x, _surface_triangles(tets), n_pairs, dhat, seed)
+
+
+def seed_pt_pairs_on(x: np.ndarray, tris: np.ndarray, n_pairs: int, dhat: float, seed: int):
     """Seeded point-triangle pairs with prescribed squared distance d^2 ~ U(0, dhat^2).
 
-    Vertex p is a random surface vertex and (t0,t1,t2) a random surface triangle
+    Vertex p is a random vertex of ``tris`` and (t0,t1,t2) a random triangle
     not incident to p.  The barrier Hessian depends on the relative geometry
     only, so each pair carries its own local geometry: the triangle at its
     current positions and the point placed at distance d along the triangle
@@ -184,7 +283,6 @@ def seed_pt_pairs(x: np.ndarray, tets: np.ndarray, n_pairs: int, dhat: float, se
     Returns (pairs (p,4) int32 in the reference encoding, pair_x (p,4,3)).
     """
     rng = np.random.default_rng(seed)
-    tris = _surface_triangles(tets)
     surf_v = np.unique(tris)
     pv = surf_v[rng.integers(0, surf_v.size, size=n_pairs)]
     ti = rng.integers(0, tris.shape[0], size=n_pairs)
@@ -269,3 +367,98 @@ def scene_c1(n_pairs: int = 50_000) -> TetScene:
 def scene_tiny(N: int = 6, n_pairs: int = 12, E_loguniform=(1e4, 1e9)) -> TetScene:
     """Small grid with contacts and a wide stiffness range, for dense checks."""
     return tet_grid_scene(f"tiny_grid{N}", N, E_loguniform=E_loguniform, n_pairs=n_pairs)
+
+
+def _tri_edges_csr(n: int, tris: np.ndarray):
+    """Unique triangle edges with their opposite vertices, and the symmetric
+    vertex-neighbour CSR of the edges."""
+    m = tris.shape[0]
+    e = np.concatenate([tris[:, [0, 1]], tris[:, [1, 2]], tris[:, [2, 0]]])
+    opp = np.concatenate([tris[:, 2], tris[:, 0], tris[:, 1]])
+    tid = np.tile(np.arange(m), 3)
+    lo, hi = np.minimum(e[:, 0], e[:, 1]), np.maximum(e[:, 0], e[:, 1])
+    key = lo.astype(np.int64) * n + hi
+    order = np.lexsort((tid, key))
+    key, opp, lo, hi = key[order], opp[order], lo[order], hi[order]
+    head = np.r_[True, key[1:] != key[:-1]]
+    first = np.flatnonzero(head)
+    cnt = np.diff(np.r_[first, key.size])
+    assert cnt.max() <= 2, "non-manifold edge"
+    edges = np.stack([lo[first], hi[first]], axis=1)
+    adj = np.full((first.size, 2), -1, dtype=np.int64)
+    adj[:, 0] = opp[first]
+    two = cnt == 2
+    adj[two, 1] = opp[first[two] + 1]
+    src = np.concatenate([edges[:, 0], edges[:, 1]])
+    dst = np.concatenate([edges[:, 1], edges[:, 0]])
+    o = np.lexsort((dst, src))
+    src, dst = src[o], dst[o]
+    num = np.bincount(src, minlength=n).astype(np.uint32)
+    start = np.zeros(n, dtype=np.uint32)
+    start[1:] = np.cumsum(num)[:-1]
+    return edges.astype(np.int32), adj.astype(np.int32), start, num, dst.astype(np.uint32)
+
+
+def cloth_grid_scene(name: str, N: int, *, spacing=1e-3, jitter=0.1, E=1e6, nu=0.49, thickness=1e-3,
+                     density=2e2, strain_rate=100.0, k_bend=1e-4, drape_radius=1.0, dt=0.01, n_pairs=0,
+                     dhat=1e-3, kappa=1e6, seed_jitter=0, seed_rhs=1, seed_pairs=3) -> ClothScene:
+    """N x N vertex sheet (2 triangles per quad, 2D Morton order), flat at rest
+    in the xy plane, currently draped over a sphere of radius
+    drape_radius * side length (z = -(u^2+v^2) / (2 R)) with a seeded in-plane
+    jitter, so stretch, compression, shear and bending all appear."""
+    ii, jj = np.meshgrid(np.arange(N), np.arange(N), indexing="ij")
+    ii, jj = ii.ravel(), jj.ravel()
+    code = morton2(ii, jj)
+    order = np.argsort(code, kind="stable")
+    relabel = np.empty_like(order)
+    relabel[order] = np.arange(order.size)
+    n = N * N
+
+    def vid(i, j):
+        return relabel[i * N + j]
+
+    qi, qj = np.meshgrid(np.arange(N - 1), np.arange(N - 1), indexing="ij")
+    qi, qj = qi.ravel(), qj.ravel()
+    v00, v10, v01, v11 = vid(qi, qj), vid(qi + 1, qj), vid(qi, qj + 1), vid(qi + 1, qj + 1)
+    tris = np.concatenate([np.stack([v00, v10, v11], axis=1), np.stack([v00, v11, v01], axis=1)]).astype(np.int32)
+    lat = np.stack([ii[order], jj[order]], axis=1).astype(np.float64)
+    side = (N - 1) * spacing
+    rest = np.zeros((n, 3))
+    rest[:, :2] = lat * spacing - 0.5 * side
+    rng = np.random.default_rng(seed_jitter)
+    x = rest.copy()
+    x[:, :2] += rng.uniform(-jitter * spacing, jitter * spacing, size=(n, 2))
+    R = drape_radius * side
+    x[:, 2] = -(x[:, 0] ** 2 + x[:, 1] ** 2) / (2.0 * R)
+
+    a, b, c = rest[tris[:, 0]], rest[tris[:, 1]], rest[tris[:, 2]]
+    Dm = np.stack([(b - a)[:, :2], (c - a)[:, :2]], axis=2)   # columns u0, u1
+    detm = np.linalg.det(Dm)
+    assert (detm > 0).all()
+    dm_inv2 = np.linalg.inv(Dm)
+    area = 0.5 * np.abs(detm) * thickness
+    mass = np.zeros(n)
+    np.add.at(mass, tris.ravel(), np.repeat(density * area / 3.0, 3))
+    k_stretch = E / (2.0 * (1.0 + nu))
+    k_shear = 0.03 * k_stretch * strain_rate
+    edges, adj, start, num, lst = _tri_edges_csr(n, tris)
+    if n_pairs:
+        pairs, pair_x = seed_pt_pairs_on(x, tris, n_pairs, dhat, seed_pairs)
+    else:
+        pairs, pair_x = np.zeros((0, 4), dtype=np.int32), None
+    rhs = np.random.default_rng(seed_rhs).standard_normal(3 * n)
+    return ClothScene(name=name, x=np.ascontiguousarray(x), rest=rest, tris=tris,
+                      dm_inv2=np.ascontiguousarray(dm_inv2), area=area, k_stretch=k_stretch, k_shear=k_shear,
+                      strain_rate=strain_rate, edges=edges, adj=adj, k_bend=k_bend, mass=mass, dt=dt,
+                      nbr_start=start, nbr_num=num, nbr_list=lst, pairs=pairs, pair_x=pair_x, dhat=dhat,
+                      kappa=kappa, rhs=rhs, seeds=dict(jitter=seed_jitter, rhs=seed_rhs, pairs=seed_pairs))
+
+
+def scene_cloth_tiny(N: int = 10, n_pairs: int = 16) -> ClothScene:
+    """Small draped sheet with contacts, for dense checks."""
+    return cloth_grid_scene(f"cloth_tiny{N}", N, n_pairs=n_pairs)
+
+
+def scene_c4(n_pairs: int = 200_000) -> ClothScene:
+    """1024 x 1024 sheet, 1 mm spacing, E = 1e6 Pa, k_bend = 1e-4, 200k pairs."""
+    return cloth_grid_scene("c4_cloth1024", 1024, n_pairs=n_pairs)

@@ -2,7 +2,7 @@
 CPU oracle on the same seeded inputs.
 
 Tolerances (floating point; integer/index work is compared bit-exactly):
-  FP64 SpMV               max|y_gpu - y_orc| <= 1e-13 * max|y_orc|   (atomic order)
+  FP64 SpMV               max|y_gpu - y_orc| <= 1e-13 * max|y_orc|   (summation order)
   FP32 SpMV               <= 2e-6 relative
   triplet -> BCOO         bit-exact (both sum each block in arena order)
   tet / barrier Hessians  max|dH| <= 1e-10 * max|H| per element; rows/cols exact
@@ -52,6 +52,36 @@ def test_spmv_f64_matches_oracle(pkg, orc, gpu_ctx, which, scene_tiny, scene_sma
     pkg.Spmv(gpu_ctx).srbk_spmv(1.0, dv, dr, dc, rows.size, dx, 0.0, dy, s.n)
     ref0 = orc.spmv(1.0, vals, rows, cols, x, 0.0, np.zeros(3 * s.n), s.n)
     assert rel(dy.download(), ref0) <= 1e-13
+
+
+@pytest.mark.parametrize("which", ["tiny", "small", "c0", "cloth_tiny", "cloth64"])
+def test_tiled_spmv_matches_oracle(pkg, orc, gpu_ctx, which, scene_tiny, scene_small, scene_c0):
+    """The PCG loop's row-tiled SpMV (one tile for the tiny scenes, several
+    with a partly filled last tile for the others), FP64 and FP32."""
+    s = {"tiny": scene_tiny, "small": scene_small, "c0": scene_c0}.get(which)
+    if s is None:
+        s = (pkg.scenes.scene_cloth_tiny(N=12, n_pairs=30) if which == "cloth_tiny"
+             else pkg.scenes.cloth_grid_scene("cloth64", 64, n_pairs=400))
+    vals, rows, cols = orc.assemble(s)
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal(3 * s.n)
+    y0 = rng.standard_normal(3 * s.n)
+    dv, dr, dc = upload_bcoo(gpu_ctx, vals, rows, cols)
+    A = pkg.SpMat(gpu_ctx, dv, dr, dc, rows.size, s.n, want_f32=True)
+    try:
+        dx, dy = gpu_ctx.to_device(x), gpu_ctx.to_device(y0)
+        A.apply(0.9, dx, -0.4, dy)
+        assert rel(dy.download(), orc.spmv(0.9, vals, rows, cols, x, -0.4, y0, s.n)) <= 1e-13
+        dy.upload(np.full(3 * s.n, np.nan))
+        A.apply(1.0, dx, 0.0, dy)
+        ref = orc.spmv(1.0, vals, rows, cols, x, 0.0, np.zeros(3 * s.n), s.n)
+        assert rel(dy.download(), ref) <= 1e-13
+        x32 = gpu_ctx.to_device(x.astype(np.float32), np.float32)
+        y32 = gpu_ctx.to_device(np.zeros(3 * s.n, np.float32), np.float32)
+        A.apply(1.0, x32, 0.0, y32)
+        assert rel(y32.download().astype(np.float64), ref) <= 2e-6
+    finally:
+        A.close()
 
 
 def test_spmv_f32_matches_oracle(pkg, orc, gpu_ctx, scene_c0):
