@@ -126,6 +126,40 @@ int sgipc_quad_bending_gradient_hessian(sgipc_ctx* ctx, const double* x, const d
                                         int hess_offset, double* grad, double* tvals, int* trows,
                                         int* tcols);
 
+/* Replaces ABDSystem::_cal_abd_body_gradient_and_hessian and the body part of
+ * _setup_abd_system_hessian (StiffGIPC/abd_system/abd_system_function/
+ * setup_abd_system_gradient_and_hessian.cu:200-289,464-476): per body
+ * H = M + PSD(kappa vol dt^2 d2V/da2), G = M (q - q~) + kappa vol dt^2 dV/da
+ * (G = 0, H = M when fixed), 10 upper blocks at rows 4b..4b+3 written to
+ * tvals[10*b ...]; grad (optional) receives 12 per body. */
+int sgipc_abd_body_gradient_hessian(sgipc_ctx* ctx, int n_bodies, const double* q,
+                                    const double* q_tilde, const double* mass10,
+                                    const double* body_volume, double kappa, double dt,
+                                    const int* body_fixed, double* grad, double* tvals, int* trows,
+                                    int* tcols);
+
+/* Replaces GIPC::partitionContactHessian with the ABD contact transforms
+ * (StiffGIPC/GIPC.cu:10080-10256; setup_abd_system_gradient_and_hessian.cu:
+ * 87-141,424-558): point-level upper contact blocks become, class by class in
+ * input order, FEM-FEM blocks (rows shifted by 4*n_bodies - n_abd_points),
+ * 4 ABD-FEM blocks J^T H and 16 ABD-ABD blocks J_i^T H J_j (lower ones
+ * zeroed).  *out_n receives the count; SGIPC_CAPACITY if it exceeds
+ * capacity (nothing written then). */
+int sgipc_contact_partition(sgipc_ctx* ctx, const double* vals, const int* rows, const int* cols,
+                            int n, int n_abd_points, const int* point_body, const double* x_bar,
+                            int n_bodies, const int* body_fixed, double* ovals, int* orows,
+                            int* ocols, int capacity, int* out_n);
+
+/* Replaces ABDSystem::_cal_abd_system_preconditioner (setup_abd_system_
+ * gradient_and_hessian.cu:560-655): inverse of every body's 12x12 diagonal
+ * block of the assembled upper BCOO (row-major, 144 doubles per body);
+ * SGIPC_NOT_SPD on a singular block.  apply: z = P r on the first
+ * 12*n_bodies entries. */
+int sgipc_abd_jacobi_build(sgipc_ctx* ctx, const double* vals, const int* rows, const int* cols,
+                           int nnz, int n_bodies, double* inv144);
+int sgipc_abd_jacobi_apply(sgipc_ctx* ctx, const double* inv144, int n_bodies, const double* r,
+                           double* z);
+
 /* Replaces the mass-diagonal writer of computeGradientAndHessian
  * (StiffGIPC/GIPC.cu:10466-10485): mass[i]*I at (i+off, i+off). */
 int sgipc_mass_diagonal(sgipc_ctx* ctx, const double* mass, int n, int hess_offset,
@@ -275,6 +309,20 @@ typedef struct sgipc_scene_desc {
     const double* rest_x;     /* 3n rest positions                    */
     const uint32_t* hinge_edges; /* 2 per hinge                       */
     const uint32_t* hinge_adj;   /* 2 per hinge, 0xFFFFFFFF = boundary */
+    /* affine bodies (ABD).  With n_bodies > 0 the system rows are
+     * [4 per body | FEM vertices], contact pairs use global point ids
+     * [body points | FEM vertices] and pair_x is required; x, tets, tris,
+     * hinges, mass and the neighbour lists stay FEM-local. */
+    int n_bodies;
+    int n_abd_points;
+    double kappa_abd;
+    const double* q;          /* 12 per body: p, a1, a2, a3          */
+    const double* q_tilde;    /* 12 per body                          */
+    const double* mass10;     /* 10 per body: m, m xb, m xb xb^T upper */
+    const double* body_volume;
+    const int* body_fixed;    /* per body, nonzero = fixed            */
+    const int* point_body;    /* per body point                       */
+    const double* x_bar;      /* 3 per body point, body-frame rest    */
 } sgipc_scene_desc;
 
 typedef struct sgipc_solve_timing {

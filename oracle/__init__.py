@@ -71,6 +71,19 @@ def lib():
                               ctypes.c_int, c_dbl_p]
         L.orc_pcg.restype = ctypes.c_int
         L.orc_set_fault.argtypes = [ctypes.c_int]
+        L.orc_abd_body.argtypes = [ctypes.c_int, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p, ctypes.c_double,
+                                   ctypes.c_double, c_int_p, c_dbl_p, c_dbl_p, c_int_p, c_int_p]
+        L.orc_abd_shape_energy.argtypes = [c_dbl_p]
+        L.orc_abd_shape_energy.restype = ctypes.c_double
+        L.orc_abd_shape_grad_hess.argtypes = [c_dbl_p, c_dbl_p, c_dbl_p]
+        L.orc_abd_contact_map.argtypes = [ctypes.c_long, c_dbl_p, c_int_p, c_int_p, ctypes.c_int, c_int_p, c_dbl_p,
+                                          ctypes.c_int, c_int_p, c_dbl_p, c_int_p, c_int_p]
+        L.orc_abd_contact_map.restype = ctypes.c_long
+        L.orc_abd_jacobi_build.argtypes = [ctypes.c_long, c_dbl_p, c_int_p, c_int_p, ctypes.c_int, c_dbl_p]
+        L.orc_abd_jacobi_build.restype = ctypes.c_int
+        L.orc_pcg2.argtypes = [ctypes.c_long, c_dbl_p, c_int_p, c_int_p, ctypes.c_int, ctypes.c_void_p, c_dbl_p,
+                               c_dbl_p, ctypes.c_double, ctypes.c_int, ctypes.c_int, c_dbl_p]
+        L.orc_pcg2.restype = ctypes.c_int
         L.orc_tri_bw.argtypes = [ctypes.c_int, c_dbl_p, c_int_p, c_dbl_p, c_dbl_p] + [ctypes.c_double] * 4 + \
             [ctypes.c_int, c_dbl_p, c_dbl_p, c_int_p, c_int_p]
         L.orc_tri_bw_energy.argtypes = [c_dbl_p, c_dbl_p] + [ctypes.c_double] * 4
@@ -180,6 +193,63 @@ def quad_bending(scene, Q=None, hess_offset=0, with_grad=True):
     return vals, rows, cols, grad
 
 
+def abd_body(scene, with_grad=True):
+    """Affine-body terms: 10 blocks per body at rows 4b..4b+3."""
+    nb = scene.n_bodies
+    vals = np.zeros((10 * nb, 9))
+    rows = np.zeros(10 * nb, dtype=np.int32)
+    cols = np.zeros(10 * nb, dtype=np.int32)
+    grad = np.zeros(12 * nb) if with_grad else None
+    if nb:
+        fixed = np.ascontiguousarray(scene.body_fixed, dtype=np.int32)
+        lib().orc_abd_body(nb, _ptr(np.ascontiguousarray(scene.q), c_dbl_p),
+                           _ptr(np.ascontiguousarray(scene.q_tilde), c_dbl_p),
+                           _ptr(np.ascontiguousarray(scene.mass10), c_dbl_p),
+                           _ptr(np.ascontiguousarray(scene.body_volume), c_dbl_p), scene.kappa_abd, scene.dt,
+                           _ptr(fixed, c_int_p), _ptr(grad, c_dbl_p), _ptr(vals, c_dbl_p), _ptr(rows, c_int_p),
+                           _ptr(cols, c_int_p))
+    return vals, rows, cols, grad
+
+
+def abd_shape(q12):
+    q12 = np.ascontiguousarray(q12, dtype=np.float64)
+    g, H = np.zeros(9), np.zeros(81)
+    e = lib().orc_abd_shape_energy(_ptr(q12, c_dbl_p))
+    lib().orc_abd_shape_grad_hess(_ptr(q12, c_dbl_p), _ptr(g, c_dbl_p), _ptr(H, c_dbl_p))
+    return e, g, H.reshape(9, 9)
+
+
+def abd_contact_map(scene, vals, rows, cols):
+    """Vertex-level contact triplets -> system triplets by class (FEM-FEM,
+    ABD-FEM, ABD-ABD)."""
+    n = rows.size
+    cap = 16 * max(n, 1)
+    ov = np.zeros((cap, 9))
+    orr = np.zeros(cap, dtype=np.int32)
+    oc = np.zeros(cap, dtype=np.int32)
+    vals = np.ascontiguousarray(vals, dtype=np.float64)
+    rows = np.ascontiguousarray(rows, dtype=np.int32)
+    cols = np.ascontiguousarray(cols, dtype=np.int32)
+    m = lib().orc_abd_contact_map(n, _ptr(vals, c_dbl_p), _ptr(rows, c_int_p), _ptr(cols, c_int_p),
+                                  scene.n_abd_points, _ptr(np.ascontiguousarray(scene.point_body, dtype=np.int32), c_int_p),
+                                  _ptr(np.ascontiguousarray(scene.x_bar), c_dbl_p), scene.n_bodies,
+                                  _ptr(np.ascontiguousarray(scene.body_fixed, dtype=np.int32), c_int_p),
+                                  _ptr(ov, c_dbl_p), _ptr(orr, c_int_p), _ptr(oc, c_int_p))
+    return ov[:m].copy(), orr[:m].copy(), oc[:m].copy()
+
+
+def abd_jacobi(vals, rows, cols, n_bodies):
+    """Inverse 12x12 diagonal blocks (n_bodies, 12, 12)."""
+    out = np.zeros((max(n_bodies, 1), 144))
+    st = lib().orc_abd_jacobi_build(rows.size, _ptr(np.ascontiguousarray(vals, dtype=np.float64), c_dbl_p),
+                                    _ptr(np.ascontiguousarray(rows, dtype=np.int32), c_int_p),
+                                    _ptr(np.ascontiguousarray(cols, dtype=np.int32), c_int_p), n_bodies,
+                                    _ptr(out, c_dbl_p))
+    if st != 0:
+        raise RuntimeError("singular body block")
+    return out[:n_bodies].reshape(-1, 12, 12)
+
+
 def mass_diag(scene, hess_offset=0):
     n = scene.n
     vals = np.zeros((n, 9))
@@ -192,19 +262,31 @@ def mass_diag(scene, hess_offset=0):
 
 def triplets(scene, with_grad=False):
     """All triplets in the reference's arena order (GIPC.cu:10291-10485):
-    contacts, tets, bending hinges, cloth triangles, mass."""
-    parts = [barrier_pt(scene, with_grad=with_grad)]
+    contacts, affine bodies, tets, bending hinges, cloth triangles, mass.  On
+    a scene with affine bodies the contact blocks are mapped to the system
+    rows by class first, and the FEM rows start at 4 * n_bodies."""
+    nb = getattr(scene, "n_bodies", 0)
+    off = 4 * nb
+    bv, br, bc, bg = barrier_pt(scene, with_grad=with_grad and not nb)
+    if nb:
+        bv, br, bc = abd_contact_map(scene, bv, br, bc)
+        bg = None
+    parts = [(bv, br, bc, bg)]
+    if nb:
+        parts.append(abd_body(scene, with_grad=with_grad))
     if getattr(scene, "n_tets", 0):
-        parts.append(tet_snh(scene, with_grad=with_grad))
+        parts.append(tet_snh(scene, hess_offset=off, with_grad=with_grad))
     if getattr(scene, "n_hinges", 0):
-        parts.append(quad_bending(scene, with_grad=with_grad))
+        parts.append(quad_bending(scene, hess_offset=off, with_grad=with_grad))
     if getattr(scene, "n_tris", 0):
-        parts.append(tri_bw(scene, with_grad=with_grad))
-    mv, mr, mc = mass_diag(scene)
+        parts.append(tri_bw(scene, hess_offset=off, with_grad=with_grad))
+    mv, mr, mc = mass_diag(scene, hess_offset=off)
     vals = np.concatenate([p[0] for p in parts] + [mv])
     rows = np.concatenate([p[1] for p in parts] + [mr])
     cols = np.concatenate([p[2] for p in parts] + [mc])
-    grad = sum(p[3] for p in parts) if with_grad else None
+    grad = None
+    if with_grad and not nb:
+        grad = sum(p[3] for p in parts)
     return vals, rows, cols, grad
 
 
@@ -243,6 +325,7 @@ class Mas:
         if nbr is None:
             s = scene_or_n
             n, nbr = s.n, (s.nbr_start, s.nbr_num, s.nbr_list)
+            collision_node_offset = collision_node_offset or getattr(s, "n_abd_points", 0)
         else:
             n = int(scene_or_n)
         self.n, self.B = n, bank_size
@@ -327,14 +410,48 @@ def pcg(vals, rows, cols, n_rows, b, mas=None, tol=1e-5, stop_mode=1, max_iter=N
     return x, int(stats[0]), float(stats[1])
 
 
+class _Precond(ctypes.Structure):
+    _fields_ = [("mas", ctypes.c_void_p), ("mas_row_offset", ctypes.c_int), ("mas_fp32", ctypes.c_int),
+                ("n_bodies", ctypes.c_int), ("abd_inv", c_dbl_p)]
+
+
+def pcg2(vals, rows, cols, n_rows, b, mas=None, mas_row_offset=0, abd_inv=None, tol=1e-5, stop_mode=1,
+         max_iter=None, mas_fp32=False):
+    """PCG with the subsystem preconditioner (block-Jacobi on the body rows,
+    MAS on the rows from mas_row_offset)."""
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros_like(b)
+    if max_iter is None:
+        max_iter = max(2, int(0.3 * b.size))
+    stats = np.zeros(4)
+    inv = None if abd_inv is None else np.ascontiguousarray(abd_inv, dtype=np.float64).reshape(-1)
+    pc = _Precond(mas.h if mas is not None else None, mas_row_offset, int(bool(mas_fp32)),
+                  0 if abd_inv is None else int(abd_inv.shape[0]), _ptr(inv, c_dbl_p))
+    lib().orc_pcg2(rows.size, _ptr(np.ascontiguousarray(vals), c_dbl_p), _ptr(rows, c_int_p), _ptr(cols, c_int_p),
+                   n_rows, ctypes.byref(pc), _ptr(b, c_dbl_p), _ptr(x, c_dbl_p), tol, stop_mode, max_iter,
+                   _ptr(stats, c_dbl_p))
+    return x, int(stats[0]), float(stats[1])
+
+
 def solve_scene(scene, bank_size=32, tol=1e-5, stop_mode=1, mas_fp32=False, max_iter=None):
-    """Assemble, build MAS and run PCG on the CPU (the plain FP64 path)."""
+    """Assemble, build the preconditioner and run PCG on the CPU (the plain
+    FP64 path).  Scenes with affine bodies use block-Jacobi on the body rows
+    and MAS on the FEM rows."""
     t0 = time.perf_counter()
     vals, rows, cols = assemble(scene)
-    mas = Mas(scene, bank_size)
-    mas.build(vals, rows, cols, 0, scene.pairs)
-    t1 = time.perf_counter()
-    x, it, res = pcg(vals, rows, cols, scene.n, scene.rhs, mas, tol, stop_mode, max_iter, mas_fp32)
+    nb = getattr(scene, "n_bodies", 0)
+    if nb:
+        mas = Mas(scene, bank_size, collision_node_offset=scene.n_abd_points)
+        mas.build(vals, rows, cols, 4 * nb, scene.pairs)
+        inv = abd_jacobi(vals, rows, cols, nb)
+        t1 = time.perf_counter()
+        x, it, res = pcg2(vals, rows, cols, scene.n_sys, scene.rhs, mas, 4 * nb, inv, tol, stop_mode, max_iter,
+                          mas_fp32)
+    else:
+        mas = Mas(scene, bank_size)
+        mas.build(vals, rows, cols, 0, scene.pairs)
+        t1 = time.perf_counter()
+        x, it, res = pcg(vals, rows, cols, scene.n, scene.rhs, mas, tol, stop_mode, max_iter, mas_fp32)
     t2 = time.perf_counter()
     mas.close()
     return OracleSolve(x=x, iterations=it, true_rel_residual=res, seconds=t2 - t0, build_seconds=t1 - t0)

@@ -499,6 +499,276 @@ int orc_quad_bending(int n_edges, const double* x, const double* rest_x, const i
 }
 
 /* ------------------------------------------------------------------------- */
+/* Affine body dynamics (ABD)                                                 */
+/* StiffGIPC/abd_system/abd_system_function/setup_abd_system_gradient_and_    */
+/* hessian.cu:200-289 (body term), 87-141 (J^T H J), 424-558 (contact classes) */
+/* , 560-655 (block-Jacobi); abd_system/details/abd_energy.inl (shape energy),*/
+/* details/abd_jacobi_matrix.inl (J).  q = [p, a1, a2, a3]; a point with rest */
+/* coordinates xb sits at x = p + (xb.a1, xb.a2, xb.a3), J = [I | e_r xb^T].   */
+/* mass10 = (m, m xb (3), m xb xb^T upper (xx, xy, xz, yy, yz, zz)) summed     */
+/* over the body's mass points; M = sum m J^T J.                               */
+/* ------------------------------------------------------------------------- */
+static void abd_mass12(const double* m10, double M[144])
+{
+    memset(M, 0, 144 * sizeof(double));
+    double mm = m10[0];
+    const double* mx = m10 + 1;
+    double D[3][3] = {{m10[4], m10[5], m10[6]}, {m10[5], m10[7], m10[8]}, {m10[6], m10[8], m10[9]}};
+    for (int r = 0; r < 3; ++r) M[r * 12 + r] = mm;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            M[r * 12 + 3 + 3 * r + c] = mx[c];
+            M[(3 + 3 * r + c) * 12 + r] = mx[c];
+        }
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+            for (int d = 0; d < 3; ++d) M[(3 + 3 * r + c) * 12 + 3 + 3 * r + d] = D[c][d];
+}
+
+double orc_abd_shape_energy(const double* q)
+{
+    const double *a1 = q + 3, *a2 = q + 6, *a3 = q + 9;
+#define D3(u, v) ((u)[0] * (v)[0] + (u)[1] * (v)[1] + (u)[2] * (v)[2])
+    double e = 0.0, t;
+    t = D3(a1, a1) - 1.0; e += t * t;
+    t = D3(a2, a2) - 1.0; e += t * t;
+    t = D3(a3, a3) - 1.0; e += t * t;
+    t = D3(a1, a2); e += 2.0 * t * t;
+    t = D3(a2, a3); e += 2.0 * t * t;
+    t = D3(a3, a1); e += 2.0 * t * t;
+    return e;
+}
+
+/* gradient (9) and unprojected Hessian (9x9 row-major) over (a1, a2, a3) */
+void orc_abd_shape_grad_hess(const double* q, double* g9, double* H81)
+{
+    const double* a[3] = {q + 3, q + 6, q + 9};
+    for (int i = 0; i < 3; ++i) {
+        const double* ai = a[i];
+        for (int d = 0; d < 3; ++d) {
+            double v = 4.0 * (D3(ai, ai) - 1.0) * ai[d];
+            for (int j = 0; j < 3; ++j)
+                if (j != i) v += 4.0 * D3(a[j], ai) * a[j][d];
+            if (g9) g9[3 * i + d] = v;
+        }
+    }
+    if (!H81) return;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            for (int c = 0; c < 3; ++c)
+                for (int d = 0; d < 3; ++d) {
+                    double v;
+                    if (i == j) {
+                        v = 8.0 * a[i][c] * a[i][d] + (c == d ? 4.0 * (D3(a[i], a[i]) - 1.0) : 0.0);
+                        for (int k = 0; k < 3; ++k)
+                            if (k != i) v += 4.0 * a[k][c] * a[k][d];
+                    } else {
+                        /* d2/dai daj of 2 (ai.aj)^2 = 4 (aj ai^T + (ai.aj) I) */
+                        v = 4.0 * a[j][c] * a[i][d] + (c == d ? 4.0 * D3(a[i], a[j]) : 0.0);
+                    }
+                    H81[(3 * i + c) * 9 + 3 * j + d] = v;
+                }
+}
+
+static void project_psd_n(int n, double* H)
+{
+    double A[144], V[144];
+    memcpy(A, H, sizeof(double) * n * n);
+    for (int i = 0; i < n; ++i)
+        for (int j = i + 1; j < n; ++j) {
+            double sm = 0.5 * (A[i * n + j] + A[j * n + i]);
+            A[i * n + j] = A[j * n + i] = sm;
+        }
+    jacobi_eig(n, A, V);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            double sm = 0.0;
+            for (int k = 0; k < n; ++k)
+                if (A[k * n + k] > 0.0) sm += V[i * n + k] * A[k * n + k] * V[j * n + k];
+            H[i * n + j] = sm;
+        }
+}
+
+int orc_abd_body(int n_bodies, const double* q, const double* q_tilde, const double* mass10, const double* volume,
+                 double kappa, double dt, const int* fixed, double* grad, double* tvals, int* trows, int* tcols)
+{
+#pragma omp parallel for schedule(static)
+    for (int b = 0; b < n_bodies; ++b) {
+        double M[144], H[144], G[12];
+        abd_mass12(mass10 + 10 * (size_t)b, M);
+        memcpy(H, M, sizeof(H));
+        for (int i = 0; i < 12; ++i) G[i] = 0.0;
+        if (!(fixed && fixed[b])) {
+            const double* qb = q + 12 * (size_t)b;
+            const double* qt = q_tilde + 12 * (size_t)b;
+            for (int i = 0; i < 12; ++i) {
+                double sm = 0.0;
+                for (int j = 0; j < 12; ++j) sm += M[i * 12 + j] * (qb[j] - qt[j]);
+                G[i] = sm;
+            }
+            double kvt2 = kappa * volume[b] * dt * dt;
+            double g9[9], H9[81];
+            orc_abd_shape_grad_hess(qb, g9, H9);
+            for (int i = 0; i < 81; ++i) H9[i] *= kvt2;
+            project_psd_n(9, H9);
+            for (int i = 0; i < 9; ++i) {
+                G[3 + i] += kvt2 * g9[i];
+                for (int j = 0; j < 9; ++j) H[(3 + i) * 12 + 3 + j] += H9[i * 9 + j];
+            }
+        }
+        if (grad)
+            for (int i = 0; i < 12; ++i) grad[12 * (size_t)b + i] = G[i];
+        int kk = 0;
+        for (int ii = 0; ii < 4; ++ii)
+            for (int jj = ii; jj < 4; ++jj, ++kk) {
+                size_t o = 10 * (size_t)b + kk;
+                trows[o] = 4 * b + ii;
+                tcols[o] = 4 * b + jj;
+                for (int r = 0; r < 3; ++r)
+                    for (int c = 0; c < 3; ++c) tvals[9 * o + c * 3 + r] = H[(3 * ii + r) * 12 + 3 * jj + c];
+            }
+    }
+    return 0;
+}
+
+/* J^T H J between points with rest coordinates x (left) and y (right); 12x12 row-major */
+static void abd_jt_h_j(const double* x, const double* Hrm /* 3x3 row-major */, const double* y, double* out)
+{
+    double J1[36], J2[36];
+    memset(J1, 0, sizeof(J1));
+    memset(J2, 0, sizeof(J2));
+    for (int r = 0; r < 3; ++r) {
+        J1[r * 12 + r] = 1.0;
+        J2[r * 12 + r] = 1.0;
+        for (int c = 0; c < 3; ++c) {
+            J1[r * 12 + 3 + 3 * r + c] = x[c];
+            J2[r * 12 + 3 + 3 * r + c] = y[c];
+        }
+    }
+    for (int i = 0; i < 12; ++i)
+        for (int j = 0; j < 12; ++j) {
+            double sm = 0.0;
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) sm += J1[a * 12 + i] * Hrm[a * 3 + b] * J2[b * 12 + j];
+            out[i * 12 + j] = sm;
+        }
+}
+
+long orc_abd_contact_map(long n_trip, const double* vals, const int* rows, const int* cols, int n_abd_points,
+                         const int* point_body, const double* x_bar, int n_bodies, const int* body_fixed,
+                         double* ovals, int* orows, int* ocols)
+{
+    long o = 0;
+    const int fem0 = 4 * n_bodies - n_abd_points;  /* FEM point id -> system row */
+    for (int cls = 0; cls < 3; ++cls)
+        for (long k = 0; k < n_trip; ++k) {
+            int i = rows[k], j = cols[k];
+            int ia = i < n_abd_points, ja = j < n_abd_points;
+            const double* v = vals + 9 * (size_t)k;
+            double Hrm[9];
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) Hrm[r * 3 + c] = v[c * 3 + r];
+            if (cls == 0 && !ia && !ja) {
+                orows[o] = i + fem0;
+                ocols[o] = j + fem0;
+                memcpy(ovals + 9 * (size_t)o, v, 9 * sizeof(double));
+                ++o;
+            } else if (cls == 1 && ia && !ja) {
+                int bi = point_body[i];
+                int zero = body_fixed && body_fixed[bi];
+                const double* xb = x_bar + 3 * (size_t)i;
+                /* (J^T H) rows 3a..3a+2: a = 0 -> H, a = 1 + r -> xb (x) H row r */
+                for (int a = 0; a < 4; ++a) {
+                    double* d = ovals + 9 * (size_t)o;
+                    for (int r = 0; r < 3; ++r)
+                        for (int c = 0; c < 3; ++c) {
+                            double val = a == 0 ? Hrm[r * 3 + c] : xb[r] * Hrm[(a - 1) * 3 + c];
+                            d[c * 3 + r] = zero ? 0.0 : val;
+                        }
+                    orows[o] = 4 * bi + a;
+                    ocols[o] = j + fem0;
+                    ++o;
+                }
+            } else if (cls == 2 && ia && ja) {
+                int bi = point_body[i], bj = point_body[j];
+                int zero = body_fixed && (body_fixed[bi] || body_fixed[bj]);
+                double X[144];
+                abd_jt_h_j(x_bar + 3 * (size_t)i, Hrm, x_bar + 3 * (size_t)j, X);
+                if (bi == bj && i != j) {
+                    /* the stored upper vertex block stands for (i,j) and (j,i): X + X^T */
+                    double T[144];
+                    for (int a = 0; a < 12; ++a)
+                        for (int b = 0; b < 12; ++b) T[a * 12 + b] = X[a * 12 + b] + X[b * 12 + a];
+                    memcpy(X, T, sizeof(T));
+                }
+                for (int a = 0; a < 4; ++a)
+                    for (int b = 0; b < 4; ++b) {
+                        int row = 4 * bi + a, col = 4 * bj + b;
+                        double* d = ovals + 9 * (size_t)o;
+                        int keep = row <= col && !zero;
+                        orows[o] = row <= col ? row : col;
+                        ocols[o] = row <= col ? col : row;
+                        for (int r = 0; r < 3; ++r)
+                            for (int c = 0; c < 3; ++c) d[c * 3 + r] = keep ? X[(3 * a + r) * 12 + 3 * b + c] : 0.0;
+                        ++o;
+                    }
+            }
+        }
+    return o;
+}
+
+/* 12x12 inverse by Gauss-Jordan with partial pivoting (row-major, in place) */
+static int inv12(double* A)
+{
+    double I[144];
+    int n = 12;
+    memset(I, 0, sizeof(I));
+    for (int i = 0; i < n; ++i) I[i * n + i] = 1.0;
+    for (int c = 0; c < n; ++c) {
+        int pr = c;
+        for (int r = c + 1; r < n; ++r)
+            if (fabs(A[r * n + c]) > fabs(A[pr * n + c])) pr = r;
+        if (A[pr * n + c] == 0.0) return -1;
+        if (pr != c)
+            for (int k = 0; k < n; ++k) {
+                double t = A[c * n + k]; A[c * n + k] = A[pr * n + k]; A[pr * n + k] = t;
+                t = I[c * n + k]; I[c * n + k] = I[pr * n + k]; I[pr * n + k] = t;
+            }
+        double piv = 1.0 / A[c * n + c];
+        for (int k = 0; k < n; ++k) { A[c * n + k] *= piv; I[c * n + k] *= piv; }
+        for (int r = 0; r < n; ++r) {
+            if (r == c) continue;
+            double f = A[r * n + c];
+            if (f == 0.0) continue;
+            for (int k = 0; k < n; ++k) { A[r * n + k] -= f * A[c * n + k]; I[r * n + k] -= f * I[c * n + k]; }
+        }
+    }
+    memcpy(A, I, sizeof(I));
+    return 0;
+}
+
+int orc_abd_jacobi_build(long nnz, const double* vals, const int* rows, const int* cols, int n_bodies,
+                         double* inv144)
+{
+    memset(inv144, 0, sizeof(double) * 144 * (size_t)n_bodies);
+    for (long k = 0; k < nnz; ++k) {
+        int r = rows[k], c = cols[k];
+        if (r >= 4 * n_bodies || c >= 4 * n_bodies || r / 4 != c / 4) continue;
+        double* P = inv144 + 144 * (size_t)(r / 4);
+        const double* v = vals + 9 * (size_t)k;
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) {
+                P[(3 * (r % 4) + a) * 12 + 3 * (c % 4) + b] = v[b * 3 + a];
+                if (r != c) P[(3 * (c % 4) + b) * 12 + 3 * (r % 4) + a] = v[b * 3 + a];
+            }
+    }
+    int bad = 0;
+    for (int b = 0; b < n_bodies; ++b)
+        if (inv12(inv144 + 144 * (size_t)b)) bad = 1;
+    return bad ? -1 : 0;
+}
+
+/* ------------------------------------------------------------------------- */
 /* Point-triangle barrier, GIPC rank-2 variant                               */
 /* StiffGIPC/GIPC.cu:4733-4984 (NEWF branch), mlbvh.cu:234-242 (_d_PT),       */
 /* GIPC_PDerivative.cuh:2095-2291 (only column 8 of dF/dx is non-zero: the    */
@@ -1101,17 +1371,44 @@ void orc_mas_matrices(const orc_mas* m, double* out, int inverse)
 /* ------------------------------------------------------------------------- */
 /* PCGSolver::pcg -- StiffGIPC/linear_system/solver/pcg_solver.cu:191-332     */
 /* ------------------------------------------------------------------------- */
+/* fixed 64-way split summed in order: the result does not depend on the
+ * thread count or on how OpenMP combines reductions */
 static double dot(long n, const double* a, const double* b)
 {
+    double part[64];
+    const long chunk = (n + 63) / 64;
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < 64; ++c) {
+        double s = 0;
+        const long lo = c * chunk, hi = lo + chunk < n ? lo + chunk : n;
+        for (long i = lo; i < hi; ++i) s += a[i] * b[i];
+        part[c] = s;
+    }
     double s = 0;
-#pragma omp parallel for reduction(+ : s) schedule(static)
-    for (long i = 0; i < n; ++i) s += a[i] * b[i];
+    for (int c = 0; c < 64; ++c) s += part[c];
     return s;
 }
 
-int orc_pcg(long nnz, const double* vals, const int* rows, const int* cols, int n_rows,
-            orc_mas* mas, const double* b, double* x, double tol, int stop_mode, int max_iter,
-            int mas_fp32, double* stats)
+static void precond_apply(const orc_precond* pc, long n3, const double* r, double* z)
+{
+    memcpy(z, r, sizeof(double) * n3);
+    if (!pc) return;
+    if (pc->abd_inv) {
+        for (int b = 0; b < pc->n_bodies; ++b) {
+            const double* P = pc->abd_inv + 144 * (size_t)b;
+            for (int i = 0; i < 12; ++i) {
+                double sm = 0.0;
+                for (int j = 0; j < 12; ++j) sm += P[i * 12 + j] * r[12 * (size_t)b + j];
+                z[12 * (size_t)b + i] = sm;
+            }
+        }
+    }
+    if (pc->mas) orc_mas_apply(pc->mas, r + 3 * (size_t)pc->mas_row_offset, z + 3 * (size_t)pc->mas_row_offset,
+                               pc->mas_fp32);
+}
+
+int orc_pcg2(long nnz, const double* vals, const int* rows, const int* cols, int n_rows, const orc_precond* pc,
+             const double* b, double* x, double tol, int stop_mode, int max_iter, double* stats)
 {
     long n3 = 3L * n_rows;
     double* r = (double*)malloc(sizeof(double) * n3);
@@ -1123,8 +1420,7 @@ int orc_pcg(long nnz, const double* vals, const int* rows, const int* cols, int 
     int k = 0, iters = 0;
     double rz = 0, rz0 = 0;
     if (bnorm > 0) {
-        if (mas) orc_mas_apply(mas, r, z, mas_fp32);
-        else memcpy(z, r, sizeof(double) * n3);
+        precond_apply(pc, n3, r, z);
         rz = dot(n3, r, z);
         memcpy(p, z, sizeof(double) * n3);
         rz0 = rz;
@@ -1142,8 +1438,7 @@ int orc_pcg(long nnz, const double* vals, const int* rows, const int* cols, int 
             } else {
                 if (sqrt(dot(n3, r, r)) <= tol * bnorm) break;
             }
-            if (mas) orc_mas_apply(mas, r, z, mas_fp32);
-            else memcpy(z, r, sizeof(double) * n3);
+            precond_apply(pc, n3, r, z);
             double rz_new = dot(n3, r, z);
             double beta = rz_new / rz;
 #pragma omp parallel for schedule(static)
@@ -1162,4 +1457,12 @@ int orc_pcg(long nnz, const double* vals, const int* rows, const int* cols, int 
     }
     free(r); free(z); free(p); free(Ap);
     return k;
+}
+
+int orc_pcg(long nnz, const double* vals, const int* rows, const int* cols, int n_rows,
+            orc_mas* mas, const double* b, double* x, double tol, int stop_mode, int max_iter,
+            int mas_fp32, double* stats)
+{
+    orc_precond pc = {mas, 0, mas_fp32, 0, NULL};
+    return orc_pcg2(nnz, vals, rows, cols, n_rows, mas ? &pc : NULL, b, x, tol, stop_mode, max_iter, stats);
 }

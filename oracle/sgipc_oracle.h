@@ -123,6 +123,45 @@ int orc_pcg(long nnz, const double* vals, const int* rows, const int* cols, int 
             orc_mas* mas, const double* b, double* x, double tol, int stop_mode, int max_iter,
             int mas_fp32, double* stats);
 
+/* ---- affine bodies (ABD) ---------------------------------------------------
+ * Body term: 10 upper blocks per body at rows 4b..4b+3 (kinetic M(q - q~) and
+ * the PSD-projected orthogonality energy kappa*vol*dt^2 V(a)); grad (optional)
+ * receives 12 per body.  setup_abd_system_gradient_and_hessian.cu:200-289. */
+int orc_abd_body(int n_bodies, const double* q, const double* q_tilde, const double* mass10,
+                 const double* volume, double kappa, double dt, const int* fixed, double* grad,
+                 double* tvals, int* trows, int* tcols);
+double orc_abd_shape_energy(const double* q12);
+void orc_abd_shape_grad_hess(const double* q12, double* g9, double* H81);
+/* Vertex-level contact triplets (upper, as the barrier writes them) -> system
+ * triplets by class, in the order FEM-FEM (1 block), ABD-FEM (4 blocks J^T H),
+ * ABD-ABD (16 blocks J_i^T H J_j, lower ones zeroed at (col,row); a same-body
+ * pair i != j stands for both (i,j) and (j,i)).  Point ids < n_abd_points are
+ * ABD points; FEM point j goes to row 4*n_bodies + j - n_abd_points.  Returns
+ * the count.  GIPC.cu:10080-10256, setup_abd_system_gradient_and_hessian.cu:
+ * 87-141,424-558. */
+long orc_abd_contact_map(long n_trip, const double* vals, const int* rows, const int* cols,
+                         int n_abd_points, const int* point_body, const double* x_bar, int n_bodies,
+                         const int* body_fixed, double* ovals, int* orows, int* ocols);
+/* Inverse of every body's 12x12 diagonal block of the assembled BCOO
+ * (row-major, 144 per body).  setup_abd_system_gradient_and_hessian.cu:560-655. */
+int orc_abd_jacobi_build(long nnz, const double* vals, const int* rows, const int* cols, int n_bodies,
+                         double* inv144);
+
+/* PCG with the subsystem preconditioner: z = r, then block-Jacobi on the
+ * body rows [0, 4*n_bodies) when abd_inv is set and MAS on the rows from
+ * mas_row_offset when mas is set (GlobalLinearSystem::apply_preconditioner,
+ * global_linear_system.cu:150-220). */
+typedef struct orc_precond {
+    orc_mas* mas;
+    int mas_row_offset;
+    int mas_fp32;
+    int n_bodies;
+    const double* abd_inv;
+} orc_precond;
+int orc_pcg2(long nnz, const double* vals, const int* rows, const int* cols, int n_rows,
+             const orc_precond* pc, const double* b, double* x, double tol, int stop_mode, int max_iter,
+             double* stats);
+
 /* Deliberate-mistake switch for the tests that show the checks are sensitive:
  * 0 = none, 1 = SpMV drops the transposed (lower) contribution, 2 = Galerkin
  * coarse levels drop H^T on merged diagonals, 3 = barrier Hessian uses lambda0

@@ -46,7 +46,9 @@ class SceneDesc(ctypes.Structure):
                 ("nbr_list", vp),
                 ("n_tris", ci), ("k_stretch", cd), ("k_shear", cd), ("strain_rate", cd), ("tris", vp),
                 ("dm_inv2", vp), ("area", vp),
-                ("n_hinges", ci), ("k_bend", cd), ("rest_x", vp), ("hinge_edges", vp), ("hinge_adj", vp)]
+                ("n_hinges", ci), ("k_bend", cd), ("rest_x", vp), ("hinge_edges", vp), ("hinge_adj", vp),
+                ("n_bodies", ci), ("n_abd_points", ci), ("kappa_abd", cd), ("q", vp), ("q_tilde", vp),
+                ("mass10", vp), ("body_volume", vp), ("body_fixed", vp), ("point_body", vp), ("x_bar", vp)]
 
 
 class SolveTiming(ctypes.Structure):
@@ -72,6 +74,10 @@ _SIGS = {
     "sgipc_fem_tri_gradient_hessian": (ci, [vp, vp, vp, vp, vp, ci, cd, cd, cd, cd, ci, vp, vp, vp, vp]),
     "sgipc_quad_bending_prepare": (ci, [vp, vp, vp, vp, ci, vp]),
     "sgipc_quad_bending_gradient_hessian": (ci, [vp, vp, vp, vp, vp, vp, ci, cd, cd, ci, vp, vp, vp, vp]),
+    "sgipc_abd_body_gradient_hessian": (ci, [vp, ci, vp, vp, vp, vp, cd, cd, vp, vp, vp, vp, vp]),
+    "sgipc_contact_partition": (ci, [vp, vp, vp, vp, ci, ci, vp, vp, ci, vp, vp, vp, vp, ci, ctypes.POINTER(ci)]),
+    "sgipc_abd_jacobi_build": (ci, [vp, vp, vp, vp, ci, ci, vp]),
+    "sgipc_abd_jacobi_apply": (ci, [vp, vp, ci, vp, vp]),
     "sgipc_mass_diagonal": (ci, [vp, vp, ci, ci, vp, vp, vp]),
     "sgipc_convert_srbk": (ci, [vp, vp, vp, vp, ci, vp, vp, vp, ci, ctypes.POINTER(ci)]),
     "sgipc_spmv_srbk": (ci, [vp, cd, vp, vp, vp, ci, vp, cd, vp, ci]),

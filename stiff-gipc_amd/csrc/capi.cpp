@@ -411,9 +411,10 @@ namespace sgipc {
 
 // Shared by sgipc_pcg_solve and the linear-system object.
 int pcg_solve_impl(Context& c, SolveWorkspace& ws, const double* vals, const int* rows, const int* cols,
-                   int nnz, int n_rows, Mas* mas, const double* b, double* x, const sgipc_pcg_opts& o,
-                   sgipc_pcg_stats* stats)
+                   int nnz, int n_rows, const SubsystemPrec& prec, const double* b, double* x,
+                   const sgipc_pcg_opts& o, sgipc_pcg_stats* stats)
 {
+    const bool has_prec = prec.mas != nullptr || (prec.abd != nullptr && prec.abd->n_bodies > 0);
     auto t0 = std::chrono::steady_clock::now();
     hipStream_t s = c.stream;
     const long n3 = 3L * n_rows;
@@ -428,13 +429,13 @@ int pcg_solve_impl(Context& c, SolveWorkspace& ws, const double* vals, const int
         A.nnz = ws.full.nnz;
         A.max_partials = ws.full.partials();
         A.m = &ws.full;
-        MasPrecond<double> M(mas);
+        SubsystemPrecond<double> M(prec);
         PcgOptions po;
         po.tol = o.tol;
         po.stop_mode = o.stop_mode;
         po.max_iter = max_iter;
         po.check_every = o.check_every;
-        PcgResult r = ws.pcg64.solve(s, A, mas ? &M : nullptr, b, x, n3, po);
+        PcgResult r = ws.pcg64.solve(s, A, has_prec ? &M : nullptr, b, x, n3, po);
         st.iterations = r.iterations;
         st.outer_iterations = 1;
         st.rz = r.rz;
@@ -452,7 +453,7 @@ int pcg_solve_impl(Context& c, SolveWorkspace& ws, const double* vals, const int
         A.nnz = ws.full.nnz;
         A.max_partials = ws.full.partials();
         A.m = &ws.full;
-        MasPrecond<float> M(mas);
+        SubsystemPrecond<float> M(prec);
         launch_fill<double>(s, x, 0.0, n3);
         SGIPC_HIP(hipMemcpyAsync(ws.r64.get(), b, sizeof(double) * n3, hipMemcpyDeviceToDevice, s));
         double* hp = nullptr;
@@ -480,7 +481,7 @@ int pcg_solve_impl(Context& c, SolveWorkspace& ws, const double* vals, const int
             po.stop_mode = SGIPC_STOP_TRUE_RESIDUAL;
             po.max_iter = max_iter - total_it;
             po.check_every = o.check_every;
-            PcgResult r = ws.pcg32.solve(s, A, mas ? &M : nullptr, ws.r32.get(), ws.d32.get(), n3, po);
+            PcgResult r = ws.pcg32.solve(s, A, has_prec ? &M : nullptr, ws.r32.get(), ws.d32.get(), n3, po);
             total_it += r.iterations;
             if (r.not_spd) {
                 status = SGIPC_NOT_SPD;
@@ -554,7 +555,9 @@ extern "C" int sgipc_pcg_solve(sgipc_ctx* ctx, const double* vals, const int* ro
     int st = guarded(ctx, [&] {
         auto& ws = workspace_of(ctx);
         if (!ws) ws = std::make_unique<SolveWorkspace>();
-        status = pcg_solve_impl(ctx->c, *ws, vals, rows, cols, nnz, n_rows, mas ? mas->m.get() : nullptr, b, x, o,
+        SubsystemPrec prec;
+        prec.mas = mas ? mas->m.get() : nullptr;
+        status = pcg_solve_impl(ctx->c, *ws, vals, rows, cols, nnz, n_rows, prec, b, x, o,
                                 stats);
     });
     return st != SGIPC_OK ? st : status;

@@ -4,6 +4,7 @@
 #include <new>
 #include <string>
 
+#include "abd.hpp"
 #include "assembly.hpp"
 #include "mas.hpp"
 #include "pcg.hpp"
@@ -61,8 +62,35 @@ struct SolveWorkspace {
     DevBuf<double> part;
 };
 
+// The subsystem preconditioner of GlobalLinearSystem::apply_preconditioner
+// (global_linear_system.cu:150-220): z = r, block-Jacobi on the affine-body
+// rows [0, 4 nb), MAS on the FEM rows [mas_row_offset, n).
+struct SubsystemPrec {
+    Mas* mas = nullptr;
+    int mas_row_offset = 0;
+    const AbdJacobi* abd = nullptr;
+};
+
+template <typename TV>
+struct SubsystemPrecond : Preconditioner<TV> {
+    SubsystemPrec p;
+    explicit SubsystemPrecond(const SubsystemPrec& q) : p(q) {}
+    int apply(hipStream_t s, const TV* r, TV* z, double* rz_part, const PcgStopDesc& sd) override
+    {
+        int k = 0;
+        // the MAS evaluates the stop rule in its first kernel; the body
+        // blocks run after it and skip once the flag is set
+        if (p.mas)
+            k = p.mas->template apply<TV>(s, r + 3L * p.mas_row_offset, z + 3L * p.mas_row_offset, rz_part, sd);
+        if (p.abd && p.abd->n_bodies > 0)
+            k += p.abd->template apply<TV>(s, r, z, rz_part ? rz_part + k : nullptr, sd.ctl);
+        return k;
+    }
+    int max_partials() const override { return 1024 + (p.abd ? p.abd->partials() : 0); }
+};
+
 int pcg_solve_impl(Context& c, SolveWorkspace& ws, const double* vals, const int* rows, const int* cols,
-                   int nnz, int n_rows, Mas* mas, const double* b, double* x, const sgipc_pcg_opts& o,
-                   sgipc_pcg_stats* stats);
+                   int nnz, int n_rows, const SubsystemPrec& prec, const double* b, double* x,
+                   const sgipc_pcg_opts& o, sgipc_pcg_stats* stats);
 
 }  // namespace sgipc

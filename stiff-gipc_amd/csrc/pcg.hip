@@ -288,7 +288,10 @@ PcgResult PcgSolver<TV>::solve(hipStream_t s, const SpmvOp<TV>& A, Preconditione
                                const TV* b, TV* x_out, long n3, const PcgOptions& opt)
 {
     PcgResult res{};
-    reserve(n3, A.nnz, A.max_partials);
+    {
+        const int pm = M ? M->max_partials() : 0;
+        reserve(n3, A.nnz, A.max_partials > pm ? A.max_partials : pm);
+    }
     if (!ev_[0]) {
         SGIPC_HIP(hipEventCreateWithFlags(&ev_[0], hipEventDisableTiming));
         SGIPC_HIP(hipEventCreateWithFlags(&ev_[1], hipEventDisableTiming));

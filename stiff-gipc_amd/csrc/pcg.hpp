@@ -61,6 +61,8 @@ template <typename TV>
 struct Preconditioner {
     virtual ~Preconditioner() = default;
     virtual int apply(hipStream_t s, const TV* r, TV* z, double* rz_part, const PcgStopDesc& stop) = 0;
+    // upper bound of the partial count apply() returns
+    virtual int max_partials() const { return 1024; }
 };
 
 template <typename TV>

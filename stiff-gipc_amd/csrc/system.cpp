@@ -43,9 +43,16 @@ struct sgipc_system {
     DevBuf<double> bvals;
     DevBuf<int> brows, bcols;
     int nnz = 0;
-    long n_trip = 0;
-    int capacity = 0;
+    long n_trip = 0;        // triplets of the fixed (non-contact) sections
+    long arena_cap = 0;     // triplets the arena holds
+    int capacity = 0;       // BCOO capacity
+    int n_sys = 0;          // system block rows: 4 per body + FEM vertices
     Converter conv;
+    // affine bodies: point-level contact scratch, class mapping, block-Jacobi
+    DevBuf<double> cvals;
+    DevBuf<int> crows, ccols;
+    ContactMapper cmap;
+    AbdJacobi abdj;
     SolveWorkspace* ws = nullptr;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
 };
@@ -121,6 +128,68 @@ int sgipc_quad_bending_gradient_hessian(sgipc_ctx* ctx, const double* x, const d
     });
 }
 
+int sgipc_abd_body_gradient_hessian(sgipc_ctx* ctx, int n_bodies, const double* q, const double* q_tilde,
+                                    const double* mass10, const double* body_volume, double kappa, double dt,
+                                    const int* body_fixed, double* grad, double* tvals, int* trows, int* tcols)
+{
+    if (!ctx || n_bodies < 0) return SGIPC_BAD_ARG;
+    if (n_bodies == 0) return SGIPC_EMPTY;
+    if (!q || !q_tilde || !mass10 || !body_volume || !tvals || !trows || !tcols) return SGIPC_BAD_ARG;
+    return run(ctx, [&] {
+        launch_abd_body(ctx->c.stream, n_bodies, q, q_tilde, mass10, body_volume, kappa, dt, body_fixed, grad, tvals,
+                        trows, tcols);
+        SGIPC_HIP(hipStreamSynchronize(ctx->c.stream));
+    });
+}
+
+int sgipc_contact_partition(sgipc_ctx* ctx, const double* vals, const int* rows, const int* cols, int n,
+                            int n_abd_points, const int* point_body, const double* x_bar, int n_bodies,
+                            const int* body_fixed, double* ovals, int* orows, int* ocols, int capacity, int* out_n)
+{
+    if (!ctx || !out_n || n < 0 || n_abd_points < 0 || n_bodies < 0 || capacity < 0) return SGIPC_BAD_ARG;
+    *out_n = 0;
+    if (n == 0) return SGIPC_EMPTY;
+    if (!vals || !rows || !cols || !ovals || !orows || !ocols) return SGIPC_BAD_ARG;
+    if (n_abd_points > 0 && (!point_body || !x_bar)) return SGIPC_BAD_ARG;
+    return run(ctx, [&] {
+        ContactMapper cm;
+        const long total = cm.count(ctx->c.stream, rows, cols, n, n_abd_points);
+        *out_n = (int)total;
+        cm.write(ctx->c.stream, vals, rows, cols, n_abd_points, point_body, x_bar, n_bodies, body_fixed, ovals, orows,
+                 ocols, capacity);
+        SGIPC_HIP(hipStreamSynchronize(ctx->c.stream));
+    });
+}
+
+int sgipc_abd_jacobi_build(sgipc_ctx* ctx, const double* vals, const int* rows, const int* cols, int nnz,
+                           int n_bodies, double* inv144)
+{
+    if (!ctx || nnz < 0 || n_bodies < 0) return SGIPC_BAD_ARG;
+    if (n_bodies == 0) return SGIPC_EMPTY;
+    if (!vals || !rows || !cols || !inv144) return SGIPC_BAD_ARG;
+    return run(ctx, [&] {
+        DevBuf<int> bad;
+        bad.reserve(1);
+        SGIPC_HIP(hipMemsetAsync(bad.get(), 0, sizeof(int), ctx->c.stream));
+        launch_abd_jacobi_build(ctx->c.stream, vals, rows, cols, nnz, n_bodies, inv144, bad.get());
+        int h = 0;
+        SGIPC_HIP(hipMemcpyAsync(&h, bad.get(), sizeof(int), hipMemcpyDeviceToHost, ctx->c.stream));
+        SGIPC_HIP(hipStreamSynchronize(ctx->c.stream));
+        SGIPC_CHECK(h == 0, SGIPC_NOT_SPD, "singular affine-body block");
+    });
+}
+
+int sgipc_abd_jacobi_apply(sgipc_ctx* ctx, const double* inv144, int n_bodies, const double* r, double* z)
+{
+    if (!ctx || n_bodies < 0) return SGIPC_BAD_ARG;
+    if (n_bodies == 0) return SGIPC_EMPTY;
+    if (!inv144 || !r || !z) return SGIPC_BAD_ARG;
+    return run(ctx, [&] {
+        launch_abd_jacobi_apply<double>(ctx->c.stream, inv144, n_bodies, r, z, nullptr, nullptr);
+        SGIPC_HIP(hipStreamSynchronize(ctx->c.stream));
+    });
+}
+
 int sgipc_mass_diagonal(sgipc_ctx* ctx, const double* mass, int n, int hess_offset, double* tvals, int* trows,
                         int* tcols)
 {
@@ -163,6 +232,10 @@ int sgipc_system_create(sgipc_ctx* ctx, sgipc_system** out, const sgipc_scene_de
     if (d.n_tris < 0 || d.n_hinges < 0) return SGIPC_BAD_ARG;
     if (d.n_tris > 0 && (!d.tris || !d.dm_inv2 || !d.area)) return SGIPC_BAD_ARG;
     if (d.n_hinges > 0 && (!d.rest_x || !d.hinge_edges || !d.hinge_adj)) return SGIPC_BAD_ARG;
+    if (d.n_bodies < 0 || d.n_abd_points < 0) return SGIPC_BAD_ARG;
+    if (d.n_bodies > 0 && (!d.q || !d.q_tilde || !d.mass10 || !d.body_volume)) return SGIPC_BAD_ARG;
+    if (d.n_abd_points > 0 && (!d.point_body || !d.x_bar || d.n_bodies == 0)) return SGIPC_BAD_ARG;
+    if (d.n_bodies > 0 && d.n_pairs > 0 && !d.pair_x) return SGIPC_BAD_ARG;
     if (bank_size != 16 && bank_size != 32) return SGIPC_BAD_ARG;
     auto* sys = new (std::nothrow) sgipc_system();
     if (!sys) return SGIPC_HIP_ERROR;
@@ -172,19 +245,29 @@ int sgipc_system_create(sgipc_ctx* ctx, sgipc_system** out, const sgipc_scene_de
     int st = sgipc_mas_create(ctx, &sys->mas, bank_size);
     if (st == SGIPC_OK)
         st = run(ctx, [&] {
-            sys->mas->m->init_neighbor(d.n_verts, 0, d.total_neighbor_num, d.nbr_start, d.nbr_num, d.nbr_list);
+            sys->n_sys = 4 * d.n_bodies + d.n_verts;
+            sys->mas->m->init_neighbor(d.n_verts, d.n_abd_points, d.total_neighbor_num, d.nbr_start, d.nbr_num,
+                                       d.nbr_list);
             sys->mas->m->init_matrix();
-            sys->n_trip = 16L * d.n_pairs + 10L * d.n_tets + 10L * d.n_hinges + 6L * d.n_tris + d.n_verts;
+            sys->n_trip = 10L * d.n_bodies + 10L * d.n_tets + 10L * d.n_hinges + 6L * d.n_tris + d.n_verts;
             if (d.n_hinges > 0) {
                 sys->hinge_q.reserve(16L * d.n_hinges);
                 launch_quad_bending_q(ctx->c.stream, d.n_hinges, d.rest_x, d.hinge_edges, d.hinge_adj,
                                       sys->hinge_q.get());
             }
-            SGIPC_CHECK(sys->n_trip < (1L << 31), SGIPC_CAPACITY, "triplet count exceeds 2^31");
-            sys->tvals.reserve((size_t)sys->n_trip * 9);
-            sys->trows.reserve(sys->n_trip);
-            sys->tcols.reserve(sys->n_trip);
-            sys->capacity = (int)sys->n_trip;
+            long contact = 16L * d.n_pairs;
+            if (d.n_bodies > 0) {
+                sys->cvals.reserve((size_t)(contact > 0 ? contact : 1) * 9);
+                sys->crows.reserve(contact > 0 ? contact : 1);
+                sys->ccols.reserve(contact > 0 ? contact : 1);
+                contact *= 4;  // first guess of the mapped blocks; grown on demand
+            }
+            sys->arena_cap = sys->n_trip + contact;
+            SGIPC_CHECK(sys->arena_cap < (1L << 31), SGIPC_CAPACITY, "triplet count exceeds 2^31");
+            sys->tvals.reserve((size_t)sys->arena_cap * 9);
+            sys->trows.reserve(sys->arena_cap);
+            sys->tcols.reserve(sys->arena_cap);
+            sys->capacity = (int)sys->arena_cap;
             sys->bvals.reserve((size_t)sys->capacity * 9);
             sys->brows.reserve(sys->capacity);
             sys->bcols.reserve(sys->capacity);
@@ -236,38 +319,71 @@ int sgipc_system_solve(sgipc_system* sys, const double* b, double* x, double* gr
         hipStream_t s = sys->ctx->c.stream;
         auto t0 = std::chrono::steady_clock::now();
         SGIPC_HIP(hipEventRecord(sys->ev[0], s));
-        // ---- assembly (arena order: contacts, tets, hinges, triangles, mass)
-        if (grad) launch_fill<double>(s, grad, 0.0, 3L * d.n_verts);
+        // ---- assembly (arena order: contacts, bodies, tets, hinges, triangles, mass)
+        const int nb = d.n_bodies;
+        const int off0 = 4 * nb;  // FEM block rows start after the bodies
+        if (grad) launch_fill<double>(s, grad, 0.0, 3L * sys->n_sys);
         long off = 0;
-        launch_barrier_pt(s, d.n_pairs, d.x, d.pairs, d.pair_x, d.dhat_sq, d.kappa, 0, grad, sys->tvals.get(),
-                          sys->trows.get(), sys->tcols.get());
-        off += 16L * d.n_pairs;
-        launch_tet_snh(s, d.n_tets, d.x, d.tets, d.dm_inv, d.vol, d.mu, d.lam, d.dt, 0, grad,
+        if (nb == 0) {
+            launch_barrier_pt(s, d.n_pairs, d.x, d.pairs, d.pair_x, d.dhat_sq, d.kappa, 0, grad, sys->tvals.get(),
+                              sys->trows.get(), sys->tcols.get());
+            off += 16L * d.n_pairs;
+        } else {
+            SGIPC_CHECK(grad == nullptr, SGIPC_BAD_ARG, "the gradient is not assembled for scenes with affine bodies");
+            // point-level contact blocks, then the class mapping into the arena
+            launch_barrier_pt(s, d.n_pairs, d.x, d.pairs, d.pair_x, d.dhat_sq, d.kappa, 0, nullptr, sys->cvals.get(),
+                              sys->crows.get(), sys->ccols.get());
+            const long m = sys->cmap.count(s, sys->crows.get(), sys->ccols.get(), 16L * d.n_pairs, d.n_abd_points);
+            if (m + sys->n_trip > sys->arena_cap) {
+                sys->arena_cap = m + sys->n_trip;
+                SGIPC_CHECK(sys->arena_cap < (1L << 31), SGIPC_CAPACITY, "triplet count exceeds 2^31");
+                sys->tvals.reserve((size_t)sys->arena_cap * 9);
+                sys->trows.reserve(sys->arena_cap);
+                sys->tcols.reserve(sys->arena_cap);
+                sys->capacity = (int)sys->arena_cap;
+                sys->bvals.reserve((size_t)sys->capacity * 9);
+                sys->brows.reserve(sys->capacity);
+                sys->bcols.reserve(sys->capacity);
+            }
+            sys->cmap.write(s, sys->cvals.get(), sys->crows.get(), sys->ccols.get(), d.n_abd_points, d.point_body,
+                            d.x_bar, nb, d.body_fixed, sys->tvals.get(), sys->trows.get(), sys->tcols.get(), m);
+            off += m;
+            launch_abd_body(s, nb, d.q, d.q_tilde, d.mass10, d.body_volume, d.kappa_abd, d.dt, d.body_fixed, nullptr,
+                            sys->tvals.get() + 9 * off, sys->trows.get() + off, sys->tcols.get() + off);
+            off += 10L * nb;
+        }
+        double* gf = grad ? grad + 3L * off0 : nullptr;
+        launch_tet_snh(s, d.n_tets, d.x, d.tets, d.dm_inv, d.vol, d.mu, d.lam, d.dt, off0, gf,
                        sys->tvals.get() + 9 * off, sys->trows.get() + off, sys->tcols.get() + off);
         off += 10L * d.n_tets;
         launch_quad_bending(s, d.n_hinges, d.x, d.rest_x, d.hinge_edges, d.hinge_adj, sys->hinge_q.get(), d.k_bend,
-                            d.dt, 0, grad, sys->tvals.get() + 9 * off, sys->trows.get() + off, sys->tcols.get() + off);
+                            d.dt, off0, gf, sys->tvals.get() + 9 * off, sys->trows.get() + off, sys->tcols.get() + off);
         off += 10L * d.n_hinges;
-        launch_tri_bw(s, d.n_tris, d.x, d.tris, d.dm_inv2, d.area, d.k_stretch, d.k_shear, d.strain_rate, d.dt, 0, grad,
-                      sys->tvals.get() + 9 * off, sys->trows.get() + off, sys->tcols.get() + off);
+        launch_tri_bw(s, d.n_tris, d.x, d.tris, d.dm_inv2, d.area, d.k_stretch, d.k_shear, d.strain_rate, d.dt, off0,
+                      gf, sys->tvals.get() + 9 * off, sys->trows.get() + off, sys->tcols.get() + off);
         off += 6L * d.n_tris;
-        launch_mass_diag(s, d.n_verts, d.mass, 0, sys->tvals.get() + 9 * off, sys->trows.get() + off,
+        launch_mass_diag(s, d.n_verts, d.mass, off0, sys->tvals.get() + 9 * off, sys->trows.get() + off,
                          sys->tcols.get() + off);
         off += d.n_verts;
         SGIPC_HIP(hipEventRecord(sys->ev[1], s));
         // ---- triplets -> unique upper BCOO
-        sys->nnz = sys->conv.convert(s, sys->tvals.get(), sys->trows.get(), sys->tcols.get(), off, d.n_verts,
+        sys->nnz = sys->conv.convert(s, sys->tvals.get(), sys->trows.get(), sys->tcols.get(), off, sys->n_sys,
                                      sys->bvals.get(), sys->brows.get(), sys->bcols.get(), sys->capacity);
         SGIPC_HIP(hipEventRecord(sys->ev[2], s));
-        // ---- MAS rebuild with the current contacts
+        // ---- preconditioners: MAS on the FEM rows (current contacts), block-Jacobi on the bodies
         const int mprec = (o.precision == SGIPC_FP64) ? SGIPC_FP64 : SGIPC_FP32_MAS;
-        sys->mas->m->set_bcoo(sys->bvals.get(), sys->brows.get(), sys->bcols.get(), sys->nnz, 0, d.pairs, d.n_pairs,
-                              mprec);
+        sys->mas->m->set_bcoo(sys->bvals.get(), sys->brows.get(), sys->bcols.get(), sys->nnz, off0, d.pairs,
+                              d.n_pairs, mprec);
+        if (nb > 0) sys->abdj.build(s, sys->bvals.get(), sys->brows.get(), sys->bcols.get(), sys->nnz, nb);
         SGIPC_HIP(hipEventRecord(sys->ev[3], s));
         // ---- PCG
         sgipc_pcg_stats pst{};
+        SubsystemPrec prec;
+        prec.mas = sys->mas->m.get();
+        prec.mas_row_offset = off0;
+        prec.abd = nb > 0 ? &sys->abdj : nullptr;
         status = pcg_solve_impl(sys->ctx->c, *sys->ws, sys->bvals.get(), sys->brows.get(), sys->bcols.get(), sys->nnz,
-                                d.n_verts, sys->mas->m.get(), b, x, o, &pst);
+                                sys->n_sys, prec, b, x, o, &pst);
         SGIPC_HIP(hipEventRecord(sys->ev[4], s));
         SGIPC_HIP(hipEventSynchronize(sys->ev[4]));
         auto t1 = std::chrono::steady_clock::now();

@@ -236,6 +236,18 @@ def upload_scene(ctx: Context, scene):
     desc.pairs = put("pairs", scene.pairs.reshape(-1) if scene.n_pairs else None, np.int32)
     desc.pair_x = put("pair_x", scene.pair_x.reshape(-1) if scene.pair_x is not None and scene.n_pairs else None,
                       np.float64)
+    nb = getattr(scene, "n_bodies", 0)
+    desc.n_bodies = nb
+    if nb:
+        desc.n_abd_points = scene.n_abd_points
+        desc.kappa_abd = scene.kappa_abd
+        desc.q = put("q", scene.q.reshape(-1), np.float64)
+        desc.q_tilde = put("q_tilde", scene.q_tilde.reshape(-1), np.float64)
+        desc.mass10 = put("mass10", scene.mass10.reshape(-1), np.float64)
+        desc.body_volume = put("body_volume", scene.body_volume, np.float64)
+        desc.body_fixed = put("body_fixed", scene.body_fixed, np.int32)
+        desc.point_body = put("point_body", scene.point_body, np.int32)
+        desc.x_bar = put("x_bar", scene.x_bar.reshape(-1), np.float64)
     desc.nbr_start = put("nbr_start", scene.nbr_start, np.uint32)
     desc.nbr_num = put("nbr_num", scene.nbr_num, np.uint32)
     desc.nbr_list = put("nbr_list", scene.nbr_list if scene.nbr_list.size else np.zeros(1, np.uint32), np.uint32)
@@ -255,8 +267,9 @@ class GlobalLinearSystem:
         check(lib().sgipc_system_create(ctx.h, ctypes.byref(h), ctypes.byref(self.desc), bank_size), ctx.h)
         self.h = h
         self.b = ctx.to_device(scene.rhs, np.float64)
-        self.x = ctx.empty(3 * scene.n, np.float64)
-        self.grad = ctx.empty(3 * scene.n, np.float64)
+        n_sys = getattr(scene, "n_sys", scene.n)
+        self.x = ctx.empty(3 * n_sys, np.float64)
+        self.grad = ctx.empty(3 * n_sys, np.float64)
 
     def solve_linear_system(self, tol=1e-5, precision="fp64", stop_mode=STOP_TRUE_RESIDUAL, want_grad=False,
                             **kw):

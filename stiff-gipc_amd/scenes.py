@@ -33,6 +33,10 @@ import numpy as np
 
 __all__ = [
     "TetScene",
+    "HybridScene",
+    "hybrid_scene",
+    "scene_c2",
+    "scene_hybrid_tiny",
     "ClothScene",
     "cloth_grid_scene",
     "scene_cloth_tiny",
@@ -462,3 +466,113 @@ def scene_cloth_tiny(N: int = 10, n_pairs: int = 16) -> ClothScene:
 def scene_c4(n_pairs: int = 200_000) -> ClothScene:
     """1024 x 1024 sheet, 1 mm spacing, E = 1e6 Pa, k_bend = 1e-4, 200k pairs."""
     return cloth_grid_scene("c4_cloth1024", 1024, n_pairs=n_pairs)
+
+
+@dataclasses.dataclass
+class HybridScene(TetScene):
+    """A tet mesh plus affine bodies (ABD), numbered as the reference numbers
+    them (``GIPC.cu:10329-10330,10482-10483``): system rows are
+    [4 per body | FEM vertices], and global point ids are [body points | FEM
+    vertices].  Contact pairs use global point ids; ``tets``, ``nbr_*`` and
+    ``mass`` use FEM-local ids.  Each body carries q = [p, a1, a2, a3], the
+    previous-step prediction q_tilde, its dyadic mass (m, m xb, m xb xb^T) over
+    its mass points, its volume and a fixed flag
+    (``abd_system/abd_jacobi_matrix.h:162-199``)."""
+
+    n_bodies: int = 0
+    q: np.ndarray | None = None            # (nb, 12)
+    q_tilde: np.ndarray | None = None      # (nb, 12)
+    mass10: np.ndarray | None = None       # (nb, 10): m, m xb (3), m xb xb^T (xx, xy, xz, yy, yz, zz)
+    body_volume: np.ndarray | None = None  # (nb,)
+    kappa_abd: float = 1e8
+    body_fixed: np.ndarray | None = None   # (nb,) int32
+    n_abd_points: int = 0
+    point_body: np.ndarray | None = None   # (n_abd_points,) int32
+    x_bar: np.ndarray | None = None        # (n_abd_points, 3) rest coordinates in the body frame
+
+    @property
+    def n_sys(self) -> int:
+        return 4 * self.n_bodies + self.n
+
+    @property
+    def n_triplets(self) -> int:
+        """Upper bound: 16 vertex blocks per pair, each up to 16 body blocks."""
+        return 256 * self.n_pairs + 10 * self.n_bodies + 10 * self.n_tets + self.n
+
+
+def hybrid_scene(name: str, N: int, n_bodies: int, n_pairs: int, *, E=1e5, body_side_cells=2.0,
+                 kappa_abd=1e8, density=1e3, n_fixed=2, spacing=0.01, dhat=1e-3, kappa=1e6, dt=0.01,
+                 seed_bodies=4, seed_pairs=3, seed_rhs=1) -> HybridScene:
+    """N^3 Neo-Hookean grid (Kuhn split, Morton order) with n_bodies affine
+    cubes resting above its top face; n_pairs seeded point-triangle pairs
+    between body corner points and the grid's top-face triangles, each with
+    its own local geometry at d^2 ~ U(0, dhat^2)."""
+    base = tet_grid_scene(name, N, E=E, spacing=spacing, n_pairs=0, dt=dt)
+    rng = np.random.default_rng(seed_bodies)
+    side = body_side_cells * spacing
+    corners = np.array([[sx, sy, sz] for sx in (-0.5, 0.5) for sy in (-0.5, 0.5) for sz in (-0.5, 0.5)]) * side
+    nb = n_bodies
+    top = base.x[:, 2].max()
+    grid_len = (N - 1) * spacing
+    cols_ = int(np.ceil(np.sqrt(nb)))
+    q = np.zeros((nb, 12))
+    for b in range(nb):
+        cx = (0.5 + b % cols_) / cols_ * grid_len
+        cy = (0.5 + b // cols_) / cols_ * grid_len
+        q[b, 0:3] = [cx, cy, top + 0.5 * side + 1e-4]
+        A = np.eye(3) + 0.01 * rng.standard_normal((3, 3))
+        q[b, 3:12] = A.reshape(-1)           # rows a1, a2, a3
+    q_tilde = q + 1e-4 * rng.standard_normal(q.shape)
+    m = density * side ** 3
+    mass10 = np.zeros((nb, 10))
+    mass10[:, 0] = m
+    d = m * (0.5 * side) ** 2             # sum over 8 corners of (m/8) xb xb^T = m (side/2)^2 I
+    mass10[:, 4] = d
+    mass10[:, 7] = d
+    mass10[:, 9] = d
+    body_volume = np.full(nb, side ** 3)
+    body_fixed = np.zeros(nb, np.int32)
+    body_fixed[:n_fixed] = 1
+    n_abd_points = 8 * nb
+    point_body = np.repeat(np.arange(nb), 8).astype(np.int32)
+    x_bar = np.tile(corners, (nb, 1))
+    # body point positions x = p + A xb
+    A = q[:, 3:12].reshape(nb, 3, 3)
+    xpts = q[point_body, 0:3] + np.einsum("bij,bj->bi", A[point_body], x_bar)
+    # pairs: body point vs FEM top-face triangle (global ids)
+    tris = _surface_triangles(base.tets)
+    ztop = base.rest[:, 2].max()
+    tris = tris[np.all(base.rest[tris][:, :, 2] >= ztop - 1e-12, axis=1)]
+    rp = np.random.default_rng(seed_pairs)
+    pv = rp.integers(0, n_abd_points, size=n_pairs)
+    ti = rp.integers(0, tris.shape[0], size=n_pairs)
+    t = tris[ti]
+    d2 = np.maximum(rp.uniform(0.0, 1.0, size=n_pairs) * dhat * dhat, 1e-12 * dhat * dhat)
+    bary = rp.uniform(0.1, 1.0, size=(n_pairs, 3))
+    bary /= bary.sum(axis=1, keepdims=True)
+    t0, t1, t2 = base.x[t[:, 0]], base.x[t[:, 1]], base.x[t[:, 2]]
+    nrm = np.cross(t1 - t0, t2 - t0)
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    foot = bary[:, 0:1] * t0 + bary[:, 1:2] * t1 + bary[:, 2:3] * t2
+    ppos = foot + np.sqrt(d2)[:, None] * nrm
+    pairs = np.stack([-pv - 1, t[:, 0] + n_abd_points, t[:, 1] + n_abd_points, t[:, 2] + n_abd_points],
+                     axis=1).astype(np.int32)
+    pair_x = np.stack([ppos, t0, t1, t2], axis=1)
+    n_sys = 4 * nb + base.n
+    rhs = np.random.default_rng(seed_rhs).standard_normal(3 * n_sys)
+    fields = {f.name: getattr(base, f.name) for f in dataclasses.fields(TetScene)}
+    fields.update(name=name, pairs=pairs, pair_x=pair_x, dhat=dhat, kappa=kappa, rhs=rhs,
+                  seeds=dict(base.seeds, bodies=seed_bodies, pairs=seed_pairs))
+    return HybridScene(**fields, n_bodies=nb, q=q, q_tilde=q_tilde, mass10=mass10, body_volume=body_volume,
+                       kappa_abd=kappa_abd, body_fixed=body_fixed, n_abd_points=n_abd_points,
+                       point_body=point_body, x_bar=x_bar)
+
+
+def scene_hybrid_tiny(N: int = 8, n_bodies: int = 6, n_pairs: int = 40) -> HybridScene:
+    """Small hybrid scene for dense checks."""
+    return hybrid_scene(f"hybrid_tiny{N}", N, n_bodies, n_pairs)
+
+
+def scene_c2(n_pairs: int = 30_000) -> HybridScene:
+    """48^3 Neo-Hookean grid (E = 1e5) + 200 affine bodies (kappa 1e8), 30k pairs."""
+    return hybrid_scene("c2_hybrid48", 48, 200, n_pairs)
