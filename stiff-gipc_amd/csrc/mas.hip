@@ -17,9 +17,11 @@
 //
 // MI355X mapping:
 //  * level-0 domain matrices are never materialised: the inverse kernel pulls
-//    each domain's blocks straight from the sorted BCOO into register tiles,
-//    reduces the domain's contribution to the coarse levels in-block (one
-//    3x3 per level in the common single-component case) and inverts in FP64;
+//    each domain's blocks straight from the sorted BCOO into register tiles
+//    and inverts in FP64; the coarse operators come from recursive
+//    coarsening: every upper block is mapped one level up (H, H^T, or
+//    H + H^T on a merged diagonal), merged by the deterministic triplet
+//    converter, and the domain blocks are extracted from the result;
 //  * the inverse keeps a 6x6 tile of the 3Bx3B matrix per thread (256
 //    threads for B=32) and broadcasts the pivot row/column through LDS, one
 //    barrier per pivot (double-buffered);
