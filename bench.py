@@ -131,20 +131,25 @@ def main():
     x = ctx.to_device(np.random.default_rng(9).standard_normal(3 * n))
     avg = ctypes.c_double()
     stored = ctypes.c_int()
-    pkg._lib.check(L.sgipc_spmv_time(ctx.h, pkg.FP64, v, rr, cc, nnz, n, args.spmv_reps, ctypes.byref(avg),
+    # cold cache: the matrix comes from HBM, as inside PCG (the MAS apply evicts it)
+    pkg._lib.check(L.sgipc_spmv_time(ctx.h, pkg.FP64, v, rr, cc, nnz, n, args.spmv_reps, 1, ctypes.byref(avg),
                                      ctypes.byref(stored)), ctx.h)
     spmv_ms = avg.value
     stored = stored.value
+    pkg._lib.check(L.sgipc_spmv_time(ctx.h, pkg.FP64, v, rr, cc, nnz, n, args.spmv_reps, 0, ctypes.byref(avg),
+                                     ctypes.byref(ctypes.c_int())), ctx.h)
+    spmv_warm_ms = avg.value
     b64 = spmv_bytes(nnz, n)
     gbs64 = b64 / (spmv_ms * 1e-3) / 1e9
     moved64 = layout_bytes(stored, n) / (spmv_ms * 1e-3) / 1e9
-    pkg._lib.check(L.sgipc_spmv_time(ctx.h, pkg.FP32_IR, v, rr, cc, nnz, n, args.spmv_reps, ctypes.byref(avg),
+    pkg._lib.check(L.sgipc_spmv_time(ctx.h, pkg.FP32_IR, v, rr, cc, nnz, n, args.spmv_reps, 1, ctypes.byref(avg),
                                      ctypes.byref(ctypes.c_int())), ctx.h)
     spmv32_ms = avg.value
     gbs32 = spmv_bytes(nnz, n, fp32=True) / (spmv32_ms * 1e-3) / 1e9
-    log(f"SpMV fp64: {spmv_ms * 1e3:.1f} us/launch, {nnz} upper blocks, {b64 / 1e6:.1f} MB algorithmic -> "
-        f"{gbs64:.0f} GB/s ({gbs64 / HBM_PEAK_GBS:.1%} of 8 TB/s); layout stores {stored} blocks, streams "
-        f"{moved64:.0f} GB/s; fp32: {spmv32_ms * 1e3:.1f} us, {gbs32:.0f} GB/s algorithmic")
+    log(f"SpMV fp64 (cold cache): {spmv_ms * 1e3:.1f} us/launch, {nnz} upper blocks, {b64 / 1e6:.1f} MB algorithmic "
+        f"-> {gbs64:.0f} GB/s ({gbs64 / HBM_PEAK_GBS:.1%} of 8 TB/s); layout stores {stored} blocks, streams "
+        f"{moved64:.0f} GB/s; back-to-back (matrix partly in the 256 MB Infinity Cache): {spmv_warm_ms * 1e3:.1f} us; "
+        f"fp32 cold: {spmv32_ms * 1e3:.1f} us, {gbs32:.0f} GB/s algorithmic")
 
     # MAS apply timing
     mas_ms = None
@@ -156,7 +161,8 @@ def main():
         log(f"MAS apply ({args.precision} build): {mas_ms * 1e3:.1f} us")
 
     if args.only_spmv:
-        print(json.dumps({"spmv_us": spmv_ms * 1e3, "spmv_GBs": gbs64, "spmv32_us": spmv32_ms * 1e3,
+        print(json.dumps({"spmv_us": spmv_ms * 1e3, "spmv_GBs": gbs64, "spmv_warm_us": spmv_warm_ms * 1e3,
+                          "spmv32_us": spmv32_ms * 1e3,
                           "spmv32_GBs": gbs32, "nnz": nnz, "stored_blocks": stored, "layout_GBs": moved64}),
               flush=True)
         ls.close()
@@ -214,6 +220,9 @@ def main():
             "stored_blocks": stored,
             "layout_streamed_GBs": round(moved64, 1),
             "avg_launch_us": round(spmv_ms * 1e3, 3),
+            "timing": "HIP events around each launch; a 512 MiB buffer is streamed before every launch so the "
+                      "matrix comes from HBM, as inside PCG",
+            "back_to_back_us": round(spmv_warm_ms * 1e3, 3),
         },
         "cpu_baseline": cpu,
     }

@@ -198,12 +198,16 @@ int sgipc_spmat_apply_f32(sgipc_spmat* m, float a, const float* x, float b, floa
 
 /* Measurement helper: lays the upper BCOO out as the PCG solver does (row
  * tiles, in-tile blocks once, cross-tile blocks in both rows) and returns the
- * average duration (ms) of the PCG SpMV kernel (emits the p.Ap partials) over
- * `reps` back-to-back launches, timed with HIP events on the context's stream.
- * precision: SGIPC_FP64 or SGIPC_FP32_IR (FP32 values).  *stored_blocks
- * (optional) receives the number of blocks the layout stores. */
+ * average duration (ms) of the PCG SpMV kernel (emits the p.Ap partials)
+ * over `reps` launches, each timed with its own HIP event pair on the
+ * context's stream.  cold_cache != 0 streams a 512 MiB buffer before every
+ * launch, so the matrix comes from HBM as it does inside PCG, where the MAS
+ * apply evicts it; 0 times back-to-back launches.  precision: SGIPC_FP64 or
+ * SGIPC_FP32_IR (FP32 values).  *stored_blocks (optional) receives the number
+ * of blocks the layout stores. */
 int sgipc_spmv_time(sgipc_ctx* ctx, int precision, const double* vals, const int* rows,
-                    const int* cols, int nnz, int n_rows, int reps, double* avg_ms, int* stored_blocks);
+                    const int* cols, int nnz, int n_rows, int reps, int cold_cache, double* avg_ms,
+                    int* stored_blocks);
 
 /* ---- MAS preconditioner (MASPreconditioner, StiffGIPC/MASPreconditioner.cuh:59-107) */
 /* bank_size is the domain size B (16 = the reference's BANKSIZE, or 32). */
@@ -248,7 +252,8 @@ typedef struct sgipc_pcg_opts {
     int precision;     /* enum sgipc_precision                                         */
     int check_every;   /* iterations between host looks at the stop flag (<= 0: 8)   */
     int ir_max_outer;  /* FP32_IR: refinement steps (<= 0: 20)                         */
-    double ir_inner_tol; /* FP32_IR: inner relative tolerance (<= 0: 1e-3)            */
+    double ir_inner_tol; /* FP32_IR: floor of the inner tolerance (<= 0: none; each pass
+                            asks for the residual reduction still missing)           */
 } sgipc_pcg_opts;
 
 typedef struct sgipc_pcg_stats {

@@ -83,6 +83,8 @@ class PcgSolver {
 
 int vec_grid(long n);
 void launch_axpy_f64(hipStream_t s, double a, const double* x, double* y, long n);
+// stream n doubles through the caches (read only; sink is never written in practice)
+void launch_stream_read(hipStream_t s, const double* buf, long n, double* sink);
 template <typename TA, typename TB>
 void launch_convert(hipStream_t s, const TA* a, TB* b, long n);
 template <typename T>

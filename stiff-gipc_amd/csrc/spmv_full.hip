@@ -461,6 +461,24 @@ int FullMatrix::apply(hipStream_t s, TV a, const TV* x, TV b, TV* y, double* dot
     SGIPC_LAUNCH_CHECK();
     return dot_part ? g : 0;
 }
+int FullMatrix::apply_f32_values(hipStream_t s, double a, const double* x, double b, double* y, double* dot_part,
+                                 const int* done) const
+{
+    SGIPC_CHECK(n_tiles > 0, SGIPC_EMPTY, "matrix was not built");
+    SGIPC_CHECK(has_f32, SGIPC_BAD_ARG, "FP32 copy of the matrix was not built");
+    const int g = n_tiles;
+    if (dot_part)
+        k_spmv_tsym<float, double, true><<<g, kThreads, 0, s>>>(sell_vals32.get(), sell_cols.get(), slice_ptr.get(),
+                                                               slice_w.get(), perm.get(), plen.get(), tinfo.get(), a,
+                                                               x, b, y, dot_part, done);
+    else
+        k_spmv_tsym<float, double, false><<<g, kThreads, 0, s>>>(sell_vals32.get(), sell_cols.get(),
+                                                                slice_ptr.get(), slice_w.get(), perm.get(),
+                                                                plen.get(), tinfo.get(), a, x, b, y, nullptr, done);
+    SGIPC_LAUNCH_CHECK();
+    return dot_part ? g : 0;
+}
+
 template int FullMatrix::apply<double>(hipStream_t, double, const double*, double, double*, double*, const int*) const;
 template int FullMatrix::apply<float>(hipStream_t, float, const float*, float, float*, double*, const int*) const;
 

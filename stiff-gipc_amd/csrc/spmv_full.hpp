@@ -17,6 +17,10 @@ class FullMatrix {
     // when *done != 0.
     template <typename TV>
     int apply(hipStream_t s, TV a, const TV* x, TV b, TV* y, double* dot_part, const int* done) const;
+    // FP64 vectors against the FP32 copy of the values (half the matrix
+    // bytes), FP64 accumulation; needs want_f32 at build.
+    int apply_f32_values(hipStream_t s, double a, const double* x, double b, double* y, double* dot_part,
+                         const int* done) const;
     // partial count of apply() with dot_part
     int partials() const;
     // blocks stored in the layout (upper blocks + cross-tile transposed

@@ -154,7 +154,7 @@ class MASPreconditioner:
 
 
 def pcg_opts(tol=1e-5, stop_mode=STOP_TRUE_RESIDUAL, max_iter=0, precision=FP64, check_every=8, ir_max_outer=20,
-             ir_inner_tol=1e-3):
+             ir_inner_tol=0.0):
     o = PcgOpts()
     lib().sgipc_pcg_default_opts(ctypes.byref(o))
     o.tol = tol

@@ -4,7 +4,9 @@ hinges) through the C ABI against the CPU oracle.
 Tolerances: element blocks max|dH| <= 1e-10 max|H| per element (the device
 uses the closed-form shear eigensystem, the oracle a Jacobi eigen-solve);
 rows/cols exact; gradients <= 1e-10 relative; bending Q <= 1e-13 relative;
-MAS hierarchy exact and FP64 apply <= 1e-9; system BCOO <= 1e-10; the
+MAS hierarchy exact; each inverse block's residual |M^-1 M - I| against
+the set-based Galerkin restatement within 10x the oracle's own; apply == restriction / the device's inverse blocks /
+prolongation to 1e-12; system BCOO <= 1e-10; the
 solution's residual under the oracle's operator <= 1e-5; on a heavier sheet,
 residual norms after a fixed iteration count within 1e-6 and iterates within
 1e-6 (the default sheets amplify round-off differences along the PCG
@@ -14,6 +16,7 @@ import ctypes
 
 import numpy as np
 import pytest
+from test_oracle_linalg import blocks_to_dense, mas_reference
 
 pytestmark = pytest.mark.gpu
 vp = ctypes.c_void_p
@@ -145,11 +148,37 @@ def test_cloth_mas_matches_oracle(pkg, orc, gpu_ctx, cloth, cloth_big, which):
         ct, _ = m.tables()
         oct_, _ = om.tables()
         np.testing.assert_array_equal(ct, oct_)
+        # per-domain inverses judged by their residual against the set-based
+        # Galerkin restatement: the coarse sheet domains are ill-conditioned
+        # (membrane stiffness over mass in the near-rigid modes), so two
+        # Gauss-Jordan runs in different operation orders differ far more
+        # than eps; the device's inverse must be as accurate as the oracle's
+        inv = m.inverse_blocks()
+        oinv = om.matrices(inverse=True)
+        _, _, _, dom, _ = mas_reference(s, vals, rows, cols, B)
+        assert inv.shape[0] == len(dom)
+        for d in range(inv.shape[0]):
+            Md = dom[d].copy()
+            pad = np.diag(Md) == 0
+            Md[pad, pad] = 1.0
+            e_gpu = np.abs(blocks_to_dense(inv[d], B) @ Md - np.eye(3 * B)).max()
+            e_orc = np.abs(blocks_to_dense(oinv[d], B) @ Md - np.eye(3 * B)).max()
+            assert e_gpu <= 10 * e_orc + 1e-12, (d, e_gpu, e_orc)
+        # the apply itself: restriction, the device's own inverse blocks, prolongation
         r = np.random.default_rng(6).standard_normal(3 * s.n)
         dr_ = gpu_ctx.to_device(r)
         dz = gpu_ctx.empty(3 * s.n, np.float64)
         m.preconditioning(dr_, dz)
-        assert rel(dz.download(), om.apply(r)) <= 1e-9
+        slot = np.concatenate([np.arange(s.n)[:, None], ct], axis=1)
+        z = np.zeros(3 * s.n)
+        for lv in range(L):
+            R = np.zeros(3 * total)
+            np.add.at(R.reshape(-1, 3), slot[:, lv], r.reshape(-1, 3))
+            Z = np.zeros(3 * total)
+            for d in range(off[lv] // B, off[lv + 1] // B):
+                Z[3 * B * d:3 * B * (d + 1)] = blocks_to_dense(inv[d], B) @ R[3 * B * d:3 * B * (d + 1)]
+            z += Z.reshape(-1, 3)[slot[:, lv]].reshape(-1)
+        assert rel(dz.download(), z) <= 1e-12
     finally:
         m.close()
         om.close()
