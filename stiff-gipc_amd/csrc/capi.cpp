@@ -25,6 +25,13 @@ struct FullOp : SpmvOp<TV> {
             if (f32_values) return m->apply_f32_values(s, 1.0, p, 0.0, Ap, part, ctl ? &ctl->done : nullptr);
         return m->apply<TV>(s, TV(1), p, TV(0), Ap, part, ctl ? &ctl->done : nullptr);
     }
+    bool fuses_update_p() const override { return true; }
+    int apply_dot_update_p(hipStream_t s, const TV* z, const TV* p_old, TV* p_new, TV* Ap, double* part,
+                           const PcgCtl* ctl, const double* rz_new, const double* rz_old, int n_rz) const override
+    {
+        return m->apply_update_p<TV>(s, f32_values, z, p_old, p_new, Ap, part, ctl ? &ctl->done : nullptr, rz_new,
+                                     rz_old, n_rz);
+    }
 };
 
 // r = b - A x in FP64 with ||r||^2 partials; returns the partial count.
@@ -415,6 +422,7 @@ void sgipc_pcg_default_opts(sgipc_pcg_opts* o)
     o->max_iter = 0;
     o->precision = SGIPC_FP64;
     o->check_every = 8;
+    o->fuse = 1;
     o->ir_max_outer = 20;
     o->ir_inner_tol = 0.0;
 }
@@ -449,6 +457,7 @@ int pcg_solve_impl(Context& c, SolveWorkspace& ws, const double* vals, const int
         po.stop_mode = o.stop_mode;
         po.max_iter = max_iter;
         po.check_every = o.check_every;
+        po.fuse = o.fuse != 0;
         PcgResult r = ws.pcg64.solve(s, A, has_prec ? &M : nullptr, b, x, n3, po);
         st.iterations = r.iterations;
         st.outer_iterations = 1;
@@ -499,6 +508,7 @@ int pcg_solve_impl(Context& c, SolveWorkspace& ws, const double* vals, const int
             po.stop_mode = SGIPC_STOP_TRUE_RESIDUAL;
             po.max_iter = max_iter - total_it;
             po.check_every = o.check_every;
+            po.fuse = o.fuse != 0;
             PcgResult r = ws.pcg64.solve(s, A, has_prec ? &M : nullptr, ws.r64.get(), ws.d64.get(), n3, po);
             total_it += r.iterations;
             if (r.not_spd) {

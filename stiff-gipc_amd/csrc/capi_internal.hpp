@@ -86,7 +86,15 @@ struct SubsystemPrecond : Preconditioner<TV> {
             k += p.abd->template apply<TV>(s, r, z, rz_part ? rz_part + k : nullptr, sd.ctl);
         return k;
     }
+    // the fused x/r update covers the MAS rows only: offered when they are
+    // the whole system
+    int apply_update_xr(hipStream_t s, const FusedXR<TV>& fx, TV* z, double* rz_part, int* n_rr) override
+    {
+        if (!p.mas || p.mas_row_offset != 0 || (p.abd && p.abd->n_bodies > 0)) return -1;
+        return p.mas->template apply_update_xr<TV>(s, fx, z, rz_part, n_rr);
+    }
     int max_partials() const override { return 1024 + (p.abd ? p.abd->partials() : 0); }
+    int max_rr_partials() const override { return p.mas ? p.mas->rr_partials() : 0; }
 };
 
 int pcg_solve_impl(Context& c, SolveWorkspace& ws, const double* vals, const int* rows, const int* cols,

@@ -591,13 +591,114 @@ __global__ __launch_bounds__(256) void k_restrict0(PcgStopDesc sd, const int* __
     }
 }
 
+// x += alpha p, r -= alpha Ap (alpha = rz / pAp from partials), ||r||^2
+// partials, then the restriction of the new r as in k_restrict0.  The stop
+// rule is left to the first coarse restriction, which sees the new rr.
+template <int B, typename TV, typename TS>
+__global__ __launch_bounds__(256) void k_restrict0_xr(FusedXR<TV> fx, const int* __restrict__ meta, int n, int L,
+                                                      const int* __restrict__ map1, TS* __restrict__ R)
+{
+    __shared__ double smem[16];
+    __shared__ int s_done;
+    if (threadIdx.x == 0) s_done = fx.ctl->done;
+    __syncthreads();
+    if (s_done) return;
+    const double rz = block_sum_array(fx.rz, fx.n_rz, smem);
+    const double pap = block_sum_array(fx.pap, fx.n_pap, smem);
+    const double alpha = rz / pap;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int npad = meta[kMetaOff + 1];
+    const bool real = s < n;
+    TS v0 = 0, v1 = 0, v2 = 0;
+    double rrv = 0.0;
+    if (real) {
+        double rv[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const long k = 3 * (long)s + c;
+            rv[c] = (double)fx.r[k] - alpha * (double)fx.Ap[k];
+            fx.x[k] = (TV)((double)fx.x[k] + alpha * (double)fx.p[k]);
+            fx.r[k] = (TV)rv[c];
+            rrv += rv[c] * rv[c];
+        }
+        v0 = (TS)(TV)rv[0];
+        v1 = (TS)(TV)rv[1];
+        v2 = (TS)(TV)rv[2];
+    }
+    if (s < npad) {
+        R[3 * (long)s] = v0;
+        R[3 * (long)s + 1] = v1;
+        R[3 * (long)s + 2] = v2;
+    }
+    {
+        int node = real ? map1[s] : -1;
+        bool pending = node >= 0;
+        const int lane = threadIdx.x & 63;
+        while (true) {
+            const uint64_t act = __ballot(pending);
+            if (act == 0) break;
+            const int leader = __ffsll((unsigned long long)act) - 1;
+            const int key = __shfl(node, leader, 64);
+            const bool mine = pending && node == key;
+            TS s0 = wave_sum(mine ? v0 : TS(0));
+            TS s1 = wave_sum(mine ? v1 : TS(0));
+            TS s2 = wave_sum(mine ? v2 : TS(0));
+            if (lane == leader) {
+                const long o = 3 * (long)(npad + key);
+                R[o] = s0;
+                R[o + 1] = s1;
+                R[o + 2] = s2;
+            }
+            if (mine) pending = false;
+        }
+        const int c1 = meta[kMetaCnt + 1];
+        const int pad = meta[kMetaOff + 2] - npad - c1;
+        if (s < pad) {
+            const long o = 3 * (long)(npad + c1 + s);
+            R[o] = 0;
+            R[o + 1] = 0;
+            R[o + 2] = 0;
+        }
+    }
+    const double rr = block_sum(rrv, smem);
+    if (threadIdx.x == 0) {
+        fx.rr[blockIdx.x] = rr;
+        if (blockIdx.x == 0) {
+            fx.ctl->iters += 1;
+            fx.ctl->last_alpha = alpha;
+            if (!(pap > 0.0)) fx.ctl->not_spd = 1;
+        }
+    }
+}
+
 // Level l-1 -> level l sums (l >= 2).
 template <typename TS>
 __global__ __launch_bounds__(256) void k_restrict_up(const int* __restrict__ done, const int* __restrict__ meta,
                                                      int level, const int* __restrict__ going_next,
-                                                     TS* __restrict__ R)
+                                                     TS* __restrict__ R, PcgStopDesc sd)
 {
-    if (done != nullptr && *done) return;
+    if (sd.ctl != nullptr) {
+        // first coarse restriction after a fused x/r update: the stop rule
+        __shared__ double smem[16];
+        __shared__ int s_done;
+        if (threadIdx.x == 0) s_done = sd.ctl->done;
+        __syncthreads();
+        if (s_done) return;
+        const double rz = block_sum_array(sd.rz, sd.n_rz, smem);
+        const double rr = block_sum_array(sd.rr, sd.n_rr, smem);
+        const bool stop = (sd.ctl->stop_mode == 0) ? fabs(rz) <= sd.ctl->tol * sd.ctl->rz0
+                                                   : rr <= sd.ctl->tol * sd.ctl->tol * sd.ctl->bb;
+        if (stop) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                sd.ctl->done = 1;
+                sd.ctl->final_rr = rr;
+                sd.ctl->final_rz = rz;
+            }
+            return;
+        }
+    } else if (done != nullptr && *done) {
+        return;
+    }
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     const int cprev = meta[kMetaCnt + level - 1];
     const int offprev = meta[kMetaOff + level - 1];
@@ -966,7 +1067,8 @@ int Mas::apply_t(hipStream_t s, const TV* r, TV* z, double* rz_part, const PcgSt
         int c = h_cnt[l - 1];
         int pad = h_off[l + 1] - h_off[l] - h_cnt[l];
         int work = c > pad ? c : pad;
-        k_restrict_up<TS><<<grid_for(work > 0 ? work : 1, 256), 256, 0, s>>>(done, meta_.get(), l, going_next_.get(), R);
+        k_restrict_up<TS><<<grid_for(work > 0 ? work : 1, 256), 256, 0, s>>>(done, meta_.get(), l, going_next_.get(), R,
+                                                                           PcgStopDesc{});
         SGIPC_LAUNCH_CHECK();
     }
     k_local_solve<BB, TS><<<total / BB, 256, 0, s>>>(done, minv, R, Z);
@@ -976,6 +1078,60 @@ int Mas::apply_t(hipStream_t s, const TV* r, TV* z, double* rz_part, const PcgSt
     SGIPC_LAUNCH_CHECK();
     return g;
 }
+
+template <int BB, typename TV, typename TS>
+int Mas::apply_xr_t(hipStream_t s, const FusedXR<TV>& fx, TV* z, double* rz_part, int* n_rr)
+{
+    TS* R;
+    TS* Z;
+    const TS* minv;
+    if constexpr (sizeof(TS) == 8) {
+        R = R64_.get();
+        Z = Z64_.get();
+        minv = minv64_.get();
+    } else {
+        R = R32_.get();
+        Z = Z32_.get();
+        minv = minv32_.get();
+    }
+    const int* done = &fx.ctl->done;
+    const int npad = h_off[1];
+    const int g0 = grid_for(npad, 256);
+    if (g0 > fx.rr_cap) return -1;
+    k_restrict0_xr<BB, TV, TS><<<g0, 256, 0, s>>>(fx, meta_.get(), n, L, map_.get() + n, R);
+    SGIPC_LAUNCH_CHECK();
+    *n_rr = g0;
+    const PcgStopDesc sd{fx.ctl, fx.rz, fx.n_rz, fx.rr, g0};
+    for (int l = 2; l < L; ++l) {
+        int c = h_cnt[l - 1];
+        int pad = h_off[l + 1] - h_off[l] - h_cnt[l];
+        int work = c > pad ? c : pad;
+        k_restrict_up<TS><<<grid_for(work > 0 ? work : 1, 256), 256, 0, s>>>(done, meta_.get(), l, going_next_.get(), R,
+                                                                           l == 2 ? sd : PcgStopDesc{});
+        SGIPC_LAUNCH_CHECK();
+    }
+    k_local_solve<BB, TS><<<total / BB, 256, 0, s>>>(done, minv, R, Z);
+    SGIPC_LAUNCH_CHECK();
+    const int g = vec_grid(n);
+    k_prolong<TV, TS><<<g, 256, 0, s>>>(done, meta_.get(), n, L, map_.get(), Z, fx.r, z, rz_part);
+    SGIPC_LAUNCH_CHECK();
+    return g;
+}
+
+template <typename TV>
+int Mas::apply_update_xr(hipStream_t s, const FusedXR<TV>& fx, TV* z, double* rz_part, int* n_rr)
+{
+    SGIPC_CHECK(built, SGIPC_BAD_ARG, "MAS used before setPreconditioner");
+    if (L <= 2 || fx.ctl == nullptr || fx.n3 != 3L * n) return -1;
+    if (precision == SGIPC_FP64) {
+        if (B == 32) return apply_xr_t<32, TV, double>(s, fx, z, rz_part, n_rr);
+        return apply_xr_t<16, TV, double>(s, fx, z, rz_part, n_rr);
+    }
+    if (B == 32) return apply_xr_t<32, TV, float>(s, fx, z, rz_part, n_rr);
+    return apply_xr_t<16, TV, float>(s, fx, z, rz_part, n_rr);
+}
+template int Mas::apply_update_xr<double>(hipStream_t, const FusedXR<double>&, double*, double*, int*);
+template int Mas::apply_update_xr<float>(hipStream_t, const FusedXR<float>&, float*, double*, int*);
 
 template <typename TV>
 int Mas::apply(hipStream_t s, const TV* r, TV* z, double* rz_part, const PcgStopDesc& sd)

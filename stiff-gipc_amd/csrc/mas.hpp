@@ -23,6 +23,12 @@ class Mas {
     // z = M^-1 r, rz partials; see Preconditioner.
     template <typename TV>
     int apply(hipStream_t s, const TV* r, TV* z, double* rz_part, const PcgStopDesc& sd);
+    // x/r update fused into the restriction; needs L > 2 (the stop rule is
+    // evaluated by the first coarse restriction).  Returns -1 otherwise.
+    template <typename TV>
+    int apply_update_xr(hipStream_t s, const FusedXR<TV>& fx, TV* z, double* rz_part, int* n_rr);
+    // rr partials apply_update_xr writes
+    int rr_partials() const { return (h_off[1] + 255) / 256; }
     int levels(int* offsets, int* counts) const;
     void tables(int* coarse_table, int* going_next) const;
     void inverse_blocks(double* out) const;
@@ -44,6 +50,8 @@ class Mas {
     void assemble_invert_t(const double* vals, const int* rows, const int* cols, int nnz, int offset);
     template <int BB, typename TV, typename TS>
     int apply_t(hipStream_t s, const TV* r, TV* z, double* rz_part, const PcgStopDesc& sd);
+    template <int BB, typename TV, typename TS>
+    int apply_xr_t(hipStream_t s, const FusedXR<TV>& fx, TV* z, double* rz_part, int* n_rr);
 
     DevBuf<uint32_t> nbr_start_, nbr_num_, nbr_list_;
     DevBuf<int> map_;        // [L][n], row 0 unused (identity)
@@ -72,6 +80,11 @@ struct MasPrecond : Preconditioner<TV> {
     {
         return mas->template apply<TV>(s, r, z, rz_part, sd);
     }
+    int apply_update_xr(hipStream_t s, const FusedXR<TV>& fx, TV* z, double* rz_part, int* n_rr) override
+    {
+        return mas->template apply_update_xr<TV>(s, fx, z, rz_part, n_rr);
+    }
+    int max_rr_partials() const override { return mas->rr_partials(); }
 };
 
 int mas_num_levels(int n, int bank);

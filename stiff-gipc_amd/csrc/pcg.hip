@@ -282,6 +282,7 @@ void PcgSolver<TV>::reserve(long n3, int nnz, int max_partials)
     r_.reserve(n3);
     z_.reserve(n3);
     p_.reserve(n3);
+    p2_.reserve(n3);
     Ap_.reserve(n3);
     int g = vec_grid(n3);
     int gs = max_partials > 0 ? max_partials : spmv_partials_needed(nnz > 0 ? nnz : 1);
@@ -312,6 +313,7 @@ PcgResult PcgSolver<TV>::solve(hipStream_t s, const SpmvOp<TV>& A, Preconditione
     {
         const int pm = M ? M->max_partials() : 0;
         reserve(n3, A.nnz, A.max_partials > pm ? A.max_partials : pm);
+        if (M && M->max_rr_partials() > 0) part_rr_.reserve((size_t)M->max_rr_partials());
     }
     if (!ev_[0]) {
         SGIPC_HIP(hipEventCreateWithFlags(&ev_[0], hipEventDisableTiming));
@@ -320,7 +322,8 @@ PcgResult PcgSolver<TV>::solve(hipStream_t s, const SpmvOp<TV>& A, Preconditione
     TV* x = x_out;
     TV* r = r_.get();
     TV* z = z_.get();
-    TV* p = p_.get();
+    TV* pbuf[2] = {p_.get(), p2_.get()};
+    int pb = 0;  // pbuf[pb] is the current search direction
     TV* Ap = Ap_.get();
     PcgCtl* ctl = ctl_.get();
 
@@ -340,9 +343,17 @@ PcgResult PcgSolver<TV>::solve(hipStream_t s, const SpmvOp<TV>& A, Preconditione
     pcg_init_ctl<<<1, kVecThreads, 0, s>>>(ctl, part_rz_[cur].get(), n_rz, part_bb_.get(), n_bb,
                                            opt.tol, opt.stop_mode);
     SGIPC_LAUNCH_CHECK();
-    SGIPC_HIP(hipMemcpyAsync(p, z, sizeof(TV) * n3, hipMemcpyDeviceToDevice, s));
+    SGIPC_HIP(hipMemcpyAsync(pbuf[pb], z, sizeof(TV) * n3, hipMemcpyDeviceToDevice, s));
     launch_fill<TV>(s, Ap, TV(0), n3);
 
+    // Fused iteration (when both operators offer it):
+    //   SpMV:  p_k = z + beta p_{k-1} gathered on the fly, Ap = A p_k, p.Ap
+    //   MAS:   x += alpha p_k, r -= alpha Ap, ||r||^2 in the restriction;
+    //          stop rule in the first coarse restriction; z = M r, r.z
+    // i.e. the p update of iteration k-1 is deferred into the SpMV of k.
+    const bool fuse_p = opt.fuse && A.fuses_update_p();
+    bool fuse_xr = opt.fuse && M != nullptr;  // cleared when the preconditioner declines
+    bool p_pending = false;                   // p update of the previous iteration not yet applied
     const int gv = vec_grid(n3);
     const int chunk = opt.check_every > 0 ? opt.check_every : 8;
     int issued = 0;  // iterations enqueued
@@ -355,25 +366,59 @@ PcgResult PcgSolver<TV>::solve(hipStream_t s, const SpmvOp<TV>& A, Preconditione
         if (todo > chunk) todo = chunk;
         if (todo > 0) {
             for (int it = 0; it < todo; ++it) {
-                int nxt = cur ^ 1;
-                int n_pap = A.apply_dot(s, p, Ap, part_pap_.get(), ctl);
-                pcg_update_xr<TV><<<gv, kVecThreads, 0, s>>>(ctl, part_rz_[cur].get(), n_rz,
-                                                            part_pap_.get(), n_pap, x, r, p, Ap, n3,
-                                                            part_rr_.get());
-                SGIPC_LAUNCH_CHECK();
-                PcgStopDesc sd{ctl, part_rz_[cur].get(), n_rz, part_rr_.get(), gv};
-                int n_rzn;
-                if (M) {
-                    n_rzn = M->apply(s, r, z, part_rz_[nxt].get(), sd);
+                const int nxt = cur ^ 1;
+                int n_pap;
+                if (p_pending) {
+                    // beta = rz[cur] / rz[nxt]: rz[nxt] still holds the previous rz
+                    n_pap = A.apply_dot_update_p(s, z, pbuf[pb], pbuf[pb ^ 1], Ap, part_pap_.get(), ctl,
+                                                 part_rz_[cur].get(), part_rz_[nxt].get(), n_rz);
+                    pb ^= 1;
+                    p_pending = false;
                 } else {
-                    pcg_identity_apply<TV><<<gv, kVecThreads, 0, s>>>(sd, r, z, n3, part_rz_[nxt].get());
+                    n_pap = A.apply_dot(s, pbuf[pb], Ap, part_pap_.get(), ctl);
+                }
+                TV* p = pbuf[pb];
+                int n_rzn = -1;
+                if (fuse_xr) {
+                    FusedXR<TV> fx;
+                    fx.ctl = ctl;
+                    fx.rz = part_rz_[cur].get();
+                    fx.n_rz = n_rz;
+                    fx.pap = part_pap_.get();
+                    fx.n_pap = n_pap;
+                    fx.x = x;
+                    fx.r = r;
+                    fx.p = p;
+                    fx.Ap = Ap;
+                    fx.rr = part_rr_.get();
+                    fx.rr_cap = (int)part_rr_.n;
+                    fx.n3 = n3;
+                    int n_rr = 0;
+                    n_rzn = M->apply_update_xr(s, fx, z, part_rz_[nxt].get(), &n_rr);
+                    if (n_rzn < 0) fuse_xr = false;
+                }
+                if (n_rzn < 0) {
+                    pcg_update_xr<TV><<<gv, kVecThreads, 0, s>>>(ctl, part_rz_[cur].get(), n_rz,
+                                                                part_pap_.get(), n_pap, x, r, p, Ap, n3,
+                                                                part_rr_.get());
                     SGIPC_LAUNCH_CHECK();
-                    n_rzn = gv;
+                    PcgStopDesc sd{ctl, part_rz_[cur].get(), n_rz, part_rr_.get(), gv};
+                    if (M) {
+                        n_rzn = M->apply(s, r, z, part_rz_[nxt].get(), sd);
+                    } else {
+                        pcg_identity_apply<TV><<<gv, kVecThreads, 0, s>>>(sd, r, z, n3, part_rz_[nxt].get());
+                        SGIPC_LAUNCH_CHECK();
+                        n_rzn = gv;
+                    }
                 }
                 if (n_rzn != n_rz) throw Error(SGIPC_HIP_ERROR, "preconditioner partial count changed");
-                pcg_update_p<TV><<<gv, kVecThreads, 0, s>>>(ctl, part_rz_[nxt].get(), part_rz_[cur].get(),
-                                                           n_rz, p, z, Ap, n3);
-                SGIPC_LAUNCH_CHECK();
+                if (fuse_p) {
+                    p_pending = true;
+                } else {
+                    pcg_update_p<TV><<<gv, kVecThreads, 0, s>>>(ctl, part_rz_[nxt].get(), part_rz_[cur].get(),
+                                                               n_rz, p, z, Ap, n3);
+                    SGIPC_LAUNCH_CHECK();
+                }
                 cur = nxt;
             }
             issued += todo;

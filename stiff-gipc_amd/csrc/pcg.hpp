@@ -33,6 +33,28 @@ struct PcgOptions {
     int stop_mode = 1;     // 0 = reference |rz| <= tol*rz0, 1 = ||r|| <= tol*||b||
     int max_iter = 1000;
     int check_every = 8;
+    // fold the x/r update into the preconditioner's first kernel and the p
+    // update into the SpMV when both offer it
+    bool fuse = true;
+};
+
+// Inputs of the x/r update (x += alpha p, r -= alpha Ap, alpha = rz/pAp) that
+// a preconditioner may perform in its first kernel; rr receives ||r||^2
+// partials (rr_count of them).
+template <typename TV>
+struct FusedXR {
+    PcgCtl* ctl = nullptr;
+    const double* rz = nullptr;
+    int n_rz = 0;
+    const double* pap = nullptr;
+    int n_pap = 0;
+    TV* x = nullptr;
+    TV* r = nullptr;
+    const TV* p = nullptr;
+    const TV* Ap = nullptr;
+    double* rr = nullptr;
+    int rr_cap = 0;  // capacity of rr
+    long n3 = 0;     // length of x, r, p, Ap
 };
 
 struct PcgResult {
@@ -52,6 +74,15 @@ struct SpmvOp {
     int max_partials = 0;  // partials apply_dot may write
     virtual ~SpmvOp() = default;
     virtual int apply_dot(hipStream_t s, const TV* p, TV* Ap, double* pap_part, const PcgCtl* ctl) const = 0;
+    // p_new = z + beta p_old (beta = rz_new / rz_old from partials) fused
+    // into Ap = A p_new; returns the p.Ap partial count.  Called only when
+    // fuses_update_p() is true.
+    virtual bool fuses_update_p() const { return false; }
+    virtual int apply_dot_update_p(hipStream_t, const TV*, const TV*, TV*, TV*, double*, const PcgCtl*,
+                                   const double*, const double*, int) const
+    {
+        return -1;
+    }
 };
 
 // z = M r + per-workgroup partials of r.z; evaluates `stop` first when its ctl
@@ -61,8 +92,15 @@ template <typename TV>
 struct Preconditioner {
     virtual ~Preconditioner() = default;
     virtual int apply(hipStream_t s, const TV* r, TV* z, double* rz_part, const PcgStopDesc& stop) = 0;
+    // x/r update fused into the first kernel, z = M r with r.z partials; the
+    // stop rule (previous rz, the new rr) is evaluated by a later kernel of
+    // the apply.  Returns the rz partial count (and *n_rr), or -1 when the
+    // preconditioner has no fused form.
+    virtual int apply_update_xr(hipStream_t, const FusedXR<TV>&, TV*, double*, int*) { return -1; }
     // upper bound of the partial count apply() returns
     virtual int max_partials() const { return 1024; }
+    // upper bound of the rr partial count apply_update_xr() writes
+    virtual int max_rr_partials() const { return 0; }
 };
 
 template <typename TV>
@@ -74,7 +112,7 @@ class PcgSolver {
     void reserve(long n3, int nnz, int max_partials);
 
   private:
-    DevBuf<TV> x_, r_, z_, p_, Ap_;
+    DevBuf<TV> x_, r_, z_, p_, p2_, Ap_;
     DevBuf<double> part_rz_[2], part_pap_, part_rr_, part_bb_;
     DevBuf<PcgCtl> ctl_;
     PcgCtl* host_ctl_ = nullptr;

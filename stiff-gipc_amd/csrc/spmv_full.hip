@@ -41,19 +41,32 @@ constexpr int kThreads = kTile * kSplit;   // 1024
 constexpr int kTcap = 4096;                // in-tile transposed slots per tile (LDS)
 constexpr uint32_t kInTile = 0x80000000u;
 
-template <typename TM, typename TV, bool kDot>
+// With kFuseP the gathered vector is p_new = z + beta p_old, formed on the fly
+// (beta = rz_new / rz_old from partials, summed in a fixed order), and every
+// owned row of p_new is written: the PCG p update rides along with the SpMV.
+template <typename TM, typename TV, bool kDot, bool kFuseP>
 __global__ __launch_bounds__(kThreads) void k_spmv_tsym(const TM* __restrict__ vals, const uint32_t* __restrict__ cw,
                                                         const int* __restrict__ slice_ptr,
                                                         const int* __restrict__ slice_w, const int* __restrict__ perm,
                                                         const int* __restrict__ plen,
                                                         const uint32_t* __restrict__ tinfo, TV a,
                                                         const TV* __restrict__ x, TV b, TV* __restrict__ y,
-                                                        double* __restrict__ dot_part, const int* __restrict__ done)
+                                                        double* __restrict__ dot_part, const int* __restrict__ done,
+                                                        const TV* __restrict__ zf, TV* __restrict__ p_new,
+                                                        const double* __restrict__ rzn,
+                                                        const double* __restrict__ rzo, int n_rz)
 {
     __shared__ TV tl[kTcap * 3];
     __shared__ TV red[3 * kTile];
     __shared__ double dred[16];
     if (done != nullptr && *done) return;
+    double beta = 0.0;
+    if (kFuseP) beta = block_sum_array(rzn, n_rz, dred) / block_sum_array(rzo, n_rz, dred);
+    // gathered value: x itself, or p_new = z + beta p_old
+    auto gx = [&](long k) -> TV {
+        if (kFuseP) return (TV)((double)zf[k] + beta * (double)x[k]);
+        return x[k];
+    };
     const int tile = blockIdx.x;
     const int part = threadIdx.x / kTile;
     const int r = threadIdx.x % kTile;
@@ -65,9 +78,14 @@ __global__ __launch_bounds__(kThreads) void k_spmv_tsym(const TM* __restrict__ v
     const long base = (long)tile * kTile;
     TV xi0 = 0, xi1 = 0, xi2 = 0;
     if (i >= 0) {
-        xi0 = x[3 * (long)i];
-        xi1 = x[3 * (long)i + 1];
-        xi2 = x[3 * (long)i + 2];
+        xi0 = gx(3 * (long)i);
+        xi1 = gx(3 * (long)i + 1);
+        xi2 = gx(3 * (long)i + 2);
+        if (kFuseP && part == 0) {
+            p_new[3 * (long)i] = xi0;
+            p_new[3 * (long)i + 1] = xi1;
+            p_new[3 * (long)i + 2] = xi2;
+        }
     }
     const long S = slice_ptr[q];
     const int W = slice_w[q];
@@ -90,8 +108,8 @@ __global__ __launch_bounds__(kThreads) void k_spmv_tsym(const TM* __restrict__ v
         const long j0 = (w0 & kInTile) ? base + ((w0 >> 16) & 0x7FFF) : (long)w0;
         const long j1 = (w1 & kInTile) ? base + ((w1 >> 16) & 0x7FFF) : (long)w1;
         const TV m0 = act0 ? TV(1) : TV(0), m1 = act1 ? TV(1) : TV(0);
-        const TV x0 = m0 * x[3 * j0], x1 = m0 * x[3 * j0 + 1], x2 = m0 * x[3 * j0 + 2];
-        const TV z0 = m1 * x[3 * j1], z1 = m1 * x[3 * j1 + 1], z2 = m1 * x[3 * j1 + 2];
+        const TV x0 = m0 * gx(3 * j0), x1 = m0 * gx(3 * j0 + 1), x2 = m0 * gx(3 * j0 + 2);
+        const TV z0 = m1 * gx(3 * j1), z1 = m1 * gx(3 * j1 + 1), z2 = m1 * gx(3 * j1 + 2);
         a0 += (TV)h[0] * x0 + (TV)h[3] * x1 + (TV)h[6] * x2;
         a1 += (TV)h[1] * x0 + (TV)h[4] * x1 + (TV)h[7] * x2;
         a2 += (TV)h[2] * x0 + (TV)h[5] * x1 + (TV)h[8] * x2;
@@ -121,7 +139,7 @@ __global__ __launch_bounds__(kThreads) void k_spmv_tsym(const TM* __restrict__ v
         const bool in0 = act0 && (w0 & kInTile);
         const long j0 = (w0 & kInTile) ? base + ((w0 >> 16) & 0x7FFF) : (long)w0;
         const TV m0 = act0 ? TV(1) : TV(0);
-        const TV x0 = m0 * x[3 * j0], x1 = m0 * x[3 * j0 + 1], x2 = m0 * x[3 * j0 + 2];
+        const TV x0 = m0 * gx(3 * j0), x1 = m0 * gx(3 * j0 + 1), x2 = m0 * gx(3 * j0 + 2);
         a0 += (TV)h[0] * x0 + (TV)h[3] * x1 + (TV)h[6] * x2;
         a1 += (TV)h[1] * x0 + (TV)h[4] * x1 + (TV)h[7] * x2;
         a2 += (TV)h[2] * x0 + (TV)h[5] * x1 + (TV)h[8] * x2;
@@ -433,53 +451,75 @@ long FullMatrix::stored_blocks(hipStream_t s) const
     return t;
 }
 
-template <typename TV>
-int FullMatrix::apply(hipStream_t s, TV a, const TV* x, TV b, TV* y, double* dot_part, const int* done) const
+template <typename TM, typename TV>
+int FullMatrix::launch(hipStream_t s, const TM* vals, TV a, const TV* x, TV b, TV* y, double* dot_part,
+                       const int* done, const TV* z, TV* p_new, const double* rzn, const double* rzo,
+                       int n_rz) const
 {
     SGIPC_CHECK(n_tiles > 0, SGIPC_EMPTY, "matrix was not built");
     const int g = n_tiles;
-    if constexpr (sizeof(TV) == 8) {
-        if (dot_part)
-            k_spmv_tsym<double, double, true><<<g, kThreads, 0, s>>>(
-                sell_vals.get(), sell_cols.get(), slice_ptr.get(), slice_w.get(), perm.get(), plen.get(), tinfo.get(),
-                a, x, b, y, dot_part, done);
-        else
-            k_spmv_tsym<double, double, false><<<g, kThreads, 0, s>>>(
-                sell_vals.get(), sell_cols.get(), slice_ptr.get(), slice_w.get(), perm.get(), plen.get(), tinfo.get(),
-                a, x, b, y, nullptr, done);
+#define SGIPC_TSYM(DOT, FUSE)                                                                                   \
+    k_spmv_tsym<TM, TV, DOT, FUSE><<<g, kThreads, 0, s>>>(vals, sell_cols.get(), slice_ptr.get(), slice_w.get(), \
+                                                          perm.get(), plen.get(), tinfo.get(), a, x, b, y,        \
+                                                          dot_part, done, z, p_new, rzn, rzo, n_rz)
+    if (p_new) {
+        if (dot_part) SGIPC_TSYM(true, true);
+        else SGIPC_TSYM(false, true);
     } else {
-        SGIPC_CHECK(has_f32, SGIPC_BAD_ARG, "FP32 copy of the matrix was not built");
-        if (dot_part)
-            k_spmv_tsym<float, float, true><<<g, kThreads, 0, s>>>(
-                sell_vals32.get(), sell_cols.get(), slice_ptr.get(), slice_w.get(), perm.get(), plen.get(),
-                tinfo.get(), a, x, b, y, dot_part, done);
-        else
-            k_spmv_tsym<float, float, false><<<g, kThreads, 0, s>>>(
-                sell_vals32.get(), sell_cols.get(), slice_ptr.get(), slice_w.get(), perm.get(), plen.get(),
-                tinfo.get(), a, x, b, y, nullptr, done);
+        if (dot_part) SGIPC_TSYM(true, false);
+        else SGIPC_TSYM(false, false);
     }
-    SGIPC_LAUNCH_CHECK();
-    return dot_part ? g : 0;
-}
-int FullMatrix::apply_f32_values(hipStream_t s, double a, const double* x, double b, double* y, double* dot_part,
-                                 const int* done) const
-{
-    SGIPC_CHECK(n_tiles > 0, SGIPC_EMPTY, "matrix was not built");
-    SGIPC_CHECK(has_f32, SGIPC_BAD_ARG, "FP32 copy of the matrix was not built");
-    const int g = n_tiles;
-    if (dot_part)
-        k_spmv_tsym<float, double, true><<<g, kThreads, 0, s>>>(sell_vals32.get(), sell_cols.get(), slice_ptr.get(),
-                                                               slice_w.get(), perm.get(), plen.get(), tinfo.get(), a,
-                                                               x, b, y, dot_part, done);
-    else
-        k_spmv_tsym<float, double, false><<<g, kThreads, 0, s>>>(sell_vals32.get(), sell_cols.get(),
-                                                                slice_ptr.get(), slice_w.get(), perm.get(),
-                                                                plen.get(), tinfo.get(), a, x, b, y, nullptr, done);
+#undef SGIPC_TSYM
     SGIPC_LAUNCH_CHECK();
     return dot_part ? g : 0;
 }
 
+template <typename TV>
+int FullMatrix::apply(hipStream_t s, TV a, const TV* x, TV b, TV* y, double* dot_part, const int* done) const
+{
+    if constexpr (sizeof(TV) == 8) {
+        return launch<double, double>(s, sell_vals.get(), a, x, b, y, dot_part, done, nullptr, nullptr, nullptr,
+                                      nullptr, 0);
+    } else {
+        SGIPC_CHECK(has_f32, SGIPC_BAD_ARG, "FP32 copy of the matrix was not built");
+        return launch<float, float>(s, sell_vals32.get(), a, x, b, y, dot_part, done, nullptr, nullptr, nullptr,
+                                    nullptr, 0);
+    }
+}
+
+int FullMatrix::apply_f32_values(hipStream_t s, double a, const double* x, double b, double* y, double* dot_part,
+                                 const int* done) const
+{
+    SGIPC_CHECK(has_f32, SGIPC_BAD_ARG, "FP32 copy of the matrix was not built");
+    return launch<float, double>(s, sell_vals32.get(), a, x, b, y, dot_part, done, nullptr, nullptr, nullptr,
+                                 nullptr, 0);
+}
+
+template <typename TV>
+int FullMatrix::apply_update_p(hipStream_t s, bool f32_values, const TV* z, const TV* p_old, TV* p_new, TV* Ap,
+                               double* dot_part, const int* done, const double* rz_new, const double* rz_old,
+                               int n_rz) const
+{
+    if constexpr (sizeof(TV) == 8) {
+        if (f32_values) {
+            SGIPC_CHECK(has_f32, SGIPC_BAD_ARG, "FP32 copy of the matrix was not built");
+            return launch<float, double>(s, sell_vals32.get(), 1.0, p_old, 0.0, Ap, dot_part, done, z, p_new, rz_new,
+                                         rz_old, n_rz);
+        }
+        return launch<double, double>(s, sell_vals.get(), 1.0, p_old, 0.0, Ap, dot_part, done, z, p_new, rz_new,
+                                      rz_old, n_rz);
+    } else {
+        SGIPC_CHECK(has_f32, SGIPC_BAD_ARG, "FP32 copy of the matrix was not built");
+        return launch<float, float>(s, sell_vals32.get(), 1.0f, p_old, 0.0f, Ap, dot_part, done, z, p_new, rz_new,
+                                    rz_old, n_rz);
+    }
+}
+
 template int FullMatrix::apply<double>(hipStream_t, double, const double*, double, double*, double*, const int*) const;
 template int FullMatrix::apply<float>(hipStream_t, float, const float*, float, float*, double*, const int*) const;
+template int FullMatrix::apply_update_p<double>(hipStream_t, bool, const double*, const double*, double*, double*,
+                                                double*, const int*, const double*, const double*, int) const;
+template int FullMatrix::apply_update_p<float>(hipStream_t, bool, const float*, const float*, float*, float*, double*,
+                                               const int*, const double*, const double*, int) const;
 
 }  // namespace sgipc

@@ -21,6 +21,13 @@ class FullMatrix {
     // bytes), FP64 accumulation; needs want_f32 at build.
     int apply_f32_values(hipStream_t s, double a, const double* x, double b, double* y, double* dot_part,
                          const int* done) const;
+    // PCG iteration with the p update fused in: p_new = z + beta p_old
+    // (beta = rz_new / rz_old, summed from n_rz partials) is formed while it
+    // is gathered, written for every owned row, and Ap = A p_new.
+    template <typename TV>
+    int apply_update_p(hipStream_t s, bool f32_values, const TV* z, const TV* p_old, TV* p_new, TV* Ap,
+                       double* dot_part, const int* done, const double* rz_new, const double* rz_old,
+                       int n_rz) const;
     // partial count of apply() with dot_part
     int partials() const;
     // blocks stored in the layout (upper blocks + cross-tile transposed
@@ -39,6 +46,10 @@ class FullMatrix {
     DevBuf<int> slice_ptr, slice_w, perm, plen;
 
   private:
+    template <typename TM, typename TV>
+    int launch(hipStream_t s, const TM* vals, TV a, const TV* x, TV b, TV* y, double* dot_part, const int* done,
+               const TV* z, TV* p_new, const double* rzn, const double* rzo, int n_rz) const;
+
     DevBuf<uint64_t> keys_, skeys_, rkey_, rkey_sorted_;
     DevBuf<int> idx_, sidx_, cand_, cscan_, up_ptr_, col_ptr_, low_cnt_, len_, iota_, sorted_;
     DevBuf<uint32_t> up_word_;

@@ -39,7 +39,7 @@ def test_argument_checks_without_gpu(pkg):
     assert L.sgipc_abd_jacobi_apply(None, None, 1, None, None) == BAD_ARG
     opts = pkg._lib.PcgOpts()
     L.sgipc_pcg_default_opts(ctypes.byref(opts))
-    assert opts.tol > 0 and opts.max_iter == 0
+    assert opts.tol > 0 and opts.max_iter == 0 and opts.fuse == 1
     assert L.sgipc_abi_version() >= 1
 
 
