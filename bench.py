@@ -59,8 +59,9 @@ def main():
     ap.add_argument("--grid", type=int, default=64)
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-fuse", action="store_true",
-                    help="separate x/r and p update kernels instead of the fused PCG iteration")
+    ap.add_argument("--fuse", type=int, default=1, choices=[0, 1, 2],
+                    help="0: separate x/r and p update kernels; 1: x/r update inside the first MAS kernel; "
+                         "2: also the p update inside the SpMV")
     ap.add_argument("--cpu-pcg-iters", type=int, default=10,
                     help="PCG iterations timed on the host for the CPU baseline sample")
     ap.add_argument("--only-spmv", action="store_true", help="time the SpMV kernel only (for profiling runs)")
@@ -92,7 +93,7 @@ def main():
     if not args.only_spmv:
         # ---- warm-up solves (also sizes every buffer)
         for w in range(args.warmup):
-            r = ls.solve(tol=1e-5, precision=args.precision, fuse=not args.no_fuse)
+            r = ls.solve(tol=1e-5, precision=args.precision, fuse=args.fuse)
             log(f"warmup {w}: {r.timing['ms_total']:.2f} ms iters={r.iterations} "
                 f"outer={r.outer_iterations} res={r.true_rel_residual:.2e}")
         # ---- timed steps
@@ -101,7 +102,7 @@ def main():
         tt0 = time.perf_counter()
         for k in range(args.steps):
             s0 = time.perf_counter()
-            r = ls.solve(tol=1e-5, precision=args.precision, fuse=not args.no_fuse)
+            r = ls.solve(tol=1e-5, precision=args.precision, fuse=args.fuse)
             times.append((time.perf_counter() - s0) * 1e3)
             iters.append(r.iterations)
             res_list.append(r.true_rel_residual)
@@ -197,7 +198,7 @@ def main():
                          f"solve to ||b-Ax||/||b|| <= 1e-5"),
             "precision": args.precision,
             "bank_size": args.bank,
-            "fused_iteration": not args.no_fuse,
+            "fused_iteration": args.fuse,
             "n_verts": scene.n,
             "nnz_blocks": out["nnz"],
             "n_triplets": out["n_triplets"],

@@ -254,9 +254,10 @@ typedef struct sgipc_pcg_opts {
     int ir_max_outer;  /* FP32_IR: refinement steps (<= 0: 20)                         */
     double ir_inner_tol; /* FP32_IR: floor of the inner tolerance (<= 0: none; each pass
                             asks for the residual reduction still missing)           */
-    int fuse;          /* 1: fold the x/r update into the first MAS kernel and the p
-                          update into the SpMV (MAS on every row, >= 3 levels);
-                          0: separate vector kernels.  Same iterates either way. */
+    int fuse;          /* 1 (default): fold the x/r update into the first MAS kernel
+                          (MAS on every row, >= 3 levels); 2: also fold the p update
+                          into the SpMV; 0: separate vector kernels.  Same
+                          iterates either way. */
 } sgipc_pcg_opts;
 
 typedef struct sgipc_pcg_stats {

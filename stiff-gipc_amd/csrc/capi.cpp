@@ -457,7 +457,7 @@ int pcg_solve_impl(Context& c, SolveWorkspace& ws, const double* vals, const int
         po.stop_mode = o.stop_mode;
         po.max_iter = max_iter;
         po.check_every = o.check_every;
-        po.fuse = o.fuse != 0;
+        po.fuse = o.fuse;
         PcgResult r = ws.pcg64.solve(s, A, has_prec ? &M : nullptr, b, x, n3, po);
         st.iterations = r.iterations;
         st.outer_iterations = 1;
@@ -508,7 +508,7 @@ int pcg_solve_impl(Context& c, SolveWorkspace& ws, const double* vals, const int
             po.stop_mode = SGIPC_STOP_TRUE_RESIDUAL;
             po.max_iter = max_iter - total_it;
             po.check_every = o.check_every;
-            po.fuse = o.fuse != 0;
+            po.fuse = o.fuse;
             PcgResult r = ws.pcg64.solve(s, A, has_prec ? &M : nullptr, ws.r64.get(), ws.d64.get(), n3, po);
             total_it += r.iterations;
             if (r.not_spd) {

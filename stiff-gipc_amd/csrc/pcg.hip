@@ -351,8 +351,8 @@ PcgResult PcgSolver<TV>::solve(hipStream_t s, const SpmvOp<TV>& A, Preconditione
     //   MAS:   x += alpha p_k, r -= alpha Ap, ||r||^2 in the restriction;
     //          stop rule in the first coarse restriction; z = M r, r.z
     // i.e. the p update of iteration k-1 is deferred into the SpMV of k.
-    const bool fuse_p = opt.fuse && A.fuses_update_p();
-    bool fuse_xr = opt.fuse && M != nullptr;  // cleared when the preconditioner declines
+    const bool fuse_p = opt.fuse >= 2 && A.fuses_update_p();
+    bool fuse_xr = opt.fuse >= 1 && M != nullptr;  // cleared when the preconditioner declines
     bool p_pending = false;                   // p update of the previous iteration not yet applied
     const int gv = vec_grid(n3);
     const int chunk = opt.check_every > 0 ? opt.check_every : 8;

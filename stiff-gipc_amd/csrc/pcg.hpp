@@ -33,9 +33,10 @@ struct PcgOptions {
     int stop_mode = 1;     // 0 = reference |rz| <= tol*rz0, 1 = ||r|| <= tol*||b||
     int max_iter = 1000;
     int check_every = 8;
-    // fold the x/r update into the preconditioner's first kernel and the p
-    // update into the SpMV when both offer it
-    bool fuse = true;
+    // 1: fold the x/r update into the preconditioner's first kernel;
+    // 2: also fold the p update into the SpMV (gathers z and p_old, which
+    // costs more than the separate update on this part); 0: neither
+    int fuse = 1;
 };
 
 // Inputs of the x/r update (x += alpha p, r -= alpha Ap, alpha = rz/pAp) that

@@ -154,7 +154,7 @@ class MASPreconditioner:
 
 
 def pcg_opts(tol=1e-5, stop_mode=STOP_TRUE_RESIDUAL, max_iter=0, precision=FP64, check_every=8, ir_max_outer=20,
-             ir_inner_tol=0.0, fuse=True):
+             ir_inner_tol=0.0, fuse=1):
     o = PcgOpts()
     lib().sgipc_pcg_default_opts(ctypes.byref(o))
     o.tol = tol
@@ -164,7 +164,7 @@ def pcg_opts(tol=1e-5, stop_mode=STOP_TRUE_RESIDUAL, max_iter=0, precision=FP64,
     o.check_every = check_every
     o.ir_max_outer = ir_max_outer
     o.ir_inner_tol = ir_inner_tol
-    o.fuse = 1 if fuse else 0
+    o.fuse = int(fuse)
     return o
 
 

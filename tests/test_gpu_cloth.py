@@ -163,7 +163,10 @@ def test_cloth_mas_matches_oracle(pkg, orc, gpu_ctx, cloth, cloth_big, which):
             Md[pad, pad] = 1.0
             e_gpu = np.abs(blocks_to_dense(inv[d], B) @ Md - np.eye(3 * B)).max()
             e_orc = np.abs(blocks_to_dense(oinv[d], B) @ Md - np.eye(3 * B)).max()
-            assert e_gpu <= 10 * e_orc + 1e-12, (d, e_gpu, e_orc)
+            # as accurate as the oracle's, or within the rounding bound of an
+            # unpivoted elimination, a few hundred eps times cond(M_d)
+            bound = 300 * np.finfo(float).eps * np.linalg.cond(Md)
+            assert e_gpu <= max(10 * e_orc, bound) + 1e-12, (d, e_gpu, e_orc, bound)
         # the apply itself: restriction, the device's own inverse blocks, prolongation
         r = np.random.default_rng(6).standard_normal(3 * s.n)
         dr_ = gpu_ctx.to_device(r)

@@ -342,20 +342,21 @@ def test_mixed_precision_ir_converges(pkg, orc, scene_c0):
         ls.close()
 
 
+@pytest.mark.parametrize("fuse", [1, 2])
 @pytest.mark.parametrize("precision", ["fp64", "fp32_mas"])
-def test_fused_iteration_matches_unfused(pkg, scene_c0, precision):
+def test_fused_iteration_matches_unfused(pkg, scene_c0, precision, fuse):
     """The fused iteration (p update inside the SpMV, x/r update inside the
     first MAS kernel) performs the same arithmetic as the separate kernels;
     only the grouping of the ||r||^2 partials of the stop rule differs."""
     ls = pkg.LinearSystem(scene_c0, bank_size=32)
     try:
         k = 25
-        a = ls.solve(tol=1e-30, max_iter=k, precision=precision, fuse=True)
-        b = ls.solve(tol=1e-30, max_iter=k, precision=precision, fuse=False)
+        a = ls.solve(tol=1e-30, max_iter=k, precision=precision, fuse=fuse)
+        b = ls.solve(tol=1e-30, max_iter=k, precision=precision, fuse=0)
         assert a.iterations == b.iterations == k - 1
         assert np.linalg.norm(a.x - b.x) <= 1e-14 * np.linalg.norm(b.x)
-        a = ls.solve(tol=1e-6, precision=precision, fuse=True)
-        b = ls.solve(tol=1e-6, precision=precision, fuse=False)
+        a = ls.solve(tol=1e-6, precision=precision, fuse=fuse)
+        b = ls.solve(tol=1e-6, precision=precision, fuse=0)
         assert abs(a.iterations - b.iterations) <= 1, (a.iterations, b.iterations)
         assert a.true_rel_residual <= 1e-6 and b.true_rel_residual <= 1e-6
     finally:
