@@ -59,6 +59,8 @@ def main():
     ap.add_argument("--grid", type=int, default=64)
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--coarse", type=int, default=0, choices=[0, 1],
+                    help="MAS coarse levels: 0 additive (the reference), 1 V(1,1) cycle on level 1")
     ap.add_argument("--fuse", type=int, default=1, choices=[0, 1, 2],
                     help="0: separate x/r and p update kernels; 1: x/r update inside the first MAS kernel; "
                          "2: also the p update inside the SpMV")
@@ -84,7 +86,7 @@ def main():
     log(f"scene {scene.name}: n={scene.n} tets={scene.n_tets} pairs={scene.n_pairs} "
         f"({time.perf_counter() - t0:.1f}s to generate)")
 
-    ls = pkg.LinearSystem(scene, bank_size=args.bank)
+    ls = pkg.LinearSystem(scene, bank_size=args.bank, coarse_mode=args.coarse)
     ctx = ls.ctx
     L = pkg._lib.lib()
     import ctypes
@@ -199,6 +201,7 @@ def main():
             "precision": args.precision,
             "bank_size": args.bank,
             "fused_iteration": args.fuse,
+            "mas_coarse_mode": "level-1 V(1,1) cycle" if args.coarse else "additive",
             "n_verts": scene.n,
             "nnz_blocks": out["nnz"],
             "n_triplets": out["n_triplets"],

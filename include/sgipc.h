@@ -239,6 +239,15 @@ int sgipc_mas_levels(sgipc_mas* mas, int* level_num, int* offsets, int* counts);
 int sgipc_mas_tables(sgipc_mas* mas, int* coarse_table, int* going_next);
 /* Measurement helper: average duration (ms) of one preconditioning (all of its
  * kernels) over `reps` applications, HIP events on the context's stream. */
+/* Coarse levels of the apply.  0 (default): additive over all levels, as
+ * MASPreconditioner::preconditioning (MASPreconditioner.cu:2248-2289).
+ * 1: no counterpart in the reference -- the levels >= 1 become a symmetric
+ * V(1,1) cycle on level 1 (level-1 domain solves as the smoother, the additive
+ * levels >= 2 as its coarse correction, residuals with the level-1 Galerkin
+ * operator); level 0 stays additive, so the preconditioner is still
+ * symmetric.  Takes effect at the next sgipc_mas_set_preconditioner_bcoo (or
+ * system solve). */
+int sgipc_mas_set_coarse_mode(sgipc_mas* mas, int mode);
 int sgipc_mas_time_apply(sgipc_mas* mas, const double* r, double* z, int reps, double* avg_ms);
 /* Inverse domain matrices as FP64 upper blocks (device output,
  * total_slots/B * B(B+1)/2 * 9 doubles). */
@@ -258,6 +267,10 @@ typedef struct sgipc_pcg_opts {
                           (MAS on every row, >= 3 levels); 2: also fold the p update
                           into the SpMV; 0: separate vector kernels.  Same
                           iterates either way. */
+    int flexible;      /* 1: flexible PCG, beta = -(z.Ap)/(p.Ap) (a preconditioner that
+                          may change between iterations, e.g. the learned level); needs
+                          the MAS on every row.  Same iterates as PCG for a fixed SPD
+                          preconditioner, up to rounding. */
 } sgipc_pcg_opts;
 
 typedef struct sgipc_pcg_stats {
@@ -356,6 +369,51 @@ int sgipc_system_solve(sgipc_system* sys, const double* b, double* x, double* gr
 int sgipc_system_bcoo(sgipc_system* sys, const double** vals, const int** rows, const int** cols,
                       int* nnz);
 sgipc_mas* sgipc_system_mas(sgipc_system* sys);
+
+/* ---------------------------------------------------------------------------
+ * Learned coarse level.  No counterpart in the reference (it has no learned
+ * component): a message-passing network over the MAS level-1 aggregate graph,
+ * run with BF16 MFMA on gfx950, whose output is a coarse correction added to
+ * the MAS result inside flexible PCG.
+ *   X (3 features per row: S P^T r, S^-1 zc, S P^T r - Ahat S^-1 zc)
+ *   V0 = X W_in; V <- V + act([V, Ahat V] W_l); chat = V w_out
+ * with Ahat = S A_1 S the Jacobi-scaled level-1 Galerkin operator, act = relu
+ * (linear == 0) or the identity.  Parameters: W_in (3 x 64), W_l (128 x 64
+ * each), w_out (64), row-major, in that order.  Training minimises
+ * (0.5 chat^T Ahat chat - chat^T q) / rows per sample with Adam.
+ * ------------------------------------------------------------------------- */
+typedef struct sgipc_mpnn sgipc_mpnn;
+typedef struct sgipc_mpnn_data sgipc_mpnn_data;
+int sgipc_mpnn_create(sgipc_ctx* ctx, int layers, int linear, sgipc_mpnn** out);
+int sgipc_mpnn_destroy(sgipc_mpnn* net);
+int sgipc_mpnn_num_params(const sgipc_mpnn* net);
+int sgipc_mpnn_set_params(sgipc_mpnn* net, const float* host, int n);
+int sgipc_mpnn_get_params(const sgipc_mpnn* net, float* host, int n);
+int sgipc_mpnn_init_params(sgipc_mpnn* net, unsigned long long seed);
+/* A training set over one level-1 graph, given as the upper BCOO of A_1
+ * (device pointers, FP64 column-major blocks, sorted unique). */
+int sgipc_mpnn_data_create(sgipc_ctx* ctx, int n_nodes, const double* vals, const int* rows,
+                           const int* cols, int nb, sgipc_mpnn_data** out);
+/* Append samples: X [n_samples][3 n_nodes][3], q [n_samples][3 n_nodes] (device). */
+int sgipc_mpnn_data_add(sgipc_mpnn_data* d, const float* X, const float* q, int n_samples);
+int sgipc_mpnn_data_size(const sgipc_mpnn_data* d);
+int sgipc_mpnn_data_nodes(const sgipc_mpnn_data* d);
+int sgipc_mpnn_data_samples(const sgipc_mpnn_data* d, int first, int count, float* X_host,
+                            float* q_host);
+/* The graph as built: CSR over nodes (both directions, diagonal included),
+ * Jacobi-scaled row-major 3x3 blocks, node scales (host buffers). */
+int sgipc_mpnn_data_graph(const sgipc_mpnn_data* d, int* row_ptr, int* col, float* blk, float* scale);
+int sgipc_mpnn_data_graph_nnz(const sgipc_mpnn_data* d);
+int sgipc_mpnn_data_destroy(sgipc_mpnn_data* d);
+/* chat of samples [first, first+count) into chat (device, count * 3 n_nodes). */
+int sgipc_mpnn_forward(sgipc_mpnn* net, const sgipc_mpnn_data* d, int first, int count, float* chat);
+/* Mean loss and gradient (host, num_params floats) over samples [first, first+count). */
+int sgipc_mpnn_loss_grad(sgipc_mpnn* net, const sgipc_mpnn_data* d, int first, int count,
+                         double* loss, float* grad_host);
+/* Adam over shuffled minibatches; epoch_loss (host, optional) gets the mean
+ * minibatch loss of every epoch. */
+int sgipc_mpnn_train(sgipc_mpnn* net, sgipc_mpnn_data* d, int epochs, int batch, float lr,
+                     unsigned seed, float* epoch_loss);
 
 #ifdef __cplusplus
 }

@@ -18,8 +18,9 @@ Model (C channels, L layers, rows = 3 * nodes, one row per node component):
   rho = ghat - Ahat zhat, where zc is the coarse part (levels >= 1) of the
   MAS solution restricted to level 1.
 * V0 = X W_in; for each layer l: U = [V, Ahat V], Y = U W_l,
-  V <- V + relu(Y); chat = V w_out.  No biases, so the network is positively
-  homogeneous: f(a X) = a f(X) for a > 0.
+  V <- V + act(Y); chat = V w_out, act = relu or the identity.  No biases,
+  so the network is positively homogeneous: f(a X) = a f(X) for a > 0 (and
+  linear in X with the identity activation).
 * Correction c = S chat, added to the MAS output as P c.
 * Loss per sample: (0.5 chat^T Ahat chat - chat^T qhat) / rows with
   qhat = S P^T (r - A M r).  In original units this is the A-norm error of
@@ -161,8 +162,9 @@ def _r(x, on):
     return bf16(x) if on else x
 
 
-def forward(p: Params, Ahat, X, bf16_ops=False):
-    """chat for one sample; returns (chat, cache)."""
+def forward(p: Params, Ahat, X, bf16_ops=False, linear=False):
+    """chat for one sample; returns (chat, cache).  linear=True uses the
+    identity activation."""
     V = X @ p.W_in
     cache = []
     for l in range(p.L):
@@ -170,7 +172,7 @@ def forward(p: Params, Ahat, X, bf16_ops=False):
         Ub = _r(U, bf16_ops)
         Y = Ub @ _r(p.W[l], bf16_ops)
         cache.append((V, Ub, Y))
-        V = V + np.maximum(Y, 0.0)
+        V = V + (Y if linear else np.maximum(Y, 0.0))
     chat = V @ p.w_out
     cache.append((V,))
     return chat, cache
@@ -181,7 +183,7 @@ def loss(chat, Ahat, qhat):
     return (0.5 * chat @ (Ahat @ chat) - chat @ qhat) / rows
 
 
-def loss_grad(p: Params, Ahat, samples, bf16_ops=False):
+def loss_grad(p: Params, Ahat, samples, bf16_ops=False, linear=False):
     """Mean loss over samples [(X, qhat), ...] and its gradient (Params)."""
     gW_in = np.zeros_like(p.W_in)
     gW = np.zeros_like(p.W)
@@ -189,7 +191,7 @@ def loss_grad(p: Params, Ahat, samples, bf16_ops=False):
     total = 0.0
     ns = len(samples)
     for X, qhat in samples:
-        chat, cache = forward(p, Ahat, X, bf16_ops)
+        chat, cache = forward(p, Ahat, X, bf16_ops, linear)
         rows = chat.size
         total += loss(chat, Ahat, qhat)
         dc = (Ahat @ chat - qhat) / (rows * ns)
@@ -198,7 +200,7 @@ def loss_grad(p: Params, Ahat, samples, bf16_ops=False):
         G = np.outer(dc, p.w_out)
         for l in range(p.L - 1, -1, -1):
             V, Ub, Y = cache[l]
-            dY = G * (Y > 0)
+            dY = G if linear else G * (Y > 0)
             dYb = _r(dY, bf16_ops)
             gW[l] += Ub.T @ dYb
             dU = dYb @ _r(p.W[l], bf16_ops).T
@@ -216,7 +218,8 @@ def optimal_loss(Ahat, qhat):
 
 
 # ----------------------------------------------------------------- training
-def adam(p: Params, Ahat, samples, epochs, batch, lr=1e-3, seed=0, b1=0.9, b2=0.999, eps=1e-8, bf16_ops=False):
+def adam(p: Params, Ahat, samples, epochs, batch, lr=1e-3, seed=0, b1=0.9, b2=0.999, eps=1e-8, bf16_ops=False,
+         linear=False):
     """Plain Adam over shuffled minibatches; returns (params, loss history)."""
     v = p.flat()
     m = np.zeros_like(v)
@@ -230,7 +233,7 @@ def adam(p: Params, Ahat, samples, epochs, batch, lr=1e-3, seed=0, b1=0.9, b2=0.
         for b0 in range(0, len(order), batch):
             idx = order[b0:b0 + batch]
             cur = Params.from_flat(v, C, L)
-            lv, g = loss_grad(cur, Ahat, [samples[i] for i in idx], bf16_ops)
+            lv, g = loss_grad(cur, Ahat, [samples[i] for i in idx], bf16_ops, linear)
             g = g.flat()
             t += 1
             m = b1 * m + (1 - b1) * g
@@ -292,6 +295,14 @@ class MasParts:
         return z.reshape(-1), zc.reshape(-1)
 
 
+def level2_operator(parts: MasParts, graph):
+    """Dense Galerkin operator of level 2, P_12^T A_1 P_12."""
+    off = parts.off
+    map2 = parts.slot[parts.first, 2] - off[2]
+    P12 = prolongation(map2, parts.cnt[2])
+    return (P12.T @ graph.A1 @ P12).toarray()
+
+
 def sample(A, graph, parts, r):
     """Features and target of one residual r (rows of the fine system)."""
     z, zc = parts.apply(r)
@@ -301,3 +312,64 @@ def sample(A, graph, parts, r):
     sc = np.sqrt(np.mean(X[:, 0] ** 2))
     sc = sc if sc > 0 else 1.0
     return X / sc, qhat / sc, z
+
+
+# ------------------------------------------------------- level-1 coarse cycle
+def float_operator(graph):
+    """A_1 as the device applies it: S^-1 Ahat S^-1 with s rounded to float32
+    first and the scaled blocks (float32(s_i s_j A_ij)) rounded to float32."""
+    s32 = graph.s.astype(np.float32).astype(np.float64)
+    S = sp.diags(np.repeat(s32, 3))
+    blk = (S @ graph.A1 @ S).tocsr()
+    blk.data = blk.data.astype(np.float32).astype(np.float64)
+    w = np.where(s32 > 0, 1.0 / np.where(s32 > 0, s32, 1.0), 0.0)
+    W = sp.diags(np.repeat(w, 3))
+    return (W @ blk @ W).tocsr()
+
+
+def mas_cycle_apply(parts: MasParts, A1f, r, A2=None):
+    """MAS apply with the level-1 V(1,1) cycle (the library's coarse mode 1):
+    level 0 additive; on level 1 pre-smoothing with the level-1 domain
+    solves, a coarse correction of the residual, post-smoothing; residuals
+    with A1f.  The coarse correction is the exact solve with A2 (the level-2
+    Galerkin operator, dense) when given, else the additive levels >= 2."""
+    n, B, total, L = parts.n, parts.B, parts.total, parts.L
+    off = parts.off
+    c1 = parts.cnt[1]
+    r3 = r.reshape(n, 3)
+    inv = parts.inv
+
+    def solve(R, lo_slot, hi_slot):
+        Z = np.zeros((total, 3))
+        d0, d1 = lo_slot // B, hi_slot // B
+        Z.reshape(-1, 3 * B)[d0:d1] = np.einsum("dij,dj->di", inv[d0:d1], R.reshape(-1, 3 * B)[d0:d1])
+        return Z
+
+    R = np.zeros((total, 3))
+    np.add.at(R, parts.slot[:, 0], r3)
+    np.add.at(R, parts.slot[:, 1], r3)
+    Z = solve(R, 0, off[2])                    # level-0 solves and c1 = D1^-1 g
+    g = R[off[1]:off[1] + c1].reshape(-1).copy()
+    c1v = Z[off[1]:off[1] + c1].reshape(-1).copy()
+    rho = g - A1f @ c1v
+    slots1 = parts.slot[parts.first]           # ancestors of every level-1 node
+    C2 = c1v.reshape(-1, 3).copy()
+    if A2 is not None:
+        c2 = parts.cnt[2]
+        R2 = np.zeros((c2, 3))
+        np.add.at(R2, slots1[:, 2] - off[2], rho.reshape(-1, 3))
+        e2 = np.linalg.solve(A2, R2.reshape(-1)).reshape(-1, 3)
+        C2 += e2[slots1[:, 2] - off[2]]
+    else:
+        Rc = np.zeros((total, 3))
+        for l in range(2, L):
+            np.add.at(Rc, slots1[:, l], rho.reshape(-1, 3))
+        Zc = solve(Rc, off[2], off[L])
+        for l in range(2, L):
+            C2 += Zc[slots1[:, l]]
+    rho2 = g - A1f @ C2.reshape(-1)
+    R2 = np.zeros((total, 3))
+    R2[off[1]:off[1] + c1] = rho2.reshape(-1, 3)
+    Z1 = solve(R2, off[1], off[2])[off[1]:off[1] + c1]
+    z = Z[:n] + C2[parts.map1] + Z1[parts.map1]
+    return z.reshape(-1)

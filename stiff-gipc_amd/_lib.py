@@ -30,7 +30,7 @@ csz = ctypes.c_size_t
 
 class PcgOpts(ctypes.Structure):
     _fields_ = [("tol", cd), ("stop_mode", ci), ("max_iter", ci), ("precision", ci), ("check_every", ci),
-                ("ir_max_outer", ci), ("ir_inner_tol", cd), ("fuse", ci)]
+                ("ir_max_outer", ci), ("ir_inner_tol", cd), ("fuse", ci), ("flexible", ci)]
 
 
 class PcgStats(ctypes.Structure):
@@ -106,6 +106,24 @@ _SIGS = {
                                 ctypes.POINTER(SolveTiming)]),
     "sgipc_system_bcoo": (ci, [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(ci)]),
     "sgipc_system_mas": (vp, [vp]),
+    "sgipc_mas_set_coarse_mode": (ci, [vp, ci]),
+    "sgipc_mpnn_create": (ci, [vp, ci, ci, ctypes.POINTER(vp)]),
+    "sgipc_mpnn_destroy": (ci, [vp]),
+    "sgipc_mpnn_num_params": (ci, [vp]),
+    "sgipc_mpnn_set_params": (ci, [vp, vp, ci]),
+    "sgipc_mpnn_get_params": (ci, [vp, vp, ci]),
+    "sgipc_mpnn_init_params": (ci, [vp, ctypes.c_ulonglong]),
+    "sgipc_mpnn_data_create": (ci, [vp, ci, vp, vp, vp, ci, ctypes.POINTER(vp)]),
+    "sgipc_mpnn_data_add": (ci, [vp, vp, vp, ci]),
+    "sgipc_mpnn_data_size": (ci, [vp]),
+    "sgipc_mpnn_data_nodes": (ci, [vp]),
+    "sgipc_mpnn_data_samples": (ci, [vp, ci, ci, vp, vp]),
+    "sgipc_mpnn_data_graph": (ci, [vp, vp, vp, vp, vp]),
+    "sgipc_mpnn_data_graph_nnz": (ci, [vp]),
+    "sgipc_mpnn_data_destroy": (ci, [vp]),
+    "sgipc_mpnn_forward": (ci, [vp, vp, ci, ci, vp]),
+    "sgipc_mpnn_loss_grad": (ci, [vp, vp, ci, ci, ctypes.POINTER(cd), vp]),
+    "sgipc_mpnn_train": (ci, [vp, vp, ci, ci, ctypes.c_float, ctypes.c_uint, vp]),
 }
 
 _LIB = None

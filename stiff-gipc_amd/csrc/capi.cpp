@@ -370,6 +370,13 @@ int sgipc_mas_preconditioning(sgipc_mas* mas, const double* r, double* z)
     });
 }
 
+int sgipc_mas_set_coarse_mode(sgipc_mas* mas, int mode)
+{
+    if (!mas || mode < 0 || mode > 1) return SGIPC_BAD_ARG;
+    mas->m->coarse_mode = mode;
+    return SGIPC_OK;
+}
+
 int sgipc_mas_time_apply(sgipc_mas* mas, const double* r, double* z, int reps, double* avg_ms)
 {
     if (!mas || !r || !z || !avg_ms || reps < 1) return SGIPC_BAD_ARG;
@@ -423,6 +430,7 @@ void sgipc_pcg_default_opts(sgipc_pcg_opts* o)
     o->precision = SGIPC_FP64;
     o->check_every = 8;
     o->fuse = 1;
+    o->flexible = 0;
     o->ir_max_outer = 20;
     o->ir_inner_tol = 0.0;
 }
@@ -458,6 +466,7 @@ int pcg_solve_impl(Context& c, SolveWorkspace& ws, const double* vals, const int
         po.max_iter = max_iter;
         po.check_every = o.check_every;
         po.fuse = o.fuse;
+        po.flexible = o.flexible != 0;
         PcgResult r = ws.pcg64.solve(s, A, has_prec ? &M : nullptr, b, x, n3, po);
         st.iterations = r.iterations;
         st.outer_iterations = 1;
@@ -509,6 +518,8 @@ int pcg_solve_impl(Context& c, SolveWorkspace& ws, const double* vals, const int
             po.max_iter = max_iter - total_it;
             po.check_every = o.check_every;
             po.fuse = o.fuse;
+            po.flexible = o.flexible != 0;
+        po.flexible = o.flexible != 0;
             PcgResult r = ws.pcg64.solve(s, A, has_prec ? &M : nullptr, ws.r64.get(), ws.d64.get(), n3, po);
             total_it += r.iterations;
             if (r.not_spd) {

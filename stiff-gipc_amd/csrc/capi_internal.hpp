@@ -95,6 +95,14 @@ struct SubsystemPrecond : Preconditioner<TV> {
     }
     int max_partials() const override { return 1024 + (p.abd ? p.abd->partials() : 0); }
     int max_rr_partials() const override { return p.mas ? p.mas->rr_partials() : 0; }
+    // flexible PCG needs z.Ap over every row: only when the MAS covers them
+    bool set_flex(const TV* Ap, double* zap) override
+    {
+        if (!p.mas || p.mas_row_offset != 0 || (p.abd && p.abd->n_bodies > 0)) return false;
+        p.mas->flex_Ap_ = Ap;
+        p.mas->flex_zap_ = zap;
+        return true;
+    }
 };
 
 int pcg_solve_impl(Context& c, SolveWorkspace& ws, const double* vals, const int* rows, const int* cols,

@@ -31,6 +31,8 @@
 //    with coalesced 16-byte loads and computes each row in a fixed order;
 //    prolongation fuses the r.z partial for PCG.
 #include "mas.hpp"
+
+#include <rocsolver/rocsolver.h>
 #include "assembly.hpp"
 #include "wave_ops.hpp"
 
@@ -762,7 +764,7 @@ struct Vec16<float> {
 // a fixed order.
 template <int B, typename TS>
 __global__ __launch_bounds__(256) void k_local_solve(const int* __restrict__ done, const TS* __restrict__ minv,
-                                                      const TS* __restrict__ R, TS* __restrict__ Z)
+                                                      const TS* __restrict__ R, TS* __restrict__ Z, int dom0)
 {
     if (done != nullptr && *done) return;
     constexpr int nblk = B * (B + 1) / 2;
@@ -774,7 +776,7 @@ __global__ __launch_bounds__(256) void k_local_solve(const int* __restrict__ don
     constexpr int G = 256 / B;  // threads per block row
     __shared__ __align__(16) TS sM[nval];
     __shared__ TS sR[3 * B];
-    const int d = blockIdx.x;
+    const int d = dom0 + blockIdx.x;
     const V* vsrc = reinterpret_cast<const V*>(minv + (long)d * nval);
     V tmp[per];
     // unconditional (clamped) loads so all of them are in flight together
@@ -822,35 +824,182 @@ __global__ __launch_bounds__(256) void k_local_solve(const int* __restrict__ don
     }
 }
 
-// z[v] = sum_l Z[slot_l(v)], with r.z partials.
+// z[v] = sum_l Z[slot_l(v)], with r.z partials.  With a level-1 cycle
+// (C2 != nullptr) the coarse part is C2[map1(v)] + Z[slot_1(v)] (the cycle's
+// coarse-corrected iterate and its post-smoothing).  corr (level-1, float)
+// adds the learned correction; with Ap set the z.Ap partials of flexible PCG
+// go to zap_part.
 template <typename TV, typename TS>
 __global__ __launch_bounds__(256) void k_prolong(const int* __restrict__ done, const int* __restrict__ meta,
                                                  int n, int L, const int* __restrict__ map,
                                                  const TS* __restrict__ Z, const TV* __restrict__ r,
-                                                 TV* __restrict__ z, double* __restrict__ rz_part)
+                                                 TV* __restrict__ z, double* __restrict__ rz_part,
+                                                 const TS* __restrict__ C2, const float* __restrict__ corr,
+                                                 const TV* __restrict__ Ap, double* __restrict__ zap_part)
 {
     __shared__ double smem[16];
     if (done != nullptr && *done) return;
     int offs[kMasMaxLevels];
 #pragma unroll
     for (int l = 0; l < kMasMaxLevels; ++l) offs[l] = l < L ? meta[kMetaOff + l] : 0;
-    double acc = 0.0;
+    double acc = 0.0, acc2 = 0.0;
     for (long u = (long)blockIdx.x * blockDim.x + threadIdx.x; u < n; u += (long)gridDim.x * blockDim.x) {
         TS z0 = Z[3 * u], z1 = Z[3 * u + 1], z2 = Z[3 * u + 2];
-        for (int l = 1; l < L; ++l) {
-            const long o = 3 * (long)(offs[l] + map[(long)l * n + u]);
-            z0 += Z[o];
-            z1 += Z[o + 1];
-            z2 += Z[o + 2];
+        if (C2 != nullptr) {
+            const long k = map[(long)n + u];
+            const long o = 3 * (offs[1] + k);
+            z0 += C2[3 * k] + Z[o];
+            z1 += C2[3 * k + 1] + Z[o + 1];
+            z2 += C2[3 * k + 2] + Z[o + 2];
+        } else {
+            for (int l = 1; l < L; ++l) {
+                const long o = 3 * (long)(offs[l] + map[(long)l * n + u]);
+                z0 += Z[o];
+                z1 += Z[o + 1];
+                z2 += Z[o + 2];
+            }
+        }
+        if (corr != nullptr) {
+            const long k = map[(long)n + u];
+            z0 += (TS)corr[3 * k];
+            z1 += (TS)corr[3 * k + 1];
+            z2 += (TS)corr[3 * k + 2];
         }
         const TV w0 = (TV)z0, w1 = (TV)z1, w2 = (TV)z2;
         z[3 * u] = w0;
         z[3 * u + 1] = w1;
         z[3 * u + 2] = w2;
         acc += (double)r[3 * u] * (double)w0 + (double)r[3 * u + 1] * (double)w1 + (double)r[3 * u + 2] * (double)w2;
+        if (Ap != nullptr)
+            acc2 += (double)Ap[3 * u] * (double)w0 + (double)Ap[3 * u + 1] * (double)w1 +
+                    (double)Ap[3 * u + 2] * (double)w2;
     }
     double s = block_sum(acc, smem);
     if (threadIdx.x == 0 && rz_part) rz_part[blockIdx.x] = s;
+    if (Ap != nullptr) {
+        s = block_sum(acc2, smem);
+        if (threadIdx.x == 0) zap_part[blockIdx.x] = s;
+    }
+}
+
+// Level-1 residual of the coarse cycle, rho = g - A_1 c with
+// A_1 = S^-1 Ahat S^-1 (the Jacobi-scaled operator of the level-1 graph),
+// written over the level-1 slots of R.  One wave per level-1 node, lanes over
+// its CSR row.  First pass: c = Z at the level-1 slots (the pre-smoothed
+// iterate), g read from R and saved in G1.  Second pass: c = C2, g from G1.
+template <typename TS, bool SECOND>
+__global__ __launch_bounds__(256) void k_l1_residual(const int* __restrict__ done, const int* __restrict__ meta, int c1,
+                                                     const int* __restrict__ rp, const int* __restrict__ colx,
+                                                     const float* __restrict__ blk, const float* __restrict__ sc,
+                                                     TS* __restrict__ R, const TS* __restrict__ Z,
+                                                     TS* __restrict__ G1, const TS* __restrict__ C2)
+{
+    if (done != nullptr && *done) return;
+    const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (k >= c1) return;
+    const int off1 = meta[kMetaOff + 1];
+    const TS* cv = SECOND ? C2 : Z + 3L * off1;
+    TS a0 = 0, a1 = 0, a2 = 0;
+    for (int e = rp[k] + lane; e < rp[k + 1]; e += 64) {
+        const int j = colx[e];
+        const float* b = blk + 9L * e;
+        const float sj = sc[j];
+        const TS wj = sj > 0.f ? (TS)1 / (TS)sj : (TS)0;
+        const TS c0 = wj * cv[3L * j], c1v = wj * cv[3L * j + 1], c2v = wj * cv[3L * j + 2];
+        a0 += (TS)b[0] * c0 + (TS)b[1] * c1v + (TS)b[2] * c2v;
+        a1 += (TS)b[3] * c0 + (TS)b[4] * c1v + (TS)b[5] * c2v;
+        a2 += (TS)b[6] * c0 + (TS)b[7] * c1v + (TS)b[8] * c2v;
+    }
+    a0 = wave_sum(a0);
+    a1 = wave_sum(a1);
+    a2 = wave_sum(a2);
+    if (lane != 0) return;
+    const long o = 3 * (long)(off1 + k);
+    TS g0, g1, g2;
+    if (!SECOND) {
+        g0 = R[o];
+        g1 = R[o + 1];
+        g2 = R[o + 2];
+        G1[3L * k] = g0;
+        G1[3L * k + 1] = g1;
+        G1[3L * k + 2] = g2;
+    } else {
+        g0 = G1[3L * k];
+        g1 = G1[3L * k + 1];
+        g2 = G1[3L * k + 2];
+    }
+    const float sk = sc[k];
+    const TS wk = sk > 0.f ? (TS)1 / (TS)sk : (TS)0;
+    R[o] = g0 - wk * a0;
+    R[o + 1] = g1 - wk * a1;
+    R[o + 2] = g2 - wk * a2;
+}
+
+// C2 = (pre-smoothed level-1 iterate) + the coarse correction of levels
+// 2..Lc-1 along going_next.
+template <typename TS>
+__global__ __launch_bounds__(256) void k_l1_chain(const int* __restrict__ done, const int* __restrict__ meta, int Lc,
+                                                  const int* __restrict__ going_next, int c1, const TS* __restrict__ Z,
+                                                  TS* __restrict__ C2)
+{
+    if (done != nullptr && *done) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= c1) return;
+    long sl = meta[kMetaOff + 1] + k;
+    TS a = Z[3 * sl], b = Z[3 * sl + 1], c = Z[3 * sl + 2];
+    for (int l = 2; l < Lc; ++l) {
+        sl = going_next[sl];
+        a += Z[3 * sl];
+        b += Z[3 * sl + 1];
+        c += Z[3 * sl + 2];
+    }
+    C2[3L * k] = a;
+    C2[3L * k + 1] = b;
+    C2[3L * k + 2] = c;
+}
+
+// y = Minv x for the dense FP64 inverse of the level-2 operator (m x m,
+// row-major, symmetric); one wave per row.
+template <typename TS>
+__global__ __launch_bounds__(256) void k_dense_gemv(const int* __restrict__ done, int m,
+                                                    const double* __restrict__ Minv, const TS* __restrict__ x,
+                                                    TS* __restrict__ y)
+{
+    if (done != nullptr && *done) return;
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= m) return;
+    const double* a = Minv + (long)row * m;
+    double acc = 0.0;
+    for (int c = lane; c < m; c += 64) acc += a[c] * (double)x[c];
+    acc = wave_sum(acc);
+    if (lane == 0) y[row] = (TS)acc;
+}
+
+// Dense symmetric level-2 operator from its upper BCOO (column-major blocks).
+__global__ void k_dense_scatter(const double* __restrict__ v, const int* __restrict__ r, const int* __restrict__ c,
+                                int nb, int m, double* __restrict__ D)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nb) return;
+    const long i0 = 3L * r[k], j0 = 3L * c[k];
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+            const double x = v[9L * k + 3 * b + a];  // M[a][b]
+            D[(i0 + a) * m + j0 + b] = x;
+            D[(j0 + b) * m + i0 + a] = x;
+        }
+}
+
+// rocSOLVER leaves the inverse in its column-major lower triangle, i.e. the
+// row-major upper one: mirror it below the diagonal
+__global__ void k_mirror_upper(int m, double* __restrict__ D)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)m * m) return;
+    const int i = (int)(t / m), j = (int)(t % m);
+    if (j < i) D[t] = D[(long)j * m + i];
 }
 
 int grid_for(long n, int bs) { return (int)((n + bs - 1) / bs); }
@@ -991,6 +1140,15 @@ void Mas::assemble_invert_t(const double* vals, const int* rows, const int* cols
                                                                      coarse_c_[q].get(), cn, h_off[l], ndom0,
                                                                      mgal_.get());
             SGIPC_LAUNCH_CHECK();
+            if (l == 2 && coarse_mode == 1) build_dense_level2(s, coarse_v_[q].get(), coarse_r_[q].get(),
+                                                              coarse_c_[q].get(), cn);
+            if (l == 1 && want_graph()) {
+                graph_.build(s, coarse_v_[q].get(), coarse_r_[q].get(), coarse_c_[q].get(), cn, h_cnt[1]);
+                g1d_.reserve(3L * h_cnt[1]);
+                c2d_.reserve(3L * h_cnt[1]);
+                g1f_.reserve(3L * h_cnt[1]);
+                c2f_.reserve(3L * h_cnt[1]);
+            }
             pv = coarse_v_[q].get();
             pr = coarse_r_[q].get();
             pc = coarse_c_[q].get();
@@ -1022,6 +1180,43 @@ void Mas::assemble_invert_t(const double* vals, const int* rows, const int* cols
     Z32_.reserve((size_t)total * 3);
 }
 
+void Mas::build_dense_level2(hipStream_t s, const double* v, const int* r, const int* c, int nb)
+{
+    dense_m_ = 0;
+    const long m = 3L * h_cnt[2];
+    if (m < 1 || m > kDenseMax || nb < 1) return;
+    dense_inv_.reserve((size_t)m * m);
+    launch_fill<double>(s, dense_inv_.get(), 0.0, m * m);
+    k_dense_scatter<<<grid_for(nb, 256), 256, 0, s>>>(v, r, c, nb, (int)m, dense_inv_.get());
+    SGIPC_LAUNCH_CHECK();
+    if (!rb_) {
+        rocblas_handle h = nullptr;
+        SGIPC_CHECK(rocblas_create_handle(&h) == rocblas_status_success, SGIPC_HIP_ERROR, "rocblas_create_handle failed");
+        rb_ = h;
+    }
+    rocblas_handle h = static_cast<rocblas_handle>(rb_);
+    SGIPC_CHECK(rocblas_set_stream(h, s) == rocblas_status_success, SGIPC_HIP_ERROR, "rocblas_set_stream failed");
+    dense_info_.reserve(2);
+    SGIPC_CHECK(rocsolver_dpotrf(h, rocblas_fill_lower, (rocblas_int)m, dense_inv_.get(), (rocblas_int)m,
+                                 dense_info_.get()) == rocblas_status_success,
+                SGIPC_HIP_ERROR, "rocsolver_dpotrf failed");
+    SGIPC_CHECK(rocsolver_dpotri(h, rocblas_fill_lower, (rocblas_int)m, dense_inv_.get(), (rocblas_int)m,
+                                 dense_info_.get() + 1) == rocblas_status_success,
+                SGIPC_HIP_ERROR, "rocsolver_dpotri failed");
+    int info[2] = {0, 0};
+    SGIPC_HIP(hipMemcpyAsync(info, dense_info_.get(), sizeof(info), hipMemcpyDeviceToHost, s));
+    SGIPC_HIP(hipStreamSynchronize(s));
+    if (info[0] != 0 || info[1] != 0) return;  // not positive definite: keep the additive levels
+    k_mirror_upper<<<grid_for(m * m, 256), 256, 0, s>>>((int)m, dense_inv_.get());
+    SGIPC_LAUNCH_CHECK();
+    dense_m_ = (int)m;
+}
+
+Mas::~Mas()
+{
+    if (rb_) (void)rocblas_destroy_handle(static_cast<rocblas_handle>(rb_));
+}
+
 void Mas::init_matrix()
 {
     SGIPC_CHECK(n >= 1, SGIPC_EMPTY, "init_neighbor first");
@@ -1036,6 +1231,8 @@ void Mas::set_bcoo(const double* vals, const int* rows, const int* cols, int nnz
     SGIPC_CHECK(n_pairs == 0 || pairs != nullptr, SGIPC_BAD_ARG, "null pairs");
     auto t0 = std::chrono::steady_clock::now();
     precision = prec == SGIPC_FP64 ? SGIPC_FP64 : SGIPC_FP32_MAS;
+    graph_.n = 0;
+    dense_m_ = 0;
     build_hierarchy(pairs, n_pairs);
     if (B == 32) assemble_invert_t<32>(vals, rows, cols, nnz, offset);
     else assemble_invert_t<16>(vals, rows, cols, nnz, offset);
@@ -1063,18 +1260,78 @@ int Mas::apply_t(hipStream_t s, const TV* r, TV* z, double* rz_part, const PcgSt
     const int npad = h_off[1];
     k_restrict0<BB, TV, TS><<<grid_for(npad, 256), 256, 0, s>>>(sd, meta_.get(), n, L, map_.get() + n, r, R);
     SGIPC_LAUNCH_CHECK();
-    for (int l = 2; l < L; ++l) {
+    return finish_t<BB, TV, TS>(s, R, Z, minv, done, PcgStopDesc{}, r, z, rz_part);
+}
+
+template <int BB, typename TV, typename TS>
+int Mas::finish_t(hipStream_t s, TS* R, TS* Z, const TS* minv, const int* done, const PcgStopDesc& sd2, const TV* r,
+                  TV* z, double* rz_part)
+{
+    const int ndom0 = h_off[1] / BB;
+    const int ndom01 = L > 1 ? h_off[2] / BB : ndom0;
+    const int ndom = total / BB;
+    const bool cyc = cycle_active();
+    auto restrict_up_level = [&](int l) {
         int c = h_cnt[l - 1];
         int pad = h_off[l + 1] - h_off[l] - h_cnt[l];
         int work = c > pad ? c : pad;
-        k_restrict_up<TS><<<grid_for(work > 0 ? work : 1, 256), 256, 0, s>>>(done, meta_.get(), l, going_next_.get(), R,
-                                                                           PcgStopDesc{});
+        k_restrict_up<TS><<<grid_for(work > 0 ? work : 1, 256), 256, 0, s>>>(
+            done, meta_.get(), l, going_next_.get(), R, l == 2 ? sd2 : PcgStopDesc{});
+        SGIPC_LAUNCH_CHECK();
+    };
+    auto restrict_up = [&]() {
+        for (int l = 2; l < L; ++l) restrict_up_level(l);
+    };
+    TS* G1 = nullptr;
+    TS* C2 = nullptr;
+    if (cyc) {
+        if constexpr (sizeof(TS) == 8) {
+            G1 = g1d_.get();
+            C2 = c2d_.get();
+        } else {
+            G1 = g1f_.get();
+            C2 = c2f_.get();
+        }
+        const int c1 = h_cnt[1];
+        const int g1w = grid_for(c1, 4);  // one wave per level-1 node
+        // pre-smoothing (level-1 domain solves, with the level-0 solves)
+        k_local_solve<BB, TS><<<ndom01, 256, 0, s>>>(done, minv, R, Z, 0);
+        SGIPC_LAUNCH_CHECK();
+        k_l1_residual<TS, false><<<g1w, 256, 0, s>>>(done, meta_.get(), c1, graph_.row_ptr.get(), graph_.col.get(),
+                                                      graph_.blk.get(), graph_.s.get(), R, Z, G1, C2);
+        SGIPC_LAUNCH_CHECK();
+        // coarse correction of the level-1 residual: the exact level-2 solve
+        // (dense inverse) when it was built, else the additive levels >= 2
+        int Lc = L;
+        if (dense_m_ > 0) {
+            restrict_up_level(2);
+            k_dense_gemv<TS><<<grid_for(dense_m_, 4), 256, 0, s>>>(done, dense_m_, dense_inv_.get(),
+                                                                    R + 3L * h_off[2], Z + 3L * h_off[2]);
+            SGIPC_LAUNCH_CHECK();
+            Lc = 3;
+        } else {
+            restrict_up();
+            if (ndom > ndom01) {
+                k_local_solve<BB, TS><<<ndom - ndom01, 256, 0, s>>>(done, minv, R, Z, ndom01);
+                SGIPC_LAUNCH_CHECK();
+            }
+        }
+        k_l1_chain<TS><<<grid_for(c1, 256), 256, 0, s>>>(done, meta_.get(), Lc, going_next_.get(), c1, Z, C2);
+        SGIPC_LAUNCH_CHECK();
+        k_l1_residual<TS, true><<<g1w, 256, 0, s>>>(done, meta_.get(), c1, graph_.row_ptr.get(), graph_.col.get(),
+                                                     graph_.blk.get(), graph_.s.get(), R, Z, G1, C2);
+        SGIPC_LAUNCH_CHECK();
+        // post-smoothing
+        k_local_solve<BB, TS><<<ndom01 - ndom0, 256, 0, s>>>(done, minv, R, Z, ndom0);
+        SGIPC_LAUNCH_CHECK();
+    } else {
+        restrict_up();
+        k_local_solve<BB, TS><<<ndom, 256, 0, s>>>(done, minv, R, Z, 0);
         SGIPC_LAUNCH_CHECK();
     }
-    k_local_solve<BB, TS><<<total / BB, 256, 0, s>>>(done, minv, R, Z);
-    SGIPC_LAUNCH_CHECK();
     const int g = vec_grid(n);
-    k_prolong<TV, TS><<<g, 256, 0, s>>>(done, meta_.get(), n, L, map_.get(), Z, r, z, rz_part);
+    k_prolong<TV, TS><<<g, 256, 0, s>>>(done, meta_.get(), n, L, map_.get(), Z, r, z, rz_part, C2, nullptr,
+                                        static_cast<const TV*>(flex_Ap_), flex_zap_);
     SGIPC_LAUNCH_CHECK();
     return g;
 }
@@ -1102,20 +1359,7 @@ int Mas::apply_xr_t(hipStream_t s, const FusedXR<TV>& fx, TV* z, double* rz_part
     SGIPC_LAUNCH_CHECK();
     *n_rr = g0;
     const PcgStopDesc sd{fx.ctl, fx.rz, fx.n_rz, fx.rr, g0};
-    for (int l = 2; l < L; ++l) {
-        int c = h_cnt[l - 1];
-        int pad = h_off[l + 1] - h_off[l] - h_cnt[l];
-        int work = c > pad ? c : pad;
-        k_restrict_up<TS><<<grid_for(work > 0 ? work : 1, 256), 256, 0, s>>>(done, meta_.get(), l, going_next_.get(), R,
-                                                                           l == 2 ? sd : PcgStopDesc{});
-        SGIPC_LAUNCH_CHECK();
-    }
-    k_local_solve<BB, TS><<<total / BB, 256, 0, s>>>(done, minv, R, Z);
-    SGIPC_LAUNCH_CHECK();
-    const int g = vec_grid(n);
-    k_prolong<TV, TS><<<g, 256, 0, s>>>(done, meta_.get(), n, L, map_.get(), Z, fx.r, z, rz_part);
-    SGIPC_LAUNCH_CHECK();
-    return g;
+    return finish_t<BB, TV, TS>(s, R, Z, minv, done, sd, fx.r, z, rz_part);
 }
 
 template <typename TV>

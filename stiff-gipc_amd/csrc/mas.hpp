@@ -1,6 +1,7 @@
 // Multilevel additive Schwarz preconditioner (host object).
 #pragma once
 #include "assembly.hpp"
+#include "mpnn.hpp"
 #include "pcg.hpp"
 #include "sgipc_internal.hpp"
 
@@ -15,6 +16,9 @@ constexpr int kMetaSize = 32;
 class Mas {
   public:
     Mas(Context* ctx, int bank);
+    ~Mas();
+    Mas(const Mas&) = delete;
+    Mas& operator=(const Mas&) = delete;
     void init_neighbor(int n, int coff, int total_nbr, const uint32_t* start, const uint32_t* num,
                        const uint32_t* list);
     void init_matrix();
@@ -30,6 +34,15 @@ class Mas {
     // rr partials apply_update_xr writes
     int rr_partials() const { return (h_off[1] + 255) / 256; }
     int levels(int* offsets, int* counts) const;
+    // 0: additive levels (MASPreconditioner::preconditioning); 1: a symmetric
+    // V(1,1) cycle on level 1 (level-1 domain solves as smoother, the levels
+    // >= 2 as its coarse correction) with the level-0 domain solves additive.
+    // Takes effect at the next set_bcoo.
+    int coarse_mode = 0;
+    bool keep_graph = false;  // build the level-1 graph also without the cycle
+    // flexible PCG: z.Ap partials written by the prolongation when set
+    const void* flex_Ap_ = nullptr;
+    double* flex_zap_ = nullptr;
     void tables(int* coarse_table, int* going_next) const;
     void inverse_blocks(double* out) const;
 
@@ -52,6 +65,25 @@ class Mas {
     int apply_t(hipStream_t s, const TV* r, TV* z, double* rz_part, const PcgStopDesc& sd);
     template <int BB, typename TV, typename TS>
     int apply_xr_t(hipStream_t s, const FusedXR<TV>& fx, TV* z, double* rz_part, int* n_rr);
+    // levels >= 1 (additive, or the level-1 cycle) and the prolongation
+    template <int BB, typename TV, typename TS>
+    int finish_t(hipStream_t s, TS* R, TS* Z, const TS* minv, const int* done, const PcgStopDesc& sd2, const TV* r,
+                 TV* z, double* rz_part);
+    bool want_graph() const { return coarse_mode == 1 || keep_graph; }
+    // the cycle needs level 1 to span several domains (one domain is solved
+    // exactly already) and a coarser level
+    bool cycle_active() const { return coarse_mode == 1 && L > 2 && graph_.n == h_cnt[1] && h_cnt[1] > B; }
+
+    void build_dense_level2(hipStream_t s, const double* v, const int* r, const int* c, int nb);
+    static constexpr long kDenseMax = 4608;  // largest dense level-2 system (3 c_2)
+    DevBuf<double> dense_inv_;   // inverse of the level-2 operator (FP64, m x m)
+    DevBuf<int> dense_info_;
+    int dense_m_ = 0;
+    void* rb_ = nullptr;         // rocBLAS handle of the factorisation
+
+    CoarseGraph graph_;          // level-1 operator (coarse cycle, learned level)
+    DevBuf<double> g1d_, c2d_;   // level-1 cycle vectors (FP64 apply)
+    DevBuf<float> g1f_, c2f_;    // level-1 cycle vectors (FP32 apply)
 
     DevBuf<uint32_t> nbr_start_, nbr_num_, nbr_list_;
     DevBuf<int> map_;        // [L][n], row 0 unused (identity)
@@ -85,6 +117,12 @@ struct MasPrecond : Preconditioner<TV> {
         return mas->template apply_update_xr<TV>(s, fx, z, rz_part, n_rr);
     }
     int max_rr_partials() const override { return mas->rr_partials(); }
+    bool set_flex(const TV* Ap, double* zap) override
+    {
+        mas->flex_Ap_ = Ap;
+        mas->flex_zap_ = zap;
+        return true;
+    }
 };
 
 int mas_num_levels(int n, int bank);

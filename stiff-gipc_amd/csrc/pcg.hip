@@ -104,6 +104,22 @@ __global__ __launch_bounds__(kVecThreads) void pcg_update_p(
     }
 }
 
+// Flexible p update: beta = -(z.Ap) / (p.Ap), both from partials.
+template <typename TV>
+__global__ __launch_bounds__(kVecThreads) void pcg_update_p_flex(
+    const PcgCtl* __restrict__ ctl, const double* __restrict__ zap_part, int n_zap,
+    const double* __restrict__ pap_part, int n_pap, TV* __restrict__ p, const TV* __restrict__ z, long n3)
+{
+    __shared__ double smem[16];
+    if (ctl->done) return;
+    const double zap = sum_partials(zap_part, n_zap, smem);
+    const double pap = sum_partials(pap_part, n_pap, smem);
+    const double beta = -zap / pap;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (long)gridDim.x * blockDim.x) {
+        p[i] = (TV)((double)z[i] + beta * (double)p[i]);
+    }
+}
+
 // Identity preconditioner z = r with r.z partials and the stop test.
 template <typename TV>
 __global__ __launch_bounds__(kVecThreads) void pcg_identity_apply(
@@ -293,6 +309,7 @@ void PcgSolver<TV>::reserve(long n3, int nnz, int max_partials)
     part_pap_.reserve(gmax);
     part_rr_.reserve(gmax);
     part_bb_.reserve(gmax);
+    part_zap_.reserve(gmax);
     ctl_.reserve(1);
     if (!host_ctl_) SGIPC_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_ctl_), 2 * sizeof(PcgCtl)));
 }
@@ -351,7 +368,9 @@ PcgResult PcgSolver<TV>::solve(hipStream_t s, const SpmvOp<TV>& A, Preconditione
     //   MAS:   x += alpha p_k, r -= alpha Ap, ||r||^2 in the restriction;
     //          stop rule in the first coarse restriction; z = M r, r.z
     // i.e. the p update of iteration k-1 is deferred into the SpMV of k.
-    const bool fuse_p = opt.fuse >= 2 && A.fuses_update_p();
+    // flexible: z.Ap partials from the preconditioner's last kernel
+    const bool flex = opt.flexible && M != nullptr && M->set_flex(Ap, part_zap_.get());
+    const bool fuse_p = !flex && opt.fuse >= 2 && A.fuses_update_p();
     bool fuse_xr = opt.fuse >= 1 && M != nullptr;  // cleared when the preconditioner declines
     bool p_pending = false;                   // p update of the previous iteration not yet applied
     const int gv = vec_grid(n3);
@@ -412,7 +431,11 @@ PcgResult PcgSolver<TV>::solve(hipStream_t s, const SpmvOp<TV>& A, Preconditione
                     }
                 }
                 if (n_rzn != n_rz) throw Error(SGIPC_HIP_ERROR, "preconditioner partial count changed");
-                if (fuse_p) {
+                if (flex) {
+                    pcg_update_p_flex<TV><<<gv, kVecThreads, 0, s>>>(ctl, part_zap_.get(), n_rz, part_pap_.get(),
+                                                                    n_pap, p, z, n3);
+                    SGIPC_LAUNCH_CHECK();
+                } else if (fuse_p) {
                     p_pending = true;
                 } else {
                     pcg_update_p<TV><<<gv, kVecThreads, 0, s>>>(ctl, part_rz_[nxt].get(), part_rz_[cur].get(),
@@ -438,6 +461,7 @@ PcgResult PcgSolver<TV>::solve(hipStream_t s, const SpmvOp<TV>& A, Preconditione
             pending = slot;
         }
     }
+    if (flex) M->set_flex(nullptr, nullptr);
     SGIPC_HIP(hipStreamSynchronize(s));
     SGIPC_HIP(hipMemcpy(host_ctl_, ctl, sizeof(PcgCtl), hipMemcpyDeviceToHost));
     const PcgCtl& c = host_ctl_[0];

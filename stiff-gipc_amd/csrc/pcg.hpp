@@ -37,6 +37,9 @@ struct PcgOptions {
     // 2: also fold the p update into the SpMV (gathers z and p_old, which
     // costs more than the separate update on this part); 0: neither
     int fuse = 1;
+    // flexible PCG (preconditioner that may change between iterations):
+    // beta = -(z_new . Ap) / (p . Ap) instead of rz_new / rz_old
+    bool flexible = false;
 };
 
 // Inputs of the x/r update (x += alpha p, r -= alpha Ap, alpha = rz/pAp) that
@@ -102,6 +105,9 @@ struct Preconditioner {
     virtual int max_partials() const { return 1024; }
     // upper bound of the rr partial count apply_update_xr() writes
     virtual int max_rr_partials() const { return 0; }
+    // flexible PCG: where apply() writes the z.Ap partials (same count as
+    // the r.z partials); false when the preconditioner cannot
+    virtual bool set_flex(const TV*, double*) { return false; }
 };
 
 template <typename TV>
@@ -114,7 +120,7 @@ class PcgSolver {
 
   private:
     DevBuf<TV> x_, r_, z_, p_, p2_, Ap_;
-    DevBuf<double> part_rz_[2], part_pap_, part_rr_, part_bb_;
+    DevBuf<double> part_rz_[2], part_pap_, part_rr_, part_bb_, part_zap_;
     DevBuf<PcgCtl> ctl_;
     PcgCtl* host_ctl_ = nullptr;
     hipEvent_t ev_[2] = {nullptr, nullptr};

@@ -102,6 +102,11 @@ class MASPreconditioner:
         self.h = h
         self.vert_num = 0
 
+    def set_coarse_mode(self, mode):
+        """0: additive levels (MASPreconditioner::preconditioning); 1: a
+        symmetric V(1,1) cycle on level 1.  Applies at the next build."""
+        check(lib().sgipc_mas_set_coarse_mode(self.h, int(mode)), self.ctx.h)
+
     def initPreconditioner_Neighbor(self, vertNum, collision_node_offset, totalNeighborNum, nbr_start, nbr_num,
                                     nbr_list):
         self.vert_num = vertNum
@@ -154,7 +159,7 @@ class MASPreconditioner:
 
 
 def pcg_opts(tol=1e-5, stop_mode=STOP_TRUE_RESIDUAL, max_iter=0, precision=FP64, check_every=8, ir_max_outer=20,
-             ir_inner_tol=0.0, fuse=1):
+             ir_inner_tol=0.0, fuse=1, flexible=False):
     o = PcgOpts()
     lib().sgipc_pcg_default_opts(ctypes.byref(o))
     o.tol = tol
@@ -165,6 +170,7 @@ def pcg_opts(tol=1e-5, stop_mode=STOP_TRUE_RESIDUAL, max_iter=0, precision=FP64,
     o.ir_max_outer = ir_max_outer
     o.ir_inner_tol = ir_inner_tol
     o.fuse = int(fuse)
+    o.flexible = int(bool(flexible))
     return o
 
 
@@ -308,10 +314,15 @@ class GlobalLinearSystem:
 class LinearSystem:
     """Convenience owner of a Context + GlobalLinearSystem for one scene."""
 
-    def __init__(self, scene, bank_size=32, device=0):
+    def __init__(self, scene, bank_size=32, device=0, coarse_mode=0):
         self.ctx = Context(device)
         self.sys = GlobalLinearSystem(self.ctx, scene, bank_size)
         self.scene = scene
+        self.set_coarse_mode(coarse_mode)
+
+    def set_coarse_mode(self, mode):
+        """0: additive MAS levels (the reference); 1: V(1,1) cycle on level 1."""
+        check(lib().sgipc_mas_set_coarse_mode(self.sys.mas_handle(), int(mode)), self.ctx.h)
 
     def solve(self, tol=1e-5, precision="fp64", stop_mode=STOP_TRUE_RESIDUAL, want_grad=False, **kw):
         st, tm = self.sys.solve_linear_system(tol=tol, precision=precision, stop_mode=stop_mode, want_grad=want_grad,

@@ -1,0 +1,101 @@
+"""MAS with the level-1 V(1,1) cycle (coarse mode 1) against its numpy
+restatement (oracle/mpnn.py: mas_cycle_apply), symmetry, and solves."""
+import numpy as np
+import pytest
+
+from oracle import mpnn
+from test_gpu_parity import _gpu_mas
+
+pytestmark = pytest.mark.gpu
+
+
+def _cycle_mas(pkg, ctx, s, B, vals, rows, cols, precision):
+    """The device MAS with the cycle switched on before the build."""
+    nbr = [ctx.to_device(a, np.uint32) for a in (s.nbr_start, s.nbr_num,
+                                                 s.nbr_list if s.nbr_list.size else np.zeros(1, np.uint32))]
+    mas = pkg.MASPreconditioner(ctx, B)
+    mas.set_coarse_mode(1)
+    mas.initPreconditioner_Neighbor(s.n, 0, int(s.nbr_list.size), *nbr)
+    mas.initPreconditioner_Matrix()
+    dv = ctx.to_device(vals, np.float64)
+    dr = ctx.to_device(rows, np.int32)
+    dc = ctx.to_device(cols, np.int32)
+    pairs = ctx.to_device(s.pairs.reshape(-1), np.int32) if s.n_pairs else None
+    mas.setPreconditioner_bcoo(dv, dr, dc, rows.size, 0, pairs, s.n_pairs, precision)
+    return mas, (nbr, dv, dr, dc, pairs)
+
+
+@pytest.mark.parametrize("B", [16, 32])
+@pytest.mark.parametrize("which", ["tiny", "small", "c0"])
+def test_cycle_apply_matches_oracle(pkg, orc, gpu_ctx, B, which, scene_tiny, scene_small, scene_c0):
+    s = {"tiny": scene_tiny, "small": scene_small, "c0": scene_c0}[which]
+    vals, rows, cols = orc.assemble(s)
+    om = orc.Mas(s, B)
+    om.build(vals, rows, cols, 0, s.pairs)
+    parts = mpnn.MasParts(om)
+    if parts.L <= 2 or parts.cnt[1] <= B:
+        pytest.skip("level 1 is one domain (solved exactly): the cycle stays off")
+    mas, keep = _cycle_mas(pkg, gpu_ctx, s, B, vals, rows, cols, pkg.FP64)
+    try:
+        # the restatement with the device's own domain inverses: what remains
+        # is summation order and the float32 level-1 operator (in both)
+        parts.inv = mpnn._domain_dense(mas.inverse_blocks().reshape(-1, B * (B + 1) // 2, 9), B)
+        A = mpnn.bcoo_to_csr(vals, rows, cols, s.n)
+        graph = mpnn.coarse_graph(A, parts.map1, parts.cnt[1])
+        A1f = mpnn.float_operator(graph)
+        A2 = mpnn.level2_operator(parts, graph)
+        rng = np.random.default_rng(4)
+        for k in range(3):
+            r = rng.standard_normal(3 * s.n)
+            ref = mpnn.mas_cycle_apply(parts, A1f, r, A2=A2)
+            dr_ = gpu_ctx.to_device(r)
+            dz = gpu_ctx.empty(3 * s.n, np.float64)
+            mas.preconditioning(dr_, dz)
+            got = dz.download()
+            err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+            # float32 level-1 operator on both sides; what remains is the
+            # summation order and the Cholesky vs LU level-2 solve
+            assert err <= 1e-7, (k, err)
+    finally:
+        mas.close()
+        om.close()
+
+
+def test_cycle_operator_is_spd(pkg, orc, gpu_ctx):
+    s = pkg.scenes.tet_grid_scene("small_grid8", 8, E_loguniform=(1e4, 1e9), n_pairs=40)
+    B = 16
+    vals, rows, cols = orc.assemble(s)
+    mas, keep = _cycle_mas(pkg, gpu_ctx, s, B, vals, rows, cols, pkg.FP64)
+    try:
+        n3 = 3 * s.n
+        M = np.zeros((n3, n3))
+        dz = gpu_ctx.empty(n3, np.float64)
+        for j in range(n3):
+            e = np.zeros(n3)
+            e[j] = 1.0
+            mas.preconditioning(gpu_ctx.to_device(e), dz)
+            M[:, j] = dz.download()
+        asym = np.abs(M - M.T).max() / np.abs(M).max()
+        assert asym <= 1e-9, asym
+        assert np.linalg.eigvalsh(0.5 * (M + M.T)).min() > 0
+    finally:
+        mas.close()
+
+
+@pytest.mark.parametrize("B", [16, 32])
+def test_cycle_solves_in_fewer_iterations(pkg, B):
+    s = pkg.scenes.tet_grid_scene("cycle_grid16", 16, E_loguniform=(1e4, 1e9), n_pairs=800)
+    plain = pkg.LinearSystem(s, bank_size=B, coarse_mode=0)
+    cyc = pkg.LinearSystem(s, bank_size=B, coarse_mode=1)
+    try:
+        a = plain.solve(tol=1e-5)
+        b = cyc.solve(tol=1e-5)
+        print(f"B={B}: additive {a.iterations} iterations, cycle {b.iterations}")
+        assert a.status == 0 and b.status == 0
+        assert b.true_rel_residual <= 1e-5
+        assert b.iterations < a.iterations
+        # same solution to the solve tolerance
+        assert np.linalg.norm(a.x - b.x) <= 1e-4 * np.linalg.norm(a.x)
+    finally:
+        plain.close()
+        cyc.close()

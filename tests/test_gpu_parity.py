@@ -361,3 +361,21 @@ def test_fused_iteration_matches_unfused(pkg, scene_c0, precision, fuse):
         assert a.true_rel_residual <= 1e-6 and b.true_rel_residual <= 1e-6
     finally:
         ls.close()
+
+
+@pytest.mark.parametrize("coarse", [0, 1])
+def test_flexible_pcg_matches_pcg_for_a_fixed_preconditioner(pkg, scene_c0, coarse):
+    """With a fixed SPD preconditioner, beta = -(z.Ap)/(p.Ap) equals
+    rz_new/rz_old in exact arithmetic: the flexible iteration follows PCG."""
+    ls = pkg.LinearSystem(scene_c0, bank_size=32, coarse_mode=coarse)
+    try:
+        k = 20
+        a = ls.solve(tol=1e-30, max_iter=k, flexible=True)
+        b = ls.solve(tol=1e-30, max_iter=k, flexible=False)
+        assert a.iterations == b.iterations == k - 1
+        assert np.linalg.norm(a.x - b.x) <= 1e-8 * np.linalg.norm(b.x)
+        a = ls.solve(tol=1e-6, flexible=True)
+        b = ls.solve(tol=1e-6, flexible=False)
+        assert abs(a.iterations - b.iterations) <= 1 and a.true_rel_residual <= 1e-6
+    finally:
+        ls.close()

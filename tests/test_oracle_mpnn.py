@@ -35,20 +35,21 @@ def test_bf16_matches_torch():
     np.testing.assert_array_equal(mpnn.bf16(x), ref)
 
 
-def test_gradient_matches_finite_differences():
+@pytest.mark.parametrize("linear", [False, True])
+def test_gradient_matches_finite_differences(linear):
     A, rng = _random_spd_graph(12, 1)
     C, L = 8, 2
     p = mpnn.init_params(C, L, seed=3)
     p.w_out *= 10.0  # a correction large enough for every path to matter
     samples = [(rng.standard_normal((A.shape[0], 3)), rng.standard_normal(A.shape[0])) for _ in range(3)]
-    f0, g = mpnn.loss_grad(p, A, samples)
+    f0, g = mpnn.loss_grad(p, A, samples, linear=linear)
     v0 = p.flat()
     gflat = g.flat()
     for k in range(6):
         d = rng.standard_normal(v0.size)
         h = 1e-6
-        fp, _ = mpnn.loss_grad(mpnn.Params.from_flat(v0 + h * d, C, L), A, samples)
-        fm, _ = mpnn.loss_grad(mpnn.Params.from_flat(v0 - h * d, C, L), A, samples)
+        fp, _ = mpnn.loss_grad(mpnn.Params.from_flat(v0 + h * d, C, L), A, samples, linear=linear)
+        fm, _ = mpnn.loss_grad(mpnn.Params.from_flat(v0 - h * d, C, L), A, samples, linear=linear)
         fd = (fp - fm) / (2 * h)
         assert abs(fd - gflat @ d) <= 1e-6 * max(1.0, abs(fd)), (k, fd, gflat @ d)
 
