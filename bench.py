@@ -61,6 +61,13 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--coarse", type=int, default=0, choices=[0, 1],
                     help="MAS coarse levels: 0 additive (the reference), 1 V(1,1) cycle on level 1")
+    ap.add_argument("--learned", type=int, default=0, choices=[0, 1, 2],
+                    help="learned coarse level: 0 off, 1 always, 2 automatic (falls back to plain MAS when worse); "
+                         "the network is trained on a separately seeded scene before the timed solves")
+    ap.add_argument("--train-samples", type=int, default=600)
+    ap.add_argument("--train-epochs", type=int, default=10)
+    ap.add_argument("--compare-plain", type=int, default=3,
+                    help="steps of plain FP64 additive-MAS (B=32) solves timed in the same run for the speedup")
     ap.add_argument("--fuse", type=int, default=1, choices=[0, 1, 2],
                     help="0: separate x/r and p update kernels; 1: x/r update inside the first MAS kernel; "
                          "2: also the p update inside the SpMV")
@@ -92,6 +99,10 @@ def main():
     import ctypes
 
     out = {}
+    learned_info = None
+    net = None
+    if args.learned and not args.only_spmv:
+        learned_info, net = train_learned(pkg, ls, args)
     if not args.only_spmv:
         # ---- warm-up solves (also sizes every buffer)
         for w in range(args.warmup):
@@ -124,6 +135,7 @@ def main():
                          for k in ("ms_assemble", "ms_convert", "ms_mas_build", "ms_pcg")}
         out["nnz"] = int(phases[-1]["nnz"])
         out["n_triplets"] = int(phases[-1]["n_triplets"])
+        out["learned_used"] = bool(r.learned)
 
     # ---- SpMV kernel on the assembled matrix, HIP events on the library stream
     if args.only_spmv:
@@ -173,6 +185,24 @@ def main():
         ls.close()
         return
 
+    # ---- plain FP64 additive MAS (B=32) on the same scene, same process
+    plain = None
+    plain_cfg = (args.precision, args.bank, args.coarse, args.learned) == ("fp64", 32, 0, 0)
+    if args.compare_plain > 0 and not plain_cfg:
+        pl = pkg.LinearSystem(scene, bank_size=32, coarse_mode=0)
+        pl.solve(tol=1e-5, precision="fp64")
+        pl.ctx.synchronize()
+        t0 = time.perf_counter()
+        its = []
+        for _ in range(args.compare_plain):
+            its.append(pl.solve(tol=1e-5, precision="fp64").iterations)
+        pl.ctx.synchronize()
+        pms = (time.perf_counter() - t0) * 1e3 / args.compare_plain
+        pl.close()
+        plain = {"ms": round(pms, 3), "iterations": float(np.mean(its)), "speedup": round(pms / out["ms"], 3)}
+        log(f"plain FP64 additive MAS B=32: {pms:.2f} ms/solve, {np.mean(its):.0f} iterations -> "
+            f"speedup {pms / out['ms']:.2f}x")
+
     # ---- CPU baseline: the oracle on the host cores, bounded sample
     cpu = None
     if not args.no_cpu_baseline:
@@ -201,7 +231,11 @@ def main():
             "precision": args.precision,
             "bank_size": args.bank,
             "fused_iteration": args.fuse,
-            "mas_coarse_mode": "level-1 V(1,1) cycle" if args.coarse else "additive",
+            "mas_coarse_mode": ("level-1 V(1,1) cycle, exact dense level-2 solve" if args.coarse
+                                else "additive"),
+            "learned": learned_info,
+            "learned_used_in_timed_steps": out.get("learned_used"),
+            "plain_fp64_b32_same_run": plain,
             "n_verts": scene.n,
             "nnz_blocks": out["nnz"],
             "n_triplets": out["n_triplets"],
@@ -234,6 +268,45 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def train_learned(pkg, ls, args):
+    """Train the learned coarse level on a separately seeded 64^3 scene
+    (same generator, other seeds) from its plain solver's residuals, on the
+    GPU, then attach it to the benchmark system."""
+    t0 = time.perf_counter()
+    tr_scene = pkg.scenes.tet_grid_scene("train_grid64", args.grid, E_loguniform=(1e4, 1e9), n_pairs=args.pairs,
+                                         seed_jitter=10, seed_rhs=11, seed_E=12, seed_pairs=13)
+    tr = pkg.LinearSystem(tr_scene, bank_size=args.bank, coarse_mode=args.coarse)
+    per = max(args.train_samples // 3, 1)
+    data = tr.collect(stride=1, max_samples=per, precision=args.precision)
+    for k in range(2):
+        b = np.random.default_rng(50 + k).standard_normal(3 * tr_scene.n)
+        more = tr.collect(stride=1, max_samples=per, b=b, precision=args.precision)
+        X, q = more.samples()
+        data.add(X, q)
+        more.close()
+    t1 = time.perf_counter()
+    net = pkg.learned.Mpnn(ls.ctx, layers=3, linear=False, seed=1)
+    hist = net.train(data, epochs=args.train_epochs, batch=16, lr=1e-3, seed=0)
+    ls.ctx.synchronize()
+    t2 = time.perf_counter()
+    n_samples = data.size
+    data.close()
+    tr.close()
+    ls.set_learned(net, mode=args.learned)
+    # the automatic mode measures both variants on its first two solves
+    probe = [ls.solve(tol=1e-5, precision=args.precision) for _ in range(2)]
+    info = {
+        "mode": {1: "always", 2: "automatic"}[args.learned],
+        "layers": 3, "channels": 64, "samples": n_samples, "epochs": args.train_epochs,
+        "collect_s": round(t1 - t0, 2), "train_s": round(t2 - t1, 2),
+        "loss_first_epoch": float(hist[0]), "loss_last_epoch": float(hist[-1]),
+        "probe_iterations": {("learned" if r.learned else "plain"): r.iterations for r in probe},
+    }
+    log(f"learned level: {n_samples} samples ({t1 - t0:.1f}s), trained {args.train_epochs} epochs ({t2 - t1:.1f}s), "
+        f"loss {hist[0]:.3e} -> {hist[-1]:.3e}; probe {info['probe_iterations']}")
+    return info, net
 
 
 def cpu_baseline(ge, scene, bank, gpu_iters, cpu_iters):

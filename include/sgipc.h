@@ -281,6 +281,7 @@ typedef struct sgipc_pcg_stats {
     double rz0;
     double ms_total;         /* wall time of the call                                  */
     int status;
+    int learned;             /* 1 when the learned coarse correction was applied        */
 } sgipc_pcg_stats;
 
 void sgipc_pcg_default_opts(sgipc_pcg_opts* opts);
@@ -414,6 +415,18 @@ int sgipc_mpnn_loss_grad(sgipc_mpnn* net, const sgipc_mpnn_data* d, int first, i
  * minibatch loss of every epoch. */
 int sgipc_mpnn_train(sgipc_mpnn* net, sgipc_mpnn_data* d, int epochs, int batch, float lr,
                      unsigned seed, float* epoch_loss);
+/* Training samples from a system: one plain solve of b (learned level off)
+ * records the residual every `stride` iterations (at most max_samples); each
+ * becomes (X, q = rho) over the system's level-1 graph, normalised by the RMS
+ * of ghat.  FEM-only systems. */
+int sgipc_system_collect(sgipc_system* sys, const double* b, const sgipc_pcg_opts* opts, int stride,
+                         int max_samples, sgipc_mpnn_data** out);
+/* Attach a network to a system's MAS.  mode 0: off; 1: always (flexible PCG);
+ * 2: automatic -- a solve with and one without the correction, then the one
+ * with fewer iterations, measuring the other again every 8th solve (falls
+ * back to plain MAS whenever the learned iteration count is worse).
+ * stats.learned reports which one ran. */
+int sgipc_system_set_learned(sgipc_system* sys, sgipc_mpnn* net, int mode);
 
 #ifdef __cplusplus
 }

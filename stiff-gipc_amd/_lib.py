@@ -35,7 +35,7 @@ class PcgOpts(ctypes.Structure):
 
 class PcgStats(ctypes.Structure):
     _fields_ = [("iterations", ci), ("outer_iterations", ci), ("rel_residual", cd), ("rz", cd), ("rz0", cd),
-                ("ms_total", cd), ("status", ci)]
+                ("ms_total", cd), ("status", ci), ("learned", ci)]
 
 
 class SceneDesc(ctypes.Structure):
@@ -124,6 +124,8 @@ _SIGS = {
     "sgipc_mpnn_forward": (ci, [vp, vp, ci, ci, vp]),
     "sgipc_mpnn_loss_grad": (ci, [vp, vp, ci, ci, ctypes.POINTER(cd), vp]),
     "sgipc_mpnn_train": (ci, [vp, vp, ci, ci, ctypes.c_float, ctypes.c_uint, vp]),
+    "sgipc_system_collect": (ci, [vp, vp, ctypes.POINTER(PcgOpts), ci, ci, ctypes.POINTER(vp)]),
+    "sgipc_system_set_learned": (ci, [vp, vp, ci]),
 }
 
 _LIB = None

@@ -442,7 +442,7 @@ namespace sgipc {
 // Shared by sgipc_pcg_solve and the linear-system object.
 int pcg_solve_impl(Context& c, SolveWorkspace& ws, const double* vals, const int* rows, const int* cols,
                    int nnz, int n_rows, const SubsystemPrec& prec, const double* b, double* x,
-                   const sgipc_pcg_opts& o, sgipc_pcg_stats* stats)
+                   const sgipc_pcg_opts& o, sgipc_pcg_stats* stats, const PcgOptions* extra)
 {
     const bool has_prec = prec.mas != nullptr || (prec.abd != nullptr && prec.abd->n_bodies > 0);
     auto t0 = std::chrono::steady_clock::now();
@@ -467,6 +467,13 @@ int pcg_solve_impl(Context& c, SolveWorkspace& ws, const double* vals, const int
         po.check_every = o.check_every;
         po.fuse = o.fuse;
         po.flexible = o.flexible != 0;
+        if (extra) {
+            po.record = extra->record;
+            po.record_stride = extra->record_stride;
+            po.record_cap = extra->record_cap;
+            po.record_count = extra->record_count;
+            po.flexible = po.flexible || extra->flexible;
+        }
         PcgResult r = ws.pcg64.solve(s, A, has_prec ? &M : nullptr, b, x, n3, po);
         st.iterations = r.iterations;
         st.outer_iterations = 1;
@@ -518,8 +525,7 @@ int pcg_solve_impl(Context& c, SolveWorkspace& ws, const double* vals, const int
             po.max_iter = max_iter - total_it;
             po.check_every = o.check_every;
             po.fuse = o.fuse;
-            po.flexible = o.flexible != 0;
-        po.flexible = o.flexible != 0;
+            po.flexible = o.flexible != 0 || (extra && extra->flexible);
             PcgResult r = ws.pcg64.solve(s, A, has_prec ? &M : nullptr, ws.r64.get(), ws.d64.get(), n3, po);
             total_it += r.iterations;
             if (r.not_spd) {

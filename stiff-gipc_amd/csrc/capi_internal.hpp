@@ -105,8 +105,14 @@ struct SubsystemPrecond : Preconditioner<TV> {
     }
 };
 
+// learned-level helpers (capi_mpnn.cpp)
+sgipc_mpnn_data* mpnn_data_new(sgipc_ctx* ctx);
+CoarseGraph& mpnn_data_graph(sgipc_mpnn_data* d);
+void mpnn_data_append(sgipc_mpnn_data* d, const float* X, const float* q, int n_samples);
+Mpnn* mpnn_of(sgipc_mpnn* h);
+
 int pcg_solve_impl(Context& c, SolveWorkspace& ws, const double* vals, const int* rows, const int* cols,
                    int nnz, int n_rows, const SubsystemPrec& prec, const double* b, double* x,
-                   const sgipc_pcg_opts& o, sgipc_pcg_stats* stats);
+                   const sgipc_pcg_opts& o, sgipc_pcg_stats* stats, const PcgOptions* extra = nullptr);
 
 }  // namespace sgipc

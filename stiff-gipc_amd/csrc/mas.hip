@@ -959,6 +959,97 @@ __global__ __launch_bounds__(256) void k_l1_chain(const int* __restrict__ done, 
     C2[3L * k + 2] = c;
 }
 
+// Inputs of the learned correction per level-1 node k (one wave per node):
+// ghat = s_k g_k, zhat = zc_k / s_k and rho = ghat - Ahat zhat, where g is
+// the level-1 residual sum and zc the coarse part of the MAS output at level
+// 1 (levels >= 1 along going_next, or C2 + Z_1 with the level-1 cycle).
+template <typename TS>
+__global__ __launch_bounds__(256) void k_l1_features(const int* __restrict__ done, const int* __restrict__ meta, int L,
+                                                     const int* __restrict__ going_next, int c1,
+                                                     const int* __restrict__ rp, const int* __restrict__ colx,
+                                                     const float* __restrict__ blk, const float* __restrict__ sc,
+                                                     const TS* __restrict__ R, const TS* __restrict__ Z,
+                                                     const TS* __restrict__ G1, const TS* __restrict__ C2,
+                                                     float* __restrict__ X)
+{
+    if (done != nullptr && *done) return;
+    const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (k >= c1) return;
+    const int off1 = meta[kMetaOff + 1];
+    auto zc = [&](int j, double& a, double& b, double& c) {
+        long sl = off1 + j;
+        a = (double)Z[3 * sl];
+        b = (double)Z[3 * sl + 1];
+        c = (double)Z[3 * sl + 2];
+        if (C2 != nullptr) {
+            a += (double)C2[3L * j];
+            b += (double)C2[3L * j + 1];
+            c += (double)C2[3L * j + 2];
+        } else {
+            for (int l = 2; l < L; ++l) {
+                sl = going_next[sl];
+                a += (double)Z[3 * sl];
+                b += (double)Z[3 * sl + 1];
+                c += (double)Z[3 * sl + 2];
+            }
+        }
+    };
+    double a0 = 0, a1 = 0, a2 = 0;
+    for (int e = rp[k] + lane; e < rp[k + 1]; e += 64) {
+        const int j = colx[e];
+        const float* b = blk + 9L * e;
+        const float sj = sc[j];
+        const double wj = sj > 0.f ? 1.0 / (double)sj : 0.0;
+        double c0, c1v, c2v;
+        zc(j, c0, c1v, c2v);
+        c0 *= wj;
+        c1v *= wj;
+        c2v *= wj;
+        a0 += (double)b[0] * c0 + (double)b[1] * c1v + (double)b[2] * c2v;
+        a1 += (double)b[3] * c0 + (double)b[4] * c1v + (double)b[5] * c2v;
+        a2 += (double)b[6] * c0 + (double)b[7] * c1v + (double)b[8] * c2v;
+    }
+    a0 = wave_sum(a0);
+    a1 = wave_sum(a1);
+    a2 = wave_sum(a2);
+    if (lane != 0) return;
+    const TS* gp = G1 != nullptr ? G1 + 3L * k : R + 3L * (off1 + k);
+    const double sk = (double)sc[k];
+    const double wk = sk > 0.0 ? 1.0 / sk : 0.0;
+    double z0, z1, z2;
+    zc(k, z0, z1, z2);
+    const double g[3] = {sk * (double)gp[0], sk * (double)gp[1], sk * (double)gp[2]};
+    const double zh[3] = {wk * z0, wk * z1, wk * z2};
+    const double ah[3] = {a0, a1, a2};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        float* x = X + (3L * k + d) * 3;
+        x[0] = (float)g[d];
+        x[1] = (float)zh[d];
+        x[2] = (float)(g[d] - ah[d]);
+    }
+}
+
+// Per-sample normalisation of training features: divide X (and write q =
+// the rho column) by the RMS of the ghat column; one workgroup, fixed-order
+// sums.
+__global__ __launch_bounds__(1024) void k_sample_normalise(int rows, float* __restrict__ X, float* __restrict__ q)
+{
+    __shared__ double sm[16];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < rows; i += blockDim.x) acc += (double)X[3L * i] * (double)X[3L * i];
+    const double ss = block_sum(acc, sm);
+    const double rms = sqrt(ss / (double)(rows > 0 ? rows : 1));
+    const float inv = rms > 0.0 ? (float)(1.0 / rms) : 1.f;
+    for (int i = threadIdx.x; i < rows; i += blockDim.x) {
+        X[3L * i] *= inv;
+        X[3L * i + 1] *= inv;
+        X[3L * i + 2] *= inv;
+        q[i] = X[3L * i + 2];
+    }
+}
+
 // y = Minv x for the dense FP64 inverse of the level-2 operator (m x m,
 // row-major, symmetric); one wave per row.
 template <typename TS>
@@ -1217,6 +1308,29 @@ Mas::~Mas()
     if (rb_) (void)rocblas_destroy_handle(static_cast<rocblas_handle>(rb_));
 }
 
+void Mas::learned_sample(hipStream_t s, const double* r, double* ztmp, float* X, float* q)
+{
+    SGIPC_CHECK(built && graph_.n == h_cnt[1] && h_cnt[1] > 0, SGIPC_BAD_ARG, "MAS has no level-1 graph");
+    Mpnn* keep = net_;
+    net_ = nullptr;  // the plain preconditioner
+    apply<double>(s, r, ztmp, nullptr, PcgStopDesc{});
+    net_ = keep;
+    const int c1 = h_cnt[1];
+    const bool cyc = cycle_active();
+    if (precision == SGIPC_FP64) {
+        k_l1_features<double><<<grid_for(c1, 4), 256, 0, s>>>(
+            nullptr, meta_.get(), L, going_next_.get(), c1, graph_.row_ptr.get(), graph_.col.get(), graph_.blk.get(),
+            graph_.s.get(), R64_.get(), Z64_.get(), cyc ? g1d_.get() : nullptr, cyc ? c2d_.get() : nullptr, X);
+    } else {
+        k_l1_features<float><<<grid_for(c1, 4), 256, 0, s>>>(
+            nullptr, meta_.get(), L, going_next_.get(), c1, graph_.row_ptr.get(), graph_.col.get(), graph_.blk.get(),
+            graph_.s.get(), R32_.get(), Z32_.get(), cyc ? g1f_.get() : nullptr, cyc ? c2f_.get() : nullptr, X);
+    }
+    SGIPC_LAUNCH_CHECK();
+    k_sample_normalise<<<1, 1024, 0, s>>>(3 * c1, X, q);
+    SGIPC_LAUNCH_CHECK();
+}
+
 void Mas::init_matrix()
 {
     SGIPC_CHECK(n >= 1, SGIPC_EMPTY, "init_neighbor first");
@@ -1329,8 +1443,27 @@ int Mas::finish_t(hipStream_t s, TS* R, TS* Z, const TS* minv, const int* done, 
         k_local_solve<BB, TS><<<ndom, 256, 0, s>>>(done, minv, R, Z, 0);
         SGIPC_LAUNCH_CHECK();
     }
+    const float* corr = nullptr;
+    if (learned_active()) {
+        const int c1 = h_cnt[1];
+        lX_.reserve(9L * c1);
+        lchat_.reserve(3L * c1);
+        lcorr_.reserve(3L * c1);
+        if (cyc) {
+            k_l1_features<TS><<<grid_for(c1, 4), 256, 0, s>>>(done, meta_.get(), L, going_next_.get(), c1,
+                                                               graph_.row_ptr.get(), graph_.col.get(), graph_.blk.get(),
+                                                               graph_.s.get(), R, Z, G1, C2, lX_.get());
+        } else {
+            k_l1_features<TS><<<grid_for(c1, 4), 256, 0, s>>>(done, meta_.get(), L, going_next_.get(), c1,
+                                                               graph_.row_ptr.get(), graph_.col.get(), graph_.blk.get(),
+                                                               graph_.s.get(), R, Z, nullptr, nullptr, lX_.get());
+        }
+        SGIPC_LAUNCH_CHECK();
+        net_->forward(s, graph_, lX_.get(), 1, lchat_.get(), lcorr_.get(), done);
+        corr = lcorr_.get();
+    }
     const int g = vec_grid(n);
-    k_prolong<TV, TS><<<g, 256, 0, s>>>(done, meta_.get(), n, L, map_.get(), Z, r, z, rz_part, C2, nullptr,
+    k_prolong<TV, TS><<<g, 256, 0, s>>>(done, meta_.get(), n, L, map_.get(), Z, r, z, rz_part, C2, corr,
                                         static_cast<const TV*>(flex_Ap_), flex_zap_);
     SGIPC_LAUNCH_CHECK();
     return g;

@@ -43,6 +43,13 @@ class Mas {
     // flexible PCG: z.Ap partials written by the prolongation when set
     const void* flex_Ap_ = nullptr;
     double* flex_zap_ = nullptr;
+    // learned coarse correction (nullptr: off); needs the level-1 graph
+    Mpnn* net_ = nullptr;
+    bool learned_active() const { return net_ != nullptr && graph_.n == h_cnt[1] && h_cnt[1] > 0 && L > 1; }
+    // training sample of residual r: features X (3 c_1 rows x 3) and target q
+    // (3 c_1), normalised by the RMS of the first feature; ztmp: 3n scratch
+    void learned_sample(hipStream_t s, const double* r, double* ztmp, float* X, float* q);
+    const CoarseGraph& level1_graph() const { return graph_; }
     void tables(int* coarse_table, int* going_next) const;
     void inverse_blocks(double* out) const;
 
@@ -84,6 +91,7 @@ class Mas {
     CoarseGraph graph_;          // level-1 operator (coarse cycle, learned level)
     DevBuf<double> g1d_, c2d_;   // level-1 cycle vectors (FP64 apply)
     DevBuf<float> g1f_, c2f_;    // level-1 cycle vectors (FP32 apply)
+    DevBuf<float> lX_, lchat_, lcorr_;  // learned correction: features, output, scaled output
 
     DevBuf<uint32_t> nbr_start_, nbr_num_, nbr_list_;
     DevBuf<int> map_;        // [L][n], row 0 unused (identity)

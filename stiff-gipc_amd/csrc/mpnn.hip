@@ -132,9 +132,11 @@ __global__ __launch_bounds__(256) void k_layer(int n, const int* __restrict__ rp
                                                const uint16_t* __restrict__ Wt, float* __restrict__ Vout,
                                                const float* __restrict__ w_out, float* __restrict__ chat,
                                                const float* __restrict__ s_node, float* __restrict__ scaled,
-                                               uint16_t* __restrict__ Uout, float* __restrict__ Yout, int linear)
+                                               uint16_t* __restrict__ Uout, float* __restrict__ Yout, int linear,
+                                               const int* __restrict__ done)
 {
     __shared__ __align__(16) uint16_t sU[kRows * kUld];
+    if (done != nullptr && *done) return;
     __shared__ __align__(16) uint16_t sW[C * kWld];
     __shared__ float sV[kRows * C];
     __shared__ float sPart[4 * kRows];
@@ -675,7 +677,8 @@ void Mpnn::reserve_train(long rows, int tiles_total)
     rows_cap_ = rows;
 }
 
-void Mpnn::forward(hipStream_t st, const CoarseGraph& g, const float* X, int S, float* chat, float* scaled)
+void Mpnn::forward(hipStream_t st, const CoarseGraph& g, const float* X, int S, float* chat, float* scaled,
+                   const int* done)
 {
     SGIPC_CHECK(g.n >= 1 && S >= 1, SGIPC_EMPTY, "empty graph or batch");
     SGIPC_CHECK(scaled == nullptr || S == 1, SGIPC_BAD_ARG, "scaled output needs a single sample");
@@ -694,7 +697,7 @@ void Mpnn::forward(hipStream_t st, const CoarseGraph& g, const float* X, int S, 
 #define SGIPC_LAYER(F, LA)                                                                                     \
     k_layer<F, LA, false><<<grid, 256, 0, st>>>(g.n, g.row_ptr.get(), g.col.get(), g.blk.get(), vin, X, w_in, wt, \
                                                 LA ? nullptr : vout, w_out, chat, g.s.get(), scaled, nullptr, nullptr, \
-                                                linear_)
+                                                linear_, done)
         if (first && last) SGIPC_LAYER(true, true);
         else if (first) SGIPC_LAYER(true, false);
         else if (last) SGIPC_LAYER(false, true);
@@ -723,7 +726,7 @@ double Mpnn::loss_grad(hipStream_t st, const CoarseGraph& g, const float* X, con
 #define SGIPC_LAYER(F, LA)                                                                                   \
     k_layer<F, LA, true><<<grid, 256, 0, st>>>(n, g.row_ptr.get(), g.col.get(), g.blk.get(), V_[l].get(), X, \
                                                w_in, wt, V_[l + 1].get(), w_out, chat_.get(), g.s.get(),    \
-                                               nullptr, U_[l].get(), Y_[l].get(), linear_)
+                                               nullptr, U_[l].get(), Y_[l].get(), linear_, nullptr)
         if (first && last) SGIPC_LAYER(true, true);
         else if (first) SGIPC_LAYER(true, false);
         else if (last) SGIPC_LAYER(false, true);

@@ -61,7 +61,7 @@ class Mpnn {
     // chat for n_samples stacked samples (X: [S][3n][3] float, chat: [S][3n]);
     // with scaled != nullptr (S == 1) also writes s .* chat there.
     void forward(hipStream_t st, const CoarseGraph& g, const float* X, int n_samples, float* chat,
-                 float* scaled);
+                 float* scaled, const int* done = nullptr);
     // mean loss over the samples and its gradient (device, n_params floats)
     double loss_grad(hipStream_t st, const CoarseGraph& g, const float* X, const float* q, int n_samples,
                      float* grad);

@@ -431,6 +431,14 @@ PcgResult PcgSolver<TV>::solve(hipStream_t s, const SpmvOp<TV>& A, Preconditione
                     }
                 }
                 if (n_rzn != n_rz) throw Error(SGIPC_HIP_ERROR, "preconditioner partial count changed");
+                if constexpr (sizeof(TV) == 8) {
+                    if (opt.record && opt.record_count && *opt.record_count < opt.record_cap &&
+                        (issued + it) % (opt.record_stride > 0 ? opt.record_stride : 1) == 0) {
+                        SGIPC_HIP(hipMemcpyAsync(opt.record + (size_t)(*opt.record_count) * n3, r, sizeof(TV) * n3,
+                                                 hipMemcpyDeviceToDevice, s));
+                        ++*opt.record_count;
+                    }
+                }
                 if (flex) {
                     pcg_update_p_flex<TV><<<gv, kVecThreads, 0, s>>>(ctl, part_zap_.get(), n_rz, part_pap_.get(),
                                                                     n_pap, p, z, n3);

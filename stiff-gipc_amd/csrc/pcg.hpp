@@ -40,6 +40,12 @@ struct PcgOptions {
     // flexible PCG (preconditioner that may change between iterations):
     // beta = -(z_new . Ap) / (p . Ap) instead of rz_new / rz_old
     bool flexible = false;
+    // copies of the residual after every record_stride-th iteration (FP64
+    // solver only), for training the learned coarse level
+    double* record = nullptr;
+    int record_stride = 1;
+    int record_cap = 0;
+    int* record_count = nullptr;  // host
 };
 
 // Inputs of the x/r update (x += alpha p, r -= alpha Ap, alpha = rz/pAp) that

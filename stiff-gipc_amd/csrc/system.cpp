@@ -55,6 +55,12 @@ struct sgipc_system {
     AbdJacobi abdj;
     SolveWorkspace* ws = nullptr;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    // learned coarse level: network, mode (0 off, 1 on, 2 automatic), and
+    // the last iteration counts with and without it (automatic mode)
+    sgipc_mpnn* net = nullptr;
+    int learned_mode = 0;
+    int plain_it = -1, learned_it = -1;
+    long n_solves = 0;
 };
 
 extern "C" {
@@ -305,8 +311,25 @@ int sgipc_system_destroy(sgipc_system* sys)
     return st;
 }
 
-int sgipc_system_solve(sgipc_system* sys, const double* b, double* x, double* grad, const sgipc_pcg_opts* opts,
-                       sgipc_pcg_stats* stats, sgipc_solve_timing* timing)
+}  // extern "C"
+
+namespace {
+
+// Whether this solve uses the learned correction.  Automatic mode measures
+// both variants once, then keeps the one with fewer iterations and measures
+// the other again every 8th solve (the matrix drifts over Newton steps).
+bool use_learned(const sgipc_system* sys)
+{
+    if (!sys->net || sys->learned_mode == 0) return false;
+    if (sys->learned_mode == 1) return true;
+    if (sys->plain_it < 0) return false;
+    if (sys->learned_it < 0) return true;
+    const bool better = sys->learned_it < sys->plain_it;
+    return (sys->n_solves % 8 == 7) ? !better : better;
+}
+
+int system_solve(sgipc_system* sys, const double* b, double* x, double* grad, const sgipc_pcg_opts* opts,
+                 sgipc_pcg_stats* stats, sgipc_solve_timing* timing, const PcgOptions* record)
 {
     if (!sys || !b || !x) return SGIPC_BAD_ARG;
     sgipc_pcg_opts o;
@@ -372,6 +395,8 @@ int sgipc_system_solve(sgipc_system* sys, const double* b, double* x, double* gr
         SGIPC_HIP(hipEventRecord(sys->ev[2], s));
         // ---- preconditioners: MAS on the FEM rows (current contacts), block-Jacobi on the bodies
         const int mprec = (o.precision == SGIPC_FP64) ? SGIPC_FP64 : SGIPC_FP32_MAS;
+        const bool learned = record == nullptr && use_learned(sys);
+        sys->mas->m->keep_graph = sys->net != nullptr || record != nullptr;
         sys->mas->m->set_bcoo(sys->bvals.get(), sys->brows.get(), sys->bcols.get(), sys->nnz, off0, d.pairs,
                               d.n_pairs, mprec);
         if (nb > 0) sys->abdj.build(s, sys->bvals.get(), sys->brows.get(), sys->bcols.get(), sys->nnz, nb);
@@ -382,8 +407,26 @@ int sgipc_system_solve(sgipc_system* sys, const double* b, double* x, double* gr
         prec.mas = sys->mas->m.get();
         prec.mas_row_offset = off0;
         prec.abd = nb > 0 ? &sys->abdj : nullptr;
-        status = pcg_solve_impl(sys->ctx->c, *sys->ws, sys->bvals.get(), sys->brows.get(), sys->bcols.get(), sys->nnz,
-                                sys->n_sys, prec, b, x, o, &pst);
+        PcgOptions extra;
+        if (record) extra = *record;
+        extra.flexible = learned;
+        sys->mas->m->net_ = learned ? mpnn_of(sys->net) : nullptr;
+        if (learned && !sys->mas->m->learned_active()) sys->mas->m->net_ = nullptr;
+        const bool used = sys->mas->m->net_ != nullptr;
+        extra.flexible = used;
+        try {
+            status = pcg_solve_impl(sys->ctx->c, *sys->ws, sys->bvals.get(), sys->brows.get(), sys->bcols.get(),
+                                    sys->nnz, sys->n_sys, prec, b, x, o, &pst, &extra);
+        } catch (...) {
+            sys->mas->m->net_ = nullptr;
+            throw;
+        }
+        sys->mas->m->net_ = nullptr;
+        pst.learned = used ? 1 : 0;
+        if (record == nullptr && status == SGIPC_OK) {
+            (used ? sys->learned_it : sys->plain_it) = pst.iterations;
+            ++sys->n_solves;
+        }
         SGIPC_HIP(hipEventRecord(sys->ev[4], s));
         SGIPC_HIP(hipEventSynchronize(sys->ev[4]));
         auto t1 = std::chrono::steady_clock::now();
@@ -404,6 +447,72 @@ int sgipc_system_solve(sgipc_system* sys, const double* b, double* x, double* gr
         }
     });
     return st != SGIPC_OK ? st : status;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sgipc_system_solve(sgipc_system* sys, const double* b, double* x, double* grad, const sgipc_pcg_opts* opts,
+                       sgipc_pcg_stats* stats, sgipc_solve_timing* timing)
+{
+    return system_solve(sys, b, x, grad, opts, stats, timing, nullptr);
+}
+
+int sgipc_system_set_learned(sgipc_system* sys, sgipc_mpnn* net, int mode)
+{
+    if (!sys || mode < 0 || mode > 2 || (mode != 0 && !net)) return SGIPC_BAD_ARG;
+    sys->net = mode ? net : nullptr;
+    sys->learned_mode = mode;
+    sys->plain_it = sys->learned_it = -1;
+    sys->n_solves = 0;
+    return SGIPC_OK;
+}
+
+int sgipc_system_collect(sgipc_system* sys, const double* b, const sgipc_pcg_opts* opts, int stride,
+                         int max_samples, sgipc_mpnn_data** out)
+{
+    if (!sys || !b || !out || stride < 1 || max_samples < 1) return SGIPC_BAD_ARG;
+    *out = nullptr;
+    const long n3 = 3L * sys->n_sys;
+    DevBuf<double> rec, xtmp, ztmp;
+    int count = 0;
+    int st = run(sys->ctx, [&] {
+        rec.reserve((size_t)max_samples * n3);
+        xtmp.reserve(n3);
+        ztmp.reserve(n3);
+    });
+    if (st != SGIPC_OK) return st;
+    PcgOptions r;
+    r.record = rec.get();
+    r.record_stride = stride;
+    r.record_cap = max_samples;
+    r.record_count = &count;
+    st = system_solve(sys, b, xtmp.get(), nullptr, opts, nullptr, nullptr, &r);
+    if (st != SGIPC_OK && st != SGIPC_MAXITER) return st;
+    sgipc_mpnn_data* d = mpnn_data_new(sys->ctx);
+    st = run(sys->ctx, [&] {
+        hipStream_t s = sys->ctx->c.stream;
+        Mas& m = *sys->mas->m;
+        SGIPC_CHECK(m.level1_graph().n > 0, SGIPC_EMPTY, "the MAS has no level-1 graph (one level)");
+        SGIPC_CHECK(sys->d.n_bodies == 0, SGIPC_BAD_ARG, "training samples need a FEM-only system");
+        mpnn_data_graph(d).copy_from(s, m.level1_graph());
+        const long rows = 3L * m.level1_graph().n;
+        DevBuf<float> X, q;
+        X.reserve((size_t)rows * 3);
+        q.reserve(rows);
+        for (int k = 0; k < count; ++k) {
+            m.learned_sample(s, rec.get() + (size_t)k * n3, ztmp.get(), X.get(), q.get());
+            mpnn_data_append(d, X.get(), q.get(), 1);
+        }
+        SGIPC_HIP(hipStreamSynchronize(s));
+    });
+    if (st != SGIPC_OK) {
+        (void)sgipc_mpnn_data_destroy(d);
+        return st;
+    }
+    *out = d;
+    return SGIPC_OK;
 }
 
 int sgipc_system_bcoo(sgipc_system* sys, const double** vals, const int** rows, const int** cols, int* nnz)

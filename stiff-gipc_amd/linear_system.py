@@ -197,6 +197,7 @@ class SolveResult:
     true_rel_residual: float
     status: int
     timing: dict
+    learned: bool = False
 
 
 def upload_scene(ctx: Context, scene):
@@ -330,7 +331,23 @@ class LinearSystem:
         x = self.sys.x.download()
         timing = {k: getattr(tm, k) for k, _ in SolveTiming._fields_}
         return SolveResult(x=x, iterations=st.iterations, outer_iterations=st.outer_iterations,
-                           true_rel_residual=st.rel_residual, status=st.status, timing=timing)
+                           true_rel_residual=st.rel_residual, status=st.status, timing=timing,
+                           learned=bool(st.learned))
+
+    def collect(self, stride=1, max_samples=256, tol=1e-5, precision="fp64", b=None):
+        """Training samples (MpnnData) from one plain solve of the scene's rhs
+        (or of b, host array): residuals every `stride` iterations."""
+        from .learned import MpnnData
+        o = pcg_opts(tol=tol, precision=precision)
+        rhs = self.sys.b if b is None else self.ctx.to_device(np.ascontiguousarray(b, np.float64))
+        h = ctypes.c_void_p()
+        check(lib().sgipc_system_collect(self.sys.h, rhs.vp, ctypes.byref(o), int(stride), int(max_samples),
+                                         ctypes.byref(h)), self.ctx.h)
+        return MpnnData(self.ctx, h)
+
+    def set_learned(self, net, mode=2):
+        """Attach a learned coarse level: mode 0 off, 1 always, 2 automatic."""
+        check(lib().sgipc_system_set_learned(self.sys.h, net.h if net is not None else None, int(mode)), self.ctx.h)
 
     def gradient(self):
         return self.sys.grad.download()
