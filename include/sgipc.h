@@ -422,10 +422,11 @@ int sgipc_mpnn_train(sgipc_mpnn* net, sgipc_mpnn_data* d, int epochs, int batch,
 int sgipc_system_collect(sgipc_system* sys, const double* b, const sgipc_pcg_opts* opts, int stride,
                          int max_samples, sgipc_mpnn_data** out);
 /* Attach a network to a system's MAS.  mode 0: off; 1: always (flexible PCG);
- * 2: automatic -- a solve with and one without the correction, then the one
- * with fewer iterations, measuring the other again every 8th solve (falls
- * back to plain MAS whenever the learned iteration count is worse).
- * stats.learned reports which one ran. */
+ * 2: automatic -- a solve without and one with the correction, then the one
+ * with fewer iterations (falls back to plain MAS whenever the learned
+ * iteration count is worse); the other is measured again 8 solves later, the
+ * gap doubling each time the learned level loses.  stats.learned reports
+ * which one ran. */
 int sgipc_system_set_learned(sgipc_system* sys, sgipc_mpnn* net, int mode);
 
 #ifdef __cplusplus

@@ -125,9 +125,82 @@ __global__ void k_g_blocks(const uint64_t* __restrict__ sk, const int* __restric
 }
 
 // ------------------------------------------------------------------ layers
+// Y = Ahat V on 64 channels: one wave per node (lanes = channels), so a
+// launch keeps thousands of independent neighbour walks in flight.
+__global__ __launch_bounds__(256) void k_spmm64(int n, const int* __restrict__ rp, const int* __restrict__ colx,
+                                                const float* __restrict__ blk, const float* __restrict__ V,
+                                                float* __restrict__ Y)
+{
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int ch = threadIdx.x & 63;
+    if (i >= n) return;
+    const long sbase = (long)blockIdx.y * 3 * n;
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+    const int e1 = rp[i + 1];
+    int e = rp[i];
+    for (; e + 1 < e1; e += 2) {  // two neighbours per step: two load chains in flight
+        const int j0 = colx[e], j1 = colx[e + 1];
+        const float* b0 = blk + 9L * e;
+        const float* b1 = b0 + 9;
+        const float* v0 = V + (sbase + 3L * j0) * C + ch;
+        const float* v1 = V + (sbase + 3L * j1) * C + ch;
+        const float x0 = v0[0], x1 = v0[C], x2 = v0[2 * C];
+        const float y0 = v1[0], y1 = v1[C], y2 = v1[2 * C];
+        a0 += b0[0] * x0 + b0[1] * x1 + b0[2] * x2 + b1[0] * y0 + b1[1] * y1 + b1[2] * y2;
+        a1 += b0[3] * x0 + b0[4] * x1 + b0[5] * x2 + b1[3] * y0 + b1[4] * y1 + b1[5] * y2;
+        a2 += b0[6] * x0 + b0[7] * x1 + b0[8] * x2 + b1[6] * y0 + b1[7] * y1 + b1[8] * y2;
+    }
+    if (e < e1) {
+        const int j = colx[e];
+        const float* b = blk + 9L * e;
+        const float* vj = V + (sbase + 3L * j) * C + ch;
+        const float x0 = vj[0], x1 = vj[C], x2 = vj[2 * C];
+        a0 += b[0] * x0 + b[1] * x1 + b[2] * x2;
+        a1 += b[3] * x0 + b[4] * x1 + b[5] * x2;
+        a2 += b[6] * x0 + b[7] * x1 + b[8] * x2;
+    }
+    float* y = Y + (sbase + 3L * i) * C + ch;
+    y[0] = a0;
+    y[C] = a1;
+    y[2 * C] = a2;
+}
+
+// Y = Ahat X on the 3 input features: one wave per node, lanes over the
+// neighbours, fixed-order wave reduction.
+__global__ __launch_bounds__(256) void k_spmm3(int n, const int* __restrict__ rp, const int* __restrict__ colx,
+                                               const float* __restrict__ blk, const float* __restrict__ X,
+                                               float* __restrict__ Y, const int* __restrict__ done)
+{
+    if (done != nullptr && *done) return;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= n) return;
+    const long sbase = (long)blockIdx.y * 3 * n;
+    float a[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int e = rp[i] + lane; e < rp[i + 1]; e += 64) {
+        const int j = colx[e];
+        const float* b = blk + 9L * e;
+        const float* xj = X + (sbase + 3L * j) * 3;
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int f = 0; f < 3; ++f)
+#pragma unroll
+                for (int g = 0; g < 3; ++g) a[3 * d + g] += b[3 * d + f] * xj[3 * f + g];
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) a[k] = wave_sum(a[k]);
+    if (lane == 0) {
+        float* y = Y + (sbase + 3L * i) * 3;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) y[k] = a[k];
+    }
+}
+
+// (Ahat V) of the layer's input comes precomputed in AV (k_spmm64, or
+// k_spmm3 on X for the first layer): the kernel itself reads rows only.
 template <bool FIRST, bool LAST, bool TRAIN>
-__global__ __launch_bounds__(256) void k_layer(int n, const int* __restrict__ rp, const int* __restrict__ colx,
-                                               const float* __restrict__ blk, const float* __restrict__ Vin,
+__global__ __launch_bounds__(256) void k_layer(int n, const float* __restrict__ AV, const float* __restrict__ Vin,
                                                const float* __restrict__ X, const float* __restrict__ W_in,
                                                const uint16_t* __restrict__ Wt, float* __restrict__ Vout,
                                                const float* __restrict__ w_out, float* __restrict__ chat,
@@ -161,37 +234,18 @@ __global__ __launch_bounds__(256) void k_layer(int n, const int* __restrict__ rp
         const int i = node0 + li;
         float v[3] = {0.f, 0.f, 0.f}, a[3] = {0.f, 0.f, 0.f};
         if (i < n) {
-            if (FIRST) {
-                float ax[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-                for (int e = rp[i]; e < rp[i + 1]; ++e) {
-                    const int j = colx[e];
-                    const float* b = blk + 9L * e;
-                    const float* xj = X + (sbase + 3L * j) * 3;
 #pragma unroll
-                    for (int d = 0; d < 3; ++d)
-#pragma unroll
-                        for (int f = 0; f < 3; ++f)
-#pragma unroll
-                            for (int g = 0; g < 3; ++g) ax[3 * d + g] += b[3 * d + f] * xj[3 * f + g];
-                }
-#pragma unroll
-                for (int d = 0; d < 3; ++d) {
-                    const float* xi = X + (sbase + 3L * i + d) * 3;
+            for (int d = 0; d < 3; ++d) {
+                const long row = sbase + 3L * i + d;
+                if (FIRST) {
+                    const float* xi = X + row * 3;
+                    const float* ai = AV + row * 3;
                     v[d] = xi[0] * wi0 + xi[1] * wi1 + xi[2] * wi2;
-                    a[d] = ax[3 * d] * wi0 + ax[3 * d + 1] * wi1 + ax[3 * d + 2] * wi2;
+                    a[d] = ai[0] * wi0 + ai[1] * wi1 + ai[2] * wi2;
+                } else {
+                    v[d] = Vin[row * C + ch];
+                    a[d] = AV[row * C + ch];
                 }
-            } else {
-                for (int e = rp[i]; e < rp[i + 1]; ++e) {
-                    const int j = colx[e];
-                    const float* b = blk + 9L * e;
-                    const float* vj = Vin + (sbase + 3L * j) * C + ch;
-                    const float x0 = vj[0], x1 = vj[C], x2 = vj[2 * C];
-                    a[0] += b[0] * x0 + b[1] * x1 + b[2] * x2;
-                    a[1] += b[3] * x0 + b[4] * x1 + b[5] * x2;
-                    a[2] += b[6] * x0 + b[7] * x1 + b[8] * x2;
-                }
-#pragma unroll
-                for (int d = 0; d < 3; ++d) v[d] = Vin[(sbase + 3L * i + d) * C + ch];
             }
         }
 #pragma unroll
@@ -373,34 +427,29 @@ __global__ __launch_bounds__(256) void k_bwd_dydu(int n, const float* __restrict
     }
 }
 
-// G_new = G + dU[:, :C] + Ahat dU[:, C:] (one lane per channel, 8 nodes per wave).
+// G_new = G + dU[:, :C] + Ahat dU[:, C:] (one wave per node, lanes = channels).
 __global__ __launch_bounds__(256) void k_bwd_gather(int n, const int* __restrict__ rp, const int* __restrict__ colx,
                                                     const float* __restrict__ blk, const float* __restrict__ G,
                                                     const float* __restrict__ dU, float* __restrict__ Gn)
 {
-    const int t = threadIdx.x;
-    const int ch = t & 63;
-    const int wv = t >> 6;
-    const int node0 = blockIdx.x * kMpnnTile;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int ch = threadIdx.x & 63;
+    if (i >= n) return;
     const long sbase = (long)blockIdx.y * 3 * n;
-    for (int m = 0; m < kMpnnTile / 4; ++m) {
-        const int i = node0 + wv + 4 * m;
-        if (i >= n) break;
-        float a[3] = {0.f, 0.f, 0.f};
-        for (int e = rp[i]; e < rp[i + 1]; ++e) {
-            const int j = colx[e];
-            const float* b = blk + 9L * e;
-            const float* uj = dU + (sbase + 3L * j) * 2 * C + C + ch;
-            const float x0 = uj[0], x1 = uj[2 * C], x2 = uj[4 * C];
-            a[0] += b[0] * x0 + b[1] * x1 + b[2] * x2;
-            a[1] += b[3] * x0 + b[4] * x1 + b[5] * x2;
-            a[2] += b[6] * x0 + b[7] * x1 + b[8] * x2;
-        }
+    float a[3] = {0.f, 0.f, 0.f};
+    for (int e = rp[i]; e < rp[i + 1]; ++e) {
+        const int j = colx[e];
+        const float* b = blk + 9L * e;
+        const float* uj = dU + (sbase + 3L * j) * 2 * C + C + ch;
+        const float x0 = uj[0], x1 = uj[2 * C], x2 = uj[4 * C];
+        a[0] += b[0] * x0 + b[1] * x1 + b[2] * x2;
+        a[1] += b[3] * x0 + b[4] * x1 + b[5] * x2;
+        a[2] += b[6] * x0 + b[7] * x1 + b[8] * x2;
+    }
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            const long row = sbase + 3L * i + d;
-            Gn[row * C + ch] = G[row * C + ch] + dU[row * 2 * C + ch] + a[d];
-        }
+    for (int d = 0; d < 3; ++d) {
+        const long row = sbase + 3L * i + d;
+        Gn[row * C + ch] = G[row * C + ch] + dU[row * 2 * C + ch] + a[d];
     }
 }
 
@@ -686,7 +735,9 @@ void Mpnn::forward(hipStream_t st, const CoarseGraph& g, const float* X, int S, 
     const long rows = 3L * g.n * S;
     V_[0].reserve((size_t)rows * C);
     V_[1].reserve((size_t)rows * C);
+    AV_.reserve((size_t)rows * C);
     const dim3 grid((g.n + kMpnnTile - 1) / kMpnnTile, S);
+    const dim3 gw((g.n + 3) / 4, S);  // one wave per node
     const float* w_in = w_.get();
     const float* w_out = w_.get() + 3 * C + (long)L * 2 * C * C;
     const float* vin = nullptr;
@@ -694,10 +745,12 @@ void Mpnn::forward(hipStream_t st, const CoarseGraph& g, const float* X, int S, 
         const uint16_t* wt = wt_.get() + (long)l * 2 * C * C;
         float* vout = V_[(l + 1) & 1].get();
         const bool first = l == 0, last = l == L - 1;
+        if (first) k_spmm3<<<gw, 256, 0, st>>>(g.n, g.row_ptr.get(), g.col.get(), g.blk.get(), X, AV_.get(), done);
+        else k_spmm64<<<gw, 256, 0, st>>>(g.n, g.row_ptr.get(), g.col.get(), g.blk.get(), vin, AV_.get());
+        SGIPC_LAUNCH_CHECK();
 #define SGIPC_LAYER(F, LA)                                                                                     \
-    k_layer<F, LA, false><<<grid, 256, 0, st>>>(g.n, g.row_ptr.get(), g.col.get(), g.blk.get(), vin, X, w_in, wt, \
-                                                LA ? nullptr : vout, w_out, chat, g.s.get(), scaled, nullptr, nullptr, \
-                                                linear_, done)
+    k_layer<F, LA, false><<<grid, 256, 0, st>>>(g.n, AV_.get(), vin, X, w_in, wt, LA ? nullptr : vout, w_out, chat, \
+                                                g.s.get(), scaled, nullptr, nullptr, linear_, done)
         if (first && last) SGIPC_LAYER(true, true);
         else if (first) SGIPC_LAYER(true, false);
         else if (last) SGIPC_LAYER(false, true);
@@ -717,16 +770,21 @@ double Mpnn::loss_grad(hipStream_t st, const CoarseGraph& g, const float* X, con
     const int tiles = (n + kMpnnTile - 1) / kMpnnTile;
     reserve_train(rows, tiles * S);
     const dim3 grid(tiles, S);
+    const dim3 gw((n + 3) / 4, S);  // one wave per node
     const float* w_in = w_.get();
     const float* w_out = w_.get() + 3 * C + (long)L * 2 * C * C;
+    AV_.reserve((size_t)rows * C);
     // forward, keeping U, Y and every V
     for (int l = 0; l < L; ++l) {
         const uint16_t* wt = wt_.get() + (long)l * 2 * C * C;
         const bool first = l == 0, last = l == L - 1;
+        if (first) k_spmm3<<<gw, 256, 0, st>>>(n, g.row_ptr.get(), g.col.get(), g.blk.get(), X, AV_.get(), nullptr);
+        else k_spmm64<<<gw, 256, 0, st>>>(n, g.row_ptr.get(), g.col.get(), g.blk.get(), V_[l].get(), AV_.get());
+        SGIPC_LAUNCH_CHECK();
 #define SGIPC_LAYER(F, LA)                                                                                   \
-    k_layer<F, LA, true><<<grid, 256, 0, st>>>(n, g.row_ptr.get(), g.col.get(), g.blk.get(), V_[l].get(), X, \
-                                               w_in, wt, V_[l + 1].get(), w_out, chat_.get(), g.s.get(),    \
-                                               nullptr, U_[l].get(), Y_[l].get(), linear_, nullptr)
+    k_layer<F, LA, true><<<grid, 256, 0, st>>>(n, AV_.get(), V_[l].get(), X, w_in, wt, V_[l + 1].get(), w_out, \
+                                               chat_.get(), g.s.get(), nullptr, U_[l].get(), Y_[l].get(),   \
+                                               linear_, nullptr)
         if (first && last) SGIPC_LAYER(true, true);
         else if (first) SGIPC_LAYER(true, false);
         else if (last) SGIPC_LAYER(false, true);
@@ -768,7 +826,7 @@ double Mpnn::loss_grad(hipStream_t st, const CoarseGraph& g, const float* X, con
                                                          grad + 3 * C + (long)l * 2 * C * C);
         SGIPC_LAUNCH_CHECK();
         if (l > 0) {
-            k_bwd_gather<<<grid, 256, 0, st>>>(n, g.row_ptr.get(), g.col.get(), g.blk.get(), Gc, dU_.get(), Gn);
+            k_bwd_gather<<<gw, 256, 0, st>>>(n, g.row_ptr.get(), g.col.get(), g.blk.get(), Gc, dU_.get(), Gn);
             SGIPC_LAUNCH_CHECK();
             std::swap(Gc, Gn);
         } else {

@@ -87,7 +87,7 @@ class Mpnn {
     DevBuf<float> V_[kMpnnMaxLayers + 1];
     DevBuf<uint16_t> U_[kMpnnMaxLayers];
     DevBuf<float> Y_[kMpnnMaxLayers];
-    DevBuf<float> G_, G2_, dU_;
+    DevBuf<float> G_, G2_, dU_, AV_;
     DevBuf<uint16_t> dY_;
     DevBuf<float> chat_, dchat_, part_;
     DevBuf<double> lpart_;

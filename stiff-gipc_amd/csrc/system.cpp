@@ -61,6 +61,8 @@ struct sgipc_system {
     int learned_mode = 0;
     int plain_it = -1, learned_it = -1;
     long n_solves = 0;
+    long next_probe = 8;     // solve index of the next measurement of the losing variant
+    long probe_gap = 8;      // doubles after every probe the learned level loses
 };
 
 extern "C" {
@@ -317,7 +319,9 @@ namespace {
 
 // Whether this solve uses the learned correction.  Automatic mode measures
 // both variants once, then keeps the one with fewer iterations and measures
-// the other again every 8th solve (the matrix drifts over Newton steps).
+// the other again later (the matrix drifts over Newton steps): after 8
+// solves, with the gap doubling every time the learned level loses again and
+// reset to 8 when it wins.
 bool use_learned(const sgipc_system* sys)
 {
     if (!sys->net || sys->learned_mode == 0) return false;
@@ -325,7 +329,7 @@ bool use_learned(const sgipc_system* sys)
     if (sys->plain_it < 0) return false;
     if (sys->learned_it < 0) return true;
     const bool better = sys->learned_it < sys->plain_it;
-    return (sys->n_solves % 8 == 7) ? !better : better;
+    return sys->n_solves == sys->next_probe ? !better : better;
 }
 
 int system_solve(sgipc_system* sys, const double* b, double* x, double* grad, const sgipc_pcg_opts* opts,
@@ -425,6 +429,11 @@ int system_solve(sgipc_system* sys, const double* b, double* x, double* grad, co
         pst.learned = used ? 1 : 0;
         if (record == nullptr && status == SGIPC_OK) {
             (used ? sys->learned_it : sys->plain_it) = pst.iterations;
+            if (sys->learned_mode == 2 && sys->plain_it >= 0 && sys->learned_it >= 0 &&
+                sys->n_solves >= sys->next_probe) {
+                sys->probe_gap = sys->learned_it < sys->plain_it ? 8 : 2 * sys->probe_gap;
+                sys->next_probe = sys->n_solves + sys->probe_gap;
+            }
             ++sys->n_solves;
         }
         SGIPC_HIP(hipEventRecord(sys->ev[4], s));
@@ -466,6 +475,7 @@ int sgipc_system_set_learned(sgipc_system* sys, sgipc_mpnn* net, int mode)
     sys->learned_mode = mode;
     sys->plain_it = sys->learned_it = -1;
     sys->n_solves = 0;
+    sys->next_probe = sys->probe_gap = 8;
     return SGIPC_OK;
 }
 
