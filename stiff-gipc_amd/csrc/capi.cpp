@@ -26,6 +26,15 @@ struct FullOp : SpmvOp<TV> {
         return m->apply<TV>(s, TV(1), p, TV(0), Ap, part, ctl ? &ctl->done : nullptr);
     }
     bool fuses_update_p() const override { return true; }
+    // r holds b on entry: r = b - A x with the FP64 values
+    bool residual(hipStream_t s, const TV*, const TV* x, TV* r, const PcgCtl* ctl) const override
+    {
+        if constexpr (sizeof(TV) == 8) {
+            m->apply<double>(s, -1.0, x, 1.0, r, nullptr, ctl ? &ctl->done : nullptr);
+            return true;
+        }
+        return false;
+    }
     int apply_dot_update_p(hipStream_t s, const TV* z, const TV* p_old, TV* p_new, TV* Ap, double* part,
                            const PcgCtl* ctl, const double* rz_new, const double* rz_old, int n_rz) const override
     {
@@ -431,6 +440,7 @@ void sgipc_pcg_default_opts(sgipc_pcg_opts* o)
     o->check_every = 8;
     o->fuse = 1;
     o->flexible = 0;
+    o->ir_replace_every = 16;
     o->ir_max_outer = 20;
     o->ir_inner_tol = 0.0;
 }
@@ -485,8 +495,10 @@ int pcg_solve_impl(Context& c, SolveWorkspace& ws, const double* vals, const int
         // FP64 iterative refinement: the inner PCG keeps FP64 vectors, dot
         // products and alpha/beta, and reads the FP32 copies of the matrix
         // (SpMV) and of the MAS inverse blocks (apply); the outer residual
-        // b - A x uses the FP64 matrix.  Each correction solve is asked for
-        // just the reduction still missing, so one or two passes suffice.
+        // b - A x uses the FP64 matrix.  Inside a pass the recursive residual
+        // is replaced by the FP64 one every ir_replace_every iterations, so
+        // one Krylov run usually meets the tolerance; a further pass (a
+        // restart) only follows when it did not.
         ws.r64.reserve(n3);
         ws.d64.reserve(n3);
         FullOp<double> A;
@@ -494,6 +506,10 @@ int pcg_solve_impl(Context& c, SolveWorkspace& ws, const double* vals, const int
         A.max_partials = ws.full.partials();
         A.m = &ws.full;
         A.f32_values = true;
+        FullOp<double> A64;
+        A64.nnz = ws.full.nnz;
+        A64.max_partials = ws.full.partials();
+        A64.m = &ws.full;
         SubsystemPrecond<double> M(prec);
         launch_fill<double>(s, x, 0.0, n3);
         SGIPC_HIP(hipMemcpyAsync(ws.r64.get(), b, sizeof(double) * n3, hipMemcpyDeviceToDevice, s));
@@ -517,7 +533,7 @@ int pcg_solve_impl(Context& c, SolveWorkspace& ws, const double* vals, const int
             ++outer;
             PcgOptions po;
             // reduce ||r|| by the factor still missing (at least by half)
-            double need = o.tol * std::sqrt(bb / rr) * 0.9;
+            double need = o.tol * std::sqrt(bb / rr) * 0.8;
             if (o.ir_inner_tol > 0 && need < o.ir_inner_tol) need = o.ir_inner_tol;
             if (need > 0.5) need = 0.5;
             po.tol = need;
@@ -526,6 +542,8 @@ int pcg_solve_impl(Context& c, SolveWorkspace& ws, const double* vals, const int
             po.check_every = o.check_every;
             po.fuse = o.fuse;
             po.flexible = o.flexible != 0 || (extra && extra->flexible);
+            po.true_op = &A64;
+            po.replace_every = o.ir_replace_every;
             PcgResult r = ws.pcg64.solve(s, A, has_prec ? &M : nullptr, ws.r64.get(), ws.d64.get(), n3, po);
             total_it += r.iterations;
             if (r.not_spd) {

@@ -372,6 +372,7 @@ PcgResult PcgSolver<TV>::solve(hipStream_t s, const SpmvOp<TV>& A, Preconditione
     const bool flex = opt.flexible && M != nullptr && M->set_flex(Ap, part_zap_.get());
     const bool fuse_p = !flex && opt.fuse >= 2 && A.fuses_update_p();
     bool fuse_xr = opt.fuse >= 1 && M != nullptr;  // cleared when the preconditioner declines
+    const bool replace = sizeof(TV) == 8 && opt.true_op != nullptr && opt.replace_every > 0;
     bool p_pending = false;                   // p update of the previous iteration not yet applied
     const int gv = vec_grid(n3);
     const int chunk = opt.check_every > 0 ? opt.check_every : 8;
@@ -398,7 +399,8 @@ PcgResult PcgSolver<TV>::solve(hipStream_t s, const SpmvOp<TV>& A, Preconditione
                 }
                 TV* p = pbuf[pb];
                 int n_rzn = -1;
-                if (fuse_xr) {
+                const bool rep_now = replace && ((issued + it + 1) % opt.replace_every == 0);
+                if (fuse_xr && !rep_now) {
                     FusedXR<TV> fx;
                     fx.ctl = ctl;
                     fx.rz = part_rz_[cur].get();
@@ -421,7 +423,17 @@ PcgResult PcgSolver<TV>::solve(hipStream_t s, const SpmvOp<TV>& A, Preconditione
                                                                 part_pap_.get(), n_pap, x, r, p, Ap, n3,
                                                                 part_rr_.get());
                     SGIPC_LAUNCH_CHECK();
-                    PcgStopDesc sd{ctl, part_rz_[cur].get(), n_rz, part_rr_.get(), gv};
+                    int n_rr = gv;
+                    if constexpr (sizeof(TV) == 8) {
+                        if (rep_now) {
+                            // the true residual replaces the recursive one; the
+                            // stop rule below sees its norm
+                            SGIPC_HIP(hipMemcpyAsync(r, b, sizeof(TV) * n3, hipMemcpyDeviceToDevice, s));
+                            opt.true_op->residual(s, b, x, r, ctl);
+                            n_rr = launch_dot_partials<TV, TV>(s, r, r, n3, part_rr_.get());
+                        }
+                    }
+                    PcgStopDesc sd{ctl, part_rz_[cur].get(), n_rz, part_rr_.get(), n_rr};
                     if (M) {
                         n_rzn = M->apply(s, r, z, part_rz_[nxt].get(), sd);
                     } else {

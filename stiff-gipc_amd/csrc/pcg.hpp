@@ -19,6 +19,9 @@ struct PcgCtl {
     int not_spd;
 };
 
+template <typename TV>
+struct SpmvOp;
+
 // What the first kernel after the x/r update needs to evaluate the stop rule.
 struct PcgStopDesc {
     PcgCtl* ctl = nullptr;
@@ -46,6 +49,12 @@ struct PcgOptions {
     int record_stride = 1;
     int record_cap = 0;
     int* record_count = nullptr;  // host
+    // residual replacement: every replace_every iterations the recursive
+    // residual becomes b - A_true x, with A_true = true_op (the FP64 matrix
+    // when the iteration's SpMV reads FP32 values); p and the Krylov
+    // recurrences continue
+    const SpmvOp<double>* true_op = nullptr;
+    int replace_every = 0;
 };
 
 // Inputs of the x/r update (x += alpha p, r -= alpha Ap, alpha = rz/pAp) that
@@ -93,6 +102,8 @@ struct SpmvOp {
     {
         return -1;
     }
+    // r = b - A x (residual replacement); false when not offered
+    virtual bool residual(hipStream_t, const TV*, const TV*, TV*, const PcgCtl*) const { return false; }
 };
 
 // z = M r + per-workgroup partials of r.z; evaluates `stop` first when its ctl

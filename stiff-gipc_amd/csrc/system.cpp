@@ -498,7 +498,13 @@ int sgipc_system_collect(sgipc_system* sys, const double* b, const sgipc_pcg_opt
     r.record_stride = stride;
     r.record_cap = max_samples;
     r.record_count = &count;
-    st = system_solve(sys, b, xtmp.get(), nullptr, opts, nullptr, nullptr, &r);
+    // residuals are recorded by the single-run solver: FP32_IR collects with
+    // the FP32 MAS apply and the FP64 SpMV
+    sgipc_pcg_opts o;
+    sgipc_pcg_default_opts(&o);
+    if (opts) o = *opts;
+    if (o.precision == SGIPC_FP32_IR) o.precision = SGIPC_FP32_MAS;
+    st = system_solve(sys, b, xtmp.get(), nullptr, &o, nullptr, nullptr, &r);
     if (st != SGIPC_OK && st != SGIPC_MAXITER) return st;
     sgipc_mpnn_data* d = mpnn_data_new(sys->ctx);
     st = run(sys->ctx, [&] {

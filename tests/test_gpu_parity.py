@@ -379,3 +379,24 @@ def test_flexible_pcg_matches_pcg_for_a_fixed_preconditioner(pkg, scene_c0, coar
         assert abs(a.iterations - b.iterations) <= 1 and a.true_rel_residual <= 1e-6
     finally:
         ls.close()
+
+
+@pytest.mark.parametrize("coarse", [0, 1])
+def test_fp32_refinement_with_residual_replacement(pkg, scene_c0, coarse):
+    """FP32-valued SpMV and FP32 MAS apply with the recursive residual
+    replaced by the FP64 one every 16 iterations: one Krylov run meets the
+    FP64 tolerance in about the iterations of the FP64 solve, fewer than
+    refinement by restarts."""
+    ls = pkg.LinearSystem(scene_c0, bank_size=32, coarse_mode=coarse)
+    try:
+        plain = ls.solve(tol=1e-6, precision="fp64")
+        rep = ls.solve(tol=1e-6, precision="fp32_ir", ir_replace_every=16)
+        rst = ls.solve(tol=1e-6, precision="fp32_ir", ir_replace_every=0)
+        print(f"coarse={coarse}: fp64 {plain.iterations}, replacement {rep.iterations} "
+              f"({rep.outer_iterations} pass), restarts {rst.iterations} ({rst.outer_iterations} passes)")
+        assert rep.true_rel_residual <= 1e-6 and rst.true_rel_residual <= 1e-6
+        assert rep.iterations <= rst.iterations
+        assert rep.iterations <= 1.25 * plain.iterations + 8
+        assert np.linalg.norm(rep.x - plain.x) <= 1e-4 * np.linalg.norm(plain.x)
+    finally:
+        ls.close()
