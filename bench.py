@@ -41,22 +41,31 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def spmv_bytes(nnz, n_rows, fp32=False):
+def spmv_bytes(nnz, n_rows, f32_values=False):
     """Algorithmic bytes of one symmetric SpMV over `nnz` unique upper blocks:
     every upper block's values + row/col index once, x read once and y written
-    once (B_spmv = nb*80 + 48*n_rows in FP64, nb*44 + 24*n_rows in FP32)."""
-    if fp32:
-        return nnz * (36 + 8) + 2 * 12 * n_rows
-    return nnz * (72 + 8) + 2 * 24 * n_rows
+    once, vectors in FP64: nb*80 + 48*n_rows with FP64 values, nb*44 + 48*n_rows
+    with FP32 values (the mixed-precision PCG's SpMV)."""
+    return nnz * ((36 if f32_values else 72) + 8) + 2 * 24 * n_rows
 
 
-def layout_bytes(stored, n_rows, fp32=False):
+def layout_bytes(stored, n_rows, f32_values=False):
     """Bytes the tiled layout actually streams: every stored block (upper
     blocks + cross-tile transposed copies) with its 4-byte column word, the
     per-row slice metadata, x once and y once."""
-    v = 36 if fp32 else 72
-    w = 12 if fp32 else 24
-    return stored * (v + 4) + 12 * n_rows + 2 * w * n_rows
+    v = 36 if f32_values else 72
+    return stored * (v + 4) + 12 * n_rows + 2 * 24 * n_rows
+
+
+def measured_traffic(kernel_key):
+    """HBM bytes per SpMV launch from the FETCH_SIZE / WRITE_SIZE profile
+    summary committed under profiles/ (None when absent)."""
+    path = os.path.join(ROOT, "profiles", "spmv_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get(kernel_key)
+    except (OSError, ValueError):
+        return None
 
 
 def main():
@@ -185,11 +194,13 @@ def main():
     pkg._lib.check(L.sgipc_spmv_time(ctx.h, pkg.FP32_IR, v, rr, cc, nnz, n, args.spmv_reps, 1, ctypes.byref(avg),
                                      ctypes.byref(ctypes.c_int())), ctx.h)
     spmv32_ms = avg.value
-    gbs32 = spmv_bytes(nnz, n, fp32=True) / (spmv32_ms * 1e-3) / 1e9
+    b32 = spmv_bytes(nnz, n, f32_values=True)
+    gbs32 = b32 / (spmv32_ms * 1e-3) / 1e9
+    moved32 = layout_bytes(stored, n, f32_values=True) / (spmv32_ms * 1e-3) / 1e9
     log(f"SpMV fp64 (cold cache): {spmv_ms * 1e3:.1f} us/launch, {nnz} upper blocks, {b64 / 1e6:.1f} MB algorithmic "
         f"-> {gbs64:.0f} GB/s ({gbs64 / HBM_PEAK_GBS:.1%} of 8 TB/s); layout stores {stored} blocks, streams "
         f"{moved64:.0f} GB/s; back-to-back (matrix partly in the 256 MB Infinity Cache): {spmv_warm_ms * 1e3:.1f} us; "
-        f"fp32 cold: {spmv32_ms * 1e3:.1f} us, {gbs32:.0f} GB/s algorithmic")
+        f"FP32 values (FP64 vectors) cold: {spmv32_ms * 1e3:.1f} us, {gbs32:.0f} GB/s algorithmic")
 
     # MAS apply timing
     mas_ms = None
@@ -232,6 +243,33 @@ def main():
         cpu = cpu_baseline(ge, scene, args.bank, out["iters"], args.cpu_pcg_iters)
 
     ls.close()
+    mixed = args.precision == "fp32_ir"
+    key = "k_spmv_tsym<float,double>" if mixed else "k_spmv_tsym<double,double>"
+    t_ms, b_alg, gbs, moved = (spmv32_ms, b32, gbs32, moved32) if mixed else (spmv_ms, b64, gbs64, moved64)
+    tr = measured_traffic(key)
+    roof = {
+        "bound": "hbm",
+        "kernel": (f"{key} (the PCG loop's SpMV in this configuration: 512-row tiles, in-tile upper blocks once, "
+                   "emits p.Ap partials)"),
+        "achieved": round(gbs, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(gbs / HBM_PEAK_GBS, 4),
+        "traffic": tr["bytes_per_launch"] if tr else None,
+        "traffic_source": tr["source"] if tr else None,
+        "bytes_per_launch": b_alg,
+        "bytes_basis": ("algorithmic: nnz_upper*(" + ("36" if mixed else "72") +
+                        " B values + 8 B row/col) + 24 B/row x + 24 B/row y (FP64 vectors)"),
+        "nnz_upper_blocks": nnz,
+        "stored_blocks": stored,
+        "layout_streamed_GBs": round(moved, 1),
+        "avg_launch_us": round(t_ms * 1e3, 3),
+        "timing": "HIP events around each launch; a 512 MiB buffer is streamed before every launch so the "
+                  "matrix comes from HBM, as inside PCG",
+        "fp64_values_avg_launch_us": round(spmv_ms * 1e3, 3),
+        "fp64_values_GBs": round(gbs64, 1),
+        "fp64_values_back_to_back_us": round(spmv_warm_ms * 1e3, 3),
+    }
     dtype = {"fp64": "f64", "fp32_mas": "f64 (MAS apply f32)",
              "fp32_ir": "f32 SpMV values + MAS apply, f64 vectors/dots/refinement"}[args.precision]
     line = {
@@ -271,25 +309,7 @@ def main():
             "spmv_fp32_us": round(spmv32_ms * 1e3, 2),
             "spmv_fp32_GBs": round(gbs32, 1),
         },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": "k_spmv_tsym<double,double,true> (PCG SpMV: 512-row tiles, in-tile upper blocks once, "
-                      "emits p.Ap partials)",
-            "achieved": round(gbs64, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(gbs64 / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "bytes_per_launch": b64,
-            "bytes_basis": "algorithmic: nnz_upper*(72 B values + 8 B row/col) + 24 B/row x + 24 B/row y",
-            "nnz_upper_blocks": nnz,
-            "stored_blocks": stored,
-            "layout_streamed_GBs": round(moved64, 1),
-            "avg_launch_us": round(spmv_ms * 1e3, 3),
-            "timing": "HIP events around each launch; a 512 MiB buffer is streamed before every launch so the "
-                      "matrix comes from HBM, as inside PCG",
-            "back_to_back_us": round(spmv_warm_ms * 1e3, 3),
-        },
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

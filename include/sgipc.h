@@ -203,8 +203,9 @@ int sgipc_spmat_apply_f32(sgipc_spmat* m, float a, const float* x, float b, floa
  * context's stream.  cold_cache != 0 streams a 512 MiB buffer before every
  * launch, so the matrix comes from HBM as it does inside PCG, where the MAS
  * apply evicts it; 0 times back-to-back launches.  precision: SGIPC_FP64 or
- * SGIPC_FP32_IR (FP32 values).  *stored_blocks (optional) receives the number
- * of blocks the layout stores. */
+ * SGIPC_FP32_IR (FP32 matrix values with FP64 vectors, the kernel PCG runs in
+ * that mode).  *stored_blocks (optional) receives the number of blocks the
+ * layout stores. */
 int sgipc_spmv_time(sgipc_ctx* ctx, int precision, const double* vals, const int* rows,
                     const int* cols, int nnz, int n_rows, int reps, int cold_cache, double* avg_ms,
                     int* stored_blocks);

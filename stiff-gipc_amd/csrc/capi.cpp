@@ -259,18 +259,12 @@ int sgipc_spmv_time(sgipc_ctx* ctx, int precision, const double* vals, const int
         A.build(s, vals, rows, cols, nnz, n_rows, precision == SGIPC_FP32_IR);
         if (stored_blocks) *stored_blocks = (int)A.stored_blocks(s);
         const long n3 = 3L * n_rows;
+        // FP32_IR times the PCG loop's kernel: FP32 matrix values, FP64 vectors
         DevBuf<double> x64, y64, part, flush;
-        DevBuf<float> x32, y32;
         part.reserve(A.partials() > 0 ? A.partials() : 1);
-        if (precision == SGIPC_FP64) {
-            x64.reserve(n3);
-            y64.reserve(n3);
-            launch_fill<double>(s, x64.get(), 1.0, n3);
-        } else {
-            x32.reserve(n3);
-            y32.reserve(n3);
-            launch_fill<float>(s, x32.get(), 1.0f, n3);
-        }
+        x64.reserve(n3);
+        y64.reserve(n3);
+        launch_fill<double>(s, x64.get(), 1.0, n3);
         // 512 MiB, twice the Infinity Cache: streaming it evicts the matrix
         const long n_flush = cold_cache ? (512L << 20) / (long)sizeof(double) : 0;
         if (cold_cache) {
@@ -281,7 +275,7 @@ int sgipc_spmv_time(sgipc_ctx* ctx, int precision, const double* vals, const int
             if (precision == SGIPC_FP64)
                 A.apply<double>(s, 1.0, x64.get(), 0.0, y64.get(), part.get(), nullptr);
             else
-                A.apply<float>(s, 1.0f, x32.get(), 0.0f, y32.get(), part.get(), nullptr);
+                A.apply_f32_values(s, 1.0, x64.get(), 0.0, y64.get(), part.get(), nullptr);
         };
         launch();  // warm-up
         std::vector<hipEvent_t> ev(2 * (size_t)reps);
