@@ -41,6 +41,17 @@ constexpr int kThreads = kTile * kSplit;   // 1024
 constexpr int kTcap = 4096;                // in-tile transposed slots per tile (LDS)
 constexpr uint32_t kInTile = 0x80000000u;
 
+// Workgroups are handed to the 8 XCDs of the part round-robin in blockIdx
+// order.  Mapping block b to tile start(b % 8) + b / 8 gives every XCD a
+// contiguous run of tiles, so the x entries neighbouring tiles share are
+// fetched into one XCD's L2 instead of into all of them.  A bijection on
+// [0, n): XCD x owns n / 8 + (x < n % 8) tiles starting at x (n / 8) + min(x, n % 8).
+__device__ __forceinline__ int xcd_tile(int b, int n)
+{
+    const int x = b & 7, q = n >> 3, rem = n & 7;
+    return x * q + (x < rem ? x : rem) + (b >> 3);
+}
+
 // With kFuseP the gathered vector is p_new = z + beta p_old, formed on the fly
 // (beta = rz_new / rz_old from partials, summed in a fixed order), and every
 // owned row of p_new is written: the PCG p update rides along with the SpMV.
@@ -67,7 +78,7 @@ __global__ __launch_bounds__(kThreads) void k_spmv_tsym(const TM* __restrict__ v
         if (kFuseP) return (TV)((double)zf[k] + beta * (double)x[k]);
         return x[k];
     };
-    const int tile = blockIdx.x;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int part = threadIdx.x / kTile;
     const int r = threadIdx.x % kTile;
     const int lane = r & 63;
