@@ -240,7 +240,10 @@ def main():
     # ---- CPU baseline: the oracle on the host cores, bounded sample
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(ge, scene, args.bank, out["iters"], args.cpu_pcg_iters)
+        # the reference's method (additive MAS, B=32, FP64): its iteration
+        # count from the same-run plain solve when there is one
+        ref_iters = plain["iterations"] if plain else out["iters"]
+        cpu = cpu_baseline(ge, scene, 32, ref_iters, args.cpu_pcg_iters)
 
     ls.close()
     mixed = args.precision == "fp32_ir"
@@ -357,7 +360,8 @@ def train_learned(pkg, ls, args):
 def cpu_baseline(ge, scene, bank, gpu_iters, cpu_iters):
     """Oracle (C restatement of the reference algorithm, OpenMP) on the host:
     assembly + conversion + MAS build timed in full, PCG timed for a bounded
-    number of iterations and scaled to the GPU's iteration count."""
+    number of iterations and scaled to the iteration count of the reference's
+    method on this scene (additive MAS, measured by the same-run GPU solve)."""
     import ctypes
     orc = ge.load_oracle()
     threads = os.cpu_count() or 1
@@ -384,9 +388,10 @@ def cpu_baseline(ge, scene, bank, gpu_iters, cpu_iters):
         "unit": "ms",
         "cores": threads,
         "kind": "port",
-        "sample": (f"oracle/sgipc_oracle.c (OpenMP, {threads} threads) on the same scene: assembly+conversion "
-                   f"({t1 - t0:.2f}s) and MAS build ({t2 - t1:.2f}s) timed in full, {it} PCG iterations timed "
-                   f"({t3 - t2:.2f}s) and scaled to the GPU solve's {gpu_iters:.0f} iterations"),
+        "sample": (f"oracle/sgipc_oracle.c (OpenMP, {threads} threads), the reference's method (additive MAS "
+                   f"B={bank}, FP64) on the same scene: assembly+conversion ({t1 - t0:.2f}s) and MAS build "
+                   f"({t2 - t1:.2f}s) timed in full, {it} PCG iterations timed ({t3 - t2:.2f}s) and scaled to the "
+                   f"method's {gpu_iters:.0f} iterations"),
     }
 
 

@@ -65,11 +65,13 @@ __global__ __launch_bounds__(kThreads) void k_spmv_tsym(const TM* __restrict__ v
                                                         double* __restrict__ dot_part, const int* __restrict__ done,
                                                         const TV* __restrict__ zf, TV* __restrict__ p_new,
                                                         const double* __restrict__ rzn,
-                                                        const double* __restrict__ rzo, int n_rz)
+                                                        const double* __restrict__ rzo, int n_rz, int n_rows)
 {
     __shared__ TV tl[kTcap * 3];
     __shared__ TV red[3 * kTile];
+    __shared__ TV xs[3 * kTile];  // the gathered vector over this tile's own rows
     __shared__ double dred[16];
+    constexpr int kU = sizeof(TM) == 4 ? 3 : 2;  // slots per step (loads in flight)
     if (done != nullptr && *done) return;
     double beta = 0.0;
     if (kFuseP) beta = block_sum_array(rzn, n_rz, dred) / block_sum_array(rzo, n_rz, dred);
@@ -79,6 +81,12 @@ __global__ __launch_bounds__(kThreads) void k_spmv_tsym(const TM* __restrict__ v
         return x[k];
     };
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    {
+        // in-tile columns read x from LDS: one coalesced load of the tile's rows
+        const long b3 = 3L * tile * kTile, e3 = 3L * n_rows;
+        for (int t = threadIdx.x; t < 3 * kTile; t += kThreads) xs[t] = b3 + t < e3 ? gx(b3 + t) : TV(0);
+        __syncthreads();
+    }
     const int part = threadIdx.x / kTile;
     const int r = threadIdx.x % kTile;
     const int lane = r & 63;
@@ -86,7 +94,6 @@ __global__ __launch_bounds__(kThreads) void k_spmv_tsym(const TM* __restrict__ v
     const int p = q * 64 + lane;
     const int i = perm[p];
     const int len = plen[p];
-    const long base = (long)tile * kTile;
     TV xi0 = 0, xi1 = 0, xi2 = 0;
     if (i >= 0) {
         xi0 = gx(3 * (long)i);
@@ -105,61 +112,58 @@ __global__ __launch_bounds__(kThreads) void k_spmv_tsym(const TM* __restrict__ v
     TV a0 = 0, a1 = 0, a2 = 0;
     const uint32_t* cp = cw + S * 64;
     const TM* vp = vals + S * 9 * 64;
-    for (; s + 2 <= s1; s += 2) {
-        const bool act0 = s < len, act1 = s + 1 < len;
-        const int l0 = act0 ? lane : 0, l1 = act1 ? lane : 0;
-        const uint32_t w0 = cp[(long)s * 64 + l0], w1 = cp[(long)(s + 1) * 64 + l1];
-        TM h[9], g[9];
-#pragma unroll
-        for (int c = 0; c < 9; ++c) {
-            h[c] = vp[((long)s * 9 + c) * 64 + l0];
-            g[c] = vp[((long)(s + 1) * 9 + c) * 64 + l1];
+    // one slot: its block times x_j into the row sums, and its transposed
+    // product into the LDS slot when the block is in-tile
+    auto slot = [&](const TM* h, uint32_t w, bool act) {
+        const bool in = act && (w & kInTile);
+        const TV m = act ? TV(1) : TV(0);
+        TV x0, x1, x2;
+        if (w & kInTile) {
+            const int lc = 3 * ((w >> 16) & 0x7FFF);
+            x0 = m * xs[lc];
+            x1 = m * xs[lc + 1];
+            x2 = m * xs[lc + 2];
+        } else {
+            const long j = (long)w;
+            x0 = m * gx(3 * j);
+            x1 = m * gx(3 * j + 1);
+            x2 = m * gx(3 * j + 2);
         }
-        const bool in0 = act0 && (w0 & kInTile), in1 = act1 && (w1 & kInTile);
-        const long j0 = (w0 & kInTile) ? base + ((w0 >> 16) & 0x7FFF) : (long)w0;
-        const long j1 = (w1 & kInTile) ? base + ((w1 >> 16) & 0x7FFF) : (long)w1;
-        const TV m0 = act0 ? TV(1) : TV(0), m1 = act1 ? TV(1) : TV(0);
-        const TV x0 = m0 * gx(3 * j0), x1 = m0 * gx(3 * j0 + 1), x2 = m0 * gx(3 * j0 + 2);
-        const TV z0 = m1 * gx(3 * j1), z1 = m1 * gx(3 * j1 + 1), z2 = m1 * gx(3 * j1 + 2);
         a0 += (TV)h[0] * x0 + (TV)h[3] * x1 + (TV)h[6] * x2;
         a1 += (TV)h[1] * x0 + (TV)h[4] * x1 + (TV)h[7] * x2;
         a2 += (TV)h[2] * x0 + (TV)h[5] * x1 + (TV)h[8] * x2;
-        a0 += (TV)g[0] * z0 + (TV)g[3] * z1 + (TV)g[6] * z2;
-        a1 += (TV)g[1] * z0 + (TV)g[4] * z1 + (TV)g[7] * z2;
-        a2 += (TV)g[2] * z0 + (TV)g[5] * z1 + (TV)g[8] * z2;
-        if (in0) {
-            const int ts = w0 & 0xFFFF;
+        if (in) {
+            const int ts = w & 0xFFFF;
             tl[3 * ts] = (TV)h[0] * xi0 + (TV)h[1] * xi1 + (TV)h[2] * xi2;
             tl[3 * ts + 1] = (TV)h[3] * xi0 + (TV)h[4] * xi1 + (TV)h[5] * xi2;
             tl[3 * ts + 2] = (TV)h[6] * xi0 + (TV)h[7] * xi1 + (TV)h[8] * xi2;
         }
-        if (in1) {
-            const int ts = w1 & 0xFFFF;
-            tl[3 * ts] = (TV)g[0] * xi0 + (TV)g[1] * xi1 + (TV)g[2] * xi2;
-            tl[3 * ts + 1] = (TV)g[3] * xi0 + (TV)g[4] * xi1 + (TV)g[5] * xi2;
-            tl[3 * ts + 2] = (TV)g[6] * xi0 + (TV)g[7] * xi1 + (TV)g[8] * xi2;
+    };
+    // kU slots per step, all their loads issued first; the matrix stream is
+    // read once, so its loads are non-temporal and leave the L2 to x
+    for (; s + kU <= s1; s += kU) {
+        TM h[kU][9];
+        uint32_t w[kU];
+        bool act[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            act[u] = s + u < len;
+            const int l = act[u] ? lane : 0;
+            w[u] = __builtin_nontemporal_load(cp + (long)(s + u) * 64 + l);
+#pragma unroll
+            for (int c = 0; c < 9; ++c) h[u][c] = __builtin_nontemporal_load(vp + ((long)(s + u) * 9 + c) * 64 + l);
         }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) slot(h[u], w[u], act[u]);
     }
-    if (s < s1) {
+    for (; s < s1; ++s) {
         const bool act0 = s < len;
         const int l0 = act0 ? lane : 0;
-        const uint32_t w0 = cp[(long)s * 64 + l0];
         TM h[9];
+        const uint32_t w0 = __builtin_nontemporal_load(cp + (long)s * 64 + l0);
 #pragma unroll
-        for (int c = 0; c < 9; ++c) h[c] = vp[((long)s * 9 + c) * 64 + l0];
-        const bool in0 = act0 && (w0 & kInTile);
-        const long j0 = (w0 & kInTile) ? base + ((w0 >> 16) & 0x7FFF) : (long)w0;
-        const TV m0 = act0 ? TV(1) : TV(0);
-        const TV x0 = m0 * gx(3 * j0), x1 = m0 * gx(3 * j0 + 1), x2 = m0 * gx(3 * j0 + 2);
-        a0 += (TV)h[0] * x0 + (TV)h[3] * x1 + (TV)h[6] * x2;
-        a1 += (TV)h[1] * x0 + (TV)h[4] * x1 + (TV)h[7] * x2;
-        a2 += (TV)h[2] * x0 + (TV)h[5] * x1 + (TV)h[8] * x2;
-        if (in0) {
-            const int ts = w0 & 0xFFFF;
-            tl[3 * ts] = (TV)h[0] * xi0 + (TV)h[1] * xi1 + (TV)h[2] * xi2;
-            tl[3 * ts + 1] = (TV)h[3] * xi0 + (TV)h[4] * xi1 + (TV)h[5] * xi2;
-            tl[3 * ts + 2] = (TV)h[6] * xi0 + (TV)h[7] * xi1 + (TV)h[8] * xi2;
-        }
+        for (int c = 0; c < 9; ++c) h[c] = __builtin_nontemporal_load(vp + ((long)s * 9 + c) * 64 + l0);
+        slot(h, w0, act0);
     }
     if (part != 0) {
         red[r] = a0;
@@ -472,7 +476,7 @@ int FullMatrix::launch(hipStream_t s, const TM* vals, TV a, const TV* x, TV b, T
 #define SGIPC_TSYM(DOT, FUSE)                                                                                   \
     k_spmv_tsym<TM, TV, DOT, FUSE><<<g, kThreads, 0, s>>>(vals, sell_cols.get(), slice_ptr.get(), slice_w.get(), \
                                                           perm.get(), plen.get(), tinfo.get(), a, x, b, y,        \
-                                                          dot_part, done, z, p_new, rzn, rzo, n_rz)
+                                                          dot_part, done, z, p_new, rzn, rzo, n_rz, n_rows)
     if (p_new) {
         if (dot_part) SGIPC_TSYM(true, true);
         else SGIPC_TSYM(false, true);
