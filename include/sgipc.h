@@ -195,6 +195,17 @@ int sgipc_spmat_create(sgipc_ctx* ctx, sgipc_spmat** out, const double* vals, co
 int sgipc_spmat_destroy(sgipc_spmat* m);
 int sgipc_spmat_apply(sgipc_spmat* m, double a, const double* x, double b, double* y);
 int sgipc_spmat_apply_f32(sgipc_spmat* m, float a, const float* x, float b, float* y);
+/* FP32 copy of the values (want_f32) against FP64 x and y, FP64 accumulation:
+ * the SpMV of the FP32_IR solver's inner PCG. */
+int sgipc_spmat_apply_f32_values(sgipc_spmat* m, double a, const double* x, double b, double* y);
+/* Layout report (no counterpart in the reference): *halo_form = 1 when the
+ * column words index each tile's LDS copy of x (own rows + halo), 0 for the
+ * global-gather form (taken when that copy does not fit in LDS, or when the
+ * environment variable SGIPC_SPMV_GATHER is set to a non-zero value at
+ * create()); *halo_max = largest halo of a 512-row tile (vertices);
+ * *tl_slots = in-tile transposed slots the fullest tile uses.  Any output
+ * pointer may be NULL. */
+int sgipc_spmat_layout(const sgipc_spmat* m, int* halo_form, int* halo_max, int* tl_slots);
 
 /* Measurement helper: lays the upper BCOO out as the PCG solver does (row
  * tiles, in-tile blocks once, cross-tile blocks in both rows) and returns the

@@ -248,6 +248,24 @@ int sgipc_spmat_apply_f32(sgipc_spmat* m, float a, const float* x, float b, floa
     });
 }
 
+int sgipc_spmat_apply_f32_values(sgipc_spmat* m, double a, const double* x, double b, double* y)
+{
+    if (!m || !x || !y) return SGIPC_BAD_ARG;
+    return guarded(m->ctx, [&] {
+        m->m.apply_f32_values(m->ctx->c.stream, a, x, b, y, nullptr, nullptr);
+        SGIPC_HIP(hipStreamSynchronize(m->ctx->c.stream));
+    });
+}
+
+int sgipc_spmat_layout(const sgipc_spmat* m, int* halo_form, int* halo_max, int* tl_slots)
+{
+    if (!m) return SGIPC_BAD_ARG;
+    if (halo_form) *halo_form = m->m.use_halo ? 1 : 0;
+    if (halo_max) *halo_max = m->m.halo_max;
+    if (tl_slots) *tl_slots = m->m.tl_max;
+    return SGIPC_OK;
+}
+
 int sgipc_spmv_time(sgipc_ctx* ctx, int precision, const double* vals, const int* rows, const int* cols, int nnz,
                     int n_rows, int reps, int cold_cache, double* avg_ms, int* stored_blocks)
 {

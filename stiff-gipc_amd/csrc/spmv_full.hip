@@ -22,6 +22,17 @@
 //   * two waves share a slice, each taking half of its slots, and combine the
 //     two partial rows through LDS.
 // Per tile: kTile rows, kSplit * kTile threads; x^T(Ax) partials per tile.
+//
+// Halo form (used whenever its LDS fits): after packing, every column word
+// that is not an in-tile transposed slot is rewritten as an index into the
+// tile's local copy of the gathered vector -- the tile's own rows, then its
+// halo, the distinct outside vertices its stored blocks reference (ascending).
+// The local copy is staged in LDS before the slots are streamed, so the slot
+// loop reads no vector entry from memory: the global gather of a slot's
+// column sat between that slot's loads and its products and cost one more
+// memory round trip per slot.
+#include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -40,6 +51,10 @@ constexpr int kSplit = 2;                  // waves per slice
 constexpr int kThreads = kTile * kSplit;   // 1024
 constexpr int kTcap = 4096;                // in-tile transposed slots per tile (LDS)
 constexpr uint32_t kInTile = 0x80000000u;
+constexpr int kWaves = kThreads / 64;          // waves of a workgroup
+constexpr int kSlices = kTile / 64;            // SELL-64 slices of a tile
+constexpr int kPieces = kWaves + kSlices - 1;  // (wave, slice) partial rows of the halo kernel
+constexpr int kLdsBytes = 163840;              // LDS of one CU on gfx950
 
 // Workgroups are handed to the 8 XCDs of the part round-robin in blockIdx
 // order.  Mapping block b to tile start(b % 8) + b / 8 gives every XCD a
@@ -194,6 +209,219 @@ __global__ __launch_bounds__(kThreads) void k_spmv_tsym(const TM* __restrict__ v
             yi[2] = a * a2 + b * yi[2];
         }
         if (kDot) dot = (double)xi0 * (double)a0 + (double)xi1 * (double)a1 + (double)xi2 * (double)a2;
+    }
+    if (kDot) {
+        const double sblk = block_sum(dot, dred);
+        if (threadIdx.x == 0) dot_part[tile] = sblk;
+    }
+}
+
+// Halo kernel.  The tile's slots (its slices back to back) are cut into kWaves
+// equal ranges, one per wave, across slice boundaries; a wave parks one partial
+// row sum per slice it touches in LDS piece wave + slice (distinct for distinct
+// non-empty pieces, as both cut lists increase) and each row adds its pieces in
+// wave order.  Each step's loads are issued before the previous step is used.
+// Tiles run in `order` (heaviest first inside each XCD's range), so the
+// lightest tiles fill the second round of workgroups.
+// Dynamic LDS: tl[3 tcap], piece[3 * 64 kPieces], xl[3 (kTile + hcap)] of TV.
+template <typename TM, typename TV, bool kDot, bool kFuseP>
+__global__ __launch_bounds__(kThreads) void k_spmv_halo(const TM* __restrict__ vals, const uint32_t* __restrict__ cw,
+                                                        const int* __restrict__ slice_ptr,
+                                                        const int* __restrict__ perm, const int* __restrict__ plen,
+                                                        const uint32_t* __restrict__ tinfo,
+                                                        const int* __restrict__ halo_ptr,
+                                                        const int* __restrict__ halo_list,
+                                                        const int* __restrict__ order, int tcap, TV a,
+                                                        const TV* __restrict__ x, TV b, TV* __restrict__ y,
+                                                        double* __restrict__ dot_part, const int* __restrict__ done,
+                                                        const TV* __restrict__ zf, TV* __restrict__ p_new,
+                                                        const double* __restrict__ rzn,
+                                                        const double* __restrict__ rzo, int n_rz, int n_rows)
+{
+    extern __shared__ __align__(16) unsigned char lds_dyn[];
+    TV* const tl = reinterpret_cast<TV*>(lds_dyn);
+    TV* const piece = tl + 3 * tcap;
+    TV* const xl = piece + kPieces * 3 * 64;
+    __shared__ int s_len[kTile];
+    __shared__ int s_row[kTile];  // row of each tile position, relative to the tile (-1: padding)
+    __shared__ int s_sb[kSlices + 1];
+    __shared__ double dred[16];
+    constexpr int kU = sizeof(TM) == 4 ? 2 : 1;  // slots per step
+    if (done != nullptr && *done) return;
+    double beta = 0.0;
+    if (kFuseP) beta = block_sum_array(rzn, n_rz, dred) / block_sum_array(rzo, n_rz, dred);
+    auto gx = [&](long k) -> TV {
+        if (kFuseP) return (TV)((double)zf[k] + beta * (double)x[k]);
+        return x[k];
+    };
+    const int tile = order[xcd_tile(blockIdx.x, gridDim.x)];
+    const long b3 = 3L * tile * kTile;
+    const int q0 = tile * kSlices;
+    const long A = slice_ptr[q0];
+    const int tot = (int)((long)slice_ptr[q0 + kSlices] - A);
+    uint32_t ti_r = 0;
+    {
+        // stage the tile's own rows, its halo, its row table and slice bounds
+        const long e3 = 3L * n_rows;
+        for (int t = threadIdx.x; t < 3 * kTile; t += kThreads) xl[t] = b3 + t < e3 ? gx(b3 + t) : TV(0);
+        const int h0 = halo_ptr[tile], hn = halo_ptr[tile + 1] - h0;
+        for (int k = threadIdx.x; k < hn; k += kThreads) {
+            const long j = halo_list[h0 + k];
+            TV* d = xl + 3 * (kTile + k);
+            d[0] = gx(3 * j);
+            d[1] = gx(3 * j + 1);
+            d[2] = gx(3 * j + 2);
+        }
+        if (threadIdx.x < kTile) {
+            const int pr = perm[q0 * 64 + threadIdx.x];
+            s_row[threadIdx.x] = pr >= 0 ? pr - tile * kTile : -1;
+            s_len[threadIdx.x] = plen[q0 * 64 + threadIdx.x];
+            if (pr >= 0) ti_r = tinfo[pr];
+        }
+        if (threadIdx.x <= kSlices) s_sb[threadIdx.x] = (int)((long)slice_ptr[q0 + threadIdx.x] - A);
+        __syncthreads();
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int w0 = (int)((long)tot * wave / kWaves), w1 = (int)((long)tot * (wave + 1) / kWaves);
+    const uint32_t* cp = cw + A * 64;
+    const TM* vp = vals + A * 9 * 64;
+    if (w0 < w1) {
+        // the loads of the step at local slot ps of slice sl (matrix stream read
+        // once: non-temporal); slots past the wave's range repeat slot ps, masked
+        auto issue = [&](int ps, int sl, uint32_t* W, TM (*H)[9], bool* act) {
+            const int send = min(w1, s_sb[sl + 1]);
+            const int len = s_len[sl * 64 + lane];
+            const int s0 = ps - s_sb[sl];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const bool in = ps + u < send;
+                act[u] = in && s0 + u < len;
+                const int l = act[u] ? lane : 0;
+                const long g = in ? ps + u : ps;
+                W[u] = __builtin_nontemporal_load(cp + g * 64 + l);
+#pragma unroll
+                for (int c = 0; c < 9; ++c) H[u][c] = __builtin_nontemporal_load(vp + (g * 9 + c) * 64 + l);
+            }
+        };
+        int sq = 0;
+        while (s_sb[sq + 1] <= w0) ++sq;
+        int pos = w0;
+        uint32_t Wa[kU];
+        TM Ha[kU][9];
+        bool aa[kU];
+        issue(pos, sq, Wa, Ha, aa);
+        int cur = -1;
+        TV a0 = 0, a1 = 0, a2 = 0, xi0 = 0, xi1 = 0, xi2 = 0;
+        while (true) {
+            const int npos = min(pos + kU, min(w1, s_sb[sq + 1]));
+            int nsq = sq;
+            while (nsq < kSlices - 1 && s_sb[nsq + 1] <= npos) ++nsq;
+            const bool more = npos < w1;
+            uint32_t Wb[kU];
+            TM Hb[kU][9];
+            bool ab[kU];
+            if (more) issue(npos, nsq, Wb, Hb, ab);
+            if (sq != cur) {
+                if (cur >= 0) {
+                    TV* pc = piece + (wave + cur) * 3 * 64 + lane;
+                    pc[0] = a0;
+                    pc[64] = a1;
+                    pc[128] = a2;
+                }
+                cur = sq;
+                const int lr = s_row[sq * 64 + lane];
+                const int li = 3 * (lr >= 0 ? lr : 0);
+                xi0 = xl[li];
+                xi1 = xl[li + 1];
+                xi2 = xl[li + 2];
+                a0 = a1 = a2 = TV(0);
+            }
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const uint32_t w = Wa[u];
+                const TV m = aa[u] ? TV(1) : TV(0);
+                const int lc = 3 * ((w >> 16) & 0x3FFF);
+                const TV x0 = m * xl[lc], x1 = m * xl[lc + 1], x2 = m * xl[lc + 2];
+                const TM* h = Ha[u];
+                a0 += (TV)h[0] * x0 + (TV)h[3] * x1 + (TV)h[6] * x2;
+                a1 += (TV)h[1] * x0 + (TV)h[4] * x1 + (TV)h[7] * x2;
+                a2 += (TV)h[2] * x0 + (TV)h[5] * x1 + (TV)h[8] * x2;
+                if (aa[u] && (w & kInTile)) {
+                    const int ts = w & 0xFFFF;
+                    tl[3 * ts] = (TV)h[0] * xi0 + (TV)h[1] * xi1 + (TV)h[2] * xi2;
+                    tl[3 * ts + 1] = (TV)h[3] * xi0 + (TV)h[4] * xi1 + (TV)h[5] * xi2;
+                    tl[3 * ts + 2] = (TV)h[6] * xi0 + (TV)h[7] * xi1 + (TV)h[8] * xi2;
+                }
+            }
+            if (!more) break;
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                Wa[u] = Wb[u];
+                aa[u] = ab[u];
+#pragma unroll
+                for (int c = 0; c < 9; ++c) Ha[u][c] = Hb[u][c];
+            }
+            pos = npos;
+            sq = nsq;
+        }
+        TV* pc = piece + (wave + cur) * 3 * 64 + lane;
+        pc[0] = a0;
+        pc[64] = a1;
+        pc[128] = a2;
+    }
+    __syncthreads();
+    double dot = 0.0;
+    if (threadIdx.x < kTile) {
+        const int sq = threadIdx.x >> 6;
+        const int lr = s_row[threadIdx.x];
+        if (lr >= 0) {
+            const int Sq = s_sb[sq], Sq1 = s_sb[sq + 1];
+            TV r0 = 0, r1 = 0, r2 = 0;
+            for (int wv = 0; wv < kWaves; ++wv) {
+                const int lo = (int)((long)tot * wv / kWaves), hi = (int)((long)tot * (wv + 1) / kWaves);
+                if ((lo > Sq ? lo : Sq) >= (hi < Sq1 ? hi : Sq1)) continue;
+                const TV* pc = piece + (wv + sq) * 3 * 64 + lane;
+                r0 += pc[0];
+                r1 += pc[64];
+                r2 += pc[128];
+            }
+            const int tb = (int)(ti_r >> 16), te = tb + (int)(ti_r & 0xFFFF);
+            int k = tb;
+            for (; k + 4 <= te; k += 4) {
+                TV v[12];
+#pragma unroll
+                for (int e = 0; e < 12; ++e) v[e] = tl[3 * k + e];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    r0 += v[3 * e];
+                    r1 += v[3 * e + 1];
+                    r2 += v[3 * e + 2];
+                }
+            }
+            for (; k < te; ++k) {
+                r0 += tl[3 * k];
+                r1 += tl[3 * k + 1];
+                r2 += tl[3 * k + 2];
+            }
+            const long i = (long)tile * kTile + lr;
+            const TV v0 = xl[3 * lr], v1 = xl[3 * lr + 1], v2 = xl[3 * lr + 2];
+            if (kFuseP) {
+                p_new[3 * i] = v0;
+                p_new[3 * i + 1] = v1;
+                p_new[3 * i + 2] = v2;
+            }
+            TV* yi = y + 3 * i;
+            if (b == TV(0)) {
+                yi[0] = a * r0;
+                yi[1] = a * r1;
+                yi[2] = a * r2;
+            } else {
+                yi[0] = a * r0 + b * yi[0];
+                yi[1] = a * r1 + b * yi[1];
+                yi[2] = a * r2 + b * yi[2];
+            }
+            if (kDot) dot = (double)v0 * (double)r0 + (double)v1 * (double)r1 + (double)v2 * (double)r2;
+        }
     }
     if (kDot) {
         const double sblk = block_sum(dot, dred);
@@ -359,6 +587,116 @@ __global__ void k_pack(const double* __restrict__ vals, const uint32_t* __restri
     }
 }
 
+// ---- halo build: one thread per (slice, lane) walks that lane's slots.
+// A word that is not an in-tile transposed slot names a global column; it is
+// external when the column lies outside the slot's tile.  Padding rows are
+// skipped (their words are rewritten to local index 0 and never used).
+__global__ void k_ext_count(const uint32_t* __restrict__ cw, const int* __restrict__ slice_ptr,
+                            const int* __restrict__ perm, int n_slices, int* __restrict__ cnt)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_slices * 64) return;
+    const int q = t >> 6, lane = t & 63;
+    int c = 0;
+    if (perm[t] >= 0) {
+        const int tile = q / kSlices;
+        for (long s = slice_ptr[q]; s < slice_ptr[q + 1]; ++s) {
+            const uint32_t w = cw[s * 64 + lane];
+            if (!(w & kInTile) && (int)w / kTile != tile) ++c;
+        }
+    }
+    cnt[t] = c;
+}
+
+__global__ void k_ext_emit(const uint32_t* __restrict__ cw, const int* __restrict__ slice_ptr,
+                           const int* __restrict__ perm, const int* __restrict__ off, int n_slices,
+                           uint64_t* __restrict__ keys)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_slices * 64 || perm[t] < 0) return;
+    const int q = t >> 6, lane = t & 63;
+    const int tile = q / kSlices;
+    int o = off[t];
+    for (long s = slice_ptr[q]; s < slice_ptr[q + 1]; ++s) {
+        const uint32_t w = cw[s * 64 + lane];
+        if (!(w & kInTile) && (int)w / kTile != tile) keys[o++] = ((uint64_t)tile << 32) | (uint64_t)w;
+    }
+}
+
+// first occurrence of each (tile, vertex) in the sorted keys
+__global__ void k_uniq_flag(const uint64_t* __restrict__ k, int n, int* __restrict__ flag)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    flag[i] = (i == 0 || k[i] != k[i - 1]) ? 1 : 0;
+}
+
+// halo_list[uid] = vertex for the first occurrence (inc = inclusive scan of the flags)
+__global__ void k_halo_list(const uint64_t* __restrict__ k, const int* __restrict__ flag,
+                            const int* __restrict__ inc, int n, int* __restrict__ halo_list)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !flag[i]) return;
+    halo_list[inc[i] - 1] = (int)(uint32_t)k[i];
+}
+
+// halo_ptr[t] = uid of the first key of tile t (keys of a tile are contiguous)
+__global__ void k_halo_ptr(const uint64_t* __restrict__ k, const int* __restrict__ inc, int n, int n_tiles,
+                           int* __restrict__ halo_ptr)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > n_tiles) return;
+    const uint64_t key = (uint64_t)t << 32;
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (k[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    halo_ptr[t] = lo < n ? inc[lo] - 1 : (n > 0 ? inc[n - 1] : 0);
+}
+
+// stats[0] = largest halo of a tile, stats[1] = in-tile transposed slots a tile uses
+__global__ void k_tile_stats(const int* __restrict__ halo_ptr, const uint32_t* __restrict__ tinfo, int n_tiles, int n,
+                             int* __restrict__ stats)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_tiles) atomicMax(&stats[0], halo_ptr[i + 1] - halo_ptr[i]);
+    if (i < n) atomicMax(&stats[1], (int)(tinfo[i] >> 16) + (int)(tinfo[i] & 0xFFFF));
+}
+
+// rewrite the words as local indices: own rows (column - tile start), halo
+// vertices (kTile + rank in the tile's ascending halo list)
+__global__ void k_localize(uint32_t* __restrict__ cw, const int* __restrict__ slice_ptr,
+                           const int* __restrict__ perm, const int* __restrict__ halo_ptr,
+                           const int* __restrict__ halo_list, int n_slices)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_slices * 64) return;
+    const int q = t >> 6, lane = t & 63;
+    const int tile = q / kSlices;
+    const bool pad = perm[t] < 0;
+    const int h0 = halo_ptr[tile], h1 = halo_ptr[tile + 1];
+    for (long s = slice_ptr[q]; s < slice_ptr[q + 1]; ++s) {
+        const uint32_t w = cw[s * 64 + lane];
+        if (w & kInTile) continue;
+        uint32_t lw = 0;
+        if (!pad) {
+            const int j = (int)w;
+            if (j / kTile == tile) {
+                lw = (uint32_t)(j - tile * kTile) << 16;
+            } else {
+                int lo = h0, hi = h1;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (halo_list[mid] < j) lo = mid + 1; else hi = mid;
+                }
+                lw = (uint32_t)(kTile + (lo - h0)) << 16;
+            }
+        }
+        cw[s * 64 + lane] = lw;
+    }
+}
+
 inline int g256(long n) { return (int)((n + 255) / 256); }
 
 }  // namespace
@@ -452,6 +790,119 @@ void FullMatrix::build(hipStream_t s, const double* vals, const int* rows, const
                                        perm.get(), slice_ptr.get(), slice_w.get(), n_pos, kbits, sell_vals.get(),
                                        want_f32 ? sell_vals32.get() : nullptr, sell_cols.get());
     SGIPC_LAUNCH_CHECK();
+    build_halo(s, tbits);
+}
+
+void FullMatrix::build_halo(hipStream_t s, int tbits)
+{
+    use_halo = false;
+    const int nl = n_slices * 64;
+    hcnt_.reserve(nl + 1);
+    hoff_.reserve(nl + 1);
+    halo_ptr.reserve(n_tiles + 1);
+    stats_.reserve(2);
+    size_t need = 0;
+    SGIPC_HIP(rocprim::exclusive_scan(nullptr, need, hcnt_.get(), hoff_.get(), 0, (size_t)nl + 1, rocprim::plus<int>(),
+                                      s));
+    temp_.reserve(need);
+    k_ext_count<<<g256(nl), 256, 0, s>>>(sell_cols.get(), slice_ptr.get(), perm.get(), n_slices, hcnt_.get());
+    SGIPC_LAUNCH_CHECK();
+    SGIPC_HIP(hipMemsetAsync(hcnt_.get() + nl, 0, sizeof(int), s));
+    SGIPC_HIP(rocprim::exclusive_scan(temp_.get(), need, hcnt_.get(), hoff_.get(), 0, (size_t)nl + 1,
+                                      rocprim::plus<int>(), s));
+    int n_ext = 0;
+    SGIPC_HIP(hipMemcpyAsync(&n_ext, hoff_.get() + nl, sizeof(int), hipMemcpyDeviceToHost, s));
+    SGIPC_HIP(hipStreamSynchronize(s));
+    SGIPC_CHECK(n_ext >= 0, SGIPC_CAPACITY, "halo entry count overflows int");
+    if (n_ext > 0) {
+        hkeys_.reserve(n_ext);
+        hkeys_sorted_.reserve(n_ext);
+        hflag_.reserve(n_ext);
+        hinc_.reserve(n_ext);
+        halo_list.reserve(n_ext);
+        size_t n1 = 0, n2 = 0;
+        SGIPC_HIP(rocprim::radix_sort_keys(nullptr, n1, hkeys_.get(), hkeys_sorted_.get(), (size_t)n_ext, 0, 32 + tbits,
+                                           s));
+        SGIPC_HIP(rocprim::inclusive_scan(nullptr, n2, hflag_.get(), hinc_.get(), (size_t)n_ext, rocprim::plus<int>(),
+                                          s));
+        temp_.reserve(n1 > n2 ? n1 : n2);
+        k_ext_emit<<<g256(nl), 256, 0, s>>>(sell_cols.get(), slice_ptr.get(), perm.get(), hoff_.get(), n_slices,
+                                            hkeys_.get());
+        SGIPC_LAUNCH_CHECK();
+        SGIPC_HIP(rocprim::radix_sort_keys(temp_.get(), n1, hkeys_.get(), hkeys_sorted_.get(), (size_t)n_ext, 0,
+                                           32 + tbits, s));
+        k_uniq_flag<<<g256(n_ext), 256, 0, s>>>(hkeys_sorted_.get(), n_ext, hflag_.get());
+        SGIPC_LAUNCH_CHECK();
+        SGIPC_HIP(rocprim::inclusive_scan(temp_.get(), n2, hflag_.get(), hinc_.get(), (size_t)n_ext,
+                                          rocprim::plus<int>(), s));
+        k_halo_list<<<g256(n_ext), 256, 0, s>>>(hkeys_sorted_.get(), hflag_.get(), hinc_.get(), n_ext,
+                                                halo_list.get());
+        SGIPC_LAUNCH_CHECK();
+    } else {
+        halo_list.reserve(1);
+    }
+    k_halo_ptr<<<g256(n_tiles + 1), 256, 0, s>>>(hkeys_sorted_.get(), hinc_.get(), n_ext, n_tiles, halo_ptr.get());
+    SGIPC_LAUNCH_CHECK();
+    SGIPC_HIP(hipMemsetAsync(stats_.get(), 0, 2 * sizeof(int), s));
+    k_tile_stats<<<g256(n_tiles > n_rows ? n_tiles : n_rows), 256, 0, s>>>(halo_ptr.get(), tinfo.get(), n_tiles, n_rows,
+                                                                             stats_.get());
+    SGIPC_LAUNCH_CHECK();
+    int st[2] = {0, 0};
+    SGIPC_HIP(hipMemcpyAsync(st, stats_.get(), sizeof(st), hipMemcpyDeviceToHost, s));
+    std::vector<int> sp((size_t)n_slices + 1);
+    SGIPC_HIP(hipMemcpyAsync(sp.data(), slice_ptr.get(), sizeof(int) * sp.size(), hipMemcpyDeviceToHost, s));
+    SGIPC_HIP(hipStreamSynchronize(s));
+    halo_max = st[0];
+    tl_max = st[1] > 0 ? st[1] : 1;
+    // the FP64-vector kernels have the largest LDS; use the halo form only when
+    // it fits beside the largest static LDS among them
+    if (halo_dyn_bytes(sizeof(double)) + halo_static_lds() > (size_t)kLdsBytes) return;
+    const char* env = std::getenv("SGIPC_SPMV_GATHER");
+    if (env && env[0] && env[0] != '0') return;  // keep the global-gather form (tests of both forms)
+    k_localize<<<g256(nl), 256, 0, s>>>(sell_cols.get(), slice_ptr.get(), perm.get(), halo_ptr.get(), halo_list.get(),
+                                        n_slices);
+    SGIPC_LAUNCH_CHECK();
+    // heaviest tiles first inside each XCD's contiguous range of the launch order
+    std::vector<int> ord((size_t)n_tiles);
+    const int q = n_tiles / 8, rem = n_tiles % 8;
+    for (int xc = 0; xc < 8; ++xc) {
+        const int b0 = xc * q + (xc < rem ? xc : rem), cnt = q + (xc < rem ? 1 : 0);
+        for (int k = 0; k < cnt; ++k) ord[(size_t)b0 + k] = b0 + k;
+        std::stable_sort(ord.begin() + b0, ord.begin() + b0 + cnt, [&](int u, int v) {
+            return sp[(size_t)(u + 1) * kSlices] - sp[(size_t)u * kSlices] >
+                   sp[(size_t)(v + 1) * kSlices] - sp[(size_t)v * kSlices];
+        });
+    }
+    order.reserve(n_tiles);
+    SGIPC_HIP(hipMemcpyAsync(order.get(), ord.data(), sizeof(int) * ord.size(), hipMemcpyHostToDevice, s));
+    SGIPC_HIP(hipStreamSynchronize(s));
+    use_halo = true;
+}
+
+size_t FullMatrix::halo_dyn_bytes(size_t tv_size) const
+{
+    return (3 * (size_t)tl_max + (size_t)kPieces * 3 * 64 + 3 * ((size_t)kTile + (size_t)halo_max)) * tv_size;
+}
+
+size_t FullMatrix::halo_static_lds()
+{
+    static size_t cached = 0;
+    if (cached == 0) {
+        const void* fns[] = {reinterpret_cast<const void*>(&k_spmv_halo<double, double, false, false>),
+                             reinterpret_cast<const void*>(&k_spmv_halo<double, double, true, false>),
+                             reinterpret_cast<const void*>(&k_spmv_halo<double, double, false, true>),
+                             reinterpret_cast<const void*>(&k_spmv_halo<double, double, true, true>),
+                             reinterpret_cast<const void*>(&k_spmv_halo<float, double, true, false>),
+                             reinterpret_cast<const void*>(&k_spmv_halo<float, double, true, true>)};
+        size_t m = 0;
+        for (const void* f : fns) {
+            hipFuncAttributes fa;
+            SGIPC_HIP(hipFuncGetAttributes(&fa, f));
+            if (fa.sharedSizeBytes > m) m = fa.sharedSizeBytes;
+        }
+        cached = m;
+    }
+    return cached;
 }
 
 int FullMatrix::partials() const { return n_tiles; }
@@ -473,6 +924,36 @@ int FullMatrix::launch(hipStream_t s, const TM* vals, TV a, const TV* x, TV b, T
 {
     SGIPC_CHECK(n_tiles > 0, SGIPC_EMPTY, "matrix was not built");
     const int g = n_tiles;
+    if (use_halo) {
+        const size_t dyn = halo_dyn_bytes(sizeof(TV));
+#define SGIPC_HALO(DOT, FUSE)                                                                                       \
+    do {                                                                                                            \
+        const void* fn = reinterpret_cast<const void*>(&k_spmv_halo<TM, TV, DOT, FUSE>);                            \
+        static int max_dyn = -1;                                                                                    \
+        if (max_dyn < 0) {                                                                                          \
+            hipFuncAttributes fa;                                                                                   \
+            SGIPC_HIP(hipFuncGetAttributes(&fa, fn));                                                               \
+            const int m = kLdsBytes - (int)fa.sharedSizeBytes;                                                      \
+            SGIPC_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, m));                      \
+            max_dyn = m;                                                                                            \
+        }                                                                                                           \
+        SGIPC_CHECK(dyn <= (size_t)max_dyn, SGIPC_CAPACITY, "halo SpMV needs more LDS than a CU has");              \
+        k_spmv_halo<TM, TV, DOT, FUSE><<<g, kThreads, dyn, s>>>(vals, sell_cols.get(), slice_ptr.get(), perm.get(), \
+                                                                plen.get(), tinfo.get(), halo_ptr.get(),            \
+                                                                halo_list.get(), order.get(), tl_max, a, x, b, y,   \
+                                                                dot_part, done, z, p_new, rzn, rzo, n_rz, n_rows);  \
+    } while (0)
+        if (p_new) {
+            if (dot_part) SGIPC_HALO(true, true);
+            else SGIPC_HALO(false, true);
+        } else {
+            if (dot_part) SGIPC_HALO(true, false);
+            else SGIPC_HALO(false, false);
+        }
+#undef SGIPC_HALO
+        SGIPC_LAUNCH_CHECK();
+        return dot_part ? g : 0;
+    }
 #define SGIPC_TSYM(DOT, FUSE)                                                                                   \
     k_spmv_tsym<TM, TV, DOT, FUSE><<<g, kThreads, 0, s>>>(vals, sell_cols.get(), slice_ptr.get(), slice_w.get(), \
                                                           perm.get(), plen.get(), tinfo.get(), a, x, b, y,        \

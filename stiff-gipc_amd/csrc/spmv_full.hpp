@@ -36,6 +36,9 @@ class FullMatrix {
 
     int n_rows = 0;
     int nnz = 0;         // unique upper blocks of the source matrix
+    bool use_halo = false;  // words are local indices (halo kernel); else global columns
+    int halo_max = 0;       // largest halo of a tile (vertices)
+    int tl_max = 0;         // in-tile transposed slots the fullest tile uses
     long n_slots = 0;    // SELL slots (64 entries each, padding included)
     int n_slices = 0;
     int n_tiles = 0;
@@ -44,8 +47,15 @@ class FullMatrix {
     DevBuf<float> sell_vals32;
     DevBuf<uint32_t> sell_cols, tinfo;
     DevBuf<int> slice_ptr, slice_w, perm, plen;
+    DevBuf<int> halo_ptr, halo_list, order;  // per-tile halo lists; launch order of the tiles
+
+    // dynamic LDS of the halo kernel with vector entries of tv_size bytes
+    size_t halo_dyn_bytes(size_t tv_size) const;
+    // largest static LDS among the FP64-vector halo kernels (queried once)
+    static size_t halo_static_lds();
 
   private:
+    void build_halo(hipStream_t s, int tbits);
     template <typename TM, typename TV>
     int launch(hipStream_t s, const TM* vals, TV a, const TV* x, TV b, TV* y, double* dot_part, const int* done,
                const TV* z, TV* p_new, const double* rzn, const double* rzo, int n_rz) const;
@@ -54,6 +64,8 @@ class FullMatrix {
     DevBuf<int> idx_, sidx_, cand_, cscan_, up_ptr_, col_ptr_, low_cnt_, len_, iota_, sorted_;
     DevBuf<uint32_t> up_word_;
     DevBuf<unsigned char> temp_;
+    DevBuf<uint64_t> hkeys_, hkeys_sorted_;
+    DevBuf<int> hcnt_, hoff_, hflag_, hinc_, stats_;
 };
 
 }  // namespace sgipc

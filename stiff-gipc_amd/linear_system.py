@@ -63,6 +63,16 @@ class SpMat:
             return check(lib().sgipc_spmat_apply_f32(self.h, a, _p(x), b, _p(y)), self.ctx.h)
         return check(lib().sgipc_spmat_apply(self.h, a, _p(x), b, _p(y)), self.ctx.h)
 
+    def apply_f32_values(self, a, x, b, y):
+        """y = a*A*x + b*y with the FP32 copy of the values and float64 vectors (want_f32)."""
+        return check(lib().sgipc_spmat_apply_f32_values(self.h, a, _p(x), b, _p(y)), self.ctx.h)
+
+    def layout(self):
+        """(halo_form, halo_max, tl_slots) of the row-tiled layout."""
+        v = [ctypes.c_int() for _ in range(3)]
+        check(lib().sgipc_spmat_layout(self.h, *[ctypes.byref(t) for t in v]), self.ctx.h)
+        return tuple(t.value for t in v)
+
     def close(self):
         if self.h:
             lib().sgipc_spmat_destroy(self.h)

@@ -54,21 +54,36 @@ def test_spmv_f64_matches_oracle(pkg, orc, gpu_ctx, which, scene_tiny, scene_sma
     assert rel(dy.download(), ref0) <= 1e-13
 
 
-@pytest.mark.parametrize("which", ["tiny", "small", "c0", "cloth_tiny", "cloth64"])
-def test_tiled_spmv_matches_oracle(pkg, orc, gpu_ctx, which, scene_tiny, scene_small, scene_c0):
+@pytest.mark.parametrize("form", ["halo", "gather"])
+@pytest.mark.parametrize("which", ["tiny", "small", "c0", "cloth_tiny", "cloth64", "grid24_contact"])
+def test_tiled_spmv_matches_oracle(pkg, orc, gpu_ctx, which, form, scene_tiny, scene_small, scene_c0, monkeypatch):
     """The PCG loop's row-tiled SpMV (one tile for the tiny scenes, several
-    with a partly filled last tile for the others), FP64 and FP32."""
+    with a partly filled last tile for the others), FP64, FP32 and FP32
+    values with FP64 vectors; both layout forms: column words as indices of
+    the tile's LDS copy of x (halo), and as global columns (gather)."""
     s = {"tiny": scene_tiny, "small": scene_small, "c0": scene_c0}.get(which)
     if s is None:
-        s = (pkg.scenes.scene_cloth_tiny(N=12, n_pairs=30) if which == "cloth_tiny"
-             else pkg.scenes.cloth_grid_scene("cloth64", 64, n_pairs=400))
+        if which == "cloth_tiny":
+            s = pkg.scenes.scene_cloth_tiny(N=12, n_pairs=30)
+        elif which == "cloth64":
+            s = pkg.scenes.cloth_grid_scene("cloth64", 64, n_pairs=400)
+        else:  # 27 tiles, contact pairs across the whole grid: large halos
+            s = pkg.scenes.tet_grid_scene("grid24_contact", 24, E_loguniform=(1e4, 1e9), n_pairs=3000)
     vals, rows, cols = orc.assemble(s)
     rng = np.random.default_rng(11)
     x = rng.standard_normal(3 * s.n)
     y0 = rng.standard_normal(3 * s.n)
     dv, dr, dc = upload_bcoo(gpu_ctx, vals, rows, cols)
+    if form == "gather":
+        monkeypatch.setenv("SGIPC_SPMV_GATHER", "1")
     A = pkg.SpMat(gpu_ctx, dv, dr, dc, rows.size, s.n, want_f32=True)
+    monkeypatch.delenv("SGIPC_SPMV_GATHER", raising=False)
     try:
+        halo_form, halo_max, tl_slots = A.layout()
+        assert halo_form == (1 if form == "halo" else 0)
+        assert tl_slots >= 1 and halo_max >= 0
+        if s.n <= 512:
+            assert halo_max == 0  # one tile: nothing outside it
         dx, dy = gpu_ctx.to_device(x), gpu_ctx.to_device(y0)
         A.apply(0.9, dx, -0.4, dy)
         assert rel(dy.download(), orc.spmv(0.9, vals, rows, cols, x, -0.4, y0, s.n)) <= 1e-13
@@ -80,6 +95,10 @@ def test_tiled_spmv_matches_oracle(pkg, orc, gpu_ctx, which, scene_tiny, scene_s
         y32 = gpu_ctx.to_device(np.zeros(3 * s.n, np.float32), np.float32)
         A.apply(1.0, x32, 0.0, y32)
         assert rel(y32.download().astype(np.float64), ref) <= 2e-6
+        # FP32 values, FP64 vectors: the matrix rounding only (~6e-8 relative per value)
+        dy.upload(y0)
+        A.apply_f32_values(0.9, dx, -0.4, dy)
+        assert rel(dy.download(), orc.spmv(0.9, vals, rows, cols, x, -0.4, y0, s.n)) <= 2e-6
     finally:
         A.close()
 
