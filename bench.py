@@ -57,6 +57,13 @@ def layout_bytes(stored, n_rows, f32_values=False):
     return stored * (v + 4) + 12 * n_rows + 2 * 24 * n_rows
 
 
+class _Ptr:
+    """Device pointer (ctypes) in the form the Python handles take."""
+
+    def __init__(self, p):
+        self.ptr = p.value
+
+
 def measured_traffic(kernel_key):
     """HBM bytes per SpMV launch from the FETCH_SIZE / WRITE_SIZE profile
     summary committed under profiles/ (None when absent)."""
@@ -169,7 +176,7 @@ def main():
         log(f"solution error vs FP64 solve to 1e-10: {out['err_vs_fp64']:.3e} "
             f"(FP64 reference: {ref.iterations} iterations, residual {ref.true_rel_residual:.1e})")
 
-    # ---- SpMV kernel on the assembled matrix, HIP events on the library stream
+    # ---- SpMV kernel on the assembled matrix, HIP events recorded by the launches themselves
     if args.only_spmv:
         ls.solve(tol=1e-5, precision="fp64", max_iter=2)
     v, rr, cc, nnz = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_int()
@@ -185,6 +192,10 @@ def main():
                                      ctypes.byref(stored)), ctx.h)
     spmv_ms = avg.value
     stored = stored.value
+    # layout form the solver uses on this matrix (halo: x staged per tile in LDS)
+    spm = pkg.SpMat(ctx, _Ptr(v), _Ptr(rr), _Ptr(cc), nnz, n)
+    halo_form, halo_max, tl_slots = spm.layout()
+    spm.close()
     pkg._lib.check(L.sgipc_spmv_time(ctx.h, pkg.FP64, v, rr, cc, nnz, n, args.spmv_reps, 0, ctypes.byref(avg),
                                      ctypes.byref(ctypes.c_int())), ctx.h)
     spmv_warm_ms = avg.value
@@ -214,7 +225,8 @@ def main():
     if args.only_spmv:
         print(json.dumps({"spmv_us": spmv_ms * 1e3, "spmv_GBs": gbs64, "spmv_warm_us": spmv_warm_ms * 1e3,
                           "spmv32_us": spmv32_ms * 1e3,
-                          "spmv32_GBs": gbs32, "nnz": nnz, "stored_blocks": stored, "layout_GBs": moved64}),
+                          "spmv32_GBs": gbs32, "nnz": nnz, "stored_blocks": stored, "layout_GBs": moved64,
+                          "halo_form": bool(halo_form), "largest_tile_halo": halo_max}),
               flush=True)
         ls.close()
         return
@@ -247,13 +259,16 @@ def main():
 
     ls.close()
     mixed = args.precision == "fp32_ir"
-    key = "k_spmv_tsym<float,double>" if mixed else "k_spmv_tsym<double,double>"
+    kname = "k_spmv_halo" if halo_form else "k_spmv_tsym"
+    key = f"{kname}<float,double>" if mixed else f"{kname}<double,double>"
     t_ms, b_alg, gbs, moved = (spmv32_ms, b32, gbs32, moved32) if mixed else (spmv_ms, b64, gbs64, moved64)
     tr = measured_traffic(key)
     roof = {
         "bound": "hbm",
         "kernel": (f"{key} (the PCG loop's SpMV in this configuration: 512-row tiles, in-tile upper blocks once, "
-                   "emits p.Ap partials)"),
+                   + ("x of each tile and its halo staged in LDS, " if halo_form else "x gathered from memory, ")
+                   + "emits p.Ap partials)"),
+        "layout": {"halo_form": bool(halo_form), "largest_tile_halo": halo_max, "in_tile_slots_max": tl_slots},
         "achieved": round(gbs, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
@@ -267,8 +282,9 @@ def main():
         "stored_blocks": stored,
         "layout_streamed_GBs": round(moved, 1),
         "avg_launch_us": round(t_ms * 1e3, 3),
-        "timing": "HIP events around each launch; a 512 MiB buffer is streamed before every launch so the "
-                  "matrix comes from HBM, as inside PCG",
+        "timing": "HIP event pair recorded by each launch at the kernel's start and end (hipExtLaunchKernelGGL), "
+                  "on the library stream; a 512 MiB buffer is streamed before every launch so the matrix comes "
+                  "from HBM, as inside PCG",
         "fp64_values_avg_launch_us": round(spmv_ms * 1e3, 3),
         "fp64_values_GBs": round(gbs64, 1),
         "fp64_values_back_to_back_us": round(spmv_warm_ms * 1e3, 3),

@@ -210,8 +210,11 @@ int sgipc_spmat_layout(const sgipc_spmat* m, int* halo_form, int* halo_max, int*
 /* Measurement helper: lays the upper BCOO out as the PCG solver does (row
  * tiles, in-tile blocks once, cross-tile blocks in both rows) and returns the
  * average duration (ms) of the PCG SpMV kernel (emits the p.Ap partials)
- * over `reps` launches, each timed with its own HIP event pair on the
- * context's stream.  cold_cache != 0 streams a 512 MiB buffer before every
+ * over `reps` launches, each timed with its own HIP event pair that the
+ * launch records at the kernel's start and end (hipExtLaunchKernelGGL), so the
+ * figure is the kernel's duration as a kernel trace reports it, without the
+ * dispatch gaps of separately recorded events.  cold_cache != 0 streams a
+ * 512 MiB buffer before every
  * launch, so the matrix comes from HBM as it does inside PCG, where the MAS
  * apply evicts it; 0 times back-to-back launches.  precision: SGIPC_FP64 or
  * SGIPC_FP32_IR (FP32 matrix values with FP64 vectors, the kernel PCG runs in
@@ -249,8 +252,6 @@ int sgipc_mas_levels(sgipc_mas* mas, int* level_num, int* offsets, int* counts);
 /* Hierarchy tables (device outputs): coarse_table[vert_num*(level_num-1)]
  * (global slot ids, vertex-major) and going_next[total slots]. */
 int sgipc_mas_tables(sgipc_mas* mas, int* coarse_table, int* going_next);
-/* Measurement helper: average duration (ms) of one preconditioning (all of its
- * kernels) over `reps` applications, HIP events on the context's stream. */
 /* Coarse levels of the apply.  0 (default): additive over all levels, as
  * MASPreconditioner::preconditioning (MASPreconditioner.cu:2248-2289).
  * 1: no counterpart in the reference -- the levels >= 1 become a symmetric
@@ -260,6 +261,8 @@ int sgipc_mas_tables(sgipc_mas* mas, int* coarse_table, int* going_next);
  * symmetric.  Takes effect at the next sgipc_mas_set_preconditioner_bcoo (or
  * system solve). */
 int sgipc_mas_set_coarse_mode(sgipc_mas* mas, int mode);
+/* Measurement helper: average duration (ms) of one preconditioning (all of its
+ * kernels) over `reps` applications, HIP events on the context's stream. */
 int sgipc_mas_time_apply(sgipc_mas* mas, const double* r, double* z, int reps, double* avg_ms);
 /* Inverse domain matrices as FP64 upper blocks (device output,
  * total_slots/B * B(B+1)/2 * 9 doubles). */

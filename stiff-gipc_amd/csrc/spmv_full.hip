@@ -36,6 +36,7 @@
 #include <cstring>
 #include <vector>
 
+#include <hip/hip_ext.h>
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 
@@ -699,6 +700,15 @@ __global__ void k_localize(uint32_t* __restrict__ cw, const int* __restrict__ sl
 
 inline int g256(long n) { return (int)((n + 255) / 256); }
 
+// Launch on stream s; with e0/e1 the events are recorded by the kernel's own
+// dispatch (its start and end), not as separate stream commands.
+template <typename... KArgs, typename... Args>
+void go(void (*kernel)(KArgs...), int grid, size_t shmem, hipStream_t s, hipEvent_t e0, hipEvent_t e1, Args... args)
+{
+    if (e0 && e1) hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(kThreads), (uint32_t)shmem, s, e0, e1, 0, args...);
+    else kernel<<<grid, kThreads, shmem, s>>>(args...);
+}
+
 }  // namespace
 
 void FullMatrix::build(hipStream_t s, const double* vals, const int* rows, const int* cols, int nnz_up, int n,
@@ -938,10 +948,11 @@ int FullMatrix::launch(hipStream_t s, const TM* vals, TV a, const TV* x, TV b, T
             max_dyn = m;                                                                                            \
         }                                                                                                           \
         SGIPC_CHECK(dyn <= (size_t)max_dyn, SGIPC_CAPACITY, "halo SpMV needs more LDS than a CU has");              \
-        k_spmv_halo<TM, TV, DOT, FUSE><<<g, kThreads, dyn, s>>>(vals, sell_cols.get(), slice_ptr.get(), perm.get(), \
-                                                                plen.get(), tinfo.get(), halo_ptr.get(),            \
-                                                                halo_list.get(), order.get(), tl_max, a, x, b, y,   \
-                                                                dot_part, done, z, p_new, rzn, rzo, n_rz, n_rows);  \
+        go(&k_spmv_halo<TM, TV, DOT, FUSE>, g, dyn, s, time_start, time_stop, vals,                                \
+           (const uint32_t*)sell_cols.get(), (const int*)slice_ptr.get(), (const int*)perm.get(),                   \
+           (const int*)plen.get(), (const uint32_t*)tinfo.get(), (const int*)halo_ptr.get(),                        \
+           (const int*)halo_list.get(), (const int*)order.get(), tl_max, a, x, b, y, dot_part, done, z, p_new, rzn, \
+           rzo, n_rz, n_rows);                                                                                      \
     } while (0)
         if (p_new) {
             if (dot_part) SGIPC_HALO(true, true);
@@ -955,9 +966,9 @@ int FullMatrix::launch(hipStream_t s, const TM* vals, TV a, const TV* x, TV b, T
         return dot_part ? g : 0;
     }
 #define SGIPC_TSYM(DOT, FUSE)                                                                                   \
-    k_spmv_tsym<TM, TV, DOT, FUSE><<<g, kThreads, 0, s>>>(vals, sell_cols.get(), slice_ptr.get(), slice_w.get(), \
-                                                          perm.get(), plen.get(), tinfo.get(), a, x, b, y,        \
-                                                          dot_part, done, z, p_new, rzn, rzo, n_rz, n_rows)
+    go(&k_spmv_tsym<TM, TV, DOT, FUSE>, g, 0, s, time_start, time_stop, vals, (const uint32_t*)sell_cols.get(),     \
+       (const int*)slice_ptr.get(), (const int*)slice_w.get(), (const int*)perm.get(), (const int*)plen.get(),       \
+       (const uint32_t*)tinfo.get(), a, x, b, y, dot_part, done, z, p_new, rzn, rzo, n_rz, n_rows)
     if (p_new) {
         if (dot_part) SGIPC_TSYM(true, true);
         else SGIPC_TSYM(false, true);

@@ -48,6 +48,10 @@ class FullMatrix {
     DevBuf<uint32_t> sell_cols, tinfo;
     DevBuf<int> slice_ptr, slice_w, perm, plen;
     DevBuf<int> halo_ptr, halo_list, order;  // per-tile halo lists; launch order of the tiles
+    // when both are set, every SpMV launch records them at its own start and
+    // end (hipExtLaunchKernelGGL): the kernel's duration without the
+    // dispatch overhead that separate event records add
+    hipEvent_t time_start = nullptr, time_stop = nullptr;
 
     // dynamic LDS of the halo kernel with vector entries of tv_size bytes
     size_t halo_dyn_bytes(size_t tv_size) const;
