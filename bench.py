@@ -86,8 +86,11 @@ def main():
     ap.add_argument("--grid", type=int, default=64)
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--coarse", type=int, default=1, choices=[0, 1],
-                    help="MAS coarse levels: 0 additive (the reference), 1 V(1,1) cycle on level 1")
+    ap.add_argument("--coarse", type=int, default=1, choices=[0, 1, 2],
+                    help="MAS coarse levels: 0 additive (the reference), 1 V(1,1) cycle on level 1, "
+                         "2 that cycle inside a two-grid cycle on level 0")
+    ap.add_argument("--omega", type=float, default=None,
+                    help="level-0 smoothing weight of --coarse 2 (library default 0.5)")
     ap.add_argument("--learned", type=int, default=2, choices=[0, 1, 2],
                     help="learned coarse level: 0 off, 1 always, 2 automatic (falls back to plain MAS when worse); "
                          "the network is trained on a separately seeded scene before the timed solves")
@@ -123,6 +126,8 @@ def main():
         f"({time.perf_counter() - t0:.1f}s to generate)")
 
     ls = pkg.LinearSystem(scene, bank_size=args.bank, coarse_mode=args.coarse)
+    if args.omega is not None:
+        ls.set_smoothing_weight(args.omega)
     ctx = ls.ctx
     L = pkg._lib.lib()
     import ctypes
@@ -312,8 +317,11 @@ def main():
             "precision": args.precision,
             "bank_size": args.bank,
             "fused_iteration": args.fuse,
-            "mas_coarse_mode": ("level-1 V(1,1) cycle, exact dense level-2 solve" if args.coarse
-                                else "additive"),
+            "mas_coarse_mode": {0: "additive",
+                                1: "level-1 V(1,1) cycle, exact dense level-2 solve",
+                                2: "two-grid cycle on level 0 (weighted level-0 domain solves before and after) "
+                                   "around the level-1 V(1,1) cycle, exact dense level-2 solve"}[args.coarse],
+            "level0_smoothing_weight": (args.omega if args.omega is not None else 0.5) if args.coarse == 2 else None,
             "learned": learned_info,
             "learned_used_in_timed_steps": out.get("learned_used"),
             "plain_fp64_b32_same_run": plain,

@@ -258,9 +258,22 @@ int sgipc_mas_tables(sgipc_mas* mas, int* coarse_table, int* going_next);
  * V(1,1) cycle on level 1 (level-1 domain solves as the smoother, the additive
  * levels >= 2 as its coarse correction, residuals with the level-1 Galerkin
  * operator); level 0 stays additive, so the preconditioner is still
- * symmetric.  Takes effect at the next sgipc_mas_set_preconditioner_bcoo (or
- * system solve). */
+ * symmetric.
+ * 2: no counterpart in the reference -- mode 1's level-1 cycle becomes the
+ * coarse solve of a symmetric two-grid cycle on level 0: z0 = w M0 r,
+ * r1 = r - A z0, c = (level-1 cycle of P^T r1), r2 = r1 - A P c,
+ * z = z0 + P c + w M0 r2, with M0 the level-0 domain solves, P the level-1
+ * aggregation and w the smoothing weight (default 0.5).  It needs the fine
+ * matrix A inside the apply: the PCG solver supplies its own when the MAS
+ * covers every row of the system (else the apply falls back to mode 1); a
+ * standalone MAS keeps a copy made at set_preconditioner_bcoo.  Symmetric
+ * positive definite for weights below 2 / lambda_max(M0 A).
+ * Modes take effect at the next sgipc_mas_set_preconditioner_bcoo (or system
+ * solve). */
 int sgipc_mas_set_coarse_mode(sgipc_mas* mas, int mode);
+/* Smoothing weight w of coarse mode 2, 0 < w < 2 (no counterpart in the
+ * reference). */
+int sgipc_mas_set_smoothing_weight(sgipc_mas* mas, double omega);
 /* Measurement helper: average duration (ms) of one preconditioning (all of its
  * kernels) over `reps` applications, HIP events on the context's stream. */
 int sgipc_mas_time_apply(sgipc_mas* mas, const double* r, double* z, int reps, double* avg_ms);

@@ -327,31 +327,37 @@ def float_operator(graph):
     return (W @ blk @ W).tocsr()
 
 
-def mas_cycle_apply(parts: MasParts, A1f, r, A2=None):
-    """MAS apply with the level-1 V(1,1) cycle (the library's coarse mode 1):
-    level 0 additive; on level 1 pre-smoothing with the level-1 domain
-    solves, a coarse correction of the residual, post-smoothing; residuals
-    with A1f.  The coarse correction is the exact solve with A2 (the level-2
-    Galerkin operator, dense) when given, else the additive levels >= 2."""
-    n, B, total, L = parts.n, parts.B, parts.total, parts.L
-    off = parts.off
+def _domain_solve(parts: MasParts, R, lo_slot, hi_slot):
+    """Domain solves of the slot range [lo_slot, hi_slot) (whole domains)."""
+    B, total = parts.B, parts.total
+    Z = np.zeros((total, 3))
+    d0, d1 = lo_slot // B, hi_slot // B
+    Z.reshape(-1, 3 * B)[d0:d1] = np.einsum("dij,dj->di", parts.inv[d0:d1], R.reshape(-1, 3 * B)[d0:d1])
+    return Z
+
+
+def level0_apply(parts: MasParts, r):
+    """The level-0 domain solves alone, M_0 r (the identity slot map of level 0)."""
+    n = parts.n
+    R = np.zeros((parts.total, 3))
+    R[:n] = r.reshape(n, 3)
+    return _domain_solve(parts, R, 0, parts.off[1])[:n].reshape(-1)
+
+
+def level1_cycle(parts: MasParts, A1f, g, A2=None):
+    """The level-1 V(1,1) cycle on the level-1 right-hand side g (c1 x 3,
+    the restriction P^T r): pre-smoothing with the level-1 domain solves, a
+    coarse correction of the residual, post-smoothing; residuals with A1f.
+    The coarse correction is the exact solve with A2 (the level-2 Galerkin
+    operator, dense) when given, else the additive levels >= 2.  Returns the
+    level-1 correction c (c1 x 3)."""
+    off, L, total = parts.off, parts.L, parts.total
     c1 = parts.cnt[1]
-    r3 = r.reshape(n, 3)
-    inv = parts.inv
-
-    def solve(R, lo_slot, hi_slot):
-        Z = np.zeros((total, 3))
-        d0, d1 = lo_slot // B, hi_slot // B
-        Z.reshape(-1, 3 * B)[d0:d1] = np.einsum("dij,dj->di", inv[d0:d1], R.reshape(-1, 3 * B)[d0:d1])
-        return Z
-
     R = np.zeros((total, 3))
-    np.add.at(R, parts.slot[:, 0], r3)
-    np.add.at(R, parts.slot[:, 1], r3)
-    Z = solve(R, 0, off[2])                    # level-0 solves and c1 = D1^-1 g
-    g = R[off[1]:off[1] + c1].reshape(-1).copy()
-    c1v = Z[off[1]:off[1] + c1].reshape(-1).copy()
-    rho = g - A1f @ c1v
+    R[off[1]:off[1] + c1] = g.reshape(-1, 3)
+    c1v = _domain_solve(parts, R, off[1], off[2])[off[1]:off[1] + c1].reshape(-1)
+    gv = g.reshape(-1)
+    rho = gv - A1f @ c1v
     slots1 = parts.slot[parts.first]           # ancestors of every level-1 node
     C2 = c1v.reshape(-1, 3).copy()
     if A2 is not None:
@@ -364,12 +370,46 @@ def mas_cycle_apply(parts: MasParts, A1f, r, A2=None):
         Rc = np.zeros((total, 3))
         for l in range(2, L):
             np.add.at(Rc, slots1[:, l], rho.reshape(-1, 3))
-        Zc = solve(Rc, off[2], off[L])
+        Zc = _domain_solve(parts, Rc, off[2], off[L])
         for l in range(2, L):
             C2 += Zc[slots1[:, l]]
-    rho2 = g - A1f @ C2.reshape(-1)
+    rho2 = gv - A1f @ C2.reshape(-1)
     R2 = np.zeros((total, 3))
     R2[off[1]:off[1] + c1] = rho2.reshape(-1, 3)
-    Z1 = solve(R2, off[1], off[2])[off[1]:off[1] + c1]
-    z = Z[:n] + C2[parts.map1] + Z1[parts.map1]
-    return z.reshape(-1)
+    Z1 = _domain_solve(parts, R2, off[1], off[2])[off[1]:off[1] + c1]
+    return C2 + Z1
+
+
+def mas_cycle_apply(parts: MasParts, A1f, r, A2=None):
+    """MAS apply with the level-1 V(1,1) cycle (the library's coarse mode 1):
+    level 0 additive, the level-1 cycle on P^T r prolonged."""
+    n = parts.n
+    g = np.zeros((parts.cnt[1], 3))
+    np.add.at(g, parts.map1, r.reshape(n, 3))
+    c = level1_cycle(parts, A1f, g, A2)
+    return level0_apply(parts, r) + c[parts.map1].reshape(-1)
+
+
+def ap_operator(A, map1, n_coarse):
+    """A P (P the level-1 aggregation), blocks summed in float64 and rounded to
+    float32, as the library stores it for the post-smoothing residual."""
+    AP = (A @ prolongation(map1, n_coarse)).tocsr()
+    AP.data = AP.data.astype(np.float32).astype(np.float64)
+    return AP
+
+
+def mas_vcycle_apply(parts: MasParts, A1f, A, APf, r, A2=None, omega=0.5):
+    """The library's coarse mode 2, a symmetric two-grid cycle on level 0
+    around the level-1 cycle (no counterpart in the reference):
+        z0 = omega M0 r,  r1 = r - A z0,  c = level-1 cycle of P^T r1,
+        r2 = r1 - A P c,  z = z0 + P c + omega M0 r2,
+    with M0 the level-0 domain solves and A P in float32 (APf).  The library
+    forms M0 (r + r2) in one pass of solves; that is the same z."""
+    n = parts.n
+    z0 = omega * level0_apply(parts, r)
+    r1 = r - A @ z0
+    g = np.zeros((parts.cnt[1], 3))
+    np.add.at(g, parts.map1, r1.reshape(n, 3))
+    c = level1_cycle(parts, A1f, g, A2)
+    r2 = r1 - APf @ c.reshape(-1)
+    return z0 + c[parts.map1].reshape(-1) + omega * level0_apply(parts, r2)

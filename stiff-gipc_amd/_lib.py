@@ -110,6 +110,7 @@ _SIGS = {
     "sgipc_system_bcoo": (ci, [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(ci)]),
     "sgipc_system_mas": (vp, [vp]),
     "sgipc_mas_set_coarse_mode": (ci, [vp, ci]),
+    "sgipc_mas_set_smoothing_weight": (ci, [vp, cd]),
     "sgipc_mpnn_create": (ci, [vp, ci, ci, ctypes.POINTER(vp)]),
     "sgipc_mpnn_destroy": (ci, [vp]),
     "sgipc_mpnn_num_params": (ci, [vp]),

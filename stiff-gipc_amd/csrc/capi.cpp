@@ -396,8 +396,15 @@ int sgipc_mas_preconditioning(sgipc_mas* mas, const double* r, double* z)
 
 int sgipc_mas_set_coarse_mode(sgipc_mas* mas, int mode)
 {
-    if (!mas || mode < 0 || mode > 1) return SGIPC_BAD_ARG;
+    if (!mas || mode < 0 || mode > 2) return SGIPC_BAD_ARG;
     mas->m->coarse_mode = mode;
+    return SGIPC_OK;
+}
+
+int sgipc_mas_set_smoothing_weight(sgipc_mas* mas, double omega)
+{
+    if (!mas || !(omega > 0.0) || !(omega < 2.0)) return SGIPC_BAD_ARG;
+    mas->m->omega0 = omega;
     return SGIPC_OK;
 }
 
@@ -478,7 +485,13 @@ int pcg_solve_impl(Context& c, SolveWorkspace& ws, const double* vals, const int
     sgipc_pcg_stats st{};
     int status = SGIPC_OK;
     ws.part.reserve(4096);
-    ws.full.build(s, vals, rows, cols, nnz, n_rows, o.precision == SGIPC_FP32_IR);
+    // coarse mode 2 runs the PCG matrix inside the MAS apply: only when the
+    // MAS covers every row; an FP32 MAS apply reads its FP32 copy
+    const bool mas_fine = prec.mas != nullptr && prec.mas->coarse_mode == 2 && prec.mas_row_offset == 0 &&
+                          !(prec.abd != nullptr && prec.abd->n_bodies > 0) && prec.mas->n == n_rows;
+    const bool want_f32 = o.precision == SGIPC_FP32_IR || (mas_fine && prec.mas->precision != SGIPC_FP64);
+    ws.full.build(s, vals, rows, cols, nnz, n_rows, want_f32);
+    if (prec.mas != nullptr && (mas_fine || !prec.mas->build_own_fine)) prec.mas->fine_ = mas_fine ? &ws.full : nullptr;
     if (o.precision != SGIPC_FP32_IR) {
         FullOp<double> A;
         A.nnz = ws.full.nnz;

@@ -34,7 +34,11 @@
 
 #include <rocsolver/rocsolver.h>
 #include "assembly.hpp"
+#include "spmv_full.hpp"
 #include "wave_ops.hpp"
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <chrono>
 #include <cstring>
@@ -824,7 +828,141 @@ __global__ __launch_bounds__(256) void k_local_solve(const int* __restrict__ don
     }
 }
 
-// z[v] = sum_l Z[slot_l(v)], with r.z partials.  With a level-1 cycle
+template <typename TS>
+struct Vec4Of;
+template <>
+struct Vec4Of<float> {
+    using T = float4;
+};
+template <>
+struct Vec4Of<double> {
+    using T = double4;
+};
+
+// c = C2 + Z1 per level-1 node, padded to 4 components: one aligned load per
+// gather in k_local_solve_post
+template <typename TS>
+__global__ void k_l1_corr4(const int* __restrict__ done, int c1, const TS* __restrict__ C2, const TS* __restrict__ Z1,
+                           typename Vec4Of<TS>::T* __restrict__ c4)
+{
+    if (done != nullptr && *done) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= c1) return;
+    typename Vec4Of<TS>::T v;
+    v.x = C2[3L * k] + Z1[3L * k];
+    v.y = C2[3L * k + 1] + Z1[3L * k + 1];
+    v.z = C2[3L * k + 2] + Z1[3L * k + 2];
+    v.w = 0;
+    c4[k] = v;
+}
+
+// Level-0 post-smoothing of coarse mode 2 with its right-hand side formed in
+// the kernel: for the domain's rows u, R2_u = r_u + R0_u - (A P c)_u (R0 holds
+// r1 = r - omega A z0, c4 the level-1 correction), then Z = Minv R2 as in
+// k_local_solve.  The domain's A P entries are contiguous (CSR rows in
+// order) and load coalesced; each thread forms one entry's product into LDS
+// per pass, and each row component sums its row's entries in CSR order (fixed
+// order).  The inverse blocks are loaded after that phase, so no register
+// holds them across its barriers.
+template <int B, typename TV, typename TS>
+__global__ __launch_bounds__(256) void k_local_solve_post(const int* __restrict__ done, const TS* __restrict__ minv,
+                                                           const TS* __restrict__ R, TS* __restrict__ Z, int n,
+                                                           const int* __restrict__ ap_ptr,
+                                                           const int* __restrict__ ap_col,
+                                                           const float* __restrict__ ap_blk,
+                                                           const typename Vec4Of<TS>::T* __restrict__ c4,
+                                                           const TV* __restrict__ r)
+{
+    if (done != nullptr && *done) return;
+    constexpr int nblk = B * (B + 1) / 2;
+    constexpr int nval = nblk * 9;
+    using V = typename Vec16<TS>::V;
+    constexpr int kv = Vec16<TS>::k;
+    constexpr int nvec = nval / kv;
+    constexpr int per = (nvec + 255) / 256;
+    constexpr int G = 256 / B;
+    constexpr int kChunk = 256;  // A P entries per pass
+    __shared__ __align__(16) TS sM[nval];
+    __shared__ TS sR[3 * B];
+    __shared__ TS sP[3 * kChunk];
+    const int d = blockIdx.x;
+    const int u0 = d * B;
+    const int u1 = u0 + B < n ? u0 + B : n;
+    const int e0 = u0 < n ? ap_ptr[u0] : 0, e1 = u0 < n ? ap_ptr[u1] : 0;
+    const int my_row = threadIdx.x / 3, my_c = threadIdx.x - 3 * (threadIdx.x / 3);
+    const int u = u0 + my_row;
+    const bool own = threadIdx.x < 3 * B && u < n;
+    const int rb0 = own ? ap_ptr[u] : 0, rb1 = own ? ap_ptr[u + 1] : 0;
+    TS acc = 0;
+    for (int c0 = e0; c0 < e1; c0 += kChunk) {
+        const int e = c0 + (int)threadIdx.x;
+        if (e < e1) {
+            const long g = ap_col[e];
+            const float* b = ap_blk + 9L * e;
+            const typename Vec4Of<TS>::T cv = c4[g];
+            sP[3 * threadIdx.x] = (TS)b[0] * cv.x + (TS)b[3] * cv.y + (TS)b[6] * cv.z;
+            sP[3 * threadIdx.x + 1] = (TS)b[1] * cv.x + (TS)b[4] * cv.y + (TS)b[7] * cv.z;
+            sP[3 * threadIdx.x + 2] = (TS)b[2] * cv.x + (TS)b[5] * cv.y + (TS)b[8] * cv.z;
+        }
+        __syncthreads();
+        if (own) {
+            const int k0 = rb0 > c0 ? rb0 : c0, k1 = rb1 < c0 + kChunk ? rb1 : c0 + kChunk;
+            for (int k = k0; k < k1; ++k) acc += sP[3 * (k - c0) + my_c];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 3 * B) {
+        const long o = 3L * u0 + threadIdx.x;
+        sR[threadIdx.x] = own ? (TS)r[o] + R[o] - acc : TS(0);
+    }
+    // the domain's inverse blocks (loaded after the A P phase, so nothing is
+    // held in registers across its barriers)
+    const V* vsrc = reinterpret_cast<const V*>(minv + (long)d * nval);
+    V tmp[per];
+#pragma unroll
+    for (int q = 0; q < per; ++q) {
+        const int e = threadIdx.x + 256 * q;
+        tmp[q] = vsrc[e < nvec ? e : nvec - 1];
+    }
+#pragma unroll
+    for (int q = 0; q < per; ++q) {
+        const int e = threadIdx.x + 256 * q;
+        if (e < nvec) reinterpret_cast<V*>(sM)[e] = tmp[q];
+    }
+    __syncthreads();
+    const int rb = threadIdx.x / G, g = threadIdx.x % G;
+    TS z0 = 0, z1 = 0, z2 = 0;
+#pragma unroll
+    for (int cb = g; cb < B; cb += G) {
+        const TS r0 = sR[3 * cb], r1 = sR[3 * cb + 1], r2 = sR[3 * cb + 2];
+        if (cb >= rb) {
+            const TS* m = sM + 9 * blk_index(B, rb, cb);
+            z0 += m[0] * r0 + m[3] * r1 + m[6] * r2;
+            z1 += m[1] * r0 + m[4] * r1 + m[7] * r2;
+            z2 += m[2] * r0 + m[5] * r1 + m[8] * r2;
+        } else {
+            const TS* m = sM + 9 * blk_index(B, cb, rb);
+            z0 += m[0] * r0 + m[1] * r1 + m[2] * r2;
+            z1 += m[3] * r0 + m[4] * r1 + m[5] * r2;
+            z2 += m[6] * r0 + m[7] * r1 + m[8] * r2;
+        }
+    }
+#pragma unroll
+    for (int off = G / 2; off >= 1; off >>= 1) {
+        z0 += __shfl_xor(z0, off, 64);
+        z1 += __shfl_xor(z1, off, 64);
+        z2 += __shfl_xor(z2, off, 64);
+    }
+    if (g == 0) {
+        const long o = 3 * ((long)d * B + rb);
+        Z[o] = z0;
+        Z[o + 1] = z1;
+        Z[o + 2] = z2;
+    }
+}
+
+// z[v] = w0 Z[v] + sum_{l>=1} Z[slot_l(v)], with r.z partials (w0 = 1 but
+// for the two-grid cycle of coarse mode 2, where it is the smoothing weight).  With a level-1 cycle
 // (C2 != nullptr) the coarse part is C2[map1(v)] + Z[slot_1(v)] (the cycle's
 // coarse-corrected iterate and its post-smoothing).  corr (level-1, float)
 // adds the learned correction; with Ap set the z.Ap partials of flexible PCG
@@ -835,7 +973,7 @@ __global__ __launch_bounds__(256) void k_prolong(const int* __restrict__ done, c
                                                  const TS* __restrict__ Z, const TV* __restrict__ r,
                                                  TV* __restrict__ z, double* __restrict__ rz_part,
                                                  const TS* __restrict__ C2, const float* __restrict__ corr,
-                                                 const TV* __restrict__ Ap, double* __restrict__ zap_part)
+                                                 const TV* __restrict__ Ap, double* __restrict__ zap_part, TS w0)
 {
     __shared__ double smem[16];
     if (done != nullptr && *done) return;
@@ -844,7 +982,7 @@ __global__ __launch_bounds__(256) void k_prolong(const int* __restrict__ done, c
     for (int l = 0; l < kMasMaxLevels; ++l) offs[l] = l < L ? meta[kMetaOff + l] : 0;
     double acc = 0.0, acc2 = 0.0;
     for (long u = (long)blockIdx.x * blockDim.x + threadIdx.x; u < n; u += (long)gridDim.x * blockDim.x) {
-        TS z0 = Z[3 * u], z1 = Z[3 * u + 1], z2 = Z[3 * u + 2];
+        TS z0 = w0 * Z[3 * u], z1 = w0 * Z[3 * u + 1], z2 = w0 * Z[3 * u + 2];
         if (C2 != nullptr) {
             const long k = map[(long)n + u];
             const long o = 3 * (offs[1] + k);
@@ -1050,6 +1188,84 @@ __global__ __launch_bounds__(1024) void k_sample_normalise(int rows, float* __re
     }
 }
 
+// ---- coarse mode 2: A P of the post-smoothing residual ----------------------
+// (row u, level-1 node g) keys of A P: each upper block (i, j) adds A_ij to
+// (i, map1[j]) and, off the diagonal, A_ij^T to (j, map1[i]).  Entry e = 2b + t
+// (t = 1: transposed); the unused transposed entry of a diagonal block gets
+// row n, which sorts after every real key.
+__global__ void k_ap_keys(const int* __restrict__ rows, const int* __restrict__ cols, int nnz, int n,
+                          const int* __restrict__ map1, uint64_t* __restrict__ keys, int* __restrict__ idx)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nnz) return;
+    const int i = rows[b], j = cols[b];
+    keys[2L * b] = ((uint64_t)(uint32_t)i << 32) | (uint32_t)map1[j];
+    keys[2L * b + 1] = i != j ? (((uint64_t)(uint32_t)j << 32) | (uint32_t)map1[i]) : ((uint64_t)(uint32_t)n << 32);
+    idx[2L * b] = 2 * b;
+    idx[2L * b + 1] = 2 * b + 1;
+}
+
+// first entry of every distinct real key
+__global__ void k_ap_flags(const uint64_t* __restrict__ k, int m, int n, int* __restrict__ flag)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m) return;
+    flag[e] = ((int)(k[e] >> 32) < n && (e == 0 || k[e] != k[e - 1])) ? 1 : 0;
+}
+
+__global__ void k_ap_cols(const uint64_t* __restrict__ k, const int* __restrict__ flag, const int* __restrict__ inc,
+                          int m, int* __restrict__ col, int* __restrict__ seg_start)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m || !flag[e]) return;
+    col[inc[e] - 1] = (int)(uint32_t)k[e];
+    seg_start[inc[e] - 1] = e;
+}
+
+// ap_ptr[u] = first entry of row u (lower bound of u << 32 among the sorted
+// keys); for u = n the lower bound is the count of real keys, which ends the
+// last segment (seg_start[n_uniq])
+__global__ void k_ap_ptr(const uint64_t* __restrict__ k, const int* __restrict__ inc, int m, int n, int n_uniq,
+                         int* __restrict__ ptr, int* __restrict__ seg_start)
+{
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u > n) return;
+    const uint64_t key = (uint64_t)(uint32_t)u << 32;
+    int lo = 0, hi = m;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (k[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    ptr[u] = (lo < m && (int)(k[lo] >> 32) < n) ? inc[lo] - 1 : n_uniq;
+    if (u == n) seg_start[n_uniq] = lo;
+}
+
+// every distinct (u, g): its blocks summed in sorted order in FP64, stored in
+// FP32 (column-major 3x3, as the BCOO)
+__global__ void k_ap_sum(const double* __restrict__ vals, const int* __restrict__ idx, const int* __restrict__ seg_start,
+                         int n_uniq, float* __restrict__ blk)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n_uniq) return;
+    const int e0 = seg_start[q], e1 = seg_start[q + 1];
+    double a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int e = e0; e < e1; ++e) {
+        const int v = idx[e];
+        const double* h = vals + 9L * (v >> 1);
+        if (v & 1) {
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) a[c * 3 + r] += h[r * 3 + c];
+        } else {
+#pragma unroll
+            for (int t = 0; t < 9; ++t) a[t] += h[t];
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 9; ++t) blk[9L * q + t] = (float)a[t];
+}
+
 // y = Minv x for the dense FP64 inverse of the level-2 operator (m x m,
 // row-major, symmetric); one wave per row.
 template <typename TS>
@@ -1231,7 +1447,7 @@ void Mas::assemble_invert_t(const double* vals, const int* rows, const int* cols
                                                                      coarse_c_[q].get(), cn, h_off[l], ndom0,
                                                                      mgal_.get());
             SGIPC_LAUNCH_CHECK();
-            if (l == 2 && coarse_mode == 1) build_dense_level2(s, coarse_v_[q].get(), coarse_r_[q].get(),
+            if (l == 2 && coarse_mode >= 1) build_dense_level2(s, coarse_v_[q].get(), coarse_r_[q].get(),
                                                               coarse_c_[q].get(), cn);
             if (l == 1 && want_graph()) {
                 graph_.build(s, coarse_v_[q].get(), coarse_r_[q].get(), coarse_c_[q].get(), cn, h_cnt[1]);
@@ -1331,6 +1547,51 @@ void Mas::learned_sample(hipStream_t s, const double* r, double* ztmp, float* X,
     SGIPC_LAUNCH_CHECK();
 }
 
+void Mas::build_ap(const double* vals, const int* rows, const int* cols, int nnz)
+{
+    hipStream_t s = ctx->stream;
+    SGIPC_CHECK(nnz < (1 << 30), SGIPC_CAPACITY, "A P entry count overflows int");
+    const int m = 2 * nnz;
+    int kb = 1;
+    while (kb < 31 && (1L << kb) <= (long)n) ++kb;  // row n (the sentinel) fits too
+    apk_.reserve(m);
+    apk_sorted_.reserve(m);
+    api_.reserve(m);
+    api_sorted_.reserve(m);
+    apf_.reserve(m);
+    apinc_.reserve(m);
+    size_t n1 = 0, n2 = 0;
+    SGIPC_HIP(rocprim::radix_sort_pairs(nullptr, n1, apk_.get(), apk_sorted_.get(), api_.get(), api_sorted_.get(),
+                                        (size_t)m, 0, 32 + kb, s));
+    SGIPC_HIP(rocprim::inclusive_scan(nullptr, n2, apf_.get(), apinc_.get(), (size_t)m, rocprim::plus<int>(), s));
+    ap_temp_.reserve(n1 > n2 ? n1 : n2);
+    k_ap_keys<<<grid_for(nnz, 256), 256, 0, s>>>(rows, cols, nnz, n, map_.get() + n, apk_.get(), api_.get());
+    SGIPC_LAUNCH_CHECK();
+    SGIPC_HIP(rocprim::radix_sort_pairs(ap_temp_.get(), n1, apk_.get(), apk_sorted_.get(), api_.get(),
+                                        api_sorted_.get(), (size_t)m, 0, 32 + kb, s));
+    k_ap_flags<<<grid_for(m, 256), 256, 0, s>>>(apk_sorted_.get(), m, n, apf_.get());
+    SGIPC_LAUNCH_CHECK();
+    SGIPC_HIP(rocprim::inclusive_scan(ap_temp_.get(), n2, apf_.get(), apinc_.get(), (size_t)m, rocprim::plus<int>(),
+                                      s));
+    int n_uniq = 0;
+    SGIPC_HIP(hipMemcpyAsync(&n_uniq, apinc_.get() + (m - 1), sizeof(int), hipMemcpyDeviceToHost, s));
+    SGIPC_HIP(hipStreamSynchronize(s));
+    SGIPC_CHECK(n_uniq > 0, SGIPC_EMPTY, "A P has no entries");
+    ap_ptr_.reserve(n + 1);
+    ap_col_.reserve(n_uniq);
+    ap_seg_.reserve(n_uniq + 1);
+    ap_blk_.reserve(9L * n_uniq);
+    k_ap_cols<<<grid_for(m, 256), 256, 0, s>>>(apk_sorted_.get(), apf_.get(), apinc_.get(), m, ap_col_.get(),
+                                               ap_seg_.get());
+    SGIPC_LAUNCH_CHECK();
+    k_ap_ptr<<<grid_for(n + 1, 256), 256, 0, s>>>(apk_sorted_.get(), apinc_.get(), m, n, n_uniq, ap_ptr_.get(),
+                                                  ap_seg_.get());
+    SGIPC_LAUNCH_CHECK();
+    k_ap_sum<<<grid_for(n_uniq, 256), 256, 0, s>>>(vals, api_sorted_.get(), ap_seg_.get(), n_uniq, ap_blk_.get());
+    SGIPC_LAUNCH_CHECK();
+    ap_nnz_ = n_uniq;
+}
+
 void Mas::init_matrix()
 {
     SGIPC_CHECK(n >= 1, SGIPC_EMPTY, "init_neighbor first");
@@ -1350,6 +1611,14 @@ void Mas::set_bcoo(const double* vals, const int* rows, const int* cols, int nnz
     build_hierarchy(pairs, n_pairs);
     if (B == 32) assemble_invert_t<32>(vals, rows, cols, nnz, offset);
     else assemble_invert_t<16>(vals, rows, cols, nnz, offset);
+    ap_nnz_ = 0;
+    if (coarse_mode == 2 && offset == 0 && L > 2 && nnz > 0) {
+        build_ap(vals, rows, cols, nnz);
+        if (build_own_fine) {
+            own_fine_.build(ctx->stream, vals, rows, cols, nnz, n, precision != SGIPC_FP64);
+            fine_ = &own_fine_;
+        }
+    }
     SGIPC_HIP(hipStreamSynchronize(ctx->stream));
     ms_build = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     built = true;
@@ -1374,7 +1643,88 @@ int Mas::apply_t(hipStream_t s, const TV* r, TV* z, double* rz_part, const PcgSt
     const int npad = h_off[1];
     k_restrict0<BB, TV, TS><<<grid_for(npad, 256), 256, 0, s>>>(sd, meta_.get(), n, L, map_.get() + n, r, R);
     SGIPC_LAUNCH_CHECK();
+    if (vcycle_active(sizeof(TS) == 4)) return finish_v_t<BB, TV, TS>(s, R, Z, minv, done, PcgStopDesc{}, r, z, rz_part);
     return finish_t<BB, TV, TS>(s, R, Z, minv, done, PcgStopDesc{}, r, z, rz_part);
+}
+
+template <int BB, typename TV, typename TS>
+int Mas::finish_v_t(hipStream_t s, TS* R, TS* Z, const TS* minv, const int* done, const PcgStopDesc& sd2, const TV* r,
+                    TV* z, double* rz_part)
+{
+    // R holds r at the level-0 slots (the identity slot map: slot u = row u)
+    const int ndom0 = h_off[1] / BB;
+    const int ndom01 = h_off[2] / BB;
+    const int ndom = total / BB;
+    const int npad = h_off[1];
+    const int c1 = h_cnt[1];
+    TS* G1;
+    TS* C2;
+    if constexpr (sizeof(TS) == 8) {
+        G1 = g1d_.get();
+        C2 = c2d_.get();
+    } else {
+        G1 = g1f_.get();
+        C2 = c2f_.get();
+    }
+    const TS w = (TS)omega0;
+    // pre-smoothing: Z0 = M0 r; then R0 = r1 = r - omega A Z0
+    k_local_solve<BB, TS><<<ndom0, 256, 0, s>>>(done, minv, R, Z, 0);
+    SGIPC_LAUNCH_CHECK();
+    fine_->template apply<TS>(s, -w, Z, TS(1), R, nullptr, done);
+    // level-1 sums of r1
+    k_restrict0<BB, TS, TS><<<grid_for(npad, 256), 256, 0, s>>>(PcgStopDesc{}, meta_.get(), n, L, map_.get() + n, R, R);
+    SGIPC_LAUNCH_CHECK();
+    // the level-1 cycle on P^T r1: its correction is C2 + Z1 (Z1 = Z at the level-1 slots)
+    const int g1w = grid_for(c1, 4);
+    k_local_solve<BB, TS><<<ndom01 - ndom0, 256, 0, s>>>(done, minv, R, Z, ndom0);
+    SGIPC_LAUNCH_CHECK();
+    k_l1_residual<TS, false><<<g1w, 256, 0, s>>>(done, meta_.get(), c1, graph_.row_ptr.get(), graph_.col.get(),
+                                                  graph_.blk.get(), graph_.s.get(), R, Z, G1, C2);
+    SGIPC_LAUNCH_CHECK();
+    auto restrict_up_level = [&](int l) {
+        int c = h_cnt[l - 1];
+        int pad = h_off[l + 1] - h_off[l] - h_cnt[l];
+        int work = c > pad ? c : pad;
+        k_restrict_up<TS><<<grid_for(work > 0 ? work : 1, 256), 256, 0, s>>>(
+            done, meta_.get(), l, going_next_.get(), R, l == 2 ? sd2 : PcgStopDesc{});
+        SGIPC_LAUNCH_CHECK();
+    };
+    int Lc = L;
+    if (dense_m_ > 0) {
+        restrict_up_level(2);
+        k_dense_gemv<TS><<<grid_for(dense_m_, 4), 256, 0, s>>>(done, dense_m_, dense_inv_.get(), R + 3L * h_off[2],
+                                                                Z + 3L * h_off[2]);
+        SGIPC_LAUNCH_CHECK();
+        Lc = 3;
+    } else {
+        for (int l = 2; l < L; ++l) restrict_up_level(l);
+        if (ndom > ndom01) {
+            k_local_solve<BB, TS><<<ndom - ndom01, 256, 0, s>>>(done, minv, R, Z, ndom01);
+            SGIPC_LAUNCH_CHECK();
+        }
+    }
+    k_l1_chain<TS><<<grid_for(c1, 256), 256, 0, s>>>(done, meta_.get(), Lc, going_next_.get(), c1, Z, C2);
+    SGIPC_LAUNCH_CHECK();
+    k_l1_residual<TS, true><<<g1w, 256, 0, s>>>(done, meta_.get(), c1, graph_.row_ptr.get(), graph_.col.get(),
+                                                 graph_.blk.get(), graph_.s.get(), R, Z, G1, C2);
+    SGIPC_LAUNCH_CHECK();
+    k_local_solve<BB, TS><<<ndom01 - ndom0, 256, 0, s>>>(done, minv, R, Z, ndom0);
+    SGIPC_LAUNCH_CHECK();
+    // post-smoothing of r2 = r1 - A P c, as one pass of solves over r + r2:
+    // Z0 = M0 (r + r2), z = omega Z0 + P c
+    using V4 = typename Vec4Of<TS>::T;
+    c4_.reserve((size_t)c1);  // double4 elements: room for c1 float4 too
+    V4* c4 = reinterpret_cast<V4*>(c4_.get());
+    k_l1_corr4<TS><<<grid_for(c1, 256), 256, 0, s>>>(done, c1, C2, Z + 3L * h_off[1], c4);
+    SGIPC_LAUNCH_CHECK();
+    k_local_solve_post<BB, TV, TS><<<ndom0, 256, 0, s>>>(done, minv, R, Z, n, ap_ptr_.get(), ap_col_.get(),
+                                                          ap_blk_.get(), c4, r);
+    SGIPC_LAUNCH_CHECK();
+    const int g = vec_grid(n);
+    k_prolong<TV, TS><<<g, 256, 0, s>>>(done, meta_.get(), n, L, map_.get(), Z, r, z, rz_part, C2, nullptr,
+                                        static_cast<const TV*>(flex_Ap_), flex_zap_, w);
+    SGIPC_LAUNCH_CHECK();
+    return g;
 }
 
 template <int BB, typename TV, typename TS>
@@ -1464,7 +1814,7 @@ int Mas::finish_t(hipStream_t s, TS* R, TS* Z, const TS* minv, const int* done, 
     }
     const int g = vec_grid(n);
     k_prolong<TV, TS><<<g, 256, 0, s>>>(done, meta_.get(), n, L, map_.get(), Z, r, z, rz_part, C2, corr,
-                                        static_cast<const TV*>(flex_Ap_), flex_zap_);
+                                        static_cast<const TV*>(flex_Ap_), flex_zap_, TS(1));
     SGIPC_LAUNCH_CHECK();
     return g;
 }
@@ -1492,6 +1842,7 @@ int Mas::apply_xr_t(hipStream_t s, const FusedXR<TV>& fx, TV* z, double* rz_part
     SGIPC_LAUNCH_CHECK();
     *n_rr = g0;
     const PcgStopDesc sd{fx.ctl, fx.rz, fx.n_rz, fx.rr, g0};
+    if (vcycle_active(sizeof(TS) == 4)) return finish_v_t<BB, TV, TS>(s, R, Z, minv, done, sd, fx.r, z, rz_part);
     return finish_t<BB, TV, TS>(s, R, Z, minv, done, sd, fx.r, z, rz_part);
 }
 

@@ -4,6 +4,7 @@
 #include "mpnn.hpp"
 #include "pcg.hpp"
 #include "sgipc_internal.hpp"
+#include "spmv_full.hpp"
 
 namespace sgipc {
 
@@ -36,16 +37,31 @@ class Mas {
     int levels(int* offsets, int* counts) const;
     // 0: additive levels (MASPreconditioner::preconditioning); 1: a symmetric
     // V(1,1) cycle on level 1 (level-1 domain solves as smoother, the levels
-    // >= 2 as its coarse correction) with the level-0 domain solves additive.
-    // Takes effect at the next set_bcoo.
+    // >= 2 as its coarse correction) with the level-0 domain solves additive;
+    // 2: mode 1's level-1 cycle as the coarse solve of a symmetric two-grid
+    // cycle on level 0 (level-0 domain solves weighted by omega0 before and
+    // after it; needs the fine operator).  Takes effect at the next set_bcoo.
     int coarse_mode = 0;
+    double omega0 = 0.5;
+    // the fine operator of coarse mode 2 (the PCG matrix, with its FP32 copy
+    // when the apply is FP32); set_bcoo builds its own unless told not to
+    const FullMatrix* fine_ = nullptr;
+    bool build_own_fine = true;
+    bool vcycle_active(bool f32) const
+    {
+        return coarse_mode == 2 && cycle_active() && ap_nnz_ > 0 && fine_ != nullptr && fine_->n_rows == n &&
+               (!f32 || fine_->has_f32);
+    }
     bool keep_graph = false;  // build the level-1 graph also without the cycle
     // flexible PCG: z.Ap partials written by the prolongation when set
     const void* flex_Ap_ = nullptr;
     double* flex_zap_ = nullptr;
     // learned coarse correction (nullptr: off); needs the level-1 graph
     Mpnn* net_ = nullptr;
-    bool learned_active() const { return net_ != nullptr && graph_.n == h_cnt[1] && h_cnt[1] > 0 && L > 1; }
+    bool learned_active() const
+    {
+        return net_ != nullptr && coarse_mode != 2 && graph_.n == h_cnt[1] && h_cnt[1] > 0 && L > 1;
+    }
     // training sample of residual r: features X (3 c_1 rows x 3) and target q
     // (3 c_1), normalised by the RMS of the first feature; ztmp: 3n scratch
     void learned_sample(hipStream_t s, const double* r, double* ztmp, float* X, float* q);
@@ -76,10 +92,28 @@ class Mas {
     template <int BB, typename TV, typename TS>
     int finish_t(hipStream_t s, TS* R, TS* Z, const TS* minv, const int* done, const PcgStopDesc& sd2, const TV* r,
                  TV* z, double* rz_part);
-    bool want_graph() const { return coarse_mode == 1 || keep_graph; }
+    // coarse mode 2 from the level-0 slots of r on
+    template <int BB, typename TV, typename TS>
+    int finish_v_t(hipStream_t s, TS* R, TS* Z, const TS* minv, const int* done, const PcgStopDesc& sd2,
+                   const TV* r, TV* z, double* rz_part);
+    bool want_graph() const { return coarse_mode >= 1 || keep_graph; }
     // the cycle needs level 1 to span several domains (one domain is solved
     // exactly already) and a coarser level
-    bool cycle_active() const { return coarse_mode == 1 && L > 2 && graph_.n == h_cnt[1] && h_cnt[1] > B; }
+    bool cycle_active() const
+    {
+        return coarse_mode >= 1 && L > 2 && graph_.n == h_cnt[1] && h_cnt[1] > B;
+    }
+    // A P for the post-smoothing residual of coarse mode 2 (CSR over the fine
+    // rows, level-1 columns, FP32 3x3 blocks)
+    void build_ap(const double* vals, const int* rows, const int* cols, int nnz);
+    DevBuf<int> ap_ptr_, ap_col_, ap_seg_;
+    DevBuf<float> ap_blk_;
+    DevBuf<double4> c4_;  // level-1 correction, 4-padded (float4 view in FP32 applies)
+    int ap_nnz_ = 0;
+    DevBuf<uint64_t> apk_, apk_sorted_;
+    DevBuf<int> api_, api_sorted_, apf_, apinc_;
+    DevBuf<unsigned char> ap_temp_;
+    FullMatrix own_fine_;
 
     void build_dense_level2(hipStream_t s, const double* v, const int* r, const int* c, int nb);
     static constexpr long kDenseMax = 4608;  // largest dense level-2 system (3 c_2)

@@ -247,7 +247,8 @@ __global__ __launch_bounds__(kThreads) void k_spmv_halo(const TM* __restrict__ v
     __shared__ int s_row[kTile];  // row of each tile position, relative to the tile (-1: padding)
     __shared__ int s_sb[kSlices + 1];
     __shared__ double dred[16];
-    constexpr int kU = sizeof(TM) == 4 ? 2 : 1;  // slots per step
+    // slots per step (one for the fused p update, whose staging needs more registers)
+    constexpr int kU = (sizeof(TM) == 4 && !kFuseP) ? 2 : 1;
     if (done != nullptr && *done) return;
     double beta = 0.0;
     if (kFuseP) beta = block_sum_array(rzn, n_rz, dred) / block_sum_array(rzo, n_rz, dred);

@@ -401,6 +401,7 @@ int system_solve(sgipc_system* sys, const double* b, double* x, double* grad, co
         const int mprec = (o.precision == SGIPC_FP64) ? SGIPC_FP64 : SGIPC_FP32_MAS;
         const bool learned = record == nullptr && use_learned(sys);
         sys->mas->m->keep_graph = sys->net != nullptr || record != nullptr;
+        sys->mas->m->build_own_fine = false;  // the PCG matrix serves as coarse mode 2's fine operator
         sys->mas->m->set_bcoo(sys->bvals.get(), sys->brows.get(), sys->bcols.get(), sys->nnz, off0, d.pairs,
                               d.n_pairs, mprec);
         if (nb > 0) sys->abdj.build(s, sys->bvals.get(), sys->brows.get(), sys->bcols.get(), sys->nnz, nb);

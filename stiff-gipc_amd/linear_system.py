@@ -114,8 +114,13 @@ class MASPreconditioner:
 
     def set_coarse_mode(self, mode):
         """0: additive levels (MASPreconditioner::preconditioning); 1: a
-        symmetric V(1,1) cycle on level 1.  Applies at the next build."""
+        symmetric V(1,1) cycle on level 1; 2: that cycle as the coarse solve
+        of a symmetric two-grid cycle on level 0.  Applies at the next build."""
         check(lib().sgipc_mas_set_coarse_mode(self.h, int(mode)), self.ctx.h)
+
+    def set_smoothing_weight(self, omega):
+        """Weight of the level-0 smoothing in coarse mode 2 (default 0.5)."""
+        check(lib().sgipc_mas_set_smoothing_weight(self.h, float(omega)), self.ctx.h)
 
     def initPreconditioner_Neighbor(self, vertNum, collision_node_offset, totalNeighborNum, nbr_start, nbr_num,
                                     nbr_list):
@@ -333,8 +338,13 @@ class LinearSystem:
         self.set_coarse_mode(coarse_mode)
 
     def set_coarse_mode(self, mode):
-        """0: additive MAS levels (the reference); 1: V(1,1) cycle on level 1."""
+        """0: additive MAS levels (the reference); 1: V(1,1) cycle on level 1;
+        2: two-grid cycle on level 0 around it."""
         check(lib().sgipc_mas_set_coarse_mode(self.sys.mas_handle(), int(mode)), self.ctx.h)
+
+    def set_smoothing_weight(self, omega):
+        """Weight of the level-0 smoothing in coarse mode 2 (default 0.5)."""
+        check(lib().sgipc_mas_set_smoothing_weight(self.sys.mas_handle(), float(omega)), self.ctx.h)
 
     def solve(self, tol=1e-5, precision="fp64", stop_mode=STOP_TRUE_RESIDUAL, want_grad=False, **kw):
         st, tm = self.sys.solve_linear_system(tol=tol, precision=precision, stop_mode=stop_mode, want_grad=want_grad,

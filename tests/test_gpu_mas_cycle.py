@@ -99,3 +99,101 @@ def test_cycle_solves_in_fewer_iterations(pkg, B):
     finally:
         plain.close()
         cyc.close()
+
+
+# ---- coarse mode 2: two-grid cycle on level 0 around the level-1 cycle
+def _vcycle_mas(pkg, ctx, s, B, vals, rows, cols, precision, omega=None):
+    nbr = [ctx.to_device(a, np.uint32) for a in (s.nbr_start, s.nbr_num,
+                                                 s.nbr_list if s.nbr_list.size else np.zeros(1, np.uint32))]
+    mas = pkg.MASPreconditioner(ctx, B)
+    mas.set_coarse_mode(2)
+    if omega is not None:
+        mas.set_smoothing_weight(omega)
+    mas.initPreconditioner_Neighbor(s.n, 0, int(s.nbr_list.size), *nbr)
+    mas.initPreconditioner_Matrix()
+    dv = ctx.to_device(vals, np.float64)
+    dr = ctx.to_device(rows, np.int32)
+    dc = ctx.to_device(cols, np.int32)
+    pairs = ctx.to_device(s.pairs.reshape(-1), np.int32) if s.n_pairs else None
+    mas.setPreconditioner_bcoo(dv, dr, dc, rows.size, 0, pairs, s.n_pairs, precision)
+    return mas, (nbr, dv, dr, dc, pairs)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("B", [16, 32])
+@pytest.mark.parametrize("which", ["small", "c0"])
+def test_vcycle_apply_matches_oracle(pkg, orc, gpu_ctx, B, which, precision, scene_small, scene_c0):
+    """The device apply of coarse mode 2 against mpnn.mas_vcycle_apply with the
+    device's own domain inverses.  FP64: what remains is summation order and
+    the float32 level-1 operator and A P (rounded the same way in both).
+    FP32 (slots, inverses, the fine SpMV's values and vectors in float32):
+    within 2e-4 of the FP64 restatement."""
+    s = {"small": scene_small, "c0": scene_c0}[which]
+    vals, rows, cols = orc.assemble(s)
+    om = orc.Mas(s, B)
+    om.build(vals, rows, cols, 0, s.pairs)
+    parts = mpnn.MasParts(om)
+    if parts.L <= 2 or parts.cnt[1] <= B:
+        pytest.skip("level 1 is one domain (solved exactly): the cycle stays off")
+    prec = pkg.FP64 if precision == "fp64" else pkg.FP32_MAS
+    mas, keep = _vcycle_mas(pkg, gpu_ctx, s, B, vals, rows, cols, prec, omega=0.45)
+    try:
+        parts.inv = mpnn._domain_dense(mas.inverse_blocks().reshape(-1, B * (B + 1) // 2, 9), B)
+        A = mpnn.bcoo_to_csr(vals, rows, cols, s.n)
+        graph = mpnn.coarse_graph(A, parts.map1, parts.cnt[1])
+        A1f = mpnn.float_operator(graph)
+        A2 = mpnn.level2_operator(parts, graph)
+        APf = mpnn.ap_operator(A, parts.map1, parts.cnt[1])
+        rng = np.random.default_rng(5)
+        for k in range(3):
+            r = rng.standard_normal(3 * s.n)
+            ref = mpnn.mas_vcycle_apply(parts, A1f, A, APf, r, A2=A2, omega=0.45)
+            dz = gpu_ctx.empty(3 * s.n, np.float64)
+            mas.preconditioning(gpu_ctx.to_device(r), dz)
+            got = dz.download()
+            err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+            assert err <= (1e-7 if precision == "fp64" else 2e-4), (k, err)
+    finally:
+        mas.close()
+        om.close()
+
+
+def test_vcycle_operator_is_spd(pkg, orc, gpu_ctx):
+    s = pkg.scenes.tet_grid_scene("small_grid8", 8, E_loguniform=(1e4, 1e9), n_pairs=40)
+    B = 16
+    vals, rows, cols = orc.assemble(s)
+    mas, keep = _vcycle_mas(pkg, gpu_ctx, s, B, vals, rows, cols, pkg.FP64)
+    try:
+        n3 = 3 * s.n
+        M = np.zeros((n3, n3))
+        dz = gpu_ctx.empty(n3, np.float64)
+        for j in range(n3):
+            e = np.zeros(n3)
+            e[j] = 1.0
+            mas.preconditioning(gpu_ctx.to_device(e), dz)
+            M[:, j] = dz.download()
+        # A P in float32 makes the two residual paths differ by ~1e-7
+        asym = np.abs(M - M.T).max() / np.abs(M).max()
+        assert asym <= 1e-6, asym
+        assert np.linalg.eigvalsh(0.5 * (M + M.T)).min() > 0
+    finally:
+        mas.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
+@pytest.mark.parametrize("B", [16, 32])
+def test_vcycle_solves_in_fewer_iterations(pkg, B, precision):
+    s = pkg.scenes.tet_grid_scene("cycle_grid16", 16, E_loguniform=(1e4, 1e9), n_pairs=800)
+    cyc = pkg.LinearSystem(s, bank_size=B, coarse_mode=1)
+    two = pkg.LinearSystem(s, bank_size=B, coarse_mode=2)
+    try:
+        a = cyc.solve(tol=1e-5, precision=precision)
+        b = two.solve(tol=1e-5, precision=precision)
+        print(f"B={B} {precision}: level-1 cycle {a.iterations} iterations, two-grid on level 0 {b.iterations}")
+        assert a.status == 0 and b.status == 0
+        assert b.true_rel_residual <= 1e-5
+        assert b.iterations < a.iterations
+        assert np.linalg.norm(a.x - b.x) <= 1e-4 * np.linalg.norm(a.x)
+    finally:
+        cyc.close()
+        two.close()
