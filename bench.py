@@ -8,7 +8,9 @@ conversion to unique upper BCOO, MAS rebuild (hierarchy, Galerkin, FP64
 inverses, dense FP64 level-2 inverse) and PCG until ||b - A x|| <= 1e-5 ||b||
 (checked in FP64).
 
-Default configuration: MAS B=32 with the level-1 V(1,1) cycle; mixed
+Default configuration: MAS B=32 with the level-1 V(1,1) cycle as the coarse
+solve of a symmetric two-grid cycle on level 0 (level-0 domain solves
+weighted 0.5 before and after it); mixed
 precision -- the SpMV reads FP32 matrix values and the MAS applies FP32
 inverse blocks, while vectors, dot products, alpha/beta and the MAS
 factorisations stay FP64, inside FP64 refinement (the recursive residual is
@@ -86,7 +88,7 @@ def main():
     ap.add_argument("--grid", type=int, default=64)
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--coarse", type=int, default=1, choices=[0, 1, 2],
+    ap.add_argument("--coarse", type=int, default=2, choices=[0, 1, 2],
                     help="MAS coarse levels: 0 additive (the reference), 1 V(1,1) cycle on level 1, "
                          "2 that cycle inside a two-grid cycle on level 0")
     ap.add_argument("--omega", type=float, default=None,
@@ -375,6 +377,7 @@ def train_learned(pkg, ls, args):
         "collect_s": round(t1 - t0, 2), "train_s": round(t2 - t1, 2),
         "loss_first_epoch": float(hist[0]), "loss_last_epoch": float(hist[-1]),
         "probe_iterations": {("learned" if r.learned else "plain"): r.iterations for r in probe},
+        "probe_learned_abandoned": any(r.learned_abandoned for r in probe),
     }
     log(f"learned level: {n_samples} samples ({t1 - t0:.1f}s), trained {args.train_epochs} epochs ({t2 - t1:.1f}s), "
         f"loss {hist[0]:.3e} -> {hist[-1]:.3e}; probe {info['probe_iterations']}")

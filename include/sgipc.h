@@ -312,7 +312,9 @@ typedef struct sgipc_pcg_stats {
     double rz0;
     double ms_total;         /* wall time of the call                                  */
     int status;
-    int learned;             /* 1 when the learned coarse correction was applied        */
+    int learned;             /* 1: the learned coarse correction was applied; 2: a run  */
+                             /* with it did not converge within twice the plain count   */
+                             /* (automatic mode) and the solve was redone without it    */
 } sgipc_pcg_stats;
 
 void sgipc_pcg_default_opts(sgipc_pcg_opts* opts);
@@ -456,8 +458,10 @@ int sgipc_system_collect(sgipc_system* sys, const double* b, const sgipc_pcg_opt
  * 2: automatic -- a solve without and one with the correction, then the one
  * with fewer iterations (falls back to plain MAS whenever the learned
  * iteration count is worse); the other is measured again 8 solves later, the
- * gap doubling each time the learned level loses.  stats.learned reports
- * which one ran. */
+ * gap doubling each time the learned level loses.  A run with the correction
+ * is capped at twice the plain count plus 16 iterations: when it does not
+ * converge there it loses and the same call solves again without it.
+ * stats.learned reports which one ran. */
 int sgipc_system_set_learned(sgipc_system* sys, sgipc_mpnn* net, int mode);
 
 #ifdef __cplusplus

@@ -1710,6 +1710,19 @@ int Mas::finish_v_t(hipStream_t s, TS* R, TS* Z, const TS* minv, const int* done
     SGIPC_LAUNCH_CHECK();
     k_local_solve<BB, TS><<<ndom01 - ndom0, 256, 0, s>>>(done, minv, R, Z, ndom0);
     SGIPC_LAUNCH_CHECK();
+    // learned correction (flexible PCG), from the level-1 state of the cycle
+    const float* corr = nullptr;
+    if (learned_active()) {
+        lX_.reserve(9L * c1);
+        lchat_.reserve(3L * c1);
+        lcorr_.reserve(3L * c1);
+        k_l1_features<TS><<<grid_for(c1, 4), 256, 0, s>>>(done, meta_.get(), L, going_next_.get(), c1,
+                                                           graph_.row_ptr.get(), graph_.col.get(), graph_.blk.get(),
+                                                           graph_.s.get(), R, Z, G1, C2, lX_.get());
+        SGIPC_LAUNCH_CHECK();
+        net_->forward(s, graph_, lX_.get(), 1, lchat_.get(), lcorr_.get(), done);
+        corr = lcorr_.get();
+    }
     // post-smoothing of r2 = r1 - A P c, as one pass of solves over r + r2:
     // Z0 = M0 (r + r2), z = omega Z0 + P c
     using V4 = typename Vec4Of<TS>::T;
@@ -1721,7 +1734,7 @@ int Mas::finish_v_t(hipStream_t s, TS* R, TS* Z, const TS* minv, const int* done
                                                           ap_blk_.get(), c4, r);
     SGIPC_LAUNCH_CHECK();
     const int g = vec_grid(n);
-    k_prolong<TV, TS><<<g, 256, 0, s>>>(done, meta_.get(), n, L, map_.get(), Z, r, z, rz_part, C2, nullptr,
+    k_prolong<TV, TS><<<g, 256, 0, s>>>(done, meta_.get(), n, L, map_.get(), Z, r, z, rz_part, C2, corr,
                                         static_cast<const TV*>(flex_Ap_), flex_zap_, w);
     SGIPC_LAUNCH_CHECK();
     return g;

@@ -60,7 +60,7 @@ class Mas {
     Mpnn* net_ = nullptr;
     bool learned_active() const
     {
-        return net_ != nullptr && coarse_mode != 2 && graph_.n == h_cnt[1] && h_cnt[1] > 0 && L > 1;
+        return net_ != nullptr && graph_.n == h_cnt[1] && h_cnt[1] > 0 && L > 1;
     }
     // training sample of residual r: features X (3 c_1 rows x 3) and target q
     // (3 c_1), normalised by the RMS of the first feature; ztmp: 3n scratch

@@ -5,6 +5,7 @@
 // triangles, mass; StiffGIPC/GIPC.cu:10291-10485), conversion to unique upper
 // BCOO, MAS rebuild with the current contact pairs, PCG.
 #include <chrono>
+#include <limits>
 #include <cmath>
 #include <memory>
 #include <vector>
@@ -417,17 +418,38 @@ int system_solve(sgipc_system* sys, const double* b, double* x, double* grad, co
         extra.flexible = learned;
         sys->mas->m->net_ = learned ? mpnn_of(sys->net) : nullptr;
         if (learned && !sys->mas->m->learned_active()) sys->mas->m->net_ = nullptr;
-        const bool used = sys->mas->m->net_ != nullptr;
+        bool used = sys->mas->m->net_ != nullptr;
         extra.flexible = used;
-        try {
-            status = pcg_solve_impl(sys->ctx->c, *sys->ws, sys->bvals.get(), sys->brows.get(), sys->bcols.get(),
-                                    sys->nnz, sys->n_sys, prec, b, x, o, &pst, &extra);
-        } catch (...) {
+        // automatic mode: a run with the learned correction gets at most twice
+        // the plain variant's iterations; one that does not converge there has
+        // lost, and the call solves again without it
+        const bool capped = used && sys->learned_mode == 2 && sys->plain_it > 0;
+        sgipc_pcg_opts oc = o;
+        if (capped) {
+            const int cap = 2 * sys->plain_it + 16;
+            if (oc.max_iter <= 0 || oc.max_iter > cap) oc.max_iter = cap;
+        }
+        auto run_pcg = [&](const sgipc_pcg_opts& oo) {
+            try {
+                return pcg_solve_impl(sys->ctx->c, *sys->ws, sys->bvals.get(), sys->brows.get(), sys->bcols.get(),
+                                      sys->nnz, sys->n_sys, prec, b, x, oo, &pst, &extra);
+            } catch (...) {
+                sys->mas->m->net_ = nullptr;
+                throw;
+            }
+        };
+        status = run_pcg(capped ? oc : o);
+        bool abandoned = false;
+        if (capped && status != SGIPC_OK) {
+            sys->learned_it = std::numeric_limits<int>::max() / 2;
             sys->mas->m->net_ = nullptr;
-            throw;
+            extra.flexible = false;
+            used = false;
+            abandoned = true;
+            status = run_pcg(o);
         }
         sys->mas->m->net_ = nullptr;
-        pst.learned = used ? 1 : 0;
+        pst.learned = used ? 1 : (abandoned ? 2 : 0);
         if (record == nullptr && status == SGIPC_OK) {
             (used ? sys->learned_it : sys->plain_it) = pst.iterations;
             if (sys->learned_mode == 2 && sys->plain_it >= 0 && sys->learned_it >= 0 &&

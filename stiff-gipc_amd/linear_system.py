@@ -214,6 +214,7 @@ class SolveResult:
     status: int
     timing: dict
     learned: bool = False
+    learned_abandoned: bool = False  # a learned run missed twice the plain count; redone without it
 
 
 def upload_scene(ctx: Context, scene):
@@ -353,7 +354,7 @@ class LinearSystem:
         timing = {k: getattr(tm, k) for k, _ in SolveTiming._fields_}
         return SolveResult(x=x, iterations=st.iterations, outer_iterations=st.outer_iterations,
                            true_rel_residual=st.rel_residual, status=st.status, timing=timing,
-                           learned=bool(st.learned))
+                           learned=st.learned == 1, learned_abandoned=st.learned == 2)
 
     def collect(self, stride=1, max_samples=256, tol=1e-5, precision="fp64", b=None):
         """Training samples (MpnnData) from one plain solve of the scene's rhs

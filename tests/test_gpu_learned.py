@@ -60,7 +60,7 @@ def test_collected_samples_match_oracle(pkg, orc):
         ls.close()
 
 
-@pytest.mark.parametrize("coarse", [0, 1])
+@pytest.mark.parametrize("coarse", [0, 1, 2])
 def test_learned_level_on_a_held_out_scene(pkg, coarse):
     """Train on residuals of one seeded scene, solve another: the flexible
     solve with the correction reaches the tolerance, and the automatic mode
@@ -100,3 +100,30 @@ def test_learned_level_on_a_held_out_scene(pkg, coarse):
     finally:
         train.close()
         test.close()
+
+
+def test_automatic_mode_drops_a_harmful_correction(pkg):
+    """A network whose output is scaled by 1e4 spoils the flexible run: in
+    automatic mode that run is capped at twice the plain count, the call then
+    solves again without the correction (still converged), and the following
+    solves stay plain."""
+    ls = pkg.LinearSystem(_scene(pkg, 2), bank_size=16, coarse_mode=1)
+    try:
+        net = pkg.learned.Mpnn(ls.ctx, layers=2, linear=True, seed=3)
+        p = net.get_params()
+        p[-64:] *= 1e4  # the output weights
+        net.set_params(p)
+        ls.set_learned(net, mode=2)
+        plain = ls.solve(tol=1e-5)
+        probe = ls.solve(tol=1e-5)
+        assert not plain.learned
+        assert probe.status == 0 and probe.true_rel_residual <= 1e-5
+        if probe.learned:
+            assert probe.iterations <= 2 * plain.iterations + 16
+        else:
+            assert probe.learned_abandoned
+        after = [ls.solve(tol=1e-5) for _ in range(2)]
+        assert all(not r.learned and r.status == 0 for r in after)
+        net.close()
+    finally:
+        ls.close()
