@@ -10,7 +10,7 @@ inverses, dense FP64 level-2 inverse) and PCG until ||b - A x|| <= 1e-5 ||b||
 
 Default configuration: MAS B=32 with the level-1 V(1,1) cycle as the coarse
 solve of a symmetric two-grid cycle on level 0 (level-0 domain solves
-weighted 0.5 before and after it); mixed
+weighted 0.4 before and after it); mixed
 precision -- the SpMV reads FP32 matrix values and the MAS applies FP32
 inverse blocks, while vectors, dot products, alpha/beta and the MAS
 factorisations stay FP64, inside FP64 refinement (the recursive residual is
@@ -92,7 +92,7 @@ def main():
                     help="MAS coarse levels: 0 additive (the reference), 1 V(1,1) cycle on level 1, "
                          "2 that cycle inside a two-grid cycle on level 0")
     ap.add_argument("--omega", type=float, default=None,
-                    help="level-0 smoothing weight of --coarse 2 (library default 0.5)")
+                    help="level-0 smoothing weight of --coarse 2 (library default 0.4)")
     ap.add_argument("--learned", type=int, default=2, choices=[0, 1, 2],
                     help="learned coarse level: 0 off, 1 always, 2 automatic (falls back to plain MAS when worse); "
                          "the network is trained on a separately seeded scene before the timed solves")
@@ -323,7 +323,7 @@ def main():
                                 1: "level-1 V(1,1) cycle, exact dense level-2 solve",
                                 2: "two-grid cycle on level 0 (weighted level-0 domain solves before and after) "
                                    "around the level-1 V(1,1) cycle, exact dense level-2 solve"}[args.coarse],
-            "level0_smoothing_weight": (args.omega if args.omega is not None else 0.5) if args.coarse == 2 else None,
+            "level0_smoothing_weight": (args.omega if args.omega is not None else 0.4) if args.coarse == 2 else None,
             "learned": learned_info,
             "learned_used_in_timed_steps": out.get("learned_used"),
             "plain_fp64_b32_same_run": plain,

@@ -263,16 +263,19 @@ int sgipc_mas_tables(sgipc_mas* mas, int* coarse_table, int* going_next);
  * coarse solve of a symmetric two-grid cycle on level 0: z0 = w M0 r,
  * r1 = r - A z0, c = (level-1 cycle of P^T r1), r2 = r1 - A P c,
  * z = z0 + P c + w M0 r2, with M0 the level-0 domain solves, P the level-1
- * aggregation and w the smoothing weight (default 0.5).  It needs the fine
+ * aggregation and w the smoothing weight (default 0.4).  It needs the fine
  * matrix A inside the apply: the PCG solver supplies its own when the MAS
  * covers every row of the system (else the apply falls back to mode 1); a
  * standalone MAS keeps a copy made at set_preconditioner_bcoo.  Symmetric
- * positive definite for weights below 2 / lambda_max(M0 A).
+ * positive definite for weights below 2 / lambda_max(M0 A); PCG stops with
+ * SGIPC_NOT_SPD when r.z <= 0 shows that it is not, and the system solver
+ * then redoes the solve with half the weight (twice at most, the weight kept
+ * for later solves), then with mode 1.
  * Modes take effect at the next sgipc_mas_set_preconditioner_bcoo (or system
  * solve). */
 int sgipc_mas_set_coarse_mode(sgipc_mas* mas, int mode);
-/* Smoothing weight w of coarse mode 2, 0 < w < 2 (no counterpart in the
- * reference). */
+/* Smoothing weight w of coarse mode 2, 0 < w < 2 (default 0.4; no
+ * counterpart in the reference). */
 int sgipc_mas_set_smoothing_weight(sgipc_mas* mas, double omega);
 /* Measurement helper: average duration (ms) of one preconditioning (all of its
  * kernels) over `reps` applications, HIP events on the context's stream. */

@@ -42,7 +42,7 @@ class Mas {
     // cycle on level 0 (level-0 domain solves weighted by omega0 before and
     // after it; needs the fine operator).  Takes effect at the next set_bcoo.
     int coarse_mode = 0;
-    double omega0 = 0.5;
+    double omega0 = 0.4;
     // the fine operator of coarse mode 2 (the PCG matrix, with its FP32 copy
     // when the apply is FP32); set_bcoo builds its own unless told not to
     const FullMatrix* fine_ = nullptr;

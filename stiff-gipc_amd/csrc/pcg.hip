@@ -33,7 +33,8 @@ __device__ __forceinline__ double sum_partials(const double* __restrict__ a, int
 
 // Stop rule after the x/r update of iteration k (ctl->done set by block 0).
 // mode 0: |rz_k| <= tol * rz0 (the reference rule, evaluated with the rz used
-// to form alpha); mode 1: ||r||^2 <= tol^2 ||b||^2.
+// to form alpha); mode 1: ||r||^2 <= tol^2 ||b||^2.  rz_k <= 0 before either
+// holds stops the run with not_spd set.
 __device__ __forceinline__ bool eval_stop(const PcgStopDesc& sd, double* smem)
 {
     __shared__ int s_done;
@@ -49,12 +50,16 @@ __device__ __forceinline__ bool eval_stop(const PcgStopDesc& sd, double* smem)
         stop = fabs(rz) <= sd.ctl->tol * sd.ctl->rz0;
     else
         stop = rr <= sd.ctl->tol * sd.ctl->tol * sd.ctl->bb;
-    if (stop && blockIdx.x == 0 && threadIdx.x == 0) {
+    // r.z <= 0 short of convergence: the preconditioner is not positive
+    // definite on r, and the iteration cannot continue
+    const bool bad = !stop && !(rz > 0.0);
+    if ((stop || bad) && blockIdx.x == 0 && threadIdx.x == 0) {
         sd.ctl->done = 1;
         sd.ctl->final_rr = rr;
         sd.ctl->final_rz = rz;
+        if (bad) sd.ctl->not_spd = 1;
     }
-    return stop;
+    return stop || bad;
 }
 
 // x += alpha p; r -= alpha Ap; rr partials.  alpha = rz / pAp.

@@ -449,6 +449,28 @@ int system_solve(sgipc_system* sys, const double* b, double* x, double* grad, co
             status = run_pcg(o);
         }
         sys->mas->m->net_ = nullptr;
+        // coarse mode 2 is positive definite only for smoothing weights below
+        // 2 / lambda_max(M0 A): a run that meets r.z <= 0 is redone with half
+        // the weight (kept for later solves), and after two halvings with the
+        // level-1 cycle alone
+        Mas* mm = sys->mas->m.get();
+        if (status == SGIPC_NOT_SPD && mm->coarse_mode == 2 && !used) {
+            for (int tries = 0; tries < 3 && status == SGIPC_NOT_SPD; ++tries) {
+                if (tries < 2) {
+                    mm->omega0 *= 0.5;
+                    status = run_pcg(o);
+                } else {
+                    mm->coarse_mode = 1;
+                    try {
+                        status = run_pcg(o);
+                    } catch (...) {
+                        mm->coarse_mode = 2;
+                        throw;
+                    }
+                    mm->coarse_mode = 2;
+                }
+            }
+        }
         pst.learned = used ? 1 : (abandoned ? 2 : 0);
         if (record == nullptr && status == SGIPC_OK) {
             (used ? sys->learned_it : sys->plain_it) = pst.iterations;

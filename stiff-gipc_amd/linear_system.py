@@ -119,7 +119,7 @@ class MASPreconditioner:
         check(lib().sgipc_mas_set_coarse_mode(self.h, int(mode)), self.ctx.h)
 
     def set_smoothing_weight(self, omega):
-        """Weight of the level-0 smoothing in coarse mode 2 (default 0.5)."""
+        """Weight of the level-0 smoothing in coarse mode 2 (default 0.4)."""
         check(lib().sgipc_mas_set_smoothing_weight(self.h, float(omega)), self.ctx.h)
 
     def initPreconditioner_Neighbor(self, vertNum, collision_node_offset, totalNeighborNum, nbr_start, nbr_num,
@@ -344,7 +344,7 @@ class LinearSystem:
         check(lib().sgipc_mas_set_coarse_mode(self.sys.mas_handle(), int(mode)), self.ctx.h)
 
     def set_smoothing_weight(self, omega):
-        """Weight of the level-0 smoothing in coarse mode 2 (default 0.5)."""
+        """Weight of the level-0 smoothing in coarse mode 2 (default 0.4)."""
         check(lib().sgipc_mas_set_smoothing_weight(self.sys.mas_handle(), float(omega)), self.ctx.h)
 
     def solve(self, tol=1e-5, precision="fp64", stop_mode=STOP_TRUE_RESIDUAL, want_grad=False, **kw):

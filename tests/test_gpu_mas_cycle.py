@@ -197,3 +197,23 @@ def test_vcycle_solves_in_fewer_iterations(pkg, B, precision):
     finally:
         cyc.close()
         two.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
+def test_vcycle_recovers_from_a_too_large_weight(pkg, precision):
+    """With w = 1.9 the two-grid preconditioner is indefinite (w exceeds
+    2 / lambda_max(M0 A)): the solver must still return a converged solution
+    (PCG stops on r.z <= 0 and the system retries with half the weight)."""
+    s = pkg.scenes.tet_grid_scene("cycle_grid16", 16, E_loguniform=(1e4, 1e9), n_pairs=800)
+    ls = pkg.LinearSystem(s, bank_size=16, coarse_mode=2)
+    ref = pkg.LinearSystem(s, bank_size=16, coarse_mode=2)
+    try:
+        ls.set_smoothing_weight(1.9)
+        a = ls.solve(tol=1e-5, precision=precision)
+        b = ref.solve(tol=1e-5, precision=precision)
+        print(f"{precision}: w=1.9 start -> {a.iterations} iterations; default w: {b.iterations}")
+        assert a.status == 0 and a.true_rel_residual <= 1e-5
+        assert np.linalg.norm(a.x - b.x) <= 1e-4 * np.linalg.norm(b.x)
+    finally:
+        ls.close()
+        ref.close()
