@@ -14,7 +14,7 @@ weighted 0.4 before and after it); mixed
 precision -- the SpMV reads FP32 matrix values and the MAS applies FP32
 inverse blocks, while vectors, dot products, alpha/beta and the MAS
 factorisations stay FP64, inside FP64 refinement (the recursive residual is
-replaced by the FP64 one every 16 iterations, and the final residual is
+replaced by the FP64 one every 8 iterations, and the final residual is
 checked in FP64); the learned coarse level in automatic mode (trained before
 the timed solves on a separately seeded scene; used only when it lowers the
 iteration count).  The plain FP64 additive-MAS solve is timed in the same
@@ -100,7 +100,7 @@ def main():
     ap.add_argument("--train-epochs", type=int, default=10)
     ap.add_argument("--compare-plain", type=int, default=3,
                     help="steps of plain FP64 additive-MAS (B=32) solves timed in the same run for the speedup")
-    ap.add_argument("--ir-replace", type=int, default=16,
+    ap.add_argument("--ir-replace", type=int, default=8,
                     help="fp32_ir: iterations between replacements of the recursive residual by the FP64 one")
     ap.add_argument("--fuse", type=int, default=1, choices=[0, 1, 2],
                     help="0: separate x/r and p update kernels; 1: x/r update inside the first MAS kernel; "

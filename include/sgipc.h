@@ -303,7 +303,7 @@ typedef struct sgipc_pcg_opts {
                           the MAS on every row.  Same iterates as PCG for a fixed SPD
                           preconditioner, up to rounding. */
     int ir_replace_every; /* FP32_IR: replace the recursive residual by the FP64 one every
-                             that many iterations (default 16; 0: none, refinement by
+                             that many iterations (default 8; 0: none, refinement by
                              restarts only) */
 } sgipc_pcg_opts;
 

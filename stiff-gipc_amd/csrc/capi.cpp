@@ -462,7 +462,7 @@ void sgipc_pcg_default_opts(sgipc_pcg_opts* o)
     o->check_every = 8;
     o->fuse = 1;
     o->flexible = 0;
-    o->ir_replace_every = 16;
+    o->ir_replace_every = 8;
     o->ir_max_outer = 20;
     o->ir_inner_tol = 0.0;
 }

@@ -174,7 +174,7 @@ class MASPreconditioner:
 
 
 def pcg_opts(tol=1e-5, stop_mode=STOP_TRUE_RESIDUAL, max_iter=0, precision=FP64, check_every=8, ir_max_outer=20,
-             ir_inner_tol=0.0, fuse=1, flexible=False, ir_replace_every=16):
+             ir_inner_tol=0.0, fuse=1, flexible=False, ir_replace_every=8):
     o = PcgOpts()
     lib().sgipc_pcg_default_opts(ctypes.byref(o))
     o.tol = tol
