@@ -601,9 +601,10 @@ __global__ __launch_bounds__(256) void k_restrict0(PcgStopDesc sd, const int* __
 }
 
 // x += alpha p, r -= alpha Ap (alpha = rz / pAp from partials), ||r||^2
-// partials, then the restriction of the new r as in k_restrict0.  The stop
-// rule is left to the first coarse restriction, which sees the new rr.
-template <int B, typename TV, typename TS>
+// partials, then the restriction of the new r as in k_restrict0 (level 0
+// only with SUMS = false: coarse mode 2 restricts r1 later).  The stop rule is
+// left to the first coarse restriction, which sees the new rr.
+template <int B, typename TV, typename TS, bool SUMS = true>
 __global__ __launch_bounds__(256) void k_restrict0_xr(FusedXR<TV> fx, const int* __restrict__ meta, int n, int L,
                                                       const int* __restrict__ map1, TS* __restrict__ R)
 {
@@ -639,7 +640,7 @@ __global__ __launch_bounds__(256) void k_restrict0_xr(FusedXR<TV> fx, const int*
         R[3 * (long)s + 1] = v1;
         R[3 * (long)s + 2] = v2;
     }
-    {
+    if (SUMS) {
         int node = real ? map1[s] : -1;
         bool pending = node >= 0;
         const int lane = threadIdx.x & 63;
@@ -766,6 +767,17 @@ struct Vec16<float> {
     static constexpr int k = 4;
 };
 
+template <typename TS>
+struct Vec4Of;
+template <>
+struct Vec4Of<float> {
+    using T = float4;
+};
+template <>
+struct Vec4Of<double> {
+    using T = double4;
+};
+
 // Z[domain] = Minv[domain] * R[domain], one 256-thread workgroup per domain.
 // All loads of the domain's inverse blocks are issued before any LDS store so
 // the whole 19/38 KB domain is in flight at once.  Thread (rb, g) forms the
@@ -774,7 +786,9 @@ struct Vec16<float> {
 // a fixed order.
 template <int B, typename TS>
 __global__ __launch_bounds__(256) void k_local_solve(const int* __restrict__ done, const TS* __restrict__ minv,
-                                                      const TS* __restrict__ R, TS* __restrict__ Z, int dom0)
+                                                      const TS* __restrict__ R, TS* __restrict__ Z, int dom0,
+                                                      const TS* __restrict__ C2 = nullptr, void* __restrict__ c4 = nullptr,
+                                                      int off1 = 0, int c1 = 0)
 {
     if (done != nullptr && *done) return;
     constexpr int nblk = B * (B + 1) / 2;
@@ -831,35 +845,20 @@ __global__ __launch_bounds__(256) void k_local_solve(const int* __restrict__ don
         Z[o] = z0;
         Z[o + 1] = z1;
         Z[o + 2] = z2;
+        // level-1 post-smoothing of coarse mode 2: the cycle's correction
+        // C2 + Z1, packed per level-1 node for the post-smoothing's gathers
+        if (c4 != nullptr) {
+            const long k = (long)d * B + rb - off1;
+            if (k >= 0 && k < c1) {
+                typename Vec4Of<TS>::T v;
+                v.x = C2[3 * k] + z0;
+                v.y = C2[3 * k + 1] + z1;
+                v.z = C2[3 * k + 2] + z2;
+                v.w = 0;
+                static_cast<typename Vec4Of<TS>::T*>(c4)[k] = v;
+            }
+        }
     }
-}
-
-template <typename TS>
-struct Vec4Of;
-template <>
-struct Vec4Of<float> {
-    using T = float4;
-};
-template <>
-struct Vec4Of<double> {
-    using T = double4;
-};
-
-// c = C2 + Z1 per level-1 node, padded to 4 components: one aligned load per
-// gather in k_local_solve_post
-template <typename TS>
-__global__ void k_l1_corr4(const int* __restrict__ done, int c1, const TS* __restrict__ C2, const TS* __restrict__ Z1,
-                           typename Vec4Of<TS>::T* __restrict__ c4)
-{
-    if (done != nullptr && *done) return;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= c1) return;
-    typename Vec4Of<TS>::T v;
-    v.x = C2[3L * k] + Z1[3L * k];
-    v.y = C2[3L * k + 1] + Z1[3L * k + 1];
-    v.z = C2[3L * k + 2] + Z1[3L * k + 2];
-    v.w = 0;
-    c4[k] = v;
 }
 
 // Level-0 post-smoothing of coarse mode 2 with its right-hand side formed in
@@ -1714,7 +1713,10 @@ int Mas::finish_v_t(hipStream_t s, TS* R, TS* Z, const TS* minv, const int* done
     k_l1_residual<TS, true><<<g1w, 256, 0, s>>>(done, meta_.get(), c1, graph_.row_ptr.get(), graph_.col.get(),
                                                  graph_.blk.get(), graph_.s.get(), R, Z, G1, C2);
     SGIPC_LAUNCH_CHECK();
-    k_local_solve<BB, TS><<<ndom01 - ndom0, 256, 0, s>>>(done, minv, R, Z, ndom0);
+    using V4 = typename Vec4Of<TS>::T;
+    c4_.reserve((size_t)c1);  // double4 elements: room for c1 float4 too
+    V4* c4 = reinterpret_cast<V4*>(c4_.get());
+    k_local_solve<BB, TS><<<ndom01 - ndom0, 256, 0, s>>>(done, minv, R, Z, ndom0, C2, c4, h_off[1], c1);
     SGIPC_LAUNCH_CHECK();
     // learned correction (flexible PCG), from the level-1 state of the cycle
     const float* corr = nullptr;
@@ -1731,11 +1733,6 @@ int Mas::finish_v_t(hipStream_t s, TS* R, TS* Z, const TS* minv, const int* done
     }
     // post-smoothing of r2 = r1 - A P c, as one pass of solves over r + r2:
     // Z0 = M0 (r + r2), z = omega Z0 + P c
-    using V4 = typename Vec4Of<TS>::T;
-    c4_.reserve((size_t)c1);  // double4 elements: room for c1 float4 too
-    V4* c4 = reinterpret_cast<V4*>(c4_.get());
-    k_l1_corr4<TS><<<grid_for(c1, 256), 256, 0, s>>>(done, c1, C2, Z + 3L * h_off[1], c4);
-    SGIPC_LAUNCH_CHECK();
     k_local_solve_post<BB, TV, TS><<<ndom0, 256, 0, s>>>(done, minv, R, Z, n, ap_ptr_.get(), ap_col_.get(),
                                                           ap_blk_.get(), c4, r);
     SGIPC_LAUNCH_CHECK();
@@ -1857,7 +1854,10 @@ int Mas::apply_xr_t(hipStream_t s, const FusedXR<TV>& fx, TV* z, double* rz_part
     const int npad = h_off[1];
     const int g0 = grid_for(npad, 256);
     if (g0 > fx.rr_cap) return -1;
-    k_restrict0_xr<BB, TV, TS><<<g0, 256, 0, s>>>(fx, meta_.get(), n, L, map_.get() + n, R);
+    if (vcycle_active(sizeof(TS) == 4))
+        k_restrict0_xr<BB, TV, TS, false><<<g0, 256, 0, s>>>(fx, meta_.get(), n, L, map_.get() + n, R);
+    else
+        k_restrict0_xr<BB, TV, TS><<<g0, 256, 0, s>>>(fx, meta_.get(), n, L, map_.get() + n, R);
     SGIPC_LAUNCH_CHECK();
     *n_rr = g0;
     const PcgStopDesc sd{fx.ctl, fx.rz, fx.n_rz, fx.rr, g0};
