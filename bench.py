@@ -100,6 +100,8 @@ def main():
     ap.add_argument("--train-epochs", type=int, default=10)
     ap.add_argument("--compare-plain", type=int, default=3,
                     help="steps of plain FP64 additive-MAS (B=32) solves timed in the same run for the speedup")
+    ap.add_argument("--graph", type=int, default=0, choices=[0, 1],
+                    help="1: PCG replays its full chunks of iterations from a captured HIP graph")
     ap.add_argument("--ir-replace", type=int, default=8,
                     help="fp32_ir: iterations between replacements of the recursive residual by the FP64 one")
     ap.add_argument("--fuse", type=int, default=1, choices=[0, 1, 2],
@@ -143,7 +145,7 @@ def main():
         # ---- warm-up solves (also sizes every buffer)
         for w in range(args.warmup):
             r = ls.solve(tol=1e-5, precision=args.precision, fuse=args.fuse,
-                         ir_replace_every=args.ir_replace)
+                         ir_replace_every=args.ir_replace, graph=bool(args.graph))
             log(f"warmup {w}: {r.timing['ms_total']:.2f} ms iters={r.iterations} "
                 f"outer={r.outer_iterations} res={r.true_rel_residual:.2e}")
         # ---- timed steps
@@ -153,7 +155,7 @@ def main():
         for k in range(args.steps):
             s0 = time.perf_counter()
             r = ls.solve(tol=1e-5, precision=args.precision, fuse=args.fuse,
-                         ir_replace_every=args.ir_replace)
+                         ir_replace_every=args.ir_replace, graph=bool(args.graph))
             times.append((time.perf_counter() - s0) * 1e3)
             iters.append(r.iterations)
             res_list.append(r.true_rel_residual)
@@ -319,6 +321,7 @@ def main():
             "precision": args.precision,
             "bank_size": args.bank,
             "fused_iteration": args.fuse,
+            "pcg_graph_replay": bool(args.graph),
             "mas_coarse_mode": {0: "additive",
                                 1: "level-1 V(1,1) cycle, exact dense level-2 solve",
                                 2: "two-grid cycle on level 0 (weighted level-0 domain solves before and after) "
@@ -370,7 +373,8 @@ def train_learned(pkg, ls, args):
     tr.close()
     ls.set_learned(net, mode=args.learned)
     # the automatic mode measures both variants on its first two solves
-    probe = [ls.solve(tol=1e-5, precision=args.precision, ir_replace_every=args.ir_replace) for _ in range(2)]
+    probe = [ls.solve(tol=1e-5, precision=args.precision, ir_replace_every=args.ir_replace,
+                      graph=bool(args.graph)) for _ in range(2)]
     info = {
         "mode": {1: "always", 2: "automatic"}[args.learned],
         "layers": 3, "channels": 64, "samples": n_samples, "epochs": args.train_epochs,

@@ -305,6 +305,10 @@ typedef struct sgipc_pcg_opts {
     int ir_replace_every; /* FP32_IR: replace the recursive residual by the FP64 one every
                              that many iterations (default 8; 0: none, refinement by
                              restarts only) */
+    int graph;         /* 1: the kernels of each full chunk of check_every iterations
+                          after the first are captured once per solve into a HIP graph
+                          and replayed (fewer launch gaps; same kernels, same iterates);
+                          not with flexible PCG or residual recording.  0: eager. */
 } sgipc_pcg_opts;
 
 typedef struct sgipc_pcg_stats {

@@ -31,7 +31,7 @@ csz = ctypes.c_size_t
 class PcgOpts(ctypes.Structure):
     _fields_ = [("tol", cd), ("stop_mode", ci), ("max_iter", ci), ("precision", ci), ("check_every", ci),
                 ("ir_max_outer", ci), ("ir_inner_tol", cd), ("fuse", ci), ("flexible", ci),
-                ("ir_replace_every", ci)]
+                ("ir_replace_every", ci), ("graph", ci)]
 
 
 class PcgStats(ctypes.Structure):

@@ -463,6 +463,7 @@ void sgipc_pcg_default_opts(sgipc_pcg_opts* o)
     o->fuse = 1;
     o->flexible = 0;
     o->ir_replace_every = 8;
+    o->graph = 0;
     o->ir_max_outer = 20;
     o->ir_inner_tol = 0.0;
 }
@@ -504,6 +505,7 @@ int pcg_solve_impl(Context& c, SolveWorkspace& ws, const double* vals, const int
         po.max_iter = max_iter;
         po.check_every = o.check_every;
         po.fuse = o.fuse;
+        po.graph = o.graph != 0;
         po.flexible = o.flexible != 0;
         if (extra) {
             po.record = extra->record;
@@ -569,6 +571,7 @@ int pcg_solve_impl(Context& c, SolveWorkspace& ws, const double* vals, const int
             po.max_iter = max_iter - total_it;
             po.check_every = o.check_every;
             po.fuse = o.fuse;
+            po.graph = o.graph != 0;
             po.flexible = o.flexible != 0 || (extra && extra->flexible);
             po.true_op = &A64;
             po.replace_every = o.ir_replace_every;

@@ -386,10 +386,41 @@ PcgResult PcgSolver<TV>::solve(hipStream_t s, const SpmvOp<TV>& A, Preconditione
     int round = 0;
     const int max_it = opt.max_iter > 1 ? opt.max_iter - 1 : 1;  // the reference loops k=1..max_iter-1
     bool finished = false;
+    // HIP graph of one full chunk: from the second chunk on the loop's state
+    // (cur, pb, p_pending, fuse_xr, the replacement positions) is the same at
+    // every chunk boundary when the chunk length is even and a multiple of the
+    // replacement period, so one capture serves every later full chunk
+    bool use_graph = opt.graph && !flex && opt.record == nullptr && (chunk % 2 == 0) &&
+                     (!replace || chunk % opt.replace_every == 0);
+    hipGraphExec_t gexec = nullptr;
+    int n_chunks = 0;
     while (!finished) {
         int todo = max_it - issued;
         if (todo > chunk) todo = chunk;
-        if (todo > 0) {
+        const bool full = todo == chunk;
+        if (todo > 0 && use_graph && full && gexec != nullptr) {
+            SGIPC_HIP(hipGraphLaunch(gexec, s));
+            issued += todo;
+            for (int it = 0; it < todo; ++it) cur ^= 1;
+            if (fuse_p) pb ^= (todo & 1);
+        } else if (todo > 0) {
+            const bool capture = use_graph && full && n_chunks >= 1;
+            const int cur0 = cur, pb0 = pb;
+            const bool pp0 = p_pending, fx0 = fuse_xr;
+            if (capture) SGIPC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            // an error while capturing must close the capture before the
+            // stream is used again
+            struct CaptureGuard {
+                hipStream_t s;
+                bool open;
+                ~CaptureGuard()
+                {
+                    if (open) {
+                        hipGraph_t g = nullptr;
+                        if (hipStreamEndCapture(s, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+                    }
+                }
+            } guard{s, capture};
             for (int it = 0; it < todo; ++it) {
                 const int nxt = cur ^ 1;
                 int n_pap;
@@ -469,7 +500,29 @@ PcgResult PcgSolver<TV>::solve(hipStream_t s, const SpmvOp<TV>& A, Preconditione
                 }
                 cur = nxt;
             }
+            if (capture) {
+                hipGraph_t graph = nullptr;
+                guard.open = false;
+                SGIPC_HIP(hipStreamEndCapture(s, &graph));
+                const bool periodic = cur == cur0 && pb == pb0 && p_pending == pp0 && fuse_xr == fx0;
+                if (periodic && hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0) == hipSuccess) {
+                    SGIPC_HIP(hipGraphDestroy(graph));
+                    SGIPC_HIP(hipGraphLaunch(gexec, s));
+                } else {
+                    // not replayable: nothing ran during the capture, so the
+                    // chunk is issued again eagerly from its start state
+                    if (graph) (void)hipGraphDestroy(graph);
+                    gexec = nullptr;
+                    use_graph = false;
+                    cur = cur0;
+                    pb = pb0;
+                    p_pending = pp0;
+                    fuse_xr = fx0;
+                    continue;
+                }
+            }
             issued += todo;
+            ++n_chunks;
         }
         int slot = (round++) & 1;
         SGIPC_HIP(hipMemcpyAsync(host_ctl_ + slot, ctl, sizeof(PcgCtl), hipMemcpyDeviceToHost, s));
@@ -487,6 +540,7 @@ PcgResult PcgSolver<TV>::solve(hipStream_t s, const SpmvOp<TV>& A, Preconditione
         }
     }
     if (flex) M->set_flex(nullptr, nullptr);
+    if (gexec) (void)hipGraphExecDestroy(gexec);
     SGIPC_HIP(hipStreamSynchronize(s));
     SGIPC_HIP(hipMemcpy(host_ctl_, ctl, sizeof(PcgCtl), hipMemcpyDeviceToHost));
     const PcgCtl& c = host_ctl_[0];

@@ -55,6 +55,10 @@ struct PcgOptions {
     // recurrences continue
     const SpmvOp<double>* true_op = nullptr;
     int replace_every = 0;
+    // capture the kernels of a full chunk (check_every iterations, from the
+    // second chunk on, when the loop's state repeats chunk to chunk) into a
+    // HIP graph once per solve and replay it for the later full chunks
+    bool graph = false;
 };
 
 // Inputs of the x/r update (x += alpha p, r -= alpha Ap, alpha = rz/pAp) that

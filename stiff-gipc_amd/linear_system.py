@@ -174,7 +174,7 @@ class MASPreconditioner:
 
 
 def pcg_opts(tol=1e-5, stop_mode=STOP_TRUE_RESIDUAL, max_iter=0, precision=FP64, check_every=8, ir_max_outer=20,
-             ir_inner_tol=0.0, fuse=1, flexible=False, ir_replace_every=8):
+             ir_inner_tol=0.0, fuse=1, flexible=False, ir_replace_every=8, graph=None):
     o = PcgOpts()
     lib().sgipc_pcg_default_opts(ctypes.byref(o))
     o.tol = tol
@@ -187,6 +187,8 @@ def pcg_opts(tol=1e-5, stop_mode=STOP_TRUE_RESIDUAL, max_iter=0, precision=FP64,
     o.fuse = int(fuse)
     o.flexible = int(bool(flexible))
     o.ir_replace_every = int(ir_replace_every)
+    if graph is not None:
+        o.graph = int(bool(graph))
     return o
 
 

@@ -419,3 +419,25 @@ def test_fp32_refinement_with_residual_replacement(pkg, scene_c0, coarse):
         assert np.linalg.norm(rep.x - plain.x) <= 1e-4 * np.linalg.norm(plain.x)
     finally:
         ls.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
+@pytest.mark.parametrize("coarse", [1, 2])
+def test_graph_replay_matches_eager(pkg, precision, coarse):
+    """PCG with its full chunks captured into a HIP graph and replayed gives
+    bit-identical iterates to the eager loop (same kernels, same order; every
+    kernel of the loop is deterministic).  Three graph solves, each followed
+    by an eager one, so replays alternate with the program's own eager work."""
+    s = pkg.scenes.tet_grid_scene("graph_grid16", 16, E_loguniform=(1e4, 1e9), n_pairs=800)
+    ls = pkg.LinearSystem(s, bank_size=32, coarse_mode=coarse)
+    try:
+        ref = ls.solve(tol=1e-7, precision=precision, graph=False)
+        assert ref.status == 0 and ref.iterations > 16  # several full chunks
+        for _ in range(3):
+            g = ls.solve(tol=1e-7, precision=precision, graph=True)
+            assert g.status == 0 and g.iterations == ref.iterations
+            assert np.array_equal(g.x, ref.x)
+            e = ls.solve(tol=1e-7, precision=precision, graph=False)
+            assert e.iterations == ref.iterations and np.array_equal(e.x, ref.x)
+    finally:
+        ls.close()
