@@ -1492,41 +1492,72 @@ void Mas::assemble_invert_t(const double* vals, const int* rows, const int* cols
     Z32_.reserve((size_t)total * 3);
 }
 
+// The dense level-2 inverse is factorised on a second stream (default
+// priority): rocSOLVER's small-matrix kernels occupy few CUs, so the rest of
+// the build (coarser levels, level-0 inverses, A P) runs beside them.  The
+// main stream waits only for the scatter out of the ping-pong Galerkin buffer
+// (reused two levels later); finish_dense_level2 joins the streams.
 void Mas::build_dense_level2(hipStream_t s, const double* v, const int* r, const int* c, int nb)
 {
     dense_m_ = 0;
+    dense_pending_ = false;
     const long m = 3L * h_cnt[2];
     if (m < 1 || m > kDenseMax || nb < 1) return;
+    if (!s2_) {
+        SGIPC_HIP(hipStreamCreateWithFlags(&s2_, hipStreamNonBlocking));
+        SGIPC_HIP(hipEventCreateWithFlags(&ev_in_, hipEventDisableTiming));
+        SGIPC_HIP(hipEventCreateWithFlags(&ev_scatter_, hipEventDisableTiming));
+        SGIPC_HIP(hipEventCreateWithFlags(&ev_done_, hipEventDisableTiming));
+    }
     dense_inv_.reserve((size_t)m * m);
-    launch_fill<double>(s, dense_inv_.get(), 0.0, m * m);
-    k_dense_scatter<<<grid_for(nb, 256), 256, 0, s>>>(v, r, c, nb, (int)m, dense_inv_.get());
+    dense_info_.reserve(2);
+    SGIPC_HIP(hipEventRecord(ev_in_, s));
+    SGIPC_HIP(hipStreamWaitEvent(s2_, ev_in_, 0));
+    launch_fill<double>(s2_, dense_inv_.get(), 0.0, m * m);
+    k_dense_scatter<<<grid_for(nb, 256), 256, 0, s2_>>>(v, r, c, nb, (int)m, dense_inv_.get());
     SGIPC_LAUNCH_CHECK();
+    SGIPC_HIP(hipEventRecord(ev_scatter_, s2_));
+    SGIPC_HIP(hipStreamWaitEvent(s, ev_scatter_, 0));  // the Galerkin buffer may be reused from here
     if (!rb_) {
         rocblas_handle h = nullptr;
         SGIPC_CHECK(rocblas_create_handle(&h) == rocblas_status_success, SGIPC_HIP_ERROR, "rocblas_create_handle failed");
         rb_ = h;
     }
     rocblas_handle h = static_cast<rocblas_handle>(rb_);
-    SGIPC_CHECK(rocblas_set_stream(h, s) == rocblas_status_success, SGIPC_HIP_ERROR, "rocblas_set_stream failed");
-    dense_info_.reserve(2);
+    SGIPC_CHECK(rocblas_set_stream(h, s2_) == rocblas_status_success, SGIPC_HIP_ERROR, "rocblas_set_stream failed");
     SGIPC_CHECK(rocsolver_dpotrf(h, rocblas_fill_lower, (rocblas_int)m, dense_inv_.get(), (rocblas_int)m,
                                  dense_info_.get()) == rocblas_status_success,
                 SGIPC_HIP_ERROR, "rocsolver_dpotrf failed");
     SGIPC_CHECK(rocsolver_dpotri(h, rocblas_fill_lower, (rocblas_int)m, dense_inv_.get(), (rocblas_int)m,
                                  dense_info_.get() + 1) == rocblas_status_success,
                 SGIPC_HIP_ERROR, "rocsolver_dpotri failed");
+    k_mirror_upper<<<grid_for(m * m, 256), 256, 0, s2_>>>((int)m, dense_inv_.get());
+    SGIPC_LAUNCH_CHECK();
+    SGIPC_HIP(hipEventRecord(ev_done_, s2_));
+    dense_pending_ = true;
+    dense_m_pending_ = (int)m;
+}
+
+void Mas::finish_dense_level2(hipStream_t s)
+{
+    if (!dense_pending_) return;
+    dense_pending_ = false;
+    SGIPC_HIP(hipStreamWaitEvent(s, ev_done_, 0));
     int info[2] = {0, 0};
     SGIPC_HIP(hipMemcpyAsync(info, dense_info_.get(), sizeof(info), hipMemcpyDeviceToHost, s));
     SGIPC_HIP(hipStreamSynchronize(s));
-    if (info[0] != 0 || info[1] != 0) return;  // not positive definite: keep the additive levels
-    k_mirror_upper<<<grid_for(m * m, 256), 256, 0, s>>>((int)m, dense_inv_.get());
-    SGIPC_LAUNCH_CHECK();
-    dense_m_ = (int)m;
+    // not positive definite: the additive levels >= 2 stay
+    if (info[0] == 0 && info[1] == 0) dense_m_ = dense_m_pending_;
 }
 
 Mas::~Mas()
 {
+    if (s2_) (void)hipStreamSynchronize(s2_);
     if (rb_) (void)rocblas_destroy_handle(static_cast<rocblas_handle>(rb_));
+    if (ev_in_) (void)hipEventDestroy(ev_in_);
+    if (ev_scatter_) (void)hipEventDestroy(ev_scatter_);
+    if (ev_done_) (void)hipEventDestroy(ev_done_);
+    if (s2_) (void)hipStreamDestroy(s2_);
 }
 
 void Mas::learned_sample(hipStream_t s, const double* r, double* ztmp, float* X, float* q)
@@ -1613,6 +1644,10 @@ void Mas::set_bcoo(const double* vals, const int* rows, const int* cols, int nnz
     precision = prec == SGIPC_FP64 ? SGIPC_FP64 : SGIPC_FP32_MAS;
     graph_.n = 0;
     dense_m_ = 0;
+    if (dense_pending_) {  // a build that ended in an error left the factorisation running
+        SGIPC_HIP(hipStreamSynchronize(s2_));
+        dense_pending_ = false;
+    }
     build_hierarchy(pairs, n_pairs);
     if (B == 32) assemble_invert_t<32>(vals, rows, cols, nnz, offset);
     else assemble_invert_t<16>(vals, rows, cols, nnz, offset);
@@ -1624,6 +1659,7 @@ void Mas::set_bcoo(const double* vals, const int* rows, const int* cols, int nnz
             fine_ = &own_fine_;
         }
     }
+    finish_dense_level2(ctx->stream);
     SGIPC_HIP(hipStreamSynchronize(ctx->stream));
     ms_build = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     built = true;

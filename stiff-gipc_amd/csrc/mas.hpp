@@ -116,11 +116,16 @@ class Mas {
     FullMatrix own_fine_;
 
     void build_dense_level2(hipStream_t s, const double* v, const int* r, const int* c, int nb);
+    void finish_dense_level2(hipStream_t s);  // joins the factorisation stream
     static constexpr long kDenseMax = 4608;  // largest dense level-2 system (3 c_2)
     DevBuf<double> dense_inv_;   // inverse of the level-2 operator (FP64, m x m)
     DevBuf<int> dense_info_;
     int dense_m_ = 0;
+    int dense_m_pending_ = 0;
+    bool dense_pending_ = false;
     void* rb_ = nullptr;         // rocBLAS handle of the factorisation
+    hipStream_t s2_ = nullptr;   // factorisation stream (default priority)
+    hipEvent_t ev_in_ = nullptr, ev_scatter_ = nullptr, ev_done_ = nullptr;
 
     CoarseGraph graph_;          // level-1 operator (coarse cycle, learned level)
     DevBuf<double> g1d_, c2d_;   // level-1 cycle vectors (FP64 apply)
