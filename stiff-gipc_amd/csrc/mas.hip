@@ -40,6 +40,7 @@
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 
@@ -50,25 +51,51 @@ namespace {
 __device__ __host__ __forceinline__ int blk_index(int B, int r, int c) { return B * r - r * (r + 1) / 2 + c; }
 
 // ------------------------------------------------------------------ hierarchy
-template <int B>
-__global__ void k_mask_fine(int level, int n, const int* __restrict__ map_l,
-                            const uint32_t* __restrict__ ns, const uint32_t* __restrict__ nn,
-                            const uint32_t* __restrict__ nl, uint32_t* __restrict__ mask)
+// OR of mask m into dst[key], combined over the lanes of the wave that share
+// a key: on the coarse levels whole waves of fine nodes map to one or two
+// coarse nodes, and one atomic per lane would serialise on those words.
+// After a few distinct keys the remaining lanes issue their own atomics
+// (keys are then spread out).  Every lane of the wave must call it.
+__device__ __forceinline__ void wave_or_atomic(uint32_t* __restrict__ dst, int key, uint32_t m)
 {
-    int u = blockIdx.x * blockDim.x + threadIdx.x;
-    if (u >= n) return;
-    int a = map_l ? map_l[u] : u;
-    uint32_t m = 0;
-    const uint32_t st = ns[u], cnt = nn[u];
-    for (uint32_t k = 0; k < cnt; ++k) {
-        int v = (int)nl[st + k];
-        int b = map_l ? map_l[v] : v;
-        if (a != b && a / B == b / B) m |= 1u << (b % B);
+    const int lane = lane_id();
+    uint64_t pending = __ballot(m != 0u);
+    for (int round = 0; round < 4 && pending; ++round) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const int k = __shfl(key, leader);
+        const bool mine = ((pending >> lane) & 1ull) && key == k;
+        pending &= ~__ballot(mine);
+        uint32_t v = mine ? m : 0u;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) v |= (uint32_t)__shfl_xor((int)v, off);
+        if (lane == leader) atomicOr(dst + k, v);
     }
-    if (level == 0)
-        mask[u] = m;
-    else if (m)
-        atomicOr(mask + a, m);
+    if ((pending >> lane) & 1ull) atomicOr(dst + key, m);
+}
+
+template <int B>
+__global__ __launch_bounds__(256) void k_mask_fine(int level, int n, const int* __restrict__ map_l,
+                                                   const uint32_t* __restrict__ ns, const uint32_t* __restrict__ nn,
+                                                   const uint32_t* __restrict__ nl, uint32_t* __restrict__ mask)
+{
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool act = u < n;
+    int a = 0;
+    uint32_t m = 0;
+    if (act) {
+        a = map_l ? map_l[u] : u;
+        const uint32_t st = ns[u], cnt = nn[u];
+        for (uint32_t k = 0; k < cnt; ++k) {
+            int v = (int)nl[st + k];
+            int b = map_l ? map_l[v] : v;
+            if (a != b && a / B == b / B) m |= 1u << (b % B);
+        }
+    }
+    if (level == 0) {
+        if (act) mask[u] = m;
+    } else {
+        wave_or_atomic(mask, a, m);
+    }
 }
 
 __device__ __forceinline__ void decode_pair(int4 pr, int coff, int n, int v[4])
@@ -101,16 +128,25 @@ __device__ __forceinline__ void decode_pair(int4 pr, int coff, int n, int v[4])
     }
 }
 
+// Masks contributed by the contact pairs.  When the level has few nodes (the
+// coarse levels) every pair lands on the same few mask words: a block-private
+// copy in LDS takes the ORs and each block then adds only its non-zero words;
+// otherwise the ORs go to global memory, combined per wave.  Grid-stride over
+// the pairs; the path is uniform per block (it depends on the level's count).
+constexpr int kMaskLdsWords = 8192;
+
 template <int B>
-__global__ void k_mask_contact(int n, int coff, const int4* __restrict__ pairs, int np,
-                               const int* __restrict__ map_l, uint32_t* __restrict__ mask)
+__device__ __forceinline__ void contact_masks(int p, int np, int n, int coff, const int4* __restrict__ pairs,
+                                              const int* __restrict__ map_l, int cv[4], uint32_t m[4])
 {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        cv[i] = -1;
+        m[i] = 0;
+    }
     if (p >= np) return;
     int v[4];
     decode_pair(pairs[p], coff, n, v);
-    int cv[4];
-    uint32_t m[4] = {0, 0, 0, 0};
 #pragma unroll
     for (int i = 0; i < 4; ++i) cv[i] = v[i] >= 0 ? (map_l ? map_l[v[i]] : v[i]) : -1;
 #pragma unroll
@@ -121,9 +157,38 @@ __global__ void k_mask_contact(int n, int coff, const int4* __restrict__ pairs, 
                 m[i] |= 1u << (cv[j] % B);
                 m[j] |= 1u << (cv[i] % B);
             }
+}
+
+template <int B>
+__global__ __launch_bounds__(256) void k_mask_contact(const int* __restrict__ meta, int level, int n, int coff,
+                                                      const int4* __restrict__ pairs, int np,
+                                                      const int* __restrict__ map_l, uint32_t* __restrict__ mask)
+{
+    __shared__ uint32_t sm[kMaskLdsWords];
+    const int c = meta[kMetaCnt + level];
+    const int stride = gridDim.x * blockDim.x;
+    int cv[4];
+    uint32_t m[4];
+    if (c <= kMaskLdsWords) {
+        for (int i = threadIdx.x; i < c; i += blockDim.x) sm[i] = 0u;
+        __syncthreads();
+        for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += stride) {
+            contact_masks<B>(p, np, n, coff, pairs, map_l, cv, m);
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-        if (cv[i] >= 0 && m[i]) atomicOr(mask + cv[i], m[i]);
+            for (int i = 0; i < 4; ++i)
+                if (cv[i] >= 0 && cv[i] < c && m[i]) atomicOr(sm + cv[i], m[i]);
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < c; i += blockDim.x)
+            if (sm[i]) atomicOr(mask + i, sm[i]);
+    } else {
+        // wave-uniform trip count: every lane takes part in the wave's combine
+        for (int p0 = blockIdx.x * blockDim.x; p0 < np; p0 += stride) {
+            contact_masks<B>(p0 + (int)threadIdx.x, np, n, coff, pairs, map_l, cv, m);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) wave_or_atomic(mask, cv[i] >= 0 ? cv[i] : 0, cv[i] >= 0 ? m[i] : 0u);
+        }
+    }
 }
 
 // Connected components inside each domain (closure of the bitmasks) and the
@@ -1380,8 +1445,9 @@ void Mas::build_hierarchy_t(const int* pairs, int n_pairs)
         k_mask_fine<BB><<<gn, 256, 0, s>>>(l, n, map_l, nbr_start_.get(), nbr_num_.get(), nbr_list_.get(), mask_.get());
         SGIPC_LAUNCH_CHECK();
         if (n_pairs > 0) {
-            k_mask_contact<BB><<<grid_for(n_pairs, 256), 256, 0, s>>>(n, coff, reinterpret_cast<const int4*>(pairs),
-                                                                     n_pairs, map_l, mask_.get());
+            const int gc = std::min(grid_for(n_pairs, 256), 128);
+            k_mask_contact<BB><<<gc, 256, 0, s>>>(meta_.get(), l, n, coff, reinterpret_cast<const int4*>(pairs),
+                                                  n_pairs, map_l, mask_.get());
             SGIPC_LAUNCH_CHECK();
         }
         // upper bound on the node count of level l is n, padded to whole blocks
