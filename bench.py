@@ -91,6 +91,8 @@ def main():
     ap.add_argument("--coarse", type=int, default=2, choices=[0, 1, 2],
                     help="MAS coarse levels: 0 additive (the reference), 1 V(1,1) cycle on level 1, "
                          "2 that cycle inside a two-grid cycle on level 0")
+    ap.add_argument("--level0-half", type=int, default=1, choices=[0, 1],
+                    help="--coarse 2 float applies: level-0 domain inverses as unit-diagonal scaled FP16 (1) or float (0)")
     ap.add_argument("--omega", type=float, default=None,
                     help="level-0 smoothing weight of --coarse 2 (library default 0.4)")
     ap.add_argument("--learned", type=int, default=2, choices=[0, 1, 2],
@@ -132,6 +134,7 @@ def main():
     ls = pkg.LinearSystem(scene, bank_size=args.bank, coarse_mode=args.coarse)
     if args.omega is not None:
         ls.set_smoothing_weight(args.omega)
+    ls.set_level0_half(args.level0_half)
     ctx = ls.ctx
     L = pkg._lib.lib()
     import ctypes
@@ -300,6 +303,8 @@ def main():
     }
     dtype = {"fp64": "f64", "fp32_mas": "f64 (MAS apply f32)",
              "fp32_ir": "f32 SpMV values + MAS apply, f64 vectors/dots/refinement"}[args.precision]
+    if args.coarse == 2 and args.level0_half and args.precision != "fp64":
+        dtype += "; level-0 domain inverses stored f16 (unit-diagonal scaled), applied in f32"
     line = {
         "metric": "ms per Newton linear solve to rel. residual 1e-5; SpMV HBM GB/s vs peak",
         "value": round(out["ms"], 3),
@@ -327,6 +332,9 @@ def main():
                                 2: "two-grid cycle on level 0 (weighted level-0 domain solves before and after) "
                                    "around the level-1 V(1,1) cycle, exact dense level-2 solve"}[args.coarse],
             "level0_smoothing_weight": (args.omega if args.omega is not None else 0.4) if args.coarse == 2 else None,
+            "level0_inverse_storage": ("fp16 scaled to a unit diagonal (D C D, D per slot in f32)"
+                                       if args.coarse == 2 and args.level0_half and args.precision != "fp64"
+                                       else "same as the apply"),
             "learned": learned_info,
             "learned_used_in_timed_steps": out.get("learned_used"),
             "plain_fp64_b32_same_run": plain,

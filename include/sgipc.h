@@ -277,6 +277,14 @@ int sgipc_mas_set_coarse_mode(sgipc_mas* mas, int mode);
 /* Smoothing weight w of coarse mode 2, 0 < w < 2 (default 0.4; no
  * counterpart in the reference). */
 int sgipc_mas_set_smoothing_weight(sgipc_mas* mas, double omega);
+/* Storage of the level-0 domain inverses read by the float applies of coarse
+ * mode 2 (no counterpart in the reference): on (the default), FP16 scaled to a
+ * unit diagonal, C = D^-1 Minv D^-1 with D^2 = diag(Minv) and D kept per slot
+ * in float, so that Minv r = D C D r (|C_ij| <= 1 for SPD domain inverses;
+ * half the bytes of the float blocks); off, the float blocks.  FP64 applies
+ * and the other modes always use the full inverses.  Takes effect at the next
+ * sgipc_mas_set_preconditioner_bcoo (or system solve). */
+int sgipc_mas_set_level0_half(sgipc_mas* mas, int on);
 /* Measurement helper: average duration (ms) of one preconditioning (all of its
  * kernels) over `reps` applications, HIP events on the context's stream. */
 int sgipc_mas_time_apply(sgipc_mas* mas, const double* r, double* z, int reps, double* avg_ms);

@@ -111,6 +111,7 @@ _SIGS = {
     "sgipc_system_mas": (vp, [vp]),
     "sgipc_mas_set_coarse_mode": (ci, [vp, ci]),
     "sgipc_mas_set_smoothing_weight": (ci, [vp, cd]),
+    "sgipc_mas_set_level0_half": (ci, [vp, ci]),
     "sgipc_mpnn_create": (ci, [vp, ci, ci, ctypes.POINTER(vp)]),
     "sgipc_mpnn_destroy": (ci, [vp]),
     "sgipc_mpnn_num_params": (ci, [vp]),

@@ -408,6 +408,13 @@ int sgipc_mas_set_smoothing_weight(sgipc_mas* mas, double omega)
     return SGIPC_OK;
 }
 
+int sgipc_mas_set_level0_half(sgipc_mas* mas, int on)
+{
+    if (!mas || on < 0 || on > 1) return SGIPC_BAD_ARG;
+    mas->m->half_level0 = on != 0;
+    return SGIPC_OK;
+}
+
 int sgipc_mas_time_apply(sgipc_mas* mas, const double* r, double* z, int reps, double* avg_ms)
 {
     if (!mas || !r || !z || !avg_ms || reps < 1) return SGIPC_BAD_ARG;

@@ -43,6 +43,11 @@ class Mas {
     // after it; needs the fine operator).  Takes effect at the next set_bcoo.
     int coarse_mode = 0;
     double omega0 = 0.4;
+    // coarse mode 2 with the float apply: the level-0 domain solves read the
+    // inverse as FP16 scaled to a unit diagonal (C = D^-1 Minv D^-1, D^2 its
+    // diagonal) and the per-slot scales, half the bytes of the float blocks.
+    // Takes effect at the next set_bcoo.
+    bool half_level0 = true;
     // the fine operator of coarse mode 2 (the PCG matrix, with its FP32 copy
     // when the apply is FP32); set_bcoo builds its own unless told not to
     const FullMatrix* fine_ = nullptr;
@@ -118,6 +123,9 @@ class Mas {
     void build_dense_level2(hipStream_t s, const double* v, const int* r, const int* c, int nb);
     void finish_dense_level2(hipStream_t s);  // joins the factorisation stream
     static constexpr long kDenseMax = 4608;  // largest dense level-2 system (3 c_2)
+    DevBuf<uint16_t> minv16_;    // level-0 inverse, FP16 scaled form (half_level0)
+    DevBuf<float> msc_;          // its per-slot scales
+    bool half_ready_ = false;
     DevBuf<double> dense_inv_;   // inverse of the level-2 operator (FP64, m x m)
     DevBuf<int> dense_info_;
     int dense_m_ = 0;

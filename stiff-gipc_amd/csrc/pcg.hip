@@ -101,10 +101,26 @@ __global__ __launch_bounds__(kVecThreads) void pcg_update_p(
 {
     __shared__ double smem[16];
     if (ctl->done) return;
+    // the first kPer grid strides are loaded before the partial sums
+    constexpr int kPer = 4;
+    const long i0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long stride = (long)gridDim.x * blockDim.x;
+    double zv[kPer], pv[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const long i = i0 + j * stride;
+        zv[j] = i < n3 ? (double)z[i] : 0.0;
+        pv[j] = i < n3 ? (double)p[i] : 0.0;
+    }
     double rzn = sum_partials(rz_new_part, n_rz, smem);
     double rzo = sum_partials(rz_old_part, n_rz, smem);
     double beta = rzn / rzo;
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (long)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const long i = i0 + j * stride;
+        if (i < n3) p[i] = (TV)(zv[j] + beta * pv[j]);
+    }
+    for (long i = i0 + kPer * stride; i < n3; i += stride) {
         p[i] = (TV)((double)z[i] + beta * (double)p[i]);
     }
 }

@@ -122,6 +122,11 @@ class MASPreconditioner:
         """Weight of the level-0 smoothing in coarse mode 2 (default 0.4)."""
         check(lib().sgipc_mas_set_smoothing_weight(self.h, float(omega)), self.ctx.h)
 
+    def set_level0_half(self, on):
+        """FP16 (unit-diagonal scaled) level-0 inverses in the float applies of
+        coarse mode 2 (default on).  Applies at the next build."""
+        check(lib().sgipc_mas_set_level0_half(self.h, int(bool(on))), self.ctx.h)
+
     def initPreconditioner_Neighbor(self, vertNum, collision_node_offset, totalNeighborNum, nbr_start, nbr_num,
                                     nbr_list):
         self.vert_num = vertNum
@@ -348,6 +353,11 @@ class LinearSystem:
     def set_smoothing_weight(self, omega):
         """Weight of the level-0 smoothing in coarse mode 2 (default 0.4)."""
         check(lib().sgipc_mas_set_smoothing_weight(self.sys.mas_handle(), float(omega)), self.ctx.h)
+
+    def set_level0_half(self, on):
+        """FP16 (unit-diagonal scaled) level-0 inverses in the float applies of
+        coarse mode 2 (default on)."""
+        check(lib().sgipc_mas_set_level0_half(self.sys.mas_handle(), int(bool(on))), self.ctx.h)
 
     def solve(self, tol=1e-5, precision="fp64", stop_mode=STOP_TRUE_RESIDUAL, want_grad=False, **kw):
         st, tm = self.sys.solve_linear_system(tol=tol, precision=precision, stop_mode=stop_mode, want_grad=want_grad,

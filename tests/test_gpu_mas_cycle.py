@@ -102,13 +102,15 @@ def test_cycle_solves_in_fewer_iterations(pkg, B):
 
 
 # ---- coarse mode 2: two-grid cycle on level 0 around the level-1 cycle
-def _vcycle_mas(pkg, ctx, s, B, vals, rows, cols, precision, omega=None):
+def _vcycle_mas(pkg, ctx, s, B, vals, rows, cols, precision, omega=None, half=None):
     nbr = [ctx.to_device(a, np.uint32) for a in (s.nbr_start, s.nbr_num,
                                                  s.nbr_list if s.nbr_list.size else np.zeros(1, np.uint32))]
     mas = pkg.MASPreconditioner(ctx, B)
     mas.set_coarse_mode(2)
     if omega is not None:
         mas.set_smoothing_weight(omega)
+    if half is not None:
+        mas.set_level0_half(half)
     mas.initPreconditioner_Neighbor(s.n, 0, int(s.nbr_list.size), *nbr)
     mas.initPreconditioner_Matrix()
     dv = ctx.to_device(vals, np.float64)
@@ -119,7 +121,25 @@ def _vcycle_mas(pkg, ctx, s, B, vals, rows, cols, precision, omega=None):
     return mas, (nbr, dv, dr, dc, pairs)
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def _half_level0(inv, n_dom0):
+    """The FP16 storage of the level-0 inverses restated in numpy: per slot
+    d = sqrt(diag) and 1/d in float32, C = (M / d_i) / d_j rounded float32 then
+    to float16, the apply's matrix D C D (float32 arithmetic as on the
+    device)."""
+    out = inv.copy()
+    for k in range(n_dom0):
+        M = inv[k].astype(np.float32)
+        v = np.diag(M).copy()
+        pos = v > 0
+        d = np.where(pos, np.sqrt(np.where(pos, v, 1)).astype(np.float32), np.float32(0))
+        dinv = np.where(pos, np.float32(1) / np.where(pos, d, 1), np.float32(0)).astype(np.float32)
+        C = ((M * dinv[:, None]).astype(np.float32) * dinv[None, :]).astype(np.float32)
+        C = np.clip(C, -65504, 65504).astype(np.float16).astype(np.float64)
+        out[k] = d[:, None].astype(np.float64) * C * d[None, :].astype(np.float64)
+    return out
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "fp32_half"])
 @pytest.mark.parametrize("B", [16, 32])
 @pytest.mark.parametrize("which", ["small", "c0"])
 def test_vcycle_apply_matches_oracle(pkg, orc, gpu_ctx, B, which, precision, scene_small, scene_c0):
@@ -127,7 +147,10 @@ def test_vcycle_apply_matches_oracle(pkg, orc, gpu_ctx, B, which, precision, sce
     device's own domain inverses.  FP64: what remains is summation order and
     the float32 level-1 operator and A P (rounded the same way in both).
     FP32 (slots, inverses, the fine SpMV's values and vectors in float32):
-    within 2e-4 of the FP64 restatement."""
+    within 2e-4 of the FP64 restatement.  FP32 with the FP16 level-0 storage:
+    within 2e-4 of the restatement given the same FP16 level-0 matrices, and
+    within 3e-3 of the one with the float inverses (the storage's own error:
+    FP16 keeps 11 significant bits of each scaled entry)."""
     s = {"small": scene_small, "c0": scene_c0}[which]
     vals, rows, cols = orc.assemble(s)
     om = orc.Mas(s, B)
@@ -136,9 +159,13 @@ def test_vcycle_apply_matches_oracle(pkg, orc, gpu_ctx, B, which, precision, sce
     if parts.L <= 2 or parts.cnt[1] <= B:
         pytest.skip("level 1 is one domain (solved exactly): the cycle stays off")
     prec = pkg.FP64 if precision == "fp64" else pkg.FP32_MAS
-    mas, keep = _vcycle_mas(pkg, gpu_ctx, s, B, vals, rows, cols, prec, omega=0.45)
+    half = precision == "fp32_half"
+    mas, keep = _vcycle_mas(pkg, gpu_ctx, s, B, vals, rows, cols, prec, omega=0.45, half=half)
     try:
         parts.inv = mpnn._domain_dense(mas.inverse_blocks().reshape(-1, B * (B + 1) // 2, 9), B)
+        inv_float = parts.inv
+        if half:
+            parts.inv = _half_level0(inv_float, parts.off[1] // B)
         A = mpnn.bcoo_to_csr(vals, rows, cols, s.n)
         graph = mpnn.coarse_graph(A, parts.map1, parts.cnt[1])
         A1f = mpnn.float_operator(graph)
@@ -153,6 +180,13 @@ def test_vcycle_apply_matches_oracle(pkg, orc, gpu_ctx, B, which, precision, sce
             got = dz.download()
             err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
             assert err <= (1e-7 if precision == "fp64" else 2e-4), (k, err)
+            if half:
+                parts.inv = inv_float
+                ref_f = mpnn.mas_vcycle_apply(parts, A1f, A, APf, r, A2=A2, omega=0.45)
+                parts.inv = _half_level0(inv_float, parts.off[1] // B)
+                err_f = np.linalg.norm(got - ref_f) / np.linalg.norm(ref_f)
+                print(f"fp16 level 0: {err:.2e} vs its restatement, {err_f:.2e} vs float storage")
+                assert err_f <= 3e-3, (k, err_f)
     finally:
         mas.close()
         om.close()
@@ -217,3 +251,27 @@ def test_vcycle_recovers_from_a_too_large_weight(pkg, precision):
     finally:
         ls.close()
         ref.close()
+
+
+@pytest.mark.parametrize("B", [16, 32])
+def test_level0_half_storage_keeps_the_iteration_count(pkg, B):
+    """FP16 level-0 inverses (the default of the float applies of coarse mode
+    2) against float ones on a stiff contact scene: both converge to the
+    requested true residual, within 3 iterations of each other, to the same
+    solution."""
+    s = pkg.scenes.tet_grid_scene("cycle_grid16", 16, E_loguniform=(1e4, 1e9), n_pairs=800)
+    a_ls = pkg.LinearSystem(s, bank_size=B, coarse_mode=2)
+    b_ls = pkg.LinearSystem(s, bank_size=B, coarse_mode=2)
+    try:
+        a_ls.set_level0_half(False)
+        b_ls.set_level0_half(True)
+        a = a_ls.solve(tol=1e-5, precision="fp32_ir")
+        b = b_ls.solve(tol=1e-5, precision="fp32_ir")
+        print(f"B={B}: float level 0 {a.iterations} iterations, fp16 level 0 {b.iterations}")
+        assert a.status == 0 and b.status == 0
+        assert b.true_rel_residual <= 1e-5
+        assert b.iterations <= a.iterations + 3
+        assert np.linalg.norm(a.x - b.x) <= 1e-4 * np.linalg.norm(a.x)
+    finally:
+        a_ls.close()
+        b_ls.close()
