@@ -413,3 +413,25 @@ def mas_vcycle_apply(parts: MasParts, A1f, A, APf, r, A2=None, omega=0.5):
     c = level1_cycle(parts, A1f, g, A2)
     r2 = r1 - APf @ c.reshape(-1)
     return z0 + c[parts.map1].reshape(-1) + omega * level0_apply(parts, r2)
+
+
+def half_level0_inverse(inv, n_dom0):
+    """The FP16 storage of the level-0 domain inverses that the float applies
+    of coarse mode 2 read (k_half_level0 in mas.hip; no counterpart in the
+    reference): per slot d = sqrt(Minv_ii) and 1/d in float32, the scaled
+    entries C = (Minv_ij / d_i) / d_j rounded to float32 and then to float16
+    (|C_ij| <= 1 for an SPD inverse), and the matrix the apply uses,
+    D C D.  Slots with a non-positive diagonal (padding) get d = 0.
+    ``inv``: (domains, 3B, 3B) dense inverses; domains below ``n_dom0`` are
+    level 0 and are replaced, the rest are returned unchanged."""
+    out = inv.copy()
+    for k in range(n_dom0):
+        M = inv[k].astype(np.float32)
+        v = np.diag(M).copy()
+        pos = v > 0
+        d = np.where(pos, np.sqrt(np.where(pos, v, 1)).astype(np.float32), np.float32(0))
+        dinv = np.where(pos, np.float32(1) / np.where(pos, d, 1), np.float32(0)).astype(np.float32)
+        C = ((M * dinv[:, None]).astype(np.float32) * dinv[None, :]).astype(np.float32)
+        C = np.clip(C, -65504, 65504).astype(np.float16).astype(np.float64)
+        out[k] = d[:, None].astype(np.float64) * C * d[None, :].astype(np.float64)
+    return out

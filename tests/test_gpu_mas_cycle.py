@@ -121,24 +121,6 @@ def _vcycle_mas(pkg, ctx, s, B, vals, rows, cols, precision, omega=None, half=No
     return mas, (nbr, dv, dr, dc, pairs)
 
 
-def _half_level0(inv, n_dom0):
-    """The FP16 storage of the level-0 inverses restated in numpy: per slot
-    d = sqrt(diag) and 1/d in float32, C = (M / d_i) / d_j rounded float32 then
-    to float16, the apply's matrix D C D (float32 arithmetic as on the
-    device)."""
-    out = inv.copy()
-    for k in range(n_dom0):
-        M = inv[k].astype(np.float32)
-        v = np.diag(M).copy()
-        pos = v > 0
-        d = np.where(pos, np.sqrt(np.where(pos, v, 1)).astype(np.float32), np.float32(0))
-        dinv = np.where(pos, np.float32(1) / np.where(pos, d, 1), np.float32(0)).astype(np.float32)
-        C = ((M * dinv[:, None]).astype(np.float32) * dinv[None, :]).astype(np.float32)
-        C = np.clip(C, -65504, 65504).astype(np.float16).astype(np.float64)
-        out[k] = d[:, None].astype(np.float64) * C * d[None, :].astype(np.float64)
-    return out
-
-
 @pytest.mark.parametrize("precision", ["fp64", "fp32", "fp32_half"])
 @pytest.mark.parametrize("B", [16, 32])
 @pytest.mark.parametrize("which", ["small", "c0"])
@@ -165,7 +147,7 @@ def test_vcycle_apply_matches_oracle(pkg, orc, gpu_ctx, B, which, precision, sce
         parts.inv = mpnn._domain_dense(mas.inverse_blocks().reshape(-1, B * (B + 1) // 2, 9), B)
         inv_float = parts.inv
         if half:
-            parts.inv = _half_level0(inv_float, parts.off[1] // B)
+            parts.inv = mpnn.half_level0_inverse(inv_float, parts.off[1] // B)
         A = mpnn.bcoo_to_csr(vals, rows, cols, s.n)
         graph = mpnn.coarse_graph(A, parts.map1, parts.cnt[1])
         A1f = mpnn.float_operator(graph)
@@ -183,7 +165,7 @@ def test_vcycle_apply_matches_oracle(pkg, orc, gpu_ctx, B, which, precision, sce
             if half:
                 parts.inv = inv_float
                 ref_f = mpnn.mas_vcycle_apply(parts, A1f, A, APf, r, A2=A2, omega=0.45)
-                parts.inv = _half_level0(inv_float, parts.off[1] // B)
+                parts.inv = mpnn.half_level0_inverse(inv_float, parts.off[1] // B)
                 err_f = np.linalg.norm(got - ref_f) / np.linalg.norm(ref_f)
                 print(f"fp16 level 0: {err:.2e} vs its restatement, {err_f:.2e} vs float storage")
                 assert err_f <= 3e-3, (k, err_f)

@@ -104,3 +104,54 @@ def test_vcycle_beats_the_additive_level0_in_pcg(setup):
     k2 = pcg(lambda r: mpnn.mas_vcycle_apply(parts, A1f, A, APf, r, A2=A2, omega=0.5))
     print(f"level-1 cycle {k1} iterations, two-grid on level 0 {k2}")
     assert k2 < k1
+
+
+def test_half_level0_storage_keeps_spd_and_iterations(setup):
+    """The FP16 storage of the level-0 inverses (unit-diagonal scaling, then
+    float16) keeps every level-0 domain matrix positive definite, perturbs
+    each entry by at most 2^-11 of d_i d_j, and leaves the two-grid PCG
+    iteration count within one of the float64 inverses."""
+    s, parts, A, graph, A1f, A2 = setup
+    B = parts.B
+    n0 = parts.off[1] // B
+    inv = parts.inv
+    h = mpnn.half_level0_inverse(inv, n0)
+    assert np.array_equal(h[n0:], inv[n0:])
+    for k in range(n0):
+        real = np.diag(inv[k]) > 0
+        Mk, Hk = inv[k][np.ix_(real, real)], h[k][np.ix_(real, real)]
+        d = np.sqrt(np.diag(Mk))
+        # float16 keeps 11 significant bits of each |C_ij| <= 1 (plus the
+        # float32 rounding of the scaling)
+        assert np.all(np.abs(Hk - Mk) <= (2.0 ** -11 + 1e-6) * np.outer(d, d))
+        assert np.linalg.eigvalsh(Hk).min() > 0
+    APf = mpnn.ap_operator(A, parts.map1, parts.cnt[1])
+
+    def pcg(prec, tol=1e-8):
+        b = s.rhs
+        x = np.zeros_like(b)
+        r = b.copy()
+        z = prec(r)
+        p = z.copy()
+        rz = r @ z
+        for k in range(1, 2000):
+            Ap = A @ p
+            a = rz / (p @ Ap)
+            x += a * p
+            r -= a * Ap
+            if np.linalg.norm(r) <= tol * np.linalg.norm(b):
+                return k
+            z = prec(r)
+            rz_new = r @ z
+            p = z + (rz_new / rz) * p
+            rz = rz_new
+        return 2000
+
+    k_full = pcg(lambda r: mpnn.mas_vcycle_apply(parts, A1f, A, APf, r, A2=A2, omega=0.4))
+    parts.inv = h
+    try:
+        k_half = pcg(lambda r: mpnn.mas_vcycle_apply(parts, A1f, A, APf, r, A2=A2, omega=0.4))
+    finally:
+        parts.inv = inv
+    print(f"two-grid PCG: float64 level-0 inverses {k_full} iterations, fp16 storage {k_half}")
+    assert k_half <= k_full + 1
