@@ -924,15 +924,10 @@ __device__ __forceinline__ void domain_product(const TS* __restrict__ sM, const 
     }
 }
 
-// Z[domain] = Minv[domain] * R[domain], one 256-thread workgroup per domain.
-// All loads of the domain's inverse blocks are issued before any LDS store so
-// the whole 19/38 KB domain is in flight at once.  Thread (rb, g) forms the
-// partial product of block row rb with the column blocks cb = g, g+G, ...;
-// the G partials of a row sit in adjacent lanes and are summed by shuffles in
-// a fixed order.
 // FP16 storage of the level-0 inverse (coarse mode 2): C = D^-1 Minv D^-1
 // with D = diag(Minv)^(1/2) per slot, so |C_ij| <= 1 for an SPD domain
-// inverse; Minv r = D C D r.  16 bytes of storage unpacked to 8 floats.
+// inverse; Minv r = D C D r.  16 bytes of storage (8 halves) unpack to two
+// float4 units in LDS (order: half_slot_entry).
 __device__ __forceinline__ float2 half2_bits_to_float2(unsigned u)
 {
     __half2 h;
@@ -940,12 +935,23 @@ __device__ __forceinline__ float2 half2_bits_to_float2(unsigned u)
     return __half22float2(h);
 }
 
-__device__ __forceinline__ void store_half8_as_float(float* __restrict__ dst, uint4 v)
+// Order of the halves in HBM.  A lane loads 16 bytes (8 halves, group u)
+// and stores them to LDS as two float4 units; the 64 groups of a wave,
+// c = u / 64, hold the float4 units 128 c + e (first four halves) and
+// 128 c + m + e (last four), e = u % 64 and m the group count of that wave
+// (64 but in the last), so each LDS store instruction covers consecutive
+// units (no bank conflicts).  Returns the natural float index (block-major,
+// 9 entries per block) of half h.
+template <int B>
+__host__ __device__ __forceinline__ int half_slot_entry(int h)
 {
-    const float2 a = half2_bits_to_float2(v.x), b = half2_bits_to_float2(v.y);
-    const float2 c = half2_bits_to_float2(v.z), e = half2_bits_to_float2(v.w);
-    reinterpret_cast<float4*>(dst)[0] = make_float4(a.x, a.y, b.x, b.y);
-    reinterpret_cast<float4*>(dst)[1] = make_float4(c.x, c.y, e.x, e.y);
+    constexpr int nval = B * (B + 1) / 2 * 9;
+    constexpr int nvh = nval / 8;
+    const int u = h / 8, j = h % 8;
+    const int c = u / 64, e = u % 64;
+    const int m = nvh - 64 * c < 64 ? nvh - 64 * c : 64;
+    const int unit = 128 * c + (j < 4 ? e : m + e);
+    return 4 * unit + (j & 3);
 }
 
 template <int B>
@@ -967,7 +973,8 @@ __global__ __launch_bounds__(256) void k_half_level0(const float* __restrict__ m
     }
     __syncthreads();
     __half* dst = reinterpret_cast<__half*>(minv16) + (long)d * nval;
-    for (int e = threadIdx.x; e < nval; e += blockDim.x) {
+    for (int h = threadIdx.x; h < nval; h += blockDim.x) {
+        const int e = half_slot_entry<B>(h);
         const int blk = e / 9, k = e - 9 * (e / 9);
         int rb = 0;
         while (rb + 1 < B && blk_index(B, rb + 1, rb + 1) <= blk) ++rb;
@@ -975,10 +982,16 @@ __global__ __launch_bounds__(256) void k_half_level0(const float* __restrict__ m
         // block entry k is M(3 rb + k % 3, 3 cb + k / 3)
         float c = src[e] * sinv[3 * rb + k % 3] * sinv[3 * cb + k / 3];
         c = fminf(fmaxf(c, -65504.0f), 65504.0f);
-        dst[e] = __float2half_rn(c);
+        dst[h] = __float2half_rn(c);
     }
 }
 
+// Z[domain] = Minv[domain] * R[domain], one 256-thread workgroup per domain.
+// All loads of the domain's inverse blocks are issued before any LDS store so
+// the whole 19/38 KB domain is in flight at once.  Thread (rb, g) forms the
+// partial product of block row rb with the column blocks cb = g, g+G, ...;
+// the G partials of a row sit in adjacent lanes and are summed by shuffles in
+// a fixed order.
 template <int B, typename TS>
 __global__ __launch_bounds__(256) void k_local_solve(const int* __restrict__ done, const TS* __restrict__ minv,
                                                       const TS* __restrict__ R, TS* __restrict__ Z, int dom0,
@@ -1152,12 +1165,21 @@ struct HalfDomain {
             h[q] = src[e < nvh ? e : nvh - 1];
         }
     }
+    // the two float4 units of group u (the order of half_slot_entry)
     __device__ __forceinline__ void stage(float* __restrict__ sM) const
     {
 #pragma unroll
         for (int q = 0; q < per; ++q) {
-            const int e = threadIdx.x + 256 * q;
-            if (e < nvh) store_half8_as_float(sM + 8 * e, h[q]);
+            const int u = threadIdx.x + 256 * q;
+            if (u < nvh) {
+                const int c = u / 64, e = u % 64;
+                const int m = nvh - 64 * c < 64 ? nvh - 64 * c : 64;
+                float4* dst = reinterpret_cast<float4*>(sM);
+                const float2 a = half2_bits_to_float2(h[q].x), b = half2_bits_to_float2(h[q].y);
+                const float2 cc = half2_bits_to_float2(h[q].z), dd = half2_bits_to_float2(h[q].w);
+                dst[128 * c + e] = make_float4(a.x, a.y, b.x, b.y);
+                dst[128 * c + m + e] = make_float4(cc.x, cc.y, dd.x, dd.y);
+            }
         }
     }
 };
