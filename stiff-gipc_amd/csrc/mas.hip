@@ -2048,10 +2048,15 @@ int Mas::finish_v_t(hipStream_t s, TS* R, TS* Z, const TS* minv, const int* done
     };
     solve0();
     SGIPC_LAUNCH_CHECK();
-    fine_->template apply<TS>(s, -w, Z, TS(1), R, nullptr, done);
-    // level-1 sums of r1
-    k_restrict0<BB, TS, TS><<<grid_for(npad, 256), 256, 0, s>>>(PcgStopDesc{}, meta_.get(), n, L, map_.get() + n, R, R);
-    SGIPC_LAUNCH_CHECK();
+    // and the level-1 sums of r1: in the SpMV's epilogue (halo form), else
+    // by their own kernel
+    const Level1Sums l1{map_.get() + n, npad, c1, h_off[2] - npad - c1};
+    if (!fine_->template apply_level1_sums<TS>(s, -w, Z, TS(1), R, done, l1)) {
+        fine_->template apply<TS>(s, -w, Z, TS(1), R, nullptr, done);
+        k_restrict0<BB, TS, TS><<<grid_for(npad, 256), 256, 0, s>>>(PcgStopDesc{}, meta_.get(), n, L, map_.get() + n,
+                                                                    R, R);
+        SGIPC_LAUNCH_CHECK();
+    }
     // the level-1 cycle on P^T r1: its correction is C2 + Z1 (Z1 = Z at the level-1 slots)
     const int g1w = grid_for(c1, 4);
     k_local_solve<BB, TS><<<ndom01 - ndom0, 256, 0, s>>>(done, minv, R, Z, ndom0);

@@ -225,7 +225,7 @@ __global__ __launch_bounds__(kThreads) void k_spmv_tsym(const TM* __restrict__ v
 // Tiles run in `order` (heaviest first inside each XCD's range), so the
 // lightest tiles fill the second round of workgroups.
 // Dynamic LDS: tl[3 tcap], piece[3 * 64 kPieces], xl[3 (kTile + hcap)] of TV.
-template <typename TM, typename TV, bool kDot, bool kFuseP>
+template <typename TM, typename TV, bool kDot, bool kFuseP, bool kSum1 = false>
 __global__ __launch_bounds__(kThreads) void k_spmv_halo(const TM* __restrict__ vals, const uint32_t* __restrict__ cw,
                                                         const int* __restrict__ slice_ptr,
                                                         const int* __restrict__ perm, const int* __restrict__ plen,
@@ -237,7 +237,8 @@ __global__ __launch_bounds__(kThreads) void k_spmv_halo(const TM* __restrict__ v
                                                         double* __restrict__ dot_part, const int* __restrict__ done,
                                                         const TV* __restrict__ zf, TV* __restrict__ p_new,
                                                         const double* __restrict__ rzn,
-                                                        const double* __restrict__ rzo, int n_rz, int n_rows)
+                                                        const double* __restrict__ rzo, int n_rz, int n_rows,
+                                                        Level1Sums l1)
 {
     extern __shared__ __align__(16) unsigned char lds_dyn[];
     TV* const tl = reinterpret_cast<TV*>(lds_dyn);
@@ -373,6 +374,8 @@ __global__ __launch_bounds__(kThreads) void k_spmv_halo(const TM* __restrict__ v
     }
     __syncthreads();
     double dot = 0.0;
+    int ylr = -1;           // with kSum1: this thread's row and its new y
+    TV ys0 = 0, ys1 = 0, ys2 = 0;
     if (threadIdx.x < kTile) {
         const int sq = threadIdx.x >> 6;
         const int lr = s_row[threadIdx.x];
@@ -413,14 +416,24 @@ __global__ __launch_bounds__(kThreads) void k_spmv_halo(const TM* __restrict__ v
                 p_new[3 * i + 2] = v2;
             }
             TV* yi = y + 3 * i;
+            TV o0, o1, o2;
             if (b == TV(0)) {
-                yi[0] = a * r0;
-                yi[1] = a * r1;
-                yi[2] = a * r2;
+                o0 = a * r0;
+                o1 = a * r1;
+                o2 = a * r2;
             } else {
-                yi[0] = a * r0 + b * yi[0];
-                yi[1] = a * r1 + b * yi[1];
-                yi[2] = a * r2 + b * yi[2];
+                o0 = a * r0 + b * yi[0];
+                o1 = a * r1 + b * yi[1];
+                o2 = a * r2 + b * yi[2];
+            }
+            yi[0] = o0;
+            yi[1] = o1;
+            yi[2] = o2;
+            if (kSum1) {
+                ylr = lr;
+                ys0 = o0;
+                ys1 = o1;
+                ys2 = o2;
             }
             if (kDot) dot = (double)v0 * (double)r0 + (double)v1 * (double)r1 + (double)v2 * (double)r2;
         }
@@ -428,6 +441,55 @@ __global__ __launch_bounds__(kThreads) void k_spmv_halo(const TM* __restrict__ v
     if (kDot) {
         const double sblk = block_sum(dot, dred);
         if (threadIdx.x == 0) dot_part[tile] = sblk;
+    }
+    if constexpr (kSum1) {
+        // the tile's new rows back in row order (piece is free once every
+        // row has been summed), then the level-1 sums: a level-1 node lies in
+        // one 32-row level-0 domain, so in one wave of 64 rows, whose lanes
+        // group by node (the leader writes the node's sum)
+        __syncthreads();
+        TV* const yb = piece;
+        if (ylr >= 0) {
+            yb[3 * ylr] = ys0;
+            yb[3 * ylr + 1] = ys1;
+            yb[3 * ylr + 2] = ys2;
+        }
+        __syncthreads();
+        if (threadIdx.x < kTile) {  // waves 0 .. kTile/64 - 1, whole
+            const long u = (long)tile * kTile + threadIdx.x;
+            const bool real = u < n_rows;
+            const TV v0 = real ? yb[3 * threadIdx.x] : TV(0);
+            const TV v1 = real ? yb[3 * threadIdx.x + 1] : TV(0);
+            const TV v2 = real ? yb[3 * threadIdx.x + 2] : TV(0);
+            int node = real ? l1.map1[u] : -1;
+            bool pending = node >= 0;
+            const int lane = threadIdx.x & 63;
+            while (true) {
+                const uint64_t act = __ballot(pending);
+                if (act == 0) break;
+                const int leader = __ffsll((unsigned long long)act) - 1;
+                const int key = __shfl(node, leader, 64);
+                const bool mine = pending && node == key;
+                const TV s0 = wave_sum(mine ? v0 : TV(0));
+                const TV s1 = wave_sum(mine ? v1 : TV(0));
+                const TV s2 = wave_sum(mine ? v2 : TV(0));
+                if (lane == leader) {
+                    const long o = 3 * (long)(l1.npad + key);
+                    y[o] = s0;
+                    y[o + 1] = s1;
+                    y[o + 2] = s2;
+                }
+                if (mine) pending = false;
+            }
+        }
+        if (tile == 0) {
+            for (int k = threadIdx.x; k < l1.pad; k += kThreads) {
+                const long o = 3 * (long)(l1.npad + l1.c1 + k);
+                y[o] = TV(0);
+                y[o + 1] = TV(0);
+                y[o + 2] = TV(0);
+            }
+        }
     }
 }
 
@@ -904,7 +966,8 @@ size_t FullMatrix::halo_static_lds()
                              reinterpret_cast<const void*>(&k_spmv_halo<double, double, false, true>),
                              reinterpret_cast<const void*>(&k_spmv_halo<double, double, true, true>),
                              reinterpret_cast<const void*>(&k_spmv_halo<float, double, true, false>),
-                             reinterpret_cast<const void*>(&k_spmv_halo<float, double, true, true>)};
+                             reinterpret_cast<const void*>(&k_spmv_halo<float, double, true, true>),
+                             reinterpret_cast<const void*>(&k_spmv_halo<double, double, false, false, true>)};
         size_t m = 0;
         for (const void* f : fns) {
             hipFuncAttributes fa;
@@ -931,15 +994,18 @@ long FullMatrix::stored_blocks(hipStream_t s) const
 template <typename TM, typename TV>
 int FullMatrix::launch(hipStream_t s, const TM* vals, TV a, const TV* x, TV b, TV* y, double* dot_part,
                        const int* done, const TV* z, TV* p_new, const double* rzn, const double* rzo,
-                       int n_rz) const
+                       int n_rz, const Level1Sums* l1) const
 {
     SGIPC_CHECK(n_tiles > 0, SGIPC_EMPTY, "matrix was not built");
     const int g = n_tiles;
+    SGIPC_CHECK(l1 == nullptr || (use_halo && !dot_part && !p_new), SGIPC_BAD_ARG,
+                "level-1 sums ride on the plain halo-form SpMV only");
+    const Level1Sums l1v = l1 ? *l1 : Level1Sums{};
     if (use_halo) {
         const size_t dyn = halo_dyn_bytes(sizeof(TV));
-#define SGIPC_HALO(DOT, FUSE)                                                                                       \
+#define SGIPC_HALO(DOT, FUSE, SUM1)                                                                                 \
     do {                                                                                                            \
-        const void* fn = reinterpret_cast<const void*>(&k_spmv_halo<TM, TV, DOT, FUSE>);                            \
+        const void* fn = reinterpret_cast<const void*>(&k_spmv_halo<TM, TV, DOT, FUSE, SUM1>);                      \
         static int max_dyn = -1;                                                                                    \
         if (max_dyn < 0) {                                                                                          \
             hipFuncAttributes fa;                                                                                   \
@@ -949,18 +1015,20 @@ int FullMatrix::launch(hipStream_t s, const TM* vals, TV a, const TV* x, TV b, T
             max_dyn = m;                                                                                            \
         }                                                                                                           \
         SGIPC_CHECK(dyn <= (size_t)max_dyn, SGIPC_CAPACITY, "halo SpMV needs more LDS than a CU has");              \
-        go(&k_spmv_halo<TM, TV, DOT, FUSE>, g, dyn, s, time_start, time_stop, vals,                                \
+        go(&k_spmv_halo<TM, TV, DOT, FUSE, SUM1>, g, dyn, s, time_start, time_stop, vals,                          \
            (const uint32_t*)sell_cols.get(), (const int*)slice_ptr.get(), (const int*)perm.get(),                   \
            (const int*)plen.get(), (const uint32_t*)tinfo.get(), (const int*)halo_ptr.get(),                        \
            (const int*)halo_list.get(), (const int*)order.get(), tl_max, a, x, b, y, dot_part, done, z, p_new, rzn, \
-           rzo, n_rz, n_rows);                                                                                      \
+           rzo, n_rz, n_rows, l1v);                                                                                 \
     } while (0)
-        if (p_new) {
-            if (dot_part) SGIPC_HALO(true, true);
-            else SGIPC_HALO(false, true);
+        if (l1) {
+            SGIPC_HALO(false, false, true);
+        } else if (p_new) {
+            if (dot_part) SGIPC_HALO(true, true, false);
+            else SGIPC_HALO(false, true, false);
         } else {
-            if (dot_part) SGIPC_HALO(true, false);
-            else SGIPC_HALO(false, false);
+            if (dot_part) SGIPC_HALO(true, false, false);
+            else SGIPC_HALO(false, false, false);
         }
 #undef SGIPC_HALO
         SGIPC_LAUNCH_CHECK();
@@ -994,6 +1062,29 @@ int FullMatrix::apply(hipStream_t s, TV a, const TV* x, TV b, TV* y, double* dot
                                     nullptr, 0);
     }
 }
+
+template <typename TV>
+bool FullMatrix::apply_level1_sums(hipStream_t s, TV a, const TV* x, TV b, TV* y, const int* done,
+                                   const Level1Sums& l1) const
+{
+    if (!use_halo || n_tiles <= 0) return false;
+    SGIPC_CHECK(l1.map1 != nullptr && l1.npad >= n_rows && l1.c1 >= 0 && l1.pad >= 0, SGIPC_BAD_ARG,
+                "level-1 sums: bad slot layout");
+    if constexpr (sizeof(TV) == 8) {
+        launch<double, double>(s, sell_vals.get(), a, x, b, y, nullptr, done, nullptr, nullptr, nullptr, nullptr, 0,
+                               &l1);
+    } else {
+        SGIPC_CHECK(has_f32, SGIPC_BAD_ARG, "FP32 copy of the matrix was not built");
+        launch<float, float>(s, sell_vals32.get(), a, x, b, y, nullptr, done, nullptr, nullptr, nullptr, nullptr, 0,
+                             &l1);
+    }
+    return true;
+}
+
+template bool FullMatrix::apply_level1_sums<double>(hipStream_t, double, const double*, double, double*, const int*,
+                                                    const Level1Sums&) const;
+template bool FullMatrix::apply_level1_sums<float>(hipStream_t, float, const float*, float, float*, const int*,
+                                                   const Level1Sums&) const;
 
 int FullMatrix::apply_f32_values(hipStream_t s, double a, const double* x, double b, double* y, double* dot_part,
                                  const int* done) const

@@ -6,6 +6,17 @@
 
 namespace sgipc {
 
+// Level-1 sums of a MAS level-0 residual held in the same slot array (coarse
+// mode 2): for every row u < n_rows, y[npad + map1[u]] += y[u] (3 components,
+// summed per level-1 node in a fixed order), and the level-1 padding slots
+// [npad + c1, npad + c1 + pad) set to zero.
+struct Level1Sums {
+    const int* map1 = nullptr;
+    int npad = 0;
+    int c1 = 0;
+    int pad = 0;
+};
+
 class FullMatrix {
   public:
     // Lay out a unique upper BCOO (rows sorted, row <= col, indices < n).
@@ -17,6 +28,12 @@ class FullMatrix {
     // when *done != 0.
     template <typename TV>
     int apply(hipStream_t s, TV a, const TV* x, TV b, TV* y, double* dot_part, const int* done) const;
+    // y = a*A*x + b*y followed by the level-1 sums of y's rows (l1), in one
+    // launch of the halo form (the epilogue holds whole 512-row tiles, so
+    // whole level-0 domains).  Returns false, having launched nothing, when
+    // the matrix is not in the halo form.
+    template <typename TV>
+    bool apply_level1_sums(hipStream_t s, TV a, const TV* x, TV b, TV* y, const int* done, const Level1Sums& l1) const;
     // FP64 vectors against the FP32 copy of the values (half the matrix
     // bytes), FP64 accumulation; needs want_f32 at build.
     int apply_f32_values(hipStream_t s, double a, const double* x, double b, double* y, double* dot_part,
@@ -62,7 +79,8 @@ class FullMatrix {
     void build_halo(hipStream_t s, int tbits);
     template <typename TM, typename TV>
     int launch(hipStream_t s, const TM* vals, TV a, const TV* x, TV b, TV* y, double* dot_part, const int* done,
-               const TV* z, TV* p_new, const double* rzn, const double* rzo, int n_rz) const;
+               const TV* z, TV* p_new, const double* rzn, const double* rzo, int n_rz,
+               const Level1Sums* l1 = nullptr) const;
 
     DevBuf<uint64_t> keys_, skeys_, rkey_, rkey_sorted_;
     DevBuf<int> idx_, sidx_, cand_, cscan_, up_ptr_, col_ptr_, low_cnt_, len_, iota_, sorted_;

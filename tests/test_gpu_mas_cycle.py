@@ -257,3 +257,38 @@ def test_level0_half_storage_keeps_the_iteration_count(pkg, B):
     finally:
         a_ls.close()
         b_ls.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_vcycle_level1_sums_in_the_spmv_epilogue(pkg, orc, gpu_ctx, monkeypatch, precision):
+    """Coarse mode 2 forms the level-1 sums of r1 in the epilogue of the halo
+    SpMV (r1 = r - w A z0); with the gather form of the fine matrix they come
+    from their own kernel.  Both applies agree to the SpMV forms' summation
+    order (FP64 1e-12, FP32 1e-5)."""
+    s = pkg.scenes.tet_grid_scene("cycle_grid16", 16, E_loguniform=(1e4, 1e9), n_pairs=800)
+    B = 32
+    vals, rows, cols = orc.assemble(s)
+    prec = pkg.FP64 if precision == "fp64" else pkg.FP32_MAS
+    out = []
+    for gather in (False, True):
+        if gather:
+            monkeypatch.setenv("SGIPC_SPMV_GATHER", "1")
+        else:
+            monkeypatch.delenv("SGIPC_SPMV_GATHER", raising=False)
+        mas, keep = _vcycle_mas(pkg, gpu_ctx, s, B, vals, rows, cols, prec)
+        try:
+            rng = np.random.default_rng(11)
+            zs = []
+            for k in range(2):
+                dz = gpu_ctx.empty(3 * s.n, np.float64)
+                mas.preconditioning(gpu_ctx.to_device(rng.standard_normal(3 * s.n)), dz)
+                zs.append(dz.download())
+            out.append(zs)
+        finally:
+            mas.close()
+    monkeypatch.delenv("SGIPC_SPMV_GATHER", raising=False)
+    tol = 1e-12 if precision == "fp64" else 1e-5
+    for a, b in zip(*out):
+        err = np.linalg.norm(a - b) / np.linalg.norm(b)
+        print(f"{precision}: fused vs separate level-1 sums {err:.2e}")
+        assert err <= tol, err
