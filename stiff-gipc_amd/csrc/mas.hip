@@ -1502,50 +1502,52 @@ __global__ __launch_bounds__(1024) void k_sample_normalise(int rows, float* __re
 // (i, map1[j]) and, off the diagonal, A_ij^T to (j, map1[i]).  Entry e = 2b + t
 // (t = 1: transposed); the unused transposed entry of a diagonal block gets
 // row n, which sorts after every real key.
+// Keys (row << cb) | level-1 column, cb the bits of the level-1 count, so the
+// sort covers only the bits in use.
 __global__ void k_ap_keys(const int* __restrict__ rows, const int* __restrict__ cols, int nnz, int n,
-                          const int* __restrict__ map1, uint64_t* __restrict__ keys, int* __restrict__ idx)
+                          const int* __restrict__ map1, int cb, uint64_t* __restrict__ keys, int* __restrict__ idx)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nnz) return;
     const int i = rows[b], j = cols[b];
-    keys[2L * b] = ((uint64_t)(uint32_t)i << 32) | (uint32_t)map1[j];
-    keys[2L * b + 1] = i != j ? (((uint64_t)(uint32_t)j << 32) | (uint32_t)map1[i]) : ((uint64_t)(uint32_t)n << 32);
+    keys[2L * b] = ((uint64_t)(uint32_t)i << cb) | (uint32_t)map1[j];
+    keys[2L * b + 1] = i != j ? (((uint64_t)(uint32_t)j << cb) | (uint32_t)map1[i]) : ((uint64_t)(uint32_t)n << cb);
     idx[2L * b] = 2 * b;
     idx[2L * b + 1] = 2 * b + 1;
 }
 
 // first entry of every distinct real key
-__global__ void k_ap_flags(const uint64_t* __restrict__ k, int m, int n, int* __restrict__ flag)
+__global__ void k_ap_flags(const uint64_t* __restrict__ k, int m, int n, int cb, int* __restrict__ flag)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= m) return;
-    flag[e] = ((int)(k[e] >> 32) < n && (e == 0 || k[e] != k[e - 1])) ? 1 : 0;
+    flag[e] = ((long)(k[e] >> cb) < n && (e == 0 || k[e] != k[e - 1])) ? 1 : 0;
 }
 
 __global__ void k_ap_cols(const uint64_t* __restrict__ k, const int* __restrict__ flag, const int* __restrict__ inc,
-                          int m, int* __restrict__ col, int* __restrict__ seg_start)
+                          int m, int cb, int* __restrict__ col, int* __restrict__ seg_start)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= m || !flag[e]) return;
-    col[inc[e] - 1] = (int)(uint32_t)k[e];
+    col[inc[e] - 1] = (int)(k[e] & ((1ull << cb) - 1ull));
     seg_start[inc[e] - 1] = e;
 }
 
-// ap_ptr[u] = first entry of row u (lower bound of u << 32 among the sorted
+// ap_ptr[u] = first entry of row u (lower bound of u << cb among the sorted
 // keys); for u = n the lower bound is the count of real keys, which ends the
 // last segment (seg_start[n_uniq])
-__global__ void k_ap_ptr(const uint64_t* __restrict__ k, const int* __restrict__ inc, int m, int n, int n_uniq,
-                         int* __restrict__ ptr, int* __restrict__ seg_start)
+__global__ void k_ap_ptr(const uint64_t* __restrict__ k, const int* __restrict__ inc, int m, int n, int cb,
+                         int n_uniq, int* __restrict__ ptr, int* __restrict__ seg_start)
 {
     const int u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u > n) return;
-    const uint64_t key = (uint64_t)(uint32_t)u << 32;
+    const uint64_t key = (uint64_t)(uint32_t)u << cb;
     int lo = 0, hi = m;
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
         if (k[mid] < key) lo = mid + 1; else hi = mid;
     }
-    ptr[u] = (lo < m && (int)(k[lo] >> 32) < n) ? inc[lo] - 1 : n_uniq;
+    ptr[u] = (lo < m && (long)(k[lo] >> cb) < n) ? inc[lo] - 1 : n_uniq;
     if (u == n) seg_start[n_uniq] = lo;
 }
 
@@ -1902,6 +1904,9 @@ void Mas::build_ap(const double* vals, const int* rows, const int* cols, int nnz
     const int m = 2 * nnz;
     int kb = 1;
     while (kb < 31 && (1L << kb) <= (long)n) ++kb;  // row n (the sentinel) fits too
+    int cb = 1;
+    while (cb < 31 && (1L << cb) < (long)h_cnt[1]) ++cb;  // level-1 columns 0 .. c1-1
+    const int end_bit = kb + cb;
     apk_.reserve(m);
     apk_sorted_.reserve(m);
     api_.reserve(m);
@@ -1910,14 +1915,14 @@ void Mas::build_ap(const double* vals, const int* rows, const int* cols, int nnz
     apinc_.reserve(m);
     size_t n1 = 0, n2 = 0;
     SGIPC_HIP(rocprim::radix_sort_pairs(nullptr, n1, apk_.get(), apk_sorted_.get(), api_.get(), api_sorted_.get(),
-                                        (size_t)m, 0, 32 + kb, s));
+                                        (size_t)m, 0, end_bit, s));
     SGIPC_HIP(rocprim::inclusive_scan(nullptr, n2, apf_.get(), apinc_.get(), (size_t)m, rocprim::plus<int>(), s));
     ap_temp_.reserve(n1 > n2 ? n1 : n2);
-    k_ap_keys<<<grid_for(nnz, 256), 256, 0, s>>>(rows, cols, nnz, n, map_.get() + n, apk_.get(), api_.get());
+    k_ap_keys<<<grid_for(nnz, 256), 256, 0, s>>>(rows, cols, nnz, n, map_.get() + n, cb, apk_.get(), api_.get());
     SGIPC_LAUNCH_CHECK();
     SGIPC_HIP(rocprim::radix_sort_pairs(ap_temp_.get(), n1, apk_.get(), apk_sorted_.get(), api_.get(),
-                                        api_sorted_.get(), (size_t)m, 0, 32 + kb, s));
-    k_ap_flags<<<grid_for(m, 256), 256, 0, s>>>(apk_sorted_.get(), m, n, apf_.get());
+                                        api_sorted_.get(), (size_t)m, 0, end_bit, s));
+    k_ap_flags<<<grid_for(m, 256), 256, 0, s>>>(apk_sorted_.get(), m, n, cb, apf_.get());
     SGIPC_LAUNCH_CHECK();
     SGIPC_HIP(rocprim::inclusive_scan(ap_temp_.get(), n2, apf_.get(), apinc_.get(), (size_t)m, rocprim::plus<int>(),
                                       s));
@@ -1929,10 +1934,10 @@ void Mas::build_ap(const double* vals, const int* rows, const int* cols, int nnz
     ap_col_.reserve(n_uniq);
     ap_seg_.reserve(n_uniq + 1);
     ap_blk_.reserve(9L * n_uniq);
-    k_ap_cols<<<grid_for(m, 256), 256, 0, s>>>(apk_sorted_.get(), apf_.get(), apinc_.get(), m, ap_col_.get(),
+    k_ap_cols<<<grid_for(m, 256), 256, 0, s>>>(apk_sorted_.get(), apf_.get(), apinc_.get(), m, cb, ap_col_.get(),
                                                ap_seg_.get());
     SGIPC_LAUNCH_CHECK();
-    k_ap_ptr<<<grid_for(n + 1, 256), 256, 0, s>>>(apk_sorted_.get(), apinc_.get(), m, n, n_uniq, ap_ptr_.get(),
+    k_ap_ptr<<<grid_for(n + 1, 256), 256, 0, s>>>(apk_sorted_.get(), apinc_.get(), m, n, cb, n_uniq, ap_ptr_.get(),
                                                   ap_seg_.get());
     SGIPC_LAUNCH_CHECK();
     k_ap_sum<<<grid_for(n_uniq, 256), 256, 0, s>>>(vals, api_sorted_.get(), ap_seg_.get(), n_uniq, ap_blk_.get());
