@@ -1334,13 +1334,34 @@ __global__ __launch_bounds__(256) void k_prolong(const int* __restrict__ done, c
 // written over the level-1 slots of R.  One wave per level-1 node, lanes over
 // its CSR row.  First pass: c = Z at the level-1 slots (the pre-smoothed
 // iterate), g read from R and saved in G1.  Second pass: c = C2, g from G1.
-template <typename TS, bool SECOND>
+// CHAIN (second pass): C2 is not read but formed here for every neighbour j
+// -- the pre-smoothed level-1 iterate plus the coarse correction of levels
+// 2 .. Lc-1 along going_next, added in that order (l1_chained) -- and
+// written for the wave's own node.
+template <typename TS>
+__device__ __forceinline__ void l1_chained(const TS* __restrict__ Z, const int* __restrict__ going_next, int Lc,
+                                           long sl, TS& a, TS& b, TS& c)
+{
+    a = Z[3 * sl];
+    b = Z[3 * sl + 1];
+    c = Z[3 * sl + 2];
+    for (int l = 2; l < Lc; ++l) {
+        sl = going_next[sl];
+        a += Z[3 * sl];
+        b += Z[3 * sl + 1];
+        c += Z[3 * sl + 2];
+    }
+}
+
+template <typename TS, bool SECOND, bool CHAIN = false>
 __global__ __launch_bounds__(256) void k_l1_residual(const int* __restrict__ done, const int* __restrict__ meta, int c1,
                                                      const int* __restrict__ rp, const int* __restrict__ colx,
                                                      const float* __restrict__ blk, const float* __restrict__ sc,
                                                      TS* __restrict__ R, const TS* __restrict__ Z,
-                                                     TS* __restrict__ G1, const TS* __restrict__ C2)
+                                                     TS* __restrict__ G1, TS* __restrict__ C2,
+                                                     const int* __restrict__ going_next = nullptr, int Lc = 0)
 {
+    static_assert(!CHAIN || SECOND, "the chain is formed in the second pass");
     if (done != nullptr && *done) return;
     const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
@@ -1353,7 +1374,15 @@ __global__ __launch_bounds__(256) void k_l1_residual(const int* __restrict__ don
         const float* b = blk + 9L * e;
         const float sj = sc[j];
         const TS wj = sj > 0.f ? (TS)1 / (TS)sj : (TS)0;
-        const TS c0 = wj * cv[3L * j], c1v = wj * cv[3L * j + 1], c2v = wj * cv[3L * j + 2];
+        TS x0, x1, x2;
+        if constexpr (CHAIN) {
+            l1_chained(Z, going_next, Lc, (long)off1 + j, x0, x1, x2);
+        } else {
+            x0 = cv[3L * j];
+            x1 = cv[3L * j + 1];
+            x2 = cv[3L * j + 2];
+        }
+        const TS c0 = wj * x0, c1v = wj * x1, c2v = wj * x2;
         a0 += (TS)b[0] * c0 + (TS)b[1] * c1v + (TS)b[2] * c2v;
         a1 += (TS)b[3] * c0 + (TS)b[4] * c1v + (TS)b[5] * c2v;
         a2 += (TS)b[6] * c0 + (TS)b[7] * c1v + (TS)b[8] * c2v;
@@ -1362,6 +1391,13 @@ __global__ __launch_bounds__(256) void k_l1_residual(const int* __restrict__ don
     a1 = wave_sum(a1);
     a2 = wave_sum(a2);
     if (lane != 0) return;
+    if constexpr (CHAIN) {
+        TS x0, x1, x2;
+        l1_chained(Z, going_next, Lc, (long)off1 + k, x0, x1, x2);
+        C2[3L * k] = x0;
+        C2[3L * k + 1] = x1;
+        C2[3L * k + 2] = x2;
+    }
     const long o = 3 * (long)(off1 + k);
     TS g0, g1, g2;
     if (!SECOND) {
@@ -1381,29 +1417,6 @@ __global__ __launch_bounds__(256) void k_l1_residual(const int* __restrict__ don
     R[o] = g0 - wk * a0;
     R[o + 1] = g1 - wk * a1;
     R[o + 2] = g2 - wk * a2;
-}
-
-// C2 = (pre-smoothed level-1 iterate) + the coarse correction of levels
-// 2..Lc-1 along going_next.
-template <typename TS>
-__global__ __launch_bounds__(256) void k_l1_chain(const int* __restrict__ done, const int* __restrict__ meta, int Lc,
-                                                  const int* __restrict__ going_next, int c1, const TS* __restrict__ Z,
-                                                  TS* __restrict__ C2)
-{
-    if (done != nullptr && *done) return;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= c1) return;
-    long sl = meta[kMetaOff + 1] + k;
-    TS a = Z[3 * sl], b = Z[3 * sl + 1], c = Z[3 * sl + 2];
-    for (int l = 2; l < Lc; ++l) {
-        sl = going_next[sl];
-        a += Z[3 * sl];
-        b += Z[3 * sl + 1];
-        c += Z[3 * sl + 2];
-    }
-    C2[3L * k] = a;
-    C2[3L * k + 1] = b;
-    C2[3L * k + 2] = c;
 }
 
 // Inputs of the learned correction per level-1 node k (one wave per node):
@@ -2091,10 +2104,10 @@ int Mas::finish_v_t(hipStream_t s, TS* R, TS* Z, const TS* minv, const int* done
             SGIPC_LAUNCH_CHECK();
         }
     }
-    k_l1_chain<TS><<<grid_for(c1, 256), 256, 0, s>>>(done, meta_.get(), Lc, going_next_.get(), c1, Z, C2);
-    SGIPC_LAUNCH_CHECK();
-    k_l1_residual<TS, true><<<g1w, 256, 0, s>>>(done, meta_.get(), c1, graph_.row_ptr.get(), graph_.col.get(),
-                                                 graph_.blk.get(), graph_.s.get(), R, Z, G1, C2);
+    // the chained correction C2 formed inside the second residual
+    k_l1_residual<TS, true, true><<<g1w, 256, 0, s>>>(done, meta_.get(), c1, graph_.row_ptr.get(), graph_.col.get(),
+                                                       graph_.blk.get(), graph_.s.get(), R, Z, G1, C2,
+                                                       going_next_.get(), Lc);
     SGIPC_LAUNCH_CHECK();
     using V4 = typename Vec4Of<TS>::T;
     c4_.reserve((size_t)c1);  // double4 elements: room for c1 float4 too
@@ -2190,10 +2203,9 @@ int Mas::finish_t(hipStream_t s, TS* R, TS* Z, const TS* minv, const int* done, 
                 SGIPC_LAUNCH_CHECK();
             }
         }
-        k_l1_chain<TS><<<grid_for(c1, 256), 256, 0, s>>>(done, meta_.get(), Lc, going_next_.get(), c1, Z, C2);
-        SGIPC_LAUNCH_CHECK();
-        k_l1_residual<TS, true><<<g1w, 256, 0, s>>>(done, meta_.get(), c1, graph_.row_ptr.get(), graph_.col.get(),
-                                                     graph_.blk.get(), graph_.s.get(), R, Z, G1, C2);
+        k_l1_residual<TS, true, true><<<g1w, 256, 0, s>>>(done, meta_.get(), c1, graph_.row_ptr.get(),
+                                                           graph_.col.get(), graph_.blk.get(), graph_.s.get(), R, Z,
+                                                           G1, C2, going_next_.get(), Lc);
         SGIPC_LAUNCH_CHECK();
         // post-smoothing
         k_local_solve<BB, TS><<<ndom01 - ndom0, 256, 0, s>>>(done, minv, R, Z, ndom0);
