@@ -599,6 +599,23 @@ __global__ void k_slice_w(const int* __restrict__ plen, int n_slices, int* __res
     slice_w[q] = q == n_slices ? 0 : plen[q * 64];  // longest row of the slice (sorted in its tile)
 }
 
+// The 64 lanes of a wave each need one 9-double block (b < 0: none): they
+// load the 576 words together, 64 consecutive words per instruction (the
+// blocks of ~7 lanes), through the wave's LDS buffer; afterwards lane l finds
+// its block at buf[9 l].
+__device__ __forceinline__ void wave_gather9(const double* __restrict__ vals, int b, int lane, double* __restrict__ buf)
+{
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        const int w = 64 * t + lane;
+        const int sl = w / 9, e = w - 9 * (w / 9);
+        const int bs = __shfl(b, sl, 64);
+        buf[w] = bs >= 0 ? vals[9L * bs + e] : 0.0;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
+// slot entries of a block given column major (tr: stored transposed)
 __device__ __forceinline__ void put9(const double* __restrict__ s, bool tr, long slot, int lane,
                                      double* __restrict__ sv, float* __restrict__ sv32)
 {
@@ -614,33 +631,56 @@ __device__ __forceinline__ void put9(const double* __restrict__ s, bool tr, long
 
 // One thread per tile position: the row's upper blocks (diagonal first, by
 // column), then its cross-tile lower entries (by row), then zero padding.
-__global__ void k_pack(const double* __restrict__ vals, const uint32_t* __restrict__ up_word,
-                       const uint64_t* __restrict__ skeys, const int* __restrict__ sidx,
-                       const int* __restrict__ up_ptr, const int* __restrict__ col_ptr, const int* __restrict__ perm,
-                       const int* __restrict__ slice_ptr, const int* __restrict__ slice_w, int n_pos, int kbits,
-                       double* __restrict__ sv, float* __restrict__ sv32, uint32_t* __restrict__ scw)
+// The lanes of a wave walk their rows in step (a block per lane per step,
+// loaded together by wave_gather9); the loops are per wave.
+__global__ __launch_bounds__(256) void k_pack(const double* __restrict__ vals, const uint32_t* __restrict__ up_word,
+                                              const uint64_t* __restrict__ skeys, const int* __restrict__ sidx,
+                                              const int* __restrict__ up_ptr, const int* __restrict__ col_ptr,
+                                              const int* __restrict__ perm, const int* __restrict__ slice_ptr,
+                                              const int* __restrict__ slice_w, int n_pos, int kbits,
+                                              double* __restrict__ sv, float* __restrict__ sv32,
+                                              uint32_t* __restrict__ scw)
 {
+    __shared__ double sbuf[4][576];
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n_pos) return;
+    const bool valid = p < n_pos;  // every lane stays for the wave's exchanges
     const int q = p >> 6, lane = p & 63;
-    const int i = perm[p];
-    const long S = slice_ptr[q];
-    const int W = slice_w[q];
+    double* const buf = sbuf[threadIdx.x >> 6];
+    const int i = valid ? perm[p] : -1;
+    const long S = valid ? slice_ptr[q] : 0;
+    const int W = valid ? slice_w[q] : 0;
+    const int u0 = i >= 0 ? up_ptr[i] : 0, nu = i >= 0 ? up_ptr[i + 1] - u0 : 0;
+    const int l0 = i >= 0 ? col_ptr[i] : 0, nl = i >= 0 ? col_ptr[i + 1] - l0 : 0;
     int k = 0;
-    if (i >= 0) {
-        for (int b = up_ptr[i]; b < up_ptr[i + 1]; ++b, ++k) {
+    const int mu = wave_max(nu);
+    for (int t = 0; t < mu; ++t) {
+        const int b = t < nu ? u0 + t : -1;
+        wave_gather9(vals, b, lane, buf);
+        if (b >= 0) {
             scw[(S + k) * 64 + lane] = up_word[b];
-            put9(vals + 9 * (long)b, false, S + k, lane, sv, sv32);
-        }
-        for (int s = col_ptr[i]; s < col_ptr[i + 1]; ++s) {
-            const int r = (int)(skeys[s] & ((1ull << kbits) - 1ull));
-            const int b = sidx[s];
-            if (r == i || (up_word[b] & kInTile)) continue;
-            scw[(S + k) * 64 + lane] = (uint32_t)r;
-            put9(vals + 9 * (long)b, true, S + k, lane, sv, sv32);
+            put9(buf + 9 * lane, false, S + k, lane, sv, sv32);
             ++k;
         }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // buf is reused
     }
+    const int ml = wave_max(nl);
+    for (int t = 0; t < ml; ++t) {
+        int b = -1, r = 0;
+        if (t < nl) {
+            const int e = l0 + t;
+            r = (int)(skeys[e] & ((1ull << kbits) - 1ull));
+            b = sidx[e];
+            if (r == i || (up_word[b] & kInTile)) b = -1;
+        }
+        wave_gather9(vals, b, lane, buf);
+        if (b >= 0) {
+            scw[(S + k) * 64 + lane] = (uint32_t)r;
+            put9(buf + 9 * lane, true, S + k, lane, sv, sv32);
+            ++k;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+    if (!valid) return;
     for (; k < W; ++k) {
         scw[(S + k) * 64 + lane] = (uint32_t)(i >= 0 ? i : 0);
 #pragma unroll
