@@ -166,7 +166,8 @@ def main():
             log(f"step {k}: {times[-1]:.2f} ms iters={r.iterations} outer={r.outer_iterations} "
                 f"res={r.true_rel_residual:.2e} assemble={r.timing['ms_assemble']:.2f} "
                 f"convert={r.timing['ms_convert']:.2f} mas={r.timing['ms_mas_build']:.2f} "
-                f"pcg={r.timing['ms_pcg']:.2f}")
+                f"pcg={r.timing['ms_pcg']:.2f} mode={r.coarse_mode} l2exact={int(r.level2_exact)} "
+                f"omega={r.omega:.3g}")
         ctx.synchronize()
         total_ms = (time.perf_counter() - tt0) * 1e3
         ms_step = total_ms / args.steps

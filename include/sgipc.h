@@ -330,6 +330,13 @@ typedef struct sgipc_pcg_stats {
     int learned;             /* 1: the learned coarse correction was applied; 2: a run  */
                              /* with it did not converge within twice the plain count   */
                              /* (automatic mode) and the solve was redone without it    */
+    /* system solves (diagnostics; no counterpart in the reference): the MAS    */
+    /* coarse mode of the run that returned, whether its level-1 cycle had the  */
+    /* exact dense level-2 solve (0: the factorisation was not usable and the    */
+    /* additive levels >= 2 stood in), and the mode-2 smoothing weight          */
+    int coarse_mode;
+    int level2_exact;
+    double omega;
 } sgipc_pcg_stats;
 
 void sgipc_pcg_default_opts(sgipc_pcg_opts* opts);

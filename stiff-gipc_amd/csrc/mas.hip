@@ -1591,21 +1591,21 @@ __global__ void k_ap_sum(const double* __restrict__ vals, const int* __restrict_
 }
 
 // y = Minv x for the dense FP64 inverse of the level-2 operator (m x m,
-// row-major, symmetric); one wave per row.
+// row-major, symmetric); one workgroup per row (the matrix is small: many
+// short rows in flight at once hide the load latency), summed in a fixed order.
 template <typename TS>
 __global__ __launch_bounds__(256) void k_dense_gemv(const int* __restrict__ done, int m,
                                                     const double* __restrict__ Minv, const TS* __restrict__ x,
                                                     TS* __restrict__ y)
 {
+    __shared__ double smem[16];
     if (done != nullptr && *done) return;
-    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (row >= m) return;
+    const int row = blockIdx.x;
     const double* a = Minv + (long)row * m;
     double acc = 0.0;
-    for (int c = lane; c < m; c += 64) acc += a[c] * (double)x[c];
-    acc = wave_sum(acc);
-    if (lane == 0) y[row] = (TS)acc;
+    for (int c = threadIdx.x; c < m; c += blockDim.x) acc += a[c] * (double)x[c];
+    acc = block_sum(acc, smem);
+    if (threadIdx.x == 0) y[row] = (TS)acc;
 }
 
 // Dense symmetric level-2 operator from its upper BCOO (column-major blocks).
@@ -2093,7 +2093,7 @@ int Mas::finish_v_t(hipStream_t s, TS* R, TS* Z, const TS* minv, const int* done
     int Lc = L;
     if (dense_m_ > 0) {
         restrict_up_level(2);
-        k_dense_gemv<TS><<<grid_for(dense_m_, 4), 256, 0, s>>>(done, dense_m_, dense_inv_.get(), R + 3L * h_off[2],
+        k_dense_gemv<TS><<<dense_m_, 256, 0, s>>>(done, dense_m_, dense_inv_.get(), R + 3L * h_off[2],
                                                                 Z + 3L * h_off[2]);
         SGIPC_LAUNCH_CHECK();
         Lc = 3;
@@ -2192,7 +2192,7 @@ int Mas::finish_t(hipStream_t s, TS* R, TS* Z, const TS* minv, const int* done, 
         int Lc = L;
         if (dense_m_ > 0) {
             restrict_up_level(2);
-            k_dense_gemv<TS><<<grid_for(dense_m_, 4), 256, 0, s>>>(done, dense_m_, dense_inv_.get(),
+            k_dense_gemv<TS><<<dense_m_, 256, 0, s>>>(done, dense_m_, dense_inv_.get(),
                                                                     R + 3L * h_off[2], Z + 3L * h_off[2]);
             SGIPC_LAUNCH_CHECK();
             Lc = 3;

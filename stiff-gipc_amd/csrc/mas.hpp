@@ -35,6 +35,7 @@ class Mas {
     // rr partials apply_update_xr writes
     int rr_partials() const { return (h_off[1] + 255) / 256; }
     int levels(int* offsets, int* counts) const;
+    bool level2_exact() const { return dense_m_ > 0; }
     // 0: additive levels (MASPreconditioner::preconditioning); 1: a symmetric
     // V(1,1) cycle on level 1 (level-1 domain solves as smoother, the levels
     // >= 2 as its coarse correction) with the level-0 domain solves additive;

@@ -454,6 +454,7 @@ int system_solve(sgipc_system* sys, const double* b, double* x, double* grad, co
         // the weight (kept for later solves), and after two halvings with the
         // level-1 cycle alone
         Mas* mm = sys->mas->m.get();
+        int mode_used = mm->coarse_mode;
         if (status == SGIPC_NOT_SPD && mm->coarse_mode == 2 && !used) {
             for (int tries = 0; tries < 3 && status == SGIPC_NOT_SPD; ++tries) {
                 if (tries < 2) {
@@ -461,6 +462,7 @@ int system_solve(sgipc_system* sys, const double* b, double* x, double* grad, co
                     status = run_pcg(o);
                 } else {
                     mm->coarse_mode = 1;
+                    mode_used = 1;
                     try {
                         status = run_pcg(o);
                     } catch (...) {
@@ -472,6 +474,9 @@ int system_solve(sgipc_system* sys, const double* b, double* x, double* grad, co
             }
         }
         pst.learned = used ? 1 : (abandoned ? 2 : 0);
+        pst.coarse_mode = mode_used;
+        pst.level2_exact = mm->level2_exact() ? 1 : 0;
+        pst.omega = mm->omega0;
         if (record == nullptr && status == SGIPC_OK) {
             (used ? sys->learned_it : sys->plain_it) = pst.iterations;
             if (sys->learned_mode == 2 && sys->plain_it >= 0 && sys->learned_it >= 0 &&
