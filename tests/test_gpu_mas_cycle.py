@@ -210,6 +210,9 @@ def test_vcycle_solves_in_fewer_iterations(pkg, B, precision):
         assert b.true_rel_residual <= 1e-5
         assert b.iterations < a.iterations
         assert np.linalg.norm(a.x - b.x) <= 1e-4 * np.linalg.norm(a.x)
+        # the solve statistics name the path: mode 2 with the exact level-2 solve
+        assert (b.coarse_mode, b.level2_exact, b.omega) == (2, True, 0.4)
+        assert (a.coarse_mode, a.level2_exact) == (1, True)
     finally:
         cyc.close()
         two.close()
@@ -227,8 +230,11 @@ def test_vcycle_recovers_from_a_too_large_weight(pkg, precision):
         ls.set_smoothing_weight(1.9)
         a = ls.solve(tol=1e-5, precision=precision)
         b = ref.solve(tol=1e-5, precision=precision)
-        print(f"{precision}: w=1.9 start -> {a.iterations} iterations; default w: {b.iterations}")
+        print(f"{precision}: w=1.9 start -> {a.iterations} iterations (mode {a.coarse_mode}, w {a.omega}); "
+              f"default w: {b.iterations}")
         assert a.status == 0 and a.true_rel_residual <= 1e-5
+        # the retries halved the weight (kept), or fell back to mode 1
+        assert a.omega < 1.9 and a.coarse_mode in (1, 2)
         assert np.linalg.norm(a.x - b.x) <= 1e-4 * np.linalg.norm(b.x)
     finally:
         ls.close()
