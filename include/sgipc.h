@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define SGIPC_ABI_VERSION 1
+#define SGIPC_ABI_VERSION 2
 
 enum sgipc_status {
     SGIPC_OK = 0,
@@ -128,27 +128,34 @@ int sgipc_quad_bending_gradient_hessian(sgipc_ctx* ctx, const double* x, const d
 
 /* Replaces ABDSystem::_cal_abd_body_gradient_and_hessian and the body part of
  * _setup_abd_system_hessian (StiffGIPC/abd_system/abd_system_function/
- * setup_abd_system_gradient_and_hessian.cu:200-289,464-476): per body
- * H = M + PSD(kappa vol dt^2 d2V/da2), G = M (q - q~) + kappa vol dt^2 dV/da
- * (G = 0, H = M when fixed), 10 upper blocks at rows 4b..4b+3 written to
- * tvals[10*b ...]; grad (optional) receives 12 per body. */
+ * setup_abd_system_gradient_and_hessian.cu:200-339,464-476): per body
+ * H = M + PSD(kappa vol dt^2 d2V/da2), G = M (q - q~) + kappa vol dt^2 dV/da,
+ * 10 upper blocks at rows 4b..4b+3 written to tvals[10*b ...]; grad
+ * (optional) receives 12 per body.  body_fixed[b] (NULL: all free) is the
+ * reference's BodyBoundaryType: 0 free, 1 fixed (G = 0, H = M), 2 motor
+ * (G += W (q - q_p), H += W on the a2, a3 rows, W = motor_strength times the
+ * body's 6x6 mass block there, q_p = q~ turned by motor_speed * dt about the
+ * body x axis; lines 271-339). */
 int sgipc_abd_body_gradient_hessian(sgipc_ctx* ctx, int n_bodies, const double* q,
                                     const double* q_tilde, const double* mass10,
                                     const double* body_volume, double kappa, double dt,
-                                    const int* body_fixed, double* grad, double* tvals, int* trows,
-                                    int* tcols);
+                                    const int* body_fixed, double motor_speed, double motor_strength,
+                                    double* grad, double* tvals, int* trows, int* tcols);
 
 /* Replaces GIPC::partitionContactHessian with the ABD contact transforms
  * (StiffGIPC/GIPC.cu:10080-10256; setup_abd_system_gradient_and_hessian.cu:
  * 87-141,424-558): point-level upper contact blocks become, class by class in
  * input order, FEM-FEM blocks (rows shifted by 4*n_bodies - n_abd_points),
  * 4 ABD-FEM blocks J^T H and 16 ABD-ABD blocks J_i^T H J_j (lower ones
- * zeroed).  *out_n receives the count; SGIPC_CAPACITY if it exceeds
- * capacity (nothing written then). */
+ * zeroed).  Blocks of a fixed body (body_fixed == 1) and blocks with a FEM
+ * end whose fem_boundary_type (per FEM vertex, NULL: none) is nonzero are
+ * written as zeros (GIPC.cu:10332-10363; setup_abd_system_gradient_and_
+ * hessian.cu:564-568).  *out_n receives the count; SGIPC_CAPACITY if it
+ * exceeds capacity (nothing written then). */
 int sgipc_contact_partition(sgipc_ctx* ctx, const double* vals, const int* rows, const int* cols,
                             int n, int n_abd_points, const int* point_body, const double* x_bar,
-                            int n_bodies, const int* body_fixed, double* ovals, int* orows,
-                            int* ocols, int capacity, int* out_n);
+                            int n_bodies, const int* body_fixed, const int* fem_boundary_type,
+                            double* ovals, int* orows, int* ocols, int capacity, int* out_n);
 
 /* Replaces ABDSystem::_cal_abd_system_preconditioner (setup_abd_system_
  * gradient_and_hessian.cu:560-655): inverse of every body's 12x12 diagonal
@@ -167,8 +174,9 @@ int sgipc_mass_diagonal(sgipc_ctx* ctx, const double* mass, int n, int hess_offs
 
 /* Replaces Converter::srbk_convert (StiffGIPC/linear_system/utils/converter.cu:
  * 32-51): radix sort by (row,col), sum duplicates, unique upper BCOO written to
- * ovals/orows/ocols (capacity blocks); *out_nnz receives the count.  The
- * input arrays are left unchanged. */
+ * ovals/orows/ocols (capacity blocks); *out_nnz receives the count, also
+ * with SGIPC_CAPACITY (nothing past capacity is written; call again with at
+ * least that many).  The input arrays are left unchanged. */
 int sgipc_convert_srbk(sgipc_ctx* ctx, const double* tvals, const int* trows, const int* tcols,
                        int n_triplets, double* ovals, int* orows, int* ocols, int capacity,
                        int* out_nnz);
@@ -255,10 +263,13 @@ int sgipc_mas_tables(sgipc_mas* mas, int* coarse_table, int* going_next);
 /* Coarse levels of the apply.  0 (default): additive over all levels, as
  * MASPreconditioner::preconditioning (MASPreconditioner.cu:2248-2289).
  * 1: no counterpart in the reference -- the levels >= 1 become a symmetric
- * V(1,1) cycle on level 1 (level-1 domain solves as the smoother, the additive
- * levels >= 2 as its coarse correction, residuals with the level-1 Galerkin
- * operator); level 0 stays additive, so the preconditioner is still
- * symmetric.
+ * V(1,1) cycle on level 1 (level-1 domain solves as the smoother, residuals
+ * with the level-1 Galerkin operator, and as its coarse correction the exact
+ * level-2 solve: a dense FP64 inverse of A_2 = P_12^T A_1 P_12 built by a
+ * blocked Gauss-Jordan on the device, up to 4608 unknowns, beyond which the
+ * additive levels >= 2 stand in); level 0 stays additive, so the
+ * preconditioner is still symmetric.  A level-2 operator whose inverse meets
+ * a pivot <= 0 makes sgipc_mas_set_preconditioner_bcoo return SGIPC_NOT_SPD.
  * 2: no counterpart in the reference -- mode 1's level-1 cycle becomes the
  * coarse solve of a symmetric two-grid cycle on level 0: z0 = w M0 r,
  * r1 = r - A z0, c = (level-1 cycle of P^T r1), r2 = r1 - A P c,
@@ -269,8 +280,10 @@ int sgipc_mas_tables(sgipc_mas* mas, int* coarse_table, int* going_next);
  * standalone MAS keeps a copy made at set_preconditioner_bcoo.  Symmetric
  * positive definite for weights below 2 / lambda_max(M0 A); PCG stops with
  * SGIPC_NOT_SPD when r.z <= 0 shows that it is not, and the system solver
- * then redoes the solve with half the weight (twice at most, the weight kept
- * for later solves), then with mode 1.
+ * then redoes the solve with half the weight (twice at most and not below
+ * 0.05; a weight that cures it is kept for later solves), then with mode 1
+ * and the weight restored.  Solve statistics report the path the apply took
+ * (sgipc_pcg_stats.coarse_mode), which can be lower than the mode set.
  * Modes take effect at the next sgipc_mas_set_preconditioner_bcoo (or system
  * solve). */
 int sgipc_mas_set_coarse_mode(sgipc_mas* mas, int mode);
@@ -291,6 +304,11 @@ int sgipc_mas_time_apply(sgipc_mas* mas, const double* r, double* z, int reps, d
 /* Inverse domain matrices as FP64 upper blocks (device output,
  * total_slots/B * B(B+1)/2 * 9 doubles). */
 int sgipc_mas_inverse_blocks(sgipc_mas* mas, double* out);
+/* Diagnostics (no counterpart in the reference; the reference's additive MAS
+ * has no dense level): the dense inverse of the level-2 operator that coarse
+ * modes 1 and 2 apply, as m x m row-major FP64 (device output, NULL: size
+ * only); *m = 0 when the build has no exact level-2 solve. */
+int sgipc_mas_level2_inverse(sgipc_mas* mas, double* out, int* m);
 
 /* ---- PCG (PCGSolver::solve, StiffGIPC/linear_system/solver/pcg_solver.cu:158-332) */
 typedef struct sgipc_pcg_opts {
@@ -398,9 +416,20 @@ typedef struct sgipc_scene_desc {
     const double* q_tilde;    /* 12 per body                          */
     const double* mass10;     /* 10 per body: m, m xb, m xb xb^T upper */
     const double* body_volume;
-    const int* body_fixed;    /* per body, nonzero = fixed            */
+    const int* body_fixed;    /* per body: 0 free, 1 fixed, 2 motor (or NULL) */
     const int* point_body;    /* per body point                       */
     const double* x_bar;      /* 3 per body point, body-frame rest    */
+    /* Dirichlet vertices (TetMesh.BoundaryType; GIPC.cu:10332-10363,
+     * 10444-10462, fem_linear_subsystem.cu:56-84): per FEM vertex, nonzero =
+     * fixed.  Every element and contact block with an end on a fixed vertex
+     * is zeroed (its mass block stays) and its gradient rows are zero, so the
+     * solve leaves it in place.  NULL: none. */
+    const int* boundary_type;
+    /* motor bodies (body_fixed == 2): angular speed (rad/s) about the body x
+     * axis and strength (multiple of the body mass block); the reference's
+     * defaults are 31.4 and 10 (abd_system_parms.h:13-14) */
+    double motor_speed;
+    double motor_strength;
 } sgipc_scene_desc;
 
 typedef struct sgipc_solve_timing {

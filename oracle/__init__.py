@@ -72,12 +72,14 @@ def lib():
         L.orc_pcg.restype = ctypes.c_int
         L.orc_set_fault.argtypes = [ctypes.c_int]
         L.orc_abd_body.argtypes = [ctypes.c_int, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p, ctypes.c_double,
-                                   ctypes.c_double, c_int_p, c_dbl_p, c_dbl_p, c_int_p, c_int_p]
+                                   ctypes.c_double, c_int_p, ctypes.c_double, ctypes.c_double, c_dbl_p, c_dbl_p,
+                                   c_int_p, c_int_p]
         L.orc_abd_shape_energy.argtypes = [c_dbl_p]
         L.orc_abd_shape_energy.restype = ctypes.c_double
         L.orc_abd_shape_grad_hess.argtypes = [c_dbl_p, c_dbl_p, c_dbl_p]
         L.orc_abd_contact_map.argtypes = [ctypes.c_long, c_dbl_p, c_int_p, c_int_p, ctypes.c_int, c_int_p, c_dbl_p,
-                                          ctypes.c_int, c_int_p, c_dbl_p, c_int_p, c_int_p]
+                                          ctypes.c_int, c_int_p, c_int_p, c_dbl_p, c_int_p, c_int_p]
+        L.orc_dirichlet_zero.argtypes = [ctypes.c_long, c_int_p, c_int_p, ctypes.c_int, c_int_p, c_dbl_p]
         L.orc_abd_contact_map.restype = ctypes.c_long
         L.orc_abd_jacobi_build.argtypes = [ctypes.c_long, c_dbl_p, c_int_p, c_int_p, ctypes.c_int, c_dbl_p]
         L.orc_abd_jacobi_build.restype = ctypes.c_int
@@ -206,7 +208,8 @@ def abd_body(scene, with_grad=True):
                            _ptr(np.ascontiguousarray(scene.q_tilde), c_dbl_p),
                            _ptr(np.ascontiguousarray(scene.mass10), c_dbl_p),
                            _ptr(np.ascontiguousarray(scene.body_volume), c_dbl_p), scene.kappa_abd, scene.dt,
-                           _ptr(fixed, c_int_p), _ptr(grad, c_dbl_p), _ptr(vals, c_dbl_p), _ptr(rows, c_int_p),
+                           _ptr(fixed, c_int_p), getattr(scene, "motor_speed", 31.4),
+                           getattr(scene, "motor_strength", 10.0), _ptr(grad, c_dbl_p), _ptr(vals, c_dbl_p), _ptr(rows, c_int_p),
                            _ptr(cols, c_int_p))
     return vals, rows, cols, grad
 
@@ -234,7 +237,8 @@ def abd_contact_map(scene, vals, rows, cols):
                                   scene.n_abd_points, _ptr(np.ascontiguousarray(scene.point_body, dtype=np.int32), c_int_p),
                                   _ptr(np.ascontiguousarray(scene.x_bar), c_dbl_p), scene.n_bodies,
                                   _ptr(np.ascontiguousarray(scene.body_fixed, dtype=np.int32), c_int_p),
-                                  _ptr(ov, c_dbl_p), _ptr(orr, c_int_p), _ptr(oc, c_int_p))
+                                  _ptr(_btype(scene), c_int_p), _ptr(ov, c_dbl_p), _ptr(orr, c_int_p),
+                                  _ptr(oc, c_int_p))
     return ov[:m].copy(), orr[:m].copy(), oc[:m].copy()
 
 
@@ -260,17 +264,70 @@ def mass_diag(scene, hess_offset=0):
     return vals, rows, cols
 
 
+def _btype(scene):
+    bt = getattr(scene, "boundary_type", None)
+    return None if bt is None else np.ascontiguousarray(bt, dtype=np.int32)
+
+
+def dirichlet_zero(vals, rows, cols, row0, btype):
+    """Zero the triplets with an end on a Dirichlet FEM vertex (in place)."""
+    if btype is None or rows.size == 0:
+        return vals
+    lib().orc_dirichlet_zero(rows.size, _ptr(np.ascontiguousarray(rows, dtype=np.int32), c_int_p),
+                             _ptr(np.ascontiguousarray(cols, dtype=np.int32), c_int_p), row0,
+                             _ptr(btype, c_int_p), _ptr(vals, c_dbl_p))
+    return vals
+
+
+def point_gradient_to_system(scene, gp):
+    """Contact gradient of the points (3 per point: body points, then FEM
+    points) in the system rows: J^T g into the 12 rows of each non-fixed body
+    (setup_abd_system_gradient_and_hessian.cu:349-385), g into the FEM vertex
+    rows after the 4 n_bodies body rows."""
+    nb, na = scene.n_bodies, scene.n_abd_points
+    g = np.zeros(3 * scene.n_sys)
+    g[12 * nb:] = gp[3 * na:]
+    gpp = gp[:3 * na].reshape(na, 3)
+    keep = np.asarray(scene.body_fixed)[scene.point_body] != 1
+    for i in np.nonzero(keep)[0]:
+        b = scene.point_body[i]
+        jt = np.zeros(12)
+        jt[:3] = gpp[i]
+        jt[3:] = np.outer(gpp[i], scene.x_bar[i]).reshape(-1)   # row r of A: g_r xbar
+        g[12 * b:12 * b + 12] += jt
+    return g
+
+
 def triplets(scene, with_grad=False):
     """All triplets in the reference's arena order (GIPC.cu:10291-10485):
     contacts, affine bodies, tets, bending hinges, cloth triangles, mass.  On
     a scene with affine bodies the contact blocks are mapped to the system
-    rows by class first, and the FEM rows start at 4 * n_bodies."""
+    rows by class first, and the FEM rows start at 4 * n_bodies.  Triplets
+    with an end on a Dirichlet vertex (scene.boundary_type) are zeroed, the
+    mass diagonal excepted (GIPC.cu:10332-10363,10444-10462).  The gradient
+    is the system right-hand side the subsystems assemble: body rows
+    M(q - q~) + shape + J^T (contact gradient), FEM rows element + contact
+    gradients, zero at Dirichlet vertices (fem_linear_subsystem.cu:56-84,
+    abd_linear_subsystem.cu:20-32)."""
     nb = getattr(scene, "n_bodies", 0)
     off = 4 * nb
-    bv, br, bc, bg = barrier_pt(scene, with_grad=with_grad and not nb)
+    bt = _btype(scene)
     if nb:
+        bv, br, bc, _ = barrier_pt(scene, with_grad=False)
+        gp = None
+        if with_grad and scene.n_pairs:
+            gp = np.zeros(3 * (scene.n_abd_points + scene.n))
+            lib().orc_barrier_pt(scene.n_pairs, _ptr(np.ascontiguousarray(scene.x), c_dbl_p),
+                                 _ptr(np.ascontiguousarray(scene.pairs, dtype=np.int32), c_int_p),
+                                 _ptr(np.ascontiguousarray(scene.pair_x, dtype=np.float64), c_dbl_p),
+                                 scene.dhat ** 2, scene.kappa, 0, _ptr(gp, c_dbl_p),
+                                 _ptr(np.zeros((16 * scene.n_pairs, 9)), c_dbl_p),
+                                 _ptr(np.zeros(16 * scene.n_pairs, dtype=np.int32), c_int_p),
+                                 _ptr(np.zeros(16 * scene.n_pairs, dtype=np.int32), c_int_p))
         bv, br, bc = abd_contact_map(scene, bv, br, bc)
         bg = None
+    else:
+        bv, br, bc, bg = barrier_pt(scene, with_grad=with_grad)
     parts = [(bv, br, bc, bg)]
     if nb:
         parts.append(abd_body(scene, with_grad=with_grad))
@@ -280,13 +337,27 @@ def triplets(scene, with_grad=False):
         parts.append(quad_bending(scene, hess_offset=off, with_grad=with_grad))
     if getattr(scene, "n_tris", 0):
         parts.append(tri_bw(scene, hess_offset=off, with_grad=with_grad))
+    vals = np.concatenate([p[0] for p in parts])
+    rows = np.concatenate([p[1] for p in parts])
+    cols = np.concatenate([p[2] for p in parts])
+    dirichlet_zero(vals, rows, cols, off, bt)
     mv, mr, mc = mass_diag(scene, hess_offset=off)
-    vals = np.concatenate([p[0] for p in parts] + [mv])
-    rows = np.concatenate([p[1] for p in parts] + [mr])
-    cols = np.concatenate([p[2] for p in parts] + [mc])
+    vals = np.concatenate([vals, mv])
+    rows = np.concatenate([rows, mr])
+    cols = np.concatenate([cols, mc])
     grad = None
-    if with_grad and not nb:
-        grad = sum(p[3] for p in parts)
+    if with_grad:
+        if nb:
+            grad = np.zeros(3 * scene.n_sys)
+            grad[:12 * nb] = parts[1][3]
+            for p in parts[2:]:
+                grad[12 * nb:] += p[3]
+            if gp is not None:
+                grad += point_gradient_to_system(scene, gp)
+        else:
+            grad = sum(p[3] for p in parts)
+        if bt is not None:
+            grad[3 * off:].reshape(-1, 3)[bt != 0] = 0.0
     return vals, rows, cols, grad
 
 

@@ -589,8 +589,76 @@ static void project_psd_n(int n, double* H)
         }
 }
 
+static int inv12(double* A);
+
+/* ABDJacobi{xb}.to_mat() (abd_jacobi_matrix.inl:32-43): row p has 1 at column
+ * p and xb^T at columns 3 + 3p .. 5 + 3p; written into rows r0..r0+2 of a
+ * 12x12 row-major matrix */
+static void abd_jacobi_rows(const double* xb, double* J, int r0)
+{
+    for (int p = 0; p < 3; ++p) {
+        double* row = J + 12 * (r0 + p);
+        for (int c = 0; c < 12; ++c) row[c] = 0.0;
+        row[p] = 1.0;
+        for (int c = 0; c < 3; ++c) row[3 + 3 * p + c] = xb[c];
+    }
+}
+
+/* Motor body term (setup_abd_system_gradient_and_hessian.cu:271-339), step
+ * by step as there: J over the rest points 0, e_x, e_y, e_z; J_delta over
+ * 0, 0, R e_y - e_y, R e_z - e_z (R: angle theta about x); q_p = J^-1 J_delta
+ * q~ + q~ with segments 6 and 9 normalised; dq = q - q_p with its first six
+ * entries zeroed; W = strength * M[6:12, 6:12]; G += W dq, H += W. */
+static void abd_motor_term(const double* qb, const double* qt, const double* M, double theta, double strength,
+                           double* H, double* G)
+{
+    const double bar[4][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    const double c = cos(theta), s = sin(theta);
+    const double R[3][3] = {{1, 0, 0}, {0, c, -s}, {0, s, c}};
+    double J[144], Jd[144];
+    for (int k = 0; k < 4; ++k) abd_jacobi_rows(bar[k], J, 3 * k);
+    const double zero[3] = {0, 0, 0};
+    abd_jacobi_rows(zero, Jd, 0);
+    abd_jacobi_rows(zero, Jd, 3);
+    for (int k = 2; k < 4; ++k) {
+        double d[3];
+        for (int i = 0; i < 3; ++i) d[i] = R[i][0] * bar[k][0] + R[i][1] * bar[k][1] + R[i][2] * bar[k][2] - bar[k][i];
+        abd_jacobi_rows(d, Jd, 3 * k);
+    }
+    inv12(J);
+    double t[12], qp[12];
+    for (int i = 0; i < 12; ++i) {
+        double sm = 0.0;
+        for (int j = 0; j < 12; ++j) sm += Jd[i * 12 + j] * qt[j];
+        t[i] = sm;
+    }
+    for (int i = 0; i < 12; ++i) {
+        double sm = 0.0;
+        for (int j = 0; j < 12; ++j) sm += J[i * 12 + j] * t[j];
+        qp[i] = sm + qt[i];
+    }
+    for (int seg = 6; seg <= 9; seg += 3) {
+        double nrm = sqrt(qp[seg] * qp[seg] + qp[seg + 1] * qp[seg + 1] + qp[seg + 2] * qp[seg + 2]);
+        for (int k = 0; k < 3; ++k) qp[seg + k] /= nrm;
+    }
+    double dq[12];
+    for (int i = 0; i < 12; ++i) dq[i] = i < 6 ? 0.0 : qb[i] - qp[i];
+    for (int i = 6; i < 12; ++i) {
+        double sm = 0.0;
+        for (int j = 6; j < 12; ++j) {
+            const double w = strength * M[i * 12 + j];
+            sm += w * dq[j];
+            H[i * 12 + j] += w;
+        }
+        G[i] += sm;
+    }
+}
+
+/* fixed[b] is the reference's BodyBoundaryType (body_boundary_type.h:3-8):
+ * 0 free, 1 fixed, 2 motor. */
 int orc_abd_body(int n_bodies, const double* q, const double* q_tilde, const double* mass10, const double* volume,
-                 double kappa, double dt, const int* fixed, double* grad, double* tvals, int* trows, int* tcols)
+                 double kappa, double dt, const int* fixed, double motor_speed, double motor_strength, double* grad,
+                 double* tvals, int* trows, int* tcols)
 {
 #pragma omp parallel for schedule(static)
     for (int b = 0; b < n_bodies; ++b) {
@@ -598,7 +666,8 @@ int orc_abd_body(int n_bodies, const double* q, const double* q_tilde, const dou
         abd_mass12(mass10 + 10 * (size_t)b, M);
         memcpy(H, M, sizeof(H));
         for (int i = 0; i < 12; ++i) G[i] = 0.0;
-        if (!(fixed && fixed[b])) {
+        const int bt = fixed ? fixed[b] : 0;
+        if (bt != 1) {
             const double* qb = q + 12 * (size_t)b;
             const double* qt = q_tilde + 12 * (size_t)b;
             for (int i = 0; i < 12; ++i) {
@@ -615,6 +684,7 @@ int orc_abd_body(int n_bodies, const double* q, const double* q_tilde, const dou
                 G[3 + i] += kvt2 * g9[i];
                 for (int j = 0; j < 9; ++j) H[(3 + i) * 12 + 3 + j] += H9[i * 9 + j];
             }
+            if (bt == 2) abd_motor_term(qb, qt, M, motor_speed * dt, motor_strength, H, G);
         }
         if (grad)
             for (int i = 0; i < 12; ++i) grad[12 * (size_t)b + i] = G[i];
@@ -654,9 +724,12 @@ static void abd_jt_h_j(const double* x, const double* Hrm /* 3x3 row-major */, c
         }
 }
 
+/* fem_btype (NULL: none): per FEM point, nonzero = Dirichlet; FEM-FEM and
+ * ABD-FEM blocks with such an end are zeroed (GIPC.cu:10332-10363;
+ * setup_abd_system_gradient_and_hessian.cu:564-568). */
 long orc_abd_contact_map(long n_trip, const double* vals, const int* rows, const int* cols, int n_abd_points,
                          const int* point_body, const double* x_bar, int n_bodies, const int* body_fixed,
-                         double* ovals, int* orows, int* ocols)
+                         const int* fem_btype, double* ovals, int* orows, int* ocols)
 {
     long o = 0;
     const int fem0 = 4 * n_bodies - n_abd_points;  /* FEM point id -> system row */
@@ -671,11 +744,14 @@ long orc_abd_contact_map(long n_trip, const double* vals, const int* rows, const
             if (cls == 0 && !ia && !ja) {
                 orows[o] = i + fem0;
                 ocols[o] = j + fem0;
-                memcpy(ovals + 9 * (size_t)o, v, 9 * sizeof(double));
+                if (fem_btype && (fem_btype[i - n_abd_points] || fem_btype[j - n_abd_points]))
+                    memset(ovals + 9 * (size_t)o, 0, 9 * sizeof(double));
+                else
+                    memcpy(ovals + 9 * (size_t)o, v, 9 * sizeof(double));
                 ++o;
             } else if (cls == 1 && ia && !ja) {
                 int bi = point_body[i];
-                int zero = body_fixed && body_fixed[bi];
+                int zero = (body_fixed && body_fixed[bi] == 1) || (fem_btype && fem_btype[j - n_abd_points]);
                 const double* xb = x_bar + 3 * (size_t)i;
                 /* (J^T H) rows 3a..3a+2: a = 0 -> H, a = 1 + r -> xb (x) H row r */
                 for (int a = 0; a < 4; ++a) {
@@ -691,7 +767,7 @@ long orc_abd_contact_map(long n_trip, const double* vals, const int* rows, const
                 }
             } else if (cls == 2 && ia && ja) {
                 int bi = point_body[i], bj = point_body[j];
-                int zero = body_fixed && (body_fixed[bi] || body_fixed[bj]);
+                int zero = body_fixed && (body_fixed[bi] == 1 || body_fixed[bj] == 1);
                 double X[144];
                 abd_jt_h_j(x_bar + 3 * (size_t)i, Hrm, x_bar + 3 * (size_t)j, X);
                 if (bi == bj && i != j) {
@@ -869,6 +945,15 @@ int orc_barrier_pt(int n_pairs, const double* x, const int* pairs, const double*
             }
     }
     return 0;
+}
+
+void orc_dirichlet_zero(long n_trip, const int* rows, const int* cols, int row0, const int* btype, double* vals)
+{
+    for (long k = 0; k < n_trip; ++k) {
+        const int r = rows[k], c = cols[k];
+        if ((r >= row0 && btype[r - row0]) || (c >= row0 && btype[c - row0]))
+            for (int e = 0; e < 9; ++e) vals[9 * k + e] = 0.0;
+    }
 }
 
 int orc_mass_diag(int n, const double* mass, int hess_offset, double* tvals, int* trows,

@@ -126,10 +126,12 @@ int orc_pcg(long nnz, const double* vals, const int* rows, const int* cols, int 
 /* ---- affine bodies (ABD) ---------------------------------------------------
  * Body term: 10 upper blocks per body at rows 4b..4b+3 (kinetic M(q - q~) and
  * the PSD-projected orthogonality energy kappa*vol*dt^2 V(a)); grad (optional)
- * receives 12 per body.  setup_abd_system_gradient_and_hessian.cu:200-289. */
+ * receives 12 per body.  fixed[b]: 0 free, 1 fixed (H = M, G = 0), 2 motor
+ * (the motor term of lines 271-339 added).
+ * setup_abd_system_gradient_and_hessian.cu:200-339. */
 int orc_abd_body(int n_bodies, const double* q, const double* q_tilde, const double* mass10,
-                 const double* volume, double kappa, double dt, const int* fixed, double* grad,
-                 double* tvals, int* trows, int* tcols);
+                 const double* volume, double kappa, double dt, const int* fixed, double motor_speed,
+                 double motor_strength, double* grad, double* tvals, int* trows, int* tcols);
 double orc_abd_shape_energy(const double* q12);
 void orc_abd_shape_grad_hess(const double* q12, double* g9, double* H81);
 /* Vertex-level contact triplets (upper, as the barrier writes them) -> system
@@ -141,7 +143,12 @@ void orc_abd_shape_grad_hess(const double* q12, double* g9, double* H81);
  * 87-141,424-558. */
 long orc_abd_contact_map(long n_trip, const double* vals, const int* rows, const int* cols,
                          int n_abd_points, const int* point_body, const double* x_bar, int n_bodies,
-                         const int* body_fixed, double* ovals, int* orows, int* ocols);
+                         const int* body_fixed, const int* fem_btype, double* ovals, int* orows,
+                         int* ocols);
+/* Dirichlet vertices (GIPC.cu:10444-10462): zero every triplet with an end on
+ * a FEM row (>= row0) whose vertex btype[row - row0] is nonzero. */
+void orc_dirichlet_zero(long n_trip, const int* rows, const int* cols, int row0, const int* btype,
+                        double* vals);
 /* Inverse of every body's 12x12 diagonal block of the assembled BCOO
  * (row-major, 144 per body).  setup_abd_system_gradient_and_hessian.cu:560-655. */
 int orc_abd_jacobi_build(long nnz, const double* vals, const int* rows, const int* cols, int n_bodies,

@@ -50,7 +50,8 @@ class SceneDesc(ctypes.Structure):
                 ("dm_inv2", vp), ("area", vp),
                 ("n_hinges", ci), ("k_bend", cd), ("rest_x", vp), ("hinge_edges", vp), ("hinge_adj", vp),
                 ("n_bodies", ci), ("n_abd_points", ci), ("kappa_abd", cd), ("q", vp), ("q_tilde", vp),
-                ("mass10", vp), ("body_volume", vp), ("body_fixed", vp), ("point_body", vp), ("x_bar", vp)]
+                ("mass10", vp), ("body_volume", vp), ("body_fixed", vp), ("point_body", vp), ("x_bar", vp),
+                ("boundary_type", vp), ("motor_speed", cd), ("motor_strength", cd)]
 
 
 class SolveTiming(ctypes.Structure):
@@ -76,8 +77,9 @@ _SIGS = {
     "sgipc_fem_tri_gradient_hessian": (ci, [vp, vp, vp, vp, vp, ci, cd, cd, cd, cd, ci, vp, vp, vp, vp]),
     "sgipc_quad_bending_prepare": (ci, [vp, vp, vp, vp, ci, vp]),
     "sgipc_quad_bending_gradient_hessian": (ci, [vp, vp, vp, vp, vp, vp, ci, cd, cd, ci, vp, vp, vp, vp]),
-    "sgipc_abd_body_gradient_hessian": (ci, [vp, ci, vp, vp, vp, vp, cd, cd, vp, vp, vp, vp, vp]),
-    "sgipc_contact_partition": (ci, [vp, vp, vp, vp, ci, ci, vp, vp, ci, vp, vp, vp, vp, ci, ctypes.POINTER(ci)]),
+    "sgipc_abd_body_gradient_hessian": (ci, [vp, ci, vp, vp, vp, vp, cd, cd, vp, cd, cd, vp, vp, vp, vp]),
+    "sgipc_contact_partition": (ci, [vp, vp, vp, vp, ci, ci, vp, vp, ci, vp, vp, vp, vp, vp, ci,
+                                     ctypes.POINTER(ci)]),
     "sgipc_abd_jacobi_build": (ci, [vp, vp, vp, vp, ci, ci, vp]),
     "sgipc_abd_jacobi_apply": (ci, [vp, vp, ci, vp, vp]),
     "sgipc_mass_diagonal": (ci, [vp, vp, ci, ci, vp, vp, vp]),
@@ -102,6 +104,7 @@ _SIGS = {
     "sgipc_mas_levels": (ci, [vp, ctypes.POINTER(ci), ctypes.POINTER(ci), ctypes.POINTER(ci)]),
     "sgipc_mas_tables": (ci, [vp, vp, vp]),
     "sgipc_mas_inverse_blocks": (ci, [vp, vp]),
+    "sgipc_mas_level2_inverse": (ci, [vp, vp, ctypes.POINTER(ci)]),
     "sgipc_pcg_default_opts": (None, [ctypes.POINTER(PcgOpts)]),
     "sgipc_pcg_solve": (ci, [vp, vp, vp, vp, ci, ci, vp, vp, vp, ctypes.POINTER(PcgOpts), ctypes.POINTER(PcgStats)]),
     "sgipc_system_create": (ci, [vp, ctypes.POINTER(vp), ctypes.POINTER(SceneDesc), ci]),

@@ -106,20 +106,56 @@ __device__ __forceinline__ double dot3(const double* a, const double* b)
     return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
 }
 
+// Motor body (setup_abd_system_gradient_and_hessian.cu:271-339): the target
+// q_p keeps q_tilde's rows a2, a3 and adds A~ (R e_y - e_y) and A~ (R e_z - e_z)
+// to their second and third entries (R the rotation by theta = motor_speed dt
+// about the body x axis; the reference reaches this through J^-1 J_delta over
+// the rest points 0, e_x, e_y, e_z), each row normalised; then, with
+// W = strength * M[6:12, 6:12] (the body mass), G += W (q - q_p) on the a2, a3
+// rows and H += W.
+__device__ void abd_motor(const double* __restrict__ qb, const double* __restrict__ qt, double theta,
+                          const double W[6][6], double H[12][12], double G[12])
+{
+    const double c = cos(theta), s = sin(theta);
+    const double d2[3] = {0.0, c - 1.0, s};   // R e_y - e_y
+    const double d3[3] = {0.0, -s, c - 1.0};  // R e_z - e_z
+    double dq[6];
+    for (int r = 1; r < 3; ++r) {
+        const double* ar = qt + 3 + 3 * r;  // row r of A~
+        double ap[3] = {ar[0], ar[1] + (ar[0] * d2[0] + ar[1] * d2[1] + ar[2] * d2[2]),
+                        ar[2] + (ar[0] * d3[0] + ar[1] * d3[1] + ar[2] * d3[2])};
+        const double nrm = sqrt(ap[0] * ap[0] + ap[1] * ap[1] + ap[2] * ap[2]);
+        for (int k = 0; k < 3; ++k) dq[3 * (r - 1) + k] = qb[3 + 3 * r + k] - ap[k] / nrm;
+    }
+    for (int i = 0; i < 6; ++i) {
+        double acc = 0.0;
+        for (int j = 0; j < 6; ++j) acc += W[i][j] * dq[j];
+        G[6 + i] += acc;
+        for (int j = 0; j < 6; ++j) H[6 + i][6 + j] += W[i][j];
+    }
+}
+
 __global__ __launch_bounds__(64) void k_abd_body(int nb, const double* __restrict__ q, const double* __restrict__ qt,
                                                  const double* __restrict__ mass10, const double* __restrict__ vol,
-                                                 double kappa, double dt2, const int* __restrict__ fixed,
+                                                 double kappa, double dt, const int* __restrict__ fixed,
+                                                 double motor_speed, double motor_strength,
                                                  double* __restrict__ grad, double* __restrict__ tvals,
                                                  int* __restrict__ trows, int* __restrict__ tcols)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nb) return;
+    const double dt2 = dt * dt;
     double H[12][12];
     mass12(mass10 + 10 * (long)b, H);
     double G[12];
 #pragma unroll
     for (int i = 0; i < 12; ++i) G[i] = 0.0;
-    if (!(fixed && fixed[b])) {
+    const int btype = fixed ? fixed[b] : 0;  // 0 free, 1 fixed, 2 motor (BodyBoundaryType)
+    double W[6][6];
+    if (btype == 2)
+        for (int i = 0; i < 6; ++i)
+            for (int j = 0; j < 6; ++j) W[i][j] = motor_strength * H[6 + i][6 + j];
+    if (btype != 1) {
         double qb[12], dq[12];
         for (int i = 0; i < 12; ++i) {
             qb[i] = q[12 * (long)b + i];
@@ -157,6 +193,7 @@ __global__ __launch_bounds__(64) void k_abd_body(int nb, const double* __restric
         psd_project<9>(S);
         for (int i = 0; i < 9; ++i)
             for (int j = 0; j < 9; ++j) H[3 + i][3 + j] += S[i][j];
+        if (btype == 2) abd_motor(qb, qt + 12 * (long)b, motor_speed * dt, W, H, G);
     }
     if (grad)
         for (int i = 0; i < 12; ++i) grad[12 * (long)b + i] = G[i];
@@ -198,8 +235,9 @@ __global__ __launch_bounds__(128) void k_contact_map(long n, const double* __res
                                                      const double* __restrict__ x_bar, int nb,
                                                      const int* __restrict__ body_fixed,
                                                      const int* __restrict__ s0, const int* __restrict__ s1,
-                                                     const int* __restrict__ s2, double* __restrict__ ov,
-                                                     int* __restrict__ orow, int* __restrict__ ocol)
+                                                     const int* __restrict__ s2, const int* __restrict__ fem_btype,
+                                                     double* __restrict__ ov, int* __restrict__ orow,
+                                                     int* __restrict__ ocol)
 {
     const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
@@ -214,16 +252,20 @@ __global__ __launch_bounds__(128) void k_contact_map(long n, const double* __res
         for (int c = 0; c < 3; ++c) Hm[r][c] = vals[9 * k + c * 3 + r];
     const int base1 = s0[n], base2 = s0[n] + s1[n];
     if (cls == 0) {
+        // FEM-FEM: zeroed at Dirichlet points (GIPC.cu:10332-10363)
         const long o = s0[k];
         orow[o] = i + fem0;
         ocol[o] = j + fem0;
+        const bool zero = fem_btype && (fem_btype[i - n_abd] != 0 || fem_btype[j - n_abd] != 0);
 #pragma unroll
-        for (int e = 0; e < 9; ++e) ov[9 * o + e] = vals[9 * k + e];
+        for (int e = 0; e < 9; ++e) ov[9 * o + e] = zero ? 0.0 : vals[9 * k + e];
         return;
     }
     if (cls == 1) {
         const int bi = point_body[i];
-        const bool zero = body_fixed && body_fixed[bi];
+        // ABD-FEM: zeroed for a fixed body or a Dirichlet FEM point
+        // (setup_abd_system_gradient_and_hessian.cu:564-568)
+        const bool zero = (body_fixed && body_fixed[bi] == 1) || (fem_btype && fem_btype[j - n_abd] != 0);
         const double xb[3] = {x_bar[3 * (long)i], x_bar[3 * (long)i + 1], x_bar[3 * (long)i + 2]};
         const long o0 = base1 + s1[k];
 #pragma unroll
@@ -243,7 +285,7 @@ __global__ __launch_bounds__(128) void k_contact_map(long n, const double* __res
     }
     // ABD-ABD: X = J_i^T H J_j; (a, b) block of X over [p | a1 | a2 | a3]
     const int bi = point_body[i], bj = point_body[j];
-    const bool zero = body_fixed && (body_fixed[bi] || body_fixed[bj]);
+    const bool zero = body_fixed && (body_fixed[bi] == 1 || body_fixed[bj] == 1);
     const bool sym = bi == bj && i != j;
     const double x[4] = {1.0, x_bar[3 * (long)i], x_bar[3 * (long)i + 1], x_bar[3 * (long)i + 2]};
     const double y[4] = {1.0, x_bar[3 * (long)j], x_bar[3 * (long)j + 1], x_bar[3 * (long)j + 2]};
@@ -379,15 +421,87 @@ __global__ __launch_bounds__(256) void k_abd_jacobi_apply(const double* __restri
     }
 }
 
+// Energy gradient of the contact points into the system rows
+// (setup_abd_system_gradient_and_hessian.cu:349-385, fem_linear_subsystem.cu:
+// 56-84): a body point adds J^T g to its body's 12 rows (not for a fixed
+// body), J^T g = [g, xbar_0 g, xbar_1 g, xbar_2 g] over [p | a1 | a2 | a3]
+// (a_r row r of A: rows 3 + 3 r + c take g_r xbar_c); a FEM point adds g to
+// its vertex rows, which follow the 4 nb body rows.  Atomic like the
+// reference: the order of the additions is not fixed.
+__global__ void k_point_gradient(int n_points, int n_abd, const double* __restrict__ gp,
+                                 const int* __restrict__ point_body, const double* __restrict__ x_bar,
+                                 const int* __restrict__ body_fixed, int nb, double* __restrict__ grad)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_points) return;
+    const double g[3] = {gp[3L * i], gp[3L * i + 1], gp[3L * i + 2]};
+    if (i >= n_abd) {
+        double* d = grad + 3L * (4L * nb + (i - n_abd));
+        for (int r = 0; r < 3; ++r) atomicAdd(d + r, g[r]);
+        return;
+    }
+    const int b = point_body[i];
+    if (body_fixed && body_fixed[b] == 1) return;
+    double* d = grad + 12L * b;
+    for (int r = 0; r < 3; ++r) {
+        atomicAdd(d + r, g[r]);
+        for (int c = 0; c < 3; ++c) atomicAdd(d + 3 + 3 * r + c, g[r] * x_bar[3L * i + c]);
+    }
+}
+
+// Dirichlet vertices (GIPC.cu:10444-10462): a triplet with an end on a FEM
+// row (>= row0) whose vertex is fixed is zeroed; body rows are left alone.
+__global__ void k_dirichlet_zero(long n, const int* __restrict__ rows, const int* __restrict__ cols, int row0,
+                                 const int* __restrict__ btype, double* __restrict__ vals)
+{
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int r = rows[k], c = cols[k];
+    const bool zero = (r >= row0 && btype[r - row0] != 0) || (c >= row0 && btype[c - row0] != 0);
+    if (zero)
+        for (int e = 0; e < 9; ++e) vals[9 * k + e] = 0.0;
+}
+
+// gradient rows of Dirichlet vertices are zero (fem_linear_subsystem.cu:73-76)
+__global__ void k_dirichlet_zero_rows(int n, const int* __restrict__ btype, double* __restrict__ g)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n && btype[v] != 0)
+        for (int r = 0; r < 3; ++r) g[3L * v + r] = 0.0;
+}
+
 }  // namespace
 
 void launch_abd_body(hipStream_t s, int nb, const double* q, const double* q_tilde, const double* mass10,
-                     const double* vol, double kappa, double dt, const int* fixed, double* grad, double* tvals,
-                     int* trows, int* tcols)
+                     const double* vol, double kappa, double dt, const int* fixed, double motor_speed,
+                     double motor_strength, double* grad, double* tvals, int* trows, int* tcols)
 {
     if (nb <= 0) return;
-    k_abd_body<<<(nb + 63) / 64, 64, 0, s>>>(nb, q, q_tilde, mass10, vol, kappa, dt * dt, fixed, grad, tvals, trows,
-                                              tcols);
+    k_abd_body<<<(nb + 63) / 64, 64, 0, s>>>(nb, q, q_tilde, mass10, vol, kappa, dt, fixed, motor_speed,
+                                              motor_strength, grad, tvals, trows, tcols);
+    SGIPC_LAUNCH_CHECK();
+}
+
+void launch_point_gradient(hipStream_t s, int n_points, int n_abd, const double* gp, const int* point_body,
+                           const double* x_bar, const int* body_fixed, int nb, double* grad)
+{
+    if (n_points <= 0) return;
+    k_point_gradient<<<g256(n_points), 256, 0, s>>>(n_points, n_abd, gp, point_body, x_bar, body_fixed, nb, grad);
+    SGIPC_LAUNCH_CHECK();
+}
+
+void launch_dirichlet_zero(hipStream_t s, long n, const int* rows, const int* cols, int row0, const int* btype,
+                           double* vals)
+{
+    if (n <= 0 || btype == nullptr) return;
+    k_dirichlet_zero<<<g256(n), 256, 0, s>>>(n, rows, cols, row0, btype, vals);
+    SGIPC_LAUNCH_CHECK();
+}
+
+void launch_dirichlet_zero_rows(hipStream_t s, int n, const int* btype, double* g)
+{
+    if (n <= 0 || btype == nullptr) return;
+    k_dirichlet_zero_rows<<<g256(n), 256, 0, s>>>(n, btype, g);
     SGIPC_LAUNCH_CHECK();
 }
 
@@ -417,14 +531,14 @@ long ContactMapper::count(hipStream_t s, const int* rows, const int* cols, long 
 }
 
 void ContactMapper::write(hipStream_t s, const double* vals, const int* rows, const int* cols, int n_abd,
-                          const int* point_body, const double* x_bar, int nb, const int* body_fixed, double* ov,
-                          int* orow, int* ocol, long capacity)
+                          const int* point_body, const double* x_bar, int nb, const int* body_fixed,
+                          const int* fem_btype, double* ov, int* orow, int* ocol, long capacity)
 {
     if (n_ <= 0 || total_ == 0) return;
     SGIPC_CHECK(total_ <= capacity, SGIPC_CAPACITY, "contact map exceeds the output capacity");
     k_contact_map<<<(int)((n_ + 127) / 128), 128, 0, s>>>(n_, vals, rows, cols, n_abd, point_body, x_bar, nb,
                                                           body_fixed, scan_[0].get(), scan_[1].get(), scan_[2].get(),
-                                                          ov, orow, ocol);
+                                                          fem_btype, ov, orow, ocol);
     SGIPC_LAUNCH_CHECK();
 }
 

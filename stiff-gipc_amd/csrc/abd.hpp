@@ -6,9 +6,19 @@
 namespace sgipc {
 
 // 10 upper blocks per body at rows 4b..4b+3; grad (optional) 12 per body.
+// fixed[b]: 0 free, 1 fixed, 2 motor (BodyBoundaryType)
 void launch_abd_body(hipStream_t s, int nb, const double* q, const double* q_tilde, const double* mass10,
-                     const double* vol, double kappa, double dt, const int* fixed, double* grad, double* tvals,
-                     int* trows, int* tcols);
+                     const double* vol, double kappa, double dt, const int* fixed, double motor_speed,
+                     double motor_strength, double* grad, double* tvals, int* trows, int* tcols);
+// point-space energy gradient gp (3 per point: body points, then FEM points)
+// added into the system gradient (12 per body, then 3 per FEM vertex)
+void launch_point_gradient(hipStream_t s, int n_points, int n_abd, const double* gp, const int* point_body,
+                           const double* x_bar, const int* body_fixed, int nb, double* grad);
+// zero the triplets touching a Dirichlet FEM vertex (rows >= row0 are FEM rows)
+void launch_dirichlet_zero(hipStream_t s, long n, const int* rows, const int* cols, int row0, const int* btype,
+                           double* vals);
+// zero the gradient rows of Dirichlet vertices (g: 3 per FEM vertex)
+void launch_dirichlet_zero_rows(hipStream_t s, int n, const int* btype, double* g);
 
 // Point-level upper contact blocks -> system blocks by class (FEM-FEM,
 // ABD-FEM, ABD-ABD), each class in input order.  Returns the block count.
@@ -18,7 +28,8 @@ class ContactMapper {
     long count(hipStream_t s, const int* rows, const int* cols, long n, int n_abd);
     // write the blocks counted by the last count() (same rows/cols)
     void write(hipStream_t s, const double* vals, const int* rows, const int* cols, int n_abd, const int* point_body,
-               const double* x_bar, int nb, const int* body_fixed, double* ov, int* orow, int* ocol,
+               const double* x_bar, int nb, const int* body_fixed, const int* fem_btype, double* ov, int* orow,
+               int* ocol,
                long capacity);
 
   private:

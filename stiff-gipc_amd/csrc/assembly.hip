@@ -576,6 +576,7 @@ int Converter::convert(hipStream_t s, const double* tvals, const int* trows, con
     SGIPC_HIP(hipMemcpyAsync(&h[1], scratch_int_.get(), sizeof(int), hipMemcpyDeviceToHost, s));
     SGIPC_HIP(hipStreamSynchronize(s));
     SGIPC_CHECK(h[1] == 0, SGIPC_BAD_INDEX, "triplet with row > col, a negative index or an index beyond the bound");
+    last_unique = h[0];
     SGIPC_CHECK(h[0] <= capacity, SGIPC_CAPACITY, "unique block count exceeds the output capacity");
     return h[0];
 }

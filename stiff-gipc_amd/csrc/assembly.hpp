@@ -31,6 +31,9 @@ class Converter {
                 double* ovals, int* orows, int* ocols, int capacity);
     int max_index(hipStream_t s, const int* rows, const int* cols, long n);
     static int index_bits(int bound);
+    // unique block count of the last convert, also when it threw
+    // SGIPC_CAPACITY (the capacity the caller must provide)
+    int last_unique = 0;
 
   private:
     DevBuf<uint64_t> keys_, skeys_;

@@ -53,9 +53,7 @@ int residual(hipStream_t s, const FullMatrix& A, const double* x, const double* 
 
 }  // namespace sgipc
 
-namespace {
-std::unique_ptr<sgipc::SolveWorkspace>& workspace_of(sgipc_ctx* ctx);
-}
+sgipc_ctx::~sgipc_ctx() = default;
 
 using namespace sgipc;
 
@@ -93,7 +91,9 @@ int sgipc_destroy(sgipc_ctx* ctx)
 {
     if (!ctx) return SGIPC_BAD_ARG;
     int st = guarded(ctx, [&] {
-        workspace_of(ctx).reset();
+        SGIPC_HIP(hipStreamSynchronize(ctx->c.stream));
+        ctx->ws.reset();
+        ctx->conv.reset();
         if (ctx->c.own_stream && ctx->c.stream) SGIPC_HIP(hipStreamDestroy(ctx->c.stream));
     });
     delete ctx;
@@ -458,6 +458,16 @@ int sgipc_mas_inverse_blocks(sgipc_mas* mas, double* out)
     });
 }
 
+int sgipc_mas_level2_inverse(sgipc_mas* mas, double* out, int* m)
+{
+    if (!mas || !m) return SGIPC_BAD_ARG;
+    *m = 0;
+    return guarded(mas->ctx, [&] {
+        SGIPC_CHECK(mas->m->built, SGIPC_BAD_ARG, "MAS not set");
+        *m = mas->m->level2_inverse(out);
+    });
+}
+
 // ---------------------------------------------------------------- PCG
 void sgipc_pcg_default_opts(sgipc_pcg_opts* o)
 {
@@ -550,8 +560,7 @@ int pcg_solve_impl(Context& c, SolveWorkspace& ws, const double* vals, const int
         SubsystemPrecond<double> M(prec);
         launch_fill<double>(s, x, 0.0, n3);
         SGIPC_HIP(hipMemcpyAsync(ws.r64.get(), b, sizeof(double) * n3, hipMemcpyDeviceToDevice, s));
-        double* hp = nullptr;
-        SGIPC_HIP(hipHostMalloc(reinterpret_cast<void**>(&hp), 4096 * sizeof(double)));
+        double* hp = ws.pinned();
         auto host_sum = [&](const double* dpart, int np) {
             SGIPC_HIP(hipMemcpyAsync(hp, dpart, sizeof(double) * np, hipMemcpyDeviceToHost, s));
             SGIPC_HIP(hipStreamSynchronize(s));
@@ -595,7 +604,6 @@ int pcg_solve_impl(Context& c, SolveWorkspace& ws, const double* vals, const int
             if (rr > best_rr * 0.999999 && outer > 1) break;  // stagnation guard
             if (rr < best_rr) best_rr = rr;
         }
-        (void)hipHostFree(hp);
         st.iterations = total_it;
         st.outer_iterations = outer;
         if (status == SGIPC_OK && !ok) status = SGIPC_MAXITER;
@@ -625,18 +633,6 @@ int pcg_solve_impl(Context& c, SolveWorkspace& ws, const double* vals, const int
 
 }  // namespace sgipc
 
-namespace {
-std::unique_ptr<sgipc::SolveWorkspace>& workspace_of(sgipc_ctx* ctx)
-{
-    // one workspace per context, created on first use
-    static thread_local std::vector<std::pair<sgipc_ctx*, std::unique_ptr<sgipc::SolveWorkspace>>> table;
-    for (auto& e : table)
-        if (e.first == ctx) return e.second;
-    table.emplace_back(ctx, nullptr);
-    return table.back().second;
-}
-}  // namespace
-
 extern "C" int sgipc_pcg_solve(sgipc_ctx* ctx, const double* vals, const int* rows, const int* cols, int nnz,
                                int n_rows, sgipc_mas* mas, const double* b, double* x,
                                const sgipc_pcg_opts* opts, sgipc_pcg_stats* stats)
@@ -652,7 +648,7 @@ extern "C" int sgipc_pcg_solve(sgipc_ctx* ctx, const double* vals, const int* ro
     if (mas && mas->m->n != n_rows) return SGIPC_BAD_ARG;
     int status = SGIPC_OK;
     int st = guarded(ctx, [&] {
-        auto& ws = workspace_of(ctx);
+        auto& ws = ctx->ws;
         if (!ws) ws = std::make_unique<SolveWorkspace>();
         SubsystemPrec prec;
         prec.mas = mas ? mas->m.get() : nullptr;

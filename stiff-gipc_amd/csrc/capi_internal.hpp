@@ -11,8 +11,17 @@
 #include "sgipc_internal.hpp"
 #include "spmv_full.hpp"
 
+namespace sgipc {
+struct SolveWorkspace;
+}
+
+// The context owns its solver workspace and triplet converter (created on
+// first use, freed by sgipc_destroy on whichever thread calls it).
 struct sgipc_ctx {
     sgipc::Context c;
+    std::unique_ptr<sgipc::SolveWorkspace> ws;
+    std::unique_ptr<sgipc::Converter> conv;
+    ~sgipc_ctx();
 };
 
 struct sgipc_mas {
@@ -52,7 +61,8 @@ int guarded(sgipc_ctx* ctx, F&& f)
     }
 }
 
-// Solver workspace (PCG buffers, FP32 matrix copy for refinement).
+// Solver workspace (PCG buffers, FP32 matrix copy for refinement, the
+// pinned host buffer the refinement loop reads its norms through).
 struct SolveWorkspace {
     FullMatrix full;  // tiled symmetric layout of the PCG loop's SpMV
     PcgSolver<double> pcg64;
@@ -60,6 +70,20 @@ struct SolveWorkspace {
     DevBuf<float> r32, d32;
     DevBuf<double> r64, d64;
     DevBuf<double> part;
+    static constexpr int kHostSlots = 4096;
+    double* host = nullptr;  // kHostSlots pinned doubles, allocated on first use
+    double* pinned()
+    {
+        if (!host) SGIPC_HIP(hipHostMalloc(reinterpret_cast<void**>(&host), kHostSlots * sizeof(double)));
+        return host;
+    }
+    SolveWorkspace() = default;
+    SolveWorkspace(const SolveWorkspace&) = delete;
+    SolveWorkspace& operator=(const SolveWorkspace&) = delete;
+    ~SolveWorkspace()
+    {
+        if (host) (void)hipHostFree(host);
+    }
 };
 
 // The subsystem preconditioner of GlobalLinearSystem::apply_preconditioner

@@ -33,7 +33,6 @@
 #include "mas.hpp"
 
 #include <hip/hip_fp16.h>
-#include <rocsolver/rocsolver.h>
 #include "assembly.hpp"
 #include "spmv_full.hpp"
 #include "wave_ops.hpp"
@@ -392,10 +391,12 @@ struct InvShape {
 };
 
 // In-place Gauss-Jordan without pivoting on a 3Bx3B matrix distributed in 6x6
-// register tiles (thread (ty,tx) holds rows 6ty.., cols 6tx..).
+// register tiles (thread (ty,tx) holds rows 6ty.., cols 6tx..).  With
+// nonpos != nullptr, a pivot that is not positive (the matrix is not positive
+// definite) sets *nonpos.
 template <int B>
 __device__ __forceinline__ void gj_invert(double a[6][6], double* __restrict__ colbuf,
-                                          double* __restrict__ rowbuf, int ty, int tx)
+                                          double* __restrict__ rowbuf, int ty, int tx, int* nonpos = nullptr)
 {
     constexpr int N = InvShape<B>::N;
     // zero diagonal -> 1 (padding slots), MASPreconditioner.cu:660-666
@@ -426,6 +427,7 @@ __device__ __forceinline__ void gj_invert(double a[6][6], double* __restrict__ c
         }
         __syncthreads();
         const double ip = 1.0 / cb[j];
+        if (nonpos != nullptr && threadIdx.x == 0 && !(cb[j] > 0.0)) *nonpos = 1;
         double rc[6], cr[6];
 #pragma unroll
         for (int cc = 0; cc < 6; ++cc) rc[cc] = rb[6 * tx + cc] * ip;
@@ -1590,27 +1592,29 @@ __global__ void k_ap_sum(const double* __restrict__ vals, const int* __restrict_
     for (int t = 0; t < 9; ++t) blk[9L * q + t] = (float)a[t];
 }
 
-// y = Minv x for the dense FP64 inverse of the level-2 operator (m x m,
-// row-major, symmetric); one workgroup per row (the matrix is small: many
-// short rows in flight at once hide the load latency), summed in a fixed order.
+// y = Minv x for the dense FP64 inverse of the level-2 operator (m x m of a
+// row-major array with row stride ld, symmetric); one workgroup per row (the
+// matrix is small: many short rows in flight at once hide the load latency),
+// summed in a fixed order.
 template <typename TS>
-__global__ __launch_bounds__(256) void k_dense_gemv(const int* __restrict__ done, int m,
+__global__ __launch_bounds__(256) void k_dense_gemv(const int* __restrict__ done, int m, int ld,
                                                     const double* __restrict__ Minv, const TS* __restrict__ x,
                                                     TS* __restrict__ y)
 {
     __shared__ double smem[16];
     if (done != nullptr && *done) return;
     const int row = blockIdx.x;
-    const double* a = Minv + (long)row * m;
+    const double* a = Minv + (long)row * ld;
     double acc = 0.0;
     for (int c = threadIdx.x; c < m; c += blockDim.x) acc += a[c] * (double)x[c];
     acc = block_sum(acc, smem);
     if (threadIdx.x == 0) y[row] = (TS)acc;
 }
 
-// Dense symmetric level-2 operator from its upper BCOO (column-major blocks).
+// Dense symmetric level-2 operator from its upper BCOO (column-major blocks)
+// into rows/cols [0, m) of the ld x ld array D (zero-filled beforehand).
 __global__ void k_dense_scatter(const double* __restrict__ v, const int* __restrict__ r, const int* __restrict__ c,
-                                int nb, int m, double* __restrict__ D)
+                                int nb, int ld, double* __restrict__ D)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nb) return;
@@ -1618,19 +1622,133 @@ __global__ void k_dense_scatter(const double* __restrict__ v, const int* __restr
     for (int a = 0; a < 3; ++a)
         for (int b = 0; b < 3; ++b) {
             const double x = v[9L * k + 3 * b + a];  // M[a][b]
-            D[(i0 + a) * m + j0 + b] = x;
-            D[(j0 + b) * m + i0 + a] = x;
+            D[(i0 + a) * ld + j0 + b] = x;
+            D[(j0 + b) * ld + i0 + a] = x;
         }
 }
 
-// rocSOLVER leaves the inverse in its column-major lower triangle, i.e. the
-// row-major upper one: mirror it below the diagonal
-__global__ void k_mirror_upper(int m, double* __restrict__ D)
+// Padding rows [m, ld): identity, so the blocked inverse leaves them apart.
+__global__ void k_dense_pad_identity(int m, int ld, double* __restrict__ D)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long)m * m) return;
-    const int i = (int)(t / m), j = (int)(t % m);
-    if (j < i) D[t] = D[(long)j * m + i];
+    const int i = m + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ld) D[(long)i * ld + i] = 1.0;
+}
+
+// ---------------------------------------------------- blocked Gauss-Jordan
+// In-place inverse of the SPD ld x ld row-major matrix D, ld = 96 T, without
+// pivoting (the convention of the domain inverses, MASPreconditioner.cu:
+// 630-726), in 96 x 96 tiles.  Step k of T:
+//   P = D_kk^-1 (gj_invert on one workgroup), C_i = D_ik (i != k)  k_gj_pivot
+//   D_kj = P D_kj (j != k)                                          k_gj_row
+//   D_ij -= C_i D_kj (i, j != k), D_ik = -C_i P, D_kk = P            k_gj_update
+// which is the scalar Gauss-Jordan step applied to whole tiles.  A pivot that
+// is not positive sets *nonpos.  Every product sums in a fixed order.
+constexpr int kGjTile = 96;  // = 3B at B = 32: gj_invert<32>'s 16 x 16 grid of 6 x 6 tiles
+constexpr int kGjChunk = 16; // inner-dimension chunk staged in LDS
+
+// acc += X Y for 96 x 96 tiles X (row stride ldx) and Y (row stride ldy);
+// thread (ty, tx) accumulates rows 6ty.., cols 6tx..
+__device__ __forceinline__ void gj_tile_gemm(const double* __restrict__ X, int ldx, const double* __restrict__ Y,
+                                             int ldy, double acc[6][6], double* __restrict__ sX,
+                                             double* __restrict__ sY, int ty, int tx)
+{
+    for (int t0 = 0; t0 < kGjTile; t0 += kGjChunk) {
+        for (int e = threadIdx.x; e < kGjTile * kGjChunk; e += blockDim.x) {
+            const int row = e / kGjChunk, tt = e % kGjChunk;
+            sX[tt * kGjTile + row] = X[(long)row * ldx + t0 + tt];
+            const int tr = e / kGjTile, col = e % kGjTile;
+            sY[tr * kGjTile + col] = Y[(long)(t0 + tr) * ldy + col];
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int tt = 0; tt < kGjChunk; ++tt) {
+            double xr[6], yc[6];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                xr[q] = sX[tt * kGjTile + 6 * ty + q];
+                yc[q] = sY[tt * kGjTile + 6 * tx + q];
+            }
+#pragma unroll
+            for (int r = 0; r < 6; ++r)
+#pragma unroll
+                for (int c = 0; c < 6; ++c) acc[r][c] = fma(xr[r], yc[c], acc[r][c]);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gj_pivot(const double* __restrict__ D, int ld, int k,
+                                                  double* __restrict__ P, double* __restrict__ C,
+                                                  int* __restrict__ nonpos)
+{
+    __shared__ double colbuf[2 * kGjTile], rowbuf[2 * kGjTile];
+    const int i = blockIdx.x;
+    const double* src = D + (long)i * kGjTile * ld + (long)k * kGjTile;
+    if (i != k) {
+        double* dst = C + (long)i * kGjTile * kGjTile;
+        for (int e = threadIdx.x; e < kGjTile * kGjTile; e += blockDim.x)
+            dst[e] = src[(long)(e / kGjTile) * ld + e % kGjTile];
+        return;
+    }
+    const int ty = threadIdx.x / 16, tx = threadIdx.x % 16;
+    double a[6][6];
+#pragma unroll
+    for (int r = 0; r < 6; ++r)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) a[r][c] = src[(long)(6 * ty + r) * ld + 6 * tx + c];
+    gj_invert<32>(a, colbuf, rowbuf, ty, tx, nonpos);
+#pragma unroll
+    for (int r = 0; r < 6; ++r)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) P[(6 * ty + r) * kGjTile + 6 * tx + c] = a[r][c];
+}
+
+__global__ __launch_bounds__(256) void k_gj_row(double* __restrict__ D, int ld, int k, const double* __restrict__ P)
+{
+    __shared__ double sX[kGjChunk * kGjTile], sY[kGjChunk * kGjTile];
+    const int j = blockIdx.x + (blockIdx.x >= k ? 1 : 0);
+    const int ty = threadIdx.x / 16, tx = threadIdx.x % 16;
+    double* t = D + (long)k * kGjTile * ld + (long)j * kGjTile;
+    double acc[6][6] = {};
+    gj_tile_gemm(P, kGjTile, t, ld, acc, sX, sY, ty, tx);
+#pragma unroll
+    for (int r = 0; r < 6; ++r)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) t[(long)(6 * ty + r) * ld + 6 * tx + c] = acc[r][c];
+}
+
+__global__ __launch_bounds__(256) void k_gj_update(double* __restrict__ D, int ld, int k, int T,
+                                                   const double* __restrict__ P, const double* __restrict__ C)
+{
+    __shared__ double sX[kGjChunk * kGjTile], sY[kGjChunk * kGjTile];
+    const int b = blockIdx.x;
+    double* kk = D + (long)k * kGjTile * ld + (long)k * kGjTile;
+    if (b == (T - 1) * T) {  // the pivot tile takes P
+        for (int e = threadIdx.x; e < kGjTile * kGjTile; e += blockDim.x)
+            kk[(long)(e / kGjTile) * ld + e % kGjTile] = P[e];
+        return;
+    }
+    const int i = b / T + (b / T >= k ? 1 : 0), j = b % T;
+    const int ty = threadIdx.x / 16, tx = threadIdx.x % 16;
+    const double* Ci = C + (long)i * kGjTile * kGjTile;
+    double* t = D + (long)i * kGjTile * ld + (long)j * kGjTile;
+    double acc[6][6] = {};
+    if (j == k) {
+        gj_tile_gemm(Ci, kGjTile, P, kGjTile, acc, sX, sY, ty, tx);
+#pragma unroll
+        for (int r = 0; r < 6; ++r)
+#pragma unroll
+            for (int c = 0; c < 6; ++c) t[(long)(6 * ty + r) * ld + 6 * tx + c] = -acc[r][c];
+    } else {
+        gj_tile_gemm(Ci, kGjTile, D + (long)k * kGjTile * ld + (long)j * kGjTile, ld, acc, sX, sY, ty, tx);
+#pragma unroll
+        for (int r = 0; r < 6; ++r)
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                double* e = t + (long)(6 * ty + r) * ld + 6 * tx + c;
+                *e = *e - acc[r][c];
+            }
+    }
 }
 
 int grid_for(long n, int bs) { return (int)((n + bs - 1) / bs); }
@@ -1819,73 +1937,60 @@ void Mas::assemble_invert_t(const double* vals, const int* rows, const int* cols
     Z32_.reserve((size_t)total * 3);
 }
 
-// The dense level-2 inverse is factorised on a second stream (default
-// priority): rocSOLVER's small-matrix kernels occupy few CUs, so the rest of
-// the build (coarser levels, level-0 inverses, A P) runs beside them.  The
-// main stream waits only for the scatter out of the ping-pong Galerkin buffer
-// (reused two levels later); finish_dense_level2 joins the streams.
+// The dense level-2 inverse: the Galerkin operator A_2 scattered into a
+// dense array padded to whole 96 x 96 tiles (identity on the padding), then
+// the blocked Gauss-Jordan inverse on the build stream, 3 launches per tile
+// row.  A pivot that is not positive means A_2 lost its definiteness to
+// rounding: set_bcoo then returns SGIPC_NOT_SPD rather than running a cycle
+// without its exact coarse solve.
 void Mas::build_dense_level2(hipStream_t s, const double* v, const int* r, const int* c, int nb)
 {
     dense_m_ = 0;
     dense_pending_ = false;
     const long m = 3L * h_cnt[2];
     if (m < 1 || m > kDenseMax || nb < 1) return;
-    if (!s2_) {
-        SGIPC_HIP(hipStreamCreateWithFlags(&s2_, hipStreamNonBlocking));
-        SGIPC_HIP(hipEventCreateWithFlags(&ev_in_, hipEventDisableTiming));
-        SGIPC_HIP(hipEventCreateWithFlags(&ev_scatter_, hipEventDisableTiming));
-        SGIPC_HIP(hipEventCreateWithFlags(&ev_done_, hipEventDisableTiming));
-    }
-    dense_inv_.reserve((size_t)m * m);
-    dense_info_.reserve(2);
-    SGIPC_HIP(hipEventRecord(ev_in_, s));
-    SGIPC_HIP(hipStreamWaitEvent(s2_, ev_in_, 0));
-    launch_fill<double>(s2_, dense_inv_.get(), 0.0, m * m);
-    k_dense_scatter<<<grid_for(nb, 256), 256, 0, s2_>>>(v, r, c, nb, (int)m, dense_inv_.get());
+    const int T = (int)((m + kGjTile - 1) / kGjTile);
+    const int ld = T * kGjTile;
+    dense_inv_.reserve((size_t)ld * ld);
+    dense_piv_.reserve((size_t)kGjTile * kGjTile);
+    dense_col_.reserve((size_t)T * kGjTile * kGjTile);
+    dense_info_.reserve(1);
+    launch_fill<int>(s, dense_info_.get(), 0, 1);
+    launch_fill<double>(s, dense_inv_.get(), 0.0, (long)ld * ld);
+    k_dense_scatter<<<grid_for(nb, 256), 256, 0, s>>>(v, r, c, nb, ld, dense_inv_.get());
     SGIPC_LAUNCH_CHECK();
-    SGIPC_HIP(hipEventRecord(ev_scatter_, s2_));
-    SGIPC_HIP(hipStreamWaitEvent(s, ev_scatter_, 0));  // the Galerkin buffer may be reused from here
-    if (!rb_) {
-        rocblas_handle h = nullptr;
-        SGIPC_CHECK(rocblas_create_handle(&h) == rocblas_status_success, SGIPC_HIP_ERROR, "rocblas_create_handle failed");
-        rb_ = h;
+    if (ld > m) {
+        k_dense_pad_identity<<<grid_for(ld - m, 256), 256, 0, s>>>((int)m, ld, dense_inv_.get());
+        SGIPC_LAUNCH_CHECK();
     }
-    rocblas_handle h = static_cast<rocblas_handle>(rb_);
-    SGIPC_CHECK(rocblas_set_stream(h, s2_) == rocblas_status_success, SGIPC_HIP_ERROR, "rocblas_set_stream failed");
-    SGIPC_CHECK(rocsolver_dpotrf(h, rocblas_fill_lower, (rocblas_int)m, dense_inv_.get(), (rocblas_int)m,
-                                 dense_info_.get()) == rocblas_status_success,
-                SGIPC_HIP_ERROR, "rocsolver_dpotrf failed");
-    SGIPC_CHECK(rocsolver_dpotri(h, rocblas_fill_lower, (rocblas_int)m, dense_inv_.get(), (rocblas_int)m,
-                                 dense_info_.get() + 1) == rocblas_status_success,
-                SGIPC_HIP_ERROR, "rocsolver_dpotri failed");
-    k_mirror_upper<<<grid_for(m * m, 256), 256, 0, s2_>>>((int)m, dense_inv_.get());
-    SGIPC_LAUNCH_CHECK();
-    SGIPC_HIP(hipEventRecord(ev_done_, s2_));
+    for (int k = 0; k < T; ++k) {
+        k_gj_pivot<<<T, 256, 0, s>>>(dense_inv_.get(), ld, k, dense_piv_.get(), dense_col_.get(),
+                                     dense_info_.get());
+        SGIPC_LAUNCH_CHECK();
+        if (T > 1) {
+            k_gj_row<<<T - 1, 256, 0, s>>>(dense_inv_.get(), ld, k, dense_piv_.get());
+            SGIPC_LAUNCH_CHECK();
+        }
+        k_gj_update<<<(T - 1) * T + 1, 256, 0, s>>>(dense_inv_.get(), ld, k, T, dense_piv_.get(), dense_col_.get());
+        SGIPC_LAUNCH_CHECK();
+    }
     dense_pending_ = true;
     dense_m_pending_ = (int)m;
+    dense_ld_ = ld;
 }
 
 void Mas::finish_dense_level2(hipStream_t s)
 {
     if (!dense_pending_) return;
     dense_pending_ = false;
-    SGIPC_HIP(hipStreamWaitEvent(s, ev_done_, 0));
-    int info[2] = {0, 0};
-    SGIPC_HIP(hipMemcpyAsync(info, dense_info_.get(), sizeof(info), hipMemcpyDeviceToHost, s));
+    int info = 0;
+    SGIPC_HIP(hipMemcpyAsync(&info, dense_info_.get(), sizeof(int), hipMemcpyDeviceToHost, s));
     SGIPC_HIP(hipStreamSynchronize(s));
-    // not positive definite: the additive levels >= 2 stay
-    if (info[0] == 0 && info[1] == 0) dense_m_ = dense_m_pending_;
+    SGIPC_CHECK(info == 0, SGIPC_NOT_SPD, "the level-2 Galerkin operator is not positive definite");
+    dense_m_ = dense_m_pending_;
 }
 
-Mas::~Mas()
-{
-    if (s2_) (void)hipStreamSynchronize(s2_);
-    if (rb_) (void)rocblas_destroy_handle(static_cast<rocblas_handle>(rb_));
-    if (ev_in_) (void)hipEventDestroy(ev_in_);
-    if (ev_scatter_) (void)hipEventDestroy(ev_scatter_);
-    if (ev_done_) (void)hipEventDestroy(ev_done_);
-    if (s2_) (void)hipStreamDestroy(s2_);
-}
+Mas::~Mas() = default;
 
 void Mas::learned_sample(hipStream_t s, const double* r, double* ztmp, float* X, float* q)
 {
@@ -1975,10 +2080,8 @@ void Mas::set_bcoo(const double* vals, const int* rows, const int* cols, int nnz
     graph_.n = 0;
     dense_m_ = 0;
     half_ready_ = false;
-    if (dense_pending_) {  // a build that ended in an error left the factorisation running
-        SGIPC_HIP(hipStreamSynchronize(s2_));
-        dense_pending_ = false;
-    }
+    last_path_ = -1;
+    dense_pending_ = false;  // a build that ended in an error left its check unread
     build_hierarchy(pairs, n_pairs);
     if (B == 32) assemble_invert_t<32>(vals, rows, cols, nnz, offset);
     else assemble_invert_t<16>(vals, rows, cols, nnz, offset);
@@ -2015,6 +2118,7 @@ int Mas::apply_t(hipStream_t s, const TV* r, TV* z, double* rz_part, const PcgSt
     const int npad = h_off[1];
     k_restrict0<BB, TV, TS><<<grid_for(npad, 256), 256, 0, s>>>(sd, meta_.get(), n, L, map_.get() + n, r, R);
     SGIPC_LAUNCH_CHECK();
+    note_path(sizeof(TS) == 4);
     if (vcycle_active(sizeof(TS) == 4)) return finish_v_t<BB, TV, TS>(s, R, Z, minv, done, PcgStopDesc{}, r, z, rz_part);
     return finish_t<BB, TV, TS>(s, R, Z, minv, done, PcgStopDesc{}, r, z, rz_part);
 }
@@ -2093,7 +2197,7 @@ int Mas::finish_v_t(hipStream_t s, TS* R, TS* Z, const TS* minv, const int* done
     int Lc = L;
     if (dense_m_ > 0) {
         restrict_up_level(2);
-        k_dense_gemv<TS><<<dense_m_, 256, 0, s>>>(done, dense_m_, dense_inv_.get(), R + 3L * h_off[2],
+        k_dense_gemv<TS><<<dense_m_, 256, 0, s>>>(done, dense_m_, dense_ld_, dense_inv_.get(), R + 3L * h_off[2],
                                                                 Z + 3L * h_off[2]);
         SGIPC_LAUNCH_CHECK();
         Lc = 3;
@@ -2192,7 +2296,7 @@ int Mas::finish_t(hipStream_t s, TS* R, TS* Z, const TS* minv, const int* done, 
         int Lc = L;
         if (dense_m_ > 0) {
             restrict_up_level(2);
-            k_dense_gemv<TS><<<dense_m_, 256, 0, s>>>(done, dense_m_, dense_inv_.get(),
+            k_dense_gemv<TS><<<dense_m_, 256, 0, s>>>(done, dense_m_, dense_ld_, dense_inv_.get(),
                                                                     R + 3L * h_off[2], Z + 3L * h_off[2]);
             SGIPC_LAUNCH_CHECK();
             Lc = 3;
@@ -2267,6 +2371,7 @@ int Mas::apply_xr_t(hipStream_t s, const FusedXR<TV>& fx, TV* z, double* rz_part
     SGIPC_LAUNCH_CHECK();
     *n_rr = g0;
     const PcgStopDesc sd{fx.ctl, fx.rz, fx.n_rz, fx.rr, g0};
+    note_path(sizeof(TS) == 4);
     if (vcycle_active(sizeof(TS) == 4)) return finish_v_t<BB, TV, TS>(s, R, Z, minv, done, sd, fx.r, z, rz_part);
     return finish_t<BB, TV, TS>(s, R, Z, minv, done, sd, fx.r, z, rz_part);
 }
@@ -2333,6 +2438,16 @@ void Mas::tables(int* coarse_table, int* going_next) const
     if (going_next)
         SGIPC_HIP(hipMemcpyAsync(going_next, going_next_.get(), sizeof(int) * total, hipMemcpyDeviceToDevice, s));
     SGIPC_HIP(hipStreamSynchronize(s));
+}
+
+int Mas::level2_inverse(double* out) const
+{
+    if (dense_m_ <= 0 || out == nullptr) return dense_m_;
+    hipStream_t s = ctx->stream;
+    SGIPC_HIP(hipMemcpy2DAsync(out, sizeof(double) * dense_m_, dense_inv_.get(), sizeof(double) * dense_ld_,
+                               sizeof(double) * dense_m_, dense_m_, hipMemcpyDeviceToDevice, s));
+    SGIPC_HIP(hipStreamSynchronize(s));
+    return dense_m_;
 }
 
 void Mas::inverse_blocks(double* out) const

@@ -58,6 +58,11 @@ class Mas {
         return coarse_mode == 2 && cycle_active() && ap_nnz_ > 0 && fine_ != nullptr && fine_->n_rows == n &&
                (!f32 || fine_->has_f32);
     }
+    // the coarse path the last apply took: 2 the two-grid cycle, 1 the
+    // level-1 cycle, 0 the additive levels (-1 before the first apply).  A
+    // configured mode can fall back silently (no fine operator, one level-1
+    // domain, no FP32 copy), so solves report this, not coarse_mode.
+    int last_path() const { return last_path_; }
     bool keep_graph = false;  // build the level-1 graph also without the cycle
     // flexible PCG: z.Ap partials written by the prolongation when set
     const void* flex_Ap_ = nullptr;
@@ -74,6 +79,9 @@ class Mas {
     const CoarseGraph& level1_graph() const { return graph_; }
     void tables(int* coarse_table, int* going_next) const;
     void inverse_blocks(double* out) const;
+    // the dense level-2 inverse of the level-1 cycle as m x m row-major into
+    // out (device; nullptr: size only); returns m (0: no exact level-2 solve)
+    int level2_inverse(double* out) const;
 
     Context* ctx;
     int B;
@@ -103,6 +111,8 @@ class Mas {
     int finish_v_t(hipStream_t s, TS* R, TS* Z, const TS* minv, const int* done, const PcgStopDesc& sd2,
                    const TV* r, TV* z, double* rz_part);
     bool want_graph() const { return coarse_mode >= 1 || keep_graph; }
+    int last_path_ = -1;
+    void note_path(bool f32) { last_path_ = vcycle_active(f32) ? 2 : (cycle_active() ? 1 : 0); }
     // the cycle needs level 1 to span several domains (one domain is solved
     // exactly already) and a coarser level
     bool cycle_active() const
@@ -122,19 +132,19 @@ class Mas {
     FullMatrix own_fine_;
 
     void build_dense_level2(hipStream_t s, const double* v, const int* r, const int* c, int nb);
-    void finish_dense_level2(hipStream_t s);  // joins the factorisation stream
+    void finish_dense_level2(hipStream_t s);  // reads the inverse's pivot check
     static constexpr long kDenseMax = 4608;  // largest dense level-2 system (3 c_2)
     DevBuf<uint16_t> minv16_;    // level-0 inverse, FP16 scaled form (half_level0)
     DevBuf<float> msc_;          // its per-slot scales
     bool half_ready_ = false;
-    DevBuf<double> dense_inv_;   // inverse of the level-2 operator (FP64, m x m)
-    DevBuf<int> dense_info_;
+    DevBuf<double> dense_inv_;   // inverse of the level-2 operator (FP64, m x m in ld x ld)
+    DevBuf<double> dense_piv_;   // blocked Gauss-Jordan: inverse of the pivot tile
+    DevBuf<double> dense_col_;   // blocked Gauss-Jordan: the pivot column's tiles
+    DevBuf<int> dense_info_;     // nonzero: a pivot was not positive
     int dense_m_ = 0;
+    int dense_ld_ = 0;
     int dense_m_pending_ = 0;
     bool dense_pending_ = false;
-    void* rb_ = nullptr;         // rocBLAS handle of the factorisation
-    hipStream_t s2_ = nullptr;   // factorisation stream (default priority)
-    hipEvent_t ev_in_ = nullptr, ev_scatter_ = nullptr, ev_done_ = nullptr;
 
     CoarseGraph graph_;          // level-1 operator (coarse cycle, learned level)
     DevBuf<double> g1d_, c2d_;   // level-1 cycle vectors (FP64 apply)

@@ -24,11 +24,8 @@ int run(sgipc_ctx* ctx, F&& f)
 
 Converter& converter_of(sgipc_ctx* ctx)
 {
-    static thread_local std::vector<std::pair<sgipc_ctx*, std::unique_ptr<Converter>>> table;
-    for (auto& e : table)
-        if (e.first == ctx) return *e.second;
-    table.emplace_back(ctx, std::make_unique<Converter>());
-    return *table.back().second;
+    if (!ctx->conv) ctx->conv = std::make_unique<Converter>();
+    return *ctx->conv;
 }
 
 }  // namespace
@@ -52,6 +49,7 @@ struct sgipc_system {
     // affine bodies: point-level contact scratch, class mapping, block-Jacobi
     DevBuf<double> cvals;
     DevBuf<int> crows, ccols;
+    DevBuf<double> gpt;     // point-space contact gradient (body points, then FEM points)
     ContactMapper cmap;
     AbdJacobi abdj;
     SolveWorkspace* ws = nullptr;
@@ -139,21 +137,23 @@ int sgipc_quad_bending_gradient_hessian(sgipc_ctx* ctx, const double* x, const d
 
 int sgipc_abd_body_gradient_hessian(sgipc_ctx* ctx, int n_bodies, const double* q, const double* q_tilde,
                                     const double* mass10, const double* body_volume, double kappa, double dt,
-                                    const int* body_fixed, double* grad, double* tvals, int* trows, int* tcols)
+                                    const int* body_fixed, double motor_speed, double motor_strength, double* grad,
+                                    double* tvals, int* trows, int* tcols)
 {
     if (!ctx || n_bodies < 0) return SGIPC_BAD_ARG;
     if (n_bodies == 0) return SGIPC_EMPTY;
     if (!q || !q_tilde || !mass10 || !body_volume || !tvals || !trows || !tcols) return SGIPC_BAD_ARG;
     return run(ctx, [&] {
-        launch_abd_body(ctx->c.stream, n_bodies, q, q_tilde, mass10, body_volume, kappa, dt, body_fixed, grad, tvals,
-                        trows, tcols);
+        launch_abd_body(ctx->c.stream, n_bodies, q, q_tilde, mass10, body_volume, kappa, dt, body_fixed, motor_speed,
+                        motor_strength, grad, tvals, trows, tcols);
         SGIPC_HIP(hipStreamSynchronize(ctx->c.stream));
     });
 }
 
 int sgipc_contact_partition(sgipc_ctx* ctx, const double* vals, const int* rows, const int* cols, int n,
                             int n_abd_points, const int* point_body, const double* x_bar, int n_bodies,
-                            const int* body_fixed, double* ovals, int* orows, int* ocols, int capacity, int* out_n)
+                            const int* body_fixed, const int* fem_boundary_type, double* ovals, int* orows,
+                            int* ocols, int capacity, int* out_n)
 {
     if (!ctx || !out_n || n < 0 || n_abd_points < 0 || n_bodies < 0 || capacity < 0) return SGIPC_BAD_ARG;
     *out_n = 0;
@@ -164,8 +164,8 @@ int sgipc_contact_partition(sgipc_ctx* ctx, const double* vals, const int* rows,
         ContactMapper cm;
         const long total = cm.count(ctx->c.stream, rows, cols, n, n_abd_points);
         *out_n = (int)total;
-        cm.write(ctx->c.stream, vals, rows, cols, n_abd_points, point_body, x_bar, n_bodies, body_fixed, ovals, orows,
-                 ocols, capacity);
+        cm.write(ctx->c.stream, vals, rows, cols, n_abd_points, point_body, x_bar, n_bodies, body_fixed,
+                 fem_boundary_type, ovals, orows, ocols, capacity);
         SGIPC_HIP(hipStreamSynchronize(ctx->c.stream));
     });
 }
@@ -218,11 +218,15 @@ int sgipc_convert_srbk(sgipc_ctx* ctx, const double* tvals, const int* trows, co
     *out_nnz = 0;
     if (n_triplets == 0) return SGIPC_EMPTY;
     if (!tvals || !trows || !tcols || !ovals || !orows || !ocols) return SGIPC_BAD_ARG;
-    return run(ctx, [&] {
+    const int st = run(ctx, [&] {
         Converter& cv = converter_of(ctx);
         int mx = cv.max_index(ctx->c.stream, trows, tcols, n_triplets);
+        cv.last_unique = 0;
         *out_nnz = cv.convert(ctx->c.stream, tvals, trows, tcols, n_triplets, mx + 1, ovals, orows, ocols, capacity);
     });
+    // a capacity failure reports the unique count the output needs
+    if (st == SGIPC_CAPACITY && ctx->conv) *out_nnz = ctx->conv->last_unique;
+    return st;
 }
 
 int sgipc_mas_create(sgipc_ctx* ctx, sgipc_mas** out, int bank_size);
@@ -357,9 +361,16 @@ int system_solve(sgipc_system* sys, const double* b, double* x, double* grad, co
                               sys->trows.get(), sys->tcols.get());
             off += 16L * d.n_pairs;
         } else {
-            SGIPC_CHECK(grad == nullptr, SGIPC_BAD_ARG, "the gradient is not assembled for scenes with affine bodies");
-            // point-level contact blocks, then the class mapping into the arena
-            launch_barrier_pt(s, d.n_pairs, d.x, d.pairs, d.pair_x, d.dhat_sq, d.kappa, 0, nullptr, sys->cvals.get(),
+            // point-level contact blocks and gradient, then the class mapping
+            // into the arena and J^T g into the body rows
+            double* gp = nullptr;
+            const long n_points = (long)d.n_abd_points + d.n_verts;
+            if (grad && d.n_pairs > 0) {
+                sys->gpt.reserve(3 * n_points);
+                launch_fill<double>(s, sys->gpt.get(), 0.0, 3 * n_points);
+                gp = sys->gpt.get();
+            }
+            launch_barrier_pt(s, d.n_pairs, d.x, d.pairs, d.pair_x, d.dhat_sq, d.kappa, 0, gp, sys->cvals.get(),
                               sys->crows.get(), sys->ccols.get());
             const long m = sys->cmap.count(s, sys->crows.get(), sys->ccols.get(), 16L * d.n_pairs, d.n_abd_points);
             if (m + sys->n_trip > sys->arena_cap) {
@@ -374,10 +385,15 @@ int system_solve(sgipc_system* sys, const double* b, double* x, double* grad, co
                 sys->bcols.reserve(sys->capacity);
             }
             sys->cmap.write(s, sys->cvals.get(), sys->crows.get(), sys->ccols.get(), d.n_abd_points, d.point_body,
-                            d.x_bar, nb, d.body_fixed, sys->tvals.get(), sys->trows.get(), sys->tcols.get(), m);
+                            d.x_bar, nb, d.body_fixed, d.boundary_type, sys->tvals.get(), sys->trows.get(),
+                            sys->tcols.get(), m);
             off += m;
-            launch_abd_body(s, nb, d.q, d.q_tilde, d.mass10, d.body_volume, d.kappa_abd, d.dt, d.body_fixed, nullptr,
-                            sys->tvals.get() + 9 * off, sys->trows.get() + off, sys->tcols.get() + off);
+            launch_abd_body(s, nb, d.q, d.q_tilde, d.mass10, d.body_volume, d.kappa_abd, d.dt, d.body_fixed,
+                            d.motor_speed, d.motor_strength, grad, sys->tvals.get() + 9 * off, sys->trows.get() + off,
+                            sys->tcols.get() + off);
+            if (gp)
+                launch_point_gradient(s, (int)n_points, d.n_abd_points, gp, d.point_body, d.x_bar, d.body_fixed, nb,
+                                      grad);
             off += 10L * nb;
         }
         double* gf = grad ? grad + 3L * off0 : nullptr;
@@ -390,6 +406,10 @@ int system_solve(sgipc_system* sys, const double* b, double* x, double* grad, co
         launch_tri_bw(s, d.n_tris, d.x, d.tris, d.dm_inv2, d.area, d.k_stretch, d.k_shear, d.strain_rate, d.dt, off0,
                       gf, sys->tvals.get() + 9 * off, sys->trows.get() + off, sys->tcols.get() + off);
         off += 6L * d.n_tris;
+        // Dirichlet vertices: every block before the mass diagonal with a
+        // fixed FEM end is zeroed, and their gradient rows
+        launch_dirichlet_zero(s, off, sys->trows.get(), sys->tcols.get(), off0, d.boundary_type, sys->tvals.get());
+        if (gf) launch_dirichlet_zero_rows(s, d.n_verts, d.boundary_type, gf);
         launch_mass_diag(s, d.n_verts, d.mass, off0, sys->tvals.get() + 9 * off, sys->trows.get() + off,
                          sys->tcols.get() + off);
         off += d.n_verts;
@@ -450,31 +470,33 @@ int system_solve(sgipc_system* sys, const double* b, double* x, double* grad, co
         }
         sys->mas->m->net_ = nullptr;
         // coarse mode 2 is positive definite only for smoothing weights below
-        // 2 / lambda_max(M0 A): a run that meets r.z <= 0 is redone with half
-        // the weight (kept for later solves), and after two halvings with the
-        // level-1 cycle alone
+        // 2 / lambda_max(M0 A): a run of the two-grid cycle that meets
+        // r.z <= 0 is redone with half the weight (at most twice, never below
+        // kOmegaFloor; a weight that cures it is kept for later solves), and
+        // then with the level-1 cycle alone, the weight restored (halving did
+        // not cure it, so the weight was not the cause)
         Mas* mm = sys->mas->m.get();
-        int mode_used = mm->coarse_mode;
-        if (status == SGIPC_NOT_SPD && mm->coarse_mode == 2 && !used) {
-            for (int tries = 0; tries < 3 && status == SGIPC_NOT_SPD; ++tries) {
-                if (tries < 2) {
-                    mm->omega0 *= 0.5;
+        constexpr double kOmegaFloor = 0.05;
+        if (status == SGIPC_NOT_SPD && mm->last_path() == 2 && !used) {
+            const double w0 = mm->omega0;
+            for (int tries = 0; tries < 2 && status == SGIPC_NOT_SPD && mm->omega0 * 0.5 >= kOmegaFloor; ++tries) {
+                mm->omega0 *= 0.5;
+                status = run_pcg(o);
+            }
+            if (status == SGIPC_NOT_SPD) {
+                mm->omega0 = w0;
+                mm->coarse_mode = 1;
+                try {
                     status = run_pcg(o);
-                } else {
-                    mm->coarse_mode = 1;
-                    mode_used = 1;
-                    try {
-                        status = run_pcg(o);
-                    } catch (...) {
-                        mm->coarse_mode = 2;
-                        throw;
-                    }
+                } catch (...) {
                     mm->coarse_mode = 2;
+                    throw;
                 }
+                mm->coarse_mode = 2;
             }
         }
         pst.learned = used ? 1 : (abandoned ? 2 : 0);
-        pst.coarse_mode = mode_used;
+        pst.coarse_mode = mm->last_path();
         pst.level2_exact = mm->level2_exact() ? 1 : 0;
         pst.omega = mm->omega0;
         if (record == nullptr && status == SGIPC_OK) {

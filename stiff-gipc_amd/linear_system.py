@@ -172,6 +172,18 @@ class MASPreconditioner:
         out.free()
         return a
 
+    def level2_inverse(self):
+        """The dense level-2 inverse of coarse modes 1 and 2 (m x m), or None."""
+        m = ctypes.c_int(0)
+        check(lib().sgipc_mas_level2_inverse(self.h, None, ctypes.byref(m)), self.ctx.h)
+        if m.value == 0:
+            return None
+        out = self.ctx.empty((m.value, m.value), np.float64)
+        check(lib().sgipc_mas_level2_inverse(self.h, _p(out), ctypes.byref(m)), self.ctx.h)
+        a = out.download()
+        out.free()
+        return a
+
     def close(self):
         if self.h and self.ctx.h:
             lib().sgipc_mas_destroy(self.h)
@@ -283,6 +295,10 @@ def upload_scene(ctx: Context, scene):
         desc.body_fixed = put("body_fixed", scene.body_fixed, np.int32)
         desc.point_body = put("point_body", scene.point_body, np.int32)
         desc.x_bar = put("x_bar", scene.x_bar.reshape(-1), np.float64)
+        desc.motor_speed = getattr(scene, "motor_speed", 31.4)
+        desc.motor_strength = getattr(scene, "motor_strength", 10.0)
+    bt = getattr(scene, "boundary_type", None)
+    desc.boundary_type = put("boundary_type", bt, np.int32) if bt is not None else None
     desc.nbr_start = put("nbr_start", scene.nbr_start, np.uint32)
     desc.nbr_num = put("nbr_num", scene.nbr_num, np.uint32)
     desc.nbr_list = put("nbr_list", scene.nbr_list if scene.nbr_list.size else np.zeros(1, np.uint32), np.uint32)

@@ -76,6 +76,7 @@ class TetScene:
     kappa: float
     rhs: np.ndarray          # (3n,) float64 right-hand side
     seeds: dict
+    boundary_type: np.ndarray | None = None  # (n,) int32, nonzero = Dirichlet vertex (TetMesh.BoundaryType)
 
     @property
     def n(self) -> int:
@@ -141,6 +142,7 @@ class ClothScene:
     kappa: float
     rhs: np.ndarray
     seeds: dict
+    boundary_type: np.ndarray | None = None  # (n,) int32, nonzero = Dirichlet vertex
 
     @property
     def n(self) -> int:
@@ -489,6 +491,8 @@ class HybridScene(TetScene):
     n_abd_points: int = 0
     point_body: np.ndarray | None = None   # (n_abd_points,) int32
     x_bar: np.ndarray | None = None        # (n_abd_points, 3) rest coordinates in the body frame
+    motor_speed: float = 31.4              # rad/s (abd_system_parms.h:13)
+    motor_strength: float = 10.0           # multiple of the body mass block (abd_system_parms.h:14)
 
     @property
     def n_sys(self) -> int:
@@ -501,7 +505,7 @@ class HybridScene(TetScene):
 
 
 def hybrid_scene(name: str, N: int, n_bodies: int, n_pairs: int, *, E=1e5, body_side_cells=2.0,
-                 kappa_abd=1e8, density=1e3, n_fixed=2, spacing=0.01, dhat=1e-3, kappa=1e6, dt=0.01,
+                 kappa_abd=1e8, density=1e3, n_fixed=2, n_motor=0, spacing=0.01, dhat=1e-3, kappa=1e6, dt=0.01,
                  seed_bodies=4, seed_pairs=3, seed_rhs=1) -> HybridScene:
     """N^3 Neo-Hookean grid (Kuhn split, Morton order) with n_bodies affine
     cubes resting above its top face; n_pairs seeded point-triangle pairs
@@ -531,8 +535,9 @@ def hybrid_scene(name: str, N: int, n_bodies: int, n_pairs: int, *, E=1e5, body_
     mass10[:, 7] = d
     mass10[:, 9] = d
     body_volume = np.full(nb, side ** 3)
-    body_fixed = np.zeros(nb, np.int32)
+    body_fixed = np.zeros(nb, np.int32)   # BodyBoundaryType: 0 free, 1 fixed, 2 motor
     body_fixed[:n_fixed] = 1
+    body_fixed[n_fixed:n_fixed + n_motor] = 2
     n_abd_points = 8 * nb
     point_body = np.repeat(np.arange(nb), 8).astype(np.int32)
     x_bar = np.tile(corners, (nb, 1))
@@ -576,3 +581,12 @@ def scene_hybrid_tiny(N: int = 8, n_bodies: int = 6, n_pairs: int = 40) -> Hybri
 def scene_c2(n_pairs: int = 30_000) -> HybridScene:
     """48^3 Neo-Hookean grid (E = 1e5) + 200 affine bodies (kappa 1e8), 30k pairs."""
     return hybrid_scene("c2_hybrid48", 48, 200, n_pairs)
+
+
+def with_dirichlet(scene, which="bottom"):
+    """A copy of the scene whose lowest layer of vertices (z within 1e-9 of
+    the minimum rest z; ``which="bottom"``) is Dirichlet, as a scene's
+    BoundaryType marks pinned vertices."""
+    z = scene.rest[:, 2]
+    bt = (z <= z.min() + 1e-9).astype(np.int32)
+    return dataclasses.replace(scene, boundary_type=bt)

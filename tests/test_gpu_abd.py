@@ -42,7 +42,7 @@ def test_abd_body_matches_oracle(pkg, orc, gpu_ctx, hyb, hyb_big, which):
     grad = ctx.empty(12 * nb, np.float64)
     pkg._lib.check(L.sgipc_abd_body_gradient_hessian(ctx.h, nb, vp(desc.q), vp(desc.q_tilde), vp(desc.mass10),
                                                       vp(desc.body_volume), s.kappa_abd, s.dt, vp(desc.body_fixed),
-                                                      grad.vp, tv.vp, tr.vp, tc.vp), ctx.h)
+                                                      s.motor_speed, s.motor_strength, grad.vp, tv.vp, tr.vp, tc.vp), ctx.h)
     rv, rr, rc, rg = orc.abd_body(s)
     np.testing.assert_array_equal(tr.download(), rr)
     np.testing.assert_array_equal(tc.download(), rc)
@@ -72,7 +72,7 @@ def test_contact_partition_matches_oracle(pkg, orc, gpu_ctx, hyb_big):
     n_out = ctypes.c_int()
     pkg._lib.check(L.sgipc_contact_partition(ctx.h, dv.vp, dr.vp, dc.vp, br.size, s.n_abd_points,
                                               vp(desc.point_body), vp(desc.x_bar), s.n_bodies, vp(desc.body_fixed),
-                                              gv.vp, gr.vp, gc.vp, cap, ctypes.byref(n_out)), ctx.h)
+                                              vp(desc.boundary_type), gv.vp, gr.vp, gc.vp, cap, ctypes.byref(n_out)), ctx.h)
     m = n_out.value
     assert m == orr.size
     np.testing.assert_array_equal(gr.download()[:m], orr)
@@ -118,5 +118,110 @@ def test_hybrid_system_solve_matches_oracle(pkg, orc, hyb, hyb_big, which):
             fixed = ls.solve(tol=1e-30, max_iter=k)
             xr = orc.solve_scene(s, 32, tol=1e-30, max_iter=k).x
             assert np.linalg.norm(fixed.x - xr) / np.linalg.norm(xr) <= 1e-8, k
+    finally:
+        ls.close()
+
+
+# ---- Dirichlet vertices, the assembled right-hand side, motor bodies
+@pytest.fixture(scope="module")
+def hyb_bc(pkg):
+    """Hybrid scene with the grid's bottom layer pinned (Dirichlet), two
+    fixed bodies and two motor bodies."""
+    s = pkg.scenes.hybrid_scene("hybrid12_bc", 12, 12, 300, n_fixed=2, n_motor=2)
+    return pkg.scenes.with_dirichlet(s)
+
+
+def test_motor_bodies_match_oracle(pkg, orc, gpu_ctx, hyb_bc):
+    """Body terms with motor bodies (BodyBoundaryType 2): the device's closed
+    form of the motor target against the oracle's J^-1 J_delta restatement."""
+    s = hyb_bc
+    L, ctx = pkg._lib.lib(), gpu_ctx
+    desc, keep = pkg.upload_scene(ctx, s)
+    nb = s.n_bodies
+    assert (s.body_fixed == 2).sum() == 2
+    tv, tr, tc = ctx.empty((10 * nb, 9), np.float64), ctx.empty(10 * nb, np.int32), ctx.empty(10 * nb, np.int32)
+    grad = ctx.empty(12 * nb, np.float64)
+    pkg._lib.check(L.sgipc_abd_body_gradient_hessian(ctx.h, nb, vp(desc.q), vp(desc.q_tilde), vp(desc.mass10),
+                                                      vp(desc.body_volume), s.kappa_abd, s.dt, vp(desc.body_fixed),
+                                                      s.motor_speed, s.motor_strength, grad.vp, tv.vp, tr.vp, tc.vp),
+                   ctx.h)
+    rv, rr, rc, rg = orc.abd_body(s)
+    gv = tv.download()
+    err = np.abs(gv - rv).reshape(nb, -1).max(1)
+    mag = np.abs(rv).reshape(nb, -1).max(1)
+    assert np.all(err <= 1e-10 * mag), float((err / mag).max())
+    g = grad.download().reshape(nb, 12)
+    rg = rg.reshape(nb, 12)
+    for b in range(nb):
+        assert np.abs(g[b] - rg[b]).max() <= 1e-10 * max(np.abs(rg[b]).max(), 1e-300), b
+    # the motor term is present: the motor bodies differ from the same bodies set free
+    free = pkg.scenes.hybrid_scene("hybrid12_bc", 12, 12, 300, n_fixed=2, n_motor=0)
+    _, _, _, fg = orc.abd_body(free)
+    m = s.body_fixed == 2
+    assert np.abs(fg.reshape(nb, 12)[m] - rg[m]).max() > 0
+
+
+def test_dirichlet_system_matches_oracle(pkg, orc, hyb_bc):
+    """Whole hybrid solve with Dirichlet vertices: the device BCOO (blocks
+    touching a pinned vertex zeroed, mass kept) and the assembled gradient
+    (bodies: M(q - q~) + shape + J^T contact gradient; FEM rows zero at pinned
+    vertices) equal the oracle's; solving H dx = gradient leaves the pinned
+    vertices decoupled: their rows hold only the mass block, so
+    m |dx| <= ||b - A dx|| there (the MAS coarse levels still spread into
+    them, as in the reference, whose step then leaves pinned vertices alone)."""
+    s = hyb_bc
+    ls = pkg.LinearSystem(s, bank_size=32)
+    try:
+        ov, orr, oc = orc.assemble(s)
+        rv, rr, rc, rg = orc.triplets(s, with_grad=True)
+        res = ls.solve(tol=1e-5, want_grad=True)
+        gv, gr, gc = ls.sys.bcoo()
+        np.testing.assert_array_equal(gr, orr)
+        np.testing.assert_array_equal(gc, oc)
+        assert rel(gv, ov) <= 1e-10
+        g = ls.gradient()
+        assert rel(g, rg) <= 1e-10, rel(g, rg)
+        pinned = np.nonzero(s.boundary_type)[0]
+        off = 4 * s.n_bodies
+        assert np.all(g[3 * off:].reshape(-1, 3)[pinned] == 0.0)
+        # pinned rows and columns carry only the mass block
+        bad = ((gr >= off) & np.isin(gr - off, pinned)) | ((gc >= off) & np.isin(gc - off, pinned))
+        offdiag = bad & (gr != gc)
+        assert np.all(gv[offdiag] == 0.0)
+        # the Newton system with the assembled gradient as right-hand side
+        ls.sys.b.upload(rg)
+        res = ls.solve(tol=1e-5)
+        assert res.status == 0 and res.true_rel_residual <= 1e-5
+        ax = orc.spmv(1.0, ov, orr, oc, res.x, 0.0, np.zeros(3 * s.n_sys), s.n_sys)
+        rr_ = rg - ax
+        assert np.linalg.norm(rr_) / np.linalg.norm(rg) <= 1.01e-5
+        xd = res.x[3 * off:].reshape(-1, 3)[pinned]
+        np.testing.assert_allclose(s.mass[pinned, None] * xd, -rr_[3 * off:].reshape(-1, 3)[pinned],
+                                   rtol=1e-9, atol=1e-300)
+    finally:
+        ls.close()
+
+
+def test_fem_dirichlet_gradient_matches_oracle(pkg, orc):
+    """FEM-only scene with pinned bottom vertices and contacts: BCOO and
+    gradient against the oracle, FP32 refinement with the two-grid cycle."""
+    s = pkg.scenes.with_dirichlet(pkg.scenes.tet_grid_scene("bc_grid12", 12, E_loguniform=(1e4, 1e9), n_pairs=300))
+    ls = pkg.LinearSystem(s, bank_size=32, coarse_mode=2)
+    try:
+        rv, rr, rc, rg = orc.triplets(s, with_grad=True)
+        ov, orr, oc = orc.assemble(s)
+        res = ls.solve(tol=1e-5, want_grad=True, precision="fp32_ir")
+        gv, gr, gc = ls.sys.bcoo()
+        np.testing.assert_array_equal(gr, orr)
+        assert rel(gv, ov) <= 1e-10
+        assert rel(ls.gradient(), rg) <= 1e-10
+        ls.sys.b.upload(rg)
+        res = ls.solve(tol=1e-5, precision="fp32_ir")
+        assert res.status == 0 and res.true_rel_residual <= 1e-5
+        pinned = np.nonzero(s.boundary_type)[0]
+        ax = orc.spmv(1.0, ov, orr, oc, res.x, 0.0, np.zeros(3 * s.n), s.n)
+        # decoupled pinned rows: m dx = -(b - A dx) there
+        np.testing.assert_allclose(s.mass[pinned, None] * res.x.reshape(-1, 3)[pinned],
+                                   -(rg - ax).reshape(-1, 3)[pinned], rtol=1e-9, atol=1e-300)
     finally:
         ls.close()

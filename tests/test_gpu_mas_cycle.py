@@ -61,6 +61,47 @@ def test_cycle_apply_matches_oracle(pkg, orc, gpu_ctx, B, which, scene_tiny, sce
         om.close()
 
 
+def level2_inverse_check(pkg, orc, mas, om, s, vals, rows, cols):
+    """The device's dense level-2 inverse against numpy.linalg.inv of the
+    oracle's FP64 Galerkin operator P_12^T A_1 P_12: its residual
+    max|A_2 G - I| within 10x numpy's own, the entries within 1e-6 of the
+    largest, symmetric to 1e-9."""
+    parts = mpnn.MasParts(om)
+    A = mpnn.bcoo_to_csr(vals, rows, cols, s.n)
+    graph = mpnn.coarse_graph(A, parts.map1, parts.cnt[1])
+    A2 = mpnn.level2_operator(parts, graph)
+    G = mas.level2_inverse()
+    assert G is not None and G.shape == A2.shape, (None if G is None else G.shape, A2.shape)
+    ref = np.linalg.inv(A2)
+    eye = np.eye(A2.shape[0])
+    res_gpu = np.abs(A2 @ G - eye).max()
+    res_np = np.abs(A2 @ ref - eye).max()
+    diff = np.abs(G - ref).max() / np.abs(ref).max()
+    asym = np.abs(G - G.T).max() / np.abs(G).max()
+    print(f"level 2: m={A2.shape[0]} residual {res_gpu:.2e} (numpy {res_np:.2e}) diff {diff:.2e} asym {asym:.2e}")
+    assert res_gpu <= 10 * res_np + 1e-12
+    assert diff <= 1e-6 and asym <= 1e-9
+    return A2.shape[0]
+
+
+@pytest.mark.parametrize("N", [24, 32])
+def test_level2_inverse_matches_numpy(pkg, orc, gpu_ctx, N):
+    """Blocked Gauss-Jordan over 96 x 96 tiles: 24^3 at B = 16 gives a level 2
+    of 2 tiles with padding, 32^3 one of 4 tiles."""
+    s = pkg.scenes.tet_grid_scene(f"l2_grid{N}", N, E_loguniform=(1e4, 1e9), n_pairs=20 * N * N)
+    B = 16
+    vals, rows, cols = orc.assemble(s)
+    om = orc.Mas(s, B)
+    om.build(vals, rows, cols, 0, s.pairs)
+    mas, keep = _cycle_mas(pkg, gpu_ctx, s, B, vals, rows, cols, pkg.FP64)
+    try:
+        m = level2_inverse_check(pkg, orc, mas, om, s, vals, rows, cols)
+        assert m > 96
+    finally:
+        mas.close()
+        om.close()
+
+
 def test_cycle_operator_is_spd(pkg, orc, gpu_ctx):
     s = pkg.scenes.tet_grid_scene("small_grid8", 8, E_loguniform=(1e4, 1e9), n_pairs=40)
     B = 16
@@ -239,6 +280,36 @@ def test_vcycle_recovers_from_a_too_large_weight(pkg, precision):
     finally:
         ls.close()
         ref.close()
+
+
+def test_vcycle_weight_after_a_recovery_stays_put(pkg):
+    """Two solves in a row from w = 1.9: a weight that the halving cured is
+    kept, and the second solve then runs the two-grid cycle once, in the
+    iterations of a fresh system given that weight directly; when only the
+    level-1 fallback cured it, the weight is restored, not left halved."""
+    s = pkg.scenes.tet_grid_scene("cycle_grid16", 16, E_loguniform=(1e4, 1e9), n_pairs=800)
+    ls = pkg.LinearSystem(s, bank_size=16, coarse_mode=2)
+    try:
+        ls.set_smoothing_weight(1.9)
+        a = ls.solve(tol=1e-5, precision="fp32_ir")
+        b = ls.solve(tol=1e-5, precision="fp32_ir")
+        print(f"first: mode {a.coarse_mode} w {a.omega} {a.iterations} it; second: mode {b.coarse_mode} "
+              f"w {b.omega} {b.iterations} it")
+        assert a.status == 0 and b.status == 0 and b.true_rel_residual <= 1e-5
+        assert b.omega == a.omega
+        if a.coarse_mode == 2:
+            assert 0.05 <= a.omega < 1.9 and b.coarse_mode == 2
+            fresh = pkg.LinearSystem(s, bank_size=16, coarse_mode=2)
+            try:
+                fresh.set_smoothing_weight(a.omega)
+                c = fresh.solve(tol=1e-5, precision="fp32_ir")
+            finally:
+                fresh.close()
+            assert b.iterations == c.iterations
+        else:
+            assert a.coarse_mode == 1 and a.omega == 1.9
+    finally:
+        ls.close()
 
 
 @pytest.mark.parametrize("B", [16, 32])

@@ -147,3 +147,96 @@ def test_block_jacobi_and_hybrid_pcg(orc, pkg):
     assert res.true_rel_residual <= 1e-5
     assert np.linalg.norm(s.rhs - A @ res.x) / np.linalg.norm(s.rhs) <= 1e-5 + 1e-12
     m.close()
+
+
+def _one_body_scene(pkg, q, q_tilde, fixed, theta_speed=31.4, strength=10.0, dt=0.01):
+    """A hybrid scene reduced to its body terms: one body with the given q."""
+    s = pkg.scenes.scene_hybrid_tiny(N=4, n_bodies=1, n_pairs=0)
+    import dataclasses
+    return dataclasses.replace(s, q=q[None, :], q_tilde=q_tilde[None, :], body_fixed=np.array([fixed], np.int32),
+                               motor_speed=theta_speed, motor_strength=strength, dt=dt)
+
+
+def test_motor_term_closed_form(orc, pkg):
+    """Motor body (setup_abd_system_gradient_and_hessian.cu:271-339) with
+    A~ = I: the target rows a2, a3 of q_p are the rows of R(theta) (the rest
+    frame turned about x), so the motor adds W (q - q_R) to the a2, a3 rows
+    of G and W to H there, W = strength * M[6:12, 6:12]; the motor body
+    differs from the same body set free by exactly that."""
+    rng = np.random.default_rng(3)
+    for speed in (31.4, 100.0):
+        qt = np.r_[rng.standard_normal(3), np.eye(3).ravel()]
+        q = qt + 0.01 * rng.standard_normal(12)
+        dt, strength = 0.01, 7.0
+        mot = _one_body_scene(pkg, q, qt, 2, speed, strength, dt)
+        free = _one_body_scene(pkg, q, qt, 0, speed, strength, dt)
+        vm, _, _, gm = orc.abd_body(mot)
+        vf, _, _, gf = orc.abd_body(free)
+        th = speed * dt
+        c, s_ = np.cos(th), np.sin(th)
+        target = np.array([[0, c, -s_], [0, s_, c]]).ravel()  # rows 1, 2 of R
+        M = np.zeros((12, 12))
+        m10 = mot.mass10[0]
+        M[:3, :3] = m10[0] * np.eye(3)
+        D = np.array([[m10[4], m10[5], m10[6]], [m10[5], m10[7], m10[8]], [m10[6], m10[8], m10[9]]])
+        for r in range(3):
+            M[r, 3 + 3 * r:6 + 3 * r] = m10[1:4]
+            M[3 + 3 * r:6 + 3 * r, r] = m10[1:4]
+            M[3 + 3 * r:6 + 3 * r, 3 + 3 * r:6 + 3 * r] = D
+        W = strength * M[6:, 6:]
+        expect = np.zeros(12)
+        expect[6:] = W @ (q[6:] - target)
+        assert np.max(np.abs((gm - gf) - expect)) <= 1e-12 * np.max(np.abs(expect))
+        # the Hessian difference is W on the (a2, a3) blocks: blocks (2,2), (2,3), (3,3)
+        dv = (vm - vf).reshape(10, 3, 3)          # upper blocks of rows 0..3, column-major
+        blocks = {(i, j): k for k, (i, j) in enumerate((i, j) for i in range(4) for j in range(i, 4))}
+        for (i, j), k in blocks.items():
+            want = W[3 * (i - 2):3 * (i - 1), 3 * (j - 2):3 * (j - 1)] if i >= 2 and j >= 2 else np.zeros((3, 3))
+            # (H + W) - H rounds at the size of H's shape block
+            assert np.max(np.abs(dv[k].T - want)) <= 1e-12 * np.max(np.abs(vm)), (i, j)
+
+
+def test_dirichlet_assembly_projects_the_pinned_vertices(orc, pkg):
+    """With pinned vertices the assembled matrix equals the free one on the
+    free rows and columns, and holds only the mass block on a pinned vertex's
+    rows and columns (GIPC.cu:10332-10363,10444-10462); the gradient is the
+    free one with the pinned rows zeroed (fem_linear_subsystem.cu:73-76).
+    Checked on dense matrices of a small contact scene and of a hybrid one."""
+    for s0 in (pkg.scenes.tet_grid_scene("bc6", 6, E_loguniform=(1e4, 1e9), n_pairs=30),
+               pkg.scenes.scene_hybrid_tiny(N=6, n_bodies=4, n_pairs=30)):
+        s1 = pkg.scenes.with_dirichlet(s0)
+        n_sys = getattr(s0, "n_sys", s0.n)
+        off = 4 * getattr(s0, "n_bodies", 0)
+        H0 = bcoo_to_csr(*orc.assemble(s0), n_sys).toarray()
+        H1 = bcoo_to_csr(*orc.assemble(s1), n_sys).toarray()
+        _, _, _, g0 = orc.triplets(s0, with_grad=True)
+        _, _, _, g1 = orc.triplets(s1, with_grad=True)
+        pin = np.zeros(n_sys, bool)
+        pin[off:] = s1.boundary_type != 0
+        pin3 = np.repeat(pin, 3)
+        free = ~pin3
+        assert pin.sum() > 0
+        assert np.max(np.abs(H1[np.ix_(free, free)] - H0[np.ix_(free, free)])) <= 1e-12 * np.abs(H0).max()
+        assert np.all(H1[np.ix_(pin3, free)] == 0) and np.all(H1[np.ix_(free, pin3)] == 0)
+        m = np.repeat(s0.mass[pin[off:]], 3)
+        np.testing.assert_array_equal(H1[np.ix_(pin3, pin3)], np.diag(m))
+        np.testing.assert_array_equal(g1[pin3], 0.0)
+        assert np.max(np.abs(g1[free] - g0[free])) <= 1e-13 * np.abs(g0).max()
+
+
+def test_hybrid_gradient_is_jt_of_the_point_gradient(orc, pkg):
+    """The body rows of the assembled gradient minus the body term equal
+    sum J(xbar_i)^T g_i over the body's points (dense 3 x 12 Jacobians), for
+    non-fixed bodies; fixed bodies carry zero."""
+    s = pkg.scenes.scene_hybrid_tiny(N=6, n_bodies=5, n_pairs=60)
+    rng = np.random.default_rng(2)
+    gp = rng.standard_normal(3 * (s.n_abd_points + s.n))
+    g = orc.point_gradient_to_system(s, gp)
+    want = np.zeros(12 * s.n_bodies)
+    for i in range(s.n_abd_points):
+        b = s.point_body[i]
+        if s.body_fixed[b] == 1:
+            continue
+        want[12 * b:12 * b + 12] += J_of(s.x_bar[i]).T @ gp[3 * i:3 * i + 3]
+    assert np.max(np.abs(g[:12 * s.n_bodies] - want)) <= 1e-13 * np.abs(want).max()
+    np.testing.assert_array_equal(g[12 * s.n_bodies:], gp[3 * s.n_abd_points:])
