@@ -355,6 +355,11 @@ typedef struct sgipc_pcg_stats {
     int coarse_mode;
     int level2_exact;
     double omega;
+    /* 1: a mixed-precision system solve failed (an indefinite FP32/FP16        */
+    /* preconditioner, or refinement passes that stopped reducing the FP64      */
+    /* residual) and was redone in FP64 with the preconditioner rebuilt in FP64; */
+    /* iterations then count the last mixed-precision run and the FP64 run       */
+    int fp64_fallback;
 } sgipc_pcg_stats;
 
 void sgipc_pcg_default_opts(sgipc_pcg_opts* opts);

@@ -37,7 +37,7 @@ class PcgOpts(ctypes.Structure):
 class PcgStats(ctypes.Structure):
     _fields_ = [("iterations", ci), ("outer_iterations", ci), ("rel_residual", cd), ("rz", cd), ("rz0", cd),
                 ("ms_total", cd), ("status", ci), ("learned", ci), ("coarse_mode", ci), ("level2_exact", ci),
-                ("omega", cd)]
+                ("omega", cd), ("fp64_fallback", ci)]
 
 
 class SceneDesc(ctypes.Structure):

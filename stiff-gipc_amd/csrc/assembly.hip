@@ -446,8 +446,11 @@ __global__ void k_heads(const uint64_t* __restrict__ sk, const int* __restrict__
     if (i == n - 1 && pos[i] <= capacity) head_pos[pos[i]] = (int)n;  // sentinel
     if (!flag[i]) return;
     const int u = pos[i] - 1;
-    if (u >= capacity) return;
+    if (u > capacity) return;
+    // u == capacity: the first block past the capacity, whose start ends the
+    // last block kept (head_pos holds capacity + 1 entries)
     head_pos[u] = (int)i;
+    if (u == capacity) return;
     const uint64_t k = sk[i];
     orows[u] = (int)(k >> kbits);
     ocols[u] = (int)(k & ((1ull << kbits) - 1ull));

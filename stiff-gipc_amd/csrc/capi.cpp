@@ -591,6 +591,9 @@ int pcg_solve_impl(Context& c, SolveWorkspace& ws, const double* vals, const int
             po.flexible = o.flexible != 0 || (extra && extra->flexible);
             po.true_op = &A64;
             po.replace_every = o.ir_replace_every;
+            // a pass whose FP32 operators cannot resolve the system ends once
+            // its FP64 residual has grown 1e4 times past the pass's start
+            po.diverge = 1e8;
             PcgResult r = ws.pcg64.solve(s, A, has_prec ? &M : nullptr, ws.r64.get(), ws.d64.get(), n3, po);
             total_it += r.iterations;
             if (r.not_spd) {
@@ -601,6 +604,15 @@ int pcg_solve_impl(Context& c, SolveWorkspace& ws, const double* vals, const int
             int nrr = residual(s, ws.full, x, b, ws.r64.get(), ws.part.get(), n3);
             rr = host_sum(ws.part.get(), nrr);
             if (rr <= o.tol * o.tol * bb) ok = true;
+            if (!ok && rr >= best_rr) {
+                // the pass made the FP64 residual worse (the FP32 operators
+                // cannot resolve this system): take its correction back and
+                // stop with the best iterate
+                launch_axpy_f64(s, -1.0, ws.d64.get(), x, n3);
+                nrr = residual(s, ws.full, x, b, ws.r64.get(), ws.part.get(), n3);
+                rr = host_sum(ws.part.get(), nrr);
+                break;
+            }
             if (rr > best_rr * 0.999999 && outer > 1) break;  // stagnation guard
             if (rr < best_rr) best_rr = rr;
         }

@@ -606,19 +606,7 @@ __global__ __launch_bounds__(256) void k_restrict0(PcgStopDesc sd, const int* __
         if (s_done) return;
         double rz = block_sum_array(sd.rz, sd.n_rz, smem);
         double rr = block_sum_array(sd.rr, sd.n_rr, smem);
-        bool stop = (sd.ctl->stop_mode == 0) ? fabs(rz) <= sd.ctl->tol * sd.ctl->rz0
-                                             : rr <= sd.ctl->tol * sd.ctl->tol * sd.ctl->bb;
-        // r.z <= 0 short of convergence: the preconditioner is not positive definite on r
-        const bool bad = !stop && !(rz > 0.0);
-        if (stop || bad) {
-            if (blockIdx.x == 0 && threadIdx.x == 0) {
-                sd.ctl->done = 1;
-                sd.ctl->final_rr = rr;
-                sd.ctl->final_rz = rz;
-                if (bad) sd.ctl->not_spd = 1;
-            }
-            return;
-        }
+        if (pcg_stop_verdict(sd.ctl, rz, rr)) return;
     }
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     const int npad = meta[kMetaOff + 1];
@@ -809,19 +797,7 @@ __global__ __launch_bounds__(256) void k_restrict_up(const int* __restrict__ don
         if (s_done) return;
         const double rz = block_sum_array(sd.rz, sd.n_rz, smem);
         const double rr = block_sum_array(sd.rr, sd.n_rr, smem);
-        const bool stop = (sd.ctl->stop_mode == 0) ? fabs(rz) <= sd.ctl->tol * sd.ctl->rz0
-                                                   : rr <= sd.ctl->tol * sd.ctl->tol * sd.ctl->bb;
-        // r.z <= 0 short of convergence: the preconditioner is not positive definite on r
-        const bool bad = !stop && !(rz > 0.0);
-        if (stop || bad) {
-            if (blockIdx.x == 0 && threadIdx.x == 0) {
-                sd.ctl->done = 1;
-                sd.ctl->final_rr = rr;
-                sd.ctl->final_rz = rz;
-                if (bad) sd.ctl->not_spd = 1;
-            }
-            return;
-        }
+        if (pcg_stop_verdict(sd.ctl, rz, rr)) return;
     } else if (done != nullptr && *done) {
         return;
     }

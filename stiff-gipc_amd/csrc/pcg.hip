@@ -45,21 +45,7 @@ __device__ __forceinline__ bool eval_stop(const PcgStopDesc& sd, double* smem)
     if (s_done) return true;
     double rz = sum_partials(sd.rz, sd.n_rz, smem);
     double rr = sum_partials(sd.rr, sd.n_rr, smem);
-    bool stop;
-    if (sd.ctl->stop_mode == 0)
-        stop = fabs(rz) <= sd.ctl->tol * sd.ctl->rz0;
-    else
-        stop = rr <= sd.ctl->tol * sd.ctl->tol * sd.ctl->bb;
-    // r.z <= 0 short of convergence: the preconditioner is not positive
-    // definite on r, and the iteration cannot continue
-    const bool bad = !stop && !(rz > 0.0);
-    if ((stop || bad) && blockIdx.x == 0 && threadIdx.x == 0) {
-        sd.ctl->done = 1;
-        sd.ctl->final_rr = rr;
-        sd.ctl->final_rz = rz;
-        if (bad) sd.ctl->not_spd = 1;
-    }
-    return stop || bad;
+    return pcg_stop_verdict(sd.ctl, rz, rr);
 }
 
 // x += alpha p; r -= alpha Ap; rr partials.  alpha = rz / pAp.
@@ -173,7 +159,7 @@ __global__ __launch_bounds__(kVecThreads) void dot_partials_kernel(const TA* __r
 }
 
 __global__ void pcg_init_ctl(PcgCtl* ctl, const double* rz_part, int n_rz, const double* bb_part,
-                             int n_bb, double tol, int stop_mode)
+                             int n_bb, double tol, int stop_mode, double diverge)
 {
     __shared__ double smem[16];
     double rz = sum_partials(rz_part, n_rz, smem);
@@ -186,6 +172,8 @@ __global__ void pcg_init_ctl(PcgCtl* ctl, const double* rz_part, int n_rz, const
         ctl->done = (bb == 0.0) ? 1 : 0;
         ctl->iters = 0;
         ctl->not_spd = 0;
+        ctl->diverge = diverge;
+        ctl->diverged = 0;
         ctl->final_rr = bb;
         ctl->final_rz = rz;
         ctl->last_alpha = 0.0;
@@ -379,7 +367,7 @@ PcgResult PcgSolver<TV>::solve(hipStream_t s, const SpmvOp<TV>& A, Preconditione
         n_rz = g;
     }
     pcg_init_ctl<<<1, kVecThreads, 0, s>>>(ctl, part_rz_[cur].get(), n_rz, part_bb_.get(), n_bb,
-                                           opt.tol, opt.stop_mode);
+                                           opt.tol, opt.stop_mode, opt.diverge);
     SGIPC_LAUNCH_CHECK();
     SGIPC_HIP(hipMemcpyAsync(pbuf[pb], z, sizeof(TV) * n3, hipMemcpyDeviceToDevice, s));
     launch_fill<TV>(s, Ap, TV(0), n3);
@@ -566,6 +554,7 @@ PcgResult PcgSolver<TV>::solve(hipStream_t s, const SpmvOp<TV>& A, Preconditione
     res.rz = c.final_rz;
     res.rz0 = c.rz0;
     res.not_spd = c.not_spd != 0;
+    res.diverged = c.diverged != 0;
     return res;
 }
 

@@ -17,10 +17,34 @@ struct PcgCtl {
     int done;
     int iters;
     int not_spd;
+    double diverge;  // > 0: stop (diverged = 1) once ||r||^2 > diverge ||b||^2
+    int diverged;
 };
 
 template <typename TV>
 struct SpmvOp;
+
+
+// The stop rule evaluated after an x/r update, from the summed r.z and r.r:
+// mode 0 |rz| <= tol rz0 (the reference rule), mode 1 ||r||^2 <= tol^2 ||b||^2.
+// Short of convergence, rz <= 0 (the preconditioner is not positive definite
+// on r) stops with not_spd, and under mode 1 with a divergence bound set an
+// r.r above it stops with diverged.  Block 0's thread 0 records the exit.
+// Returns whether the run stops.
+__device__ __forceinline__ bool pcg_stop_verdict(PcgCtl* ctl, double rz, double rr)
+{
+    const bool stop = (ctl->stop_mode == 0) ? fabs(rz) <= ctl->tol * ctl->rz0 : rr <= ctl->tol * ctl->tol * ctl->bb;
+    const bool bad = !stop && !(rz > 0.0);
+    const bool div = !stop && !bad && ctl->stop_mode != 0 && ctl->diverge > 0.0 && !(rr <= ctl->diverge * ctl->bb);
+    if ((stop || bad || div) && blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->done = 1;
+        ctl->final_rr = rr;
+        ctl->final_rz = rz;
+        if (bad) ctl->not_spd = 1;
+        if (div) ctl->diverged = 1;
+    }
+    return stop || bad || div;
+}
 
 // What the first kernel after the x/r update needs to evaluate the stop rule.
 struct PcgStopDesc {
@@ -59,6 +83,10 @@ struct PcgOptions {
     // second chunk on, when the loop's state repeats chunk to chunk) into a
     // HIP graph once per solve and replay it for the later full chunks
     bool graph = false;
+    // > 0 (true-residual rule only): stop once ||r|| exceeds sqrt(diverge)
+    // ||b|| -- an iteration whose operators cannot resolve the system (the
+    // FP32 passes of iterative refinement) ends instead of running on
+    double diverge = 0.0;
 };
 
 // Inputs of the x/r update (x += alpha p, r -= alpha Ap, alpha = rz/pAp) that
@@ -84,6 +112,7 @@ struct PcgResult {
     int iterations = 0;
     bool converged = false;
     bool not_spd = false;
+    bool diverged = false;
     double rel_residual = 0.0;  // recursive residual at exit
     double rz = 0.0;
     double rz0 = 0.0;

@@ -495,6 +495,29 @@ int system_solve(sgipc_system* sys, const double* b, double* x, double* grad, co
                 mm->coarse_mode = 2;
             }
         }
+        // a mixed-precision solve that still failed because the FP32
+        // operators do not resolve the system (an indefinite FP32/FP16
+        // preconditioner, or refinement passes that stopped reducing the FP64
+        // residual before the iteration cap) is redone in FP64 with the
+        // preconditioner rebuilt in FP64
+        int fallback = 0;
+        const int cap_it = o.max_iter > 0 ? o.max_iter : (int)(0.3 * 3.0 * (double)sys->n_sys);
+        const bool precision_failed =
+            status == SGIPC_NOT_SPD || (status == SGIPC_MAXITER && pst.iterations < cap_it - 1);
+        if (precision_failed && mprec != SGIPC_FP64 && record == nullptr) {
+            sgipc_pcg_opts o64 = o;
+            o64.precision = SGIPC_FP64;
+            mm->net_ = nullptr;
+            extra.flexible = false;
+            used = false;
+            mm->set_bcoo(sys->bvals.get(), sys->brows.get(), sys->bcols.get(), sys->nnz, off0, d.pairs, d.n_pairs,
+                         SGIPC_FP64);
+            const int it32 = pst.iterations;
+            status = run_pcg(o64);
+            pst.iterations += it32;
+            fallback = 1;
+        }
+        pst.fp64_fallback = fallback;
         pst.learned = used ? 1 : (abandoned ? 2 : 0);
         pst.coarse_mode = mm->last_path();
         pst.level2_exact = mm->level2_exact() ? 1 : 0;

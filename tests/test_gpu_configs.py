@@ -139,7 +139,7 @@ def test_c3_mixed_precision_matches_fp64(pkg, orc, c1):
         mixed, ms_mixed = timed_solve(ls, tol=1e-5, precision="fp32_ir")
         mixed2, _ = timed_solve(ls, tol=1e-5, precision="fp32_ir")
         plain, ms_plain = timed_solve(ls, tol=1e-5, precision="fp64")
-        tight, _ = timed_solve(ls, tol=1e-10, precision="fp64")
+        tight, _ = timed_solve(ls, tol=1e-11, precision="fp64")
         r_mixed = host_residual(orc, ref, mixed.x, s.rhs, s.n)
         nrm = np.linalg.norm(tight.x)
         e_mixed = np.linalg.norm(mixed.x - tight.x) / nrm
@@ -196,18 +196,24 @@ def c4(pkg, orc):
 
 
 def test_c4_cloth_solves_to_tolerance(pkg, orc, c4):
+    """C4's 200,000 seeded pairs join random vertices across the sheet with
+    barrier stiffness far above the cloth's, so its solves take thousands of
+    iterations.  The FP64 solve (two-grid cycle) must reach 1e-5; the
+    mixed-precision solve must too, through its FP64 fallback when the FP32
+    operators cannot resolve the system (reported in the stats)."""
     s, ref = c4
     ls = pkg.LinearSystem(s, bank_size=32, coarse_mode=2)
     try:
-        cap = 20 * 3 * s.n
-        mixed, ms_mixed = timed_solve(ls, tol=1e-5, precision="fp32_ir", max_iter=cap)
-        check_bcoo(ls, ref)
-        r32 = host_residual(orc, ref, mixed.x, s.rhs, s.n)
+        cap = 20000
         plain, ms_plain = timed_solve(ls, tol=1e-5, precision="fp64", max_iter=cap)
+        check_bcoo(ls, ref)
         r64 = host_residual(orc, ref, plain.x, s.rhs, s.n)
-        print(f"C4: fp32_ir mode {mixed.coarse_mode} {mixed.iterations} it {ms_mixed:.1f} ms res {r32:.2e}; "
-              f"fp64 {plain.iterations} it {ms_plain:.1f} ms res {r64:.2e}")
-        assert mixed.status == 0 and r32 <= 1.01e-5
+        print(f"C4: fp64 mode {plain.coarse_mode} {plain.iterations} it {ms_plain:.1f} ms res {r64:.2e}", flush=True)
+        mixed, ms_mixed = timed_solve(ls, tol=1e-5, precision="fp32_ir", max_iter=cap)
+        r32 = host_residual(orc, ref, mixed.x, s.rhs, s.n)
+        print(f"C4: fp32_ir mode {mixed.coarse_mode} {mixed.iterations} it ({mixed.outer_iterations} pass, "
+              f"fp64 fallback {mixed.fp64_fallback}) {ms_mixed:.1f} ms res {r32:.2e}", flush=True)
         assert plain.status == 0 and r64 <= 1.01e-5
+        assert mixed.status == 0 and r32 <= 1.01e-5
     finally:
         ls.close()
