@@ -181,6 +181,18 @@ int sgipc_convert_srbk(sgipc_ctx* ctx, const double* tvals, const int* trows, co
                        int n_triplets, double* ovals, int* orows, int* ocols, int capacity,
                        int* out_nnz);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): sgipc_convert_srbk with the same arguments, results and
+ * status codes, run in a fixed per-device scratch array (24 MiB, held by each
+ * GPU the library runs on until the process ends) instead of buffers the
+ * library allocates; it calls no hipMalloc/hipFree.  At most 262,144 triplets
+ * per call (SGIPC_CAPACITY beyond).  Every call on one GPU shares that GPU's
+ * scratch: calls that overlap on one GPU (two streams or threads) race. */
+int sgipc_convert_srbk_noalloc(sgipc_ctx* ctx, const double* tvals, const int* trows, const int* tcols,
+                               int n_triplets, double* ovals, int* orows, int* ocols, int capacity,
+                               int* out_nnz);
+
 /* ---- SpMV ------------------------------------------------------------------
  * Replaces Spmv::srbk_spmv (StiffGIPC/linear_system/utils/spmv.cu:188-313):
  * y = a*A*x + b*y for the symmetric matrix stored as upper BCOO. */

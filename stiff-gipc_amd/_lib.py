@@ -84,6 +84,7 @@ _SIGS = {
     "sgipc_abd_jacobi_apply": (ci, [vp, vp, ci, vp, vp]),
     "sgipc_mass_diagonal": (ci, [vp, vp, ci, ci, vp, vp, vp]),
     "sgipc_convert_srbk": (ci, [vp, vp, vp, vp, ci, vp, vp, vp, ci, ctypes.POINTER(ci)]),
+    "sgipc_convert_srbk_noalloc": (ci, [vp, vp, vp, vp, ci, vp, vp, vp, ci, ctypes.POINTER(ci)]),
     "sgipc_spmv_srbk": (ci, [vp, cd, vp, vp, vp, ci, vp, cd, vp, ci]),
     "sgipc_spmv_srbk_f32": (ci, [vp, cf, vp, vp, vp, ci, vp, cf, vp, ci]),
     "sgipc_convert_f64_to_f32": (ci, [vp, vp, vp, csz]),

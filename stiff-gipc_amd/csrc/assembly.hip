@@ -22,6 +22,7 @@
 #include <rocprim/device/device_scan.hpp>
 
 #include "assembly.hpp"
+#include "pcg.hpp"
 #include "wave_ops.hpp"
 
 namespace sgipc {
@@ -522,6 +523,67 @@ void launch_mass_diag(hipStream_t s, int n, const double* mass, int hess_offset,
     SGIPC_LAUNCH_CHECK();
 }
 
+int convert_max_index(hipStream_t s, const int* rows, const int* cols, long n, int* ints)
+{
+    launch_fill<int>(s, ints, 0, 1);
+    k_max_index<<<256, 256, 0, s>>>(rows, cols, n, ints);
+    SGIPC_LAUNCH_CHECK();
+    int m = 0;
+    SGIPC_HIP(hipMemcpyAsync(&m, ints, sizeof(int), hipMemcpyDeviceToHost, s));
+    SGIPC_HIP(hipStreamSynchronize(s));
+    return m;
+}
+
+size_t convert_temp_bytes(hipStream_t s, long n)
+{
+    size_t need = 0, need2 = 0;
+    const int kbits = 31;
+    SGIPC_HIP(rocprim::radix_sort_pairs(nullptr, need, static_cast<uint64_t*>(nullptr), static_cast<uint64_t*>(nullptr),
+                                        static_cast<int*>(nullptr), static_cast<int*>(nullptr), (size_t)n, 0, 2 * kbits,
+                                        s));
+    SGIPC_HIP(rocprim::inclusive_scan(nullptr, need2, static_cast<int*>(nullptr), static_cast<int*>(nullptr), (size_t)n,
+                                      rocprim::plus<int>(), s));
+    return need > need2 ? need : need2;
+}
+
+int convert_into(hipStream_t s, const ConvertScratch& w, const double* tvals, const int* trows, const int* tcols,
+                 long n, int index_bound, double* ovals, int* orows, int* ocols, int capacity, int* unique)
+{
+    *unique = 0;
+    if (n <= 0) return 0;
+    SGIPC_CHECK(n < (1L << 31), SGIPC_CAPACITY, "too many triplets");
+    const int kbits = Converter::index_bits(index_bound);
+    // at most n unique blocks: the kernels see the capacity clamped to n
+    // (same results), so the block starts need n + 1 entries at most
+    const int cap = capacity < n ? capacity : (int)n;
+    launch_fill<int>(s, w.ints, 0, 1);
+    k_make_keys<<<g256(n), 256, 0, s>>>(trows, tcols, n, kbits, w.keys, w.idx, w.ints);
+    SGIPC_LAUNCH_CHECK();
+    size_t need = 0, need2 = 0;
+    SGIPC_HIP(rocprim::radix_sort_pairs(nullptr, need, w.keys, w.skeys, w.idx, w.sidx, (size_t)n, 0, 2 * kbits, s));
+    SGIPC_HIP(rocprim::inclusive_scan(nullptr, need2, w.flag, w.pos, (size_t)n, rocprim::plus<int>(), s));
+    SGIPC_CHECK(need <= w.temp_bytes && need2 <= w.temp_bytes, SGIPC_CAPACITY,
+                "conversion scratch smaller than the sort needs");
+    SGIPC_HIP(rocprim::radix_sort_pairs(w.temp, need, w.keys, w.skeys, w.idx, w.sidx, (size_t)n, 0, 2 * kbits, s));
+    k_head_flags<<<g256(n), 256, 0, s>>>(w.skeys, n, w.flag);
+    SGIPC_LAUNCH_CHECK();
+    SGIPC_HIP(rocprim::inclusive_scan(w.temp, need2, w.flag, w.pos, (size_t)n, rocprim::plus<int>(), s));
+    k_heads<<<g256(n), 256, 0, s>>>(w.skeys, w.flag, w.pos, n, kbits, w.head, orows, ocols, cap);
+    SGIPC_LAUNCH_CHECK();
+    if (cap > 0) {
+        k_segment_sum<<<g256(cap), 256, 0, s>>>(tvals, w.sidx, w.head, w.pos, n, ovals, cap);
+        SGIPC_LAUNCH_CHECK();
+    }
+    int h[2] = {0, 0};
+    SGIPC_HIP(hipMemcpyAsync(&h[0], w.pos + (n - 1), sizeof(int), hipMemcpyDeviceToHost, s));
+    SGIPC_HIP(hipMemcpyAsync(&h[1], w.ints, sizeof(int), hipMemcpyDeviceToHost, s));
+    SGIPC_HIP(hipStreamSynchronize(s));
+    SGIPC_CHECK(h[1] == 0, SGIPC_BAD_INDEX, "triplet with row > col, a negative index or an index beyond the bound");
+    *unique = h[0];
+    SGIPC_CHECK(h[0] <= capacity, SGIPC_CAPACITY, "unique block count exceeds the output capacity");
+    return h[0];
+}
+
 int Converter::index_bits(int bound)
 {
     int k = 1;
@@ -532,13 +594,7 @@ int Converter::index_bits(int bound)
 int Converter::max_index(hipStream_t s, const int* rows, const int* cols, long n)
 {
     scratch_int_.reserve(4);
-    SGIPC_HIP(hipMemsetAsync(scratch_int_.get(), 0, sizeof(int), s));
-    k_max_index<<<256, 256, 0, s>>>(rows, cols, n, scratch_int_.get());
-    SGIPC_LAUNCH_CHECK();
-    int m = 0;
-    SGIPC_HIP(hipMemcpyAsync(&m, scratch_int_.get(), sizeof(int), hipMemcpyDeviceToHost, s));
-    SGIPC_HIP(hipStreamSynchronize(s));
-    return m;
+    return convert_max_index(s, rows, cols, n, scratch_int_.get());
 }
 
 int Converter::convert(hipStream_t s, const double* tvals, const int* trows, const int* tcols, long n,
@@ -546,42 +602,27 @@ int Converter::convert(hipStream_t s, const double* tvals, const int* trows, con
 {
     if (n <= 0) return 0;
     SGIPC_CHECK(n < (1L << 31), SGIPC_CAPACITY, "too many triplets");
-    const int kbits = index_bits(index_bound);
     keys_.reserve(n);
     skeys_.reserve(n);
     idx_.reserve(n);
     sidx_.reserve(n);
     flag_.reserve(n);
     pos_.reserve(n);
-    head_.reserve((size_t)capacity + 1);
+    head_.reserve((size_t)(capacity < n ? capacity : n) + 1);
     scratch_int_.reserve(4);
-    SGIPC_HIP(hipMemsetAsync(scratch_int_.get(), 0, sizeof(int), s));
-    k_make_keys<<<g256(n), 256, 0, s>>>(trows, tcols, n, kbits, keys_.get(), idx_.get(), scratch_int_.get());
-    SGIPC_LAUNCH_CHECK();
-    size_t need = 0;
-    SGIPC_HIP(rocprim::radix_sort_pairs(nullptr, need, keys_.get(), skeys_.get(), idx_.get(), sidx_.get(), (size_t)n, 0,
-                                        2 * kbits, s));
-    size_t need2 = 0;
-    SGIPC_HIP(rocprim::inclusive_scan(nullptr, need2, flag_.get(), pos_.get(), (size_t)n, rocprim::plus<int>(), s));
-    temp_.reserve(need > need2 ? need : need2);
-    SGIPC_HIP(rocprim::radix_sort_pairs(temp_.get(), need, keys_.get(), skeys_.get(), idx_.get(), sidx_.get(),
-                                        (size_t)n, 0, 2 * kbits, s));
-    k_head_flags<<<g256(n), 256, 0, s>>>(skeys_.get(), n, flag_.get());
-    SGIPC_LAUNCH_CHECK();
-    SGIPC_HIP(rocprim::inclusive_scan(temp_.get(), need2, flag_.get(), pos_.get(), (size_t)n, rocprim::plus<int>(), s));
-    k_heads<<<g256(n), 256, 0, s>>>(skeys_.get(), flag_.get(), pos_.get(), n, kbits, head_.get(), orows, ocols,
-                                    capacity);
-    SGIPC_LAUNCH_CHECK();
-    k_segment_sum<<<g256(capacity), 256, 0, s>>>(tvals, sidx_.get(), head_.get(), pos_.get(), n, ovals, capacity);
-    SGIPC_LAUNCH_CHECK();
-    int h[2] = {0, 0};
-    SGIPC_HIP(hipMemcpyAsync(&h[0], pos_.get() + (n - 1), sizeof(int), hipMemcpyDeviceToHost, s));
-    SGIPC_HIP(hipMemcpyAsync(&h[1], scratch_int_.get(), sizeof(int), hipMemcpyDeviceToHost, s));
-    SGIPC_HIP(hipStreamSynchronize(s));
-    SGIPC_CHECK(h[1] == 0, SGIPC_BAD_INDEX, "triplet with row > col, a negative index or an index beyond the bound");
-    last_unique = h[0];
-    SGIPC_CHECK(h[0] <= capacity, SGIPC_CAPACITY, "unique block count exceeds the output capacity");
-    return h[0];
+    temp_.reserve(convert_temp_bytes(s, n));
+    ConvertScratch w;
+    w.keys = keys_.get();
+    w.skeys = skeys_.get();
+    w.idx = idx_.get();
+    w.sidx = sidx_.get();
+    w.flag = flag_.get();
+    w.pos = pos_.get();
+    w.head = head_.get();
+    w.ints = scratch_int_.get();
+    w.temp = temp_.get();
+    w.temp_bytes = temp_.n;
+    return convert_into(s, w, tvals, trows, tcols, n, index_bound, ovals, orows, ocols, capacity, &last_unique);
 }
 
 }  // namespace sgipc

@@ -21,6 +21,32 @@ void launch_quad_bending(hipStream_t s, int n_edges, const double* x, const doub
 void launch_mass_diag(hipStream_t s, int n, const double* mass, int hess_offset, double* tvals, int* trows,
                       int* tcols);
 
+// The device scratch one conversion of n triplets works in: sort keys and
+// their sorted copy, triplet indices and their sorted copy, head flags and
+// their inclusive scan, block starts (n + 1 entries), 4 ints, and the radix
+// sort / scan temporary storage.
+struct ConvertScratch {
+    uint64_t* keys = nullptr;
+    uint64_t* skeys = nullptr;
+    int* idx = nullptr;
+    int* sidx = nullptr;
+    int* flag = nullptr;
+    int* pos = nullptr;
+    int* head = nullptr;
+    int* ints = nullptr;
+    void* temp = nullptr;
+    size_t temp_bytes = 0;
+};
+// Temporary storage (bytes) the radix sort and scan of n triplets need.
+size_t convert_temp_bytes(hipStream_t s, long n);
+// Largest row/col index of n triplets (ints[0] is the scratch word).
+int convert_max_index(hipStream_t s, const int* rows, const int* cols, long n, int* ints);
+// Sort + reduce-by-key into the caller's scratch; writes at most `capacity`
+// blocks and stores the unique count in *unique (also when it throws
+// SGIPC_CAPACITY).  Allocates nothing.
+int convert_into(hipStream_t s, const ConvertScratch& w, const double* tvals, const int* trows, const int* tcols,
+                 long n, int index_bound, double* ovals, int* orows, int* ocols, int capacity, int* unique);
+
 // Sort + reduce-by-key of 3x3 triplets into unique upper BCOO.  Buffers are
 // kept between calls.
 class Converter {

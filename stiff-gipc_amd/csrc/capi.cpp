@@ -78,6 +78,7 @@ int sgipc_create(sgipc_ctx** out, int device)
         hipDeviceProp_t prop;
         SGIPC_HIP(hipGetDeviceProperties(&prop, device));
         ctx->c.num_cus = prop.multiProcessorCount;
+        noalloc_setup(device);
     });
     if (st != SGIPC_OK) {
         delete ctx;

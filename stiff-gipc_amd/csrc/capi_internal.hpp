@@ -129,6 +129,9 @@ struct SubsystemPrecond : Preconditioner<TV> {
     }
 };
 
+// resolves the allocation-free scratch array's address on `device` (current)
+void noalloc_setup(int device);
+
 // learned-level helpers (capi_mpnn.cpp)
 sgipc_mpnn_data* mpnn_data_new(sgipc_ctx* ctx);
 CoarseGraph& mpnn_data_graph(sgipc_mpnn_data* d);
