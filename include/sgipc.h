@@ -98,6 +98,41 @@ int sgipc_barrier_pt_gradient_hessian(sgipc_ctx* ctx, const double* x, const int
                                       double kappa, int hess_offset, double* grad, double* tvals,
                                       int* trows, int* tcols);
 
+/* Replaces lbvh_f/lbvh_e::Construct + SelfCollitionDetect with
+ * _checkPTintersection / _checkEEintersection (StiffGIPC/mlbvh.cu:535-1060,
+ * 1306-1632, 2084-2226; smooth = false): contact pairs of the surface within
+ * squared distance dhat_sq, found by point-face queries (surf_verts against
+ * faces, uint3 each) and edge-edge queries (edges, uint2 each) over linear
+ * BVHs.  pairs (int4) use the reference's encoding: (-p-1, q, -1, w) point-
+ * point, (-p-1, a, b, w) point-edge, (-p-1, t0, t1, t2) point-triangle,
+ * (a0, a1, b0, b1) edge-edge, with w = -obj-2 for point-point/point-edge
+ * pairs of two edges nearly parallel at rest (rest_x), else -1.  mat_index
+ * numbers each class in output order.  The reference's order depends on its
+ * atomics; here it is fixed: point-face pairs by (surface slot, face), then
+ * edge-edge pairs by (edge, edge).  counts (host, 5 ints) = _cpNum: [0] pairs,
+ * [2] point-point, [3] point-edge, [4] point-triangle + edge-edge.  body_id
+ * (pairs of one body other than -1 are skipped) and boundary_type (pairs
+ * whose four vertices are all >= 2 are skipped) may be NULL.  With too small a
+ * capacity: SGIPC_CAPACITY, nothing written, counts[0] = the pairs found. */
+int sgipc_contact_detect(sgipc_ctx* ctx, const double* x, const double* rest_x, int n_verts,
+                         const uint32_t* faces, int n_faces, const uint32_t* edges, int n_edges,
+                         const uint32_t* surf_verts, int n_surf, const int* body_id,
+                         const int* boundary_type, double dhat_sq, int* pairs, int* mat_index,
+                         int capacity, int* counts);
+
+/* Replaces _calBarrierGradientAndHessian for every pair type (StiffGIPC/
+ * GIPC.cu:3176-4984, RANK 2, NEWF: edge-edge, point-point, point-edge,
+ * point-triangle) and the contact lower-block swap (GIPC.cu:10344-10362):
+ * gradient added atomically into grad (may be NULL), Hessian blocks into the
+ * contact arena [PT+EE x 16 | PE x 9 | PP x 4] at mat_index * 16,
+ * counts[4] * 16 + mat_index * 9, counts[4] * 16 + counts[3] * 9 +
+ * mat_index * 4 (GIPC.cu:3414,4172,4728,4981).  pairs, mat_index and counts
+ * as written by sgipc_contact_detect. */
+int sgipc_barrier_gradient_hessian(sgipc_ctx* ctx, const double* x, const int* pairs,
+                                   const int* mat_index, int n_pairs, const int* counts,
+                                   double dhat_sq, double kappa, int hess_offset, double* grad,
+                                   double* tvals, int* trows, int* tcols);
+
 /* Replaces _calculate_triangle_fem_gradient_hessian (StiffGIPC/femEnergy.cu:
  * 2146-2215): Baraff-Witkin stretch + shear cloth triangles, gradient
  * (atomically added into grad when non-NULL) and projected Hessian as 6 upper

@@ -95,6 +95,15 @@ def lib():
         L.orc_quad_bending.argtypes = [ctypes.c_int, c_dbl_p, c_dbl_p, c_int_p, c_int_p, c_dbl_p,
                                        ctypes.c_double, ctypes.c_double, ctypes.c_int, c_dbl_p, c_dbl_p,
                                        c_int_p, c_int_p]
+        L.orc_contact_detect.argtypes = [ctypes.c_int, c_dbl_p, c_dbl_p, ctypes.c_int, c_int_p, ctypes.c_int,
+                                         c_int_p, ctypes.c_int, c_int_p, c_int_p, c_int_p, ctypes.c_double,
+                                         ctypes.c_int, c_int_p, c_int_p, c_int_p]
+        L.orc_contact_detect.restype = ctypes.c_int
+        L.orc_barrier.argtypes = [ctypes.c_int, c_dbl_p, c_int_p, c_int_p, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_double, ctypes.c_double, ctypes.c_int, c_dbl_p, c_dbl_p, c_int_p, c_int_p]
+        L.orc_barrier.restype = ctypes.c_int
+        L.orc_barrier_energy_pairs.argtypes = [ctypes.c_int, c_dbl_p, c_int_p, ctypes.c_double, ctypes.c_double]
+        L.orc_barrier_energy_pairs.restype = ctypes.c_double
         _LIB = L
     return _LIB
 
@@ -134,6 +143,62 @@ def barrier_pt(scene, hess_offset=0, with_grad=True):
                          _ptr(px, c_dbl_p), scene.dhat ** 2, scene.kappa, hess_offset,
                          _ptr(grad, c_dbl_p), _ptr(vals, c_dbl_p), _ptr(rows, c_int_p), _ptr(cols, c_int_p))
     return vals, rows, cols, grad
+
+
+def _i32(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.int32)
+
+
+def contact_detect(x, rest, faces, edges, surf, dhat_sq, body_id=None, btype=None, capacity=None):
+    """Contact pairs by brute force (contact_oracle.c): (pairs (m, 4) int32,
+    mat_index (m,), counts [total, 0, PP, PE, PT+EE])."""
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+    rest = np.ascontiguousarray(rest, dtype=np.float64).reshape(-1)
+    faces, edges, surf = _i32(faces).reshape(-1, 3), _i32(edges).reshape(-1, 2), _i32(surf).reshape(-1)
+    body_id, btype = _i32(body_id), _i32(btype)
+    cap = capacity if capacity is not None else 1 << 16
+    while True:
+        pairs = np.zeros((cap, 4), dtype=np.int32)
+        mi = np.zeros(cap, dtype=np.int32)
+        counts = np.zeros(5, dtype=np.int32)
+        st = lib().orc_contact_detect(x.size // 3, _ptr(x, c_dbl_p), _ptr(rest, c_dbl_p), faces.shape[0],
+                                      _ptr(faces, c_int_p), edges.shape[0], _ptr(edges, c_int_p), surf.size,
+                                      _ptr(surf, c_int_p), _ptr(body_id, c_int_p), _ptr(btype, c_int_p),
+                                      float(dhat_sq), cap, _ptr(pairs, c_int_p), _ptr(mi, c_int_p),
+                                      _ptr(counts, c_int_p))
+        if st == 0:
+            n = int(counts[0])
+            return pairs[:n].copy(), mi[:n].copy(), counts
+        if capacity is not None:
+            return None, None, counts
+        cap = int(counts[0])
+
+
+def barrier(x, pairs, mat_index, counts, dhat_sq, kappa, hess_offset=0, with_grad=True):
+    """Barrier gradient and Hessian of every pair type in the contact arena
+    layout [PT+EE x 16 | PE x 9 | PP x 4]."""
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+    pairs = _i32(pairs).reshape(-1, 4)
+    mat_index = _i32(mat_index)
+    n4, n3, n2 = int(counts[4]), int(counts[3]), int(counts[2])
+    nt = 16 * n4 + 9 * n3 + 4 * n2
+    vals = np.zeros((nt, 9))
+    rows = np.zeros(nt, dtype=np.int32)
+    cols = np.zeros(nt, dtype=np.int32)
+    grad = np.zeros(x.size) if with_grad else None
+    st = lib().orc_barrier(pairs.shape[0], _ptr(x, c_dbl_p), _ptr(pairs, c_int_p), _ptr(mat_index, c_int_p), n4, n3,
+                           float(dhat_sq), float(kappa), hess_offset, _ptr(grad, c_dbl_p), _ptr(vals, c_dbl_p),
+                           _ptr(rows, c_int_p), _ptr(cols, c_int_p))
+    if st != 0:
+        raise ValueError("smooth-encoded pair: not produced by the detection")
+    return vals, rows, cols, grad
+
+
+def barrier_energy(x, pairs, dhat_sq, kappa):
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+    pairs = _i32(pairs).reshape(-1, 4)
+    return lib().orc_barrier_energy_pairs(pairs.shape[0], _ptr(x, c_dbl_p), _ptr(pairs, c_int_p), float(dhat_sq),
+                                          float(kappa))
 
 
 def tri_bw(scene, hess_offset=0, with_grad=True):

@@ -18,6 +18,7 @@
 
 static int g_fault = 0;
 void orc_set_fault(int fault) { g_fault = fault; }
+int orc_get_fault(void) { return g_fault; }
 
 /* ------------------------------------------------------------------------- */
 /* small dense helpers                                                        */

@@ -62,6 +62,23 @@ int orc_barrier_pt(int n_pairs, const double* x, const int* pairs, const double*
 double orc_barrier_energy(double I5, double dhat_sq, double kappa);
 double orc_barrier_lambda0(double I5, double dhat_sq, double kappa);
 
+/* Contact pairs within squared distance dhat_sq (contact_oracle.c): the
+ * reference's point-face and edge-edge self queries (StiffGIPC/mlbvh.cu:
+ * 535-1060,1306-1632) by brute force, pairs in the reference's int4 encoding
+ * in a fixed order (point-face by (surface slot, face), then edge-edge by
+ * (edge, edge)), MatIndex per class; counts[0] total, counts[2..4] the PP,
+ * PE and PT+EE class sizes (like _cpNum).  Returns -1 when capacity is
+ * short (counts[0] holds the total then). */
+int orc_contact_detect(int n_verts, const double* x, const double* rest, int n_faces, const int* faces, int n_edges,
+                       const int* edges, int n_surf, const int* surf, const int* body_id, const int* btype,
+                       double dhat_sq, int capacity, int* pairs, int* mat_index, int* counts);
+/* Barrier gradient and Hessian of PP, PE, PT and EE pairs (GIPC.cu:
+ * 3176-4984, RANK 2, NEWF) into the contact arena [PT+EE x 16 | PE x 9 |
+ * PP x 4] by MatIndex; n4, n3 the PT+EE and PE class sizes. */
+int orc_barrier(int n_pairs, const double* x, const int* pairs, const int* mat_index, int n4, int n3, double dhat_sq,
+                double kappa, int hess_offset, double* grad, double* tvals, int* trows, int* tcols);
+double orc_barrier_energy_pairs(int n_pairs, const double* x, const int* pairs, double dhat_sq, double kappa);
+
 /* Baraff-Witkin cloth triangle (stretch + shear), 6 upper blocks per triangle
  * at tvals[6*t ...]; dm_inv2 is the 2x2 rest inverse (row-major), area the
  * rest area.  StiffGIPC/femEnergy.cu:2146-2215. */
@@ -174,6 +191,7 @@ int orc_pcg2(long nnz, const double* vals, const int* rows, const int* cols, int
  * coarse levels drop H^T on merged diagonals, 3 = barrier Hessian uses lambda0
  * with the wrong sign of the log term, 4 = cloth shear Hessian left unprojected. */
 void orc_set_fault(int fault);
+int orc_get_fault(void);
 
 #ifdef __cplusplus
 }

@@ -4,5 +4,5 @@ declared in ``include/sgipc.h``."""
 from . import learned, scenes  # noqa: F401
 from ._lib import LIB_PATH, SgipcError, exported_symbols  # noqa: F401
 from .linear_system import (FP32_IR, FP32_MAS, FP64, STOP_REF_RZ, STOP_TRUE_RESIDUAL, Context,  # noqa: F401
-                            Converter, DevArray, GlobalLinearSystem, LinearSystem, MASPreconditioner, PCGSolver,
+                            ContactDetector, Converter, DevArray, GlobalLinearSystem, LinearSystem, MASPreconditioner, PCGSolver,
                             SolveResult, SpMat, Spmv, pcg_opts, upload_scene)

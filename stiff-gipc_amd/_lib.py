@@ -85,6 +85,8 @@ _SIGS = {
     "sgipc_mass_diagonal": (ci, [vp, vp, ci, ci, vp, vp, vp]),
     "sgipc_convert_srbk": (ci, [vp, vp, vp, vp, ci, vp, vp, vp, ci, ctypes.POINTER(ci)]),
     "sgipc_convert_srbk_noalloc": (ci, [vp, vp, vp, vp, ci, vp, vp, vp, ci, ctypes.POINTER(ci)]),
+    "sgipc_contact_detect": (ci, [vp, vp, vp, ci, vp, ci, vp, ci, vp, ci, vp, vp, cd, vp, vp, ci, vp]),
+    "sgipc_barrier_gradient_hessian": (ci, [vp, vp, vp, vp, ci, vp, cd, cd, ci, vp, vp, vp, vp]),
     "sgipc_spmv_srbk": (ci, [vp, cd, vp, vp, vp, ci, vp, cd, vp, ci]),
     "sgipc_spmv_srbk_f32": (ci, [vp, cf, vp, vp, vp, ci, vp, cf, vp, ci]),
     "sgipc_convert_f64_to_f32": (ci, [vp, vp, vp, csz]),

@@ -95,6 +95,7 @@ int sgipc_destroy(sgipc_ctx* ctx)
         SGIPC_HIP(hipStreamSynchronize(ctx->c.stream));
         ctx->ws.reset();
         ctx->conv.reset();
+        ctx->contact.reset();
         if (ctx->c.own_stream && ctx->c.stream) SGIPC_HIP(hipStreamDestroy(ctx->c.stream));
     });
     delete ctx;

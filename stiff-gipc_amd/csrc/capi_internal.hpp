@@ -6,6 +6,7 @@
 
 #include "abd.hpp"
 #include "assembly.hpp"
+#include "contact.hpp"
 #include "mas.hpp"
 #include "pcg.hpp"
 #include "sgipc_internal.hpp"
@@ -21,6 +22,7 @@ struct sgipc_ctx {
     sgipc::Context c;
     std::unique_ptr<sgipc::SolveWorkspace> ws;
     std::unique_ptr<sgipc::Converter> conv;
+    std::unique_ptr<sgipc::ContactDetector> contact;
     ~sgipc_ctx();
 };
 

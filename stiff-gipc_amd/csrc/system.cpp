@@ -229,6 +229,53 @@ int sgipc_convert_srbk(sgipc_ctx* ctx, const double* tvals, const int* trows, co
     return st;
 }
 
+int sgipc_contact_detect(sgipc_ctx* ctx, const double* x, const double* rest_x, int n_verts, const uint32_t* faces,
+                         int n_faces, const uint32_t* edges, int n_edges, const uint32_t* surf_verts, int n_surf,
+                         const int* body_id, const int* boundary_type, double dhat_sq, int* pairs, int* mat_index,
+                         int capacity, int* counts)
+{
+    if (!ctx || !counts || n_verts < 0 || n_faces < 0 || n_edges < 0 || n_surf < 0 || capacity < 0)
+        return SGIPC_BAD_ARG;
+    for (int i = 0; i < 5; ++i) counts[i] = 0;
+    if (n_verts == 0 || (n_faces == 0 && n_edges < 2)) return SGIPC_EMPTY;
+    if (!x || !(dhat_sq > 0) || (n_faces > 0 && (!faces || (n_surf > 0 && !surf_verts))) ||
+        (n_edges > 0 && (!edges || !rest_x)) || (capacity > 0 && (!pairs || !mat_index)))
+        return SGIPC_BAD_ARG;
+    int total = 0;
+    const int st = run(ctx, [&] {
+        if (!ctx->contact) ctx->contact = std::make_unique<ContactDetector>();
+        ContactCounts c = ctx->contact->detect(ctx->c.stream, x, rest_x, n_verts, faces, n_faces, edges, n_edges,
+                                               surf_verts, n_surf, body_id, boundary_type, dhat_sq, pairs, mat_index,
+                                               capacity);
+        for (int i = 0; i < 5; ++i) counts[i] = c.c[i];
+    });
+    // a capacity failure reports the pair count the output needs
+    if (st == SGIPC_CAPACITY && ctx->contact) total = ctx->contact->last_total;
+    if (st == SGIPC_CAPACITY) counts[0] = total;
+    return st;
+}
+
+int sgipc_barrier_gradient_hessian(sgipc_ctx* ctx, const double* x, const int* pairs, const int* mat_index,
+                                   int n_pairs, const int* counts, double dhat_sq, double kappa, int hess_offset,
+                                   double* grad, double* tvals, int* trows, int* tcols)
+{
+    if (!ctx || n_pairs < 0 || !counts) return SGIPC_BAD_ARG;
+    if (n_pairs == 0) return SGIPC_EMPTY;
+    if (!x || !pairs || !mat_index || !tvals || !trows || !tcols || !(dhat_sq > 0)) return SGIPC_BAD_ARG;
+    if (counts[2] + counts[3] + counts[4] != n_pairs) return SGIPC_BAD_ARG;
+    return run(ctx, [&] {
+        DevBuf<int> bad;
+        bad.reserve(1);
+        launch_fill<int>(ctx->c.stream, bad.get(), 0, 1);
+        launch_barrier_pairs(ctx->c.stream, n_pairs, x, pairs, mat_index, counts[4], counts[3], dhat_sq, kappa,
+                             hess_offset, grad, tvals, trows, tcols, bad.get());
+        int h = 0;
+        SGIPC_HIP(hipMemcpyAsync(&h, bad.get(), sizeof(int), hipMemcpyDeviceToHost, ctx->c.stream));
+        SGIPC_HIP(hipStreamSynchronize(ctx->c.stream));
+        SGIPC_CHECK(h == 0, SGIPC_BAD_ARG, "a pair in the smooth (mollified) encoding, which detection never writes");
+    });
+}
+
 int sgipc_mas_create(sgipc_ctx* ctx, sgipc_mas** out, int bank_size);
 int sgipc_mas_destroy(sgipc_mas* mas);
 

@@ -23,9 +23,9 @@ import numpy as np
 
 from . import _lib
 from ._lib import (FP32_IR, FP32_MAS, FP64, PRECISIONS, STOP_REF_RZ, STOP_TRUE_RESIDUAL, Context, DevArray,
-                   PcgOpts, PcgStats, SgipcError, SolveTiming, SceneDesc, check, lib)
+                   PcgOpts, PcgStats, SGIPC_CAPACITY, SgipcError, SolveTiming, SceneDesc, check, lib)
 
-__all__ = ["Context", "DevArray", "Spmv", "SpMat", "Converter", "MASPreconditioner", "PCGSolver", "GlobalLinearSystem",
+__all__ = ["Context", "DevArray", "Spmv", "SpMat", "Converter", "ContactDetector", "MASPreconditioner", "PCGSolver", "GlobalLinearSystem",
            "LinearSystem", "SolveResult", "SgipcError", "FP64", "FP32_MAS", "FP32_IR", "STOP_REF_RZ",
            "STOP_TRUE_RESIDUAL", "pcg_opts", "upload_scene"]
 
@@ -99,6 +99,46 @@ class Converter:
         check(lib().sgipc_convert_srbk(self.ctx.h, _p(tvals), _p(trows), _p(tcols), n_triplets, _p(ov), _p(orr),
                                        _p(oc), cap, ctypes.byref(nnz)), self.ctx.h)
         return ov, orr, oc, nnz.value
+
+
+class ContactDetector:
+    """lbvh_f / lbvh_e self queries (mlbvh.cu): contact pairs of a surface."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+
+    def detect(self, x, rest, faces, n_faces, edges, n_edges, surf, n_surf, n_verts, dhat_sq, body_id=None,
+               btype=None, capacity=None):
+        """Returns (pairs (device int4), mat_index (device int), counts[5] host)."""
+        cap = capacity if capacity is not None else max(64, 8 * (n_surf + n_edges))
+        while True:
+            pairs = self.ctx.empty((max(cap, 1), 4), np.int32)
+            mi = self.ctx.empty(max(cap, 1), np.int32)
+            counts = (ctypes.c_int * 5)()
+            st = lib().sgipc_contact_detect(self.ctx.h, _p(x), _p(rest), n_verts, _p(faces), n_faces, _p(edges),
+                                            n_edges, _p(surf), n_surf, _p(body_id), _p(btype), float(dhat_sq),
+                                            _p(pairs), _p(mi), cap, counts)
+            if st == SGIPC_CAPACITY and capacity is None:
+                cap = counts[0]
+                continue
+            check(st, self.ctx.h)
+            return pairs, mi, np.array(list(counts), dtype=np.int32)
+
+    def barrier(self, x, pairs, mat_index, counts, dhat_sq, kappa, n_verts, hess_offset=0, want_grad=True):
+        """Barrier blocks of every pair type; returns (vals, rows, cols, grad) device arrays."""
+        n = int(counts[2] + counts[3] + counts[4])
+        nt = 16 * int(counts[4]) + 9 * int(counts[3]) + 4 * int(counts[2])
+        tv = self.ctx.empty((max(nt, 1), 9), np.float64)
+        tr = self.ctx.empty(max(nt, 1), np.int32)
+        tc = self.ctx.empty(max(nt, 1), np.int32)
+        g = None
+        if want_grad:
+            g = self.ctx.to_device(np.zeros(3 * n_verts))
+        c = (ctypes.c_int * 5)(*[int(v) for v in counts])
+        check(lib().sgipc_barrier_gradient_hessian(self.ctx.h, _p(x), _p(pairs), _p(mat_index), n, c, float(dhat_sq),
+                                                   float(kappa), hess_offset, _p(g), _p(tv), _p(tr), _p(tc)),
+              self.ctx.h)
+        return tv, tr, tc, g
 
 
 class MASPreconditioner:

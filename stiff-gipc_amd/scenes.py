@@ -460,6 +460,71 @@ def cloth_grid_scene(name: str, N: int, *, spacing=1e-3, jitter=0.1, E=1e6, nu=0
                       kappa=kappa, rhs=rhs, seeds=dict(jitter=seed_jitter, rhs=seed_rhs, pairs=seed_pairs))
 
 
+@dataclasses.dataclass
+class ContactGeometry:
+    """A surface for contact detection: positions, rest positions, triangles,
+    unique edges and surface vertices (all vertices of a sheet), with the
+    barrier parameters (dhat is a distance; the reference's dHat = dhat^2)."""
+
+    name: str
+    x: np.ndarray          # (n, 3)
+    rest: np.ndarray       # (n, 3)
+    faces: np.ndarray      # (m, 3) int32
+    edges: np.ndarray      # (k, 2) int32
+    surf: np.ndarray       # (s,) int32
+    dhat: float
+    kappa: float
+    body_id: np.ndarray | None = None
+
+    @property
+    def n(self) -> int:
+        return self.x.shape[0]
+
+
+def _sheet(N, spacing, rng, angle, shift, z, zjit, xyjit):
+    ii, jj = np.meshgrid(np.arange(N), np.arange(N), indexing="ij")
+    ii, jj = ii.ravel(), jj.ravel()
+    order = np.argsort(morton2(ii, jj), kind="stable")
+    relabel = np.empty_like(order)
+    relabel[order] = np.arange(order.size)
+    lat = np.stack([ii[order], jj[order]], axis=1).astype(np.float64) * spacing
+    lat -= lat.mean(axis=0)
+    c, s_ = np.cos(angle), np.sin(angle)
+    rest = np.zeros((N * N, 3))
+    rest[:, 0] = c * lat[:, 0] - s_ * lat[:, 1] + shift[0]
+    rest[:, 1] = s_ * lat[:, 0] + c * lat[:, 1] + shift[1]
+    rest[:, 2] = z
+    x = rest.copy()
+    x[:, :2] += rng.uniform(-xyjit, xyjit, size=(N * N, 2))
+    x[:, 2] += rng.uniform(-zjit, zjit, size=N * N)
+    qi, qj = np.meshgrid(np.arange(N - 1), np.arange(N - 1), indexing="ij")
+    qi, qj = qi.ravel(), qj.ravel()
+    vid = lambda i, j: relabel[i * N + j]
+    v00, v10, v01, v11 = vid(qi, qj), vid(qi + 1, qj), vid(qi, qj + 1), vid(qi + 1, qj + 1)
+    tris = np.concatenate([np.stack([v00, v10, v11], axis=1), np.stack([v00, v11, v01], axis=1)])
+    return x, rest, tris.astype(np.int32)
+
+
+def contact_sheets(N: int = 24, *, spacing=1e-3, dhat=4e-4, gap=2.5e-4, kappa=1e6, seed=5, angle=0.12,
+                   body_ids=False) -> ContactGeometry:
+    """Two N x N sheets, the second turned by `angle` and shifted, `gap` above
+    the first with seeded jitter: vertices over triangles, vertices near
+    edges and vertices, and crossing edges all come within dhat, so every
+    pair type occurs.  With body_ids each sheet is its own body (no
+    self-contact within a sheet)."""
+    rng = np.random.default_rng(seed)
+    xa, ra, ta = _sheet(N, spacing, rng, 0.0, (0.0, 0.0), 0.0, 0.35 * gap, 0.08 * spacing)
+    xb, rb, tb = _sheet(N, spacing, rng, angle, (0.31 * spacing, 0.17 * spacing), gap, 0.35 * gap, 0.08 * spacing)
+    n1 = N * N
+    x = np.concatenate([xa, xb])
+    rest = np.concatenate([ra, rb])
+    faces = np.concatenate([ta, tb + n1]).astype(np.int32)
+    edges = _tri_edges_csr(2 * n1, faces)[0]
+    body = np.repeat(np.arange(2, dtype=np.int32), n1) if body_ids else None
+    return ContactGeometry(name=f"sheets{N}", x=x, rest=rest, faces=faces, edges=edges,
+                           surf=np.arange(2 * n1, dtype=np.int32), dhat=dhat, kappa=kappa, body_id=body)
+
+
 def scene_cloth_tiny(N: int = 10, n_pairs: int = 16) -> ClothScene:
     """Small draped sheet with contacts, for dense checks."""
     return cloth_grid_scene(f"cloth_tiny{N}", N, n_pairs=n_pairs)
