@@ -482,6 +482,23 @@ typedef struct sgipc_scene_desc {
      * defaults are 31.4 and 10 (abd_system_parms.h:13-14) */
     double motor_speed;
     double motor_strength;
+    /* contact detection on every solve (sgipc_contact_detect, scenes without
+     * affine bodies): with detect_contacts = 1, pairs / pair_x / n_pairs are
+     * ignored; the pairs of the surface (faces, edges, surf_verts, with
+     * rest_x for the nearly-parallel edge test) within dhat_sq give the
+     * barrier blocks of every pair type (sgipc_barrier_gradient_hessian) and
+     * the MAS contact connectivity, like the reference's Newton step
+     * (GIPC.cu:10291-10304).  contact_capacity: pairs held (<= 0: 8 per
+     * surface vertex and edge; grown on demand). */
+    int detect_contacts;
+    int n_faces;
+    int n_edges;
+    int n_surf;
+    const uint32_t* faces;      /* 3 per surface triangle               */
+    const uint32_t* edges;      /* 2 per surface edge                   */
+    const uint32_t* surf_verts;
+    const int* body_id;         /* per vertex or NULL                   */
+    int contact_capacity;
 } sgipc_scene_desc;
 
 typedef struct sgipc_solve_timing {
@@ -506,6 +523,10 @@ int sgipc_system_solve(sgipc_system* sys, const double* b, double* x, double* gr
 int sgipc_system_bcoo(sgipc_system* sys, const double** vals, const int** rows, const int** cols,
                       int* nnz);
 sgipc_mas* sgipc_system_mas(sgipc_system* sys);
+/* The contact pairs of the last solve (device pointers owned by the system):
+ * the detected pairs and their MatIndex with detect_contacts, else the
+ * descriptor's pairs (mat_index NULL). */
+int sgipc_system_contacts(sgipc_system* sys, const int** pairs, const int** mat_index, int* n_pairs);
 
 /* ---------------------------------------------------------------------------
  * Learned coarse level.  No counterpart in the reference (it has no learned

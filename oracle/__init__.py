@@ -363,6 +363,17 @@ def point_gradient_to_system(scene, gp):
     return g
 
 
+def scene_pairs(scene, with_index=False):
+    """The scene's contact pairs: detected from its contact surface at the
+    current positions (contact_detect), else its seeded pairs."""
+    cs = getattr(scene, "contact_surface", None)
+    if cs is None:
+        return (scene.pairs, None, None) if with_index else scene.pairs
+    pr, mi, counts = contact_detect(scene.x, scene.rest, cs.faces, cs.edges, cs.surf, scene.dhat ** 2,
+                                    body_id=cs.body_id, btype=getattr(scene, "boundary_type", None))
+    return (pr, mi, counts) if with_index else pr
+
+
 def triplets(scene, with_grad=False):
     """All triplets in the reference's arena order (GIPC.cu:10291-10485):
     contacts, affine bodies, tets, bending hinges, cloth triangles, mass.  On
@@ -391,6 +402,9 @@ def triplets(scene, with_grad=False):
                                  _ptr(np.zeros(16 * scene.n_pairs, dtype=np.int32), c_int_p))
         bv, br, bc = abd_contact_map(scene, bv, br, bc)
         bg = None
+    elif getattr(scene, "contact_surface", None) is not None:
+        pr, mi, counts = scene_pairs(scene, with_index=True)
+        bv, br, bc, bg = barrier(scene.x, pr, mi, counts, scene.dhat ** 2, scene.kappa, with_grad=with_grad)
     else:
         bv, br, bc, bg = barrier_pt(scene, with_grad=with_grad)
     parts = [(bv, br, bc, bg)]
@@ -578,14 +592,14 @@ def solve_scene(scene, bank_size=32, tol=1e-5, stop_mode=1, mas_fp32=False, max_
     nb = getattr(scene, "n_bodies", 0)
     if nb:
         mas = Mas(scene, bank_size, collision_node_offset=scene.n_abd_points)
-        mas.build(vals, rows, cols, 4 * nb, scene.pairs)
+        mas.build(vals, rows, cols, 4 * nb, scene_pairs(scene))
         inv = abd_jacobi(vals, rows, cols, nb)
         t1 = time.perf_counter()
         x, it, res = pcg2(vals, rows, cols, scene.n_sys, scene.rhs, mas, 4 * nb, inv, tol, stop_mode, max_iter,
                           mas_fp32)
     else:
         mas = Mas(scene, bank_size)
-        mas.build(vals, rows, cols, 0, scene.pairs)
+        mas.build(vals, rows, cols, 0, scene_pairs(scene))
         t1 = time.perf_counter()
         x, it, res = pcg(vals, rows, cols, scene.n, scene.rhs, mas, tol, stop_mode, max_iter, mas_fp32)
     t2 = time.perf_counter()

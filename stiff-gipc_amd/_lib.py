@@ -51,7 +51,9 @@ class SceneDesc(ctypes.Structure):
                 ("n_hinges", ci), ("k_bend", cd), ("rest_x", vp), ("hinge_edges", vp), ("hinge_adj", vp),
                 ("n_bodies", ci), ("n_abd_points", ci), ("kappa_abd", cd), ("q", vp), ("q_tilde", vp),
                 ("mass10", vp), ("body_volume", vp), ("body_fixed", vp), ("point_body", vp), ("x_bar", vp),
-                ("boundary_type", vp), ("motor_speed", cd), ("motor_strength", cd)]
+                ("boundary_type", vp), ("motor_speed", cd), ("motor_strength", cd),
+                ("detect_contacts", ci), ("n_faces", ci), ("n_edges", ci), ("n_surf", ci), ("faces", vp), ("edges", vp),
+                ("surf_verts", vp), ("body_id", vp), ("contact_capacity", ci)]
 
 
 class SolveTiming(ctypes.Structure):
@@ -116,6 +118,7 @@ _SIGS = {
                                 ctypes.POINTER(SolveTiming)]),
     "sgipc_system_bcoo": (ci, [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(ci)]),
     "sgipc_system_mas": (vp, [vp]),
+    "sgipc_system_contacts": (ci, [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(ci)]),
     "sgipc_mas_set_coarse_mode": (ci, [vp, ci]),
     "sgipc_mas_set_smoothing_weight": (ci, [vp, cd]),
     "sgipc_mas_set_level0_half": (ci, [vp, ci]),

@@ -52,6 +52,12 @@ struct sgipc_system {
     DevBuf<double> gpt;     // point-space contact gradient (body points, then FEM points)
     ContactMapper cmap;
     AbdJacobi abdj;
+    // contact detection on every solve: detected pairs, their MatIndex and
+    // the capacity held
+    ContactDetector det;
+    DevBuf<int> dpairs, dmat, dbad;
+    int dcap = 0;
+    int n_detected = 0;  // pairs of the last solve
     SolveWorkspace* ws = nullptr;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     // learned coarse level: network, mode (0 off, 1 on, 2 automatic), and
@@ -297,6 +303,11 @@ int sgipc_system_create(sgipc_ctx* ctx, sgipc_system** out, const sgipc_scene_de
     if (d.n_abd_points > 0 && (!d.point_body || !d.x_bar || d.n_bodies == 0)) return SGIPC_BAD_ARG;
     if (d.n_bodies > 0 && d.n_pairs > 0 && !d.pair_x) return SGIPC_BAD_ARG;
     if (bank_size != 16 && bank_size != 32) return SGIPC_BAD_ARG;
+    if (d.detect_contacts) {
+        if (d.n_bodies > 0 || d.n_faces < 0 || d.n_edges < 0 || d.n_surf < 0 || !(d.dhat_sq > 0)) return SGIPC_BAD_ARG;
+        if ((d.n_faces > 0 && (!d.faces || !d.surf_verts)) || (d.n_edges > 0 && (!d.edges || !d.rest_x)))
+            return SGIPC_BAD_ARG;
+    }
     auto* sys = new (std::nothrow) sgipc_system();
     if (!sys) return SGIPC_HIP_ERROR;
     sys->ctx = ctx;
@@ -315,7 +326,13 @@ int sgipc_system_create(sgipc_ctx* ctx, sgipc_system** out, const sgipc_scene_de
                 launch_quad_bending_q(ctx->c.stream, d.n_hinges, d.rest_x, d.hinge_edges, d.hinge_adj,
                                       sys->hinge_q.get());
             }
-            long contact = 16L * d.n_pairs;
+            long contact = d.detect_contacts ? 0 : 16L * d.n_pairs;
+            if (d.detect_contacts) {
+                sys->dcap = d.contact_capacity > 0 ? d.contact_capacity : 8 * (d.n_surf + d.n_edges) + 64;
+                sys->dpairs.reserve(4L * sys->dcap);
+                sys->dmat.reserve(sys->dcap);
+                sys->dbad.reserve(1);
+            }
             if (d.n_bodies > 0) {
                 sys->cvals.reserve((size_t)(contact > 0 ? contact : 1) * 9);
                 sys->crows.reserve(contact > 0 ? contact : 1);
@@ -403,7 +420,46 @@ int system_solve(sgipc_system* sys, const double* b, double* x, double* grad, co
         const int off0 = 4 * nb;  // FEM block rows start after the bodies
         if (grad) launch_fill<double>(s, grad, 0.0, 3L * sys->n_sys);
         long off = 0;
-        if (nb == 0) {
+        const int* mas_pairs = d.pairs;
+        int mas_npairs = d.n_pairs;
+        if (d.detect_contacts) {
+            // the pairs of the current positions (grown once when short), then
+            // every pair type's barrier into the contact arena
+            ContactCounts cc;
+            for (int attempt = 0;; ++attempt) {
+                try {
+                    cc = sys->det.detect(s, d.x, d.rest_x, d.n_verts, d.faces, d.n_faces, d.edges, d.n_edges,
+                                         d.surf_verts, d.n_surf, d.body_id, d.boundary_type, d.dhat_sq,
+                                         sys->dpairs.get(), sys->dmat.get(), sys->dcap);
+                    break;
+                } catch (const Error& e) {
+                    if (e.code != SGIPC_CAPACITY || attempt > 0 || sys->det.last_total <= sys->dcap) throw;
+                    sys->dcap = sys->det.last_total + sys->det.last_total / 4 + 64;
+                    sys->dpairs.reserve(4L * sys->dcap);
+                    sys->dmat.reserve(sys->dcap);
+                }
+            }
+            const long ct = 16L * cc.c[4] + 9L * cc.c[3] + 4L * cc.c[2];
+            if (ct + sys->n_trip > sys->arena_cap) {
+                sys->arena_cap = ct + sys->n_trip;
+                SGIPC_CHECK(sys->arena_cap < (1L << 31), SGIPC_CAPACITY, "triplet count exceeds 2^31");
+                sys->tvals.reserve((size_t)sys->arena_cap * 9);
+                sys->trows.reserve(sys->arena_cap);
+                sys->tcols.reserve(sys->arena_cap);
+                sys->capacity = (int)sys->arena_cap;
+                sys->bvals.reserve((size_t)sys->capacity * 9);
+                sys->brows.reserve(sys->capacity);
+                sys->bcols.reserve(sys->capacity);
+            }
+            launch_fill<int>(s, sys->dbad.get(), 0, 1);
+            launch_barrier_pairs(s, cc.c[0], d.x, sys->dpairs.get(), sys->dmat.get(), cc.c[4], cc.c[3], d.dhat_sq,
+                                 d.kappa, 0, grad, sys->tvals.get(), sys->trows.get(), sys->tcols.get(),
+                                 sys->dbad.get());
+            off += ct;
+            mas_pairs = sys->dpairs.get();
+            mas_npairs = cc.c[0];
+            sys->n_detected = cc.c[0];
+        } else if (nb == 0) {
             launch_barrier_pt(s, d.n_pairs, d.x, d.pairs, d.pair_x, d.dhat_sq, d.kappa, 0, grad, sys->tvals.get(),
                               sys->trows.get(), sys->tcols.get());
             off += 16L * d.n_pairs;
@@ -470,8 +526,8 @@ int system_solve(sgipc_system* sys, const double* b, double* x, double* grad, co
         const bool learned = record == nullptr && use_learned(sys);
         sys->mas->m->keep_graph = sys->net != nullptr || record != nullptr;
         sys->mas->m->build_own_fine = false;  // the PCG matrix serves as coarse mode 2's fine operator
-        sys->mas->m->set_bcoo(sys->bvals.get(), sys->brows.get(), sys->bcols.get(), sys->nnz, off0, d.pairs,
-                              d.n_pairs, mprec);
+        sys->mas->m->set_bcoo(sys->bvals.get(), sys->brows.get(), sys->bcols.get(), sys->nnz, off0, mas_pairs,
+                              mas_npairs, mprec);
         if (nb > 0) sys->abdj.build(s, sys->bvals.get(), sys->brows.get(), sys->bcols.get(), sys->nnz, nb);
         SGIPC_HIP(hipEventRecord(sys->ev[3], s));
         // ---- PCG
@@ -557,7 +613,7 @@ int system_solve(sgipc_system* sys, const double* b, double* x, double* grad, co
             mm->net_ = nullptr;
             extra.flexible = false;
             used = false;
-            mm->set_bcoo(sys->bvals.get(), sys->brows.get(), sys->bcols.get(), sys->nnz, off0, d.pairs, d.n_pairs,
+            mm->set_bcoo(sys->bvals.get(), sys->brows.get(), sys->bcols.get(), sys->nnz, off0, mas_pairs, mas_npairs,
                          SGIPC_FP64);
             const int it32 = pst.iterations;
             status = run_pcg(o64);
@@ -684,5 +740,14 @@ int sgipc_system_bcoo(sgipc_system* sys, const double** vals, const int** rows, 
 }
 
 sgipc_mas* sgipc_system_mas(sgipc_system* sys) { return sys ? sys->mas : nullptr; }
+
+int sgipc_system_contacts(sgipc_system* sys, const int** pairs, const int** mat_index, int* n_pairs)
+{
+    if (!sys || !pairs || !mat_index || !n_pairs) return SGIPC_BAD_ARG;
+    *pairs = sys->d.detect_contacts ? sys->dpairs.get() : sys->d.pairs;
+    *mat_index = sys->d.detect_contacts ? sys->dmat.get() : nullptr;
+    *n_pairs = sys->d.detect_contacts ? sys->n_detected : sys->d.n_pairs;
+    return SGIPC_OK;
+}
 
 }  // extern "C"

@@ -343,6 +343,17 @@ def upload_scene(ctx: Context, scene):
     desc.nbr_start = put("nbr_start", scene.nbr_start, np.uint32)
     desc.nbr_num = put("nbr_num", scene.nbr_num, np.uint32)
     desc.nbr_list = put("nbr_list", scene.nbr_list if scene.nbr_list.size else np.zeros(1, np.uint32), np.uint32)
+    surf = getattr(scene, "contact_surface", None)
+    if surf is not None:
+        # contacts detected on every solve from the surface (faces, edges, surface vertices)
+        desc.detect_contacts = 1
+        desc.n_faces, desc.n_edges, desc.n_surf = surf.faces.shape[0], surf.edges.shape[0], surf.surf.size
+        desc.faces = put("c_faces", surf.faces.reshape(-1), np.uint32)
+        desc.edges = put("c_edges", surf.edges.reshape(-1), np.uint32)
+        desc.surf_verts = put("c_surf", surf.surf, np.uint32)
+        desc.body_id = put("c_body", surf.body_id, np.int32) if surf.body_id is not None else None
+        if not desc.rest_x:
+            desc.rest_x = put("rest_x", scene.rest.reshape(-1), np.float64)
     return desc, keep
 
 
@@ -389,6 +400,21 @@ class GlobalLinearSystem:
 
     def mas_handle(self):
         return lib().sgipc_system_mas(self.h)
+
+    def contacts(self):
+        """Host copies of the last solve's contact pairs (n, 4) and MatIndex
+        (None unless the scene detects its contacts)."""
+        p, m, n = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_int()
+        check(lib().sgipc_system_contacts(self.h, ctypes.byref(p), ctypes.byref(m), ctypes.byref(n)), self.ctx.h)
+        pairs = np.empty((n.value, 4), np.int32)
+        if n.value:
+            check(lib().sgipc_memcpy_dtoh(self.ctx.h, pairs.ctypes.data, p, pairs.nbytes), self.ctx.h)
+        mi = None
+        if m.value:
+            mi = np.empty(n.value, np.int32)
+            if n.value:
+                check(lib().sgipc_memcpy_dtoh(self.ctx.h, mi.ctypes.data, m, mi.nbytes), self.ctx.h)
+        return pairs, mi
 
     def close(self):
         if self.h and self.ctx.h:

@@ -143,6 +143,8 @@ class ClothScene:
     rhs: np.ndarray
     seeds: dict
     boundary_type: np.ndarray | None = None  # (n,) int32, nonzero = Dirichlet vertex
+    # surface whose contacts are detected on every solve (pairs then unused)
+    contact_surface: "ContactGeometry | None" = None
 
     @property
     def n(self) -> int:
@@ -523,6 +525,39 @@ def contact_sheets(N: int = 24, *, spacing=1e-3, dhat=4e-4, gap=2.5e-4, kappa=1e
     body = np.repeat(np.arange(2, dtype=np.int32), n1) if body_ids else None
     return ContactGeometry(name=f"sheets{N}", x=x, rest=rest, faces=faces, edges=edges,
                            surf=np.arange(2 * n1, dtype=np.int32), dhat=dhat, kappa=kappa, body_id=body)
+
+
+def cloth_contact_scene(N: int = 24, *, gap=2.5e-4, dhat=4e-4, kappa=1e6, angle=0.12, seed=9) -> ClothScene:
+    """Two draped N x N cloth sheets, the second turned by `angle`, shifted
+    and `gap` above the first (same drape), with seeded jitter: their contacts
+    (every pair type) are detected on each solve from the contact surface
+    (all triangles, all edges, all vertices)."""
+    a = cloth_grid_scene(f"sheetA{N}", N, n_pairs=0, seed_jitter=seed, seed_rhs=seed + 1)
+    b = cloth_grid_scene(f"sheetB{N}", N, n_pairs=0, seed_jitter=seed + 2, seed_rhs=seed + 3)
+    n1 = a.n
+    rng = np.random.default_rng(seed + 4)
+    side = (N - 1) * 1e-3
+    c, s_ = np.cos(angle), np.sin(angle)
+    xb = b.x.copy()
+    xb[:, 0], xb[:, 1] = c * b.x[:, 0] - s_ * b.x[:, 1] + 0.31e-3, s_ * b.x[:, 0] + c * b.x[:, 1] + 0.17e-3
+    xb[:, 2] = -(xb[:, 0] ** 2 + xb[:, 1] ** 2) / (2.0 * side) + gap + rng.uniform(-0.35 * gap, 0.35 * gap, n1)
+    adj_b = np.where(b.adj >= 0, b.adj + n1, -1).astype(np.int32)
+    tris = np.concatenate([a.tris, b.tris + n1]).astype(np.int32)
+    edges = np.concatenate([a.edges, b.edges + n1]).astype(np.int32)
+    x = np.concatenate([a.x, xb])
+    rest = np.concatenate([a.rest, b.rest])
+    surf = ContactGeometry(name=f"cloth_contact{N}", x=x, rest=rest, faces=tris, edges=edges,
+                           surf=np.arange(2 * n1, dtype=np.int32), dhat=dhat, kappa=kappa)
+    return ClothScene(name=f"cloth_contact{N}", x=x, rest=rest, tris=tris,
+                      dm_inv2=np.concatenate([a.dm_inv2, b.dm_inv2]), area=np.concatenate([a.area, b.area]),
+                      k_stretch=a.k_stretch, k_shear=a.k_shear, strain_rate=a.strain_rate, edges=edges,
+                      adj=np.concatenate([a.adj, adj_b]), k_bend=a.k_bend, mass=np.concatenate([a.mass, b.mass]),
+                      dt=a.dt, nbr_start=np.concatenate([a.nbr_start, b.nbr_start + a.nbr_list.size]).astype(np.uint32),
+                      nbr_num=np.concatenate([a.nbr_num, b.nbr_num]).astype(np.uint32),
+                      nbr_list=np.concatenate([a.nbr_list, b.nbr_list + n1]).astype(np.uint32),
+                      pairs=np.zeros((0, 4), np.int32), pair_x=None, dhat=dhat, kappa=kappa,
+                      rhs=np.random.default_rng(seed + 5).standard_normal(6 * n1),
+                      seeds=dict(jitter=seed, pairs=None), contact_surface=surf)
 
 
 def scene_cloth_tiny(N: int = 10, n_pairs: int = 16) -> ClothScene:

@@ -101,3 +101,64 @@ def test_barrier_matches_oracle(pkg, orc, gpu_ctx, N):
     assert float(err.max()) <= 1e-9, float(err.max())
     gg = g.download()
     assert np.linalg.norm(gg - og) <= 1e-10 * np.linalg.norm(og)
+
+
+@pytest.mark.parametrize("N", [12, 24])
+def test_system_with_detected_contacts_matches_oracle(pkg, orc, N):
+    """Two draped cloth sheets whose contacts are detected on every solve:
+    the system's pairs equal the oracle's detection, the assembled BCOO equals
+    the oracle's (indices exact, blocks within 1e-10 of their row's largest),
+    12 fixed FP64 PCG iterations agree to 1e-8 and the solve reaches 1e-5.
+    kappa = 100 keeps the sheets' condition number near 1e8 (at kappa = 1e6
+    the barrier blocks push it past 5e9, where rounding differences grow
+    along the PCG trajectory; that case is checked without the trajectory
+    below)."""
+    s = pkg.scenes.cloth_contact_scene(N, kappa=100.0)
+    ref = orc.assemble(s)
+    pr, mi, counts = orc.scene_pairs(s, with_index=True)
+    ls = pkg.LinearSystem(s, bank_size=32)
+    try:
+        fixed = ls.solve(tol=1e-30, max_iter=12, precision="fp64")
+        gp, gm = ls.sys.contacts()
+        np.testing.assert_array_equal(gp, pr)
+        np.testing.assert_array_equal(gm, mi)
+        gv, gr, gc = ls.sys.bcoo()
+        ov, orr, oc = ref
+        np.testing.assert_array_equal(gr, orr)
+        np.testing.assert_array_equal(gc, oc)
+        rowmax = np.zeros(int(orr.max()) + 1)
+        np.maximum.at(rowmax, orr, np.abs(ov).max(1))
+        assert float((np.abs(gv - ov).max(1) / rowmax[orr]).max()) <= 1e-10
+        xr = orc.solve_scene(s, 32, tol=1e-30, max_iter=12).x
+        e = np.linalg.norm(fixed.x - xr) / np.linalg.norm(xr)
+        res = ls.solve(tol=1e-5, precision="fp64")
+        print(f"cloth_contact{N}: {pr.shape[0]} pairs (PP {counts[2]}, PE {counts[3]}, PT+EE {counts[4]}), "
+              f"fixed-12 rel diff {e:.2e}, solve {res.iterations} it res {res.true_rel_residual:.2e}")
+        assert e <= 1e-8
+        assert res.status == 0 and res.true_rel_residual <= 1e-5
+    finally:
+        ls.close()
+
+
+def test_system_with_stiff_detected_contacts(pkg, orc):
+    """kappa = 1e6 (condition number above 5e9): the detected pairs and the
+    assembled BCOO equal the oracle's; the two-grid FP64 solve reaches 1e-5,
+    checked with the oracle's SpMV on the oracle's matrix."""
+    s = pkg.scenes.cloth_contact_scene(24)
+    ov, orr, oc = orc.assemble(s)
+    pr, mi, _ = orc.scene_pairs(s, with_index=True)
+    ls = pkg.LinearSystem(s, bank_size=32, coarse_mode=2)
+    try:
+        res = ls.solve(tol=1e-5, precision="fp64", max_iter=20000)
+        gp, gm = ls.sys.contacts()
+        np.testing.assert_array_equal(gp, pr)
+        np.testing.assert_array_equal(gm, mi)
+        gv, gr, gc = ls.sys.bcoo()
+        np.testing.assert_array_equal(gr, orr)
+        np.testing.assert_array_equal(gc, oc)
+        ax = orc.spmv(1.0, ov, orr, oc, res.x, 0.0, np.zeros(3 * s.n), s.n)
+        r = np.linalg.norm(s.rhs - ax) / np.linalg.norm(s.rhs)
+        print(f"cloth_contact24 kappa 1e6: {pr.shape[0]} pairs, mode {res.coarse_mode} {res.iterations} it, res {r:.2e}")
+        assert res.status == 0 and r <= 1.01e-5
+    finally:
+        ls.close()
