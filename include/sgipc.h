@@ -133,6 +133,47 @@ int sgipc_barrier_gradient_hessian(sgipc_ctx* ctx, const double* x, const int* p
                                    double dhat_sq, double kappa, int hess_offset, double* grad,
                                    double* tvals, int* trows, int* tcols);
 
+/* Replaces _calFrictionLastH_DistAndTan (StiffGIPC/GIPC.cu:8431-8535, RANK 2)
+ * with the closest-point and tangent-basis helpers of FrictionUtils.cuh:
+ * for each pair at the step-start positions x_t, the normal force lambda, the
+ * closest-point coordinates dist_coord (2 per pair: PT betas, EE gammas, PE
+ * eta, PP zeros) and the tangent basis tan_basis (6 per pair: the two basis
+ * vectors, 3 doubles each). */
+int sgipc_friction_prepare(sgipc_ctx* ctx, const double* x_t, const int* pairs, int n_pairs,
+                           double dhat_sq, double kappa, double* lambda, double* dist_coord,
+                           double* tan_basis);
+
+/* Replaces _calFrictionGradient and _calFrictionHessian (StiffGIPC/GIPC.cu:
+ * 5130-5313,1482-1845; C1 static-friction clamp, FrictionUtils.cuh:330-500):
+ * lagged friction of the step-start pairs (from sgipc_friction_prepare) at x
+ * against x_t; eps_sq = the reference's fDhat * dt^2, mu = frictionRate.
+ * Gradient added atomically into grad (may be NULL); Hessian blocks
+ * [PT+EE x 16 | PE x 9 | PP x 4] by mat_index with counts of those pairs, the
+ * lower blocks moved to (col, row) and zeroed. */
+int sgipc_friction_gradient_hessian(sgipc_ctx* ctx, const double* x, const double* x_t,
+                                    const int* pairs, const int* mat_index, int n_pairs,
+                                    const int* counts, const double* lambda,
+                                    const double* dist_coord, const double* tan_basis,
+                                    double eps_sq, double mu, int hess_offset, double* grad,
+                                    double* tvals, int* trows, int* tcols);
+
+/* Replaces _calFrictionLastH_gd (StiffGIPC/GIPC.cu:8405-8429): normal force of
+ * each ground vertex at x_t for the plane normal . x = offset (normal: device,
+ * 3 doubles, unit). */
+int sgipc_ground_friction_prepare(sgipc_ctx* ctx, const double* x_t, const int* verts, int n,
+                                  const double* normal, double offset, double dhat_sq,
+                                  double kappa, double* lambda);
+
+/* Replaces _calFrictionGradient_gd and _calFrictionHessian_gd (StiffGIPC/
+ * GIPC.cu:5090-5128,1400-1480; C0 clamp): one diagonal block per ground vertex
+ * at tvals[i]; the tangent projector is I - n n^T (the reference writes the xz
+ * entries of its y-up ground, the same values). */
+int sgipc_ground_friction_gradient_hessian(sgipc_ctx* ctx, const double* x, const double* x_t,
+                                           const int* verts, int n, const double* normal,
+                                           const double* lambda, double eps_sq, double mu,
+                                           int hess_offset, double* grad, double* tvals,
+                                           int* trows, int* tcols);
+
 /* Replaces _calculate_triangle_fem_gradient_hessian (StiffGIPC/femEnergy.cu:
  * 2146-2215): Baraff-Witkin stretch + shear cloth triangles, gradient
  * (atomically added into grad when non-NULL) and projected Hessian as 6 upper

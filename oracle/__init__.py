@@ -104,6 +104,21 @@ def lib():
         L.orc_barrier.restype = ctypes.c_int
         L.orc_barrier_energy_pairs.argtypes = [ctypes.c_int, c_dbl_p, c_int_p, ctypes.c_double, ctypes.c_double]
         L.orc_barrier_energy_pairs.restype = ctypes.c_double
+        L.orc_friction_prepare.argtypes = [ctypes.c_int, c_dbl_p, c_int_p, ctypes.c_double, ctypes.c_double, c_dbl_p,
+                                           c_dbl_p, c_dbl_p]
+        L.orc_friction.argtypes = [ctypes.c_int, c_dbl_p, c_dbl_p, c_int_p, c_int_p, ctypes.c_int, ctypes.c_int, c_dbl_p,
+                                   c_dbl_p, c_dbl_p, ctypes.c_double, ctypes.c_double, ctypes.c_int, c_dbl_p, c_dbl_p,
+                                   c_int_p, c_int_p]
+        L.orc_friction_energy.argtypes = [ctypes.c_int, c_dbl_p, c_dbl_p, c_int_p, c_dbl_p, c_dbl_p, c_dbl_p,
+                                          ctypes.c_double, ctypes.c_double]
+        L.orc_friction_energy.restype = ctypes.c_double
+        L.orc_ground_friction_prepare.argtypes = [ctypes.c_int, c_dbl_p, c_int_p, c_dbl_p, ctypes.c_double,
+                                                  ctypes.c_double, ctypes.c_double, c_dbl_p]
+        L.orc_ground_friction.argtypes = [ctypes.c_int, c_dbl_p, c_dbl_p, c_int_p, c_dbl_p, c_dbl_p, ctypes.c_double,
+                                          ctypes.c_double, ctypes.c_int, c_dbl_p, c_dbl_p, c_int_p, c_int_p]
+        L.orc_ground_friction_energy.argtypes = [ctypes.c_int, c_dbl_p, c_dbl_p, c_int_p, c_dbl_p, c_dbl_p,
+                                                 ctypes.c_double, ctypes.c_double]
+        L.orc_ground_friction_energy.restype = ctypes.c_double
         _LIB = L
     return _LIB
 
@@ -199,6 +214,68 @@ def barrier_energy(x, pairs, dhat_sq, kappa):
     pairs = _i32(pairs).reshape(-1, 4)
     return lib().orc_barrier_energy_pairs(pairs.shape[0], _ptr(x, c_dbl_p), _ptr(pairs, c_int_p), float(dhat_sq),
                                           float(kappa))
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+
+
+def friction_prepare(xt, pairs, dhat_sq, kappa):
+    """(lambda, coord (n, 2), basis (n, 6)) of the pairs at the step start xt."""
+    xt, pairs = _f64(xt), _i32(pairs).reshape(-1, 4)
+    n = pairs.shape[0]
+    lam, coord, basis = np.zeros(n), np.zeros((n, 2)), np.zeros((n, 6))
+    if lib().orc_friction_prepare(n, _ptr(xt, c_dbl_p), _ptr(pairs, c_int_p), float(dhat_sq), float(kappa),
+                                  _ptr(lam, c_dbl_p), _ptr(coord, c_dbl_p), _ptr(basis, c_dbl_p)) != 0:
+        raise ValueError("smooth-encoded pair")
+    return lam, coord, basis
+
+
+def friction(x, xt, pairs, mat_index, counts, lam, coord, basis, eps2, mu, hess_offset=0, with_grad=True):
+    x, xt, pairs = _f64(x), _f64(xt), _i32(pairs).reshape(-1, 4)
+    n4, n3, n2 = int(counts[4]), int(counts[3]), int(counts[2])
+    nt = 16 * n4 + 9 * n3 + 4 * n2
+    vals, rows, cols = np.zeros((nt, 9)), np.zeros(nt, np.int32), np.zeros(nt, np.int32)
+    grad = np.zeros(x.size) if with_grad else None
+    lam, coord, basis = _f64(lam), _f64(coord), _f64(basis)
+    lib().orc_friction(pairs.shape[0], _ptr(x, c_dbl_p), _ptr(xt, c_dbl_p), _ptr(pairs, c_int_p),
+                       _ptr(_i32(mat_index), c_int_p), n4, n3, _ptr(lam, c_dbl_p), _ptr(coord, c_dbl_p),
+                       _ptr(basis, c_dbl_p), float(eps2), float(mu), hess_offset, _ptr(grad, c_dbl_p),
+                       _ptr(vals, c_dbl_p), _ptr(rows, c_int_p), _ptr(cols, c_int_p))
+    return vals, rows, cols, grad
+
+
+def friction_energy(x, xt, pairs, lam, coord, basis, eps2, mu):
+    x, xt, pairs = _f64(x), _f64(xt), _i32(pairs).reshape(-1, 4)
+    lam, coord, basis = _f64(lam), _f64(coord), _f64(basis)
+    return lib().orc_friction_energy(pairs.shape[0], _ptr(x, c_dbl_p), _ptr(xt, c_dbl_p), _ptr(pairs, c_int_p),
+                                     _ptr(lam, c_dbl_p), _ptr(coord, c_dbl_p), _ptr(basis, c_dbl_p), float(eps2),
+                                     float(mu))
+
+
+def ground_friction_prepare(xt, verts, normal, offset, dhat_sq, kappa):
+    xt, verts, normal = _f64(xt), _i32(verts), _f64(normal)
+    lam = np.zeros(verts.size)
+    lib().orc_ground_friction_prepare(verts.size, _ptr(xt, c_dbl_p), _ptr(verts, c_int_p), _ptr(normal, c_dbl_p),
+                                      float(offset), float(dhat_sq), float(kappa), _ptr(lam, c_dbl_p))
+    return lam
+
+
+def ground_friction(x, xt, verts, normal, lam, eps2, mu, hess_offset=0, with_grad=True):
+    x, xt, verts, normal, lam = _f64(x), _f64(xt), _i32(verts), _f64(normal), _f64(lam)
+    n = verts.size
+    vals, rows, cols = np.zeros((n, 9)), np.zeros(n, np.int32), np.zeros(n, np.int32)
+    grad = np.zeros(x.size) if with_grad else None
+    lib().orc_ground_friction(n, _ptr(x, c_dbl_p), _ptr(xt, c_dbl_p), _ptr(verts, c_int_p), _ptr(normal, c_dbl_p),
+                              _ptr(lam, c_dbl_p), float(eps2), float(mu), hess_offset, _ptr(grad, c_dbl_p),
+                              _ptr(vals, c_dbl_p), _ptr(rows, c_int_p), _ptr(cols, c_int_p))
+    return vals, rows, cols, grad
+
+
+def ground_friction_energy(x, xt, verts, normal, lam, eps2, mu):
+    x, xt, verts, normal, lam = _f64(x), _f64(xt), _i32(verts), _f64(normal), _f64(lam)
+    return lib().orc_ground_friction_energy(verts.size, _ptr(x, c_dbl_p), _ptr(xt, c_dbl_p), _ptr(verts, c_int_p),
+                                            _ptr(normal, c_dbl_p), _ptr(lam, c_dbl_p), float(eps2), float(mu))
 
 
 def tri_bw(scene, hess_offset=0, with_grad=True):

@@ -79,6 +79,27 @@ int orc_barrier(int n_pairs, const double* x, const int* pairs, const int* mat_i
                 double kappa, int hess_offset, double* grad, double* tvals, int* trows, int* tcols);
 double orc_barrier_energy_pairs(int n_pairs, const double* x, const int* pairs, double dhat_sq, double kappa);
 
+/* Lagged IPC friction (friction_oracle.c; StiffGIPC/GIPC.cu:1400-1845,
+ * 5090-5313, 8405-8535; FrictionUtils.cuh): per-pair normal force, closest-
+ * point coordinates (2) and tangent basis (6: two 3-vectors) at the step
+ * start xt; gradient and Hessian blocks [PT+EE x 16 | PE x 9 | PP x 4] by
+ * MatIndex; the ground's per-vertex force and diagonal blocks; energies for
+ * the finite-difference pins. */
+int orc_friction_prepare(int n, const double* xt, const int* pairs, double dhat_sq, double kappa, double* lambda,
+                         double* coord, double* basis);
+int orc_friction(int n, const double* x, const double* xt, const int* pairs, const int* mat_index, int n4, int n3,
+                 const double* lambda, const double* coord, const double* basis, double eps2, double mu, int hess_offset,
+                 double* grad, double* tvals, int* trows, int* tcols);
+double orc_friction_energy(int n, const double* x, const double* xt, const int* pairs, const double* lambda,
+                           const double* coord, const double* basis, double eps2, double mu);
+int orc_ground_friction_prepare(int n, const double* xt, const int* verts, const double* normal, double offset,
+                                double dhat_sq, double kappa, double* lambda);
+int orc_ground_friction(int n, const double* x, const double* xt, const int* verts, const double* normal,
+                        const double* lambda, double eps2, double mu, int hess_offset, double* grad, double* tvals,
+                        int* trows, int* tcols);
+double orc_ground_friction_energy(int n, const double* x, const double* xt, const int* verts, const double* normal,
+                                  const double* lambda, double eps2, double mu);
+
 /* Baraff-Witkin cloth triangle (stretch + shear), 6 upper blocks per triangle
  * at tvals[6*t ...]; dm_inv2 is the 2x2 rest inverse (row-major), area the
  * rest area.  StiffGIPC/femEnergy.cu:2146-2215. */

@@ -89,6 +89,10 @@ _SIGS = {
     "sgipc_convert_srbk_noalloc": (ci, [vp, vp, vp, vp, ci, vp, vp, vp, ci, ctypes.POINTER(ci)]),
     "sgipc_contact_detect": (ci, [vp, vp, vp, ci, vp, ci, vp, ci, vp, ci, vp, vp, cd, vp, vp, ci, vp]),
     "sgipc_barrier_gradient_hessian": (ci, [vp, vp, vp, vp, ci, vp, cd, cd, ci, vp, vp, vp, vp]),
+    "sgipc_friction_prepare": (ci, [vp, vp, vp, ci, cd, cd, vp, vp, vp]),
+    "sgipc_friction_gradient_hessian": (ci, [vp, vp, vp, vp, vp, ci, vp, vp, vp, vp, cd, cd, ci, vp, vp, vp, vp]),
+    "sgipc_ground_friction_prepare": (ci, [vp, vp, vp, ci, vp, cd, cd, cd, vp]),
+    "sgipc_ground_friction_gradient_hessian": (ci, [vp, vp, vp, vp, ci, vp, vp, cd, cd, ci, vp, vp, vp, vp]),
     "sgipc_spmv_srbk": (ci, [vp, cd, vp, vp, vp, ci, vp, cd, vp, ci]),
     "sgipc_spmv_srbk_f32": (ci, [vp, cf, vp, vp, vp, ci, vp, cf, vp, ci]),
     "sgipc_convert_f64_to_f32": (ci, [vp, vp, vp, csz]),

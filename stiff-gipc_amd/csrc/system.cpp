@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "capi_internal.hpp"
+#include "friction.hpp"
 
 using namespace sgipc;
 
@@ -279,6 +280,75 @@ int sgipc_barrier_gradient_hessian(sgipc_ctx* ctx, const double* x, const int* p
         SGIPC_HIP(hipMemcpyAsync(&h, bad.get(), sizeof(int), hipMemcpyDeviceToHost, ctx->c.stream));
         SGIPC_HIP(hipStreamSynchronize(ctx->c.stream));
         SGIPC_CHECK(h == 0, SGIPC_BAD_ARG, "a pair in the smooth (mollified) encoding, which detection never writes");
+    });
+}
+
+int sgipc_friction_prepare(sgipc_ctx* ctx, const double* x_t, const int* pairs, int n_pairs, double dhat_sq,
+                           double kappa, double* lambda, double* dist_coord, double* tan_basis)
+{
+    if (!ctx || n_pairs < 0) return SGIPC_BAD_ARG;
+    if (n_pairs == 0) return SGIPC_EMPTY;
+    if (!x_t || !pairs || !lambda || !dist_coord || !tan_basis || !(dhat_sq > 0)) return SGIPC_BAD_ARG;
+    return run(ctx, [&] {
+        DevBuf<int> bad;
+        bad.reserve(1);
+        launch_fill<int>(ctx->c.stream, bad.get(), 0, 1);
+        launch_friction_prepare(ctx->c.stream, n_pairs, x_t, pairs, dhat_sq, kappa, lambda, dist_coord, tan_basis,
+                                bad.get());
+        int h = 0;
+        SGIPC_HIP(hipMemcpyAsync(&h, bad.get(), sizeof(int), hipMemcpyDeviceToHost, ctx->c.stream));
+        SGIPC_HIP(hipStreamSynchronize(ctx->c.stream));
+        SGIPC_CHECK(h == 0, SGIPC_BAD_ARG, "a pair in the smooth (mollified) encoding, which detection never writes");
+    });
+}
+
+int sgipc_friction_gradient_hessian(sgipc_ctx* ctx, const double* x, const double* x_t, const int* pairs,
+                                    const int* mat_index, int n_pairs, const int* counts, const double* lambda,
+                                    const double* dist_coord, const double* tan_basis, double eps_sq, double mu,
+                                    int hess_offset, double* grad, double* tvals, int* trows, int* tcols)
+{
+    if (!ctx || n_pairs < 0 || !counts) return SGIPC_BAD_ARG;
+    if (n_pairs == 0) return SGIPC_EMPTY;
+    if (!x || !x_t || !pairs || !mat_index || !lambda || !dist_coord || !tan_basis || !tvals || !trows || !tcols ||
+        !(eps_sq > 0))
+        return SGIPC_BAD_ARG;
+    if (counts[2] + counts[3] + counts[4] != n_pairs) return SGIPC_BAD_ARG;
+    return run(ctx, [&] {
+        DevBuf<int> bad;
+        bad.reserve(1);
+        launch_fill<int>(ctx->c.stream, bad.get(), 0, 1);
+        launch_friction(ctx->c.stream, n_pairs, x, x_t, pairs, mat_index, counts[4], counts[3], lambda, dist_coord,
+                        tan_basis, eps_sq, mu, hess_offset, grad, tvals, trows, tcols, bad.get());
+        int h = 0;
+        SGIPC_HIP(hipMemcpyAsync(&h, bad.get(), sizeof(int), hipMemcpyDeviceToHost, ctx->c.stream));
+        SGIPC_HIP(hipStreamSynchronize(ctx->c.stream));
+        SGIPC_CHECK(h == 0, SGIPC_BAD_ARG, "a pair in the smooth (mollified) encoding, which detection never writes");
+    });
+}
+
+int sgipc_ground_friction_prepare(sgipc_ctx* ctx, const double* x_t, const int* verts, int n, const double* normal,
+                                  double offset, double dhat_sq, double kappa, double* lambda)
+{
+    if (!ctx || n < 0) return SGIPC_BAD_ARG;
+    if (n == 0) return SGIPC_EMPTY;
+    if (!x_t || !verts || !normal || !lambda || !(dhat_sq > 0)) return SGIPC_BAD_ARG;
+    return run(ctx, [&] {
+        launch_ground_friction_prepare(ctx->c.stream, n, x_t, verts, normal, offset, dhat_sq, kappa, lambda);
+        SGIPC_HIP(hipStreamSynchronize(ctx->c.stream));
+    });
+}
+
+int sgipc_ground_friction_gradient_hessian(sgipc_ctx* ctx, const double* x, const double* x_t, const int* verts, int n,
+                                           const double* normal, const double* lambda, double eps_sq, double mu,
+                                           int hess_offset, double* grad, double* tvals, int* trows, int* tcols)
+{
+    if (!ctx || n < 0) return SGIPC_BAD_ARG;
+    if (n == 0) return SGIPC_EMPTY;
+    if (!x || !x_t || !verts || !normal || !lambda || !tvals || !trows || !tcols || !(eps_sq > 0)) return SGIPC_BAD_ARG;
+    return run(ctx, [&] {
+        launch_ground_friction(ctx->c.stream, n, x, x_t, verts, normal, lambda, eps_sq, mu, hess_offset, grad, tvals,
+                               trows, tcols);
+        SGIPC_HIP(hipStreamSynchronize(ctx->c.stream));
     });
 }
 
