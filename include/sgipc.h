@@ -133,6 +133,37 @@ int sgipc_barrier_gradient_hessian(sgipc_ctx* ctx, const double* x, const int* p
                                    double dhat_sq, double kappa, int hess_offset, double* grad,
                                    double* tvals, int* trows, int* tcols);
 
+/* Replaces lbvh_f/lbvh_e::ConstructFullCCD + SelfCollitionFullDetect and
+ * GIPC::self_largestFeasibleStepSize (StiffGIPC/mlbvh.cu:1408-1774,2103-2238;
+ * GIPC.cu:7095-7198,9010-9030) with the additive CCD of StiffGIPC/ACCD.cu:
+ * 317-534: the largest t in (0, 1] (*step, host) such that x - t dir keeps
+ * every candidate pair apart, with eta = 1 - slackness and zero thickness.
+ * Candidates: point-face and edge-edge pairs whose boxes, swept to
+ * x - alpha dir, overlap within sqrt(dhat_sq) (point-face pairs whose point is
+ * behind the face's interior are skipped, as in _checkPTintersection_fullCCD).
+ * *n_candidates (host, may be NULL) receives their count. */
+int sgipc_ccd_self_step(sgipc_ctx* ctx, const double* x, const double* dir, double alpha,
+                        const uint32_t* faces, int n_faces, const uint32_t* edges, int n_edges,
+                        const uint32_t* surf_verts, int n_surf, const int* body_id,
+                        const int* boundary_type, double dhat_sq, double slackness, double* step,
+                        int* n_candidates);
+
+/* Replaces GIPC::ground_largestFeasibleStepSize (StiffGIPC/GIPC.cu:6945-7020,
+ * 9070-9090): min over surface vertices moving toward the plane
+ * normal . x = offset (normal: device, 3 doubles) of slackness * dist / coef. */
+int sgipc_ground_step(sgipc_ctx* ctx, const double* x, const uint32_t* surf_verts, int n_surf,
+                      const double* normal, double offset, const double* dir, double slackness,
+                      double* step);
+
+/* Replaces GIPC::InjectiveStepSize (StiffGIPC/GIPC.cu:1130-1300,7022-7094,
+ * 9091-9111): the smallest t > 0 at which a tet's volume along x - t dir
+ * shrinks to slackness times its current volume, minimised over the tets (1
+ * when none does).  Below error_rate in the cubic's leading coefficients the
+ * reference's quadratic / linear branches; otherwise the cubic's smallest
+ * positive root (the reference leaves that case undefined). */
+int sgipc_injective_step(sgipc_ctx* ctx, const double* x, const uint32_t* tets, int n_tets,
+                         const double* dir, double slackness, double error_rate, double* step);
+
 /* Replaces _calFrictionLastH_DistAndTan (StiffGIPC/GIPC.cu:8431-8535, RANK 2)
  * with the closest-point and tangent-basis helpers of FrictionUtils.cuh:
  * for each pair at the step-start positions x_t, the normal force lambda, the
@@ -568,6 +599,27 @@ sgipc_mas* sgipc_system_mas(sgipc_system* sys);
  * the detected pairs and their MatIndex with detect_contacts, else the
  * descriptor's pairs (mat_index NULL). */
 int sgipc_system_contacts(sgipc_system* sys, const int** pairs, const int** mat_index, int* n_pairs);
+
+/* Replaces GIPC::computeEnergy for the terms the system assembles (StiffGIPC/
+ * GIPC.cu:7373-7870): at device positions x, 1/2 |x - x~|^2_M (when x_tilde
+ * is non-NULL) + dt^2 (tet, triangle and bending energies) + the barrier of
+ * the contact pairs at x (detected there when the scene detects its
+ * contacts), summed in a fixed order into *energy (host).  Scenes with affine
+ * bodies: SGIPC_BAD_ARG. */
+int sgipc_system_energy(sgipc_system* sys, const double* x, const double* x_tilde, double* energy);
+
+/* Replaces GIPC::lineSearch with the feasible-step bound before it (StiffGIPC/
+ * GIPC.cu:10828-10920, 9010-9111): from the scene's positions x0 along
+ * x0 - a dir, a starts at min(alpha, the CCD self step when contacts are
+ * detected, the tet injectivity step) and halves while the energy exceeds
+ * the energy at x0 (c1 = 0), at most 8 times; then x0 is restored into x_out
+ * and *stopped = 1.  x_out (device, 3 n) receives the accepted positions,
+ * *alpha_out the step, energies (host, 2, may be NULL) the energies at x0 and
+ * at x_out.  Not included: the reference's edge-triangle intersection
+ * re-check and its kappa update after the search. */
+int sgipc_system_line_search(sgipc_system* sys, const double* dir, const double* x_tilde, double alpha,
+                             double slackness, double* x_out, double* alpha_out, double* energies,
+                             int* stopped);
 
 /* ---------------------------------------------------------------------------
  * Learned coarse level.  No counterpart in the reference (it has no learned

@@ -119,6 +119,23 @@ def lib():
         L.orc_ground_friction_energy.argtypes = [ctypes.c_int, c_dbl_p, c_dbl_p, c_int_p, c_dbl_p, c_dbl_p,
                                                  ctypes.c_double, ctypes.c_double]
         L.orc_ground_friction_energy.restype = ctypes.c_double
+        L.orc_ccd_self_step.argtypes = [ctypes.c_int, c_dbl_p, c_dbl_p, ctypes.c_double, ctypes.c_int, c_int_p,
+                                        ctypes.c_int, c_int_p, ctypes.c_int, c_int_p, c_int_p, c_int_p,
+                                        ctypes.c_double, ctypes.c_double, c_int_p]
+        L.orc_ccd_self_step.restype = ctypes.c_double
+        L.orc_ground_step.argtypes = [ctypes.c_int, c_dbl_p, c_int_p, c_dbl_p, ctypes.c_double, c_dbl_p,
+                                      ctypes.c_double]
+        L.orc_ground_step.restype = ctypes.c_double
+        L.orc_injective_step.argtypes = [ctypes.c_int, c_dbl_p, c_int_p, c_dbl_p, ctypes.c_double, ctypes.c_double]
+        L.orc_injective_step.restype = ctypes.c_double
+        L.orc_energy_tets.argtypes = [ctypes.c_int, c_dbl_p, c_int_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p,
+                                      ctypes.c_double]
+        L.orc_energy_tets.restype = ctypes.c_double
+        L.orc_energy_tris.argtypes = [ctypes.c_int, c_dbl_p, c_int_p, c_dbl_p, c_dbl_p] + [ctypes.c_double] * 4
+        L.orc_energy_tris.restype = ctypes.c_double
+        L.orc_energy_bending.argtypes = [ctypes.c_int, c_dbl_p, c_dbl_p, c_int_p, c_int_p, c_dbl_p, ctypes.c_double,
+                                         ctypes.c_double]
+        L.orc_energy_bending.restype = ctypes.c_double
         _LIB = L
     return _LIB
 
@@ -276,6 +293,100 @@ def ground_friction_energy(x, xt, verts, normal, lam, eps2, mu):
     x, xt, verts, normal, lam = _f64(x), _f64(xt), _i32(verts), _f64(normal), _f64(lam)
     return lib().orc_ground_friction_energy(verts.size, _ptr(x, c_dbl_p), _ptr(xt, c_dbl_p), _ptr(verts, c_int_p),
                                             _ptr(normal, c_dbl_p), _ptr(lam, c_dbl_p), float(eps2), float(mu))
+
+
+def ccd_self_step(x, direction, alpha, faces, edges, surf, dhat_sq, slackness, body_id=None, btype=None):
+    """Largest feasible fraction of the step x - t direction (additive CCD
+    over the swept-box candidates); returns (step, candidate count)."""
+    x, d = _f64(x), _f64(direction)
+    faces, edges, surf = _i32(faces).reshape(-1, 3), _i32(edges).reshape(-1, 2), _i32(surf)
+    nc = ctypes.c_int(0)
+    st = lib().orc_ccd_self_step(x.size // 3, _ptr(x, c_dbl_p), _ptr(d, c_dbl_p), float(alpha), faces.shape[0],
+                                 _ptr(faces, c_int_p), edges.shape[0], _ptr(edges, c_int_p), surf.size,
+                                 _ptr(surf, c_int_p), _ptr(_i32(body_id), c_int_p), _ptr(_i32(btype), c_int_p),
+                                 float(dhat_sq), float(slackness), ctypes.byref(nc))
+    return st, nc.value
+
+
+def ground_step(x, surf, normal, offset, direction, slackness):
+    x, surf, normal, d = _f64(x), _i32(surf), _f64(normal), _f64(direction)
+    return lib().orc_ground_step(surf.size, _ptr(x, c_dbl_p), _ptr(surf, c_int_p), _ptr(normal, c_dbl_p),
+                                 float(offset), _ptr(d, c_dbl_p), float(slackness))
+
+
+def injective_step(x, tets, direction, slackness, error_rate=1e-20):
+    x, tets, d = _f64(x), _i32(tets).reshape(-1, 4), _f64(direction)
+    return lib().orc_injective_step(tets.shape[0], _ptr(x, c_dbl_p), _ptr(tets, c_int_p), _ptr(d, c_dbl_p),
+                                    float(slackness), float(error_rate))
+
+
+def scene_energy(scene, x, x_tilde=None):
+    """Newton objective at positions x (n, 3): inertia (with x_tilde) +
+    elastic terms + the barrier of the pairs at x (detected at x for scenes
+    with a contact surface, else the seeded pairs)."""
+    L = lib()
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+    xf = x.reshape(-1)
+    e = 0.0
+    if x_tilde is not None:
+        dx = x - np.asarray(x_tilde).reshape(-1, 3)
+        e += 0.5 * float(np.sum(scene.mass * np.sum(dx * dx, axis=1)))
+    if getattr(scene, "n_tets", 0):
+        e += L.orc_energy_tets(scene.n_tets, _ptr(xf, c_dbl_p), _ptr(_i32(scene.tets).reshape(-1), c_int_p),
+                               _ptr(_f64(scene.dm_inv), c_dbl_p), _ptr(_f64(scene.vol), c_dbl_p),
+                               _ptr(_f64(scene.mu), c_dbl_p), _ptr(_f64(scene.lam), c_dbl_p), scene.dt)
+    if getattr(scene, "n_tris", 0):
+        e += L.orc_energy_tris(scene.n_tris, _ptr(xf, c_dbl_p), _ptr(_i32(scene.tris).reshape(-1), c_int_p),
+                               _ptr(_f64(scene.dm_inv2), c_dbl_p), _ptr(_f64(scene.area), c_dbl_p), scene.k_stretch,
+                               scene.k_shear, scene.strain_rate, scene.dt)
+    if getattr(scene, "n_hinges", 0):
+        q = np.zeros((scene.n_hinges, 16))
+        edges, adj = _i32(scene.edges).reshape(-1), _i32(scene.adj).reshape(-1)
+        rest = _f64(scene.rest)
+        L.orc_quad_bending_q(scene.n_hinges, _ptr(rest, c_dbl_p), _ptr(edges, c_int_p), _ptr(adj, c_int_p),
+                             _ptr(q, c_dbl_p))
+        e += L.orc_energy_bending(scene.n_hinges, _ptr(xf, c_dbl_p), _ptr(rest, c_dbl_p), _ptr(edges, c_int_p),
+                                  _ptr(adj, c_int_p), _ptr(q, c_dbl_p), scene.k_bend, scene.dt)
+    dsq = scene.dhat ** 2
+    if getattr(scene, "contact_surface", None) is not None:
+        cs = scene.contact_surface
+        pr, _, _ = contact_detect(x, scene.rest, cs.faces, cs.edges, cs.surf, dsq, body_id=cs.body_id)
+        e += barrier_energy(x, pr, dsq, scene.kappa)
+    elif scene.n_pairs:
+        if scene.pair_x is not None:
+            px = np.ascontiguousarray(scene.pair_x, dtype=np.float64).reshape(-1, 3)
+            loc = np.tile(np.array([-1, 1, 2, 3], np.int32), (scene.n_pairs, 1))
+            loc[:, 1:] += 4 * np.arange(scene.n_pairs)[:, None]
+            loc[:, 0] = -(4 * np.arange(scene.n_pairs)) - 1
+            e += barrier_energy(px, loc, dsq, scene.kappa)
+        else:
+            e += barrier_energy(x, scene.pairs, dsq, scene.kappa)
+    return e
+
+
+def line_search(scene, direction, x_tilde=None, alpha=1.0, slackness=0.8):
+    """The system's line search restated: feasible step (CCD for scenes with
+    a contact surface, tet injectivity), then halving while the energy
+    exceeds the start's, at most 8 times.  Returns (alpha, x, E0, E, stopped)."""
+    x0 = np.ascontiguousarray(scene.x, dtype=np.float64).reshape(-1, 3)
+    d = np.ascontiguousarray(direction, dtype=np.float64).reshape(-1, 3)
+    a = alpha
+    cs = getattr(scene, "contact_surface", None)
+    if cs is not None:
+        a = min(a, ccd_self_step(x0, d, a, cs.faces, cs.edges, cs.surf, scene.dhat ** 2, slackness,
+                                 body_id=cs.body_id)[0])
+    if getattr(scene, "n_tets", 0):
+        a = min(a, injective_step(x0, scene.tets, d, slackness))
+    e0 = scene_energy(scene, x0, x_tilde)
+    e = scene_energy(scene, x0 - a * d, x_tilde)
+    k = 0
+    while e > e0 and k <= 8:
+        a *= 0.5
+        k += 1
+        e = scene_energy(scene, x0 - a * d, x_tilde)
+    if k > 8:
+        return a, x0.copy(), e0, e, True
+    return a, x0 - a * d, e0, e, False
 
 
 def tri_bw(scene, hess_offset=0, with_grad=True):

@@ -483,3 +483,250 @@ double orc_barrier_energy_pairs(int n_pairs, const double* x, const int* pairs, 
     }
     return e;
 }
+
+/* ------------------------------------------------------------------------- */
+/* Feasible step: additive CCD of the candidate pairs (StiffGIPC/ACCD.cu:      */
+/* 317-534, eta = 1 - slackness, thickness 0, the direction -moveDir as in    */
+/* _reduct_min_selfTimeStep_to_double, GIPC.cu:7095-7198), candidates by the  */
+/* swept-box full-CCD queries (mlbvh.cu:1408-1508,1634-1774) with the point-  */
+/* face filter of _checkPTintersection_fullCCD (:653-683); the ground step     */
+/* (GIPC.cu:6945-7020) and the tet injectivity step (GIPC.cu:1130-1300,       */
+/* 7022-7094).                                                                */
+/* ------------------------------------------------------------------------- */
+static double pt_dist_unclassified(const double* p, const double* a, const double* b, const double* c)
+{
+    switch (dtype_pt(p, a, b, c)) {
+    case 0: return dd_pp(p, a);
+    case 1: return dd_pp(p, b);
+    case 2: return dd_pp(p, c);
+    case 3: return dd_pe(p, a, b);
+    case 4: return dd_pe(p, b, c);
+    case 5: return dd_pe(p, c, a);
+    default: return dd_pt(p, a, b, c);
+    }
+}
+static double ee_dist_unclassified(const double* a0, const double* a1, const double* b0, const double* b1)
+{
+    switch (dtype_ee(a0, a1, b0, b1)) {
+    case 0: return dd_pp(a0, b0);
+    case 1: return dd_pp(a0, b1);
+    case 2: return dd_pe(a0, b0, b1);
+    case 3: return dd_pp(a1, b0);
+    case 4: return dd_pp(a1, b1);
+    case 5: return dd_pe(a1, b0, b1);
+    case 6: return dd_pe(b0, a0, a1);
+    case 7: return dd_pe(b1, a0, a1);
+    default: return dd_ee(a0, a1, b0, b1);
+    }
+}
+
+/* conservative advancement of 4 points moving with d[4][3] (vertex 0..3), the
+ * distance function f (point-triangle or edge-edge); returns the time of
+ * contact in [0, 1] */
+static double accd(double P[4][3], double D[4][3], int ee, double eta, double th)
+{
+    double mov[3];
+    for (int d = 0; d < 3; ++d) mov[d] = -0.25 * (D[0][d] + D[1][d] + D[2][d] + D[3][d]);
+    for (int a = 0; a < 4; ++a)
+        for (int d = 0; d < 3; ++d) D[a][d] += mov[d];
+    double maxd;
+    if (ee)
+        maxd = sqrt(fmax(dot3(D[0], D[0]), dot3(D[1], D[1]))) + sqrt(fmax(dot3(D[2], D[2]), dot3(D[3], D[3])));
+    else
+        maxd = sqrt(dot3(D[0], D[0])) + sqrt(fmax(dot3(D[1], D[1]), fmax(dot3(D[2], D[2]), dot3(D[3], D[3]))));
+    if (maxd == 0.0) return 1.0;
+#define ORC_DIST2(out)                                                                    \
+    do {                                                                                  \
+        if (ee) {                                                                         \
+            out = ee_dist_unclassified(P[0], P[1], P[2], P[3]);                           \
+            if (out - th * th <= 0.0)                                                     \
+                out = fmin(fmin(dd_pp(P[0], P[2]), dd_pp(P[0], P[3])),                    \
+                           fmin(dd_pp(P[1], P[2]), dd_pp(P[1], P[3])));                   \
+        } else {                                                                          \
+            out = pt_dist_unclassified(P[0], P[1], P[2], P[3]);                           \
+        }                                                                                 \
+    } while (0)
+    double d2;
+    ORC_DIST2(d2);
+    double dc = sqrt(d2);
+    double gap = eta * (d2 - th * th) / (dc + th);
+    double toc = 0.0;
+    int count = 0;
+    for (;;) {
+        if (++count > 50000) return toc;
+        double tlb = (1.0 - eta) * (d2 - th * th) / ((dc + th) * maxd);
+        for (int a = 0; a < 4; ++a)
+            for (int d = 0; d < 3; ++d) P[a][d] += D[a][d] * tlb;
+        ORC_DIST2(d2);
+        dc = sqrt(d2);
+        if (toc != 0.0 && (d2 - th * th) / (dc + th) < gap) break;
+        toc += tlb;
+        if (toc > 1.0) return 1.0;
+    }
+#undef ORC_DIST2
+    return toc;
+}
+
+static void swept_box(Box* bx, const double* x, const double* dir, double alpha, const int* v, int k)
+{
+    box_of(bx, x, v, k);
+    for (int i = 0; i < k; ++i)
+        for (int d = 0; d < 3; ++d) {
+            double c = x[3 * (size_t)v[i] + d] - dir[3 * (size_t)v[i] + d] * alpha;
+            if (c < bx->lo[d]) bx->lo[d] = c;
+            if (c > bx->hi[d]) bx->hi[d] = c;
+        }
+}
+
+double orc_ccd_self_step(int n_verts, const double* x, const double* dir, double alpha, int n_faces, const int* faces,
+                         int n_edges, const int* edges, int n_surf, const int* surf, const int* body_id,
+                         const int* btype, double dhat_sq, double slackness, int* n_candidates)
+{
+    (void)n_verts;
+    const double gap = sqrt(dhat_sq), eta = 1.0 - slackness;
+    double tmax = 1.0; /* max of 1 / toc */
+    int cand = 0;
+    Box* fb = (Box*)malloc(sizeof(Box) * (size_t)(n_faces > 0 ? n_faces : 1));
+    for (int f = 0; f < n_faces; ++f) swept_box(&fb[f], x, dir, alpha, faces + 3 * (size_t)f, 3);
+    for (int q = 0; q < n_surf; ++q) {
+        int p = surf[q];
+        Box pb;
+        swept_box(&pb, x, dir, alpha, &p, 1);
+        for (int f = 0; f < n_faces; ++f) {
+            const int* tv = faces + 3 * (size_t)f;
+            if (!overlap(&pb, &fb[f], gap)) continue;
+            if (body_id && body_id[p] == body_id[tv[0]] && body_id[p] != -1) continue;
+            if (p == tv[0] || p == tv[1] || p == tv[2]) continue;
+            int all[4] = {p, tv[0], tv[1], tv[2]};
+            if (skip_all_fixed(btype, all, 4)) continue;
+            const double *P0 = x + 3 * (size_t)p, *A = x + 3 * (size_t)tv[0], *B = x + 3 * (size_t)tv[1],
+                         *C = x + 3 * (size_t)tv[2];
+            double e1[3], e2[3], nv[3], w[3];
+            sub3(B, A, e1);
+            sub3(C, A, e2);
+            cross3(e1, e2, nv);
+            sub3(P0, A, w);
+            if (dtype_pt(P0, A, B, C) == 6 && dot3(nv, w) < 0) continue;
+            ++cand;
+            double P[4][3], D[4][3];
+            for (int a = 0; a < 4; ++a)
+                for (int d = 0; d < 3; ++d) {
+                    P[a][d] = x[3 * (size_t)all[a] + d];
+                    D[a][d] = -dir[3 * (size_t)all[a] + d];
+                }
+            double t = 1.0 / accd(P, D, 0, eta, 0.0);
+            if (t > tmax) tmax = t;
+        }
+    }
+    free(fb);
+    Box* eb = (Box*)malloc(sizeof(Box) * (size_t)(n_edges > 0 ? n_edges : 1));
+    for (int e = 0; e < n_edges; ++e) swept_box(&eb[e], x, dir, alpha, edges + 2 * (size_t)e, 2);
+    for (int e = 0; e < n_edges; ++e) {
+        const int* ev = edges + 2 * (size_t)e;
+        for (int o = e + 1; o < n_edges; ++o) {
+            const int* ov = edges + 2 * (size_t)o;
+            if (!overlap(&eb[e], &eb[o], gap)) continue;
+            if (body_id && body_id[ev[0]] == body_id[ov[0]] && body_id[ev[0]] != -1) continue;
+            int all[4] = {ev[0], ev[1], ov[0], ov[1]};
+            if (skip_all_fixed(btype, all, 4)) continue;
+            if (ev[0] == ov[0] || ev[0] == ov[1] || ev[1] == ov[0] || ev[1] == ov[1]) continue;
+            ++cand;
+            double P[4][3], D[4][3];
+            for (int a = 0; a < 4; ++a)
+                for (int d = 0; d < 3; ++d) {
+                    P[a][d] = x[3 * (size_t)all[a] + d];
+                    D[a][d] = -dir[3 * (size_t)all[a] + d];
+                }
+            double t = 1.0 / accd(P, D, 1, eta, 0.0);
+            if (t > tmax) tmax = t;
+        }
+    }
+    free(eb);
+    if (n_candidates) *n_candidates = cand;
+    return 1.0 / tmax;
+}
+
+double orc_ground_step(int n_surf, const double* x, const int* surf, const double* normal, double offset,
+                       const double* dir, double slackness)
+{
+    double tmax = 1.0;
+    for (int q = 0; q < n_surf; ++q) {
+        const double* xv = x + 3 * (size_t)surf[q];
+        const double* dv = dir + 3 * (size_t)surf[q];
+        double coef = dot3(normal, dv);
+        if (coef > 0.0) {
+            double dist = dot3(normal, xv) - offset;
+            double t = coef / (dist * slackness);
+            if (t > tmax) tmax = t;
+        }
+    }
+    return 1.0 / tmax;
+}
+
+/* smallest t > 0 with det(Ds(x - t dir)) = slackness det(Ds(x)): the cubic
+ * c3 t^3 + c2 t^2 + c1 t + (1 - slackness) c0 with c0 = det D0, c3 = det D1
+ * (D1 the edge matrix of -dir) and c1, c2 the mixed determinants; below
+ * error_rate in |c3| (and |c2|) the reference's quadratic / linear branches */
+static double det_cols3(const double* a, const double* b, const double* c)
+{
+    double t[3];
+    cross3(b, c, t);
+    return dot3(a, t);
+}
+double orc_injective_step(int n_tets, const double* x, const int* tets, const double* dir, double slackness,
+                          double error_rate)
+{
+    double tmax = 1.0;
+    for (int e = 0; e < n_tets; ++e) {
+        const int* t = tets + 4 * (size_t)e;
+        double A[3][3], Bm[3][3];
+        for (int c = 0; c < 3; ++c)
+            for (int d = 0; d < 3; ++d) {
+                A[c][d] = x[3 * (size_t)t[c + 1] + d] - x[3 * (size_t)t[0] + d];
+                Bm[c][d] = -dir[3 * (size_t)t[c + 1] + d] + dir[3 * (size_t)t[0] + d];
+            }
+        double c0 = det_cols3(A[0], A[1], A[2]), c3 = det_cols3(Bm[0], Bm[1], Bm[2]);
+        double c1 = det_cols3(Bm[0], A[1], A[2]) + det_cols3(A[0], Bm[1], A[2]) + det_cols3(A[0], A[1], Bm[2]);
+        double c2 = det_cols3(A[0], Bm[1], Bm[2]) + det_cols3(Bm[0], A[1], Bm[2]) + det_cols3(Bm[0], Bm[1], A[2]);
+        double k0 = (1.0 - slackness) * c0;
+        double ts = 1.0;
+        if (fabs(c3) <= error_rate) {
+            if (fabs(c2) <= error_rate) {
+                ts = -k0 / c1;
+            } else {
+                double disc = c1 * c1 - 4.0 * c2 * k0;
+                if (disc > 0) {
+                    ts = (-c1 - sqrt(disc)) / (2.0 * c2);
+                    if (ts < 0) ts = (-c1 + sqrt(disc)) / (2.0 * c2);
+                } else {
+                    ts = 1.0;
+                }
+            }
+        } else {
+            /* smallest positive real root of the cubic: bracket by sampling,
+             * then bisect (the reference leaves this case undefined) */
+            double best = -1.0;
+            const int S = 256;
+            double tlo = 0.0, glo = k0;
+            for (int s = 1; s <= S && best < 0; ++s) {
+                double th = 4.0 * s / S;
+                double gh = ((c3 * th + c2) * th + c1) * th + k0;
+                if ((glo > 0) != (gh > 0) || gh == 0.0) {
+                    double lo = tlo, hi = th;
+                    for (int it = 0; it < 100; ++it) {
+                        double m = 0.5 * (lo + hi), gm = ((c3 * m + c2) * m + c1) * m + k0;
+                        if ((gm > 0) == (glo > 0)) lo = m; else hi = m;
+                    }
+                    best = 0.5 * (lo + hi);
+                }
+                tlo = th;
+                glo = gh;
+            }
+            ts = best > 0 ? best : 1.0;
+        }
+        if (ts <= 0) ts = 1.0;
+        double inv = 1.0 / ts;
+        if (inv > tmax) tmax = inv;
+    }
+    return 1.0 / tmax;
+}

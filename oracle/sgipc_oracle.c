@@ -1552,3 +1552,56 @@ int orc_pcg(long nnz, const double* vals, const int* rows, const int* cols, int 
     orc_precond pc = {mas, 0, mas_fp32, 0, NULL};
     return orc_pcg2(nnz, vals, rows, cols, n_rows, mas ? &pc : NULL, b, x, tol, stop_mode, max_iter, stats);
 }
+
+/* ------------------------------------------------------------------------- */
+/* Energy sums of the Newton objective (line search; GIPC.cu:7373-7870), with */
+/* the scaling of the assembled gradient: dt^2 vol Psi_SNH, dt^2 area Psi_BW, */
+/* 1/2 |rest edge| k_bend dt^2 x^T Q x, 1/2 m |x - x~|^2.                     */
+/* ------------------------------------------------------------------------- */
+double orc_energy_tets(int n, const double* x, const int* tets, const double* dm_inv, const double* vol,
+                       const double* mu, const double* lam, double dt)
+{
+    double e = 0.0;
+    for (int t = 0; t < n; ++t) {
+        double xs[12];
+        for (int a = 0; a < 4; ++a)
+            for (int d = 0; d < 3; ++d) xs[3 * a + d] = x[3 * (size_t)tets[4 * (size_t)t + a] + d];
+        e += dt * dt * vol[t] * orc_tet_snh_energy(xs, dm_inv + 9 * (size_t)t, mu[t], lam[t]);
+    }
+    return e;
+}
+
+double orc_energy_tris(int n, const double* x, const int* tris, const double* dm_inv2, const double* area,
+                       double k_st, double k_sh, double rate, double dt)
+{
+    double e = 0.0;
+    for (int t = 0; t < n; ++t) {
+        double xs[9];
+        for (int a = 0; a < 3; ++a)
+            for (int d = 0; d < 3; ++d) xs[3 * a + d] = x[3 * (size_t)tris[3 * (size_t)t + a] + d];
+        e += dt * dt * orc_tri_bw_energy(xs, dm_inv2 + 4 * (size_t)t, area[t], k_st, k_sh, rate);
+    }
+    return e;
+}
+
+double orc_energy_bending(int n, const double* x, const double* rest_x, const int* edges, const int* adj,
+                          const double* Q16, double k_bend, double dt)
+{
+    double e = 0.0;
+    for (int i = 0; i < n; ++i) {
+        if (adj[2 * i + 1] == -1) continue;
+        int vid[4] = {edges[2 * i], edges[2 * i + 1], adj[2 * i], adj[2 * i + 1]};
+        double L = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            double r = rest_x[3 * (size_t)vid[0] + d] - rest_x[3 * (size_t)vid[1] + d];
+            L += r * r;
+        }
+        double s = 0.0;
+        for (int d = 0; d < 3; ++d)
+            for (int a = 0; a < 4; ++a)
+                for (int b = 0; b < 4; ++b)
+                    s += x[3 * (size_t)vid[a] + d] * Q16[16 * (size_t)i + 4 * a + b] * x[3 * (size_t)vid[b] + d];
+        e += 0.5 * sqrt(L) * k_bend * dt * dt * s;
+    }
+    return e;
+}

@@ -85,6 +85,17 @@ double orc_barrier_energy_pairs(int n_pairs, const double* x, const int* pairs, 
  * start xt; gradient and Hessian blocks [PT+EE x 16 | PE x 9 | PP x 4] by
  * MatIndex; the ground's per-vertex force and diagonal blocks; energies for
  * the finite-difference pins. */
+/* Feasible step sizes (contact_oracle.c): additive CCD over the swept-box
+ * candidates (StiffGIPC/ACCD.cu:317-534, mlbvh.cu:1408-1774, GIPC.cu:
+ * 7095-7198), the ground step (GIPC.cu:6945-7020) and the tet injectivity
+ * step (GIPC.cu:1130-1300,7022-7094); positions move as x - t dir. */
+double orc_ccd_self_step(int n_verts, const double* x, const double* dir, double alpha, int n_faces, const int* faces,
+                         int n_edges, const int* edges, int n_surf, const int* surf, const int* body_id,
+                         const int* btype, double dhat_sq, double slackness, int* n_candidates);
+double orc_ground_step(int n_surf, const double* x, const int* surf, const double* normal, double offset,
+                       const double* dir, double slackness);
+double orc_injective_step(int n_tets, const double* x, const int* tets, const double* dir, double slackness,
+                          double error_rate);
 int orc_friction_prepare(int n, const double* xt, const int* pairs, double dhat_sq, double kappa, double* lambda,
                          double* coord, double* basis);
 int orc_friction(int n, const double* x, const double* xt, const int* pairs, const int* mat_index, int n4, int n3,
@@ -211,6 +222,15 @@ int orc_pcg2(long nnz, const double* vals, const int* rows, const int* cols, int
  * 0 = none, 1 = SpMV drops the transposed (lower) contribution, 2 = Galerkin
  * coarse levels drop H^T on merged diagonals, 3 = barrier Hessian uses lambda0
  * with the wrong sign of the log term, 4 = cloth shear Hessian left unprojected. */
+/* Energy sums of the Newton objective (line search), scaled like the
+ * assembled gradients: tets dt^2 vol Psi, triangles dt^2 area Psi, bending
+ * 1/2 |e| k_bend dt^2 x^T Q x. */
+double orc_energy_tets(int n, const double* x, const int* tets, const double* dm_inv, const double* vol,
+                       const double* mu, const double* lam, double dt);
+double orc_energy_tris(int n, const double* x, const int* tris, const double* dm_inv2, const double* area,
+                       double k_st, double k_sh, double rate, double dt);
+double orc_energy_bending(int n, const double* x, const double* rest_x, const int* edges, const int* adj,
+                          const double* Q16, double k_bend, double dt);
 void orc_set_fault(int fault);
 int orc_get_fault(void);
 

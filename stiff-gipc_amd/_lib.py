@@ -90,6 +90,10 @@ _SIGS = {
     "sgipc_contact_detect": (ci, [vp, vp, vp, ci, vp, ci, vp, ci, vp, ci, vp, vp, cd, vp, vp, ci, vp]),
     "sgipc_barrier_gradient_hessian": (ci, [vp, vp, vp, vp, ci, vp, cd, cd, ci, vp, vp, vp, vp]),
     "sgipc_friction_prepare": (ci, [vp, vp, vp, ci, cd, cd, vp, vp, vp]),
+    "sgipc_ccd_self_step": (ci, [vp, vp, vp, cd, vp, ci, vp, ci, vp, ci, vp, vp, cd, cd, ctypes.POINTER(cd),
+                                 ctypes.POINTER(ci)]),
+    "sgipc_ground_step": (ci, [vp, vp, vp, ci, vp, cd, vp, cd, ctypes.POINTER(cd)]),
+    "sgipc_injective_step": (ci, [vp, vp, vp, ci, vp, cd, cd, ctypes.POINTER(cd)]),
     "sgipc_friction_gradient_hessian": (ci, [vp, vp, vp, vp, vp, ci, vp, vp, vp, vp, cd, cd, ci, vp, vp, vp, vp]),
     "sgipc_ground_friction_prepare": (ci, [vp, vp, vp, ci, vp, cd, cd, cd, vp]),
     "sgipc_ground_friction_gradient_hessian": (ci, [vp, vp, vp, vp, ci, vp, vp, cd, cd, ci, vp, vp, vp, vp]),
@@ -123,6 +127,9 @@ _SIGS = {
     "sgipc_system_bcoo": (ci, [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(ci)]),
     "sgipc_system_mas": (vp, [vp]),
     "sgipc_system_contacts": (ci, [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(ci)]),
+    "sgipc_system_energy": (ci, [vp, vp, vp, ctypes.POINTER(cd)]),
+    "sgipc_system_line_search": (ci, [vp, vp, vp, cd, cd, vp, ctypes.POINTER(cd), ctypes.POINTER(cd),
+                                      ctypes.POINTER(ci)]),
     "sgipc_mas_set_coarse_mode": (ci, [vp, ci]),
     "sgipc_mas_set_smoothing_weight": (ci, [vp, cd]),
     "sgipc_mas_set_level0_half": (ci, [vp, ci]),

@@ -179,8 +179,10 @@ __device__ Found narrow_ee(const double* __restrict__ x, const double* __restric
 }
 
 // ------------------------------------------------------------ BVH build
+// with dir: the box also covers x - alpha dir (swept boxes of the CCD queries,
+// mlbvh.cu:1171-1204)
 __global__ void k_prim_boxes(int n, const double* __restrict__ x, const uint32_t* __restrict__ prims, int k,
-                             double* __restrict__ out)
+                             const double* __restrict__ dir, double alpha, double* __restrict__ out)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -191,6 +193,11 @@ __global__ void k_prim_boxes(int n, const double* __restrict__ x, const uint32_t
             const double c = x[3 * v + d];
             lo[d] = fmin(lo[d], c);
             hi[d] = fmax(hi[d], c);
+            if (dir != nullptr) {
+                const double e = c - dir[3 * v + d] * alpha;
+                lo[d] = fmin(lo[d], e);
+                hi[d] = fmax(hi[d], e);
+            }
         }
     }
     for (int d = 0; d < 3; ++d) {
@@ -602,9 +609,256 @@ __global__ __launch_bounds__(128) void k_barrier_pairs(int np, const double* __r
         }
 }
 
+
+// ------------------------------------------------------------ feasible step
+// distance of the closest features (ACCD.cu:317-371)
+__device__ double pt_dist_any(V3 p, V3 a, V3 b, V3 c)
+{
+    switch (feature_pt(p, a, b, c)) {
+    case 0: return sq_pp(p, a);
+    case 1: return sq_pp(p, b);
+    case 2: return sq_pp(p, c);
+    case 3: return sq_pe(p, a, b);
+    case 4: return sq_pe(p, b, c);
+    case 5: return sq_pe(p, c, a);
+    default: return sq_pt(p, a, b, c);
+    }
+}
+__device__ double ee_dist_any(V3 a0, V3 a1, V3 b0, V3 b1)
+{
+    switch (feature_ee(a0, a1, b0, b1)) {
+    case 0: return sq_pp(a0, b0);
+    case 1: return sq_pp(a0, b1);
+    case 2: return sq_pe(a0, b0, b1);
+    case 3: return sq_pp(a1, b0);
+    case 4: return sq_pp(a1, b1);
+    case 5: return sq_pe(a1, b0, b1);
+    case 6: return sq_pe(b0, a0, a1);
+    case 7: return sq_pe(b1, a0, a1);
+    default: return sq_ee(a0, a1, b0, b1);
+    }
+}
+
+// Additive CCD (ACCD.cu:373-534): conservative advancement of the four points
+// moving by D (centred), with eta = 1 - slackness and zero thickness; the time
+// of contact in [0, 1], at most 50000 steps.
+__device__ double accd_toc(V3 P[4], V3 D[4], bool ee, double eta)
+{
+    const V3 mov{-0.25 * (D[0].x + D[1].x + D[2].x + D[3].x), -0.25 * (D[0].y + D[1].y + D[2].y + D[3].y),
+                 -0.25 * (D[0].z + D[1].z + D[2].z + D[3].z)};
+    for (int a = 0; a < 4; ++a) D[a] = V3{D[a].x + mov.x, D[a].y + mov.y, D[a].z + mov.z};
+    const double maxd = ee ? sqrt(fmax(dot(D[0], D[0]), dot(D[1], D[1]))) + sqrt(fmax(dot(D[2], D[2]), dot(D[3], D[3])))
+                           : sqrt(dot(D[0], D[0])) +
+                                 sqrt(fmax(dot(D[1], D[1]), fmax(dot(D[2], D[2]), dot(D[3], D[3]))));
+    if (maxd == 0.0) return 1.0;
+    auto dist2 = [&]() {
+        if (!ee) return pt_dist_any(P[0], P[1], P[2], P[3]);
+        double d2 = ee_dist_any(P[0], P[1], P[2], P[3]);
+        if (d2 <= 0.0)
+            d2 = fmin(fmin(sq_pp(P[0], P[2]), sq_pp(P[0], P[3])), fmin(sq_pp(P[1], P[2]), sq_pp(P[1], P[3])));
+        return d2;
+    };
+    double d2 = dist2(), dc = sqrt(d2);
+    const double gap = eta * d2 / dc;
+    double toc = 0.0;
+    for (int count = 1;; ++count) {
+        if (count > 50000) return toc;
+        const double tlb = (1.0 - eta) * d2 / (dc * maxd);
+        for (int a = 0; a < 4; ++a) P[a] = V3{P[a].x + D[a].x * tlb, P[a].y + D[a].y * tlb, P[a].z + D[a].z * tlb};
+        d2 = dist2();
+        dc = sqrt(d2);
+        if (toc != 0.0 && d2 / dc < gap) break;
+        toc += tlb;
+        if (toc > 1.0) return 1.0;
+    }
+    return toc;
+}
+
+// point-face candidates of the swept boxes (mlbvh.cu:1408-1508 with
+// _checkPTintersection_fullCCD, :653-683): per surface vertex the largest
+// 1 / toc over its candidates
+__global__ __launch_bounds__(256) void k_ccd_vf(int n_surf, const uint32_t* __restrict__ surf,
+                                                const double* __restrict__ x, const double* __restrict__ dir,
+                                                double alpha, const uint32_t* __restrict__ faces, int nf,
+                                                const double* __restrict__ box, const int* __restrict__ child,
+                                                const int* __restrict__ sidx, const int* __restrict__ body,
+                                                const int* __restrict__ bt, double dsq, double eta,
+                                                double* __restrict__ inv_toc, int* __restrict__ cnt,
+                                                int* __restrict__ err)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n_surf) return;
+    const int p = (int)surf[q];
+    const V3 P = ld3(x, p), Pd = ld3(dir, p);
+    const double qb[6] = {fmin(P.x, P.x - Pd.x * alpha), fmin(P.y, P.y - Pd.y * alpha), fmin(P.z, P.z - Pd.z * alpha),
+                          fmax(P.x, P.x - Pd.x * alpha), fmax(P.y, P.y - Pd.y * alpha), fmax(P.z, P.z - Pd.z * alpha)};
+    double best = 1.0;
+    int m = 0;
+    const bool ok = traverse(nf, box, child, sidx, qb, sqrt(dsq), [&](int f) {
+        const int t0 = (int)faces[3L * f], t1 = (int)faces[3L * f + 1], t2 = (int)faces[3L * f + 2];
+        if (body != nullptr && body[p] == body[t0] && body[p] != -1) return;
+        if (p == t0 || p == t1 || p == t2) return;
+        if (all_fixed(bt, p, t0, t1, t2)) return;
+        const V3 A = ld3(x, t0), B = ld3(x, t1), C = ld3(x, t2);
+        if (feature_pt(P, A, B, C) == 6 && dot(cross(B - A, C - A), P - A) < 0.0) return;
+        ++m;
+        V3 Q[4] = {P, A, B, C};
+        const V3 da = ld3(dir, t0), db = ld3(dir, t1), dc = ld3(dir, t2);
+        V3 D[4] = {V3{-Pd.x, -Pd.y, -Pd.z}, V3{-da.x, -da.y, -da.z}, V3{-db.x, -db.y, -db.z}, V3{-dc.x, -dc.y, -dc.z}};
+        best = fmax(best, 1.0 / accd_toc(Q, D, false, eta));
+    });
+    if (!ok) atomicOr(err, 1);
+    inv_toc[q] = best;
+    cnt[q] = m;
+}
+
+// edge-edge candidates of the swept boxes (mlbvh.cu:1634-1774)
+__global__ __launch_bounds__(256) void k_ccd_ee(int ne, const uint32_t* __restrict__ edges,
+                                                const double* __restrict__ x, const double* __restrict__ dir,
+                                                const double* __restrict__ box, const int* __restrict__ child,
+                                                const int* __restrict__ sidx, const double* __restrict__ lb,
+                                                const int* __restrict__ body, const int* __restrict__ bt,
+                                                double dsq, double eta, double* __restrict__ inv_toc,
+                                                int* __restrict__ cnt, int* __restrict__ err)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    const int a0 = (int)edges[2L * e], a1 = (int)edges[2L * e + 1];
+    double qb[6];
+    for (int d = 0; d < 6; ++d) qb[d] = lb[6L * e + d];
+    double best = 1.0;
+    int m = 0;
+    const bool ok = traverse(ne, box, child, sidx, qb, sqrt(dsq), [&](int o) {
+        if (o <= e) return;
+        const int b0 = (int)edges[2L * o], b1 = (int)edges[2L * o + 1];
+        if (body != nullptr && body[a0] == body[b0] && body[a0] != -1) return;
+        if (all_fixed(bt, a0, a1, b0, b1)) return;
+        if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) return;
+        ++m;
+        V3 Q[4] = {ld3(x, a0), ld3(x, a1), ld3(x, b0), ld3(x, b1)};
+        const int id[4] = {a0, a1, b0, b1};
+        V3 D[4];
+        for (int a = 0; a < 4; ++a) {
+            const V3 dv = ld3(dir, id[a]);
+            D[a] = V3{-dv.x, -dv.y, -dv.z};
+        }
+        best = fmax(best, 1.0 / accd_toc(Q, D, true, eta));
+    });
+    if (!ok) atomicOr(err, 1);
+    inv_toc[e] = best;
+    cnt[e] = m;
+}
+
+// ground (GIPC.cu:6945-7020): coef / (dist slackness) for vertices moving
+// toward the plane
+__global__ void k_ground_step(int n, const uint32_t* __restrict__ surf, const double* __restrict__ x,
+                              const double* __restrict__ nrm, double offset, const double* __restrict__ dir,
+                              double slackness, double* __restrict__ inv_t)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const V3 nv{nrm[0], nrm[1], nrm[2]};
+    const int v = (int)surf[q];
+    const double coef = dot(nv, ld3(dir, v));
+    double t = 1.0;
+    if (coef > 0.0) t = coef / ((dot(nv, ld3(x, v)) - offset) * slackness);
+    inv_t[q] = t;
+}
+
+// tet injectivity (GIPC.cu:1130-1300,7022-7094): the smallest t > 0 at which
+// the tet's volume shrinks to slackness times its current one; the
+// reference's quadratic and linear branches below error_rate, the smallest
+// positive root of the cubic otherwise (a case the reference leaves undefined)
+__global__ void k_injective_step(int n, const uint32_t* __restrict__ tets, const double* __restrict__ x,
+                                 const double* __restrict__ dir, double slackness, double error_rate,
+                                 double* __restrict__ inv_t)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    int t[4];
+    for (int a = 0; a < 4; ++a) t[a] = (int)tets[4L * e + a];
+    const V3 x0 = ld3(x, t[0]), d0 = ld3(dir, t[0]);
+    V3 A[3], Bm[3];
+    for (int c = 0; c < 3; ++c) {
+        A[c] = ld3(x, t[c + 1]) - x0;
+        const V3 dc = ld3(dir, t[c + 1]);
+        Bm[c] = V3{-dc.x + d0.x, -dc.y + d0.y, -dc.z + d0.z};
+    }
+    const double c0 = det_cols(A[0], A[1], A[2]), c3 = det_cols(Bm[0], Bm[1], Bm[2]);
+    const double c1 = det_cols(Bm[0], A[1], A[2]) + det_cols(A[0], Bm[1], A[2]) + det_cols(A[0], A[1], Bm[2]);
+    const double c2 = det_cols(A[0], Bm[1], Bm[2]) + det_cols(Bm[0], A[1], Bm[2]) + det_cols(Bm[0], Bm[1], A[2]);
+    const double k0 = (1.0 - slackness) * c0;
+    double ts = 1.0;
+    if (fabs(c3) <= error_rate) {
+        if (fabs(c2) <= error_rate) {
+            ts = -k0 / c1;
+        } else {
+            const double disc = c1 * c1 - 4.0 * c2 * k0;
+            if (disc > 0) {
+                ts = (-c1 - sqrt(disc)) / (2.0 * c2);
+                if (ts < 0) ts = (-c1 + sqrt(disc)) / (2.0 * c2);
+            }
+        }
+    } else {
+        double best = -1.0, tlo = 0.0, glo = k0;
+        for (int s = 1; s <= 256 && best < 0.0; ++s) {
+            const double th = 4.0 * s / 256;
+            const double gh = ((c3 * th + c2) * th + c1) * th + k0;
+            if ((glo > 0) != (gh > 0) || gh == 0.0) {
+                double lo = tlo, hi = th;
+                for (int it = 0; it < 100; ++it) {
+                    const double mm = 0.5 * (lo + hi), gm = ((c3 * mm + c2) * mm + c1) * mm + k0;
+                    if ((gm > 0) == (glo > 0)) lo = mm;
+                    else hi = mm;
+                }
+                best = 0.5 * (lo + hi);
+            }
+            tlo = th;
+            glo = gh;
+        }
+        ts = best > 0.0 ? best : 1.0;
+    }
+    if (ts <= 0.0) ts = 1.0;
+    inv_t[e] = 1.0 / ts;
+}
+
+// max of n values and sum of n ints -> out[blockIdx.x], icnt[blockIdx.x]
+__global__ __launch_bounds__(256) void k_max_reduce(long n, const double* __restrict__ v, const int* __restrict__ c,
+                                                    double* __restrict__ out, int* __restrict__ icnt)
+{
+    __shared__ double sm[4];
+    __shared__ int si[4];
+    double m = 1.0;
+    int k = 0;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        m = fmax(m, v[i]);
+        if (c != nullptr) k += c[i];
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        m = fmax(m, __shfl_xor(m, off, 64));
+        k += __shfl_xor(k, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sm[threadIdx.x >> 6] = m;
+        si[threadIdx.x >> 6] = k;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = sm[0];
+        int rk = si[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+            r = fmax(r, sm[w]);
+            rk += si[w];
+        }
+        out[blockIdx.x] = r;
+        icnt[blockIdx.x] = rk;
+    }
+}
+
 }  // namespace
 
-void Lbvh::build(hipStream_t s, const double* x, const uint32_t* prims, int n_prims, int k)
+void Lbvh::build(hipStream_t s, const double* x, const uint32_t* prims, int n_prims, int k, const double* dir,
+                 double alpha)
 {
     n = n_prims;
     if (n <= 0) return;
@@ -620,7 +874,7 @@ void Lbvh::build(hipStream_t s, const double* x, const uint32_t* prims, int n_pr
     child.reserve(2L * (n > 1 ? n - 1 : 1));
     parent.reserve(nodes);
     flag.reserve(n > 1 ? n - 1 : 1);
-    k_prim_boxes<<<g256(n), 256, 0, s>>>(n, x, prims, k, leaf_box.get());
+    k_prim_boxes<<<g256(n), 256, 0, s>>>(n, x, prims, k, dir, alpha, leaf_box.get());
     SGIPC_LAUNCH_CHECK();
     k_box_union<<<gb, 256, 0, s>>>(n, leaf_box.get(), scene.get() + 6);
     SGIPC_LAUNCH_CHECK();
@@ -758,6 +1012,89 @@ ContactCounts ContactDetector::detect(hipStream_t s, const double* x, const doub
     out.c[3] = cls_tot[1];
     out.c[4] = cls_tot[2];
     return out;
+}
+
+
+double StepReducer::max_inv(hipStream_t s, const double* v, const int* c, long n, long* total)
+{
+    const int gb = (int)std::min<long>(256, (n + 255) / 256);
+    part_.reserve(gb + 1);
+    ipart_.reserve(gb + 1);
+    k_max_reduce<<<gb, 256, 0, s>>>(n, v, c, part_.get(), ipart_.get());
+    SGIPC_LAUNCH_CHECK();
+    k_max_reduce<<<1, 256, 0, s>>>(gb, part_.get(), ipart_.get(), part_.get() + gb, ipart_.get() + gb);
+    SGIPC_LAUNCH_CHECK();
+    double h = 1.0;
+    int hk = 0;
+    SGIPC_HIP(hipMemcpyAsync(&h, part_.get() + gb, sizeof(double), hipMemcpyDeviceToHost, s));
+    SGIPC_HIP(hipMemcpyAsync(&hk, ipart_.get() + gb, sizeof(int), hipMemcpyDeviceToHost, s));
+    SGIPC_HIP(hipStreamSynchronize(s));
+    if (total) *total = hk;
+    return h;
+}
+
+double ContactDetector::ccd_step(hipStream_t s, const double* x, const double* dir, double alpha,
+                                 const uint32_t* faces, int n_faces, const uint32_t* edges, int n_edges,
+                                 const uint32_t* surf, int n_surf, const int* body_id, const int* btype,
+                                 double dhat_sq, double slackness, long* n_candidates)
+{
+    const double eta = 1.0 - slackness;
+    const int nq_vf = n_faces > 0 ? n_surf : 0;
+    const int nq_ee = n_edges > 1 ? n_edges : 0;
+    double inv = 1.0;
+    long cand = 0;
+    err_.reserve(1);
+    launch_fill<int>(s, err_.get(), 0, 1);
+    if (nq_vf > 0) {
+        fbvh_.build(s, x, faces, n_faces, 3, dir, alpha);
+        vals_.reserve(nq_vf);
+        cnt_.reserve(nq_vf);
+        k_ccd_vf<<<g256(nq_vf), 256, 0, s>>>(nq_vf, surf, x, dir, alpha, faces, n_faces, fbvh_.box.get(),
+                                             fbvh_.child.get(), fbvh_.sidx.get(), body_id, btype, dhat_sq, eta,
+                                             vals_.get(), cnt_.get(), err_.get());
+        SGIPC_LAUNCH_CHECK();
+        long c = 0;
+        inv = std::max(inv, red_.max_inv(s, vals_.get(), cnt_.get(), nq_vf, &c));
+        cand += c;
+    }
+    if (nq_ee > 0) {
+        ebvh_.build(s, x, edges, n_edges, 2, dir, alpha);
+        vals_.reserve(nq_ee);
+        cnt_.reserve(nq_ee);
+        k_ccd_ee<<<g256(nq_ee), 256, 0, s>>>(nq_ee, edges, x, dir, ebvh_.box.get(), ebvh_.child.get(),
+                                             ebvh_.sidx.get(), ebvh_.leaf_box.get(), body_id, btype, dhat_sq, eta,
+                                             vals_.get(), cnt_.get(), err_.get());
+        SGIPC_LAUNCH_CHECK();
+        long c = 0;
+        inv = std::max(inv, red_.max_inv(s, vals_.get(), cnt_.get(), nq_ee, &c));
+        cand += c;
+    }
+    int h = 0;
+    SGIPC_HIP(hipMemcpyAsync(&h, err_.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+    SGIPC_HIP(hipStreamSynchronize(s));
+    SGIPC_CHECK(h == 0, SGIPC_CAPACITY, "BVH traversal deeper than its stack");
+    if (n_candidates) *n_candidates = cand;
+    return 1.0 / inv;
+}
+
+double ground_step(hipStream_t s, StepReducer& red, DevBuf<double>& vals, const double* x, const uint32_t* surf,
+                   int n_surf, const double* normal, double offset, const double* dir, double slackness)
+{
+    if (n_surf <= 0) return 1.0;
+    vals.reserve(n_surf);
+    k_ground_step<<<g256(n_surf), 256, 0, s>>>(n_surf, surf, x, normal, offset, dir, slackness, vals.get());
+    SGIPC_LAUNCH_CHECK();
+    return 1.0 / red.max_inv(s, vals.get(), nullptr, n_surf, nullptr);
+}
+
+double injective_step(hipStream_t s, StepReducer& red, DevBuf<double>& vals, const double* x, const uint32_t* tets,
+                      int n_tets, const double* dir, double slackness, double error_rate)
+{
+    if (n_tets <= 0) return 1.0;
+    vals.reserve(n_tets);
+    k_injective_step<<<g256(n_tets), 256, 0, s>>>(n_tets, tets, x, dir, slackness, error_rate, vals.get());
+    SGIPC_LAUNCH_CHECK();
+    return 1.0 / red.max_inv(s, vals.get(), nullptr, n_tets, nullptr);
 }
 
 void launch_barrier_pairs(hipStream_t s, int n_pairs, const double* x, const int* pairs, const int* mat_index, int n4,

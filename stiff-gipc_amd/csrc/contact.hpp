@@ -20,7 +20,9 @@ struct Lbvh {
     DevBuf<int> parent;       // per node (-1 at the root)
     DevBuf<int> flag;         // per internal node, refit arrivals
     DevBuf<unsigned char> temp;
-    void build(hipStream_t s, const double* x, const uint32_t* prims, int n_prims, int k);
+    // dir (optional): boxes swept to x - alpha dir
+    void build(hipStream_t s, const double* x, const uint32_t* prims, int n_prims, int k, const double* dir = nullptr,
+               double alpha = 0.0);
 };
 
 // Counts of the reference's _cpNum: [0] pairs, [1] unused, [2] point-point,
@@ -29,8 +31,24 @@ struct ContactCounts {
     int c[5] = {0, 0, 0, 0, 0};
 };
 
+// Max of a device array of values >= 1 (and the sum of a parallel int array).
+class StepReducer {
+  public:
+    double max_inv(hipStream_t s, const double* v, const int* c, long n, long* total);
+
+  private:
+    DevBuf<double> part_;
+    DevBuf<int> ipart_;
+};
+
 class ContactDetector {
   public:
+    // Largest t in (0, 1] such that x - t dir keeps every candidate pair of
+    // the surface apart (additive CCD with eta = 1 - slackness); candidates
+    // from the boxes swept to x - alpha dir.
+    double ccd_step(hipStream_t s, const double* x, const double* dir, double alpha, const uint32_t* faces, int n_faces,
+                    const uint32_t* edges, int n_edges, const uint32_t* surf, int n_surf, const int* body_id,
+                    const int* btype, double dhat_sq, double slackness, long* n_candidates);
     // Pairs within squared distance dhat_sq; writes at most `capacity` pairs
     // and their MatIndex.  Order: point-face pairs by (surface slot, face),
     // then edge-edge pairs by (edge, edge).  Throws SGIPC_CAPACITY (counts
@@ -51,7 +69,16 @@ class ContactDetector {
     DevBuf<int> rcls_;
     DevBuf<int> flags_, scan_;
     DevBuf<unsigned char> temp_;
+    DevBuf<double> vals_;
+    StepReducer red_;
 };
+
+// Ground step (largest t keeping surface vertices above normal . x = offset
+// with the reference's slackness rule) and tet injectivity step.
+double ground_step(hipStream_t s, StepReducer& red, DevBuf<double>& vals, const double* x, const uint32_t* surf,
+                   int n_surf, const double* normal, double offset, const double* dir, double slackness);
+double injective_step(hipStream_t s, StepReducer& red, DevBuf<double>& vals, const double* x, const uint32_t* tets,
+                      int n_tets, const double* dir, double slackness, double error_rate);
 
 // Barrier gradient (atomically added into grad when non-null) and the
 // Hessian blocks of every pair type into the contact arena

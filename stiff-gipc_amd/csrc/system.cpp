@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "capi_internal.hpp"
+#include "energy.hpp"
 #include "friction.hpp"
 
 using namespace sgipc;
@@ -59,6 +60,13 @@ struct sgipc_system {
     DevBuf<int> dpairs, dmat, dbad;
     int dcap = 0;
     int n_detected = 0;  // pairs of the last solve
+    // line search: energy terms, the pairs at trial positions, the trial
+    // positions and the step reductions
+    EnergySum esum;
+    DevBuf<int> lpairs, lmat;
+    int lcap = 0;
+    DevBuf<double> stepvals;
+    StepReducer stepred;
     SolveWorkspace* ws = nullptr;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     // learned coarse level: network, mode (0 off, 1 on, 2 automatic), and
@@ -349,6 +357,54 @@ int sgipc_ground_friction_gradient_hessian(sgipc_ctx* ctx, const double* x, cons
         launch_ground_friction(ctx->c.stream, n, x, x_t, verts, normal, lambda, eps_sq, mu, hess_offset, grad, tvals,
                                trows, tcols);
         SGIPC_HIP(hipStreamSynchronize(ctx->c.stream));
+    });
+}
+
+int sgipc_ccd_self_step(sgipc_ctx* ctx, const double* x, const double* dir, double alpha, const uint32_t* faces,
+                        int n_faces, const uint32_t* edges, int n_edges, const uint32_t* surf_verts, int n_surf,
+                        const int* body_id, const int* boundary_type, double dhat_sq, double slackness, double* step,
+                        int* n_candidates)
+{
+    if (!ctx || !step || n_faces < 0 || n_edges < 0 || n_surf < 0) return SGIPC_BAD_ARG;
+    *step = 1.0;
+    if (n_candidates) *n_candidates = 0;
+    if (!x || !dir || !(dhat_sq > 0) || !(slackness > 0 && slackness < 1) || (n_faces > 0 && (!faces || !surf_verts)) ||
+        (n_edges > 0 && !edges))
+        return SGIPC_BAD_ARG;
+    return run(ctx, [&] {
+        if (!ctx->contact) ctx->contact = std::make_unique<ContactDetector>();
+        long nc = 0;
+        *step = ctx->contact->ccd_step(ctx->c.stream, x, dir, alpha, faces, n_faces, edges, n_edges, surf_verts,
+                                       n_surf, body_id, boundary_type, dhat_sq, slackness, &nc);
+        if (n_candidates) *n_candidates = (int)nc;
+    });
+}
+
+int sgipc_ground_step(sgipc_ctx* ctx, const double* x, const uint32_t* surf_verts, int n_surf, const double* normal,
+                      double offset, const double* dir, double slackness, double* step)
+{
+    if (!ctx || !step || n_surf < 0) return SGIPC_BAD_ARG;
+    *step = 1.0;
+    if (n_surf == 0) return SGIPC_EMPTY;
+    if (!x || !surf_verts || !normal || !dir || !(slackness > 0)) return SGIPC_BAD_ARG;
+    return run(ctx, [&] {
+        StepReducer red;
+        DevBuf<double> vals;
+        *step = ground_step(ctx->c.stream, red, vals, x, surf_verts, n_surf, normal, offset, dir, slackness);
+    });
+}
+
+int sgipc_injective_step(sgipc_ctx* ctx, const double* x, const uint32_t* tets, int n_tets, const double* dir,
+                         double slackness, double error_rate, double* step)
+{
+    if (!ctx || !step || n_tets < 0) return SGIPC_BAD_ARG;
+    *step = 1.0;
+    if (n_tets == 0) return SGIPC_EMPTY;
+    if (!x || !tets || !dir || !(slackness > 0 && slackness < 1)) return SGIPC_BAD_ARG;
+    return run(ctx, [&] {
+        StepReducer red;
+        DevBuf<double> vals;
+        *step = injective_step(ctx->c.stream, red, vals, x, tets, n_tets, dir, slackness, error_rate);
     });
 }
 
@@ -728,6 +784,47 @@ int system_solve(sgipc_system* sys, const double* b, double* x, double* grad, co
 
 }  // namespace
 
+// Objective of the Newton step at positions x (GIPC::computeEnergy,
+// StiffGIPC/GIPC.cu:7373-7870 restricted to the terms this system assembles):
+// 1/2 |x - x~|^2_M (when x_tilde is given) + the elastic terms + the barrier
+// of the pairs at x (detected there when the scene detects its contacts).
+double system_energy(sgipc_system* sys, const double* x, const double* x_tilde)
+{
+    const auto& d = sys->d;
+    hipStream_t s = sys->ctx->c.stream;
+    SGIPC_CHECK(d.n_bodies == 0, SGIPC_BAD_ARG, "the energy of affine bodies is not evaluated");
+    EnergySum& e = sys->esum;
+    double E = e.inertia(s, d.n_verts, x, x_tilde, d.mass);
+    E += e.tets(s, d.n_tets, x, d.tets, d.dm_inv, d.vol, d.mu, d.lam, d.dt);
+    E += e.tris(s, d.n_tris, x, d.tris, d.dm_inv2, d.area, d.k_stretch, d.k_shear, d.strain_rate, d.dt);
+    E += e.bending(s, d.n_hinges, x, d.rest_x, d.hinge_edges, d.hinge_adj, sys->hinge_q.get(), d.k_bend, d.dt);
+    if (d.detect_contacts) {
+        if (sys->lcap == 0) {
+            sys->lcap = sys->dcap;
+            sys->lpairs.reserve(4L * sys->lcap);
+            sys->lmat.reserve(sys->lcap);
+        }
+        ContactCounts cc;
+        for (int attempt = 0;; ++attempt) {
+            try {
+                cc = sys->det.detect(s, x, d.rest_x, d.n_verts, d.faces, d.n_faces, d.edges, d.n_edges, d.surf_verts,
+                                     d.n_surf, d.body_id, d.boundary_type, d.dhat_sq, sys->lpairs.get(),
+                                     sys->lmat.get(), sys->lcap);
+                break;
+            } catch (const Error& err) {
+                if (err.code != SGIPC_CAPACITY || attempt > 0 || sys->det.last_total <= sys->lcap) throw;
+                sys->lcap = sys->det.last_total + sys->det.last_total / 4 + 64;
+                sys->lpairs.reserve(4L * sys->lcap);
+                sys->lmat.reserve(sys->lcap);
+            }
+        }
+        E += e.barrier(s, cc.c[0], x, sys->lpairs.get(), nullptr, d.dhat_sq, d.kappa);
+    } else {
+        E += e.barrier(s, d.n_pairs, x, d.pairs, d.pair_x, d.dhat_sq, d.kappa);
+    }
+    return E;
+}
+
 extern "C" {
 
 int sgipc_system_solve(sgipc_system* sys, const double* b, double* x, double* grad, const sgipc_pcg_opts* opts,
@@ -810,6 +907,60 @@ int sgipc_system_bcoo(sgipc_system* sys, const double** vals, const int** rows, 
 }
 
 sgipc_mas* sgipc_system_mas(sgipc_system* sys) { return sys ? sys->mas : nullptr; }
+
+int sgipc_system_energy(sgipc_system* sys, const double* x, const double* x_tilde, double* energy)
+{
+    if (!sys || !x || !energy) return SGIPC_BAD_ARG;
+    return run(sys->ctx, [&] { *energy = system_energy(sys, x, x_tilde); });
+}
+
+int sgipc_system_line_search(sgipc_system* sys, const double* dir, const double* x_tilde, double alpha,
+                             double slackness, double* x_out, double* alpha_out, double* energies, int* stopped)
+{
+    if (!sys || !dir || !x_out || !alpha_out || !(alpha > 0) || !(slackness > 0 && slackness < 1)) return SGIPC_BAD_ARG;
+    return run(sys->ctx, [&] {
+        const auto& d = sys->d;
+        hipStream_t s = sys->ctx->c.stream;
+        const long n3 = 3L * d.n_verts;
+        double a = alpha;
+        // feasible step: no contact along the path, no inverted tet
+        if (d.detect_contacts) {
+            if (!sys->ctx->contact) sys->ctx->contact = std::make_unique<ContactDetector>();
+            a = std::min(a, sys->ctx->contact->ccd_step(s, d.x, dir, a, d.faces, d.n_faces, d.edges, d.n_edges,
+                                                        d.surf_verts, d.n_surf, d.body_id, d.boundary_type,
+                                                        d.dhat_sq, slackness, nullptr));
+        }
+        if (d.n_tets > 0)
+            a = std::min(a, injective_step(s, sys->stepred, sys->stepvals, d.x, d.tets, d.n_tets, dir, slackness,
+                                           1e-20));
+        const double E0 = system_energy(sys, d.x, x_tilde);
+        auto trial = [&](double t) {
+            SGIPC_HIP(hipMemcpyAsync(x_out, d.x, sizeof(double) * n3, hipMemcpyDeviceToDevice, s));
+            launch_axpy_f64(s, -t, dir, x_out, n3);
+            return system_energy(sys, x_out, x_tilde);
+        };
+        double E = trial(a);
+        int k = 0;
+        // Armijo with c1 = 0 and at most 8 halvings (GIPC.cu:10828-10920)
+        while (E > E0 && k <= 8) {
+            a *= 0.5;
+            ++k;
+            E = trial(a);
+        }
+        int st = 0;
+        if (k > 8) {
+            SGIPC_HIP(hipMemcpyAsync(x_out, d.x, sizeof(double) * n3, hipMemcpyDeviceToDevice, s));
+            SGIPC_HIP(hipStreamSynchronize(s));
+            st = 1;
+        }
+        *alpha_out = a;
+        if (energies) {
+            energies[0] = E0;
+            energies[1] = E;
+        }
+        if (stopped) *stopped = st;
+    });
+}
 
 int sgipc_system_contacts(sgipc_system* sys, const int** pairs, const int** mat_index, int* n_pairs)
 {
