@@ -429,6 +429,30 @@ int sgipc_mas_inverse_blocks(sgipc_mas* mas, double* out);
  * only); *m = 0 when the build has no exact level-2 solve. */
 int sgipc_mas_level2_inverse(sgipc_mas* mas, double* out, int* m);
 
+/* ---- CEMAS partitions (host functions, no device work) ------------------- */
+/* Slot maps of a vertex partition (set_mas_partition,
+ * app/common/sim_bootstrap.cpp:90-113; the arrays MASPreconditioner reads as
+ * d_partId_map_real / d_real_map_partId, MASPreconditioner.cu:2291-2328).
+ * part_id[n] (host): part of each vertex, >= 0.  Part p owns slots
+ * [bank p, bank p + bank) and its vertices fill them in increasing vertex
+ * order; *map_size = (max part + 1) * bank.  partId_map_real[map_size] gets
+ * the vertex of each slot (-1: empty), real_map_partId[n] the slot of each
+ * vertex (both host; both NULL: size query).  Equal to the reference's arrays
+ * for part ids in non-decreasing order (its "_sorted" meshes), which is the
+ * only order its loop handles; valid for any order here.
+ * SGIPC_CAPACITY: a part with more than bank vertices. */
+int sgipc_mas_partition_maps(const int* part_id, int n, int bank, int* map_size, int* partId_map_real,
+                             int* real_map_partId);
+/* Partition of a vertex graph into parts of 1..bank vertices (replaces the
+ * METIS k-way call of MeshProcess/metis_partition/src/metis_sort.cpp:71-110,
+ * node_edge_model.cpp:5-28, which retries with more parts until the largest
+ * fits a bank): recursive multilevel bisection into ceil(n / (bank - 1))
+ * parts, numbered in bisection-tree order (consecutive ids are close, which
+ * the coarse MAS levels rely on).  Host CSR as sgipc_mas_init_neighbor takes
+ * it (symmetric); part_id[n] (host) out; *n_parts out.  Deterministic. */
+int sgipc_partition_vertices(int n, const uint32_t* nbr_start, const uint32_t* nbr_num,
+                             const uint32_t* nbr_list, int bank, int* part_id, int* n_parts);
+
 /* ---- PCG (PCGSolver::solve, StiffGIPC/linear_system/solver/pcg_solver.cu:158-332) */
 typedef struct sgipc_pcg_opts {
     double tol;        /* relative tolerance (1e-5 for the true-residual rule)        */
@@ -571,6 +595,14 @@ typedef struct sgipc_scene_desc {
     const uint32_t* surf_verts;
     const int* body_id;         /* per vertex or NULL                   */
     int contact_capacity;
+    /* CEMAS partition (host array, per FEM vertex, or NULL; FEM-only
+     * scenes): the MAS uses the slot maps of sgipc_mas_partition_maps
+     * (app/common/sim_bootstrap.cpp:90-113, load_mesh.cpp:447-478) instead of
+     * consecutive vertex domains.  The system then solves in slot space: its
+     * BCOO (sgipc_system_bcoo) and MAS have one row per slot, the empty slots
+     * of a part being rows without blocks; b, x and grad stay per vertex.
+     * Parts without a vertex are dropped (they change no MAS level). */
+    const int* part_id;
 } sgipc_scene_desc;
 
 typedef struct sgipc_solve_timing {
@@ -591,7 +623,8 @@ int sgipc_system_destroy(sgipc_system* sys);
 int sgipc_system_solve(sgipc_system* sys, const double* b, double* x, double* grad,
                        const sgipc_pcg_opts* opts, sgipc_pcg_stats* stats,
                        sgipc_solve_timing* timing);
-/* Views of the system's current BCOO (device pointers owned by the system). */
+/* Views of the system's current BCOO (device pointers owned by the system;
+ * slot rows with a partition). */
 int sgipc_system_bcoo(sgipc_system* sys, const double** vals, const int** rows, const int** cols,
                       int* nnz);
 sgipc_mas* sgipc_system_mas(sgipc_system* sys);

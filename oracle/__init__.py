@@ -793,3 +793,6 @@ def solve_scene(scene, bank_size=32, tol=1e-5, stop_mode=1, mas_fp32=False, max_
     t2 = time.perf_counter()
     mas.close()
     return OracleSolve(x=x, iterations=it, true_rel_residual=res, seconds=t2 - t0, build_seconds=t1 - t0)
+
+
+from . import cemas  # noqa: E402,F401  (CEMAS slot-map MAS, operator form)

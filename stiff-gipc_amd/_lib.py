@@ -53,7 +53,7 @@ class SceneDesc(ctypes.Structure):
                 ("mass10", vp), ("body_volume", vp), ("body_fixed", vp), ("point_body", vp), ("x_bar", vp),
                 ("boundary_type", vp), ("motor_speed", cd), ("motor_strength", cd),
                 ("detect_contacts", ci), ("n_faces", ci), ("n_edges", ci), ("n_surf", ci), ("faces", vp), ("edges", vp),
-                ("surf_verts", vp), ("body_id", vp), ("contact_capacity", ci)]
+                ("surf_verts", vp), ("body_id", vp), ("contact_capacity", ci), ("part_id", vp)]
 
 
 class SolveTiming(ctypes.Structure):
@@ -118,6 +118,8 @@ _SIGS = {
     "sgipc_mas_tables": (ci, [vp, vp, vp]),
     "sgipc_mas_inverse_blocks": (ci, [vp, vp]),
     "sgipc_mas_level2_inverse": (ci, [vp, vp, ctypes.POINTER(ci)]),
+    "sgipc_mas_partition_maps": (ci, [vp, ci, ci, ctypes.POINTER(ci), vp, vp]),
+    "sgipc_partition_vertices": (ci, [ci, vp, vp, vp, ci, vp, ctypes.POINTER(ci)]),
     "sgipc_pcg_default_opts": (None, [ctypes.POINTER(PcgOpts)]),
     "sgipc_pcg_solve": (ci, [vp, vp, vp, vp, ci, ci, vp, vp, vp, ctypes.POINTER(PcgOpts), ctypes.POINTER(PcgStats)]),
     "sgipc_system_create": (ci, [vp, ctypes.POINTER(vp), ctypes.POINTER(SceneDesc), ci]),

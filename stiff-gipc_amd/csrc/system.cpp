@@ -4,6 +4,7 @@
 // triplet arena in the reference's order (contacts, tets, bending hinges, cloth
 // triangles, mass; StiffGIPC/GIPC.cu:10291-10485), conversion to unique upper
 // BCOO, MAS rebuild with the current contact pairs, PCG.
+#include <algorithm>
 #include <chrono>
 #include <limits>
 #include <cmath>
@@ -13,6 +14,7 @@
 #include "capi_internal.hpp"
 #include "energy.hpp"
 #include "friction.hpp"
+#include "slots.hpp"
 
 using namespace sgipc;
 
@@ -77,7 +79,91 @@ struct sgipc_system {
     long n_solves = 0;
     long next_probe = 8;     // solve index of the next measurement of the losing variant
     long probe_gap = 8;      // doubles after every probe the learned level loses
+    // CEMAS partition: the system is solved in slot space (n_slots rows,
+    // vertex i at row slot_of[i]); n_slots = 0 without a partition
+    int n_slots = 0;
+    DevBuf<int> slot_of, spairs;
+    DevBuf<double> bs, xs;
 };
+
+// sgipc_mas_partition_maps is defined in partition.cpp
+extern "C" int sgipc_mas_partition_maps(const int*, int, int, int*, int*, int*);
+
+namespace {
+
+// Slot maps of the descriptor's partition (empty parts dropped: they hold no
+// vertex and give no level-1 node, so the MAS hierarchy is the same), the
+// slot-space neighbour lists for the MAS and the slot-space vectors.  Every
+// empty slot is linked to lane 0 of its bank (a real vertex, since the bank's
+// part is not empty) and lane 0 to them: an empty slot then joins lane 0's
+// level-1 node instead of forming one, and, holding no blocks and a zero
+// residual, changes neither that node's Galerkin block nor its sums.  Empty
+// slots follow the real lanes of their bank, so no node's lowest member and
+// hence no node numbering changes.
+void init_slots(sgipc_system* sys)
+{
+    const auto& d = sys->d;
+    const int n = d.n_verts, B = sys->B;
+    std::vector<int> ids(d.part_id, d.part_id + n);
+    for (int v : ids) SGIPC_CHECK(v >= 0, SGIPC_BAD_INDEX, "negative part id");
+    std::vector<int> used(ids);
+    std::sort(used.begin(), used.end());
+    used.erase(std::unique(used.begin(), used.end()), used.end());
+    for (int& v : ids) v = (int)(std::lower_bound(used.begin(), used.end(), v) - used.begin());
+    int map_size = 0;
+    int st = sgipc_mas_partition_maps(ids.data(), n, B, &map_size, nullptr, nullptr);
+    SGIPC_CHECK(st != SGIPC_CAPACITY, SGIPC_CAPACITY, "a part holds more vertices than the bank size");
+    SGIPC_CHECK(st == SGIPC_OK, st, "invalid partition");
+    std::vector<int> p2r(map_size), r2p(n);
+    st = sgipc_mas_partition_maps(ids.data(), n, B, &map_size, p2r.data(), r2p.data());
+    SGIPC_CHECK(st == SGIPC_OK, st, "invalid partition");
+    hipStream_t s = sys->ctx->c.stream;
+    std::vector<uint32_t> hs(n), hn(n), hl(d.total_neighbor_num > 0 ? d.total_neighbor_num : 0);
+    SGIPC_HIP(hipMemcpyAsync(hs.data(), d.nbr_start, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
+    SGIPC_HIP(hipMemcpyAsync(hn.data(), d.nbr_num, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
+    if (!hl.empty())
+        SGIPC_HIP(hipMemcpyAsync(hl.data(), d.nbr_list, sizeof(uint32_t) * hl.size(), hipMemcpyDeviceToHost, s));
+    SGIPC_HIP(hipStreamSynchronize(s));
+    std::vector<uint32_t> ss(map_size), sn(map_size), sl;
+    sl.reserve(hl.size() + 2L * map_size);
+    for (int t = 0; t < map_size; ++t) {
+        ss[t] = (uint32_t)sl.size();
+        const int v = p2r[t], b0 = t - t % B;
+        if (v >= 0) {
+            for (uint32_t k = 0; k < hn[v]; ++k) {
+                SGIPC_CHECK((size_t)hs[v] + k < hl.size(), SGIPC_BAD_INDEX, "neighbour list beyond its length");
+                const uint32_t j = hl[hs[v] + k];
+                SGIPC_CHECK(j < (uint32_t)n, SGIPC_BAD_INDEX, "neighbour id beyond the vertex count");
+                sl.push_back((uint32_t)r2p[j]);
+            }
+            if (t == b0)
+                for (int u = b0 + 1; u < b0 + B; ++u)
+                    if (p2r[u] < 0) sl.push_back((uint32_t)u);
+        } else {
+            sl.push_back((uint32_t)b0);
+        }
+        sn[t] = (uint32_t)sl.size() - ss[t];
+    }
+    const int total = (int)sl.size();
+    if (sl.empty()) sl.push_back(0);
+    DevBuf<uint32_t> ds, dn, dl;
+    ds.reserve(map_size);
+    dn.reserve(map_size);
+    dl.reserve(sl.size());
+    SGIPC_HIP(hipMemcpyAsync(ds.get(), ss.data(), sizeof(uint32_t) * map_size, hipMemcpyHostToDevice, s));
+    SGIPC_HIP(hipMemcpyAsync(dn.get(), sn.data(), sizeof(uint32_t) * map_size, hipMemcpyHostToDevice, s));
+    SGIPC_HIP(hipMemcpyAsync(dl.get(), sl.data(), sizeof(uint32_t) * sl.size(), hipMemcpyHostToDevice, s));
+    sys->slot_of.reserve(n);
+    SGIPC_HIP(hipMemcpyAsync(sys->slot_of.get(), r2p.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+    sys->mas->m->init_neighbor(map_size, 0, total, ds.get(), dn.get(), dl.get());
+    SGIPC_HIP(hipStreamSynchronize(s));
+    sys->n_slots = map_size;
+    sys->n_sys = map_size;
+    sys->bs.reserve(3L * map_size);
+    sys->xs.reserve(3L * map_size);
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -429,6 +515,7 @@ int sgipc_system_create(sgipc_ctx* ctx, sgipc_system** out, const sgipc_scene_de
     if (d.n_abd_points > 0 && (!d.point_body || !d.x_bar || d.n_bodies == 0)) return SGIPC_BAD_ARG;
     if (d.n_bodies > 0 && d.n_pairs > 0 && !d.pair_x) return SGIPC_BAD_ARG;
     if (bank_size != 16 && bank_size != 32) return SGIPC_BAD_ARG;
+    if (d.part_id && d.n_bodies > 0) return SGIPC_BAD_ARG;  // partitions cover FEM-only systems
     if (d.detect_contacts) {
         if (d.n_bodies > 0 || d.n_faces < 0 || d.n_edges < 0 || d.n_surf < 0 || !(d.dhat_sq > 0)) return SGIPC_BAD_ARG;
         if ((d.n_faces > 0 && (!d.faces || !d.surf_verts)) || (d.n_edges > 0 && (!d.edges || !d.rest_x)))
@@ -443,8 +530,12 @@ int sgipc_system_create(sgipc_ctx* ctx, sgipc_system** out, const sgipc_scene_de
     if (st == SGIPC_OK)
         st = run(ctx, [&] {
             sys->n_sys = 4 * d.n_bodies + d.n_verts;
-            sys->mas->m->init_neighbor(d.n_verts, d.n_abd_points, d.total_neighbor_num, d.nbr_start, d.nbr_num,
-                                       d.nbr_list);
+            if (d.part_id) {
+                init_slots(sys);
+            } else {
+                sys->mas->m->init_neighbor(d.n_verts, d.n_abd_points, d.total_neighbor_num, d.nbr_start, d.nbr_num,
+                                           d.nbr_list);
+            }
             sys->mas->m->init_matrix();
             sys->n_trip = 10L * d.n_bodies + 10L * d.n_tets + 10L * d.n_hinges + 6L * d.n_tris + d.n_verts;
             if (d.n_hinges > 0) {
@@ -535,6 +626,9 @@ int system_solve(sgipc_system* sys, const double* b, double* x, double* grad, co
     sgipc_pcg_default_opts(&o);
     if (opts) o = *opts;
     if (o.precision < SGIPC_FP64 || o.precision > SGIPC_FP32_IR) return SGIPC_BAD_ARG;
+    const bool slots = sys->n_slots > 0;
+    // the iteration cap stays 0.3 x the vertex-space unknowns in slot space
+    if (slots && o.max_iter <= 0) o.max_iter = std::max(2, (int)(0.3 * 3.0 * (double)sys->d.n_verts));
     int status = SGIPC_OK;
     int st = run(sys->ctx, [&] {
         const auto& d = sys->d;
@@ -642,6 +736,22 @@ int system_solve(sgipc_system* sys, const double* b, double* x, double* grad, co
         launch_mass_diag(s, d.n_verts, d.mass, off0, sys->tvals.get() + 9 * off, sys->trows.get() + off,
                          sys->tcols.get() + off);
         off += d.n_verts;
+        const double* bp = b;
+        double* xp = x;
+        if (slots) {
+            // vertex rows and columns renamed to slots; contact pairs for the
+            // MAS connectivity likewise; b placed at the slots
+            launch_slot_triplets(s, off, d.n_verts, sys->slot_of.get(), sys->tvals.get(), sys->trows.get(),
+                                 sys->tcols.get());
+            if (mas_npairs > 0) {
+                sys->spairs.reserve(4L * mas_npairs);
+                launch_slot_pairs(s, mas_npairs, sys->slot_of.get(), mas_pairs, sys->spairs.get());
+                mas_pairs = sys->spairs.get();
+            }
+            launch_slot_scatter(s, d.n_verts, sys->n_slots, sys->slot_of.get(), b, sys->bs.get());
+            bp = sys->bs.get();
+            xp = sys->xs.get();
+        }
         SGIPC_HIP(hipEventRecord(sys->ev[1], s));
         // ---- triplets -> unique upper BCOO
         sys->nnz = sys->conv.convert(s, sys->tvals.get(), sys->trows.get(), sys->tcols.get(), off, sys->n_sys,
@@ -681,7 +791,7 @@ int system_solve(sgipc_system* sys, const double* b, double* x, double* grad, co
         auto run_pcg = [&](const sgipc_pcg_opts& oo) {
             try {
                 return pcg_solve_impl(sys->ctx->c, *sys->ws, sys->bvals.get(), sys->brows.get(), sys->bcols.get(),
-                                      sys->nnz, sys->n_sys, prec, b, x, oo, &pst, &extra);
+                                      sys->nnz, sys->n_sys, prec, bp, xp, oo, &pst, &extra);
             } catch (...) {
                 sys->mas->m->net_ = nullptr;
                 throw;
@@ -746,6 +856,7 @@ int system_solve(sgipc_system* sys, const double* b, double* x, double* grad, co
             pst.iterations += it32;
             fallback = 1;
         }
+        if (slots) launch_slot_gather(s, d.n_verts, sys->slot_of.get(), sys->xs.get(), x);
         pst.fp64_fallback = fallback;
         pst.learned = used ? 1 : (abandoned ? 2 : 0);
         pst.coarse_mode = mm->last_path();

@@ -354,6 +354,12 @@ def upload_scene(ctx: Context, scene):
         desc.body_id = put("c_body", surf.body_id, np.int32) if surf.body_id is not None else None
         if not desc.rest_x:
             desc.rest_x = put("rest_x", scene.rest.reshape(-1), np.float64)
+    part = getattr(scene, "part_id", None)
+    if part is not None:
+        # host array (read at system creation)
+        hp = np.ascontiguousarray(part, dtype=np.int32)
+        keep["part_id"] = hp
+        desc.part_id = hp.ctypes.data
     return desc, keep
 
 

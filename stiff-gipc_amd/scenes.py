@@ -46,6 +46,9 @@ __all__ = [
     "morton3",
     "kuhn_grid",
     "tet_grid_scene",
+    "tet_mesh_scene",
+    "cloth_mesh_scene",
+    "cloth_rest_dm",
     "scene_c0",
     "scene_c1",
     "scene_tiny",
@@ -77,6 +80,7 @@ class TetScene:
     rhs: np.ndarray          # (3n,) float64 right-hand side
     seeds: dict
     boundary_type: np.ndarray | None = None  # (n,) int32, nonzero = Dirichlet vertex (TetMesh.BoundaryType)
+    part_id: np.ndarray | None = None  # (n,) int32 CEMAS partition (MAS slot maps), or None
 
     @property
     def n(self) -> int:
@@ -145,6 +149,7 @@ class ClothScene:
     boundary_type: np.ndarray | None = None  # (n,) int32, nonzero = Dirichlet vertex
     # surface whose contacts are detected on every solve (pairs then unused)
     contact_surface: "ContactGeometry | None" = None
+    part_id: np.ndarray | None = None  # (n,) int32 CEMAS partition (MAS slot maps), or None
 
     @property
     def n(self) -> int:
@@ -326,11 +331,17 @@ def tet_grid_scene(name: str, N: int, *, spacing=0.01, jitter=0.1, E=1e5, E_logu
     relabel[order] = np.arange(order.size)            # old id -> new id
     lat = lat[order]
     tets = relabel[tets]
-    n = lat.shape[0]
     rest = lat.astype(np.float64) * spacing
     rng = np.random.default_rng(seed_jitter)
     x = rest + rng.uniform(-jitter * spacing, jitter * spacing, size=rest.shape)
+    return _tet_scene(name, rest, x, tets, E=E, E_loguniform=E_loguniform, nu=nu, density=density, dt=dt,
+                      n_pairs=n_pairs, dhat=dhat, kappa=kappa, seed_rhs=seed_rhs, seed_E=seed_E,
+                      seed_pairs=seed_pairs, seeds=dict(jitter=seed_jitter, rhs=seed_rhs, E=seed_E, pairs=seed_pairs))
 
+
+def _tet_scene(name, rest, x, tets, *, E, E_loguniform, nu, density, dt, n_pairs, dhat, kappa, seed_rhs, seed_E,
+               seed_pairs, seeds, part_id=None) -> TetScene:
+    n = rest.shape[0]
     Dm = np.stack([rest[tets[:, 1]] - rest[tets[:, 0]],
                    rest[tets[:, 2]] - rest[tets[:, 0]],
                    rest[tets[:, 3]] - rest[tets[:, 0]]], axis=2)   # columns
@@ -359,7 +370,35 @@ def tet_grid_scene(name: str, N: int, *, spacing=0.01, jitter=0.1, E=1e5, E_logu
                     dm_inv=np.ascontiguousarray(dm_inv), vol=vol, mu=np.ascontiguousarray(mu),
                     lam=np.ascontiguousarray(lam), mass=mass, dt=dt, nbr_start=start, nbr_num=num,
                     nbr_list=lst, pairs=pairs, pair_x=pair_x, dhat=dhat, kappa=kappa, rhs=rhs,
-                    seeds=dict(jitter=seed_jitter, rhs=seed_rhs, E=seed_E, pairs=seed_pairs))
+                    seeds=seeds, part_id=part_id)
+
+
+def _mean_edge(x: np.ndarray, el: np.ndarray) -> float:
+    k = el.shape[1]
+    return float(np.mean([np.linalg.norm(x[el[:, i]] - x[el[:, (i + 1) % k]], axis=1).mean() for i in range(k)]))
+
+
+def tet_mesh_scene(name: str, rest: np.ndarray, tets: np.ndarray, *, part_id=None, jitter=0.1, E=1e5,
+                   E_loguniform=None, nu=0.45, density=1e3, dt=0.01, n_pairs=0, dhat=None, kappa=1e6,
+                   seed_jitter=0, seed_rhs=1, seed_E=2, seed_pairs=3) -> TetScene:
+    """A loaded tet mesh at rest, jittered by a seeded uniform offset of
+    ``jitter`` times its mean edge length; negatively oriented tets get their
+    last two vertices swapped.  ``part_id``: the CEMAS partition (per vertex)
+    the system uses for its MAS slot maps, or None."""
+    rest = np.ascontiguousarray(rest, dtype=np.float64)
+    tets = np.asarray(tets, dtype=np.int64).copy()
+    d = np.stack([rest[tets[:, i]] - rest[tets[:, 0]] for i in (1, 2, 3)], axis=2)
+    neg = np.linalg.det(d) < 0
+    tets[neg, 2], tets[neg, 3] = tets[neg, 3].copy(), tets[neg, 2].copy()
+    h = _mean_edge(rest, tets)
+    rng = np.random.default_rng(seed_jitter)
+    x = rest + rng.uniform(-jitter * h, jitter * h, size=rest.shape)
+    if dhat is None:
+        dhat = 0.1 * h
+    return _tet_scene(name, rest, x, tets, E=E, E_loguniform=E_loguniform, nu=nu, density=density, dt=dt,
+                      n_pairs=n_pairs, dhat=dhat, kappa=kappa, seed_rhs=seed_rhs, seed_E=seed_E,
+                      seed_pairs=seed_pairs, seeds=dict(jitter=seed_jitter, rhs=seed_rhs, E=seed_E, pairs=seed_pairs),
+                      part_id=None if part_id is None else np.ascontiguousarray(part_id, dtype=np.int32))
 
 
 def scene_c0() -> TetScene:
@@ -460,6 +499,74 @@ def cloth_grid_scene(name: str, N: int, *, spacing=1e-3, jitter=0.1, E=1e6, nu=0
                       strain_rate=strain_rate, edges=edges, adj=adj, k_bend=k_bend, mass=mass, dt=dt,
                       nbr_start=start, nbr_num=num, nbr_list=lst, pairs=pairs, pair_x=pair_x, dhat=dhat,
                       kappa=kappa, rhs=rhs, seeds=dict(jitter=seed_jitter, rhs=seed_rhs, pairs=seed_pairs))
+
+
+def cloth_rest_dm(rest: np.ndarray, tris: np.ndarray) -> np.ndarray:
+    """2x2 rest edge matrices [u0 u1] of each triangle after the rotation
+    that takes its unit normal n onto +z (``__calculateDm2D_double``,
+    StiffGIPC/femEnergy.cu:256-300): R = I + [v]x + [v]x^2 / (1 + c) with
+    v = n x z, c = n.z, and diag(-1, -1, 1) when n = -z; the rotated corners'
+    xy coordinates give u0 = p1 - p0, u1 = p2 - p0."""
+    a, b, c = rest[tris[:, 0]], rest[tris[:, 1]], rest[tris[:, 2]]
+    nrm = np.cross(b - a, c - a)
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    z = np.array([0.0, 0.0, 1.0])
+    v = np.cross(nrm, z)
+    cs = nrm @ z
+    m = tris.shape[0]
+    K = np.zeros((m, 3, 3))
+    K[:, 0, 1], K[:, 0, 2] = -v[:, 2], v[:, 1]
+    K[:, 1, 0], K[:, 1, 2] = v[:, 2], -v[:, 0]
+    K[:, 2, 0], K[:, 2, 1] = -v[:, 1], v[:, 0]
+    R = np.broadcast_to(np.eye(3), (m, 3, 3)).copy()
+    flip = cs + 1 == 0
+    ok = ~flip
+    R[ok] = R[ok] + K[ok] + (K[ok] @ K[ok]) / (1.0 + cs[ok])[:, None, None]
+    R[flip, 0, 0] = -1.0
+    R[flip, 1, 1] = -1.0
+    p0 = np.einsum("mij,mj->mi", R, a)[:, :2]
+    p1 = np.einsum("mij,mj->mi", R, b)[:, :2]
+    p2 = np.einsum("mij,mj->mi", R, c)[:, :2]
+    return np.stack([p1 - p0, p2 - p0], axis=2)
+
+
+def cloth_mesh_scene(name: str, rest: np.ndarray, tris: np.ndarray, *, part_id=None, jitter=0.1, E=1e6, nu=0.49,
+                     thickness=1e-3, density=2e2, strain_rate=100.0, k_bend=1e-4, drape_radius=1.0, dt=0.01,
+                     seed_jitter=0, seed_rhs=1) -> ClothScene:
+    """A loaded triangle mesh at rest (rest frames per ``cloth_rest_dm``),
+    currently jittered in place by a seeded uniform offset of ``jitter``
+    times its mean edge length and bent along its normal direction of least
+    extent into a bowl of radius drape_radius * its largest extent.
+    ``part_id``: the CEMAS partition (per vertex), or None."""
+    rest = np.ascontiguousarray(rest, dtype=np.float64)
+    tris = np.asarray(tris, dtype=np.int32)
+    n = rest.shape[0]
+    h = _mean_edge(rest, tris)
+    rng = np.random.default_rng(seed_jitter)
+    x = rest + rng.uniform(-jitter * h, jitter * h, size=rest.shape)
+    ext = rest.max(axis=0) - rest.min(axis=0)
+    ax = int(np.argmin(ext))
+    ctr = 0.5 * (rest.max(axis=0) + rest.min(axis=0))
+    uv = [i for i in range(3) if i != ax]
+    R = drape_radius * float(ext.max())
+    x[:, ax] = rest[:, ax] - ((x[:, uv[0]] - ctr[uv[0]]) ** 2 + (x[:, uv[1]] - ctr[uv[1]]) ** 2) / (2.0 * R)
+    Dm = cloth_rest_dm(rest, tris)
+    detm = np.linalg.det(Dm)
+    assert (detm > 0).all()
+    dm_inv2 = np.linalg.inv(Dm)
+    area = 0.5 * np.abs(detm) * thickness
+    mass = np.zeros(n)
+    np.add.at(mass, tris.ravel(), np.repeat(density * area / 3.0, 3))
+    k_stretch = E / (2.0 * (1.0 + nu))
+    k_shear = 0.03 * k_stretch * strain_rate
+    edges, adj, start, num, lst = _tri_edges_csr(n, tris)
+    rhs = np.random.default_rng(seed_rhs).standard_normal(3 * n)
+    return ClothScene(name=name, x=np.ascontiguousarray(x), rest=rest, tris=tris,
+                      dm_inv2=np.ascontiguousarray(dm_inv2), area=area, k_stretch=k_stretch, k_shear=k_shear,
+                      strain_rate=strain_rate, edges=edges, adj=adj, k_bend=k_bend, mass=mass, dt=dt,
+                      nbr_start=start, nbr_num=num, nbr_list=lst, pairs=np.zeros((0, 4), dtype=np.int32),
+                      pair_x=None, dhat=h, kappa=0.0, rhs=rhs, seeds=dict(jitter=seed_jitter, rhs=seed_rhs),
+                      part_id=None if part_id is None else np.ascontiguousarray(part_id, dtype=np.int32))
 
 
 @dataclasses.dataclass
