@@ -57,6 +57,7 @@ def lib():
         L.orc_mas_create.argtypes = [ctypes.c_int, ctypes.c_int, c_uint_p, c_uint_p, c_uint_p, ctypes.c_int]
         L.orc_mas_create.restype = ctypes.c_void_p
         L.orc_mas_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_mas_build.argtypes = [ctypes.c_void_p, ctypes.c_long, c_dbl_p, c_int_p, c_int_p,
                                     ctypes.c_int, c_int_p, ctypes.c_int]
         L.orc_mas_apply.argtypes = [ctypes.c_void_p, c_dbl_p, c_dbl_p, ctypes.c_int]
@@ -138,6 +139,11 @@ def lib():
         L.orc_energy_bending.restype = ctypes.c_double
         _LIB = L
     return _LIB
+
+
+def set_threads(k: int):
+    """OpenMP threads of the oracle's parallel loops."""
+    lib().orc_set_threads(int(k))
 
 
 def set_fault(k: int):
@@ -645,6 +651,40 @@ def assemble(scene):
     """Unique upper BCOO (vals (nnz,9) column-major blocks, rows, cols)."""
     v, r, c, _ = triplets(scene)
     return convert(v, r, c)
+
+
+def assemble_arena(scene):
+    """``assemble`` for tet scenes with seeded pairs, no bodies and no
+    Dirichlet vertices, writing every element's triplets straight into one
+    preallocated arena in the same order (contacts, tets, mass) instead of
+    concatenating per-kind arrays: the CPU baseline's assembly."""
+    assert getattr(scene, "n_bodies", 0) == 0 and getattr(scene, "contact_surface", None) is None
+    assert _btype(scene) is None and getattr(scene, "n_tris", 0) == 0 and getattr(scene, "n_hinges", 0) == 0
+    L = lib()
+    p, m, n = scene.n_pairs, scene.n_tets, scene.n
+    tot = 16 * p + 10 * m + n
+    vals = np.empty((tot, 9))
+    rows = np.empty(tot, dtype=np.int32)
+    cols = np.empty(tot, dtype=np.int32)
+    x = np.ascontiguousarray(scene.x, dtype=np.float64)
+
+    def at(a, k, t):
+        return ctypes.cast(a.ctypes.data + k * a.strides[0], t)
+
+    if p:
+        pairs = np.ascontiguousarray(scene.pairs, dtype=np.int32)
+        px = None if scene.pair_x is None else np.ascontiguousarray(scene.pair_x, dtype=np.float64)
+        L.orc_barrier_pt(p, _ptr(x, c_dbl_p), _ptr(pairs, c_int_p), _ptr(px, c_dbl_p), scene.dhat ** 2, scene.kappa,
+                         0, None, at(vals, 0, c_dbl_p), at(rows, 0, c_int_p), at(cols, 0, c_int_p))
+    tets = np.ascontiguousarray(scene.tets, dtype=np.int32)
+    k = 16 * p
+    L.orc_tet_snh(m, _ptr(x, c_dbl_p), _ptr(tets, c_int_p), _ptr(scene.dm_inv, c_dbl_p), _ptr(scene.vol, c_dbl_p),
+                  _ptr(scene.mu, c_dbl_p), _ptr(scene.lam, c_dbl_p), scene.dt, 0, None, at(vals, k, c_dbl_p),
+                  at(rows, k, c_int_p), at(cols, k, c_int_p))
+    k += 10 * m
+    L.orc_mass_diag(n, _ptr(np.ascontiguousarray(scene.mass), c_dbl_p), 0, at(vals, k, c_dbl_p),
+                    at(rows, k, c_int_p), at(cols, k, c_int_p))
+    return convert(vals, rows, cols)
 
 
 def spmv(a, vals, rows, cols, x, b, y, n_rows):

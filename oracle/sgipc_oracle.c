@@ -969,18 +969,64 @@ int orc_mass_diag(int n, const double* mass, int hess_offset, double* tvals, int
     return 0;
 }
 
+/* OpenMP threads of the parallel loops (the CPU baseline's thread count).   */
+void orc_set_threads(int k)
+{
+#ifdef _OPENMP
+    if (k > 0) omp_set_num_threads(k);
+#else
+    (void)k;
+#endif
+}
+
 /* ------------------------------------------------------------------------- */
 /* Converter::srbk_convert -- StiffGIPC/linear_system/utils/converter.cu:32-51, */
-/* 88-131, 171-238: radix sort on (row<<32)+col, reduce by key.               */
+/* 88-131, 171-238: radix sort on (row, col), reduce by key in sorted order.  */
 /* ------------------------------------------------------------------------- */
-static const uint64_t* g_sort_keys;
-static int cmp_idx(const void* a, const void* b)
+/* Stable LSD radix sort of the triplet indices by key (8-bit digits over the
+ * key's used bits; per-thread histograms, prefix in thread order so ties
+ * keep index order): the order of a stable sort by (key, index). */
+static void radix_sort_idx(long n, const uint64_t* keys, long* idx)
 {
-    long ia = *(const long*)a, ib = *(const long*)b;
-    uint64_t ka = g_sort_keys[ia], kb = g_sort_keys[ib];
-    if (ka < kb) return -1;
-    if (ka > kb) return 1;
-    return ia < ib ? -1 : (ia > ib);
+    uint64_t mx = 0;
+    for (long i = 0; i < n; ++i)
+        if (keys[i] > mx) mx = keys[i];
+    int bits = 0;
+    while (bits < 64 && (mx >> bits) != 0) ++bits;
+    long* tmp = (long*)malloc(sizeof(long) * n);
+    int nt = 1;
+#ifdef _OPENMP
+    nt = omp_get_max_threads();
+#endif
+    long* hist = (long*)malloc(sizeof(long) * 256 * (size_t)nt);
+    for (int sh = 0; sh < bits; sh += 8) {
+#pragma omp parallel num_threads(nt)
+        {
+            int t = 0;
+#ifdef _OPENMP
+            t = omp_get_thread_num();
+#endif
+            long lo = n * t / nt, hi = n * (t + 1) / nt;
+            long* h = hist + 256 * (size_t)t;
+            for (int d = 0; d < 256; ++d) h[d] = 0;
+            for (long i = lo; i < hi; ++i) ++h[(keys[idx[i]] >> sh) & 255];
+#pragma omp barrier
+#pragma omp single
+            {
+                long acc = 0;
+                for (int d = 0; d < 256; ++d)
+                    for (int q = 0; q < nt; ++q) {
+                        long c = hist[256 * (size_t)q + d];
+                        hist[256 * (size_t)q + d] = acc;
+                        acc += c;
+                    }
+            }
+            for (long i = lo; i < hi; ++i) tmp[h[(keys[idx[i]] >> sh) & 255]++] = idx[i];
+        }
+        memcpy(idx, tmp, sizeof(long) * n);
+    }
+    free(hist);
+    free(tmp);
 }
 
 long orc_convert(long n_trip, const double* tvals, const int* trows, const int* tcols,
@@ -989,28 +1035,38 @@ long orc_convert(long n_trip, const double* tvals, const int* trows, const int* 
     if (n_trip <= 0) return 0;
     uint64_t* keys = (uint64_t*)malloc(sizeof(uint64_t) * n_trip);
     long* idx = (long*)malloc(sizeof(long) * n_trip);
+    uint32_t mc = 0;
+    for (long i = 0; i < n_trip; ++i)
+        if ((uint32_t)tcols[i] > mc) mc = (uint32_t)tcols[i];
+    int cb = 0;
+    while (cb < 32 && (mc >> cb) != 0) ++cb;
+    /* (row, col) order: row above the column's bits */
+#pragma omp parallel for schedule(static)
     for (long i = 0; i < n_trip; ++i) {
-        keys[i] = ((uint64_t)(uint32_t)trows[i] << 32) + (uint64_t)(uint32_t)tcols[i];
+        keys[i] = ((uint64_t)(uint32_t)trows[i] << cb) + (uint64_t)(uint32_t)tcols[i];
         idx[i] = i;
     }
-    g_sort_keys = keys;
-    qsort(idx, (size_t)n_trip, sizeof(long), cmp_idx);
-    long u = -1;
-    uint64_t prev = 0;
-    for (long s = 0; s < n_trip; ++s) {
-        long i = idx[s];
-        if (u < 0 || keys[i] != prev) {
-            ++u;
-            prev = keys[i];
-            orows[u] = trows[i];
-            ocols[u] = tcols[i];
-            for (int e = 0; e < 9; ++e) ovals[9 * u + e] = 0.0;
-        }
-        for (int e = 0; e < 9; ++e) ovals[9 * u + e] += tvals[9 * i + e];
+    radix_sort_idx(n_trip, keys, idx);
+    /* unique keys: heads of the sorted runs, then each run summed in order */
+    long* head = (long*)malloc(sizeof(long) * (n_trip + 1));
+    long u = 0;
+    for (long s = 0; s < n_trip; ++s)
+        if (s == 0 || keys[idx[s]] != keys[idx[s - 1]]) head[u++] = s;
+    head[u] = n_trip;
+#pragma omp parallel for schedule(static)
+    for (long q = 0; q < u; ++q) {
+        long i0 = idx[head[q]];
+        orows[q] = trows[i0];
+        ocols[q] = tcols[i0];
+        double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (long s = head[q]; s < head[q + 1]; ++s)
+            for (int e = 0; e < 9; ++e) acc[e] += tvals[9 * idx[s] + e];
+        for (int e = 0; e < 9; ++e) ovals[9 * q + e] = acc[e];
     }
+    free(head);
     free(keys);
     free(idx);
-    return u + 1;
+    return u;
 }
 
 /* ------------------------------------------------------------------------- */
@@ -1086,6 +1142,8 @@ struct orc_mas {
     double* Mgal;    /* [domains * nblk * 9] */
     double* Minv;
     float* Minv32;
+    int* cptr;       /* [total + 1]: vertices of every node of every level, */
+    int* clist;      /* ascending (the restriction's gather lists)           */
 };
 
 int orc_mas_num_levels_for(int n, int B)
@@ -1133,6 +1191,7 @@ void orc_mas_destroy(orc_mas* m)
     if (!m) return;
     free(m->nbr_start); free(m->nbr_num); free(m->nbr_list); free(m->map);
     free(m->going_next); free(m->Mgal); free(m->Minv); free(m->Minv32);
+    free(m->cptr); free(m->clist);
     free(m);
 }
 
@@ -1288,6 +1347,20 @@ int orc_mas_build(orc_mas* m, long nnz, const double* vals, const int* rows, con
     int n = m->n, B = m->B, L = m->L, nb = m->nblk;
     mas_hierarchy(m, pairs, n_pairs);
     int ndom = m->total / B;
+    /* gather lists of the restriction: node g = off_l + map_l[v] */
+    free(m->cptr); free(m->clist);
+    m->cptr = (int*)calloc((size_t)m->total + 1, sizeof(int));
+    m->clist = (int*)malloc(sizeof(int) * (size_t)n * L);
+    for (int l = 0; l < L; ++l)
+        for (int v = 0; v < n; ++v) m->cptr[m->offsets[l] + m->map[(size_t)l * n + v] + 1]++;
+    for (int g = 0; g < m->total; ++g) m->cptr[g + 1] += m->cptr[g];
+    {
+        int* fill = (int*)malloc(sizeof(int) * (size_t)m->total);
+        memcpy(fill, m->cptr, sizeof(int) * (size_t)m->total);
+        for (int l = 0; l < L; ++l)
+            for (int v = 0; v < n; ++v) m->clist[fill[m->offsets[l] + m->map[(size_t)l * n + v]]++] = v;
+        free(fill);
+    }
     free(m->Mgal); free(m->Minv); free(m->Minv32);
     m->Mgal = (double*)calloc((size_t)ndom * nb * 9, sizeof(double));
     m->Minv = (double*)calloc((size_t)ndom * nb * 9, sizeof(double));
@@ -1362,11 +1435,10 @@ void orc_mas_apply(orc_mas* m, const double* r, double* z, int fp32)
     if (!fp32) {
         double* R = (double*)calloc((size_t)tot * 3, sizeof(double));
         double* Z = (double*)calloc((size_t)tot * 3, sizeof(double));
-        for (int l = 0; l < L; ++l) {
-            const int* ml = m->map + (size_t)l * n;
-            for (int v = 0; v < n; ++v)
-                for (int d = 0; d < 3; ++d) R[3 * (size_t)(m->offsets[l] + ml[v]) + d] += r[3 * (size_t)v + d];
-        }
+#pragma omp parallel for schedule(static)
+        for (int g = 0; g < tot; ++g)
+            for (int e = m->cptr[g]; e < m->cptr[g + 1]; ++e)
+                for (int d = 0; d < 3; ++d) R[3 * (size_t)g + d] += r[3 * (size_t)m->clist[e] + d];
 #pragma omp parallel for schedule(static)
         for (int dm = 0; dm < ndom; ++dm) {
             const double* M = m->Minv + (size_t)dm * nb * 9;
@@ -1396,11 +1468,10 @@ void orc_mas_apply(orc_mas* m, const double* r, double* z, int fp32)
     } else {
         float* R = (float*)calloc((size_t)tot * 3, sizeof(float));
         float* Z = (float*)calloc((size_t)tot * 3, sizeof(float));
-        for (int l = 0; l < L; ++l) {
-            const int* ml = m->map + (size_t)l * n;
-            for (int v = 0; v < n; ++v)
-                for (int d = 0; d < 3; ++d) R[3 * (size_t)(m->offsets[l] + ml[v]) + d] += (float)r[3 * (size_t)v + d];
-        }
+#pragma omp parallel for schedule(static)
+        for (int g = 0; g < tot; ++g)
+            for (int e = m->cptr[g]; e < m->cptr[g + 1]; ++e)
+                for (int d = 0; d < 3; ++d) R[3 * (size_t)g + d] += (float)r[3 * (size_t)m->clist[e] + d];
 #pragma omp parallel for schedule(static)
         for (int dm = 0; dm < ndom; ++dm) {
             const float* M = m->Minv32 + (size_t)dm * nb * 9;
@@ -1493,6 +1564,78 @@ static void precond_apply(const orc_precond* pc, long n3, const double* r, doubl
                                pc->mas_fp32);
 }
 
+/* The full symmetric matrix as block rows (upper blocks and the transposes
+ * of the off-diagonal ones, each row's blocks in column order), built once
+ * per solve so that the PCG's products run row-parallel without private
+ * output copies: y_i = a sum_j A_ij x_j, each row summed in column order. */
+typedef struct {
+    int n;
+    long* ptr;
+    int* col;
+    double* val; /* column-major 3x3 per entry */
+} orc_bsr;
+
+static void bsr_build(orc_bsr* m, long nnz, const double* vals, const int* rows, const int* cols, int n)
+{
+    m->n = n;
+    m->ptr = (long*)calloc((size_t)n + 1, sizeof(long));
+    for (long k = 0; k < nnz; ++k) {
+        m->ptr[rows[k] + 1]++;
+        if (rows[k] != cols[k]) m->ptr[cols[k] + 1]++;
+    }
+    for (int i = 0; i < n; ++i) m->ptr[i + 1] += m->ptr[i];
+    long tot = m->ptr[n];
+    m->col = (int*)malloc(sizeof(int) * (size_t)(tot > 0 ? tot : 1));
+    m->val = (double*)malloc(sizeof(double) * 9 * (size_t)(tot > 0 ? tot : 1));
+    long* fill = (long*)malloc(sizeof(long) * ((size_t)n + 1));
+    memcpy(fill, m->ptr, sizeof(long) * ((size_t)n + 1));
+    /* rows sorted, upper (row <= col): for row i the transposed entries
+     * (col i, from rows r < i) come first in r order, then its upper blocks in
+     * column order, so every row ends up in column order */
+    for (long k = 0; k < nnz; ++k) {
+        int i = rows[k], j = cols[k];
+        if (i != j) {
+            long e = fill[j]++;
+            m->col[e] = i;
+            const double* H = vals + 9 * k;
+            double* T = m->val + 9 * e;
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) T[c * 3 + r] = H[r * 3 + c];
+        }
+    }
+    for (long k = 0; k < nnz; ++k) {
+        long e = fill[rows[k]]++;
+        m->col[e] = cols[k];
+        memcpy(m->val + 9 * e, vals + 9 * k, sizeof(double) * 9);
+    }
+    free(fill);
+}
+
+static void bsr_free(orc_bsr* m)
+{
+    free(m->ptr);
+    free(m->col);
+    free(m->val);
+}
+
+static void bsr_spmv(const orc_bsr* m, const double* x, double* y)
+{
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < m->n; ++i) {
+        double a0 = 0, a1 = 0, a2 = 0;
+        for (long e = m->ptr[i]; e < m->ptr[i + 1]; ++e) {
+            const double* H = m->val + 9 * e;
+            const double* xj = x + 3L * m->col[e];
+            a0 += H[0] * xj[0] + H[3] * xj[1] + H[6] * xj[2];
+            a1 += H[1] * xj[0] + H[4] * xj[1] + H[7] * xj[2];
+            a2 += H[2] * xj[0] + H[5] * xj[1] + H[8] * xj[2];
+        }
+        y[3L * i] = a0;
+        y[3L * i + 1] = a1;
+        y[3L * i + 2] = a2;
+    }
+}
+
 int orc_pcg2(long nnz, const double* vals, const int* rows, const int* cols, int n_rows, const orc_precond* pc,
              const double* b, double* x, double tol, int stop_mode, int max_iter, double* stats)
 {
@@ -1501,6 +1644,8 @@ int orc_pcg2(long nnz, const double* vals, const int* rows, const int* cols, int
     double* z = (double*)malloc(sizeof(double) * n3);
     double* p = (double*)malloc(sizeof(double) * n3);
     double* Ap = (double*)malloc(sizeof(double) * n3);
+    orc_bsr A;
+    bsr_build(&A, nnz, vals, rows, cols, n_rows);
     for (long i = 0; i < n3; ++i) { x[i] = 0.0; r[i] = b[i]; }
     double bnorm = sqrt(dot(n3, b, b));
     int k = 0, iters = 0;
@@ -1511,7 +1656,7 @@ int orc_pcg2(long nnz, const double* vals, const int* rows, const int* cols, int
         memcpy(p, z, sizeof(double) * n3);
         rz0 = rz;
         for (k = 1; k < max_iter; ++k) {
-            orc_spmv(1.0, nnz, vals, rows, cols, p, 0.0, Ap, n_rows);
+            bsr_spmv(&A, p, Ap);
             double alpha = rz / dot(n3, p, Ap);
 #pragma omp parallel for schedule(static)
             for (long i = 0; i < n3; ++i) {
@@ -1533,7 +1678,7 @@ int orc_pcg2(long nnz, const double* vals, const int* rows, const int* cols, int
         }
     }
     if (stats) {
-        orc_spmv(1.0, nnz, vals, rows, cols, x, 0.0, Ap, n_rows);
+        bsr_spmv(&A, x, Ap);
         double rr = 0;
         for (long i = 0; i < n3; ++i) rr += (b[i] - Ap[i]) * (b[i] - Ap[i]);
         stats[0] = iters;
@@ -1542,6 +1687,7 @@ int orc_pcg2(long nnz, const double* vals, const int* rows, const int* cols, int
         stats[3] = rz0;
     }
     free(r); free(z); free(p); free(Ap);
+    bsr_free(&A);
     return k;
 }
 

@@ -232,6 +232,8 @@ double orc_energy_tris(int n, const double* x, const int* tris, const double* dm
 double orc_energy_bending(int n, const double* x, const double* rest_x, const int* edges, const int* adj,
                           const double* Q16, double k_bend, double dt);
 void orc_set_fault(int fault);
+/* OpenMP threads of the parallel loops (k > 0). */
+void orc_set_threads(int k);
 int orc_get_fault(void);
 
 #ifdef __cplusplus
