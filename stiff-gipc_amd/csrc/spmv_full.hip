@@ -34,6 +34,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include <hip/hip_ext.h>
@@ -225,8 +226,31 @@ __global__ __launch_bounds__(kThreads) void k_spmv_tsym(const TM* __restrict__ v
 // Tiles run in `order` (heaviest first inside each XCD's range), so the
 // lightest tiles fill the second round of workgroups.
 // Dynamic LDS: tl[3 tcap], piece[3 * 64 kPieces], xl[3 (kTile + hcap)] of TV.
+// Launch shape and LDS element type of the halo kernel per (matrix, vector)
+// type: the FP64 kernel keeps exact FP64 copies in LDS (one 1024-thread
+// workgroup per CU); the kernels with FP32 values keep FP32 copies (the
+// vector and the in-tile transposed products rounded to FP32, accumulation
+// in the vector type) in 512-thread workgroups, two of which fit a CU's LDS
+// and registers, so one workgroup's staging and reduction overlap the other's
+// stream.
+template <typename TM, typename TV>
+struct HaloCfg {
+    static constexpr int NT = 1024;
+    using TL = TV;
+};
+template <>
+struct HaloCfg<float, double> {
+    static constexpr int NT = 512;
+    using TL = float;
+};
+template <>
+struct HaloCfg<float, float> {
+    static constexpr int NT = 512;
+    using TL = float;
+};
+
 template <typename TM, typename TV, bool kDot, bool kFuseP, bool kSum1 = false>
-__global__ __launch_bounds__(kThreads) void k_spmv_halo(const TM* __restrict__ vals, const uint32_t* __restrict__ cw,
+__global__ __launch_bounds__((HaloCfg<TM, TV>::NT), 4) void k_spmv_halo(const TM* __restrict__ vals, const uint32_t* __restrict__ cw,
                                                         const int* __restrict__ slice_ptr,
                                                         const int* __restrict__ perm, const int* __restrict__ plen,
                                                         const uint32_t* __restrict__ tinfo,
@@ -240,10 +264,15 @@ __global__ __launch_bounds__(kThreads) void k_spmv_halo(const TM* __restrict__ v
                                                         const double* __restrict__ rzo, int n_rz, int n_rows,
                                                         Level1Sums l1)
 {
+    using TL = typename HaloCfg<TM, TV>::TL;
+    constexpr int kThreads = HaloCfg<TM, TV>::NT;
+    constexpr int kWaves = kThreads / 64;
+    constexpr int kPieces = kWaves + kSlices - 1;
+    constexpr bool kExactLds = std::is_same<TL, TV>::value;
     extern __shared__ __align__(16) unsigned char lds_dyn[];
-    TV* const tl = reinterpret_cast<TV*>(lds_dyn);
-    TV* const piece = tl + 3 * tcap;
-    TV* const xl = piece + kPieces * 3 * 64;
+    TL* const tl = reinterpret_cast<TL*>(lds_dyn);
+    TL* const piece = tl + 3 * tcap;
+    TL* const xl = piece + kPieces * 3 * 64;
     __shared__ int s_len[kTile];
     __shared__ int s_row[kTile];  // row of each tile position, relative to the tile (-1: padding)
     __shared__ int s_sb[kSlices + 1];
@@ -263,56 +292,60 @@ __global__ __launch_bounds__(kThreads) void k_spmv_halo(const TM* __restrict__ v
     const long A = slice_ptr[q0];
     const int tot = (int)((long)slice_ptr[q0 + kSlices] - A);
     uint32_t ti_r = 0;
-    {
-        // stage the tile's own rows, its halo, its row table and slice bounds
-        const long e3 = 3L * n_rows;
-        for (int t = threadIdx.x; t < 3 * kTile; t += kThreads) xl[t] = b3 + t < e3 ? gx(b3 + t) : TV(0);
-        const int h0 = halo_ptr[tile], hn = halo_ptr[tile + 1] - h0;
-        for (int k = threadIdx.x; k < hn; k += kThreads) {
-            const long j = halo_list[h0 + k];
-            TV* d = xl + 3 * (kTile + k);
-            d[0] = gx(3 * j);
-            d[1] = gx(3 * j + 1);
-            d[2] = gx(3 * j + 2);
-        }
-        if (threadIdx.x < kTile) {
-            const int pr = perm[q0 * 64 + threadIdx.x];
-            s_row[threadIdx.x] = pr >= 0 ? pr - tile * kTile : -1;
-            s_len[threadIdx.x] = plen[q0 * 64 + threadIdx.x];
-            if (pr >= 0) ti_r = tinfo[pr];
-        }
-        if (threadIdx.x <= kSlices) s_sb[threadIdx.x] = (int)((long)slice_ptr[q0 + threadIdx.x] - A);
-        __syncthreads();
+    // row table and slice bounds first, so the first matrix loads can go out
+    // before the vector is staged (its loads then overlap theirs)
+    if (threadIdx.x < kTile) {
+        const int pr = perm[q0 * 64 + threadIdx.x];
+        s_row[threadIdx.x] = pr >= 0 ? pr - tile * kTile : -1;
+        s_len[threadIdx.x] = plen[q0 * 64 + threadIdx.x];
+        if (pr >= 0) ti_r = tinfo[pr];
     }
+    if (threadIdx.x <= kSlices) s_sb[threadIdx.x] = (int)((long)slice_ptr[q0 + threadIdx.x] - A);
+    __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int w0 = (int)((long)tot * wave / kWaves), w1 = (int)((long)tot * (wave + 1) / kWaves);
     const uint32_t* cp = cw + A * 64;
     const TM* vp = vals + A * 9 * 64;
+    // the loads of the step at local slot ps of slice sl (matrix stream read
+    // once: non-temporal); slots past the wave's range repeat slot ps, masked
+    auto issue = [&](int ps, int sl, uint32_t* W, TM (*H)[9], bool* act) {
+        const int send = min(w1, s_sb[sl + 1]);
+        const int len = s_len[sl * 64 + lane];
+        const int s0 = ps - s_sb[sl];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const bool in = ps + u < send;
+            act[u] = in && s0 + u < len;
+            const int l = act[u] ? lane : 0;
+            const long g = in ? ps + u : ps;
+            W[u] = __builtin_nontemporal_load(cp + g * 64 + l);
+#pragma unroll
+            for (int c = 0; c < 9; ++c) H[u][c] = __builtin_nontemporal_load(vp + (g * 9 + c) * 64 + l);
+        }
+    };
+    int sq = 0, pos = w0;
+    uint32_t Wa[kU];
+    TM Ha[kU][9];
+    bool aa[kU];
     if (w0 < w1) {
-        // the loads of the step at local slot ps of slice sl (matrix stream read
-        // once: non-temporal); slots past the wave's range repeat slot ps, masked
-        auto issue = [&](int ps, int sl, uint32_t* W, TM (*H)[9], bool* act) {
-            const int send = min(w1, s_sb[sl + 1]);
-            const int len = s_len[sl * 64 + lane];
-            const int s0 = ps - s_sb[sl];
-#pragma unroll
-            for (int u = 0; u < kU; ++u) {
-                const bool in = ps + u < send;
-                act[u] = in && s0 + u < len;
-                const int l = act[u] ? lane : 0;
-                const long g = in ? ps + u : ps;
-                W[u] = __builtin_nontemporal_load(cp + g * 64 + l);
-#pragma unroll
-                for (int c = 0; c < 9; ++c) H[u][c] = __builtin_nontemporal_load(vp + (g * 9 + c) * 64 + l);
-            }
-        };
-        int sq = 0;
         while (s_sb[sq + 1] <= w0) ++sq;
-        int pos = w0;
-        uint32_t Wa[kU];
-        TM Ha[kU][9];
-        bool aa[kU];
         issue(pos, sq, Wa, Ha, aa);
+    }
+    {
+        // stage the tile's own rows and its halo
+        const long e3 = 3L * n_rows;
+        for (int t = threadIdx.x; t < 3 * kTile; t += kThreads) xl[t] = (TL)(b3 + t < e3 ? gx(b3 + t) : TV(0));
+        const int h0 = halo_ptr[tile], hn = halo_ptr[tile + 1] - h0;
+        for (int k = threadIdx.x; k < hn; k += kThreads) {
+            const long j = halo_list[h0 + k];
+            TL* d = xl + 3 * (kTile + k);
+            d[0] = (TL)gx(3 * j);
+            d[1] = (TL)gx(3 * j + 1);
+            d[2] = (TL)gx(3 * j + 2);
+        }
+        __syncthreads();
+    }
+    if (w0 < w1) {
         int cur = -1;
         TV a0 = 0, a1 = 0, a2 = 0, xi0 = 0, xi1 = 0, xi2 = 0;
         while (true) {
@@ -326,17 +359,17 @@ __global__ __launch_bounds__(kThreads) void k_spmv_halo(const TM* __restrict__ v
             if (more) issue(npos, nsq, Wb, Hb, ab);
             if (sq != cur) {
                 if (cur >= 0) {
-                    TV* pc = piece + (wave + cur) * 3 * 64 + lane;
-                    pc[0] = a0;
-                    pc[64] = a1;
-                    pc[128] = a2;
+                    TL* pc = piece + (wave + cur) * 3 * 64 + lane;
+                    pc[0] = (TL)a0;
+                    pc[64] = (TL)a1;
+                    pc[128] = (TL)a2;
                 }
                 cur = sq;
                 const int lr = s_row[sq * 64 + lane];
                 const int li = 3 * (lr >= 0 ? lr : 0);
-                xi0 = xl[li];
-                xi1 = xl[li + 1];
-                xi2 = xl[li + 2];
+                xi0 = (TV)xl[li];
+                xi1 = (TV)xl[li + 1];
+                xi2 = (TV)xl[li + 2];
                 a0 = a1 = a2 = TV(0);
             }
 #pragma unroll
@@ -344,16 +377,16 @@ __global__ __launch_bounds__(kThreads) void k_spmv_halo(const TM* __restrict__ v
                 const uint32_t w = Wa[u];
                 const TV m = aa[u] ? TV(1) : TV(0);
                 const int lc = 3 * ((w >> 16) & 0x3FFF);
-                const TV x0 = m * xl[lc], x1 = m * xl[lc + 1], x2 = m * xl[lc + 2];
+                const TV x0 = m * (TV)xl[lc], x1 = m * (TV)xl[lc + 1], x2 = m * (TV)xl[lc + 2];
                 const TM* h = Ha[u];
                 a0 += (TV)h[0] * x0 + (TV)h[3] * x1 + (TV)h[6] * x2;
                 a1 += (TV)h[1] * x0 + (TV)h[4] * x1 + (TV)h[7] * x2;
                 a2 += (TV)h[2] * x0 + (TV)h[5] * x1 + (TV)h[8] * x2;
                 if (aa[u] && (w & kInTile)) {
                     const int ts = w & 0xFFFF;
-                    tl[3 * ts] = (TV)h[0] * xi0 + (TV)h[1] * xi1 + (TV)h[2] * xi2;
-                    tl[3 * ts + 1] = (TV)h[3] * xi0 + (TV)h[4] * xi1 + (TV)h[5] * xi2;
-                    tl[3 * ts + 2] = (TV)h[6] * xi0 + (TV)h[7] * xi1 + (TV)h[8] * xi2;
+                    tl[3 * ts] = (TL)((TV)h[0] * xi0 + (TV)h[1] * xi1 + (TV)h[2] * xi2);
+                    tl[3 * ts + 1] = (TL)((TV)h[3] * xi0 + (TV)h[4] * xi1 + (TV)h[5] * xi2);
+                    tl[3 * ts + 2] = (TL)((TV)h[6] * xi0 + (TV)h[7] * xi1 + (TV)h[8] * xi2);
                 }
             }
             if (!more) break;
@@ -367,10 +400,10 @@ __global__ __launch_bounds__(kThreads) void k_spmv_halo(const TM* __restrict__ v
             pos = npos;
             sq = nsq;
         }
-        TV* pc = piece + (wave + cur) * 3 * 64 + lane;
-        pc[0] = a0;
-        pc[64] = a1;
-        pc[128] = a2;
+        TL* pc = piece + (wave + cur) * 3 * 64 + lane;
+        pc[0] = (TL)a0;
+        pc[64] = (TL)a1;
+        pc[128] = (TL)a2;
     }
     __syncthreads();
     double dot = 0.0;
@@ -385,17 +418,17 @@ __global__ __launch_bounds__(kThreads) void k_spmv_halo(const TM* __restrict__ v
             for (int wv = 0; wv < kWaves; ++wv) {
                 const int lo = (int)((long)tot * wv / kWaves), hi = (int)((long)tot * (wv + 1) / kWaves);
                 if ((lo > Sq ? lo : Sq) >= (hi < Sq1 ? hi : Sq1)) continue;
-                const TV* pc = piece + (wv + sq) * 3 * 64 + lane;
-                r0 += pc[0];
-                r1 += pc[64];
-                r2 += pc[128];
+                const TL* pc = piece + (wv + sq) * 3 * 64 + lane;
+                r0 += (TV)pc[0];
+                r1 += (TV)pc[64];
+                r2 += (TV)pc[128];
             }
             const int tb = (int)(ti_r >> 16), te = tb + (int)(ti_r & 0xFFFF);
             int k = tb;
             for (; k + 4 <= te; k += 4) {
                 TV v[12];
 #pragma unroll
-                for (int e = 0; e < 12; ++e) v[e] = tl[3 * k + e];
+                for (int e = 0; e < 12; ++e) v[e] = (TV)tl[3 * k + e];
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     r0 += v[3 * e];
@@ -404,12 +437,15 @@ __global__ __launch_bounds__(kThreads) void k_spmv_halo(const TM* __restrict__ v
                 }
             }
             for (; k < te; ++k) {
-                r0 += tl[3 * k];
-                r1 += tl[3 * k + 1];
-                r2 += tl[3 * k + 2];
+                r0 += (TV)tl[3 * k];
+                r1 += (TV)tl[3 * k + 1];
+                r2 += (TV)tl[3 * k + 2];
             }
             const long i = (long)tile * kTile + lr;
-            const TV v0 = xl[3 * lr], v1 = xl[3 * lr + 1], v2 = xl[3 * lr + 2];
+            // the row's own entries, exact (the LDS copy is rounded when TL is narrower)
+            const TV v0 = kExactLds ? (TV)xl[3 * lr] : gx(3 * i);
+            const TV v1 = kExactLds ? (TV)xl[3 * lr + 1] : gx(3 * i + 1);
+            const TV v2 = kExactLds ? (TV)xl[3 * lr + 2] : gx(3 * i + 2);
             if (kFuseP) {
                 p_new[3 * i] = v0;
                 p_new[3 * i + 1] = v1;
@@ -448,19 +484,19 @@ __global__ __launch_bounds__(kThreads) void k_spmv_halo(const TM* __restrict__ v
         // one 32-row level-0 domain, so in one wave of 64 rows, whose lanes
         // group by node (the leader writes the node's sum)
         __syncthreads();
-        TV* const yb = piece;
+        TL* const yb = piece;
         if (ylr >= 0) {
-            yb[3 * ylr] = ys0;
-            yb[3 * ylr + 1] = ys1;
-            yb[3 * ylr + 2] = ys2;
+            yb[3 * ylr] = (TL)ys0;
+            yb[3 * ylr + 1] = (TL)ys1;
+            yb[3 * ylr + 2] = (TL)ys2;
         }
         __syncthreads();
         if (threadIdx.x < kTile) {  // waves 0 .. kTile/64 - 1, whole
             const long u = (long)tile * kTile + threadIdx.x;
             const bool real = u < n_rows;
-            const TV v0 = real ? yb[3 * threadIdx.x] : TV(0);
-            const TV v1 = real ? yb[3 * threadIdx.x + 1] : TV(0);
-            const TV v2 = real ? yb[3 * threadIdx.x + 2] : TV(0);
+            const TV v0 = real ? (TV)yb[3 * threadIdx.x] : TV(0);
+            const TV v1 = real ? (TV)yb[3 * threadIdx.x + 1] : TV(0);
+            const TV v2 = real ? (TV)yb[3 * threadIdx.x + 2] : TV(0);
             int node = real ? l1.map1[u] : -1;
             bool pending = node >= 0;
             const int lane = threadIdx.x & 63;
@@ -806,10 +842,11 @@ inline int g256(long n) { return (int)((n + 255) / 256); }
 // Launch on stream s; with e0/e1 the events are recorded by the kernel's own
 // dispatch (its start and end), not as separate stream commands.
 template <typename... KArgs, typename... Args>
-void go(void (*kernel)(KArgs...), int grid, size_t shmem, hipStream_t s, hipEvent_t e0, hipEvent_t e1, Args... args)
+void go(void (*kernel)(KArgs...), int grid, int block, size_t shmem, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
+        Args... args)
 {
-    if (e0 && e1) hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(kThreads), (uint32_t)shmem, s, e0, e1, 0, args...);
-    else kernel<<<grid, kThreads, shmem, s>>>(args...);
+    if (e0 && e1) hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(block), (uint32_t)shmem, s, e0, e1, 0, args...);
+    else kernel<<<grid, block, shmem, s>>>(args...);
 }
 
 }  // namespace
@@ -969,7 +1006,7 @@ void FullMatrix::build_halo(hipStream_t s, int tbits)
     tl_max = st[1] > 0 ? st[1] : 1;
     // the FP64-vector kernels have the largest LDS; use the halo form only when
     // it fits beside the largest static LDS among them
-    if (halo_dyn_bytes(sizeof(double)) + halo_static_lds() > (size_t)kLdsBytes) return;
+    if (halo_dyn_bytes(sizeof(double), kThreads) + halo_static_lds() > (size_t)kLdsBytes) return;
     const char* env = std::getenv("SGIPC_SPMV_GATHER");
     if (env && env[0] && env[0] != '0') return;  // keep the global-gather form (tests of both forms)
     k_localize<<<g256(nl), 256, 0, s>>>(sell_cols.get(), slice_ptr.get(), perm.get(), halo_ptr.get(), halo_list.get(),
@@ -992,9 +1029,10 @@ void FullMatrix::build_halo(hipStream_t s, int tbits)
     use_halo = true;
 }
 
-size_t FullMatrix::halo_dyn_bytes(size_t tv_size) const
+size_t FullMatrix::halo_dyn_bytes(size_t tv_size, int threads) const
 {
-    return (3 * (size_t)tl_max + (size_t)kPieces * 3 * 64 + 3 * ((size_t)kTile + (size_t)halo_max)) * tv_size;
+    const size_t pieces = (size_t)(threads / 64 + kSlices - 1);
+    return (3 * (size_t)tl_max + pieces * 3 * 64 + 3 * ((size_t)kTile + (size_t)halo_max)) * tv_size;
 }
 
 size_t FullMatrix::halo_static_lds()
@@ -1042,7 +1080,8 @@ int FullMatrix::launch(hipStream_t s, const TM* vals, TV a, const TV* x, TV b, T
                 "level-1 sums ride on the plain halo-form SpMV only");
     const Level1Sums l1v = l1 ? *l1 : Level1Sums{};
     if (use_halo) {
-        const size_t dyn = halo_dyn_bytes(sizeof(TV));
+        constexpr int nt = HaloCfg<TM, TV>::NT;
+        const size_t dyn = halo_dyn_bytes(sizeof(typename HaloCfg<TM, TV>::TL), nt);
 #define SGIPC_HALO(DOT, FUSE, SUM1)                                                                                 \
     do {                                                                                                            \
         const void* fn = reinterpret_cast<const void*>(&k_spmv_halo<TM, TV, DOT, FUSE, SUM1>);                      \
@@ -1055,7 +1094,7 @@ int FullMatrix::launch(hipStream_t s, const TM* vals, TV a, const TV* x, TV b, T
             max_dyn = m;                                                                                            \
         }                                                                                                           \
         SGIPC_CHECK(dyn <= (size_t)max_dyn, SGIPC_CAPACITY, "halo SpMV needs more LDS than a CU has");              \
-        go(&k_spmv_halo<TM, TV, DOT, FUSE, SUM1>, g, dyn, s, time_start, time_stop, vals,                          \
+        go(&k_spmv_halo<TM, TV, DOT, FUSE, SUM1>, g, nt, dyn, s, time_start, time_stop, vals,                      \
            (const uint32_t*)sell_cols.get(), (const int*)slice_ptr.get(), (const int*)perm.get(),                   \
            (const int*)plen.get(), (const uint32_t*)tinfo.get(), (const int*)halo_ptr.get(),                        \
            (const int*)halo_list.get(), (const int*)order.get(), tl_max, a, x, b, y, dot_part, done, z, p_new, rzn, \
@@ -1075,7 +1114,8 @@ int FullMatrix::launch(hipStream_t s, const TM* vals, TV a, const TV* x, TV b, T
         return dot_part ? g : 0;
     }
 #define SGIPC_TSYM(DOT, FUSE)                                                                                   \
-    go(&k_spmv_tsym<TM, TV, DOT, FUSE>, g, 0, s, time_start, time_stop, vals, (const uint32_t*)sell_cols.get(),     \
+    go(&k_spmv_tsym<TM, TV, DOT, FUSE>, g, kThreads, 0, s, time_start, time_stop, vals,                            \
+       (const uint32_t*)sell_cols.get(),                                                                            \
        (const int*)slice_ptr.get(), (const int*)slice_w.get(), (const int*)perm.get(), (const int*)plen.get(),       \
        (const uint32_t*)tinfo.get(), a, x, b, y, dot_part, done, z, p_new, rzn, rzo, n_rz, n_rows)
     if (p_new) {

@@ -70,8 +70,9 @@ class FullMatrix {
     // dispatch overhead that separate event records add
     hipEvent_t time_start = nullptr, time_stop = nullptr;
 
-    // dynamic LDS of the halo kernel with vector entries of tv_size bytes
-    size_t halo_dyn_bytes(size_t tv_size) const;
+    // dynamic LDS of the halo kernel with LDS entries of tv_size bytes and
+    // `threads` threads per workgroup
+    size_t halo_dyn_bytes(size_t tv_size, int threads) const;
     // largest static LDS among the FP64-vector halo kernels (queried once)
     static size_t halo_static_lds();
 

@@ -95,7 +95,8 @@ def test_tiled_spmv_matches_oracle(pkg, orc, gpu_ctx, which, form, scene_tiny, s
         y32 = gpu_ctx.to_device(np.zeros(3 * s.n, np.float32), np.float32)
         A.apply(1.0, x32, 0.0, y32)
         assert rel(y32.download().astype(np.float64), ref) <= 2e-6
-        # FP32 values, FP64 vectors: the matrix rounding only (~6e-8 relative per value)
+        # FP32 values, FP64 vectors: the matrix, the LDS copy of x and the in-tile
+        # transposed products rounded to FP32 (~6e-8 relative each)
         dy.upload(y0)
         A.apply_f32_values(0.9, dx, -0.4, dy)
         assert rel(dy.download(), orc.spmv(0.9, vals, rows, cols, x, -0.4, y0, s.n)) <= 2e-6
