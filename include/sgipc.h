@@ -501,7 +501,9 @@ typedef struct sgipc_pcg_stats {
     /* 1: a mixed-precision system solve failed (an indefinite FP32/FP16        */
     /* preconditioner, or refinement passes that stopped reducing the FP64      */
     /* residual) and was redone in FP64 with the preconditioner rebuilt in FP64; */
-    /* iterations then count the last mixed-precision run and the FP64 run       */
+    /* 2: the solve ran in FP64 from the start because mixed precision failed   */
+    /* on an earlier solve of the system (it is tried again after 8, 16, 32 ...  */
+    /* solves).  Iterations count every run of the call, failed ones included.  */
     int fp64_fallback;
 } sgipc_pcg_stats;
 

@@ -981,11 +981,14 @@ void orc_set_threads(int k)
 
 /* ------------------------------------------------------------------------- */
 /* Converter::srbk_convert -- StiffGIPC/linear_system/utils/converter.cu:32-51, */
-/* 88-131, 171-238: radix sort on (row, col), reduce by key in sorted order.  */
+/* 88-131, 171-238: radix sort on (row, col), reduce by key in sorted order   */
+/* (blocks of more than 64 triplets: ordered sum of 64-triplet chunk sums).   */
 /* ------------------------------------------------------------------------- */
 /* Stable LSD radix sort of the triplet indices by key (8-bit digits over the
  * key's used bits; per-thread histograms, prefix in thread order so ties
  * keep index order): the order of a stable sort by (key, index). */
+#define ORC_SEG_CHUNK 64
+
 static void radix_sort_idx(long n, const uint64_t* keys, long* idx)
 {
     uint64_t mx = 0;
@@ -1053,15 +1056,28 @@ long orc_convert(long n_trip, const double* tvals, const int* trows, const int* 
     for (long s = 0; s < n_trip; ++s)
         if (s == 0 || keys[idx[s]] != keys[idx[s - 1]]) head[u++] = s;
     head[u] = n_trip;
-#pragma omp parallel for schedule(static)
+    /* a block of more than ORC_SEG_CHUNK triplets is the ordered sum of its
+     * ORC_SEG_CHUNK-triplet chunk sums (the library's fixed order) */
+#pragma omp parallel for schedule(dynamic, 1024)
     for (long q = 0; q < u; ++q) {
         long i0 = idx[head[q]];
         orows[q] = trows[i0];
         ocols[q] = tcols[i0];
         double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        for (long s = head[q]; s < head[q + 1]; ++s)
-            for (int e = 0; e < 9; ++e) acc[e] += tvals[9 * idx[s] + e];
-        for (int e = 0; e < 9; ++e) ovals[9 * q + e] = acc[e];
+        long b = head[q], e = head[q + 1];
+        if (e - b > ORC_SEG_CHUNK) {
+            for (long c = b; c < e; c += ORC_SEG_CHUNK) {
+                double part[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                long ce = c + ORC_SEG_CHUNK < e ? c + ORC_SEG_CHUNK : e;
+                for (long s = c; s < ce; ++s)
+                    for (int k = 0; k < 9; ++k) part[k] += tvals[9 * idx[s] + k];
+                for (int k = 0; k < 9; ++k) acc[k] += part[k];
+            }
+        } else {
+            for (long s = b; s < e; ++s)
+                for (int k = 0; k < 9; ++k) acc[k] += tvals[9 * idx[s] + k];
+        }
+        for (int k = 0; k < 9; ++k) ovals[9 * q + k] = acc[k];
     }
     free(head);
     free(keys);

@@ -15,7 +15,10 @@
 //  * conversion (Converter::srbk_convert, StiffGIPC/linear_system/utils/
 //    converter.cu:32-238): a stable rocPRIM radix sort over only the bits the
 //    (row, col) key needs, then every unique block is summed by one thread in
-//    sorted (= arena) order, which makes the result deterministic.
+//    sorted (= arena) order, a block of more than 64 triplets as the ordered
+//    sum of its 64-triplet chunk sums (each chunk by its own thread), which
+//    makes the result deterministic.
+#include <algorithm>
 #include <cstring>
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -457,15 +460,48 @@ __global__ void k_heads(const uint64_t* __restrict__ sk, const int* __restrict__
     ocols[u] = (int)(k & ((1ull << kbits) - 1ull));
 }
 
-// One thread per unique block: sum its triplets in sorted (= arena) order.
-__global__ __launch_bounds__(256) void k_segment_sum(const double* __restrict__ tvals, const int* __restrict__ sidx,
-                                                     const int* __restrict__ head_pos, const int* __restrict__ pos,
-                                                     long n, double* __restrict__ ovals, int capacity)
+// Unique blocks of more than kSegChunk triplets (on coarse MAS levels and
+// affine-body rows thousands of triplets merge into one block) are summed in
+// chunks of kSegChunk consecutive sorted triplets, each by its own thread,
+// and then the chunk sums in order; shorter blocks by one thread in sorted
+// order.  Fixed order either way (deterministic).
+constexpr int kSegChunk = 64;
+
+// chunks of every unique block u < cap (0: summed directly)
+__global__ void k_seg_chunks(const int* __restrict__ head_pos, const int* __restrict__ pos, long n, int cap,
+                             int* __restrict__ cnt)
 {
     const int u = blockIdx.x * blockDim.x + threadIdx.x;
-    const int nnz = pos[n - 1];
-    if (u >= nnz || u >= capacity) return;
-    const int b = head_pos[u], e = head_pos[u + 1];
+    if (u >= cap) return;
+    const int m = pos[n - 1] < cap ? pos[n - 1] : cap;
+    int c = 0;
+    if (u < m) {
+        const int len = head_pos[u + 1] - head_pos[u];
+        if (len > kSegChunk) c = (len + kSegChunk - 1) / kSegChunk;
+    }
+    cnt[u] = c;
+}
+
+// chunk t of the long blocks (cofs: exclusive prefix of cnt over cap blocks)
+__global__ __launch_bounds__(256) void k_chunk_sum(const double* __restrict__ tvals, const int* __restrict__ sidx,
+                                                   const int* __restrict__ head_pos, const int* __restrict__ cnt,
+                                                   const int* __restrict__ cofs, int cap, long bound,
+                                                   double* __restrict__ part)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= bound) return;
+    const long total = (long)cofs[cap - 1] + cnt[cap - 1];
+    if (t >= total) return;
+    // the block holding chunk t: the last u with cofs[u] <= t
+    int lo = 0, hi = cap;  // first index with cofs > t lies in (lo, hi]
+    while (lo + 1 < hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((long)cofs[mid] <= t) lo = mid;
+        else hi = mid;
+    }
+    const int u = lo;
+    const int b = head_pos[u] + (int)(t - cofs[u]) * kSegChunk;
+    const int e = min(b + kSegChunk, head_pos[u + 1]);
     double s[9];
 #pragma unroll
     for (int q = 0; q < 9; ++q) s[q] = 0.0;
@@ -473,6 +509,40 @@ __global__ __launch_bounds__(256) void k_segment_sum(const double* __restrict__ 
         const double* v = tvals + 9 * (long)sidx[p];
 #pragma unroll
         for (int q = 0; q < 9; ++q) s[q] += v[q];
+    }
+    double* o = part + 9 * t;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) o[q] = s[q];
+}
+
+// One thread per unique block: its triplets in sorted (= arena) order, or
+// its chunk sums in order.
+__global__ __launch_bounds__(256) void k_segment_sum(const double* __restrict__ tvals, const int* __restrict__ sidx,
+                                                     const int* __restrict__ head_pos, const int* __restrict__ pos,
+                                                     long n, double* __restrict__ ovals, int capacity,
+                                                     const int* __restrict__ cnt, const int* __restrict__ cofs,
+                                                     const double* __restrict__ part)
+{
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nnz = pos[n - 1];
+    if (u >= nnz || u >= capacity) return;
+    double s[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) s[q] = 0.0;
+    if (cnt[u] > 0) {
+        const long c0 = cofs[u], c1 = c0 + cnt[u];
+        for (long c = c0; c < c1; ++c) {
+            const double* v = part + 9 * c;
+#pragma unroll
+            for (int q = 0; q < 9; ++q) s[q] += v[q];
+        }
+    } else {
+        const int b = head_pos[u], e = head_pos[u + 1];
+        for (int p = b; p < e; ++p) {
+            const double* v = tvals + 9 * (long)sidx[p];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) s[q] += v[q];
+        }
     }
     double* o = ovals + 9 * (long)u;
 #pragma unroll
@@ -543,7 +613,10 @@ size_t convert_temp_bytes(hipStream_t s, long n)
                                         s));
     SGIPC_HIP(rocprim::inclusive_scan(nullptr, need2, static_cast<int*>(nullptr), static_cast<int*>(nullptr), (size_t)n,
                                       rocprim::plus<int>(), s));
-    return need > need2 ? need : need2;
+    size_t need3 = 0;
+    SGIPC_HIP(rocprim::exclusive_scan(nullptr, need3, static_cast<int*>(nullptr), static_cast<int*>(nullptr), 0,
+                                      (size_t)n, rocprim::plus<int>(), s));
+    return std::max(need, std::max(need2, need3));
 }
 
 int convert_into(hipStream_t s, const ConvertScratch& w, const double* tvals, const int* trows, const int* tcols,
@@ -571,7 +644,22 @@ int convert_into(hipStream_t s, const ConvertScratch& w, const double* tvals, co
     k_heads<<<g256(n), 256, 0, s>>>(w.skeys, w.flag, w.pos, n, kbits, w.head, orows, ocols, cap);
     SGIPC_LAUNCH_CHECK();
     if (cap > 0) {
-        k_segment_sum<<<g256(cap), 256, 0, s>>>(tvals, w.sidx, w.head, w.pos, n, ovals, cap);
+        // long blocks in chunks: counts into flag, their exclusive prefix into
+        // idx, the chunk sums into keys (both free after the sort and scan;
+        // at most 2 n / kSegChunk chunks of 9 doubles fit in n words)
+        int* cnt = w.flag;
+        int* cofs = w.idx;
+        double* part = reinterpret_cast<double*>(w.keys);
+        k_seg_chunks<<<g256(cap), 256, 0, s>>>(w.head, w.pos, n, cap, cnt);
+        SGIPC_LAUNCH_CHECK();
+        size_t need3 = 0;
+        SGIPC_HIP(rocprim::exclusive_scan(nullptr, need3, cnt, cofs, 0, (size_t)cap, rocprim::plus<int>(), s));
+        SGIPC_CHECK(need3 <= w.temp_bytes, SGIPC_CAPACITY, "conversion scratch smaller than the scan needs");
+        SGIPC_HIP(rocprim::exclusive_scan(w.temp, need3, cnt, cofs, 0, (size_t)cap, rocprim::plus<int>(), s));
+        const long bound = 2 * n / kSegChunk + 1;
+        k_chunk_sum<<<g256(bound), 256, 0, s>>>(tvals, w.sidx, w.head, cnt, cofs, cap, bound, part);
+        SGIPC_LAUNCH_CHECK();
+        k_segment_sum<<<g256(cap), 256, 0, s>>>(tvals, w.sidx, w.head, w.pos, n, ovals, cap, cnt, cofs, part);
         SGIPC_LAUNCH_CHECK();
     }
     int h[2] = {0, 0};

@@ -84,6 +84,12 @@ struct sgipc_system {
     int n_slots = 0;
     DevBuf<int> slot_of, spairs;
     DevBuf<double> bs, xs;
+    // mixed precision that failed on this system: later solves run FP64
+    // until solve index fp32_probe, where FP32 is tried again (the gap
+    // doubling on every further failure)
+    bool fp32_failed = false;
+    long fp32_probe = 0;
+    long fp32_gap = 8;
 };
 
 // sgipc_mas_partition_maps is defined in partition.cpp
@@ -758,6 +764,11 @@ int system_solve(sgipc_system* sys, const double* b, double* x, double* grad, co
                                      sys->bvals.get(), sys->brows.get(), sys->bcols.get(), sys->capacity);
         SGIPC_HIP(hipEventRecord(sys->ev[2], s));
         // ---- preconditioners: MAS on the FEM rows (current contacts), block-Jacobi on the bodies
+        // mixed precision failed on an earlier solve: FP64 from the start
+        // until the next probe (not while collecting training samples)
+        const bool skip32 = o.precision != SGIPC_FP64 && record == nullptr && sys->fp32_failed &&
+                            sys->n_solves < sys->fp32_probe;
+        if (skip32) o.precision = SGIPC_FP64;
         const int mprec = (o.precision == SGIPC_FP64) ? SGIPC_FP64 : SGIPC_FP32_MAS;
         const bool learned = record == nullptr && use_learned(sys);
         sys->mas->m->keep_graph = sys->net != nullptr || record != nullptr;
@@ -816,29 +827,34 @@ int system_solve(sgipc_system* sys, const double* b, double* x, double* grad, co
         // not cure it, so the weight was not the cause)
         Mas* mm = sys->mas->m.get();
         constexpr double kOmegaFloor = 0.05;
-        if (status == SGIPC_NOT_SPD && mm->last_path() == 2 && !used) {
+        int spent = 0;  // iterations of runs that failed and were redone
+        auto recover_spd = [&](const sgipc_pcg_opts& oo) {
+            if (!(status == SGIPC_NOT_SPD && mm->last_path() == 2 && !used)) return;
             const double w0 = mm->omega0;
             for (int tries = 0; tries < 2 && status == SGIPC_NOT_SPD && mm->omega0 * 0.5 >= kOmegaFloor; ++tries) {
+                spent += pst.iterations;
                 mm->omega0 *= 0.5;
-                status = run_pcg(o);
+                status = run_pcg(oo);
             }
             if (status == SGIPC_NOT_SPD) {
+                spent += pst.iterations;
                 mm->omega0 = w0;
                 mm->coarse_mode = 1;
                 try {
-                    status = run_pcg(o);
+                    status = run_pcg(oo);
                 } catch (...) {
                     mm->coarse_mode = 2;
                     throw;
                 }
                 mm->coarse_mode = 2;
             }
-        }
+        };
+        recover_spd(o);
         // a mixed-precision solve that still failed because the FP32
         // operators do not resolve the system (an indefinite FP32/FP16
         // preconditioner, or refinement passes that stopped reducing the FP64
         // residual before the iteration cap) is redone in FP64 with the
-        // preconditioner rebuilt in FP64
+        // preconditioner rebuilt in FP64, with the same recovery
         int fallback = 0;
         const int cap_it = o.max_iter > 0 ? o.max_iter : (int)(0.3 * 3.0 * (double)sys->n_sys);
         const bool precision_failed =
@@ -851,11 +867,19 @@ int system_solve(sgipc_system* sys, const double* b, double* x, double* grad, co
             used = false;
             mm->set_bcoo(sys->bvals.get(), sys->brows.get(), sys->bcols.get(), sys->nnz, off0, mas_pairs, mas_npairs,
                          SGIPC_FP64);
-            const int it32 = pst.iterations;
+            spent += pst.iterations;
             status = run_pcg(o64);
-            pst.iterations += it32;
+            recover_spd(o64);
             fallback = 1;
+            if (sys->fp32_failed) sys->fp32_gap *= 2;
+            sys->fp32_failed = true;
+            sys->fp32_probe = sys->n_solves + sys->fp32_gap;
+        } else if (mprec != SGIPC_FP64 && record == nullptr && status == SGIPC_OK) {
+            sys->fp32_failed = false;  // mixed precision works (again)
+            sys->fp32_gap = 8;
         }
+        if (skip32) fallback = 2;
+        pst.iterations += spent;
         if (slots) launch_slot_gather(s, d.n_verts, sys->slot_of.get(), sys->xs.get(), x);
         pst.fp64_fallback = fallback;
         pst.learned = used ? 1 : (abandoned ? 2 : 0);

@@ -277,7 +277,7 @@ class SolveResult:
     coarse_mode: int = 0             # MAS coarse mode of the run that returned
     level2_exact: bool = False       # its level-1 cycle had the exact dense level-2 solve
     omega: float = 0.0               # mode-2 smoothing weight
-    fp64_fallback: bool = False      # a failed mixed-precision solve was redone in FP64
+    fp64_fallback: int = 0           # 1: a failed mixed-precision solve was redone in FP64; 2: ran FP64 from the start
 
 
 def upload_scene(ctx: Context, scene):
@@ -460,7 +460,7 @@ class LinearSystem:
                            true_rel_residual=st.rel_residual, status=st.status, timing=timing,
                            learned=st.learned == 1, learned_abandoned=st.learned == 2,
                            coarse_mode=st.coarse_mode, level2_exact=st.level2_exact == 1, omega=st.omega,
-                           fp64_fallback=st.fp64_fallback == 1)
+                           fp64_fallback=int(st.fp64_fallback))
 
     def collect(self, stride=1, max_samples=256, tol=1e-5, precision="fp64", b=None):
         """Training samples (MpnnData) from one plain solve of the scene's rhs

@@ -215,5 +215,14 @@ def test_c4_cloth_solves_to_tolerance(pkg, orc, c4):
               f"fp64 fallback {mixed.fp64_fallback}) {ms_mixed:.1f} ms res {r32:.2e}", flush=True)
         assert plain.status == 0 and r64 <= 1.01e-5
         assert mixed.status == 0 and r32 <= 1.01e-5
+        if mixed.fp64_fallback:
+            # a system whose mixed-precision solve failed starts the next
+            # solves in FP64 (no failed FP32 run first)
+            again, ms_again = timed_solve(ls, tol=1e-5, precision="fp32_ir", max_iter=cap)
+            r2 = host_residual(orc, ref, again.x, s.rhs, s.n)
+            print(f"C4: next fp32_ir solve {again.iterations} it, fp64 fallback {again.fp64_fallback}, "
+                  f"{ms_again:.1f} ms res {r2:.2e}", flush=True)
+            assert again.status == 0 and r2 <= 1.01e-5 and again.fp64_fallback == 2
+            assert again.iterations < mixed.iterations
     finally:
         ls.close()

@@ -297,3 +297,29 @@ def test_pcg_reference_stop_rule(orc, scene_c0):
     _, it_true, res_true = orc.pcg(vals, rows, cols, s.n, s.rhs, m, tol=1e-4, stop_mode=1)
     assert it_ref < it_true and res_ref > res_true
     m.close()
+
+
+def test_convert_long_blocks_in_chunks(orc):
+    """A block merging more than 64 triplets is the ordered sum of its
+    64-triplet chunk sums (the library's fixed order), each chunk summed in
+    sorted (= arena) order: checked bit for bit against that order written
+    out, and to rounding against a plain sum."""
+    rng = np.random.default_rng(1)
+    m = 1000
+    r = np.concatenate([np.zeros(m, np.int32), np.ones(10, np.int32)])
+    c = np.concatenate([np.full(m, 2, np.int32), np.full(10, 3, np.int32)])
+    v = rng.standard_normal((r.size, 9)) * 10.0 ** rng.uniform(-8, 8, (r.size, 1))
+    ov, orr, oc = orc.convert(v, r, c)
+    assert list(orr) == [0, 1] and list(oc) == [2, 3]
+    want = np.zeros(9)
+    for k in range(0, m, 64):
+        part = np.zeros(9)
+        for row in v[k:min(k + 64, m)]:
+            part = part + row
+        want = want + part
+    np.testing.assert_array_equal(ov[0], want)
+    short = np.zeros(9)
+    for row in v[m:]:
+        short = short + row
+    np.testing.assert_array_equal(ov[1], short)
+    np.testing.assert_allclose(ov[0], v[:m].sum(axis=0), rtol=1e-6, atol=1e-6 * np.abs(v[:m]).sum(axis=0).max())
