@@ -333,6 +333,10 @@ int sgipc_spmat_apply_f32_values(sgipc_spmat* m, double a, const double* x, doub
  * *tl_slots = in-tile transposed slots the fullest tile uses.  Any output
  * pointer may be NULL. */
 int sgipc_spmat_layout(const sgipc_spmat* m, int* halo_form, int* halo_max, int* tl_slots);
+/* Rows the layout keeps outside the tiles (more than 128 blocks: affine-body
+ * rows, hub vertices), multiplied by a kernel of their own one workgroup per
+ * row, so that no tile's workgroup carries their length. */
+int sgipc_spmat_long_rows(const sgipc_spmat* m, int* n_long);
 
 /* Measurement helper: lays the upper BCOO out as the PCG solver does (row
  * tiles, in-tile blocks once, cross-tile blocks in both rows) and returns the

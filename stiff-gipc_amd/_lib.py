@@ -106,6 +106,7 @@ _SIGS = {
     "sgipc_spmat_apply_f32": (ci, [vp, cf, vp, cf, vp]),
     "sgipc_spmat_apply_f32_values": (ci, [vp, cd, vp, cd, vp]),
     "sgipc_spmat_layout": (ci, [vp, ctypes.POINTER(ci), ctypes.POINTER(ci), ctypes.POINTER(ci)]),
+    "sgipc_spmat_long_rows": (ci, [vp, ctypes.POINTER(ci)]),
     "sgipc_spmv_time": (ci, [vp, ci, vp, vp, vp, ci, ci, ci, ci, ctypes.POINTER(cd), ctypes.POINTER(ci)]),
     "sgipc_mas_time_apply": (ci, [vp, vp, vp, ci, ctypes.POINTER(cd)]),
     "sgipc_mas_create": (ci, [vp, ctypes.POINTER(vp), ci]),

@@ -268,6 +268,13 @@ int sgipc_spmat_layout(const sgipc_spmat* m, int* halo_form, int* halo_max, int*
     return SGIPC_OK;
 }
 
+int sgipc_spmat_long_rows(const sgipc_spmat* m, int* n_long)
+{
+    if (!m || !n_long) return SGIPC_BAD_ARG;
+    *n_long = m->m.n_long;
+    return SGIPC_OK;
+}
+
 int sgipc_spmv_time(sgipc_ctx* ctx, int precision, const double* vals, const int* rows, const int* cols, int nnz,
                     int n_rows, int reps, int cold_cache, double* avg_ms, int* stored_blocks)
 {

@@ -57,6 +57,7 @@ constexpr int kWaves = kThreads / 64;          // waves of a workgroup
 constexpr int kSlices = kTile / 64;            // SELL-64 slices of a tile
 constexpr int kPieces = kWaves + kSlices - 1;  // (wave, slice) partial rows of the halo kernel
 constexpr int kLdsBytes = 163840;              // LDS of one CU on gfx950
+constexpr int kLongRow = 128;                  // rows with more blocks are kept outside the tiles
 
 // Workgroups are handed to the 8 XCDs of the part round-robin in blockIdx
 // order.  Mapping block b to tile start(b % 8) + b / 8 gives every XCD a
@@ -561,15 +562,118 @@ __global__ void k_ptrs(const int* __restrict__ rows, const uint64_t* __restrict_
     col_ptr[i] = lo;
 }
 
-// in-tile candidate flag per col-sorted entry (i != j, same tile)
-__global__ void k_cand(const uint64_t* __restrict__ skeys, int nnz, int kbits, int* __restrict__ cand)
+// Rows of more than kLongRow blocks (upper + lower, the diagonal once) are
+// long: they hold nothing in the tiles and are multiplied by their own
+// kernel, so a few very long rows (affine-body rows, hub vertices) cannot
+// make one tile's workgroup the whole launch.
+__global__ void k_long_flags(const int* __restrict__ up_ptr, const int* __restrict__ col_ptr,
+                             const uint64_t* __restrict__ skeys, int n, int kbits, int* __restrict__ lflag,
+                             int* __restrict__ llen)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    if (i == n) {
+        lflag[n] = 0;
+        llen[n] = 0;
+        return;
+    }
+    // column i's entries (r, i) are sorted by r <= i: the diagonal, when
+    // present, is the last one
+    const int c0 = col_ptr[i], c1 = col_ptr[i + 1];
+    const bool diag = c1 > c0 && (int)(skeys[c1 - 1] & ((1ull << kbits) - 1ull)) == i;
+    const int L = (up_ptr[i + 1] - up_ptr[i]) + (c1 - c0) - (diag ? 1 : 0);
+    const int f = L > kLongRow ? 1 : 0;
+    lflag[i] = f;
+    llen[i] = f ? L : 0;
+}
+
+// in-tile candidate flag per col-sorted entry (i != j, same tile, neither row long)
+__global__ void k_cand(const uint64_t* __restrict__ skeys, int nnz, int kbits, const int* __restrict__ lflag,
+                       int* __restrict__ cand)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nnz) return;
     const uint64_t key = skeys[s];
     const int j = (int)(key >> kbits);
     const int i = (int)(key & ((1ull << kbits) - 1ull));
-    cand[s] = (i != j && i / kTile == j / kTile) ? 1 : 0;
+    cand[s] = (i != j && i / kTile == j / kTile && !lflag[i] && !lflag[j]) ? 1 : 0;
+}
+
+// The full rows of the long rows (CSR, one thread per long row): the lower
+// blocks (r, i), r < i, transposed, by r, then the upper blocks by column.
+__global__ void k_long_fill(const double* __restrict__ vals, const int* __restrict__ cols,
+                            const uint64_t* __restrict__ skeys, const int* __restrict__ sidx,
+                            const int* __restrict__ up_ptr, const int* __restrict__ col_ptr,
+                            const int* __restrict__ lflag, const int* __restrict__ lrank,
+                            const int* __restrict__ loff, int n, int kbits, int* __restrict__ long_rows,
+                            int* __restrict__ long_ptr, int* __restrict__ long_col, double* __restrict__ lv,
+                            float* __restrict__ lv32)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    if (i == n) {
+        long_ptr[lrank[n]] = loff[n];
+        return;
+    }
+    if (!lflag[i]) return;
+    const int k = lrank[i];
+    long_rows[k] = i;
+    long_ptr[k] = loff[i];
+    long e = loff[i];
+    auto put = [&](int c, int b, bool tr) {
+        long_col[e] = c;
+        const double* h = vals + 9L * b;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const double v = tr ? h[r * 3 + q] : h[q * 3 + r];
+                lv[9 * e + q * 3 + r] = v;
+                if (lv32) lv32[9 * e + q * 3 + r] = (float)v;
+            }
+        ++e;
+    };
+    for (int t = col_ptr[i]; t < col_ptr[i + 1]; ++t) {
+        const int r = (int)(skeys[t] & ((1ull << kbits) - 1ull));
+        if (r != i) put(r, sidx[t], true);
+    }
+    for (int b = up_ptr[i]; b < up_ptr[i + 1]; ++b) put(cols[b], b, false);
+}
+
+// y_i += a (A x)_i for every long row i, one workgroup per row (fixed-order
+// block reduction); the tiled kernel has already written b y_i there.  The
+// x^T(A x) partial of row i goes to dot_part[k].
+template <typename TM, typename TV, bool kDot>
+__global__ __launch_bounds__(256) void k_spmv_long(const int* __restrict__ long_rows, const int* __restrict__ long_ptr,
+                                                   const int* __restrict__ long_col, const TM* __restrict__ lv, TV a,
+                                                   const TV* __restrict__ x, TV* __restrict__ y,
+                                                   double* __restrict__ dot_part, const int* __restrict__ done)
+{
+    __shared__ TV red[16];
+    if (done != nullptr && *done) return;
+    const int k = blockIdx.x;
+    const int i = long_rows[k];
+    TV s0 = 0, s1 = 0, s2 = 0;
+    for (int e = long_ptr[k] + threadIdx.x; e < long_ptr[k + 1]; e += 256) {
+        const TM* h = lv + 9L * e;
+        const long j = long_col[e];
+        const TV x0 = x[3 * j], x1 = x[3 * j + 1], x2 = x[3 * j + 2];
+        s0 += (TV)h[0] * x0 + (TV)h[3] * x1 + (TV)h[6] * x2;
+        s1 += (TV)h[1] * x0 + (TV)h[4] * x1 + (TV)h[7] * x2;
+        s2 += (TV)h[2] * x0 + (TV)h[5] * x1 + (TV)h[8] * x2;
+    }
+    s0 = block_sum(s0, red);
+    s1 = block_sum(s1, red);
+    s2 = block_sum(s2, red);
+    if (threadIdx.x == 0) {
+        TV* yi = y + 3L * i;
+        yi[0] += a * s0;
+        yi[1] += a * s1;
+        yi[2] += a * s2;
+        if (kDot)
+            dot_part[k] = (double)x[3L * i] * (double)s0 + (double)x[3L * i + 1] * (double)s1 +
+                          (double)x[3L * i + 2] * (double)s2;
+    }
 }
 
 // Per column j: the upper word of every block (i, j), the in-tile slot range of
@@ -577,7 +681,8 @@ __global__ void k_cand(const uint64_t* __restrict__ skeys, int nnz, int kbits, i
 // A candidate whose rank in its tile reaches kTcap is stored as cross-tile.
 __global__ void k_col_words(const uint64_t* __restrict__ skeys, const int* __restrict__ sidx,
                             const int* __restrict__ cscan, const int* __restrict__ col_ptr, int n, int kbits,
-                            uint32_t* __restrict__ up_word, uint32_t* __restrict__ tinfo, int* __restrict__ low_cnt)
+                            const int* __restrict__ lflag, uint32_t* __restrict__ up_word,
+                            uint32_t* __restrict__ tinfo, int* __restrict__ low_cnt)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
@@ -593,25 +698,27 @@ __global__ void k_col_words(const uint64_t* __restrict__ skeys, const int* __res
         }
         const bool same = i / kTile == j / kTile;
         const int rank = cscan[s] - rank_base;
-        if (same && rank < kTcap) {
+        if (same && !lflag[i] && !lflag[j] && rank < kTcap) {
             up_word[b] = kInTile | ((uint32_t)(j - tile0) << 16) | (uint32_t)rank;
             if (tb < 0) tb = rank;
             ++tcount;
         } else {
+            // row i holds H (unless long), row j holds H^T (unless long)
             up_word[b] = (uint32_t)j;
-            ++low;
+            if (!lflag[j]) ++low;
         }
     }
     tinfo[j] = ((uint32_t)(tb < 0 ? 0 : tb) << 16) | (uint32_t)tcount;
     low_cnt[j] = low;
 }
 
-__global__ void k_row_keys(const int* __restrict__ up_ptr, const int* __restrict__ low_cnt, int n,
-                           int* __restrict__ len, uint64_t* __restrict__ key, int* __restrict__ iota)
+__global__ void k_row_keys(const int* __restrict__ up_ptr, const int* __restrict__ low_cnt,
+                           const int* __restrict__ lflag, int n, int* __restrict__ len, uint64_t* __restrict__ key,
+                           int* __restrict__ iota)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int L = (up_ptr[i + 1] - up_ptr[i]) + low_cnt[i];
+    const int L = lflag[i] ? 0 : (up_ptr[i + 1] - up_ptr[i]) + low_cnt[i];
     len[i] = L;
     key[i] = ((uint64_t)(i / kTile) << 16) | (uint64_t)(0xFFFFu - (uint32_t)(L < 0xFFFF ? L : 0xFFFF));
     iota[i] = i;
@@ -673,9 +780,9 @@ __global__ __launch_bounds__(256) void k_pack(const double* __restrict__ vals, c
                                               const uint64_t* __restrict__ skeys, const int* __restrict__ sidx,
                                               const int* __restrict__ up_ptr, const int* __restrict__ col_ptr,
                                               const int* __restrict__ perm, const int* __restrict__ slice_ptr,
-                                              const int* __restrict__ slice_w, int n_pos, int kbits,
-                                              double* __restrict__ sv, float* __restrict__ sv32,
-                                              uint32_t* __restrict__ scw)
+                                              const int* __restrict__ slice_w, const int* __restrict__ lflag,
+                                              int n_pos, int kbits, double* __restrict__ sv,
+                                              float* __restrict__ sv32, uint32_t* __restrict__ scw)
 {
     __shared__ double sbuf[4][576];
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -685,8 +792,9 @@ __global__ __launch_bounds__(256) void k_pack(const double* __restrict__ vals, c
     const int i = valid ? perm[p] : -1;
     const long S = valid ? slice_ptr[q] : 0;
     const int W = valid ? slice_w[q] : 0;
-    const int u0 = i >= 0 ? up_ptr[i] : 0, nu = i >= 0 ? up_ptr[i + 1] - u0 : 0;
-    const int l0 = i >= 0 ? col_ptr[i] : 0, nl = i >= 0 ? col_ptr[i + 1] - l0 : 0;
+    const bool held = i >= 0 && !lflag[i];  // a long row stores nothing here
+    const int u0 = held ? up_ptr[i] : 0, nu = held ? up_ptr[i + 1] - u0 : 0;
+    const int l0 = held ? col_ptr[i] : 0, nl = held ? col_ptr[i + 1] - l0 : 0;
     int k = 0;
     const int mu = wave_max(nu);
     for (int t = 0; t < mu; ++t) {
@@ -883,6 +991,10 @@ void FullMatrix::build(hipStream_t s, const double* vals, const int* rows, const
     tinfo.reserve(n);
     slice_w.reserve(n_slices + 1);
     slice_ptr.reserve(n_slices + 1);
+    lflag_.reserve(n + 1);
+    llen_.reserve(n + 1);
+    lrank_.reserve(n + 1);
+    loff_.reserve(n + 1);
 
     int tbits = 1;
     while (tbits < 40 && (1L << tbits) < (long)n_tiles) ++tbits;
@@ -895,7 +1007,11 @@ void FullMatrix::build(hipStream_t s, const double* vals, const int* rows, const
                                         (size_t)n, 0, 16 + tbits, s));
     SGIPC_HIP(rocprim::exclusive_scan(nullptr, need4, slice_w.get(), slice_ptr.get(), 0, (size_t)n_slices + 1,
                                       rocprim::plus<int>(), s));
+    size_t need5 = 0;
+    SGIPC_HIP(rocprim::exclusive_scan(nullptr, need5, lflag_.get(), lrank_.get(), 0, (size_t)n + 1,
+                                      rocprim::plus<int>(), s));
     size_t nt = need;
+    if (need5 > nt) nt = need5;
     if (need2 > nt) nt = need2;
     if (need3 > nt) nt = need3;
     if (need4 > nt) nt = need4;
@@ -908,17 +1024,44 @@ void FullMatrix::build(hipStream_t s, const double* vals, const int* rows, const
                                         (size_t)nnz_up, 0, 2 * kbits, s));
     k_ptrs<<<g256(n + 1), 256, 0, s>>>(rows, skeys_.get(), nnz_up, n, kbits, up_ptr_.get(), col_ptr_.get());
     SGIPC_LAUNCH_CHECK();
+    // long rows: flags, ranks and the offsets of their full rows
+    k_long_flags<<<g256(n + 1), 256, 0, s>>>(up_ptr_.get(), col_ptr_.get(), skeys_.get(), n, kbits, lflag_.get(),
+                                             llen_.get());
+    SGIPC_LAUNCH_CHECK();
+    SGIPC_HIP(rocprim::exclusive_scan(temp_.get(), need5, lflag_.get(), lrank_.get(), 0, (size_t)n + 1,
+                                      rocprim::plus<int>(), s));
+    SGIPC_HIP(rocprim::exclusive_scan(temp_.get(), need5, llen_.get(), loff_.get(), 0, (size_t)n + 1,
+                                      rocprim::plus<int>(), s));
+    {
+        int h[2] = {0, 0};
+        SGIPC_HIP(hipMemcpyAsync(&h[0], lrank_.get() + n, sizeof(int), hipMemcpyDeviceToHost, s));
+        SGIPC_HIP(hipMemcpyAsync(&h[1], loff_.get() + n, sizeof(int), hipMemcpyDeviceToHost, s));
+        SGIPC_HIP(hipStreamSynchronize(s));
+        SGIPC_CHECK(h[1] >= 0, SGIPC_CAPACITY, "long-row entry count overflows int");
+        n_long = h[0];
+        long_rows.reserve(n_long > 0 ? n_long : 1);
+        long_ptr.reserve((size_t)n_long + 1);
+        long_col.reserve(h[1] > 0 ? h[1] : 1);
+        long_vals.reserve((size_t)(h[1] > 0 ? h[1] : 1) * 9);
+        if (want_f32) long_vals32.reserve((size_t)(h[1] > 0 ? h[1] : 1) * 9);
+        k_long_fill<<<g256(n + 1), 256, 0, s>>>(vals, cols, skeys_.get(), sidx_.get(), up_ptr_.get(), col_ptr_.get(),
+                                                lflag_.get(), lrank_.get(), loff_.get(), n, kbits, long_rows.get(),
+                                                long_ptr.get(), long_col.get(), long_vals.get(),
+                                                want_f32 ? long_vals32.get() : nullptr);
+        SGIPC_LAUNCH_CHECK();
+    }
     // in-tile ranks: exclusive scan of the candidate flags in column order
-    k_cand<<<g256(nnz_up), 256, 0, s>>>(skeys_.get(), nnz_up, kbits, cand_.get());
+    k_cand<<<g256(nnz_up), 256, 0, s>>>(skeys_.get(), nnz_up, kbits, lflag_.get(), cand_.get());
     SGIPC_LAUNCH_CHECK();
     SGIPC_HIP(hipMemsetAsync(cand_.get() + nnz_up, 0, sizeof(int), s));
     SGIPC_HIP(rocprim::exclusive_scan(temp_.get(), need2, cand_.get(), cscan_.get(), 0, (size_t)nnz_up + 1,
                                       rocprim::plus<int>(), s));
     k_col_words<<<g256(n), 256, 0, s>>>(skeys_.get(), sidx_.get(), cscan_.get(), col_ptr_.get(), n, kbits,
-                                        up_word_.get(), tinfo.get(), low_cnt_.get());
+                                        lflag_.get(), up_word_.get(), tinfo.get(), low_cnt_.get());
     SGIPC_LAUNCH_CHECK();
     // rows sorted by entry count inside each tile
-    k_row_keys<<<g256(n), 256, 0, s>>>(up_ptr_.get(), low_cnt_.get(), n, len_.get(), rkey_.get(), iota_.get());
+    k_row_keys<<<g256(n), 256, 0, s>>>(up_ptr_.get(), low_cnt_.get(), lflag_.get(), n, len_.get(), rkey_.get(),
+                                       iota_.get());
     SGIPC_LAUNCH_CHECK();
     SGIPC_HIP(rocprim::radix_sort_pairs(temp_.get(), need3, rkey_.get(), rkey_sorted_.get(), iota_.get(),
                                         sorted_.get(), (size_t)n, 0, 16 + tbits, s));
@@ -937,8 +1080,8 @@ void FullMatrix::build(hipStream_t s, const double* vals, const int* rows, const
     sell_cols.reserve((size_t)n_slots * 64);
     if (want_f32) sell_vals32.reserve((size_t)n_slots * 64 * 9);
     k_pack<<<g256(n_pos), 256, 0, s>>>(vals, up_word_.get(), skeys_.get(), sidx_.get(), up_ptr_.get(), col_ptr_.get(),
-                                       perm.get(), slice_ptr.get(), slice_w.get(), n_pos, kbits, sell_vals.get(),
-                                       want_f32 ? sell_vals32.get() : nullptr, sell_cols.get());
+                                       perm.get(), slice_ptr.get(), slice_w.get(), lflag_.get(), n_pos, kbits,
+                                       sell_vals.get(), want_f32 ? sell_vals32.get() : nullptr, sell_cols.get());
     SGIPC_LAUNCH_CHECK();
     build_halo(s, tbits);
 }
@@ -1057,7 +1200,7 @@ size_t FullMatrix::halo_static_lds()
     return cached;
 }
 
-int FullMatrix::partials() const { return n_tiles; }
+int FullMatrix::partials() const { return n_tiles + n_long; }
 
 long FullMatrix::stored_blocks(hipStream_t s) const
 {
@@ -1111,7 +1254,8 @@ int FullMatrix::launch(hipStream_t s, const TM* vals, TV a, const TV* x, TV b, T
         }
 #undef SGIPC_HALO
         SGIPC_LAUNCH_CHECK();
-        return dot_part ? g : 0;
+        launch_long<TM, TV>(s, a, p_new ? p_new : x, y, dot_part, done);
+        return dot_part ? g + n_long : 0;
     }
 #define SGIPC_TSYM(DOT, FUSE)                                                                                   \
     go(&k_spmv_tsym<TM, TV, DOT, FUSE>, g, kThreads, 0, s, time_start, time_stop, vals,                            \
@@ -1127,7 +1271,25 @@ int FullMatrix::launch(hipStream_t s, const TM* vals, TV a, const TV* x, TV b, T
     }
 #undef SGIPC_TSYM
     SGIPC_LAUNCH_CHECK();
-    return dot_part ? g : 0;
+    launch_long<TM, TV>(s, a, p_new ? p_new : x, y, dot_part, done);
+    return dot_part ? g + n_long : 0;
+}
+
+template <typename TM, typename TV>
+void FullMatrix::launch_long(hipStream_t s, TV a, const TV* x, TV* y, double* dot_part, const int* done) const
+{
+    if (n_long <= 0) return;
+    const TM* lv;
+    if constexpr (sizeof(TM) == 8) lv = long_vals.get();
+    else lv = long_vals32.get();
+    double* dp = dot_part ? dot_part + n_tiles : nullptr;
+    if (dot_part)
+        go(&k_spmv_long<TM, TV, true>, n_long, 256, 0, s, nullptr, nullptr, (const int*)long_rows.get(),
+           (const int*)long_ptr.get(), (const int*)long_col.get(), lv, a, x, y, dp, done);
+    else
+        go(&k_spmv_long<TM, TV, false>, n_long, 256, 0, s, nullptr, nullptr, (const int*)long_rows.get(),
+           (const int*)long_ptr.get(), (const int*)long_col.get(), lv, a, x, y, dp, done);
+    SGIPC_LAUNCH_CHECK();
 }
 
 template <typename TV>
@@ -1147,7 +1309,7 @@ template <typename TV>
 bool FullMatrix::apply_level1_sums(hipStream_t s, TV a, const TV* x, TV b, TV* y, const int* done,
                                    const Level1Sums& l1) const
 {
-    if (!use_halo || n_tiles <= 0) return false;
+    if (!use_halo || n_tiles <= 0 || n_long > 0) return false;  // the epilogue needs every row in the tiles
     SGIPC_CHECK(l1.map1 != nullptr && l1.npad >= n_rows && l1.c1 >= 0 && l1.pad >= 0, SGIPC_BAD_ARG,
                 "level-1 sums: bad slot layout");
     if constexpr (sizeof(TV) == 8) {

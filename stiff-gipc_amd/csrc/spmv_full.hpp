@@ -60,6 +60,14 @@ class FullMatrix {
     int n_slices = 0;
     int n_tiles = 0;
     bool has_f32 = false;
+    // rows with too many blocks for the tiles: their full rows as CSR
+    // (column order, FP64 and, with want_f32, FP32 values), multiplied by a
+    // kernel of their own after the tiled one; partials() counts one x^T(Ax)
+    // partial per long row after the tiles'
+    int n_long = 0;
+    DevBuf<int> long_rows, long_ptr, long_col;
+    DevBuf<double> long_vals;
+    DevBuf<float> long_vals32;
     DevBuf<double> sell_vals;
     DevBuf<float> sell_vals32;
     DevBuf<uint32_t> sell_cols, tinfo;
@@ -79,6 +87,8 @@ class FullMatrix {
   private:
     void build_halo(hipStream_t s, int tbits);
     template <typename TM, typename TV>
+    void launch_long(hipStream_t s, TV a, const TV* x, TV* y, double* dot_part, const int* done) const;
+    template <typename TM, typename TV>
     int launch(hipStream_t s, const TM* vals, TV a, const TV* x, TV b, TV* y, double* dot_part, const int* done,
                const TV* z, TV* p_new, const double* rzn, const double* rzo, int n_rz,
                const Level1Sums* l1 = nullptr) const;
@@ -89,6 +99,7 @@ class FullMatrix {
     DevBuf<unsigned char> temp_;
     DevBuf<uint64_t> hkeys_, hkeys_sorted_;
     DevBuf<int> hcnt_, hoff_, hflag_, hinc_, stats_;
+    DevBuf<int> lflag_, llen_, lrank_, loff_;
 };
 
 }  // namespace sgipc

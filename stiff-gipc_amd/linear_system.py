@@ -73,6 +73,12 @@ class SpMat:
         check(lib().sgipc_spmat_layout(self.h, *[ctypes.byref(t) for t in v]), self.ctx.h)
         return tuple(t.value for t in v)
 
+    def long_rows(self):
+        """Rows kept outside the tiles (multiplied by their own kernel)."""
+        v = ctypes.c_int()
+        check(lib().sgipc_spmat_long_rows(self.h, ctypes.byref(v)), self.ctx.h)
+        return v.value
+
     def close(self):
         if self.h:
             lib().sgipc_spmat_destroy(self.h)

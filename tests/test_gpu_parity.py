@@ -442,3 +442,49 @@ def test_graph_replay_matches_eager(pkg, precision, coarse):
             assert e.iterations == ref.iterations and np.array_equal(e.x, ref.x)
     finally:
         ls.close()
+
+
+@pytest.mark.parametrize("form", ["halo", "gather"])
+def test_spmv_long_rows_match_oracle(pkg, orc, gpu_ctx, scene_small, monkeypatch, form):
+    """Rows of more than 128 blocks (here three hub rows joined to a few
+    hundred vertices each, one of them the first row, one in the middle and
+    one near the end) are kept outside the tiles and multiplied by their own
+    kernel; every precision of the tiled product still equals the oracle's."""
+    s = scene_small
+    vals, rows, cols = orc.assemble(s)
+    rng = np.random.default_rng(5)
+    extra_r, extra_c = [], []
+    for hub, k in ((0, 300), (s.n // 2, 200), (s.n - 3, 150)):
+        js = rng.choice(s.n, k, replace=False)
+        for j in js:
+            if j != hub:
+                extra_r.append(min(hub, j))
+                extra_c.append(max(hub, j))
+    ev = rng.standard_normal((len(extra_r), 9)) * np.abs(vals).max() * 1e-3
+    tv = np.concatenate([vals, ev])
+    tr = np.concatenate([rows, np.array(extra_r, np.int32)])
+    tc = np.concatenate([cols, np.array(extra_c, np.int32)])
+    hv, hr, hc = orc.convert(tv, tr, tc)
+    x = rng.standard_normal(3 * s.n)
+    y0 = rng.standard_normal(3 * s.n)
+    dv, dr, dc = upload_bcoo(gpu_ctx, hv, hr, hc)
+    if form == "gather":
+        monkeypatch.setenv("SGIPC_SPMV_GATHER", "1")
+    A = pkg.SpMat(gpu_ctx, dv, dr, dc, hr.size, s.n, want_f32=True)
+    monkeypatch.delenv("SGIPC_SPMV_GATHER", raising=False)
+    try:
+        assert A.long_rows() == 3
+        assert A.layout()[0] == (1 if form == "halo" else 0)
+        dx, dy = gpu_ctx.to_device(x), gpu_ctx.to_device(y0)
+        A.apply(0.9, dx, -0.4, dy)
+        ref = orc.spmv(0.9, hv, hr, hc, x, -0.4, y0, s.n)
+        assert rel(dy.download(), ref) <= 1e-13
+        x32 = gpu_ctx.to_device(x.astype(np.float32), np.float32)
+        y32 = gpu_ctx.to_device(y0.astype(np.float32), np.float32)
+        A.apply(0.9, x32, -0.4, y32)
+        assert rel(y32.download().astype(np.float64), ref) <= 2e-6
+        dy.upload(y0)
+        A.apply_f32_values(0.9, dx, -0.4, dy)
+        assert rel(dy.download(), ref) <= 2e-6
+    finally:
+        A.close()
