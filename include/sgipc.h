@@ -494,7 +494,8 @@ typedef struct sgipc_pcg_stats {
     int status;
     int learned;             /* 1: the learned coarse correction was applied; 2: a run  */
                              /* with it did not converge within twice the plain count   */
-                             /* (automatic mode) and the solve was redone without it    */
+                             /* (automatic mode) or met r.z <= 0 (any mode), and the    */
+                             /* solve was redone without it (iterations count both)     */
     /* system solves (diagnostics; no counterpart in the reference): the MAS    */
     /* coarse mode of the run that returned, whether its level-1 cycle had the  */
     /* exact dense level-2 solve (0: the factorisation was not usable and the    */
@@ -681,7 +682,9 @@ int sgipc_mpnn_set_params(sgipc_mpnn* net, const float* host, int n);
 int sgipc_mpnn_get_params(const sgipc_mpnn* net, float* host, int n);
 int sgipc_mpnn_init_params(sgipc_mpnn* net, unsigned long long seed);
 /* A training set over one level-1 graph, given as the upper BCOO of A_1
- * (device pointers, FP64 column-major blocks, sorted unique). */
+ * (device pointers, FP64 column-major blocks, sorted unique).  A training
+ * set, like a network, uses its context's device and stream: destroy it
+ * before that context. */
 int sgipc_mpnn_data_create(sgipc_ctx* ctx, int n_nodes, const double* vals, const int* rows,
                            const int* cols, int nb, sgipc_mpnn_data** out);
 /* Append samples: X [n_samples][3 n_nodes][3], q [n_samples][3 n_nodes] (device). */
@@ -707,7 +710,8 @@ int sgipc_mpnn_train(sgipc_mpnn* net, sgipc_mpnn_data* d, int epochs, int batch,
 /* Training samples from a system: one plain solve of b (learned level off)
  * records the residual every `stride` iterations (at most max_samples); each
  * becomes (X, q = rho) over the system's level-1 graph, normalised by the RMS
- * of ghat.  FEM-only systems. */
+ * of ghat.  FEM-only systems.  The set belongs to the system's context
+ * (destroy it before that context). */
 int sgipc_system_collect(sgipc_system* sys, const double* b, const sgipc_pcg_opts* opts, int stride,
                          int max_samples, sgipc_mpnn_data** out);
 /* Attach a network to a system's MAS.  mode 0: off; 1: always (flexible PCG);

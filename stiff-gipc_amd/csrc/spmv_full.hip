@@ -1032,24 +1032,6 @@ void FullMatrix::build(hipStream_t s, const double* vals, const int* rows, const
                                       rocprim::plus<int>(), s));
     SGIPC_HIP(rocprim::exclusive_scan(temp_.get(), need5, llen_.get(), loff_.get(), 0, (size_t)n + 1,
                                       rocprim::plus<int>(), s));
-    {
-        int h[2] = {0, 0};
-        SGIPC_HIP(hipMemcpyAsync(&h[0], lrank_.get() + n, sizeof(int), hipMemcpyDeviceToHost, s));
-        SGIPC_HIP(hipMemcpyAsync(&h[1], loff_.get() + n, sizeof(int), hipMemcpyDeviceToHost, s));
-        SGIPC_HIP(hipStreamSynchronize(s));
-        SGIPC_CHECK(h[1] >= 0, SGIPC_CAPACITY, "long-row entry count overflows int");
-        n_long = h[0];
-        long_rows.reserve(n_long > 0 ? n_long : 1);
-        long_ptr.reserve((size_t)n_long + 1);
-        long_col.reserve(h[1] > 0 ? h[1] : 1);
-        long_vals.reserve((size_t)(h[1] > 0 ? h[1] : 1) * 9);
-        if (want_f32) long_vals32.reserve((size_t)(h[1] > 0 ? h[1] : 1) * 9);
-        k_long_fill<<<g256(n + 1), 256, 0, s>>>(vals, cols, skeys_.get(), sidx_.get(), up_ptr_.get(), col_ptr_.get(),
-                                                lflag_.get(), lrank_.get(), loff_.get(), n, kbits, long_rows.get(),
-                                                long_ptr.get(), long_col.get(), long_vals.get(),
-                                                want_f32 ? long_vals32.get() : nullptr);
-        SGIPC_LAUNCH_CHECK();
-    }
     // in-tile ranks: exclusive scan of the candidate flags in column order
     k_cand<<<g256(nnz_up), 256, 0, s>>>(skeys_.get(), nnz_up, kbits, lflag_.get(), cand_.get());
     SGIPC_LAUNCH_CHECK();
@@ -1071,11 +1053,29 @@ void FullMatrix::build(hipStream_t s, const double* vals, const int* rows, const
     SGIPC_LAUNCH_CHECK();
     SGIPC_HIP(rocprim::exclusive_scan(temp_.get(), need4, slice_w.get(), slice_ptr.get(), 0, (size_t)n_slices + 1,
                                       rocprim::plus<int>(), s));
-    int h_slots = 0;
-    SGIPC_HIP(hipMemcpyAsync(&h_slots, slice_ptr.get() + n_slices, sizeof(int), hipMemcpyDeviceToHost, s));
+    // one read back for the slot count and the long rows' counts
+    int hc[3] = {0, 0, 0};
+    SGIPC_HIP(hipMemcpyAsync(&hc[0], slice_ptr.get() + n_slices, sizeof(int), hipMemcpyDeviceToHost, s));
+    SGIPC_HIP(hipMemcpyAsync(&hc[1], lrank_.get() + n, sizeof(int), hipMemcpyDeviceToHost, s));
+    SGIPC_HIP(hipMemcpyAsync(&hc[2], loff_.get() + n, sizeof(int), hipMemcpyDeviceToHost, s));
     SGIPC_HIP(hipStreamSynchronize(s));
-    SGIPC_CHECK(h_slots >= 0, SGIPC_CAPACITY, "SELL slot count overflows int");
-    n_slots = h_slots;
+    SGIPC_CHECK(hc[0] >= 0, SGIPC_CAPACITY, "SELL slot count overflows int");
+    SGIPC_CHECK(hc[2] >= 0, SGIPC_CAPACITY, "long-row entry count overflows int");
+    n_slots = hc[0];
+    n_long = hc[1];
+    {
+        const int ne = hc[2] > 0 ? hc[2] : 1;
+        long_rows.reserve(n_long > 0 ? n_long : 1);
+        long_ptr.reserve((size_t)n_long + 1);
+        long_col.reserve(ne);
+        long_vals.reserve((size_t)ne * 9);
+        if (want_f32) long_vals32.reserve((size_t)ne * 9);
+        k_long_fill<<<g256(n + 1), 256, 0, s>>>(vals, cols, skeys_.get(), sidx_.get(), up_ptr_.get(), col_ptr_.get(),
+                                                lflag_.get(), lrank_.get(), loff_.get(), n, kbits, long_rows.get(),
+                                                long_ptr.get(), long_col.get(), long_vals.get(),
+                                                want_f32 ? long_vals32.get() : nullptr);
+        SGIPC_LAUNCH_CHECK();
+    }
     sell_vals.reserve((size_t)n_slots * 64 * 9);
     sell_cols.reserve((size_t)n_slots * 64);
     if (want_f32) sell_vals32.reserve((size_t)n_slots * 64 * 9);

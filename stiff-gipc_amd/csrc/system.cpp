@@ -810,7 +810,12 @@ int system_solve(sgipc_system* sys, const double* b, double* x, double* grad, co
         };
         status = run_pcg(capped ? oc : o);
         bool abandoned = false;
-        if (capped && status != SGIPC_OK) {
+        int spent = 0;  // iterations of runs that failed and were redone
+        // a learned run that missed its cap, or whose correction made the
+        // preconditioner indefinite (r.z <= 0, in either mode), has lost:
+        // the same call solves again without it
+        if (used && ((capped && status != SGIPC_OK) || status == SGIPC_NOT_SPD)) {
+            spent += pst.iterations;
             sys->learned_it = std::numeric_limits<int>::max() / 2;
             sys->mas->m->net_ = nullptr;
             extra.flexible = false;
@@ -827,7 +832,6 @@ int system_solve(sgipc_system* sys, const double* b, double* x, double* grad, co
         // not cure it, so the weight was not the cause)
         Mas* mm = sys->mas->m.get();
         constexpr double kOmegaFloor = 0.05;
-        int spent = 0;  // iterations of runs that failed and were redone
         auto recover_spd = [&](const sgipc_pcg_opts& oo) {
             if (!(status == SGIPC_NOT_SPD && mm->last_path() == 2 && !used)) return;
             const double w0 = mm->omega0;

@@ -83,7 +83,8 @@ class Mpnn:
 
 class MpnnData:
     """Training samples over one level-1 graph.  Either built from the upper
-    BCOO of A_1 (``from_bcoo``) or filled by a system's collector."""
+    BCOO of A_1 (``from_bcoo``) or filled by a system's collector.  The set
+    uses the context it was made on: close it before closing that context."""
 
     def __init__(self, ctx: Context, h):
         self.ctx, self.h = ctx, h
