@@ -517,7 +517,8 @@ int pcg_solve_impl(Context& c, SolveWorkspace& ws, const double* vals, const int
     const bool mas_fine = prec.mas != nullptr && prec.mas->coarse_mode == 2 && prec.mas_row_offset == 0 &&
                           !(prec.abd != nullptr && prec.abd->n_bodies > 0) && prec.mas->n == n_rows;
     const bool want_f32 = o.precision == SGIPC_FP32_IR || (mas_fine && prec.mas->precision != SGIPC_FP64);
-    ws.full.build(s, vals, rows, cols, nnz, n_rows, want_f32);
+    if (!(ws.prebuilt && ws.full.n_rows == n_rows && ws.full.nnz == nnz && (ws.full.has_f32 || !want_f32)))
+        ws.full.build(s, vals, rows, cols, nnz, n_rows, want_f32);
     if (prec.mas != nullptr && (mas_fine || !prec.mas->build_own_fine)) prec.mas->fine_ = mas_fine ? &ws.full : nullptr;
     if (o.precision != SGIPC_FP32_IR) {
         FullOp<double> A;

@@ -67,6 +67,11 @@ int guarded(sgipc_ctx* ctx, F&& f)
 // pinned host buffer the refinement loop reads its norms through).
 struct SolveWorkspace {
     FullMatrix full;  // tiled symmetric layout of the PCG loop's SpMV
+    // set while `full` already holds the layout of the matrix being solved
+    // (built by the caller, e.g. concurrently with the MAS build); the PCG
+    // entry then skips its own build
+    bool prebuilt = false;
+    SideStream layout_side;
     PcgSolver<double> pcg64;
     PcgSolver<float> pcg32;
     DevBuf<float> r32, d32;

@@ -66,6 +66,40 @@ struct DevBuf {
     T* get() const { return p; }
 };
 
+// A second stream (default priority) for build work that is independent of
+// what the main stream runs meanwhile.  begin(main): work queued on the side
+// stream from now on starts after everything queued on main so far; end(main):
+// work queued on main from now on starts after everything queued on the side
+// stream so far.  Made on first use with the owner's device current.
+struct SideStream {
+    hipStream_t s = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    SideStream() = default;
+    SideStream(const SideStream&) = delete;
+    SideStream& operator=(const SideStream&) = delete;
+    ~SideStream()
+    {
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
+        if (s) (void)hipStreamDestroy(s);
+    }
+    void begin(hipStream_t main)
+    {
+        if (!s) {
+            SGIPC_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+            SGIPC_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+            SGIPC_HIP(hipEventCreateWithFlags(&join, hipEventDisableTiming));
+        }
+        SGIPC_HIP(hipEventRecord(fork, main));
+        SGIPC_HIP(hipStreamWaitEvent(s, fork, 0));
+    }
+    void end(hipStream_t main)
+    {
+        SGIPC_HIP(hipEventRecord(join, s));
+        SGIPC_HIP(hipStreamWaitEvent(main, join, 0));
+    }
+};
+
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;

@@ -907,9 +907,14 @@ __global__ void k_halo_ptr(const uint64_t* __restrict__ k, const int* __restrict
 __global__ void k_tile_stats(const int* __restrict__ halo_ptr, const uint32_t* __restrict__ tinfo, int n_tiles, int n,
                              int* __restrict__ stats)
 {
+    // one atomic per wave and statistic (a single address takes them all)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n_tiles) atomicMax(&stats[0], halo_ptr[i + 1] - halo_ptr[i]);
-    if (i < n) atomicMax(&stats[1], (int)(tinfo[i] >> 16) + (int)(tinfo[i] & 0xFFFF));
+    const int h = wave_max(i < n_tiles ? halo_ptr[i + 1] - halo_ptr[i] : 0);
+    const int t = wave_max(i < n ? (int)(tinfo[i] >> 16) + (int)(tinfo[i] & 0xFFFF) : 0);
+    if ((threadIdx.x & 63) == 0) {
+        if (h > 0) atomicMax(&stats[0], h);
+        if (t > 0) atomicMax(&stats[1], t);
+    }
 }
 
 // rewrite the words as local indices: own rows (column - tile start), halo

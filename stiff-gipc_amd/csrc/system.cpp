@@ -8,7 +8,9 @@
 #include <chrono>
 #include <limits>
 #include <cmath>
+#include <exception>
 #include <memory>
+#include <thread>
 #include <vector>
 
 #include "capi_internal.hpp"
@@ -773,9 +775,38 @@ int system_solve(sgipc_system* sys, const double* b, double* x, double* grad, co
         const bool learned = record == nullptr && use_learned(sys);
         sys->mas->m->keep_graph = sys->net != nullptr || record != nullptr;
         sys->mas->m->build_own_fine = false;  // the PCG matrix serves as coarse mode 2's fine operator
-        sys->mas->m->set_bcoo(sys->bvals.get(), sys->brows.get(), sys->bcols.get(), sys->nnz, off0, mas_pairs,
-                              mas_npairs, mprec);
-        if (nb > 0) sys->abdj.build(s, sys->bvals.get(), sys->brows.get(), sys->bcols.get(), sys->nnz, nb);
+        // the PCG's SpMV layout is built by a helper thread on a second stream
+        // while this thread builds the preconditioners (both only read the
+        // BCOO; each build waits on its own stream's read-backs)
+        SolveWorkspace& ws = *sys->ws;
+        ws.prebuilt = false;
+        ws.layout_side.begin(s);
+        std::exception_ptr layout_err;
+        std::thread layout([&] {
+            try {
+                sys->ctx->c.activate();
+                ws.full.build(ws.layout_side.s, sys->bvals.get(), sys->brows.get(), sys->bcols.get(), sys->nnz,
+                              sys->n_sys, o.precision != SGIPC_FP64);
+            } catch (...) {
+                layout_err = std::current_exception();
+            }
+        });
+        try {
+            sys->mas->m->set_bcoo(sys->bvals.get(), sys->brows.get(), sys->bcols.get(), sys->nnz, off0, mas_pairs,
+                                  mas_npairs, mprec);
+            if (nb > 0) sys->abdj.build(s, sys->bvals.get(), sys->brows.get(), sys->bcols.get(), sys->nnz, nb);
+        } catch (...) {
+            layout.join();
+            throw;
+        }
+        layout.join();
+        if (layout_err) std::rethrow_exception(layout_err);
+        ws.layout_side.end(s);
+        ws.prebuilt = true;
+        struct ClearPrebuilt {
+            SolveWorkspace& w;
+            ~ClearPrebuilt() { w.prebuilt = false; }
+        } clear_prebuilt{ws};
         SGIPC_HIP(hipEventRecord(sys->ev[3], s));
         // ---- PCG
         sgipc_pcg_stats pst{};
