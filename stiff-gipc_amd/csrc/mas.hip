@@ -1835,37 +1835,11 @@ void Mas::assemble_invert_t(const double* vals, const int* rows, const int* cols
     const int ndom0 = h_off[1] / BB;
     const int ndom = total / BB;
     const int ndomc = ndom - ndom0;
+    if (side_.s) side_.end(s);  // side-stream work of an earlier build that ended in an error
     mgal_.reserve((size_t)(ndomc > 0 ? ndomc : 1) * nblk * 9);
     if (ndomc > 0) launch_fill<double>(s, mgal_.get(), 0.0, (long)ndomc * nblk * 9);
     k_row_ptr<<<grid_for(n + 1, 256), 256, 0, s>>>(rows, nnz, offset, n, row_ptr_.get());
     SGIPC_LAUNCH_CHECK();
-    // The level-0 domain inverses read only the fine matrix: they run on the
-    // side stream while the main stream builds the coarse levels (a chain of
-    // small, latency-bound launches and read-backs).  Their output rows
-    // [0, ndom0) are disjoint from the coarse inverses' [ndom0, ndom).
-    constexpr int T = InvShape<BB>::T;
-    const bool half = precision != SGIPC_FP64 && half_level0 && coarse_mode == 2 && ndom0 > 0;
-    if (precision == SGIPC_FP64) {
-        minv64_.reserve((size_t)ndom * nblk * 9);
-    } else {
-        minv32_.reserve((size_t)ndom * nblk * 9);
-        if (half) {
-            minv16_.reserve((size_t)ndom0 * nblk * 9);
-            msc_.reserve((size_t)ndom0 * BB * 3);
-        }
-    }
-    side_.begin(s);
-    if (precision == SGIPC_FP64) {
-        k_inv_level0<BB, double><<<ndom0, T, 0, side_.s>>>(n, vals, cols, row_ptr_.get(), offset, minv64_.get());
-        SGIPC_LAUNCH_CHECK();
-    } else {
-        k_inv_level0<BB, float><<<ndom0, T, 0, side_.s>>>(n, vals, cols, row_ptr_.get(), offset, minv32_.get());
-        SGIPC_LAUNCH_CHECK();
-        if (half) {
-            k_half_level0<BB><<<ndom0, 256, 0, side_.s>>>(minv32_.get(), minv16_.get(), msc_.get());
-            SGIPC_LAUNCH_CHECK();
-        }
-    }
     // coarse operators A_l = P_l^T A_{l-1} P_l, level by level
     if (L > 1 && nnz > 0) {
         coarse_v_[0].reserve((size_t)nnz * 9);
@@ -1909,13 +1883,34 @@ void Mas::assemble_invert_t(const double* vals, const int* rows, const int* cols
             poff = 0;
         }
     }
-    if (ndomc > 0) {
-        if (precision == SGIPC_FP64) k_inv_coarse<BB, double><<<ndomc, T, 0, s>>>(mgal_.get(), ndom0, minv64_.get());
-        else k_inv_coarse<BB, float><<<ndomc, T, 0, s>>>(mgal_.get(), ndom0, minv32_.get());
+    // the domain inverses (the level-2 Gauss-Jordan chain, a few workgroups
+    // per launch, runs beside them on the side stream)
+    constexpr int T = InvShape<BB>::T;
+    if (precision == SGIPC_FP64) {
+        minv64_.reserve((size_t)ndom * nblk * 9);
+        k_inv_level0<BB, double><<<ndom0, T, 0, s>>>(n, vals, cols, row_ptr_.get(), offset, minv64_.get());
         SGIPC_LAUNCH_CHECK();
+        if (ndomc > 0) {
+            k_inv_coarse<BB, double><<<ndomc, T, 0, s>>>(mgal_.get(), ndom0, minv64_.get());
+            SGIPC_LAUNCH_CHECK();
+        }
+    } else {
+        minv32_.reserve((size_t)ndom * nblk * 9);
+        k_inv_level0<BB, float><<<ndom0, T, 0, s>>>(n, vals, cols, row_ptr_.get(), offset, minv32_.get());
+        SGIPC_LAUNCH_CHECK();
+        if (ndomc > 0) {
+            k_inv_coarse<BB, float><<<ndomc, T, 0, s>>>(mgal_.get(), ndom0, minv32_.get());
+            SGIPC_LAUNCH_CHECK();
+        }
+        if (half_level0 && coarse_mode == 2 && ndom0 > 0) {
+            minv16_.reserve((size_t)ndom0 * nblk * 9);
+            msc_.reserve((size_t)ndom0 * BB * 3);
+            k_half_level0<BB><<<ndom0, 256, 0, s>>>(minv32_.get(), minv16_.get(), msc_.get());
+            SGIPC_LAUNCH_CHECK();
+            half_ready_ = true;
+        }
     }
-    side_.end(s);
-    if (half) half_ready_ = true;
+    if (side_.s) side_.end(s);
     R64_.reserve((size_t)total * 3);
     Z64_.reserve((size_t)total * 3);
     R32_.reserve((size_t)total * 3);
@@ -1948,15 +1943,20 @@ void Mas::build_dense_level2(hipStream_t s, const double* v, const int* r, const
         k_dense_pad_identity<<<grid_for(ld - m, 256), 256, 0, s>>>((int)m, ld, dense_inv_.get());
         SGIPC_LAUNCH_CHECK();
     }
+    // the elimination runs on the side stream: its launches are small and
+    // serial, so the level-0 domain inverses fill the chip meanwhile
+    // (assemble_invert_t joins the streams after them)
+    side_.begin(s);
+    hipStream_t g = side_.s;
     for (int k = 0; k < T; ++k) {
-        k_gj_pivot<<<T, 256, 0, s>>>(dense_inv_.get(), ld, k, dense_piv_.get(), dense_col_.get(),
+        k_gj_pivot<<<T, 256, 0, g>>>(dense_inv_.get(), ld, k, dense_piv_.get(), dense_col_.get(),
                                      dense_info_.get());
         SGIPC_LAUNCH_CHECK();
         if (T > 1) {
-            k_gj_row<<<T - 1, 256, 0, s>>>(dense_inv_.get(), ld, k, dense_piv_.get());
+            k_gj_row<<<T - 1, 256, 0, g>>>(dense_inv_.get(), ld, k, dense_piv_.get());
             SGIPC_LAUNCH_CHECK();
         }
-        k_gj_update<<<(T - 1) * T + 1, 256, 0, s>>>(dense_inv_.get(), ld, k, T, dense_piv_.get(), dense_col_.get());
+        k_gj_update<<<(T - 1) * T + 1, 256, 0, g>>>(dense_inv_.get(), ld, k, T, dense_piv_.get(), dense_col_.get());
         SGIPC_LAUNCH_CHECK();
     }
     dense_pending_ = true;
