@@ -369,3 +369,57 @@ def test_vcycle_level1_sums_in_the_spmv_epilogue(pkg, orc, gpu_ctx, monkeypatch,
         err = np.linalg.norm(a - b) / np.linalg.norm(b)
         print(f"{precision}: fused vs separate level-1 sums {err:.2e}")
         assert err <= tol, err
+
+
+@pytest.mark.parametrize("which", ["c0", "grid20_contact"])
+def test_vcycle_pcg_matches_restated_pcg(pkg, orc, gpu_ctx, which, scene_c0):
+    """The headline preconditioner inside the whole PCG: the device's FP64
+    PCG with coarse mode 2 against the reference's PCG loop (cemas.pcg, the
+    loop of pcg_solver.cu:191-332) around the numpy restatement of the
+    two-grid cycle, given the device's own domain inverses.  After a fixed
+    11 iterations the iterates agree to 1e-6 (the applies agree to 1e-7);
+    run to ||r||/||b|| <= 1e-5 the iteration counts are at most one apart and
+    both solutions meet the tolerance."""
+    from oracle import cemas
+    if which == "c0":
+        s = scene_c0
+    else:
+        s = pkg.scenes.tet_grid_scene("cycle_pcg20", 20, E_loguniform=(1e4, 1e9), n_pairs=1500)
+    B = 32
+    vals, rows, cols = orc.assemble(s)
+    om = orc.Mas(s, B)
+    om.build(vals, rows, cols, 0, s.pairs)
+    parts = mpnn.MasParts(om)
+    if parts.L <= 2 or parts.cnt[1] <= B:
+        pytest.skip("level 1 is one domain (solved exactly): the cycle stays off")
+    mas, keep = _vcycle_mas(pkg, gpu_ctx, s, B, vals, rows, cols, pkg.FP64, omega=0.4)
+    try:
+        parts.inv = mpnn._domain_dense(mas.inverse_blocks().reshape(-1, B * (B + 1) // 2, 9), B)
+        A = mpnn.bcoo_to_csr(vals, rows, cols, s.n)
+        graph = mpnn.coarse_graph(A, parts.map1, parts.cnt[1])
+        A1f = mpnn.float_operator(graph)
+        A2 = mpnn.level2_operator(parts, graph)
+        APf = mpnn.ap_operator(A, parts.map1, parts.cnt[1])
+        M = lambda r: mpnn.mas_vcycle_apply(parts, A1f, A, APf, r, A2=A2, omega=0.4)
+        db = gpu_ctx.to_device(s.rhs)
+        # fixed iteration count
+        k = 12
+        xr, itr = cemas.pcg(A, s.rhs, M, tol=1e-30, stop_mode=1, max_iter=k)
+        dx = gpu_ctx.empty(3 * s.n, np.float64)
+        st = pkg.PCGSolver(gpu_ctx, tol=1e-30, max_iter=k).solve(keep[1], keep[2], keep[3], rows.size, s.n, mas,
+                                                                db, dx)
+        assert st.iterations == itr == k - 1
+        err = np.linalg.norm(dx.download() - xr) / np.linalg.norm(xr)
+        assert err <= 1e-6, err
+        # to the tolerance
+        xr, itr = cemas.pcg(A, s.rhs, M, tol=1e-5, stop_mode=1)
+        dx = gpu_ctx.empty(3 * s.n, np.float64)
+        st = pkg.PCGSolver(gpu_ctx, tol=1e-5, stop_mode=1).solve(keep[1], keep[2], keep[3], rows.size, s.n, mas,
+                                                                 db, dx)
+        print(f"{which}: device {st.iterations} iterations, restated {itr}")
+        assert abs(st.iterations - itr) <= 1, (st.iterations, itr)
+        assert st.rel_residual <= 1e-5
+        assert np.linalg.norm(s.rhs - A @ xr) / np.linalg.norm(s.rhs) <= 1e-5
+    finally:
+        mas.close()
+        om.close()
