@@ -109,7 +109,7 @@ def main():
     ap.add_argument("--fuse", type=int, default=1, choices=[0, 1, 2],
                     help="0: separate x/r and p update kernels; 1: x/r update inside the first MAS kernel; "
                          "2: also the p update inside the SpMV")
-    ap.add_argument("--cpu-reps", type=int, default=1,
+    ap.add_argument("--cpu-reps", type=int, default=3,
                     help="full CPU-baseline solves timed (the median is reported)")
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="OpenMP threads of the CPU baseline (0: every core this process may use)")
