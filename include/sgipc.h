@@ -18,6 +18,10 @@
  * Threading: one host thread per context; all work is ordered on the
  * context's HIP stream and every function returns after its results are
  * complete (synchronous semantics, like the reference's callers expect).
+ * Inside a call the library may run independent build work on streams of
+ * its own (default priority), ordered against the context's stream by
+ * events, and a system solve builds its SpMV layout on a helper thread that
+ * is joined before the call goes on; nothing outlives the call.
  */
 #ifndef SGIPC_H
 #define SGIPC_H
