@@ -1779,9 +1779,8 @@ void Mas::init_neighbor(int vn, int cof, int total_nbr, const uint32_t* start, c
 }
 
 template <int BB>
-void Mas::build_hierarchy_t(const int* pairs, int n_pairs)
+void Mas::build_hierarchy_t(hipStream_t s, const int* pairs, int n_pairs)
 {
-    hipStream_t s = ctx->stream;
     std::vector<int> hmeta(kMetaSize, 0);
     hmeta[kMetaCnt + 0] = n;
     hmeta[kMetaOff + 0] = 0;
@@ -1821,10 +1820,21 @@ void Mas::build_hierarchy_t(const int* pairs, int n_pairs)
     total = h_off[L];
 }
 
-void Mas::build_hierarchy(const int* pairs, int n_pairs)
+void Mas::build_hierarchy(hipStream_t s, const int* pairs, int n_pairs)
 {
-    if (B == 32) build_hierarchy_t<32>(pairs, n_pairs);
-    else build_hierarchy_t<16>(pairs, n_pairs);
+    if (B == 32) build_hierarchy_t<32>(s, pairs, n_pairs);
+    else build_hierarchy_t<16>(s, pairs, n_pairs);
+}
+
+void Mas::prebuild_hierarchy(hipStream_t s, const int* pairs, int n_pairs)
+{
+    SGIPC_CHECK(n >= 1, SGIPC_EMPTY, "init_neighbor first");
+    SGIPC_CHECK(n_pairs == 0 || pairs != nullptr, SGIPC_BAD_ARG, "null pairs");
+    hier_ready_ = false;
+    build_hierarchy(s, pairs, n_pairs);  // ends with a read-back: s is drained
+    hier_ready_ = true;
+    hier_pairs_ = pairs;
+    hier_npairs_ = n_pairs;
 }
 
 template <int BB>
@@ -2051,7 +2061,8 @@ void Mas::build_ap(const double* vals, const int* rows, const int* cols, int nnz
 void Mas::init_matrix()
 {
     SGIPC_CHECK(n >= 1, SGIPC_EMPTY, "init_neighbor first");
-    build_hierarchy(nullptr, 0);
+    hier_ready_ = false;
+    build_hierarchy(ctx->stream, nullptr, 0);
 }
 
 void Mas::set_bcoo(const double* vals, const int* rows, const int* cols, int nnz, int offset,
@@ -2067,7 +2078,12 @@ void Mas::set_bcoo(const double* vals, const int* rows, const int* cols, int nnz
     half_ready_ = false;
     last_path_ = -1;
     dense_pending_ = false;  // a build that ended in an error left its check unread
-    build_hierarchy(pairs, n_pairs);
+    if (hier_ready_ && hier_pairs_ == pairs && hier_npairs_ == n_pairs) {
+        hier_ready_ = false;  // built ahead by prebuild_hierarchy
+    } else {
+        hier_ready_ = false;
+        build_hierarchy(ctx->stream, pairs, n_pairs);
+    }
     if (B == 32) assemble_invert_t<32>(vals, rows, cols, nnz, offset);
     else assemble_invert_t<16>(vals, rows, cols, nnz, offset);
     ap_nnz_ = 0;

@@ -25,6 +25,11 @@ class Mas {
     void init_matrix();
     void set_bcoo(const double* vals, const int* rows, const int* cols, int nnz, int offset,
                   const int* pairs, int n_pairs, int precision);
+    // The hierarchy of the next set_bcoo (it depends on the connectivity and
+    // the contact pairs only), built ahead on stream s, e.g. while the caller
+    // assembles the matrix; returns after its work on s is complete.  The
+    // next set_bcoo with the same pairs pointer and count uses it.
+    void prebuild_hierarchy(hipStream_t s, const int* pairs, int n_pairs);
     // z = M^-1 r, rz partials; see Preconditioner.
     template <typename TV>
     int apply(hipStream_t s, const TV* r, TV* z, double* rz_part, const PcgStopDesc& sd);
@@ -93,9 +98,12 @@ class Mas {
     double ms_build = 0.0;
 
   private:
-    void build_hierarchy(const int* pairs, int n_pairs);
+    void build_hierarchy(hipStream_t s, const int* pairs, int n_pairs);
     template <int BB>
-    void build_hierarchy_t(const int* pairs, int n_pairs);
+    void build_hierarchy_t(hipStream_t s, const int* pairs, int n_pairs);
+    bool hier_ready_ = false;  // prebuilt for (hier_pairs_, hier_npairs_)
+    const int* hier_pairs_ = nullptr;
+    int hier_npairs_ = 0;
     template <int BB>
     void assemble_invert_t(const double* vals, const int* rows, const int* cols, int nnz, int offset);
     template <int BB, typename TV, typename TS>

@@ -761,9 +761,29 @@ int system_solve(sgipc_system* sys, const double* b, double* x, double* grad, co
             xp = sys->xs.get();
         }
         SGIPC_HIP(hipEventRecord(sys->ev[1], s));
-        // ---- triplets -> unique upper BCOO
-        sys->nnz = sys->conv.convert(s, sys->tvals.get(), sys->trows.get(), sys->tcols.get(), off, sys->n_sys,
-                                     sys->bvals.get(), sys->brows.get(), sys->bcols.get(), sys->capacity);
+        // ---- triplets -> unique upper BCOO; meanwhile a helper thread builds
+        // the MAS hierarchy (connectivity and contact pairs only) on a second
+        // stream that starts after the pairs are written
+        SolveWorkspace& ws = *sys->ws;
+        ws.layout_side.begin(s);
+        std::exception_ptr hier_err;
+        std::thread hier([&] {
+            try {
+                sys->ctx->c.activate();
+                sys->mas->m->prebuild_hierarchy(ws.layout_side.s, mas_pairs, mas_npairs);
+            } catch (...) {
+                hier_err = std::current_exception();
+            }
+        });
+        try {
+            sys->nnz = sys->conv.convert(s, sys->tvals.get(), sys->trows.get(), sys->tcols.get(), off, sys->n_sys,
+                                         sys->bvals.get(), sys->brows.get(), sys->bcols.get(), sys->capacity);
+        } catch (...) {
+            hier.join();
+            throw;
+        }
+        hier.join();
+        if (hier_err) std::rethrow_exception(hier_err);
         SGIPC_HIP(hipEventRecord(sys->ev[2], s));
         // ---- preconditioners: MAS on the FEM rows (current contacts), block-Jacobi on the bodies
         // mixed precision failed on an earlier solve: FP64 from the start
@@ -778,7 +798,6 @@ int system_solve(sgipc_system* sys, const double* b, double* x, double* grad, co
         // the PCG's SpMV layout is built by a helper thread on a second stream
         // while this thread builds the preconditioners (both only read the
         // BCOO; each build waits on its own stream's read-backs)
-        SolveWorkspace& ws = *sys->ws;
         ws.prebuilt = false;
         ws.layout_side.begin(s);
         std::exception_ptr layout_err;
