@@ -1528,10 +1528,11 @@ __global__ void k_ap_cols(const uint64_t* __restrict__ k, const int* __restrict_
 // keys); for u = n the lower bound is the count of real keys, which ends the
 // last segment (seg_start[n_uniq])
 __global__ void k_ap_ptr(const uint64_t* __restrict__ k, const int* __restrict__ inc, int m, int n, int cb,
-                         int n_uniq, int* __restrict__ ptr, int* __restrict__ seg_start)
+                         int* __restrict__ ptr, int* __restrict__ seg_start)
 {
     const int u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u > n) return;
+    const int n_uniq = inc[m - 1];
     const uint64_t key = (uint64_t)(uint32_t)u << cb;
     int lo = 0, hi = m;
     while (lo < hi) {
@@ -1545,10 +1546,10 @@ __global__ void k_ap_ptr(const uint64_t* __restrict__ k, const int* __restrict__
 // every distinct (u, g): its blocks summed in sorted order in FP64, stored in
 // FP32 (column-major 3x3, as the BCOO)
 __global__ void k_ap_sum(const double* __restrict__ vals, const int* __restrict__ idx, const int* __restrict__ seg_start,
-                         int n_uniq, float* __restrict__ blk)
+                         const int* __restrict__ inc, int m, float* __restrict__ blk)
 {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n_uniq) return;
+    if (q >= inc[m - 1]) return;  // the distinct (row, level-1 column) pairs
     const int e0 = seg_start[q], e1 = seg_start[q + 1];
     double a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int e = e0; e < e1; ++e) {
@@ -1838,7 +1839,8 @@ void Mas::prebuild_hierarchy(hipStream_t s, const int* pairs, int n_pairs)
 }
 
 template <int BB>
-void Mas::assemble_invert_t(const double* vals, const int* rows, const int* cols, int nnz, int offset)
+void Mas::assemble_invert_t(const double* vals, const int* rows, const int* cols, int nnz, int offset,
+                            bool want_ap)
 {
     hipStream_t s = ctx->stream;
     constexpr int nblk = BB * (BB + 1) / 2;
@@ -1850,6 +1852,12 @@ void Mas::assemble_invert_t(const double* vals, const int* rows, const int* cols
     if (ndomc > 0) launch_fill<double>(s, mgal_.get(), 0.0, (long)ndomc * nblk * 9);
     k_row_ptr<<<grid_for(n + 1, 256), 256, 0, s>>>(rows, nnz, offset, n, row_ptr_.get());
     SGIPC_LAUNCH_CHECK();
+    // A P of coarse mode 2 (needs the hierarchy and the fine matrix only) on
+    // the side stream, beside the coarse levels; joined below
+    if (want_ap) {
+        side_.begin(s);
+        build_ap(side_.s, vals, rows, cols, nnz);
+    }
     // coarse operators A_l = P_l^T A_{l-1} P_l, level by level
     if (L > 1 && nnz > 0) {
         coarse_v_[0].reserve((size_t)nnz * 9);
@@ -2010,9 +2018,12 @@ void Mas::learned_sample(hipStream_t s, const double* r, double* ztmp, float* X,
     SGIPC_LAUNCH_CHECK();
 }
 
-void Mas::build_ap(const double* vals, const int* rows, const int* cols, int nnz)
+// Without a read-back: the entry count n_uniq stays on the device (the last
+// element of the flags' inclusive scan) and the CSR arrays are sized for the
+// upper bound 2 nnz, so the build can run on a side stream beside the
+// coarse levels.
+void Mas::build_ap(hipStream_t s, const double* vals, const int* rows, const int* cols, int nnz)
 {
-    hipStream_t s = ctx->stream;
     SGIPC_CHECK(nnz < (1 << 30), SGIPC_CAPACITY, "A P entry count overflows int");
     const int m = 2 * nnz;
     int kb = 1;
@@ -2026,6 +2037,10 @@ void Mas::build_ap(const double* vals, const int* rows, const int* cols, int nnz
     api_sorted_.reserve(m);
     apf_.reserve(m);
     apinc_.reserve(m);
+    ap_ptr_.reserve(n + 1);
+    ap_col_.reserve(m);
+    ap_seg_.reserve((size_t)m + 1);
+    ap_blk_.reserve(9L * m);
     size_t n1 = 0, n2 = 0;
     SGIPC_HIP(rocprim::radix_sort_pairs(nullptr, n1, apk_.get(), apk_sorted_.get(), api_.get(), api_sorted_.get(),
                                         (size_t)m, 0, end_bit, s));
@@ -2039,23 +2054,16 @@ void Mas::build_ap(const double* vals, const int* rows, const int* cols, int nnz
     SGIPC_LAUNCH_CHECK();
     SGIPC_HIP(rocprim::inclusive_scan(ap_temp_.get(), n2, apf_.get(), apinc_.get(), (size_t)m, rocprim::plus<int>(),
                                       s));
-    int n_uniq = 0;
-    SGIPC_HIP(hipMemcpyAsync(&n_uniq, apinc_.get() + (m - 1), sizeof(int), hipMemcpyDeviceToHost, s));
-    SGIPC_HIP(hipStreamSynchronize(s));
-    SGIPC_CHECK(n_uniq > 0, SGIPC_EMPTY, "A P has no entries");
-    ap_ptr_.reserve(n + 1);
-    ap_col_.reserve(n_uniq);
-    ap_seg_.reserve(n_uniq + 1);
-    ap_blk_.reserve(9L * n_uniq);
     k_ap_cols<<<grid_for(m, 256), 256, 0, s>>>(apk_sorted_.get(), apf_.get(), apinc_.get(), m, cb, ap_col_.get(),
                                                ap_seg_.get());
     SGIPC_LAUNCH_CHECK();
-    k_ap_ptr<<<grid_for(n + 1, 256), 256, 0, s>>>(apk_sorted_.get(), apinc_.get(), m, n, cb, n_uniq, ap_ptr_.get(),
+    k_ap_ptr<<<grid_for(n + 1, 256), 256, 0, s>>>(apk_sorted_.get(), apinc_.get(), m, n, cb, ap_ptr_.get(),
                                                   ap_seg_.get());
     SGIPC_LAUNCH_CHECK();
-    k_ap_sum<<<grid_for(n_uniq, 256), 256, 0, s>>>(vals, api_sorted_.get(), ap_seg_.get(), n_uniq, ap_blk_.get());
+    k_ap_sum<<<grid_for(m, 256), 256, 0, s>>>(vals, api_sorted_.get(), ap_seg_.get(), apinc_.get(), m,
+                                              ap_blk_.get());
     SGIPC_LAUNCH_CHECK();
-    ap_nnz_ = n_uniq;
+    ap_nnz_ = m;  // an upper bound: A P is built
 }
 
 void Mas::init_matrix()
@@ -2084,15 +2092,13 @@ void Mas::set_bcoo(const double* vals, const int* rows, const int* cols, int nnz
         hier_ready_ = false;
         build_hierarchy(ctx->stream, pairs, n_pairs);
     }
-    if (B == 32) assemble_invert_t<32>(vals, rows, cols, nnz, offset);
-    else assemble_invert_t<16>(vals, rows, cols, nnz, offset);
     ap_nnz_ = 0;
-    if (coarse_mode == 2 && offset == 0 && L > 2 && nnz > 0) {
-        build_ap(vals, rows, cols, nnz);
-        if (build_own_fine) {
-            own_fine_.build(ctx->stream, vals, rows, cols, nnz, n, precision != SGIPC_FP64);
-            fine_ = &own_fine_;
-        }
+    const bool want_ap = coarse_mode == 2 && offset == 0 && L > 2 && nnz > 0;
+    if (B == 32) assemble_invert_t<32>(vals, rows, cols, nnz, offset, want_ap);
+    else assemble_invert_t<16>(vals, rows, cols, nnz, offset, want_ap);
+    if (want_ap && build_own_fine) {
+        own_fine_.build(ctx->stream, vals, rows, cols, nnz, n, precision != SGIPC_FP64);
+        fine_ = &own_fine_;
     }
     finish_dense_level2(ctx->stream);
     SGIPC_HIP(hipStreamSynchronize(ctx->stream));

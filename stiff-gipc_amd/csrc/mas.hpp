@@ -105,7 +105,8 @@ class Mas {
     const int* hier_pairs_ = nullptr;
     int hier_npairs_ = 0;
     template <int BB>
-    void assemble_invert_t(const double* vals, const int* rows, const int* cols, int nnz, int offset);
+    void assemble_invert_t(const double* vals, const int* rows, const int* cols, int nnz, int offset,
+                           bool want_ap);
     template <int BB, typename TV, typename TS>
     int apply_t(hipStream_t s, const TV* r, TV* z, double* rz_part, const PcgStopDesc& sd);
     template <int BB, typename TV, typename TS>
@@ -129,7 +130,7 @@ class Mas {
     }
     // A P for the post-smoothing residual of coarse mode 2 (CSR over the fine
     // rows, level-1 columns, FP32 3x3 blocks)
-    void build_ap(const double* vals, const int* rows, const int* cols, int nnz);
+    void build_ap(hipStream_t s, const double* vals, const int* rows, const int* cols, int nnz);
     DevBuf<int> ap_ptr_, ap_col_, ap_seg_;
     DevBuf<float> ap_blk_;
     DevBuf<double4> c4_;  // level-1 correction, 4-padded (float4 view in FP32 applies)
