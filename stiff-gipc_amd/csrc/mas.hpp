@@ -176,7 +176,7 @@ class Mas {
     DevBuf<double> coarse_v_[2], map_v_;
     DevBuf<int> coarse_r_[2], coarse_c_[2], map_r_, map_c_;
     Converter conv_;
-    SideStream side_;  // level-0 inverse, concurrent with the coarse levels' build
+    SideStream side_;  // A P and the level-2 elimination, beside the coarse levels and the domain inverses
 };
 
 // Adapter giving the PCG driver a preconditioner over a Mas.

@@ -356,6 +356,20 @@ PcgResult PcgSolver<TV>::solve(hipStream_t s, const SpmvOp<TV>& A, Preconditione
     launch_fill<TV>(s, x, TV(0), n3);
     SGIPC_HIP(hipMemcpyAsync(r, b, sizeof(TV) * n3, hipMemcpyDeviceToDevice, s));
     int n_bb = launch_dot_partials<TV, TV>(s, b, b, n3, part_bb_.get());
+    // flexible PCG: z.Ap partials from the preconditioner's last kernel,
+    // switched on before its first apply so every apply emits the same
+    // partial layout (Ap zero until the first SpMV); switched off on every
+    // way out
+    launch_fill<TV>(s, Ap, TV(0), n3);
+    const bool flex = opt.flexible && M != nullptr && M->set_flex(Ap, part_zap_.get());
+    struct FlexOff {
+        Preconditioner<TV>* m;
+        bool on;
+        ~FlexOff()
+        {
+            if (on) m->set_flex(nullptr, nullptr);
+        }
+    } flex_off{M, flex};
     // z = M r (no stop test), rz0 partials
     int cur = 0;
     PcgStopDesc none{};
@@ -370,15 +384,12 @@ PcgResult PcgSolver<TV>::solve(hipStream_t s, const SpmvOp<TV>& A, Preconditione
                                            opt.tol, opt.stop_mode, opt.diverge);
     SGIPC_LAUNCH_CHECK();
     SGIPC_HIP(hipMemcpyAsync(pbuf[pb], z, sizeof(TV) * n3, hipMemcpyDeviceToDevice, s));
-    launch_fill<TV>(s, Ap, TV(0), n3);
 
     // Fused iteration (when both operators offer it):
     //   SpMV:  p_k = z + beta p_{k-1} gathered on the fly, Ap = A p_k, p.Ap
     //   MAS:   x += alpha p_k, r -= alpha Ap, ||r||^2 in the restriction;
     //          stop rule in the first coarse restriction; z = M r, r.z
     // i.e. the p update of iteration k-1 is deferred into the SpMV of k.
-    // flexible: z.Ap partials from the preconditioner's last kernel
-    const bool flex = opt.flexible && M != nullptr && M->set_flex(Ap, part_zap_.get());
     const bool fuse_p = !flex && opt.fuse >= 2 && A.fuses_update_p();
     bool fuse_xr = opt.fuse >= 1 && M != nullptr;  // cleared when the preconditioner declines
     const bool replace = sizeof(TV) == 8 && opt.true_op != nullptr && opt.replace_every > 0;
@@ -543,7 +554,6 @@ PcgResult PcgSolver<TV>::solve(hipStream_t s, const SpmvOp<TV>& A, Preconditione
             pending = slot;
         }
     }
-    if (flex) M->set_flex(nullptr, nullptr);
     if (gexec) (void)hipGraphExecDestroy(gexec);
     SGIPC_HIP(hipStreamSynchronize(s));
     SGIPC_HIP(hipMemcpy(host_ctl_, ctl, sizeof(PcgCtl), hipMemcpyDeviceToHost));

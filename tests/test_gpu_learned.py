@@ -37,6 +37,7 @@ def test_collected_samples_match_oracle(pkg, orc):
     s = _scene(pkg, 1)
     B = 16
     ls = pkg.LinearSystem(s, bank_size=B)
+    data = None
     try:
         data = ls.collect(stride=1, max_samples=4)
         assert data.size == 4
@@ -55,8 +56,9 @@ def test_collected_samples_match_oracle(pkg, orc):
             assert err <= 1e-5, (k, err)
             np.testing.assert_allclose(q[k], X[k][:, 2], rtol=0, atol=0)
         m.close()
-        data.close()
     finally:
+        if data is not None:
+            data.close()  # before its context
         ls.close()
 
 
@@ -69,6 +71,7 @@ def test_learned_level_on_a_held_out_scene(pkg, coarse):
     B = 16
     train = pkg.LinearSystem(_scene(pkg, 1), bank_size=B, coarse_mode=coarse)
     test = pkg.LinearSystem(_scene(pkg, 2), bank_size=B, coarse_mode=coarse)
+    data = net = None
     try:
         data = train.collect(stride=1, max_samples=200)
         for k in range(2):
@@ -95,9 +98,12 @@ def test_learned_level_on_a_held_out_scene(pkg, coarse):
         better_learned = runs[1].iterations < runs[0].iterations
         assert runs[2].learned == better_learned
         assert all(r.true_rel_residual <= 1e-5 for r in runs)
-        net.close()
-        data.close()
     finally:
+        # the network and the training set use their contexts: close them first
+        if net is not None:
+            net.close()
+        if data is not None:
+            data.close()
         train.close()
         test.close()
 
@@ -108,6 +114,7 @@ def test_automatic_mode_drops_a_harmful_correction(pkg):
     solves again without the correction (still converged), and the following
     solves stay plain."""
     ls = pkg.LinearSystem(_scene(pkg, 2), bank_size=16, coarse_mode=1)
+    net = None
     try:
         net = pkg.learned.Mpnn(ls.ctx, layers=2, linear=True, seed=3)
         p = net.get_params()
@@ -126,6 +133,7 @@ def test_automatic_mode_drops_a_harmful_correction(pkg):
             assert probe.iterations >= plain.iterations
         after = [ls.solve(tol=1e-5) for _ in range(2)]
         assert all(not r.learned and r.status == 0 for r in after)
-        net.close()
     finally:
+        if net is not None:
+            net.close()  # before its context
         ls.close()
