@@ -7,6 +7,7 @@ library is missing -- there is no CPU fallback.
 from __future__ import annotations
 
 import ctypes
+import weakref
 import os
 
 import numpy as np
@@ -242,6 +243,12 @@ class Context:
         check(lib().sgipc_create(ctypes.byref(h), int(device)))
         self.h = h
         self._arrays = set()
+        self._deps = weakref.WeakSet()  # objects that use this context: closed first
+
+    def adopt(self, obj):
+        """Register an object (with a close() method) that uses this context;
+        close() closes it before the context goes."""
+        self._deps.add(obj)
 
     def empty(self, shape, dtype):
         shape = tuple(np.atleast_1d(shape)) if not isinstance(shape, tuple) else shape
@@ -259,6 +266,8 @@ class Context:
 
     def close(self):
         if self.h:
+            for d in list(self._deps):
+                d.close()
             for a in list(self._arrays):
                 a.free()
             lib().sgipc_destroy(self.h)

@@ -28,13 +28,15 @@ def n_params(layers: int) -> int:
 
 class Mpnn:
     """Network parameters on the device; ``linear`` selects the identity
-    activation instead of relu."""
+    activation instead of relu.  Uses the context it was made on (closing the
+    context closes the network first)."""
 
     def __init__(self, ctx: Context, layers: int = 3, linear: bool = False, seed: int = 0):
         self.ctx, self.layers, self.linear = ctx, layers, bool(linear)
         h = ctypes.c_void_p()
         check(lib().sgipc_mpnn_create(ctx.h, layers, int(self.linear), ctypes.byref(h)), ctx.h)
         self.h = h
+        ctx.adopt(self)
         if seed:
             check(lib().sgipc_mpnn_init_params(self.h, seed), ctx.h)
 
@@ -71,7 +73,8 @@ class Mpnn:
 
     def close(self):
         if getattr(self, "h", None):
-            lib().sgipc_mpnn_destroy(self.h)
+            if self.ctx.h:  # a closed context closed this network first
+                lib().sgipc_mpnn_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -84,10 +87,11 @@ class Mpnn:
 class MpnnData:
     """Training samples over one level-1 graph.  Either built from the upper
     BCOO of A_1 (``from_bcoo``) or filled by a system's collector.  The set
-    uses the context it was made on: close it before closing that context."""
+    uses the context it was made on (closing the context closes it first)."""
 
     def __init__(self, ctx: Context, h):
         self.ctx, self.h = ctx, h
+        ctx.adopt(self)
 
     @classmethod
     def from_bcoo(cls, ctx: Context, n_nodes: int, vals, rows, cols):
@@ -138,7 +142,8 @@ class MpnnData:
 
     def close(self):
         if getattr(self, "h", None):
-            lib().sgipc_mpnn_data_destroy(self.h)
+            if self.ctx.h:  # a closed context closed this set first
+                lib().sgipc_mpnn_data_destroy(self.h)
             self.h = None
 
     def __del__(self):
