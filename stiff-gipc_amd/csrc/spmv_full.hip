@@ -1284,15 +1284,17 @@ template <typename TM, typename TV>
 void FullMatrix::launch_long(hipStream_t s, TV a, const TV* x, TV* y, double* dot_part, const int* done) const
 {
     if (n_long <= 0) return;
-    const TM* lv;
-    if constexpr (sizeof(TM) == 8) lv = long_vals.get();
+    // FP64 vectors read the long rows' FP64 values also when the tiles' are FP32
+    using TLV = typename std::conditional<sizeof(TV) == 8, double, TM>::type;
+    const TLV* lv;
+    if constexpr (sizeof(TLV) == 8) lv = long_vals.get();
     else lv = long_vals32.get();
     double* dp = dot_part ? dot_part + n_tiles : nullptr;
     if (dot_part)
-        go(&k_spmv_long<TM, TV, true>, n_long, 256, 0, s, nullptr, nullptr, (const int*)long_rows.get(),
+        go(&k_spmv_long<TLV, TV, true>, n_long, 256, 0, s, nullptr, nullptr, (const int*)long_rows.get(),
            (const int*)long_ptr.get(), (const int*)long_col.get(), lv, a, x, y, dp, done);
     else
-        go(&k_spmv_long<TM, TV, false>, n_long, 256, 0, s, nullptr, nullptr, (const int*)long_rows.get(),
+        go(&k_spmv_long<TLV, TV, false>, n_long, 256, 0, s, nullptr, nullptr, (const int*)long_rows.get(),
            (const int*)long_ptr.get(), (const int*)long_col.get(), lv, a, x, y, dp, done);
     SGIPC_LAUNCH_CHECK();
 }

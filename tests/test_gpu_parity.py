@@ -449,7 +449,10 @@ def test_spmv_long_rows_match_oracle(pkg, orc, gpu_ctx, scene_small, monkeypatch
     """Rows of more than 128 blocks (here three hub rows joined to a few
     hundred vertices each, one of them the first row, one in the middle and
     one near the end) are kept outside the tiles and multiplied by their own
-    kernel; every precision of the tiled product still equals the oracle's."""
+    kernel; every precision of the tiled product still equals the oracle's.
+    With FP64 vectors and FP32 tile values the long rows still read their
+    FP64 values (stiff hub rows, e.g. affine bodies, lose most in FP32), so
+    those rows match the FP64 product."""
     s = scene_small
     vals, rows, cols = orc.assemble(s)
     rng = np.random.default_rng(5)
@@ -485,6 +488,10 @@ def test_spmv_long_rows_match_oracle(pkg, orc, gpu_ctx, scene_small, monkeypatch
         assert rel(y32.download().astype(np.float64), ref) <= 2e-6
         dy.upload(y0)
         A.apply_f32_values(0.9, dx, -0.4, dy)
-        assert rel(dy.download(), ref) <= 2e-6
+        got = dy.download()
+        assert rel(got, ref) <= 2e-6
+        hubs = np.array([0, s.n // 2, s.n - 3])
+        idx = (3 * hubs[:, None] + np.arange(3)).ravel()
+        assert rel(got[idx], ref[idx]) <= 1e-13
     finally:
         A.close()
