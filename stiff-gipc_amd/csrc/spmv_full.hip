@@ -599,15 +599,11 @@ __global__ void k_cand(const uint64_t* __restrict__ skeys, int nnz, int kbits, c
     cand[s] = (i != j && i / kTile == j / kTile && !lflag[i] && !lflag[j]) ? 1 : 0;
 }
 
-// The full rows of the long rows (CSR, one thread per long row): the lower
-// blocks (r, i), r < i, transposed, by r, then the upper blocks by column.
-__global__ void k_long_fill(const double* __restrict__ vals, const int* __restrict__ cols,
-                            const uint64_t* __restrict__ skeys, const int* __restrict__ sidx,
-                            const int* __restrict__ up_ptr, const int* __restrict__ col_ptr,
-                            const int* __restrict__ lflag, const int* __restrict__ lrank,
-                            const int* __restrict__ loff, int n, int kbits, int* __restrict__ long_rows,
-                            int* __restrict__ long_ptr, int* __restrict__ long_col, double* __restrict__ lv,
-                            float* __restrict__ lv32)
+// Rank -> row of the long rows and their CSR starts (one thread per row;
+// thread n writes the end).
+__global__ void k_long_rows(const int* __restrict__ lflag, const int* __restrict__ lrank,
+                            const int* __restrict__ loff, int n, int* __restrict__ long_rows,
+                            int* __restrict__ long_ptr)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i > n) return;
@@ -616,11 +612,34 @@ __global__ void k_long_fill(const double* __restrict__ vals, const int* __restri
         return;
     }
     if (!lflag[i]) return;
-    const int k = lrank[i];
-    long_rows[k] = i;
-    long_ptr[k] = loff[i];
-    long e = loff[i];
-    auto put = [&](int c, int b, bool tr) {
+    long_rows[lrank[i]] = i;
+    long_ptr[lrank[i]] = loff[i];
+}
+
+// The full rows of the long rows (CSR, one workgroup per long row, one entry
+// per thread): the lower blocks (r, i), r < i, transposed, by r, then the
+// upper blocks by column.  Column i's entries are sorted by r <= i, so the
+// diagonal, when present, is the last of them and is taken from the upper
+// blocks only.
+constexpr int kLongFillThreads = 128;
+__global__ __launch_bounds__(kLongFillThreads) void k_long_fill(const double* __restrict__ vals, const int* __restrict__ cols,
+                            const uint64_t* __restrict__ skeys, const int* __restrict__ sidx,
+                            const int* __restrict__ up_ptr, const int* __restrict__ col_ptr,
+                            const int* __restrict__ long_rows, const int* __restrict__ long_ptr, int kbits,
+                            int* __restrict__ long_col, double* __restrict__ lv, float* __restrict__ lv32)
+{
+    const int k = blockIdx.x;
+    const int i = long_rows[k];
+    const long e0 = long_ptr[k];
+    const int c0 = col_ptr[i], c1 = col_ptr[i + 1];
+    const bool diag = c1 > c0 && (int)(skeys[c1 - 1] & ((1ull << kbits) - 1ull)) == i;
+    const int nl = c1 - c0 - (diag ? 1 : 0);
+    const int nu = up_ptr[i + 1] - up_ptr[i];
+    for (int t = threadIdx.x; t < nl + nu; t += kLongFillThreads) {
+        const bool tr = t < nl;
+        const int c = tr ? (int)(skeys[c0 + t] & ((1ull << kbits) - 1ull)) : cols[up_ptr[i] + t - nl];
+        const int b = tr ? sidx[c0 + t] : up_ptr[i] + t - nl;
+        const long e = e0 + t;
         long_col[e] = c;
         const double* h = vals + 9L * b;
 #pragma unroll
@@ -631,13 +650,7 @@ __global__ void k_long_fill(const double* __restrict__ vals, const int* __restri
                 lv[9 * e + q * 3 + r] = v;
                 if (lv32) lv32[9 * e + q * 3 + r] = (float)v;
             }
-        ++e;
-    };
-    for (int t = col_ptr[i]; t < col_ptr[i + 1]; ++t) {
-        const int r = (int)(skeys[t] & ((1ull << kbits) - 1ull));
-        if (r != i) put(r, sidx[t], true);
     }
-    for (int b = up_ptr[i]; b < up_ptr[i + 1]; ++b) put(cols[b], b, false);
 }
 
 // y_i += a (A x)_i for every long row i, one workgroup per row (fixed-order
@@ -1075,11 +1088,16 @@ void FullMatrix::build(hipStream_t s, const double* vals, const int* rows, const
         long_col.reserve(ne);
         long_vals.reserve((size_t)ne * 9);
         if (want_f32) long_vals32.reserve((size_t)ne * 9);
-        k_long_fill<<<g256(n + 1), 256, 0, s>>>(vals, cols, skeys_.get(), sidx_.get(), up_ptr_.get(), col_ptr_.get(),
-                                                lflag_.get(), lrank_.get(), loff_.get(), n, kbits, long_rows.get(),
-                                                long_ptr.get(), long_col.get(), long_vals.get(),
-                                                want_f32 ? long_vals32.get() : nullptr);
+        k_long_rows<<<g256(n + 1), 256, 0, s>>>(lflag_.get(), lrank_.get(), loff_.get(), n, long_rows.get(),
+                                                long_ptr.get());
         SGIPC_LAUNCH_CHECK();
+        if (n_long > 0) {
+            k_long_fill<<<n_long, kLongFillThreads, 0, s>>>(vals, cols, skeys_.get(), sidx_.get(), up_ptr_.get(),
+                                                            col_ptr_.get(), long_rows.get(), long_ptr.get(), kbits,
+                                                            long_col.get(), long_vals.get(),
+                                                            want_f32 ? long_vals32.get() : nullptr);
+            SGIPC_LAUNCH_CHECK();
+        }
     }
     sell_vals.reserve((size_t)n_slots * 64 * 9);
     sell_cols.reserve((size_t)n_slots * 64);
