@@ -1070,6 +1070,26 @@ __device__ __forceinline__ TS post_rhs(int d, int n, const int* __restrict__ ap_
     return (TS)r[o] + R[o] - acc;
 }
 
+// The right-hand side of the level-0 post-smoothing (r + r1 - A P c, zero on
+// padding rows) written over R's level-0 slots, for the FP64 inverse, whose
+// post-smoothing then runs as a plain k_local_solve: in one kernel the A P
+// partials' LDS costs the FP64 inverse a workgroup per CU (3 instead of 4),
+// more than the right-hand side's round trip through HBM.  Each thread reads
+// and writes only its own slot component.
+template <int B, typename TV, typename TS>
+__global__ __launch_bounds__(256) void k_post_rhs(const int* __restrict__ done, TS* __restrict__ R, int n,
+                                                  const int* __restrict__ ap_ptr, const int* __restrict__ ap_col,
+                                                  const float* __restrict__ ap_blk,
+                                                  const typename Vec4Of<TS>::T* __restrict__ c4,
+                                                  const TV* __restrict__ r)
+{
+    if (done != nullptr && *done) return;
+    __shared__ TS sP[3 * kPostChunk];
+    const int d = blockIdx.x;
+    const TS rhs = post_rhs<B>(d, n, ap_ptr, ap_col, ap_blk, c4, r, R, sP);
+    if (threadIdx.x < 3 * B) R[3L * d * B + threadIdx.x] = rhs;
+}
+
 // Level-0 post-smoothing of coarse mode 2 with its right-hand side formed in
 // the kernel (post_rhs: R2 = r + r1 - A P c), then Z = Minv R2 as in
 // k_local_solve.  The inverse blocks are loaded after the A P phase, so no
@@ -2249,8 +2269,15 @@ int Mas::finish_v_t(hipStream_t s, TS* R, TS* Z, const TS* minv, const int* done
                 return;
             }
         }
-        k_local_solve_post<BB, TV, TS><<<ndom0, 256, 0, s>>>(done, minv, R, Z, n, ap_ptr_.get(), ap_col_.get(),
-                                                              ap_blk_.get(), c4, r);
+        if constexpr (sizeof(TS) == 8) {
+            k_post_rhs<BB, TV, TS><<<ndom0, 256, 0, s>>>(done, R, n, ap_ptr_.get(), ap_col_.get(), ap_blk_.get(),
+                                                          c4, r);
+            SGIPC_LAUNCH_CHECK();
+            k_local_solve<BB, TS><<<ndom0, 256, 0, s>>>(done, minv, R, Z, 0);
+        } else {
+            k_local_solve_post<BB, TV, TS><<<ndom0, 256, 0, s>>>(done, minv, R, Z, n, ap_ptr_.get(),
+                                                                  ap_col_.get(), ap_blk_.get(), c4, r);
+        }
     };
     solve0_post();
     SGIPC_LAUNCH_CHECK();
