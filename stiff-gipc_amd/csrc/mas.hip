@@ -306,6 +306,16 @@ __global__ void k_row_ptr(const int* __restrict__ rows, int nnz, int offset, int
     row_ptr[i] = lo;
 }
 
+// rows and columns of the BCOO entries e0.. shifted down by offset
+__global__ void k_shift_rc(const int* __restrict__ rows, const int* __restrict__ cols, long e0, int m, int offset,
+                           int* __restrict__ sr, int* __restrict__ sc)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    sr[k] = rows[e0 + k] - offset;
+    sc[k] = cols[e0 + k] - offset;
+}
+
 // ------------------------------------------------------------------ Galerkin
 __device__ __forceinline__ void load_block(const double* __restrict__ v, double h[9])
 {
@@ -1872,11 +1882,35 @@ void Mas::assemble_invert_t(const double* vals, const int* rows, const int* cols
     if (ndomc > 0) launch_fill<double>(s, mgal_.get(), 0.0, (long)ndomc * nblk * 9);
     k_row_ptr<<<grid_for(n + 1, 256), 256, 0, s>>>(rows, nnz, offset, n, row_ptr_.get());
     SGIPC_LAUNCH_CHECK();
+    // with a row offset the fine operator is the BCOO's suffix of rows >=
+    // offset (rows sorted; an upper block of such a row has its column past
+    // the offset too), renumbered from 0
+    sub_e0_ = 0;
+    sub_nnz_ = nnz;
+    if (want_ap && offset > 0) {
+        int e[2] = {0, 0};
+        SGIPC_HIP(hipMemcpyAsync(&e[0], row_ptr_.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+        SGIPC_HIP(hipMemcpyAsync(&e[1], row_ptr_.get() + n, sizeof(int), hipMemcpyDeviceToHost, s));
+        SGIPC_HIP(hipStreamSynchronize(s));
+        SGIPC_CHECK(e[1] == nnz, SGIPC_BAD_INDEX, "blocks past the MAS's rows");
+        sub_e0_ = e[0];
+        sub_nnz_ = nnz - e[0];
+        sub_rows_.reserve(sub_nnz_ > 0 ? sub_nnz_ : 1);
+        sub_cols_.reserve(sub_nnz_ > 0 ? sub_nnz_ : 1);
+        if (sub_nnz_ > 0) {
+            k_shift_rc<<<grid_for(sub_nnz_, 256), 256, 0, s>>>(rows, cols, sub_e0_, sub_nnz_, offset,
+                                                               sub_rows_.get(), sub_cols_.get());
+            SGIPC_LAUNCH_CHECK();
+        }
+    }
+    const double* fv = vals + 9 * sub_e0_;
+    const int* fr = offset > 0 ? sub_rows_.get() : rows;
+    const int* fc = offset > 0 ? sub_cols_.get() : cols;
     // A P of coarse mode 2 (needs the hierarchy and the fine matrix only) on
     // the side stream, beside the coarse levels; joined below
-    if (want_ap) {
+    if (want_ap && sub_nnz_ > 0) {
         side_.begin(s);
-        build_ap(side_.s, vals, rows, cols, nnz);
+        build_ap(side_.s, fv, fr, fc, sub_nnz_);
     }
     // coarse operators A_l = P_l^T A_{l-1} P_l, level by level
     if (L > 1 && nnz > 0) {
@@ -2113,11 +2147,15 @@ void Mas::set_bcoo(const double* vals, const int* rows, const int* cols, int nnz
         build_hierarchy(ctx->stream, pairs, n_pairs);
     }
     ap_nnz_ = 0;
-    const bool want_ap = coarse_mode == 2 && offset == 0 && L > 2 && nnz > 0;
+    // with a row offset the fine operator is always the MAS's own (the
+    // caller's matrix also holds the rows before the offset)
+    const bool want_ap = coarse_mode == 2 && L > 2 && nnz > 0 && (offset == 0 || build_own_fine);
     if (B == 32) assemble_invert_t<32>(vals, rows, cols, nnz, offset, want_ap);
     else assemble_invert_t<16>(vals, rows, cols, nnz, offset, want_ap);
-    if (want_ap && build_own_fine) {
-        own_fine_.build(ctx->stream, vals, rows, cols, nnz, n, precision != SGIPC_FP64);
+    if (want_ap && build_own_fine && sub_nnz_ > 0) {
+        const int* fr = offset > 0 ? sub_rows_.get() : rows;
+        const int* fc = offset > 0 ? sub_cols_.get() : cols;
+        own_fine_.build(ctx->stream, vals + 9 * sub_e0_, fr, fc, sub_nnz_, n, precision != SGIPC_FP64);
         fine_ = &own_fine_;
     }
     finish_dense_level2(ctx->stream);

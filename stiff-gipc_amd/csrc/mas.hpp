@@ -132,6 +132,12 @@ class Mas {
     // rows, level-1 columns, FP32 3x3 blocks)
     void build_ap(hipStream_t s, const double* vals, const int* rows, const int* cols, int nnz);
     DevBuf<int> ap_ptr_, ap_col_, ap_seg_;
+    // with a row offset (the MAS covering the rows past the body rows): the
+    // BCOO's suffix of rows >= offset, its rows and columns shifted to the
+    // MAS's own numbering (A P and the own fine operator read it)
+    DevBuf<int> sub_rows_, sub_cols_;
+    long sub_e0_ = 0;
+    int sub_nnz_ = 0;
     DevBuf<float> ap_blk_;
     DevBuf<double4> c4_;  // level-1 correction, 4-padded (float4 view in FP32 applies)
     int ap_nnz_ = 0;

@@ -794,7 +794,9 @@ int system_solve(sgipc_system* sys, const double* b, double* x, double* grad, co
         const int mprec = (o.precision == SGIPC_FP64) ? SGIPC_FP64 : SGIPC_FP32_MAS;
         const bool learned = record == nullptr && use_learned(sys);
         sys->mas->m->keep_graph = sys->net != nullptr || record != nullptr;
-        sys->mas->m->build_own_fine = false;  // the PCG matrix serves as coarse mode 2's fine operator
+        // the PCG matrix serves as coarse mode 2's fine operator when the MAS
+        // covers every row; behind body rows the MAS builds its own
+        sys->mas->m->build_own_fine = off0 > 0;
         // the PCG's SpMV layout is built by a helper thread on a second stream
         // while this thread builds the preconditioners (both only read the
         // BCOO; each build waits on its own stream's read-backs)

@@ -122,6 +122,35 @@ def test_hybrid_system_solve_matches_oracle(pkg, orc, hyb, hyb_big, which):
         ls.close()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
+def test_hybrid_two_grid_cycle(pkg, orc, hyb_big, precision):
+    """Coarse mode 2 on a hybrid system: the MAS covers the rows past the
+    body rows, so it builds A P and its own fine operator from the BCOO's
+    suffix of those rows (renumbered from 0).  The solve takes the two-grid
+    path, meets the tolerance under the oracle's product, and (FP64) agrees
+    with the level-1 cycle's solution of the same system."""
+    s = hyb_big
+    ls = pkg.LinearSystem(s, bank_size=32, coarse_mode=2)
+    m1 = None
+    try:
+        res = ls.solve(tol=1e-5, precision=precision)
+        assert res.status == 0 and res.coarse_mode == 2 and res.level2_exact, res
+        ov, orr, oc = orc.assemble(s)
+        ax = orc.spmv(1.0, ov, orr, oc, res.x, 0.0, np.zeros(3 * s.n_sys), s.n_sys)
+        assert np.linalg.norm(s.rhs - ax) / np.linalg.norm(s.rhs) <= 1.01e-5
+        if precision == "fp64":
+            tight = ls.solve(tol=1e-9)
+            assert tight.status == 0 and tight.coarse_mode == 2
+            m1 = pkg.LinearSystem(s, bank_size=32, coarse_mode=1)
+            ref = m1.solve(tol=1e-9)
+            assert ref.status == 0 and ref.coarse_mode == 1
+            assert np.linalg.norm(tight.x - ref.x) / np.linalg.norm(ref.x) <= 1e-4
+    finally:
+        if m1 is not None:
+            m1.close()
+        ls.close()
+
+
 # ---- Dirichlet vertices, the assembled right-hand side, motor bodies
 @pytest.fixture(scope="module")
 def hyb_bc(pkg):
